@@ -1,0 +1,163 @@
+"""Case table for the golden vectors (inputs only; no solver code).
+
+Each case is a small deck of the explicit finite-volume step: model, mesh,
+order and limiters, time control and the initial domains.  Phase 0 is air
+(ideal gas), phase 1 water (stiffened gas), as in the reference deck
+``libTests/referenceTestCases/kapila/1D/shockTubes/interfaceWaterAir``.
+"""
+from __future__ import annotations
+
+AIR = {"type": "IG", "name": "air", "gamma": 1.4, "pinf": 0.0, "cv": 800.0, "eref": 0.0, "sref": 0.0}
+WATER = {"type": "SG", "name": "water", "gamma": 4.4, "pinf": 6.0e8, "cv": 1000.0, "eref": 0.0, "sref": 0.0}
+
+
+def _splitmix64(state):
+    state = (state + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+    z = state
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
+    return state, z ^ (z >> 31)
+
+
+def _uniforms(seed, n):
+    s = seed
+    out = []
+    for _ in range(n):
+        s, z = _splitmix64(s)
+        out.append((z >> 11) * (1.0 / 9007199254740992.0))
+    return out
+
+
+def place_discs(seed, count, rmin, rmax, box, dx, tries=20000, log_radius=False):
+    """Non-overlapping discs/spheres, centres uniform in ``box`` (list of (lo, hi)).
+
+    Radii in cells, uniform (or log-uniform) in [rmin, rmax]; returned in metres.
+    """
+    import math
+    placed = []
+    s = seed
+    for _ in range(tries):
+        if len(placed) == count:
+            break
+        vals = []
+        for _k in range(1 + len(box)):
+            s, z = _splitmix64(s)
+            vals.append((z >> 11) * (1.0 / 9007199254740992.0))
+        if log_radius:
+            r = math.exp(math.log(rmin) + vals[0] * (math.log(rmax) - math.log(rmin))) * dx
+        else:
+            r = (rmin + vals[0] * (rmax - rmin)) * dx
+        c = [lo + r + v * (hi - lo - 2 * r) for v, (lo, hi) in zip(vals[1:], box)]
+        if any(hi - lo - 2 * r <= 0 for lo, hi in box):
+            continue
+        ok = all(sum((a - b) ** 2 for a, b in zip(c, q)) > (r + rq + dx) ** 2 for q, rq in placed)
+        if ok:
+            placed.append((c, r))
+    return placed
+
+
+def _c1(alpha_null, t_final=240e-6, cells=1000):
+    lo = {"name": "LP", "alpha": [1.0 - 1e-6, 1e-6], "rho": [50.0, 1000.0], "p": [1e5, 1e5], "u": [0.0]}
+    hi = {"name": "HP", "alpha": [1e-6, 1.0 - 1e-6], "rho": [50.0, 1000.0], "p": [1e9, 1e9], "u": [0.0]}
+    return {
+        "model": "kapila", "alpha_null": alpha_null, "eos": [AIR, WATER],
+        "cells": [cells, 1, 1], "lengths": [1.0, 0.1, 0.1], "order": 1,
+        "limiters": ["none"] * 4, "cfl": 0.8, "time_mode": "time", "t_final": t_final,
+        "domains": [{"type": "entireDomain", "state": "LP"},
+                    {"type": "halfSpace", "state": "HP", "axe": "x", "origin": 0.7, "direction": "negative"}],
+        "states": [lo, hi],
+    }
+
+
+def _shock_tube_1d(model, order, limiters, steps, cells=200, relax=False, alpha_null=False):
+    c = _c1(alpha_null, cells=cells)
+    c.update({"model": model, "order": order, "limiters": limiters, "time_mode": "iterations",
+              "steps": steps, "relax_p": relax})
+    if model == "multip":
+        c["states"] = [{"name": "LP", "alpha": [1.0 - 1e-6, 1e-6], "rho": [50.0, 1000.0], "p": [1e5, 1e5], "u": [0.0]},
+                       {"name": "HP", "alpha": [1e-6, 1.0 - 1e-6], "rho": [50.0, 1000.0], "p": [1e9, 1e9], "u": [0.0]}]
+    return c
+
+
+def _euler_1d(order, limiters, steps, cells=200):
+    return {
+        "model": "euler", "alpha_null": False, "eos": [AIR],
+        "cells": [cells, 1, 1], "lengths": [1.0, 0.1, 0.1], "order": order, "limiters": limiters,
+        "cfl": 0.8, "time_mode": "iterations", "steps": steps,
+        "domains": [{"type": "entireDomain", "state": "LP"},
+                    {"type": "halfSpace", "state": "HP", "axe": "x", "origin": 0.4, "direction": "negative"}],
+        "states": [{"name": "LP", "rho": [1.0], "p": [1e5], "u": [0.0]},
+                   {"name": "HP", "rho": [100.0], "p": [1e7], "u": [0.0]}],
+    }
+
+
+def mach_shock_air(mach, p1=1e5, rho1=1.2, gamma=1.4):
+    """Post-shock state of a normal shock moving into air at rest (Rankine-Hugoniot)."""
+    import math
+    m2 = mach * mach
+    p2 = p1 * (1.0 + 2.0 * gamma / (gamma + 1.0) * (m2 - 1.0))
+    rho2 = rho1 * (gamma + 1.0) * m2 / ((gamma - 1.0) * m2 + 2.0)
+    c1 = math.sqrt(gamma * p1 / rho1)
+    u2 = mach * c1 * (1.0 - rho1 / rho2)
+    return p2, rho2, u2
+
+
+def _shock_cylinders_2d(n=48, steps=40, ncyl=4, seed=0xEC0C0002, limiters=("vanleer",) * 4):
+    p2, rho2, u2 = mach_shock_air(2.4)
+    dx = 1.0 / n
+    # Radii scaled to the small grid (4..8 cells) with the same placement rule.
+    discs = place_discs(seed, ncyl, 4.0, 8.0, [(0.1, 0.95), (0.0, 1.0)], dx)
+    doms = [{"type": "entireDomain", "state": "air"},
+            {"type": "halfSpace", "state": "shocked", "axe": "x", "origin": 0.05, "direction": "negative"}]
+    doms += [{"type": "disc", "state": "water", "radius": r, "center": c} for c, r in discs]
+    return {
+        "model": "kapila", "alpha_null": False, "eos": [AIR, WATER],
+        "cells": [n, n, 1], "lengths": [1.0, 1.0, 1.0], "order": 2, "limiters": list(limiters),
+        "cfl": 0.8, "time_mode": "iterations", "steps": steps, "domains": doms,
+        "states": [
+            {"name": "air", "alpha": [1.0 - 1e-6, 1e-6], "rho": [1.2, 1000.0], "p": [1e5, 1e5], "u": [0.0, 0.0]},
+            {"name": "shocked", "alpha": [1.0 - 1e-6, 1e-6], "rho": [rho2, 1000.0], "p": [p2, p2], "u": [u2, 0.0]},
+            {"name": "water", "alpha": [1e-6, 1.0 - 1e-6], "rho": [1.2, 1000.0], "p": [1e5, 1e5], "u": [0.0, 0.0]},
+        ],
+    }
+
+
+def _bubbles_3d(n=16, steps=12, nb=3, seed=0xEC0C0003, limiters=("minmod",) * 4, model="multip", relax=True):
+    dx = 1.0 / n
+    spheres = place_discs(seed, nb, 2.0, 4.0, [(0.0, 1.0)] * 3, dx, log_radius=True)
+    doms = [{"type": "entireDomain", "state": "water"}]
+    doms += [{"type": "sphere", "state": "bubble", "radius": r, "center": c} for c, r in spheres]
+    return {
+        "model": model, "alpha_null": False, "eos": [AIR, WATER], "relax_p": relax,
+        "cells": [n, n, n], "lengths": [1.0, 1.0, 1.0], "order": 2, "limiters": list(limiters),
+        "cfl": 0.8, "time_mode": "iterations", "steps": steps, "domains": doms,
+        "states": [
+            {"name": "water", "alpha": [1e-6, 1.0 - 1e-6], "rho": [1.2, 1000.0], "p": [5e7, 5e7], "u": [0.0, 0.0, 0.0]},
+            {"name": "bubble", "alpha": [1.0 - 1e-6, 1e-6], "rho": [1.2, 1000.0], "p": [1e5, 1e5], "u": [0.0, 0.0, 0.0]},
+        ],
+    }
+
+
+CASES = {
+    # Config C1 in full: 1000 cells, first order, to t = 240 us.
+    "c1_kapila_fo": _c1(False),
+    "c1_kapila_fo_alphanull": _c1(True, t_final=120e-6),
+    # 1-D second order, every limiter, interface limiter differing from the global one.
+    "kapila_1d_o2_vanleer": _shock_tube_1d("kapila", 2, ["vanleer"] * 4, 60),
+    "kapila_1d_o2_minmod_superbee": _shock_tube_1d("kapila", 2, ["minmod", "superbee", "minmod", "superbee"], 60),
+    "kapila_1d_o2_vanalbada_mc": _shock_tube_1d("kapila", 2, ["vanalbada", "mc", "mc", "vanalbada"], 60),
+    "kapila_1d_o2_thinc": _shock_tube_1d("kapila", 2, ["vanleer", "vanleer", "thinc", "thinc"], 60),
+    "kapila_1d_o2_alphanull": _shock_tube_1d("kapila", 2, ["vanleer"] * 4, 60, alpha_null=True),
+    # Pressure-disequilibrium model: first order without relaxation (reference deck), MUSCL with relaxation.
+    "multip_1d_fo": _shock_tube_1d("multip", 1, ["none"] * 4, 200, cells=400),
+    "multip_1d_o2_relax": _shock_tube_1d("multip", 2, ["minmod"] * 4, 60, relax=True),
+    "multip_1d_o2_norelax": _shock_tube_1d("multip", 2, ["vanleer"] * 4, 60, relax=False),
+    # Single-phase Euler.
+    "euler_1d_fo": _euler_1d(1, ["none"] * 4, 100),
+    "euler_1d_o2": _euler_1d(2, ["vanleer"] * 4, 100),
+    # Multi-D: config C2 physics (shock on water cylinders) and C3 physics (bubbles, MultiP + relaxation).
+    "c2_kapila_2d_o2": _shock_cylinders_2d(),
+    "c2_kapila_2d_fo": dict(_shock_cylinders_2d(n=40, steps=40), order=1, limiters=["none"] * 4),
+    "c3_multip_3d_o2": _bubbles_3d(),
+    "kapila_3d_o2": _bubbles_3d(n=12, steps=10, model="kapila", relax=False, limiters=("vanleer",) * 4),
+}
