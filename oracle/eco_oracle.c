@@ -1,0 +1,1185 @@
+/*
+ * CPU oracle for the explicit finite-volume step -- TEST INFRASTRUCTURE ONLY.
+ *
+ * FP64 restatement of the reference ECOGEN step on Cartesian meshes.  Each
+ * function below names the reference file:line it follows (paths relative
+ * to the ECOGEN source tree).  The restatement keeps the reference's cached
+ * per-cell quantities (phase energies, sound speeds, mixture density and
+ * internal energy, ...) and reproduces its evaluation order, min/max
+ * semantics and clamp constants, so that it matches the reference built with
+ * g++ -O3 (no FMA) operation by operation.  Build with -ffp-contract=off.
+ *
+ * Used by tests/, __graft_entry__.smoke() and the CPU-baseline leg of
+ * bench.py, as the checker only.
+ */
+#include "eco_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define MAXPH ORC_MAXPH
+
+/* std::min / std::max / std::abs semantics (<algorithm>, <cmath>). */
+static inline double smin(double a, double b) { return (b < a) ? b : a; }
+static inline double smax(double a, double b) { return (a < b) ? b : a; }
+
+/* ------------------------------------------------------------------------ */
+/* Per-cell state mirroring Phase{Kapila,MultiP,Euler} and Mix{Kapila,MultiP} */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+  double alpha, rho, p, e, c, T, Y, Et; /* Et: phase total energy (MultiP, Euler) */
+  double u[3];                          /* Euler phase velocity */
+} Phase;
+
+typedef struct {
+  double rho, p, u[3], e, E, cF, cW;
+} Mix;
+
+typedef struct {
+  Phase ph[MAXPH];
+  Mix mx;
+} State;
+
+/* Flux{Kapila,MultiP,Euler}: alpha/masse/energ per phase, qdm, energMixture, sM. */
+typedef struct {
+  double alpha[MAXPH], masse[MAXPH], energ[MAXPH];
+  double qdm[3];
+  double energMix;
+  double sM;
+} Flux;
+
+typedef struct {
+  const orc_cfg *cfg;
+  double epsilon; /* src/Eos/Eos.cpp:42,114-122 */
+} Ctx;
+
+/* ------------------------------------------------------------------------ */
+/* EOS: src/Eos/EosSG.cpp:93-254, src/Eos/EosIG.cpp:90-251                   */
+/* ------------------------------------------------------------------------ */
+static double eos_T(const orc_eos *q, double rho, double p, double eps) {
+  if (q->type == ORC_EOS_SG) return (p + q->pinf) / (q->gamma - 1.) / smax(rho, eps) / q->cv;
+  return p / (q->gamma - 1.) / smax(rho, eps) / q->cv;
+}
+static double eos_e(const orc_eos *q, double rho, double p, double eps) {
+  if (q->type == ORC_EOS_SG) return (p + q->gamma * q->pinf) / (q->gamma - 1.) / smax(rho, eps) + q->eref;
+  return p / (q->gamma - 1.) / smax(rho, eps) + q->eref;
+}
+static double eos_p(const orc_eos *q, double rho, double e) {
+  if (q->type == ORC_EOS_SG) return (q->gamma - 1.) * rho * (e - q->eref) - q->gamma * q->pinf;
+  return (q->gamma - 1.) * rho * (e - q->eref);
+}
+static double eos_c(const orc_eos *q, double rho, double p, double eps) {
+  if (q->type == ORC_EOS_SG) return sqrt(q->gamma * (p + q->pinf) / smax(rho, eps));
+  return sqrt(q->gamma * p / smax(rho, eps));
+}
+static double eos_isentrope(const orc_eos *q, double p0, double rho0, double rho1, double eps) {
+  if (q->type == ORC_EOS_SG) return (p0 + q->pinf) * pow(rho1 / smax(rho0, eps), q->gamma) - q->pinf;
+  return p0 * pow(rho1 / smax(rho0, eps), q->gamma);
+}
+static double eos_rho_pfinal(const orc_eos *q, double p0, double rho0, double pf, double *drho, double eps) {
+  if (q->type == ORC_EOS_SG) {
+    double num = (q->gamma) * (pf + q->pinf);
+    double denom = num + p0 - pf;
+    double rf = rho0 * num / smax(denom, eps);
+    if (drho) *drho = rho0 * q->gamma * (p0 + q->pinf) / smax((denom * denom), eps);
+    return rf;
+  }
+  double num = (q->gamma) * pf;
+  double denom = num + p0 - pf;
+  double rf = rho0 * num / smax(denom, eps);
+  if (drho) *drho = rho0 * q->gamma * p0 / smax((denom * denom), eps);
+  return rf;
+}
+/* verifyAndModifyPressure: EosSG.cpp:243-247, EosIG.cpp:241-244 */
+static void eos_fix_p(const orc_eos *q, double *p) {
+  if (q->type == ORC_EOS_SG) {
+    if (*p <= -(1. - 1.e-15) * q->pinf + 1.e-15) *p = -(1. - 1.e-15) * q->pinf + 1.e-15;
+  } else {
+    if (*p < 1.e-15) *p = 1.e-15;
+  }
+}
+/* sendSpecialMixtureEos: EosSG.cpp:217-222, EosIG.cpp:213-218 */
+static void eos_special(const orc_eos *q, double *gpg, double *eref, double *og) {
+  *gpg = (q->type == ORC_EOS_SG) ? q->gamma * q->pinf / (q->gamma - 1.) : 0.;
+  *eref = q->eref;
+  *og = 1. / (q->gamma - 1.);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Limiters: src/Ordre2/Limiter*.cpp:49-59                                   */
+/* ------------------------------------------------------------------------ */
+static double limiter(int lim, double s1, double s2) {
+  double slope = 0., prod = s1 * s2;
+  switch (lim) {
+    case ORC_LIM_MINMOD:
+    case ORC_LIM_THINC: /* LimiterTHINC.cpp:49-58 limits as minmod */
+      if (prod > 1e-9) { slope = smin(fabs(s1), fabs(s2)); if (s1 < 0) slope = -slope; }
+      return slope;
+    case ORC_LIM_VANLEER: {
+      double sum = s1 + s2;
+      if ((fabs(sum) > 1e-6) && (prod > 1e-6)) slope = 2. * prod / sum;
+      return slope;
+    }
+    case ORC_LIM_VANALBADA: {
+      double sum = s1 + s2;
+      if ((fabs(s1) > 1e-6) && (fabs(s2) > 1e-6) && (fabs(sum) > 1e-6) && (prod > 1e-6))
+        slope = prod * sum / (s1 * s1 + s2 * s2);
+      return slope;
+    }
+    case ORC_LIM_MC:
+      if (prod > 1e-9) {
+        slope = smin(2. * fabs(s1), 2. * fabs(s2));
+        slope = smin(slope, 0.5 * (fabs(s1) + fabs(s2)));
+      }
+      if (s1 < 0.) slope = -slope;
+      return slope;
+    case ORC_LIM_SUPERBEE:
+      if (prod > 1e-6) slope = smax(smin(2. * fabs(s1), fabs(s2)), smin(fabs(s1), 2. * fabs(s2)));
+      if (s1 < 0.) slope = -slope;
+      return slope;
+    default:
+      return 0.;
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Phase / mixture state completion                                          */
+/* ------------------------------------------------------------------------ */
+static inline double sqnorm(const double *u) { return u[0] * u[0] + u[1] * u[1] + u[2] * u[2]; }
+
+/* PhaseKapila::verifyAndCorrectPhase (PhaseKapila.cpp:294-307),
+   PhaseMultiP::verifyAndCorrectPhase (PhaseMultiP.cpp:288-294),
+   PhaseEuler::verifyAndCorrectPhase (PhaseEuler.cpp:349-353) */
+static void verify_correct(const Ctx *c, int k, Phase *ph) {
+  const orc_cfg *cfg = c->cfg;
+  if (cfg->model == ORC_KAPILA) {
+    if (c->epsilon > 1.e-20) {
+      if (ph->alpha < 0.) ph->alpha = 0.;
+      if (ph->alpha > 1.) ph->alpha = 1.;
+      if (ph->rho <= 1.e-15) ph->rho = 1.e-15;
+    } else {
+      if (ph->alpha < 1e-15) ph->alpha = 1e-14;
+      if (ph->alpha > 1. - 1e-15) ph->alpha = 1. - 1e-14;
+      if (ph->rho < 1.e-15) ph->rho = 1.e-14;
+    }
+  } else if (cfg->model == ORC_MULTIP) {
+    if (ph->alpha < 1e-10) ph->alpha = 1e-9;
+    if (ph->alpha > 1. - 1e-10) ph->alpha = 1. - 1e-9;
+    if (ph->rho < 1.e-10) ph->rho = 1.e-9;
+  } else {
+    if (ph->rho < 1.e-10) ph->rho = 1.e-10;
+  }
+  eos_fix_p(&cfg->eos[k], &ph->p);
+}
+
+/* extendedCalculusPhase: PhaseKapila.cpp:97-102, PhaseMultiP.cpp:97-103, PhaseEuler.cpp:112-118 */
+static void extended_phase(const Ctx *c, int k, Phase *ph, const double *u) {
+  const orc_eos *q = &c->cfg->eos[k];
+  double eps = c->epsilon;
+  if (c->cfg->model == ORC_EULER) {
+    ph->e = eos_e(q, ph->rho, ph->p, eps);
+    ph->c = eos_c(q, ph->rho, ph->p, eps);
+    ph->Et = ph->e + 0.5 * sqnorm(ph->u);
+    ph->T = eos_T(q, ph->rho, ph->p, eps);
+    return;
+  }
+  ph->T = eos_T(q, ph->rho, ph->p, eps);
+  ph->e = eos_e(q, ph->rho, ph->p, eps);
+  ph->c = eos_c(q, ph->rho, ph->p, eps);
+  if (c->cfg->model == ORC_MULTIP) ph->Et = ph->e + 0.5 * sqnorm(u);
+}
+
+/* MixKapila::computeMixtureVariables (MixKapila.cpp:147-172), MixMultiP (MixMultiP.cpp:142-168) */
+static void mixture_vars(const Ctx *c, State *s) {
+  int n = c->cfg->nph;
+  Mix *m = &s->mx;
+  if (c->cfg->model == ORC_EULER) return; /* MixEuler: no-op */
+  int kap = (c->cfg->model == ORC_KAPILA);
+  m->rho = 0.; m->p = 0.;
+  for (int k = 0; k < n; k++) {
+    m->rho += s->ph[k].alpha * s->ph[k].rho;
+    m->p += s->ph[k].alpha * s->ph[k].p;
+  }
+  for (int k = 0; k < n; k++)
+    s->ph[k].Y = kap ? s->ph[k].alpha * s->ph[k].rho / smax(m->rho, c->epsilon)
+                     : s->ph[k].alpha * s->ph[k].rho / m->rho;
+  m->e = 0.; m->cF = 0.; m->cW = 0.;
+  for (int k = 0; k < n; k++) {
+    Phase *p = &s->ph[k];
+    m->e += p->Y * p->e;
+    m->cF += p->Y * p->c * p->c;
+    m->cW += kap ? p->alpha / smax((p->rho * p->c * p->c), c->epsilon) : p->alpha / (p->rho * p->c * p->c);
+  }
+  m->cF = sqrt(m->cF);
+  m->cW = 1. / sqrt(m->rho * m->cW);
+}
+
+/* Model::fulfillState: ModKapila.cpp:89-101, ModMultiP.cpp:83-91, ModEuler.cpp (phase only) */
+static void fulfill(const Ctx *c, State *s) {
+  for (int k = 0; k < c->cfg->nph; k++) extended_phase(c, k, &s->ph[k], s->mx.u);
+  mixture_vars(c, s);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Conservative <-> primitive                                                */
+/* ------------------------------------------------------------------------ */
+/* FluxKapila::buildCons (FluxKapila.cpp:123-135), FluxMultiP::buildCons (FluxMultiP.cpp:122-140),
+   FluxEuler::buildCons (FluxEuler.cpp:95-104) */
+static void build_cons(const Ctx *c, const State *s, Flux *f) {
+  const orc_cfg *cfg = c->cfg;
+  if (cfg->model == ORC_EULER) {
+    const Phase *p = &s->ph[0];
+    f->masse[0] = p->rho;
+    for (int d = 0; d < 3; d++) f->qdm[d] = p->u[d] * f->masse[0];
+    f->energMix = f->masse[0] * p->Et;
+    return;
+  }
+  for (int k = 0; k < cfg->nph; k++) {
+    const Phase *p = &s->ph[k];
+    f->alpha[k] = p->alpha;
+    f->masse[k] = p->alpha * p->rho;
+    double ein = eos_e(&cfg->eos[k], p->rho, p->p, c->epsilon);
+    f->energ[k] = p->alpha * p->rho * ein;
+  }
+  for (int d = 0; d < 3; d++) f->qdm[d] = s->mx.u[d] * s->mx.rho;
+  f->energMix = s->mx.rho * s->mx.E;
+}
+
+static void zero_small_u(double *u) {
+  if (fabs(u[0]) < 1.e-8) u[0] = 0.;
+  if (fabs(u[1]) < 1.e-8) u[1] = 0.;
+  if (fabs(u[2]) < 1.e-8) u[2] = 0.;
+}
+
+/* FluxKapila::buildPrim (FluxKapila.cpp:139-185), FluxMultiP::buildPrim (FluxMultiP.cpp:144-183),
+   FluxEuler::buildPrim (FluxEuler.cpp:107-131) */
+static void build_prim(const Ctx *c, Flux *f, State *s) {
+  const orc_cfg *cfg = c->cfg;
+  int n = cfg->nph;
+  if (cfg->model == ORC_EULER) {
+    Phase *p = &s->ph[0];
+    const orc_eos *q = &cfg->eos[0];
+    p->rho = f->masse[0];
+    for (int d = 0; d < 3; d++) p->u[d] = f->qdm[d] / f->masse[0];
+    zero_small_u(p->u);
+    p->Et = f->energMix / f->masse[0];
+    p->e = p->Et - 0.5 * sqnorm(p->u);
+    p->p = eos_p(q, f->masse[0], p->e);
+    p->c = eos_c(q, f->masse[0], p->p, c->epsilon);
+    p->T = eos_T(q, f->masse[0], p->p, c->epsilon);
+    return;
+  }
+  double un = 0.;
+  if (cfg->model == ORC_KAPILA) {
+    if (c->epsilon > 1.e-20) {
+      for (int k = 0; k < n; k++) {
+        if (f->alpha[k] < 0.) f->alpha[k] = 0.;
+        if (f->alpha[k] > 1.) f->alpha[k] = 1.;
+        un += f->alpha[k];
+      }
+    } else {
+      for (int k = 0; k < n; k++) {
+        if (f->alpha[k] <= 1.e-15) f->alpha[k] = 1e-15;
+        if (f->alpha[k] >= 1. - 1.e-15) f->alpha[k] = 1.0 - 1e-15;
+        un += f->alpha[k];
+      }
+    }
+  } else {
+    for (int k = 0; k < n; k++) {
+      if (f->alpha[k] <= 1.e-10) f->alpha[k] = 1e-9;
+      if (f->alpha[k] >= 1. - 1.e-10) f->alpha[k] = 1.0 - 1e-9;
+      un += f->alpha[k];
+    }
+  }
+  for (int k = 0; k < n; k++) f->alpha[k] /= un;
+  double rhoMel = 0.;
+  for (int k = 0; k < n; k++) {
+    Phase *p = &s->ph[k];
+    rhoMel = rhoMel + f->masse[k];
+    p->alpha = f->alpha[k];
+    double ein;
+    if (cfg->model == ORC_KAPILA) {
+      p->rho = f->masse[k] / smax(f->alpha[k], c->epsilon);
+      ein = f->energ[k] / smax(f->masse[k], c->epsilon);
+    } else {
+      p->rho = f->masse[k] / f->alpha[k];
+      ein = f->energ[k] / f->masse[k];
+    }
+    p->p = eos_p(&cfg->eos[k], p->rho, ein);
+    verify_correct(c, k, p);
+  }
+  for (int d = 0; d < 3; d++) s->mx.u[d] = f->qdm[d] / rhoMel;
+  zero_small_u(s->mx.u);
+  for (int k = 0; k < n; k++) extended_phase(c, k, &s->ph[k], s->mx.u);
+  mixture_vars(c, s);
+  s->mx.E = f->energMix / rhoMel;
+}
+
+/* FluxMultiP::schemeCorrection (FluxMultiP.cpp:237-260); buf = U^n */
+static void scheme_correction(const Ctx *c, Flux *f, const Flux *buf) {
+  if (c->cfg->model != ORC_MULTIP) return;
+  int n = c->cfg->nph;
+  double rhoN = 0., rhoNp1 = 0., sum = 0., dek[MAXPH];
+  for (int k = 0; k < n; k++) {
+    rhoN += buf->masse[k];
+    rhoNp1 += f->masse[k];
+    dek[k] = f->energ[k] - buf->energ[k];
+  }
+  for (int k = 0; k < n; k++) sum += dek[k];
+  double dE = f->energMix - buf->energMix;
+  double dRU2 = sqnorm(f->qdm) / rhoNp1 - sqnorm(buf->qdm) / rhoN;
+  if (fabs(sum) > 1.e-10 * f->energMix)
+    for (int k = 0; k < n; k++) f->energ[k] = buf->energ[k] + dek[k] / sum * (dE - 0.5 * dRU2);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Relaxation and energy correction                                          */
+/* ------------------------------------------------------------------------ */
+/* RelaxationP::stiffRelaxation (RelaxationP.cpp:49-111). Returns iterations. */
+static int relax_p(const Ctx *c, State *s) {
+  const orc_cfg *cfg = c->cfg;
+  int n = cfg->nph;
+  int relax = 1;
+  if (c->epsilon > 1.e-20)
+    for (int k = 0; k < n; k++) if (s->ph[k].alpha > (1. - 1.e-5)) relax = 0;
+  if (!relax) return 0;
+  double ak[MAXPH], pk[MAXPH], rk[MAXPH], rkS[MAXPH], akS[MAXPH];
+  double pStar = 0.;
+  for (int k = 0; k < n; k++) {
+    verify_correct(c, k, &s->ph[k]);
+    ak[k] = s->ph[k].alpha; pk[k] = s->ph[k].p; rk[k] = s->ph[k].rho;
+    pStar += ak[k] * pk[k];
+  }
+  int it = 0;
+  double f = 0., df = 1.;
+  do {
+    it++;
+    pStar -= f / df;
+    for (int k = 0; k < n; k++) eos_fix_p(&cfg->eos[k], &pStar);
+    f = -1.; df = 0.;
+    for (int k = 0; k < n; k++) {
+      double drho;
+      rkS[k] = eos_rho_pfinal(&cfg->eos[k], pk[k], rk[k], pStar, &drho, c->epsilon);
+      akS[k] = ak[k] * rk[k] / rkS[k];
+      double dalpha = ak[k] * rk[k] * drho / (rkS[k] * rkS[k]);
+      f += akS[k];
+      df -= dalpha;
+    }
+    if (it > 100) break;
+  } while (fabs(f) > 1e-10 && it < 100);
+  if (it < 100) {
+    for (int k = 0; k < n; k++) { s->ph[k].alpha = akS[k]; s->ph[k].rho = rkS[k]; s->ph[k].p = pStar; }
+    s->mx.p = pStar;
+  }
+  return it;
+}
+
+static int model_relaxes(const orc_cfg *cfg) {
+  return cfg->model == ORC_KAPILA || (cfg->model == ORC_MULTIP && cfg->relax_p);
+}
+
+/* FluxKapila::correctionEnergy (FluxKapila.cpp:239-268); the trailing
+   cell->fulfillState() acts on the time-n state (Cell.cpp:282). */
+static void correction_energy(const Ctx *c, State *s_type, State *s_n) {
+  if (c->cfg->model != ORC_KAPILA) return;
+  int n = c->cfg->nph;
+  double ak[MAXPH], rk[MAXPH];
+  for (int k = 0; k < n; k++) { ak[k] = s_type->ph[k].alpha; rk[k] = s_type->ph[k].rho; }
+  double rhoe = s_type->mx.rho * s_type->mx.e;
+  double p = rhoe, denom = 0.;
+  for (int k = 0; k < n; k++) {
+    double gpg, eref, og;
+    eos_special(&c->cfg->eos[k], &gpg, &eref, &og);
+    p -= ak[k] * (gpg + rk[k] * eref);
+    denom += ak[k] * og;
+  }
+  p /= denom;
+  for (int k = 0; k < n; k++) { s_type->ph[k].p = p; verify_correct(c, k, &s_type->ph[k]); }
+  s_type->mx.p = p;
+  fulfill(c, s_n);
+}
+
+/* Mixture::totalEnergyToInternalEnergy (MixKapila.cpp:188-195; MixEuler no-op) */
+static void total_to_internal(const Ctx *c, State *s) {
+  if (c->cfg->model == ORC_EULER) return;
+  s->mx.e = s->mx.E - 0.5 * sqnorm(s->mx.u);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Riemann solvers in the face frame                                         */
+/* ------------------------------------------------------------------------ */
+/* ModKapila::solveRiemannIntern (ModKapila.cpp:107-218) and the identical
+   ModMultiP::solveRiemannIntern (ModMultiP.cpp:101-214). */
+static void hllc_multi(const Ctx *c, const State *L, const State *R, double dxL, double dxR,
+                       double *dtMax, Flux *F) {
+  int n = c->cfg->nph;
+  double eps = c->epsilon;
+  double uL = L->mx.u[0], cL = L->mx.cF, pL = L->mx.p, rhoL = L->mx.rho;
+  double uR = R->mx.u[0], cR = R->mx.cF, pR = R->mx.p, rhoR = R->mx.rho;
+  double sL = smin(uL - cL, uR - cR);
+  double sR = smax(uR + cR, uL + cL);
+  if (fabs(sL) > 1.e-3) *dtMax = smin(*dtMax, dxL / fabs(sL));
+  if (fabs(sR) > 1.e-3) *dtMax = smin(*dtMax, dxR / fabs(sR));
+  double mL = rhoL * (sL - uL), mR = rhoR * (sR - uR);
+  double sM = (pR - pL + mL * uL - mR * uR) / (mL - mR);
+  if (fabs(sM) < 1.e-8) sM = 0.;
+  if (sL >= 0. || sR <= 0.) {
+    const State *S = (sL >= 0.) ? L : R;
+    double uS = (sL >= 0.) ? uL : uR, pS = (sL >= 0.) ? pL : pR, rhoS = (sL >= 0.) ? rhoL : rhoR;
+    for (int k = 0; k < n; k++) {
+      double a = S->ph[k].alpha, d = S->ph[k].rho, e = S->ph[k].e;
+      F->alpha[k] = a * sM;
+      F->masse[k] = a * d * uS;
+      F->energ[k] = a * d * e * uS;
+    }
+    double vY = S->mx.u[1], vZ = S->mx.u[2];
+    double Et = S->mx.e + 0.5 * sqnorm(S->mx.u);
+    F->qdm[0] = rhoS * uS * uS + pS;
+    F->qdm[1] = rhoS * vY * uS;
+    F->qdm[2] = rhoS * vZ * uS;
+    F->energMix = (rhoS * Et + pS) * uS;
+  } else {
+    int left = (sM >= 0.);
+    const State *S = left ? L : R;
+    double uS = left ? uL : uR, pS = left ? pL : pR, mS = left ? mL : mR, sS = left ? sL : sR;
+    double vY = S->mx.u[1], vZ = S->mx.u[2];
+    double Et = S->mx.e + 0.5 * sqnorm(S->mx.u);
+    double rhoStar = mS / (sS - sM);
+    double EStar = Et + (sM - uS) * (sM + pS / mS);
+    double pStar = mS * (sM - uS) + pS;
+    for (int k = 0; k < n; k++) {
+      double a = S->ph[k].alpha, d = S->ph[k].rho, pk = S->ph[k].p;
+      double mk = d * (sS - uS);
+      double rks = mk / (sS - sM);
+      double pks = eos_isentrope(&c->cfg->eos[k], pk, d, rks, eps);
+      double eks = eos_e(&c->cfg->eos[k], rks, pks, eps);
+      F->alpha[k] = a * sM;
+      F->masse[k] = a * rks * sM;
+      F->energ[k] = a * rks * eks * sM;
+    }
+    F->qdm[0] = rhoStar * sM * sM + pStar;
+    F->qdm[1] = rhoStar * vY * sM;
+    F->qdm[2] = rhoStar * vZ * sM;
+    F->energMix = (rhoStar * EStar + pStar) * sM;
+  }
+  F->sM = sM;
+}
+
+/* ModEuler::solveRiemannIntern (ModEuler.cpp:90-175) */
+static void hllc_euler(const Ctx *c, const State *Ls, const State *Rs, double dxL, double dxR,
+                       double *dtMax, Flux *F) {
+  (void)c;
+  const Phase *L = &Ls->ph[0], *R = &Rs->ph[0];
+  double uL = L->u[0], vL = L->u[1], wL = L->u[2], pL = L->p, rhoL = L->rho, cL = L->c, EL = L->Et;
+  double uR = R->u[0], vR = R->u[1], wR = R->u[2], pR = R->p, rhoR = R->rho, cR = R->c, ER = R->Et;
+  double sL = smin(uL - cL, uR - cR);
+  double sR = smax(uR + cR, uL + cL);
+  if (fabs(sL) > 1.e-3) *dtMax = smin(*dtMax, dxL / fabs(sL));
+  if (fabs(sR) > 1.e-3) *dtMax = smin(*dtMax, dxR / fabs(sR));
+  double mL = rhoL * (sL - uL), mR = rhoR * (sR - uR);
+  double sM = (pR - pL + mL * uL - mR * uR) / (mL - mR);
+  if (fabs(sM) < 1.e-8) sM = 0.;
+  if (sL > 0.) {
+    F->masse[0] = rhoL * uL;
+    F->qdm[0] = rhoL * uL * uL + pL; F->qdm[1] = rhoL * vL * uL; F->qdm[2] = rhoL * wL * uL;
+    F->energMix = (rhoL * EL + pL) * uL;
+  } else if (sR < 0.) {
+    F->masse[0] = rhoR * uR;
+    F->qdm[0] = rhoR * uR * uR + pR; F->qdm[1] = rhoR * vR * uR; F->qdm[2] = rhoR * wR * uR;
+    F->energMix = (rhoR * ER + pR) * uR;
+  } else if (sM >= 0.) {
+    double pStar = mL * (sM - uL) + pL;
+    double rhoStar = mL / (sL - sM);
+    double Estar = EL + (sM - uL) * (sM + pL / mL);
+    F->masse[0] = rhoStar * sM;
+    F->qdm[0] = rhoStar * sM * sM + pStar; F->qdm[1] = rhoStar * sM * vL; F->qdm[2] = rhoStar * sM * wL;
+    F->energMix = (rhoStar * Estar + pStar) * sM;
+  } else {
+    double pStar = mR * (sM - uR) + pR;
+    double rhoStar = mR / (sR - sM);
+    double Estar = ER + (sM - uR) * (sM + pR / mR);
+    F->masse[0] = rhoStar * sM;
+    F->qdm[0] = rhoStar * sM * sM + pStar; F->qdm[1] = rhoStar * sM * vR; F->qdm[2] = rhoStar * sM * wR;
+    F->energMix = (rhoStar * Estar + pStar) * sM;
+  }
+  F->sM = sM;
+}
+
+static void riemann(const Ctx *c, const State *L, const State *R, double dxL, double dxR,
+                    double *dtMax, Flux *F) {
+  if (c->cfg->model == ORC_EULER) hllc_euler(c, L, R, dxL, dxR, dtMax, F);
+  else hllc_multi(c, L, R, dxL, dxR, dtMax, F);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Mesh: src/Meshes/MeshCartesian.cpp:154-160, 180-552                       */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+  int left, right;    /* right = -1 for boundary faces */
+  int dir;            /* 0,1,2 */
+  double nrm[3], tan[3], bin[3];
+  double surface;
+  double pos[3];
+  int opp_left;       /* face opposite to this one across the left cell (-1: none/boundary) */
+  int opp_right;      /* face opposite across the right cell */
+} Face;
+
+typedef struct {
+  int ncell, nface, nint, dim;
+  int n[3];
+  double *posx, *posy, *posz, *dxs, *dys, *dzs;
+  double volume_dummy;
+  Face *faces;
+  int *cell_faces;    /* per cell, up to 6 face indices in accumulation order */
+  int *cell_nf;
+  double *vol, *lcfl;
+} Mesh;
+
+static inline void cell_pos(const Mesh *m, int c, double *p) {
+  int n0 = m->n[0], n1 = m->n[1];
+  int i = c % n0, j = (c / n0) % n1, k = c / (n0 * n1);
+  p[0] = m->posx[i]; p[1] = m->posy[j]; p[2] = m->posz[k];
+}
+static inline int cidx(const Mesh *m, int i, int j, int k) {
+  int idx = 0;
+  if (m->n[0] != 1) idx += i;
+  if (m->n[1] != 1) idx += j * m->n[0];
+  if (m->n[2] != 1) idx += k * m->n[0] * m->n[1];
+  return idx;
+}
+
+static void set3(double *v, double a, double b, double c) { v[0] = a; v[1] = b; v[2] = c; }
+
+static int mesh_build(Mesh *m, const orc_cfg *cfg) {
+  memset(m, 0, sizeof(*m));
+  for (int d = 0; d < 3; d++) m->n[d] = cfg->n[d];
+  int nx = m->n[0], ny = m->n[1], nz = m->n[2];
+  m->dim = (nx != 1) + (ny != 1) + (nz != 1);
+  m->ncell = nx * ny * nz;
+  double dX = cfg->L[0] / (double)nx, dY = cfg->L[1] / (double)ny, dZ = cfg->L[2] / (double)nz;
+  m->posx = malloc(sizeof(double) * nx); m->posy = malloc(sizeof(double) * ny); m->posz = malloc(sizeof(double) * nz);
+  for (int i = 0; i < nx; i++) m->posx[i] = (i + 0.5) * dX;
+  for (int i = 0; i < ny; i++) m->posy[i] = (i + 0.5) * dY;
+  for (int i = 0; i < nz; i++) m->posz[i] = (i + 0.5) * dZ;
+  double vol = dX * dY * dZ;
+  double lc = 1.e10;
+  if (nx != 1) lc = smin(lc, dX);
+  if (ny != 1) lc = smin(lc, dY);
+  if (nz != 1) lc = smin(lc, dZ);
+  if (m->dim > 1) lc *= 0.6;
+  m->vol = malloc(sizeof(double)); m->vol[0] = vol;
+  m->lcfl = malloc(sizeof(double)); m->lcfl[0] = lc;
+
+  int nf = 0, nb = 0;
+  if (nx != 1) { nf += (nx + 1) * ny * nz; nb += 2 * ny * nz; }
+  if (ny != 1) { nf += (ny + 1) * nx * nz; nb += 2 * nx * nz; }
+  if (nz != 1) { nf += (nz + 1) * nx * ny; nb += 2 * nx * ny; }
+  m->nface = nf; m->nint = nf - nb;
+  m->faces = calloc(nf, sizeof(Face));
+  int f = 0;
+  /* internal faces */
+  for (int ix = 0; ix < nx - 1; ix++) for (int iy = 0; iy < ny; iy++) for (int iz = 0; iz < nz; iz++) {
+    Face *F = &m->faces[f++];
+    F->left = cidx(m, ix, iy, iz); F->right = cidx(m, ix + 1, iy, iz); F->dir = 0;
+    set3(F->tan, 0., 1., 0.); set3(F->nrm, 1., 0., 0.); set3(F->bin, 0., 0., 1.);
+    F->surface = dY * dZ;
+    set3(F->pos, m->posx[ix] + 0.5 * dX, m->posy[iy], m->posz[iz]);
+  }
+  for (int ix = 0; ix < nx; ix++) for (int iy = 0; iy < ny - 1; iy++) for (int iz = 0; iz < nz; iz++) {
+    Face *F = &m->faces[f++];
+    F->left = cidx(m, ix, iy, iz); F->right = cidx(m, ix, iy + 1, iz); F->dir = 1;
+    set3(F->tan, -1., 0., 0.); set3(F->nrm, 0., 1., 0.); set3(F->bin, 0., 0., 1.);
+    F->surface = dX * dZ;
+    set3(F->pos, m->posx[ix], m->posy[iy] + 0.5 * dY, m->posz[iz]);
+  }
+  for (int ix = 0; ix < nx; ix++) for (int iy = 0; iy < ny; iy++) for (int iz = 0; iz < nz - 1; iz++) {
+    Face *F = &m->faces[f++];
+    F->left = cidx(m, ix, iy, iz); F->right = cidx(m, ix, iy, iz + 1); F->dir = 2;
+    set3(F->tan, 1., 0., 0.); set3(F->nrm, 0., 0., 1.); set3(F->bin, 0., 1., 0.);
+    F->surface = dX * dY;
+    set3(F->pos, m->posx[ix], m->posy[iy], m->posz[iz] + 0.5 * dZ);
+  }
+  /* boundary faces (all transmissive), MeshCartesian.cpp:399-551 */
+  if (nx != 1) {
+    for (int iy = 0; iy < ny; iy++) for (int iz = 0; iz < nz; iz++) {
+      Face *F = &m->faces[f++];
+      F->left = cidx(m, 0, iy, iz); F->right = -1; F->dir = 0;
+      set3(F->tan, 0., -1., 0.); set3(F->nrm, -1., 0., 0.); set3(F->bin, 0., 0., 1.);
+      F->surface = dY * dZ; set3(F->pos, 0., m->posy[iy], m->posz[iz]);
+    }
+    for (int iy = 0; iy < ny; iy++) for (int iz = 0; iz < nz; iz++) {
+      Face *F = &m->faces[f++];
+      F->left = cidx(m, nx - 1, iy, iz); F->right = -1; F->dir = 0;
+      set3(F->tan, 0., 1., 0.); set3(F->nrm, 1., 0., 0.); set3(F->bin, 0., 0., 1.);
+      F->surface = dY * dZ; set3(F->pos, m->posx[nx - 1] + 0.5 * dX, m->posy[iy], m->posz[iz]);
+    }
+  }
+  if (ny != 1) {
+    for (int ix = 0; ix < nx; ix++) for (int iz = 0; iz < nz; iz++) {
+      Face *F = &m->faces[f++];
+      F->left = cidx(m, ix, 0, iz); F->right = -1; F->dir = 1;
+      set3(F->tan, 1., 0., 0.); set3(F->nrm, 0., -1., 0.); set3(F->bin, 0., 0., 1.);
+      F->surface = dX * dZ; set3(F->pos, m->posx[ix], 0., m->posz[iz]);
+    }
+    for (int ix = 0; ix < nx; ix++) for (int iz = 0; iz < nz; iz++) {
+      Face *F = &m->faces[f++];
+      F->left = cidx(m, ix, ny - 1, iz); F->right = -1; F->dir = 1;
+      set3(F->tan, -1., 0., 0.); set3(F->nrm, 0., 1., 0.); set3(F->bin, 0., 0., 1.);
+      F->surface = dX * dZ; set3(F->pos, m->posx[ix], m->posy[ny - 1] + 0.5 * dY, m->posz[iz]);
+    }
+  }
+  if (nz != 1) {
+    for (int ix = 0; ix < nx; ix++) for (int iy = 0; iy < ny; iy++) {
+      Face *F = &m->faces[f++];
+      F->left = cidx(m, ix, iy, 0); F->right = -1; F->dir = 2;
+      set3(F->tan, -1., 0., 0.); set3(F->nrm, 0., 0., -1.); set3(F->bin, 0., 1., 0.);
+      F->surface = dX * dY; set3(F->pos, m->posx[ix], m->posy[iy], 0.);
+    }
+    for (int ix = 0; ix < nx; ix++) for (int iy = 0; iy < ny; iy++) {
+      Face *F = &m->faces[f++];
+      F->left = cidx(m, ix, iy, nz - 1); F->right = -1; F->dir = 2;
+      set3(F->tan, 1., 0., 0.); set3(F->nrm, 0., 0., 1.); set3(F->bin, 0., 1., 0.);
+      F->surface = dX * dY; set3(F->pos, m->posx[ix], m->posy[iy], m->posz[nz - 1] + 0.5 * dZ);
+    }
+  }
+  /* per-cell face lists in global face order (accumulation order of Run.cpp:401) */
+  m->cell_faces = malloc(sizeof(int) * 6 * m->ncell);
+  m->cell_nf = calloc(m->ncell, sizeof(int));
+  for (int fi = 0; fi < nf; fi++) {
+    Face *F = &m->faces[fi];
+    m->cell_faces[6 * F->left + m->cell_nf[F->left]++] = fi;
+    if (F->right >= 0) m->cell_faces[6 * F->right + m->cell_nf[F->right]++] = fi;
+  }
+  /* opposite internal faces (the parallel face on the other side of a cell) */
+  for (int fi = 0; fi < m->nint; fi++) {
+    Face *F = &m->faces[fi];
+    F->opp_left = -1; F->opp_right = -1;
+    for (int q = 0; q < m->cell_nf[F->left]; q++) {
+      int g = m->cell_faces[6 * F->left + q];
+      if (g != fi && g < m->nint && m->faces[g].dir == F->dir) F->opp_left = g;
+    }
+    for (int q = 0; q < m->cell_nf[F->right]; q++) {
+      int g = m->cell_faces[6 * F->right + q];
+      if (g != fi && g < m->nint && m->faces[g].dir == F->dir) F->opp_right = g;
+    }
+  }
+  return 0;
+}
+
+static void mesh_free(Mesh *m) {
+  free(m->posx); free(m->posy); free(m->posz); free(m->faces); free(m->cell_faces); free(m->cell_nf);
+  free(m->vol); free(m->lcfl);
+}
+
+static double dist3(const double *a, const double *b) {
+  double v[3] = {a[0] - b[0], a[1] - b[1], a[2] - b[2]};
+  return sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+}
+
+/* Coord::localProjection / reverseProjection (Coord.cpp:132-150) */
+static void proj(double *u, const Face *F) {
+  double a = u[0] * F->nrm[0] + u[1] * F->nrm[1] + u[2] * F->nrm[2];
+  double b = u[0] * F->tan[0] + u[1] * F->tan[1] + u[2] * F->tan[2];
+  double c = u[0] * F->bin[0] + u[1] * F->bin[1] + u[2] * F->bin[2];
+  u[0] = a; u[1] = b; u[2] = c;
+}
+static void rproj(double *u, const Face *F) {
+  double a = F->nrm[0] * u[0] + F->tan[0] * u[1] + F->bin[0] * u[2];
+  double b = F->nrm[1] * u[0] + F->tan[1] * u[1] + F->bin[1] * u[2];
+  double c = F->nrm[2] * u[0] + F->tan[2] * u[1] + F->bin[2] * u[2];
+  u[0] = a; u[1] = b; u[2] = c;
+}
+static void proj_state(const Ctx *c, State *s, const Face *F) {
+  if (c->cfg->model == ORC_EULER) proj(s->ph[0].u, F);
+  else proj(s->mx.u, F);
+}
+static void rproj_state(const Ctx *c, State *s, const Face *F) {
+  if (c->cfg->model == ORC_EULER) rproj(s->ph[0].u, F);
+  else rproj(s->mx.u, F);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Solver storage                                                            */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+  Ctx c;
+  Mesh m;
+  State *sn, *s2;     /* vecPhases and vecPhasesO2 */
+  Flux *cons;
+  Flux *fflux;        /* per-face flux (after reverse projection) */
+  double *fslope;     /* per-face primitive slopes (nvar per face) */
+  int nvar;
+  double *fdt;        /* per-face dtMax contributions */
+} Solver;
+
+/* Primitive slope vector: phases (alpha, rho, p) or Euler (rho, p, u, v, w); mixture (u, v, w). */
+static void prim_vec(const Ctx *c, const State *s, double *w) {
+  int q = 0;
+  if (c->cfg->model == ORC_EULER) {
+    w[q++] = s->ph[0].rho; w[q++] = s->ph[0].p;
+    w[q++] = s->ph[0].u[0]; w[q++] = s->ph[0].u[1]; w[q++] = s->ph[0].u[2];
+    return;
+  }
+  for (int k = 0; k < c->cfg->nph; k++) { w[q++] = s->ph[k].alpha; w[q++] = s->ph[k].rho; w[q++] = s->ph[k].p; }
+  w[q++] = s->mx.u[0]; w[q++] = s->mx.u[1]; w[q++] = s->mx.u[2];
+}
+
+/* CellInterfaceO2::computeSlopes (CellInterfaceO2.cpp:132-152) */
+static void compute_face_slopes(Solver *S, State *st) {
+  Mesh *m = &S->m;
+  int nv = S->nvar;
+#pragma omp parallel for schedule(static)
+  for (int fi = 0; fi < m->nint; fi++) {
+    Face *F = &m->faces[fi];
+    double pl[3], pr[3], wl[16], wr[16];
+    cell_pos(m, F->left, pl); cell_pos(m, F->right, pr);
+    double dist = dist3(pl, pr);
+    prim_vec(&S->c, &st[F->left], wl);
+    prim_vec(&S->c, &st[F->right], wr);
+    for (int v = 0; v < nv; v++) S->fslope[(size_t)fi * nv + v] = (wr[v] - wl[v]) / dist;
+  }
+}
+
+/* CellO2::computeLocalSlopes (CellO2.cpp:101-247) on a Cartesian mesh: slope1 is
+   the reference face's slope, slope2 the opposite parallel internal face's
+   (0 next to a boundary).  alpha_after: phase-0 alpha (time n) of the cell
+   across the opposite face.  Returns the limited slope vector. */
+static void local_slopes(Solver *S, int cell, int fref, int fopp, int cell_is_left,
+                         double alpha_cell, double alpha_other, double *alpha_after, double *out) {
+  const Ctx *c = &S->c;
+  const orc_cfg *cfg = c->cfg;
+  int nv = S->nvar;
+  const double *s1 = &S->fslope[(size_t)fref * nv];
+  double s2[16];
+  double aafter = 0.;
+  if (fopp >= 0) {
+    for (int v = 0; v < nv; v++) s2[v] = S->fslope[(size_t)fopp * nv + v];
+    const Face *G = &S->m.faces[fopp];
+    int other = (G->left == cell) ? G->right : G->left;
+    (void)cell_is_left;
+    aafter = (cfg->model == ORC_EULER) ? 0. : S->sn[other].ph[0].alpha;
+  } else {
+    for (int v = 0; v < nv; v++) s2[v] = 0.;
+  }
+  *alpha_after = aafter;
+  double epsI = 1.e-4;
+  int iface = (alpha_cell >= epsI) && (alpha_cell <= 1. - epsI) &&
+              ((alpha_other - alpha_cell) * (alpha_cell - aafter) >= 1.e-8);
+  int lg = iface ? cfg->lim[1] : cfg->lim[0];
+  int lv = iface ? cfg->lim[3] : cfg->lim[2];
+  if (cfg->model == ORC_EULER) {
+    for (int v = 0; v < nv; v++) out[v] = limiter(lg, s1[v], s2[v]);
+    return;
+  }
+  int q = 0;
+  for (int k = 0; k < cfg->nph; k++) {
+    out[q] = limiter(lv, s1[q], s2[q]); q++;
+    out[q] = limiter(lg, s1[q], s2[q]); q++;
+    out[q] = limiter(lg, s1[q], s2[q]); q++;
+  }
+  for (int d = 0; d < 3; d++, q++) out[q] = limiter(lg, s1[q], s2[q]);
+}
+
+static void extrapolate(const Ctx *c, State *s, const double *sl, double dist, double sign) {
+  int q = 0;
+  if (c->cfg->model == ORC_EULER) {
+    Phase *p = &s->ph[0];
+    p->rho += (sign * sl[0]) * dist; p->p += (sign * sl[1]) * dist;
+    p->u[0] = p->u[0] + (sign * sl[2]) * dist;
+    p->u[1] = p->u[1] + (sign * sl[3]) * dist;
+    p->u[2] = p->u[2] + (sign * sl[4]) * dist;
+    verify_correct(c, 0, p);
+    return;
+  }
+  for (int k = 0; k < c->cfg->nph; k++) {
+    Phase *p = &s->ph[k];
+    p->alpha += (sign * sl[q]) * dist; q++;
+    p->rho += (sign * sl[q]) * dist; q++;
+    p->p += (sign * sl[q]) * dist; q++;
+    verify_correct(c, k, p);
+  }
+  for (int d = 0; d < 3; d++, q++) s->mx.u[d] = s->mx.u[d] + (sign * sl[q]) * dist;
+}
+
+/* THINC reconstruction, CellInterfaceO2.cpp:228-247 (left state) and 259-277
+   (right state).  a_hi / a_lo are the phase-0 fractions on the two sides of
+   the cell as the reference orders them: (right cell, left-left cell) for the
+   left state, (right-right cell, left cell) for the right state. */
+static void thinc(State *s, double acell, double a_hi, double a_lo, int right_side) {
+  double beta = 1.6, epsI = 1.e-4, sign;
+  if ((acell >= epsI) && (acell <= 1. - epsI) && ((a_hi - acell) * (acell - a_lo) >= 1.e-8)) {
+    if (a_hi - a_lo > 0.) sign = 1.; else sign = -1.;
+    double qmin = smin(a_hi, a_lo);
+    double qmax = smax(a_hi, a_lo) - qmin;
+    double C = (acell - qmin + 1.e-20) / (qmax + 1.e-20);
+    double B = exp(sign * beta * (2. * C - 1.));
+    double A = (B / cosh(beta) - 1.) / tanh(beta);
+    double na;
+    if (!right_side) na = qmin + 0.5 * qmax * (1. + sign * (tanh(beta) + A) / (1. + A * tanh(beta)));
+    else na = qmin + 0.5 * qmax * (1. + sign * A);
+    if (na < epsI) na = epsI;
+    if (na > 1. - epsI) na = 1. - epsI;
+    s->ph[0].alpha = na;
+    s->ph[1].alpha = 1. - na;
+  }
+}
+
+static int is_thinc(const orc_cfg *cfg) { return cfg->lim[2] == ORC_LIM_THINC || cfg->lim[3] == ORC_LIM_THINC; }
+
+/* Face Riemann problems of one pass (Run.cpp:356,381,401); fluxes stored per face. */
+static void face_fluxes(Solver *S, State *st, int o2) {
+  Mesh *m = &S->m;
+  const Ctx *c = &S->c;
+  double lcfl = m->lcfl[0];
+#pragma omp parallel for schedule(static)
+  for (int fi = 0; fi < m->nface; fi++) {
+    Face *F = &m->faces[fi];
+    double dtm = 1.e10;
+    Flux *FL = &S->fflux[fi];
+    memset(FL, 0, sizeof(Flux));
+    if (F->right < 0) {
+      /* BoundCond::solveRiemann (BoundCond.cpp:82-100) + BoundCondAbs (BoundCondAbs.cpp:70-73) */
+      State L = st[F->left];
+      proj_state(c, &L, F);
+      fulfill(c, &L);
+      riemann(c, &L, &L, lcfl, lcfl, &dtm, FL);
+    } else if (!o2) {
+      /* CellInterface::solveRiemann (CellInterface.cpp:127-146): the cells' own cached state */
+      State L = st[F->left], R = st[F->right];
+      proj_state(c, &L, F); proj_state(c, &R, F);
+      riemann(c, &L, &R, lcfl, lcfl, &dtm, FL);
+    } else {
+      /* CellInterfaceO2::solveRiemann (CellInterfaceO2.cpp:197-295) */
+      State L = st[F->left], R = st[F->right];
+      double pl[3], pr[3];
+      cell_pos(m, F->left, pl); cell_pos(m, F->right, pr);
+      double dL = dist3(F->pos, pl), dR = dist3(F->pos, pr);
+      int multi = (c->cfg->model != ORC_EULER);
+      double aL = multi ? L.ph[0].alpha : 0., aR = multi ? R.ph[0].alpha : 0.;
+      double slL[16], slR[16], aLL, aRR;
+      local_slopes(S, F->left, fi, F->opp_left, 1, aL, aR, &aLL, slL);
+      extrapolate(c, &L, slL, dL, 1.);
+      if (is_thinc(c->cfg)) thinc(&L, aL, aR, aLL, 0);
+      local_slopes(S, F->right, fi, F->opp_right, 0, aR, aL, &aRR, slR);
+      extrapolate(c, &R, slR, dR, -1.);
+      if (is_thinc(c->cfg)) thinc(&R, aR, aRR, aL, 1);
+      proj_state(c, &L, F); proj_state(c, &R, F);
+      fulfill(c, &L); fulfill(c, &R);
+      riemann(c, &L, &R, lcfl, lcfl, &dtm, FL);
+    }
+    rproj(FL->qdm, F);
+    S->fdt[fi] = dtm;
+  }
+}
+
+/* CellInterface::addFlux / subtractFlux (CellInterface.cpp:150-180) with the
+   Flux*::addNonCons / subtractNonCons terms (FluxKapila.cpp:215-235). The
+   non-conservative terms use the owning cell's time-n alpha_k, p_k. */
+static void accumulate(Solver *S) {
+  Mesh *m = &S->m;
+  const Ctx *c = &S->c;
+  int n = c->cfg->nph;
+  int euler = (c->cfg->model == ORC_EULER);
+  double vol = m->vol[0];
+#pragma omp parallel for schedule(static)
+  for (int ci = 0; ci < m->ncell; ci++) {
+    Flux *U = &S->cons[ci];
+    memset(U, 0, sizeof(Flux));
+    const State *sn = &S->sn[ci];
+    for (int q = 0; q < m->cell_nf[ci]; q++) {
+      const Face *F = &m->faces[m->cell_faces[6 * ci + q]];
+      const Flux *FL = &S->fflux[m->cell_faces[6 * ci + q]];
+      double coefA = F->surface / vol;
+      coefA = coefA * 1.;
+      if (F->right == ci) {
+        if (euler) {
+          U->masse[0] += coefA * FL->masse[0];
+        } else {
+          for (int k = 0; k < n; k++) {
+            U->alpha[k] += coefA * FL->alpha[k];
+            U->masse[k] += coefA * FL->masse[k];
+            U->energ[k] += coefA * FL->energ[k];
+          }
+        }
+        for (int d = 0; d < 3; d++) U->qdm[d] += FL->qdm[d] * coefA;
+        U->energMix += coefA * FL->energMix;
+        if (!euler)
+          for (int k = 0; k < n; k++) {
+            U->alpha[k] += -coefA * sn->ph[k].alpha * FL->sM;
+            U->energ[k] += coefA * sn->ph[k].alpha * sn->ph[k].p * FL->sM;
+          }
+      } else {
+        if (euler) {
+          U->masse[0] -= coefA * FL->masse[0];
+        } else {
+          for (int k = 0; k < n; k++) {
+            U->alpha[k] -= coefA * FL->alpha[k];
+            U->masse[k] -= coefA * FL->masse[k];
+            U->energ[k] -= coefA * FL->energ[k];
+          }
+        }
+        for (int d = 0; d < 3; d++) U->qdm[d] -= FL->qdm[d] * coefA;
+        U->energMix -= coefA * FL->energMix;
+        if (!euler)
+          for (int k = 0; k < n; k++) {
+            U->alpha[k] -= -coefA * sn->ph[k].alpha * FL->sM;
+            U->energ[k] -= coefA * sn->ph[k].alpha * sn->ph[k].p * FL->sM;
+          }
+      }
+    }
+  }
+}
+
+static double reduce_dt(Solver *S) {
+  double d = 1.e10;
+  for (int fi = 0; fi < S->m.nface; fi++) d = smin(d, S->fdt[fi]);
+  return d;
+}
+
+/* U <- U*mult + Un, the Flux::multiply / addFlux(1.) pair of Cell::timeEvolution
+   (Cell.cpp:214-227) and CellO2::predictionOrdre2 (CellO2.cpp:376-397). */
+static void scale_add(const Ctx *c, Flux *U, double mult, const Flux *buf) {
+  int n = c->cfg->nph;
+  if (c->cfg->model == ORC_EULER) {
+    U->masse[0] *= mult;
+  } else {
+    for (int k = 0; k < n; k++) { U->alpha[k] *= mult; U->masse[k] *= mult; U->energ[k] *= mult; }
+  }
+  for (int d = 0; d < 3; d++) U->qdm[d] *= mult;
+  U->energMix *= mult;
+  if (c->cfg->model == ORC_EULER) {
+    U->masse[0] += 1. * buf->masse[0];
+  } else {
+    for (int k = 0; k < n; k++) {
+      U->alpha[k] += 1. * buf->alpha[k];
+      U->masse[k] += 1. * buf->masse[k];
+      U->energ[k] += 1. * buf->energ[k];
+    }
+  }
+  for (int d = 0; d < 3; d++) U->qdm[d] += buf->qdm[d] * 1.;
+  U->energMix += 1. * buf->energMix;
+}
+
+/* Run::solveRelaxations (Run.cpp:457-474) on the time-n state. */
+static void solve_relaxations(Solver *S) {
+  const Ctx *c = &S->c;
+  if (c->cfg->nph < 2 || c->cfg->model == ORC_EULER) return;
+#pragma omp parallel for schedule(static)
+  for (int ci = 0; ci < S->m.ncell; ci++) {
+    State *s = &S->sn[ci];
+    if (model_relaxes(c->cfg)) relax_p(c, s);
+    total_to_internal(c, s);
+    correction_energy(c, s, s);
+  }
+}
+
+/* One time step of Run::solver (Run.cpp:204-277 / 281-412). Returns dtMax. */
+static double step(Solver *S, double dt) {
+  const Ctx *c = &S->c;
+  Mesh *m = &S->m;
+  double dtMax = 1.e10;
+  if (c->cfg->order == 1) {
+    face_fluxes(S, S->sn, 0);
+    dtMax = smin(dtMax, reduce_dt(S));
+    accumulate(S);
+#pragma omp parallel for schedule(static)
+    for (int ci = 0; ci < m->ncell; ci++) {
+      Flux buf; memset(&buf, 0, sizeof(buf));
+      build_cons(c, &S->sn[ci], &buf);
+      scale_add(c, &S->cons[ci], dt, &buf);
+      scheme_correction(c, &S->cons[ci], &buf);
+      build_prim(c, &S->cons[ci], &S->sn[ci]);
+    }
+  } else {
+    /* integrationProcedure: slopes on U^n (Run.cpp:293-295) */
+    compute_face_slopes(S, S->sn);
+    face_fluxes(S, S->sn, 1);
+    dtMax = smin(dtMax, reduce_dt(S));
+    accumulate(S);
+    /* CellO2::predictionOrdre2 (CellO2.cpp:376-397) */
+#pragma omp parallel for schedule(static)
+    for (int ci = 0; ci < m->ncell; ci++) {
+      Flux buf; memset(&buf, 0, sizeof(buf));
+      State *sn = &S->sn[ci], *s2 = &S->s2[ci];
+      build_cons(c, sn, &buf);
+      scale_add(c, &S->cons[ci], 0.5 * dt, &buf);
+      scheme_correction(c, &S->cons[ci], &buf);
+      build_prim(c, &S->cons[ci], s2);
+      if (model_relaxes(c->cfg) && c->cfg->nph > 1) relax_p(c, s2);
+      total_to_internal(c, s2);
+      correction_energy(c, s2, sn);
+      fulfill(c, s2);
+    }
+    /* second pass on U* (Run.cpp:372-390) */
+    compute_face_slopes(S, S->s2);
+    face_fluxes(S, S->s2, 1);
+    dtMax = smin(dtMax, reduce_dt(S));
+    accumulate(S);
+#pragma omp parallel for schedule(static)
+    for (int ci = 0; ci < m->ncell; ci++) {
+      Flux buf; memset(&buf, 0, sizeof(buf));
+      build_cons(c, &S->sn[ci], &buf);
+      scale_add(c, &S->cons[ci], dt, &buf);
+      scheme_correction(c, &S->cons[ci], &buf);
+      build_prim(c, &S->cons[ci], &S->sn[ci]);
+    }
+  }
+  solve_relaxations(S);
+  return dtMax;
+}
+
+/* Initial state: Cell::fill + Cell::completeFulfillState (Cell.cpp:127-155, 271-278). */
+static void init_state(Solver *S, const double *in) {
+  const Ctx *c = &S->c;
+  int n = c->cfg->nph, nc = S->m.ncell;
+  for (int ci = 0; ci < nc; ci++) {
+    State *s = &S->sn[ci];
+    memset(s, 0, sizeof(State));
+    double u[3] = {in[(size_t)(3 * n) * nc + ci], in[(size_t)(3 * n + 1) * nc + ci], in[(size_t)(3 * n + 2) * nc + ci]};
+    if (c->cfg->model == ORC_EULER) {
+      s->ph[0].rho = in[(size_t)1 * nc + ci];
+      s->ph[0].p = in[(size_t)2 * nc + ci];
+      for (int d = 0; d < 3; d++) s->ph[0].u[d] = u[d];
+      fulfill(c, s);
+      continue;
+    }
+    for (int k = 0; k < n; k++) {
+      s->ph[k].alpha = in[(size_t)(3 * k) * nc + ci];
+      s->ph[k].rho = in[(size_t)(3 * k + 1) * nc + ci];
+      s->ph[k].p = in[(size_t)(3 * k + 2) * nc + ci];
+    }
+    for (int d = 0; d < 3; d++) s->mx.u[d] = u[d];
+    fulfill(c, s);
+    s->mx.E = s->mx.e + 0.5 * sqnorm(s->mx.u);
+  }
+}
+
+static void write_state(Solver *S, double *out) {
+  const Ctx *c = &S->c;
+  int n = c->cfg->nph, nc = S->m.ncell;
+  for (int ci = 0; ci < nc; ci++) {
+    const State *s = &S->sn[ci];
+    if (c->cfg->model == ORC_EULER) {
+      out[(size_t)0 * nc + ci] = 1.;
+      out[(size_t)1 * nc + ci] = s->ph[0].rho;
+      out[(size_t)2 * nc + ci] = s->ph[0].p;
+      for (int d = 0; d < 3; d++) out[(size_t)(3 + d) * nc + ci] = s->ph[0].u[d];
+      out[(size_t)6 * nc + ci] = s->ph[0].Et;
+      continue;
+    }
+    for (int k = 0; k < n; k++) {
+      out[(size_t)(3 * k) * nc + ci] = s->ph[k].alpha;
+      out[(size_t)(3 * k + 1) * nc + ci] = s->ph[k].rho;
+      out[(size_t)(3 * k + 2) * nc + ci] = s->ph[k].p;
+    }
+    for (int d = 0; d < 3; d++) out[(size_t)(3 * n + d) * nc + ci] = s->mx.u[d];
+    out[(size_t)(3 * n + 3) * nc + ci] = s->mx.E;
+  }
+}
+
+int orc_run(const orc_cfg *cfg, const double *state_in, double *state_out,
+            int nsteps, double t_final, int max_steps,
+            double *t_out, double *dt_next_out, double *dt_log, int nthreads) {
+  if (!cfg || cfg->nph < 1 || cfg->nph > MAXPH) return -1;
+  if (cfg->model == ORC_EULER && cfg->nph != 1) return -1;
+  if (is_thinc(cfg) && cfg->nph != 2) return -1;
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+  (void)nthreads;
+#endif
+  Solver S;
+  memset(&S, 0, sizeof(S));
+  S.c.cfg = cfg;
+  S.c.epsilon = (cfg->model == ORC_KAPILA && cfg->alpha_null) ? 1.e-15 : 0.;
+  mesh_build(&S.m, cfg);
+  S.nvar = (cfg->model == ORC_EULER) ? 5 : 3 * cfg->nph + 3;
+  S.sn = calloc(S.m.ncell, sizeof(State));
+  S.s2 = calloc(S.m.ncell, sizeof(State));
+  S.cons = calloc(S.m.ncell, sizeof(Flux));
+  S.fflux = calloc(S.m.nface, sizeof(Flux));
+  S.fslope = calloc((size_t)S.m.nint * S.nvar + 1, sizeof(double));
+  S.fdt = calloc(S.m.nface, sizeof(double));
+  if (!S.sn || !S.s2 || !S.cons || !S.fflux || !S.fslope || !S.fdt) return -2;
+  init_state(&S, state_in);
+
+  double dt = 1.e-15, t = 0.;
+  int it = 0;
+  float tf = (float)t_final;
+  for (;;) {
+    double dtMax = step(&S, dt);
+    if (dt_log) dt_log[it] = dt;
+    t += dt;
+    it++;
+    int done = (nsteps > 0) ? (it >= nsteps) : (t >= (double)tf);
+    dt = cfg->cfl * dtMax;
+    if (done || (max_steps > 0 && it >= max_steps)) break;
+  }
+  write_state(&S, state_out);
+  if (t_out) *t_out = t;
+  if (dt_next_out) *dt_next_out = dt;
+  free(S.sn); free(S.s2); free(S.cons); free(S.fflux); free(S.fslope); free(S.fdt);
+  mesh_free(&S.m);
+  return it;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Single-function entry points for known-answer tests                       */
+/* ------------------------------------------------------------------------ */
+double orc_limiter(int lim, double s1, double s2) { return limiter(lim, s1, s2); }
+double orc_eos_energy(const orc_eos *e, double rho, double p, double eps) { return eos_e(e, rho, p, eps); }
+double orc_eos_pressure(const orc_eos *e, double rho, double ein) { return eos_p(e, rho, ein); }
+double orc_eos_sound(const orc_eos *e, double rho, double p, double eps) { return eos_c(e, rho, p, eps); }
+double orc_eos_isentrope(const orc_eos *e, double p0, double rho0, double rho1, double eps) {
+  return eos_isentrope(e, p0, rho0, rho1, eps);
+}
+double orc_eos_rho_pfinal(const orc_eos *e, double p0, double rho0, double pf, double *drho, double eps) {
+  return eos_rho_pfinal(e, p0, rho0, pf, drho, eps);
+}
+
+int orc_relax_p(const orc_cfg *cfg, double *alpha, double *rho, double *p) {
+  Ctx c = {cfg, (cfg->model == ORC_KAPILA && cfg->alpha_null) ? 1.e-15 : 0.};
+  State s; memset(&s, 0, sizeof(s));
+  for (int k = 0; k < cfg->nph; k++) { s.ph[k].alpha = alpha[k]; s.ph[k].rho = rho[k]; s.ph[k].p = p[k]; }
+  int it = relax_p(&c, &s);
+  for (int k = 0; k < cfg->nph; k++) { alpha[k] = s.ph[k].alpha; rho[k] = s.ph[k].rho; p[k] = s.ph[k].p; }
+  return it;
+}
+
+void orc_hllc(const orc_cfg *cfg, const double *wl, const double *wr, double dxl, double dxr,
+              double *flux_out, double *dtmax) {
+  Ctx c = {cfg, (cfg->model == ORC_KAPILA && cfg->alpha_null) ? 1.e-15 : 0.};
+  State L, R; memset(&L, 0, sizeof(L)); memset(&R, 0, sizeof(R));
+  int n = cfg->nph;
+  const double *w[2] = {wl, wr};
+  State *s[2] = {&L, &R};
+  for (int side = 0; side < 2; side++) {
+    if (cfg->model == ORC_EULER) {
+      s[side]->ph[0].rho = w[side][1]; s[side]->ph[0].p = w[side][2];
+      for (int d = 0; d < 3; d++) s[side]->ph[0].u[d] = w[side][3 + d];
+    } else {
+      for (int k = 0; k < n; k++) {
+        s[side]->ph[k].alpha = w[side][3 * k]; s[side]->ph[k].rho = w[side][3 * k + 1]; s[side]->ph[k].p = w[side][3 * k + 2];
+      }
+      for (int d = 0; d < 3; d++) s[side]->mx.u[d] = w[side][3 * n + d];
+    }
+    fulfill(&c, s[side]);
+  }
+  Flux F; memset(&F, 0, sizeof(F));
+  riemann(&c, &L, &R, dxl, dxr, dtmax, &F);
+  int q = 0;
+  if (cfg->model == ORC_EULER) {
+    flux_out[q++] = F.masse[0];
+  } else {
+    for (int k = 0; k < n; k++) flux_out[q++] = F.alpha[k];
+    for (int k = 0; k < n; k++) flux_out[q++] = F.masse[k];
+    for (int k = 0; k < n; k++) flux_out[q++] = F.energ[k];
+  }
+  for (int d = 0; d < 3; d++) flux_out[q++] = F.qdm[d];
+  flux_out[q++] = F.energMix;
+  flux_out[q++] = F.sM;
+}

@@ -1,0 +1,83 @@
+/*
+ * CPU oracle for the explicit finite-volume step (TEST INFRASTRUCTURE ONLY).
+ *
+ * A plain-C, FP64 restatement of the reference ECOGEN step on Cartesian
+ * meshes (Euler / Kapila / MultiP models, IG and SG equations of state,
+ * first order or MUSCL predictor-corrector, the six limiters and THINC,
+ * stiff pressure relaxation, energy correction, transmissive boundaries and
+ * the lagged CFL time step).  It keeps the reference's full per-cell state
+ * (including the derived quantities the reference caches between calls) so
+ * that it reproduces the reference's arithmetic operation by operation.
+ *
+ * Only the tests, the smoke check and the CPU-baseline leg of the benchmark
+ * use this library, and only as the checker.  The product path never calls
+ * it.
+ */
+#ifndef ECO_ORACLE_H
+#define ECO_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { ORC_EULER = 0, ORC_KAPILA = 1, ORC_MULTIP = 2 };
+enum { ORC_LIM_NONE = 0, ORC_LIM_MINMOD = 1, ORC_LIM_VANLEER = 2, ORC_LIM_VANALBADA = 3,
+       ORC_LIM_SUPERBEE = 4, ORC_LIM_MC = 5, ORC_LIM_THINC = 6 };
+enum { ORC_EOS_IG = 0, ORC_EOS_SG = 1 };
+#define ORC_MAXPH 4
+
+typedef struct {
+  int type;
+  double gamma, pinf, cv, eref;
+} orc_eos;
+
+typedef struct {
+  int model;         /* ORC_EULER / ORC_KAPILA / ORC_MULTIP */
+  int nph;           /* 1 for Euler */
+  int order;         /* 1 or 2 */
+  int lim[4];        /* global, interface, global volume fraction, interface volume fraction */
+  int alpha_null;    /* Kapila alphaNull option (epsilon = 1e-15 if set, else 0) */
+  int relax_p;       /* MultiP: stiff pressure relaxation requested (Kapila always relaxes) */
+  int n[3];
+  double L[3];
+  double cfl;
+  orc_eos eos[ORC_MAXPH];
+} orc_cfg;
+
+/*
+ * State layout (field-major SoA, ncell = n[0]*n[1]*n[2] doubles per field,
+ * cell index i + j*n0 + k*n0*n1):
+ *   for each phase k: alpha_k, rho_k, p_k      (3*nph fields; Euler: alpha ignored)
+ *   u, v, w                                    (3 fields)
+ *   E  (mixture specific total energy; output only)
+ * => nfields = 3*nph + 4.
+ *
+ * orc_run: initialise from (alpha, rho, p, u) exactly as the reference does
+ * at t = 0, then march.  nsteps > 0: iteration-controlled run of nsteps
+ * steps; nsteps <= 0: time-controlled run to t_final (compared as float, as
+ * the reference reads totalTime with QueryFloatAttribute).  dt_log (optional,
+ * length >= steps taken) receives the dt used by each step.
+ * Returns the number of steps taken, or a negative error code.
+ */
+int orc_run(const orc_cfg *cfg, const double *state_in, double *state_out,
+            int nsteps, double t_final, int max_steps,
+            double *t_out, double *dt_next_out, double *dt_log, int nthreads);
+
+/* Single-function checks used by the known-answer tests. */
+double orc_limiter(int lim, double s1, double s2);
+double orc_eos_energy(const orc_eos *e, double rho, double p, double epsilon);
+double orc_eos_pressure(const orc_eos *e, double rho, double ein);
+double orc_eos_sound(const orc_eos *e, double rho, double p, double epsilon);
+double orc_eos_isentrope(const orc_eos *e, double p0, double rho0, double rho1, double epsilon);
+double orc_eos_rho_pfinal(const orc_eos *e, double p0, double rho0, double pf, double *drho, double epsilon);
+/* Relaxation on one cell: in/out alpha, rho, p (nph each). Returns iterations (100 = not applied). */
+int orc_relax_p(const orc_cfg *cfg, double *alpha, double *rho, double *p);
+/* One HLLC solve in the face frame, inputs are primitive states (alpha, rho, p per phase; u,v,w).
+   flux_out: alpha[nph], mass[nph], energy[nph], qdm[3], E, sM  (Euler: mass, qdm[3], E, sM). */
+void orc_hllc(const orc_cfg *cfg, const double *wl, const double *wr, double dxl, double dxr,
+              double *flux_out, double *dtmax);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
