@@ -1,0 +1,193 @@
+"""ctypes binding of the C ABI (include/ecogen.h) of ``libecogen_amd.so``.
+
+The library is built in-tree by ``ecogen_amd.build``; this module only loads
+it.  There is no fallback: if the shared library is missing or a call fails,
+an exception is raised.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "libecogen_amd.so"
+
+MODEL = {"euler": 0, "kapila": 1, "multip": 2}
+LIMITER = {"none": 0, "minmod": 1, "vanleer": 2, "vanalbada": 3, "superbee": 4, "mc": 5, "thinc": 6}
+EOS_TYPE = {"IG": 0, "SG": 1}
+PRECISION = {"fp64": 0, "fp32s": 1}
+
+WARN_RELAX_NOT_CONVERGED = 1
+WARN_NAN = 2
+
+EXPORTS = ["eco_create", "eco_set_state", "eco_set_state_device", "eco_get_state", "eco_step", "eco_run",
+           "eco_launch_steps", "eco_sync", "eco_state_device", "eco_destroy", "eco_strerror", "eco_version"]
+
+
+class EcoEos(C.Structure):
+    _fields_ = [("type", C.c_int), ("gamma", C.c_double), ("pinf", C.c_double), ("cv", C.c_double),
+                ("eref", C.c_double)]
+
+
+class EcoConfig(C.Structure):
+    _fields_ = [("model", C.c_int), ("nphase", C.c_int), ("order", C.c_int), ("lim", C.c_int * 4),
+                ("alpha_null", C.c_int), ("relax_p", C.c_int), ("precision", C.c_int), ("n", C.c_int * 3),
+                ("L", C.c_double * 3), ("cfl", C.c_double), ("bc", C.c_int * 6), ("eos", EcoEos * 4)]
+
+
+class EcoStatus(C.Structure):
+    _fields_ = [("steps", C.c_int), ("t", C.c_double), ("dt_last", C.c_double), ("dt_next", C.c_double),
+                ("flags", C.c_uint32), ("done", C.c_int)]
+
+
+class EcoError(RuntimeError):
+    def __init__(self, code: int, where: str):
+        super().__init__(f"{where}: {strerror(code)} ({code})")
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    """Load the device library (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise FileNotFoundError(f"{LIB_PATH} not built; run `python -m ecogen_amd.build`")
+        L = C.CDLL(str(LIB_PATH))
+        vp = C.c_void_p
+        dp = C.POINTER(C.c_double)
+        L.eco_create.argtypes = [C.POINTER(EcoConfig), vp, C.POINTER(vp)]
+        L.eco_create.restype = C.c_int
+        L.eco_set_state.argtypes = [vp, dp]
+        L.eco_set_state.restype = C.c_int
+        L.eco_set_state_device.argtypes = [vp, vp]
+        L.eco_set_state_device.restype = C.c_int
+        L.eco_get_state.argtypes = [vp, dp]
+        L.eco_get_state.restype = C.c_int
+        L.eco_step.argtypes = [vp, C.POINTER(EcoStatus)]
+        L.eco_step.restype = C.c_int
+        L.eco_run.argtypes = [vp, C.c_int, C.c_double, C.POINTER(EcoStatus)]
+        L.eco_run.restype = C.c_int
+        L.eco_launch_steps.argtypes = [vp, C.c_int]
+        L.eco_launch_steps.restype = C.c_int
+        L.eco_sync.argtypes = [vp, C.POINTER(EcoStatus)]
+        L.eco_sync.restype = C.c_int
+        L.eco_state_device.argtypes = [vp, C.POINTER(C.c_int)]
+        L.eco_state_device.restype = vp
+        L.eco_destroy.argtypes = [vp]
+        L.eco_destroy.restype = None
+        L.eco_strerror.argtypes = [C.c_int]
+        L.eco_strerror.restype = C.c_char_p
+        L.eco_version.argtypes = []
+        L.eco_version.restype = C.c_char_p
+        _lib = L
+    return _lib
+
+
+def strerror(code: int) -> str:
+    try:
+        return lib().eco_strerror(code).decode()
+    except Exception:  # library not loadable
+        return f"error {code}"
+
+
+def _check(rc: int, where: str):
+    if rc != 0:
+        raise EcoError(rc, where)
+
+
+def make_config(case) -> EcoConfig:
+    """Lower a case description (dict, see tests/golden/cases.py) to eco_config."""
+    c = EcoConfig()
+    c.model = MODEL[case["model"]]
+    c.nphase = len(case["eos"])
+    c.order = case["order"]
+    for i, l in enumerate(case["limiters"]):
+        c.lim[i] = LIMITER[l]
+    c.alpha_null = int(bool(case.get("alpha_null", False)))
+    c.relax_p = int(bool(case.get("relax_p", False)))
+    c.precision = PRECISION[case.get("precision", "fp64")]
+    for d in range(3):
+        c.n[d] = case["cells"][d]
+        c.L[d] = case["lengths"][d]
+    c.cfl = case["cfl"]
+    for b in range(6):
+        c.bc[b] = 0
+    for k, e in enumerate(case["eos"]):
+        c.eos[k] = EcoEos(EOS_TYPE[e["type"]], e["gamma"], e.get("pinf", 0.0), e["cv"], e.get("eref", 0.0))
+    return c
+
+
+def nfields(case) -> int:
+    return 3 * len(case["eos"]) + 4
+
+
+class Context:
+    """One device solve behind the C ABI."""
+
+    def __init__(self, case, stream=None):
+        self.case = case
+        self.cfg = make_config(case)
+        self.ncell = int(np.prod(case["cells"]))
+        self.nf = nfields(case)
+        h = C.c_void_p()
+        _check(lib().eco_create(C.byref(self.cfg), stream, C.byref(h)), "eco_create")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().eco_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_state(self, state):
+        s = np.zeros((self.nf, self.ncell))
+        st = np.asarray(state, dtype=np.float64)
+        s[: st.shape[0]] = st.reshape(st.shape[0], self.ncell)
+        s = np.ascontiguousarray(s)
+        _check(lib().eco_set_state(self._h, s.ctypes.data_as(C.POINTER(C.c_double))), "eco_set_state")
+
+    def set_state_device(self, ptr: int):
+        _check(lib().eco_set_state_device(self._h, C.c_void_p(ptr)), "eco_set_state_device")
+
+    def get_state(self):
+        s = np.zeros((self.nf, self.ncell))
+        _check(lib().eco_get_state(self._h, s.ctypes.data_as(C.POINTER(C.c_double))), "eco_get_state")
+        return s
+
+    def step(self) -> EcoStatus:
+        st = EcoStatus()
+        _check(lib().eco_step(self._h, C.byref(st)), "eco_step")
+        return st
+
+    def run(self, nsteps: int = 0, t_final: float = 0.0) -> EcoStatus:
+        st = EcoStatus()
+        _check(lib().eco_run(self._h, int(nsteps), float(t_final), C.byref(st)), "eco_run")
+        return st
+
+    def launch(self, nsteps: int):
+        _check(lib().eco_launch_steps(self._h, int(nsteps)), "eco_launch_steps")
+
+    def sync(self) -> EcoStatus:
+        st = EcoStatus()
+        _check(lib().eco_sync(self._h, C.byref(st)), "eco_sync")
+        return st
+
+    def state_device(self):
+        nf = C.c_int()
+        p = lib().eco_state_device(self._h, C.byref(nf))
+        return p, nf.value

@@ -1,0 +1,416 @@
+// C ABI of the explicit finite-volume step (include/ecogen.h).
+//
+// A context owns the device state of one solve: two time-level buffers
+// (ping-pong, the state after k steps lives in buffer k&1), the MUSCL
+// intermediate state U*, the per-direction geometry tables and the control
+// block that carries the lagged dt, the physical time and the done flags on
+// the device, so that steps are enqueued without host round trips.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "../../include/ecogen.h"
+#include "eco_launch.h"
+
+using namespace eco;
+
+struct eco_ctx {
+  eco_config cfg;
+  Geometry geo;
+  SchemeParams sch;
+  hipStream_t stream;
+  int own_stream;
+  int nprim;            // device primitive fields
+  double *buf[2];       // time levels, nprim+1 fields each
+  double *bstar;        // U*, nprim fields
+  double *tables;       // geometry tables
+  Control *ctl;
+  int step;             // next step index to launch
+  int have_state;
+};
+
+static int hipcode(hipError_t e) { return e == hipSuccess ? ECO_OK : ECO_ERR_HIP; }
+
+extern "C" const char *eco_strerror(int code) {
+  switch (code) {
+    case ECO_OK: return "ok";
+    case ECO_ERR_INVALID: return "invalid argument or configuration";
+    case ECO_ERR_UNSUPPORTED: return "configuration not supported by the device path";
+    case ECO_ERR_HIP: return "HIP runtime error";
+    case ECO_ERR_NOMEM: return "device allocation failed";
+    case ECO_ERR_STATE: return "call out of order";
+  }
+  return "unknown error";
+}
+
+extern "C" const char *eco_version(void) { return "ecogen-amd 0.1 (gfx950)"; }
+
+static void lower_eos(const eco_eos &e, EosParams &q) {
+  q.type = e.type;
+  q.gamma = e.gamma;
+  q.pinf = (e.type == ECO_EOS_SG) ? e.pinf : 0.;
+  q.cv = e.cv;
+  q.eref = e.eref;
+  // sendSpecialMixtureEos (src/Eos/EosSG.cpp:217-222, EosIG.cpp:213-218)
+  q.gpg = (e.type == ECO_EOS_SG) ? e.gamma * e.pinf / (e.gamma - 1.) : 0.;
+  q.og = 1. / (e.gamma - 1.);
+  q.gm1 = e.gamma - 1.;
+  q.gpinf = (e.type == ECO_EOS_SG) ? e.gamma * e.pinf : 0.;
+  q.pfloor = (e.type == ECO_EOS_SG) ? -(1. - 1.e-15) * e.pinf + 1.e-15 : 1.e-15;
+}
+
+static int validate(const eco_config *c) {
+  if (!c) return ECO_ERR_INVALID;
+  if (c->model < 0 || c->model > 2) return ECO_ERR_INVALID;
+  if (c->order != 1 && c->order != 2) return ECO_ERR_INVALID;
+  if (c->cfl <= 0.) return ECO_ERR_INVALID;
+  for (int d = 0; d < 3; d++) if (c->n[d] < 1 || !(c->L[d] > 0.)) return ECO_ERR_INVALID;
+  for (int i = 0; i < 4; i++) if (c->lim[i] < 0 || c->lim[i] > 6) return ECO_ERR_INVALID;
+  if (c->order == 2) {
+    // Input.cpp:187-249: THINC only as a volume-fraction limiter; no "none" at order 2
+    if (c->lim[0] == ECO_LIM_THINC || c->lim[1] == ECO_LIM_THINC) return ECO_ERR_INVALID;
+    for (int i = 0; i < 4; i++) if (c->lim[i] == ECO_LIM_NONE) return ECO_ERR_INVALID;
+  }
+  for (int i = 0; i < 6; i++) if (c->bc[i] != ECO_BC_ABS) return ECO_ERR_UNSUPPORTED;
+  if (c->model == ECO_MODEL_EULER) {
+    if (c->nphase != 1) return ECO_ERR_INVALID;
+  } else {
+    if (c->nphase < 2 || c->nphase > 4) return ECO_ERR_INVALID;
+    if (c->nphase != kMaxPhases) return ECO_ERR_UNSUPPORTED;
+  }
+  for (int k = 0; k < c->nphase; k++) {
+    if (c->eos[k].type != ECO_EOS_IG && c->eos[k].type != ECO_EOS_SG) return ECO_ERR_INVALID;
+    if (!(c->eos[k].gamma > 1.)) return ECO_ERR_INVALID;
+  }
+  // non-degenerate directions must come first (x, then y, then z)
+  int dim = (c->n[0] > 1) + (c->n[1] > 1) + (c->n[2] > 1);
+  for (int d = 0; d < dim; d++) if (c->n[d] < 2) return ECO_ERR_UNSUPPORTED;
+  if (dim == 0) return ECO_ERR_UNSUPPORTED;
+  // MultiP first order with relaxation reads per-cell cached phase energies and
+  // sound speeds from before the relaxation; the device state does not keep them.
+  if (c->model == ECO_MODEL_MULTIP && c->order == 1 && c->relax_p) return ECO_ERR_UNSUPPORTED;
+  if (c->precision != ECO_FP64) return ECO_ERR_UNSUPPORTED;
+  return ECO_OK;
+}
+
+extern "C" int eco_create(const eco_config *cfg, void *hip_stream, eco_ctx **out) {
+  if (!out) return ECO_ERR_INVALID;
+  *out = nullptr;
+  int rc = validate(cfg);
+  if (rc) return rc;
+  eco_ctx *x = new (std::nothrow) eco_ctx();
+  if (!x) return ECO_ERR_NOMEM;
+  x->cfg = *cfg;
+  const eco_config &c = *cfg;
+  // geometry (src/Meshes/MeshCartesian.cpp:180-262, 300-397)
+  Geometry &G = x->geo;
+  int dim = 0;
+  for (int d = 0; d < 3; d++) { G.n[d] = c.n[d]; dim += (c.n[d] > 1); }
+  G.dim = dim;
+  G.ncell = (long long)c.n[0] * c.n[1] * c.n[2];
+  double dX = c.L[0] / (double)c.n[0], dY = c.L[1] / (double)c.n[1], dZ = c.L[2] / (double)c.n[2];
+  double dd[3] = {dX, dY, dZ};
+  double vol = dX * dY * dZ;
+  double surf[3] = {dY * dZ, dX * dZ, dX * dY};
+  for (int d = 0; d < 3; d++) G.coefA[d] = surf[d] / vol;
+  double lc = 1.e10;
+  for (int d = 0; d < 3; d++) if (c.n[d] != 1) lc = std::min(lc, dd[d]);
+  if (dim > 1) lc *= 0.6;
+  G.lcfl = lc;
+  // per-direction distance tables (cell-centre and face positions as the reference builds them)
+  size_t ntab = 0;
+  for (int d = 0; d < 3; d++) ntab += 3 * (size_t)std::max(c.n[d], 1);
+  std::vector<double> tab(ntab, 0.);
+  size_t off[3];
+  size_t o = 0;
+  for (int d = 0; d < 3; d++) {
+    off[d] = o;
+    int n = c.n[d];
+    for (int i = 0; i + 1 < n; i++) {
+      double xi = (i + 0.5) * dd[d], xj = (i + 1 + 0.5) * dd[d];
+      double xf = xi + 0.5 * dd[d];
+      double v = xi - xj;
+      tab[o + i] = std::sqrt(v * v);                   // |x_i - x_{i+1}| (Element::distance)
+      double a = xf - xi, b = xf - xj;
+      tab[o + n + i] = std::sqrt(a * a);               // face - left cell (Face::distance)
+      tab[o + 2 * n + i] = std::sqrt(b * b);           // face - right cell
+    }
+    o += 3 * (size_t)n;
+  }
+  // scheme
+  SchemeParams &S = x->sch;
+  std::memset(&S, 0, sizeof(S));
+  S.model = c.model;
+  S.nph = c.nphase;
+  S.order = c.order;
+  for (int i = 0; i < 4; i++) S.lim[i] = c.lim[i];
+  S.same_lim = (c.lim[0] == c.lim[1] && c.lim[2] == c.lim[3]) ? 1 : 0;
+  S.thinc = (c.order == 2 && (c.lim[2] == ECO_LIM_THINC || c.lim[3] == ECO_LIM_THINC)) ? 1 : 0;
+  S.alpha_null = c.alpha_null;
+  S.eps = (c.model == ECO_MODEL_KAPILA && c.alpha_null) ? 1.e-15 : 0.;
+  S.relax = (c.model == ECO_MODEL_KAPILA) || (c.model == ECO_MODEL_MULTIP && c.relax_p);
+  S.cfl = c.cfl;
+  for (int k = 0; k < c.nphase && k < kMaxPhases; k++) lower_eos(c.eos[k], S.eos[k]);
+  x->nprim = (c.model == ECO_MODEL_EULER) ? 2 + dim : 3 * c.nphase + dim;
+
+  // stream
+  if (hip_stream) { x->stream = (hipStream_t)hip_stream; x->own_stream = 0; }
+  else { x->stream = nullptr; x->own_stream = 0; }
+
+  // device memory
+  size_t nc = (size_t)G.ncell;
+  hipError_t e = hipSuccess;
+  x->buf[0] = x->buf[1] = x->bstar = x->tables = nullptr;
+  x->ctl = nullptr;
+  e = hipMalloc(&x->buf[0], nc * (x->nprim + 1) * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&x->buf[1], nc * (x->nprim + 1) * sizeof(double));
+  if (e == hipSuccess && c.order == 2) e = hipMalloc(&x->bstar, nc * x->nprim * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&x->tables, ntab * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&x->ctl, sizeof(Control));
+  if (e != hipSuccess) { eco_destroy(x); return ECO_ERR_NOMEM; }
+  e = hipMemcpy(x->tables, tab.data(), ntab * sizeof(double), hipMemcpyHostToDevice);
+  if (e != hipSuccess) { eco_destroy(x); return ECO_ERR_HIP; }
+  for (int d = 0; d < 3; d++) {
+    int n = c.n[d];
+    G.dcc[d] = x->tables + off[d];
+    G.dfl[d] = x->tables + off[d] + n;
+    G.dfr[d] = x->tables + off[d] + 2 * n;
+  }
+  x->step = 0;
+  x->have_state = 0;
+  *out = x;
+  return ECO_OK;
+}
+
+static int reset_control(eco_ctx *x) {
+  Control h;
+  std::memset(&h, 0, sizeof(h));
+  h.t_final = -1.;
+  h.max_steps = 0;
+  return hipcode(hipMemcpyAsync(x->ctl, &h, sizeof(h), hipMemcpyHostToDevice, x->stream));
+}
+
+static StageArgs make_args(eco_ctx *x, int step) {
+  StageArgs A;
+  A.geo = x->geo;
+  A.sch = x->sch;
+  A.ctl = x->ctl;
+  A.step = step;
+  A.wn = x->buf[step & 1];
+  A.wsrc = A.wn;
+  A.wout = x->buf[(step + 1) & 1];
+  return A;
+}
+
+// host layout (3N+4 fields, u,v,w) -> device layout (nprim fields)
+static void host_to_dev_fields(const eco_ctx *x, std::vector<int> &map) {
+  const eco_config &c = x->cfg;
+  map.clear();
+  if (c.model == ECO_MODEL_EULER) {
+    map.push_back(1); map.push_back(2);
+    for (int d = 0; d < x->geo.dim; d++) map.push_back(3 + d);
+  } else {
+    for (int k = 0; k < c.nphase; k++) { map.push_back(3 * k); map.push_back(3 * k + 1); map.push_back(3 * k + 2); }
+    for (int d = 0; d < x->geo.dim; d++) map.push_back(3 * c.nphase + d);
+  }
+}
+
+static int finish_set_state(eco_ctx *x) {
+  int rc = reset_control(x);
+  if (rc) return rc;
+  StageArgs A = make_args(x, 0);
+  A.wout = x->buf[0];
+  rc = launch_stage(A, 3, x->stream);
+  if (rc) return rc < 0 ? ECO_ERR_UNSUPPORTED : ECO_ERR_HIP;
+  x->step = 0;
+  x->have_state = 1;
+  return hipcode(hipStreamSynchronize(x->stream));
+}
+
+extern "C" int eco_set_state(eco_ctx *x, const double *host_state) {
+  if (!x || !host_state) return ECO_ERR_INVALID;
+  std::vector<int> map;
+  host_to_dev_fields(x, map);
+  size_t nc = (size_t)x->geo.ncell;
+  for (size_t f = 0; f < map.size(); f++) {
+    hipError_t e = hipMemcpyAsync(x->buf[0] + f * nc, host_state + (size_t)map[f] * nc, nc * sizeof(double),
+                                  hipMemcpyHostToDevice, x->stream);
+    if (e != hipSuccess) return ECO_ERR_HIP;
+  }
+  return finish_set_state(x);
+}
+
+extern "C" int eco_set_state_device(eco_ctx *x, const double *dev_state) {
+  if (!x || !dev_state) return ECO_ERR_INVALID;
+  std::vector<int> map;
+  host_to_dev_fields(x, map);
+  size_t nc = (size_t)x->geo.ncell;
+  for (size_t f = 0; f < map.size(); f++) {
+    hipError_t e = hipMemcpyAsync(x->buf[0] + f * nc, dev_state + (size_t)map[f] * nc, nc * sizeof(double),
+                                  hipMemcpyDeviceToDevice, x->stream);
+    if (e != hipSuccess) return ECO_ERR_HIP;
+  }
+  return finish_set_state(x);
+}
+
+static int read_status(eco_ctx *x, eco_status *st, Control *hc) {
+  Control h;
+  hipError_t e = hipMemcpyAsync(&h, x->ctl, sizeof(Control), hipMemcpyDeviceToHost, x->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(x->stream);
+  if (e != hipSuccess) return ECO_ERR_HIP;
+  if (hc) *hc = h;
+  if (st) {
+    st->steps = h.steps;
+    st->t = (h.steps > 0) ? h.t[(h.steps - 1) & 1] : 0.;
+    st->dt_last = h.dt_last;
+    double dtn = 1.e-15;
+    if (h.steps > 0) {
+      unsigned long long b = h.smax[(h.steps - 1) % 3];
+      double sm;
+      std::memcpy(&sm, &b, sizeof(sm));
+      double dtMax = (sm > 0.) ? std::min(1.e10, x->geo.lcfl / sm) : 1.e10;
+      dtn = x->cfg.cfl * dtMax;
+    }
+    st->dt_next = dtn;
+    st->flags = h.err;
+    st->done = (h.steps > 0) ? h.done[(h.steps - 1) & 1] : 0;
+  }
+  return ECO_OK;
+}
+
+extern "C" int eco_get_state(eco_ctx *x, double *host_state) {
+  if (!x || !host_state) return ECO_ERR_INVALID;
+  if (!x->have_state) return ECO_ERR_STATE;
+  Control h;
+  int rc = read_status(x, nullptr, &h);
+  if (rc) return rc;
+  x->step = h.steps;
+  const eco_config &c = x->cfg;
+  size_t nc = (size_t)x->geo.ncell;
+  const double *src = x->buf[h.steps & 1];
+  std::vector<int> map;
+  host_to_dev_fields(x, map);
+  int nf = ECO_NFIELDS(c.nphase);
+  std::memset(host_state, 0, sizeof(double) * nf * nc);
+  hipError_t e = hipSuccess;
+  for (size_t f = 0; f < map.size() && e == hipSuccess; f++)
+    e = hipMemcpyAsync(host_state + (size_t)map[f] * nc, src + f * nc, nc * sizeof(double), hipMemcpyDeviceToHost, x->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(host_state + (size_t)(nf - 1) * nc, src + (size_t)x->nprim * nc, nc * sizeof(double),
+                       hipMemcpyDeviceToHost, x->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(x->stream);
+  if (e != hipSuccess) return ECO_ERR_HIP;
+  if (c.model == ECO_MODEL_EULER) for (size_t i = 0; i < nc; i++) host_state[i] = 1.;
+  return ECO_OK;
+}
+
+static int launch_one(eco_ctx *x) {
+  int k = x->step;
+  int rc;
+  if (x->cfg.order == 1) {
+    rc = launch_stage(make_args(x, k), 0, x->stream);
+  } else {
+    StageArgs P = make_args(x, k);
+    P.wout = x->bstar;
+    rc = launch_stage(P, 1, x->stream);
+    if (!rc) {
+      StageArgs Q = make_args(x, k);
+      Q.wsrc = x->bstar;
+      rc = launch_stage(Q, 2, x->stream);
+    }
+  }
+  if (rc) return rc < 0 ? ECO_ERR_UNSUPPORTED : ECO_ERR_HIP;
+  x->step = k + 1;
+  return ECO_OK;
+}
+
+static int set_target(eco_ctx *x, int max_steps, double t_final) {
+  // Control::t_final / max_steps / done[] are plain fields written before the launches.
+  struct { double t_final; int max_steps; } tgt = {t_final, max_steps};
+  hipError_t e = hipMemcpyAsync(&x->ctl->t_final, &tgt.t_final, sizeof(double), hipMemcpyHostToDevice, x->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(&x->ctl->max_steps, &tgt.max_steps, sizeof(int), hipMemcpyHostToDevice, x->stream);
+  int zero[2] = {0, 0};
+  if (e == hipSuccess) e = hipMemcpyAsync(&x->ctl->done[0], zero, sizeof(zero), hipMemcpyHostToDevice, x->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(x->stream);
+  return hipcode(e);
+}
+
+extern "C" int eco_launch_steps(eco_ctx *x, int nsteps) {
+  if (!x || nsteps < 0) return ECO_ERR_INVALID;
+  if (!x->have_state) return ECO_ERR_STATE;
+  for (int i = 0; i < nsteps; i++) {
+    int rc = launch_one(x);
+    if (rc) return rc;
+  }
+  return ECO_OK;
+}
+
+extern "C" int eco_sync(eco_ctx *x, eco_status *st) {
+  if (!x) return ECO_ERR_INVALID;
+  Control h;
+  int rc = read_status(x, st, &h);
+  if (rc) return rc;
+  x->step = h.steps;
+  return ECO_OK;
+}
+
+extern "C" int eco_step(eco_ctx *x, eco_status *st) {
+  if (!x) return ECO_ERR_INVALID;
+  if (!x->have_state) return ECO_ERR_STATE;
+  int rc = set_target(x, x->step + 1, -1.);
+  if (!rc) rc = launch_one(x);
+  if (!rc) rc = eco_sync(x, st);
+  return rc;
+}
+
+extern "C" int eco_run(eco_ctx *x, int nsteps, double t_final, eco_status *st) {
+  if (!x) return ECO_ERR_INVALID;
+  if (!x->have_state) return ECO_ERR_STATE;
+  int rc;
+  if (nsteps > 0) {
+    rc = set_target(x, x->step + nsteps, -1.);
+    if (rc) return rc;
+    for (int i = 0; i < nsteps; i++) {
+      rc = launch_one(x);
+      if (rc) return rc;
+    }
+    return eco_sync(x, st);
+  }
+  rc = set_target(x, 0, t_final);
+  if (rc) return rc;
+  // Time control: enqueue chunks, check the device done flag between chunks.
+  int chunk = 16;
+  eco_status s;
+  for (;;) {
+    for (int i = 0; i < chunk; i++) {
+      rc = launch_one(x);
+      if (rc) return rc;
+    }
+    rc = eco_sync(x, &s);
+    if (rc) return rc;
+    if (s.done) break;
+    chunk = std::min(chunk * 2, 512);
+  }
+  if (st) *st = s;
+  return ECO_OK;
+}
+
+extern "C" const double *eco_state_device(eco_ctx *x, int *nfields_device) {
+  if (!x) return nullptr;
+  if (nfields_device) *nfields_device = x->nprim + 1;
+  return x->buf[x->step & 1];
+}
+
+extern "C" void eco_destroy(eco_ctx *x) {
+  if (!x) return;
+  if (x->buf[0]) (void)hipFree(x->buf[0]);
+  if (x->buf[1]) (void)hipFree(x->buf[1]);
+  if (x->bstar) (void)hipFree(x->bstar);
+  if (x->tables) (void)hipFree(x->tables);
+  if (x->ctl) (void)hipFree(x->ctl);
+  delete x;
+}

@@ -1,0 +1,72 @@
+// Plain-old-data parameter blocks shared by the host descriptors and the
+// gfx950 kernels of the explicit finite-volume step.
+#pragma once
+#include <stdint.h>
+
+namespace eco {
+
+enum ModelKind : int { MODEL_EULER = 0, MODEL_KAPILA = 1, MODEL_MULTIP = 2 };
+enum LimiterKind : int { LIM_NONE = 0, LIM_MINMOD = 1, LIM_VANLEER = 2, LIM_VANALBADA = 3,
+                         LIM_SUPERBEE = 4, LIM_MC = 5, LIM_THINC = 6 };
+enum EosKind : int { EOS_IG = 0, EOS_SG = 1 };
+enum Precision : int { PREC_FP64 = 0, PREC_FP32S = 1 };
+
+constexpr int kMaxPhases = 2;  // phases handled by the device path
+
+// Equation of state constants (src/Eos/EosSG.h, src/Eos/EosIG.h) plus the
+// mixture-EOS coefficients of sendSpecialMixtureEos, computed once on the host
+// with the same expressions the reference evaluates per call.
+struct EosParams {
+  int type;
+  double gamma, pinf, cv, eref;
+  double gpg;  // gamma*pinf/(gamma-1)   (0 for IG)
+  double og;   // 1/(gamma-1)
+  double gm1;  // gamma-1
+  double gpinf;  // gamma*pinf            (0 for IG)
+  double pfloor;  // -(1-1e-15)*pinf + 1e-15 (SG) or 1e-15 (IG)
+};
+
+// Uniform Cartesian grid geometry.  Per-direction 1-D tables reproduce the
+// reference's cell-centre and face positions (src/Meshes/MeshCartesian.cpp:
+// 180-197, 300-397): dcc[i] = |x_i - x_{i+1}|, dfl[i] = |xf_{i+1/2} - x_i|,
+// dfr[i] = |xf_{i+1/2} - x_{i+1}|.
+struct Geometry {
+  int n[3];
+  int dim;
+  long long ncell;
+  double coefA[3];  // face surface / cell volume per direction
+  double lcfl;      // CFL length (x0.6 in multi-D)
+  const double *dcc[3];
+  const double *dfl[3];
+  const double *dfr[3];
+};
+
+struct SchemeParams {
+  int model, nph, order;
+  int lim[4];        // global, interface, global VF, interface VF
+  int same_lim;      // interface detection cannot change the limiter
+  int thinc;
+  int alpha_null;
+  double eps;        // 1e-15 when alphaNull, else 0 (src/Eos/Eos.cpp:114-122)
+  int relax;         // stiff pressure relaxation applied
+  double cfl;
+  EosParams eos[kMaxPhases];
+};
+
+// Device-resident control block of one solve (one per context / batch case).
+// dt handling reproduces Run::solver (src/Run.cpp:204-277): dt of step 0 is
+// 1e-15, dt of step k is cfl * dtMax(k-1); dtMax is min over all Riemann
+// problems of lcfl/|s|, kept here as max |s| (division is monotonic).
+struct Control {
+  unsigned long long smax[3];  // max wave speed bits, slot k%3 for step k
+  double t[2];                 // physical time after step k in t[k&1]
+  int done[2];                 // run finished after step k in done[k&1]
+  int steps;                   // steps taken
+  unsigned int err;            // soft-error flags (ECO_WARN_*)
+  double dt_last;              // dt used by the last step taken
+  double t_final;              // <0: iteration mode
+  int max_steps;               // iteration-mode step count
+  int pad;
+};
+
+}  // namespace eco

@@ -1,0 +1,491 @@
+// Device physics of the explicit finite-volume step (FP64), gfx950.
+//
+// Every function restates one reference routine (file:line relative to the
+// ECOGEN source tree) with the same evaluation order, clamp constants and
+// std::min/std::max semantics, so that the only arithmetic differences from
+// the reference are those of the device pow/exp (not correctly rounded).
+// Derived quantities the reference caches per cell (phase energies, sound
+// speeds, mixture density/pressure/energy, frozen sound speed) are recomputed
+// from the stored primitive state (alpha_k, rho_k, p_k, u, E) when needed.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "eco_params.h"
+
+namespace eco {
+
+#define ECO_D __device__ __forceinline__
+
+ECO_D double smin(double a, double b) { return (b < a) ? b : a; }
+ECO_D double smax(double a, double b) { return (a < b) ? b : a; }
+
+// ----------------------------------------------------------------------------
+// EOS: src/Eos/EosSG.cpp:93-254, src/Eos/EosIG.cpp:90-251
+// ----------------------------------------------------------------------------
+ECO_D double eos_e(const EosParams &q, double rho, double p, double eps) {
+  if (q.type == EOS_SG) return (p + q.gpinf) / q.gm1 / smax(rho, eps) + q.eref;
+  return p / q.gm1 / smax(rho, eps) + q.eref;
+}
+ECO_D double eos_p(const EosParams &q, double rho, double e) {
+  if (q.type == EOS_SG) return q.gm1 * rho * (e - q.eref) - q.gpinf;
+  return q.gm1 * rho * (e - q.eref);
+}
+ECO_D double eos_c(const EosParams &q, double rho, double p, double eps) {
+  if (q.type == EOS_SG) return sqrt(q.gamma * (p + q.pinf) / smax(rho, eps));
+  return sqrt(q.gamma * p / smax(rho, eps));
+}
+ECO_D double eos_isentrope(const EosParams &q, double p0, double rho0, double rho1, double eps) {
+  if (q.type == EOS_SG) return (p0 + q.pinf) * pow(rho1 / smax(rho0, eps), q.gamma) - q.pinf;
+  return p0 * pow(rho1 / smax(rho0, eps), q.gamma);
+}
+ECO_D double eos_rho_pfinal(const EosParams &q, double p0, double rho0, double pf, double &drho, double eps) {
+  if (q.type == EOS_SG) {
+    double num = q.gamma * (pf + q.pinf);
+    double denom = num + p0 - pf;
+    double rf = rho0 * num / smax(denom, eps);
+    drho = rho0 * q.gamma * (p0 + q.pinf) / smax(denom * denom, eps);
+    return rf;
+  }
+  double num = q.gamma * pf;
+  double denom = num + p0 - pf;
+  double rf = rho0 * num / smax(denom, eps);
+  drho = rho0 * q.gamma * p0 / smax(denom * denom, eps);
+  return rf;
+}
+// verifyAndModifyPressure (EosSG.cpp:243-247, EosIG.cpp:241-244)
+ECO_D double eos_fix_p(const EosParams &q, double p) {
+  if (q.type == EOS_SG) return (p <= q.pfloor) ? q.pfloor : p;
+  return (p < 1.e-15) ? 1.e-15 : p;
+}
+
+// ----------------------------------------------------------------------------
+// Limiters: src/Ordre2/Limiter*.cpp:49-59
+// ----------------------------------------------------------------------------
+ECO_D double limiter(int lim, double s1, double s2) {
+  double slope = 0., prod = s1 * s2;
+  switch (lim) {
+    case LIM_MINMOD:
+    case LIM_THINC:
+      if (prod > 1e-9) { slope = smin(fabs(s1), fabs(s2)); if (s1 < 0) slope = -slope; }
+      return slope;
+    case LIM_VANLEER: {
+      double sum = s1 + s2;
+      if ((fabs(sum) > 1e-6) && (prod > 1e-6)) slope = 2. * prod / sum;
+      return slope;
+    }
+    case LIM_VANALBADA: {
+      double sum = s1 + s2;
+      if ((fabs(s1) > 1e-6) && (fabs(s2) > 1e-6) && (fabs(sum) > 1e-6) && (prod > 1e-6))
+        slope = prod * sum / (s1 * s1 + s2 * s2);
+      return slope;
+    }
+    case LIM_MC:
+      if (prod > 1e-9) {
+        slope = smin(2. * fabs(s1), 2. * fabs(s2));
+        slope = smin(slope, 0.5 * (fabs(s1) + fabs(s2)));
+      }
+      if (s1 < 0.) slope = -slope;
+      return slope;
+    case LIM_SUPERBEE:
+      if (prod > 1e-6) slope = smax(smin(2. * fabs(s1), fabs(s2)), smin(fabs(s1), 2. * fabs(s2)));
+      if (s1 < 0.) slope = -slope;
+      return slope;
+    default:
+      return 0.;
+  }
+}
+
+// ----------------------------------------------------------------------------
+// Cell state in registers
+// ----------------------------------------------------------------------------
+// Multiphase models: per phase alpha, rho, p (+ derived e, c, Y); mixture
+// velocity u[3] and total energy E, mixture rho, p, e, frozen sound speed cF.
+// Euler: phase 0 carries rho, p, e, c; u[3]; E is the phase total energy.
+template <int NPH>
+struct CellState {
+  double a[NPH], r[NPH], p[NPH];
+  double e[NPH], c[NPH], Y[NPH];
+  double u[3];
+  double E;
+  double rho, pm, em, cF;
+};
+
+template <int NPH>
+struct FluxVec {
+  double a[NPH], m[NPH], en[NPH];
+  double q[3];
+  double E;
+  double sM;
+};
+
+ECO_D double sqn(const double *u) { return u[0] * u[0] + u[1] * u[1] + u[2] * u[2]; }
+
+// PhaseKapila::verifyAndCorrectPhase (PhaseKapila.cpp:294-307),
+// PhaseMultiP::verifyAndCorrectPhase (PhaseMultiP.cpp:288-294),
+// PhaseEuler::verifyAndCorrectPhase (PhaseEuler.cpp:349-353)
+template <int MODEL, int NPH>
+ECO_D void verify_correct(const SchemeParams &S, CellState<NPH> &s, int k) {
+  if constexpr (MODEL == MODEL_KAPILA) {
+    if (S.eps > 1.e-20) {
+      if (s.a[k] < 0.) s.a[k] = 0.;
+      if (s.a[k] > 1.) s.a[k] = 1.;
+      if (s.r[k] <= 1.e-15) s.r[k] = 1.e-15;
+    } else {
+      if (s.a[k] < 1e-15) s.a[k] = 1e-14;
+      if (s.a[k] > 1. - 1e-15) s.a[k] = 1. - 1e-14;
+      if (s.r[k] < 1.e-15) s.r[k] = 1.e-14;
+    }
+  } else if constexpr (MODEL == MODEL_MULTIP) {
+    if (s.a[k] < 1e-10) s.a[k] = 1e-9;
+    if (s.a[k] > 1. - 1e-10) s.a[k] = 1. - 1e-9;
+    if (s.r[k] < 1.e-10) s.r[k] = 1.e-9;
+  } else {
+    if (s.r[k] < 1.e-10) s.r[k] = 1.e-10;
+  }
+  s.p[k] = eos_fix_p(S.eos[k], s.p[k]);
+}
+
+// Model::fulfillState: extendedCalculusPhase (energy, sound speed) and
+// Mix*::computeMixtureVariables (MixKapila.cpp:147-172, MixMultiP.cpp:142-168).
+// Temperatures and Wood's sound speed are not used by the step and skipped.
+template <int MODEL, int NPH>
+ECO_D void fulfill(const SchemeParams &S, CellState<NPH> &s) {
+  const double eps = S.eps;
+  if constexpr (MODEL == MODEL_EULER) {
+    s.e[0] = eos_e(S.eos[0], s.r[0], s.p[0], eps);
+    s.c[0] = eos_c(S.eos[0], s.r[0], s.p[0], eps);
+    s.E = s.e[0] + 0.5 * sqn(s.u);
+  } else {
+#pragma unroll
+    for (int k = 0; k < NPH; k++) {
+      s.e[k] = eos_e(S.eos[k], s.r[k], s.p[k], eps);
+      s.c[k] = eos_c(S.eos[k], s.r[k], s.p[k], eps);
+    }
+    s.rho = 0.; s.pm = 0.;
+#pragma unroll
+    for (int k = 0; k < NPH; k++) { s.rho += s.a[k] * s.r[k]; s.pm += s.a[k] * s.p[k]; }
+#pragma unroll
+    for (int k = 0; k < NPH; k++)
+      s.Y[k] = (MODEL == MODEL_KAPILA) ? s.a[k] * s.r[k] / smax(s.rho, eps) : s.a[k] * s.r[k] / s.rho;
+    s.em = 0.; s.cF = 0.;
+#pragma unroll
+    for (int k = 0; k < NPH; k++) { s.em += s.Y[k] * s.e[k]; s.cF += s.Y[k] * s.c[k] * s.c[k]; }
+    s.cF = sqrt(s.cF);
+  }
+}
+
+// FluxKapila::buildCons (FluxKapila.cpp:123-135), FluxMultiP::buildCons
+// (FluxMultiP.cpp:122-140), FluxEuler::buildCons (FluxEuler.cpp:95-104).
+// Needs s.rho (mixture density) for the multiphase models.
+template <int MODEL, int NPH>
+ECO_D void build_cons(const SchemeParams &S, const CellState<NPH> &s, FluxVec<NPH> &f) {
+  if constexpr (MODEL == MODEL_EULER) {
+    f.m[0] = s.r[0];
+#pragma unroll
+    for (int d = 0; d < 3; d++) f.q[d] = s.u[d] * f.m[0];
+    f.E = f.m[0] * s.E;
+  } else {
+#pragma unroll
+    for (int k = 0; k < NPH; k++) {
+      f.a[k] = s.a[k];
+      f.m[k] = s.a[k] * s.r[k];
+      double ein = eos_e(S.eos[k], s.r[k], s.p[k], S.eps);
+      f.en[k] = s.a[k] * s.r[k] * ein;
+    }
+#pragma unroll
+    for (int d = 0; d < 3; d++) f.q[d] = s.u[d] * s.rho;
+    f.E = s.rho * s.E;
+  }
+}
+
+ECO_D void zero_small(double *u) {
+  if (fabs(u[0]) < 1.e-8) u[0] = 0.;
+  if (fabs(u[1]) < 1.e-8) u[1] = 0.;
+  if (fabs(u[2]) < 1.e-8) u[2] = 0.;
+}
+
+// Flux*::buildPrim: FluxKapila.cpp:139-185, FluxMultiP.cpp:144-183,
+// FluxEuler.cpp:107-131.  Leaves the derived quantities of fulfillState set.
+template <int MODEL, int NPH>
+ECO_D void build_prim(const SchemeParams &S, FluxVec<NPH> &f, CellState<NPH> &s) {
+  const double eps = S.eps;
+  if constexpr (MODEL == MODEL_EULER) {
+    s.r[0] = f.m[0];
+#pragma unroll
+    for (int d = 0; d < 3; d++) s.u[d] = f.q[d] / f.m[0];
+    zero_small(s.u);
+    s.E = f.E / f.m[0];
+    s.e[0] = s.E - 0.5 * sqn(s.u);
+    s.p[0] = eos_p(S.eos[0], f.m[0], s.e[0]);
+    s.c[0] = eos_c(S.eos[0], f.m[0], s.p[0], eps);
+  } else {
+    double un = 0.;
+    if constexpr (MODEL == MODEL_KAPILA) {
+      if (eps > 1.e-20) {
+#pragma unroll
+        for (int k = 0; k < NPH; k++) {
+          if (f.a[k] < 0.) f.a[k] = 0.;
+          if (f.a[k] > 1.) f.a[k] = 1.;
+          un += f.a[k];
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < NPH; k++) {
+          if (f.a[k] <= 1.e-15) f.a[k] = 1e-15;
+          if (f.a[k] >= 1. - 1.e-15) f.a[k] = 1.0 - 1e-15;
+          un += f.a[k];
+        }
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < NPH; k++) {
+        if (f.a[k] <= 1.e-10) f.a[k] = 1e-9;
+        if (f.a[k] >= 1. - 1.e-10) f.a[k] = 1.0 - 1e-9;
+        un += f.a[k];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < NPH; k++) f.a[k] /= un;
+    double rhoMel = 0.;
+#pragma unroll
+    for (int k = 0; k < NPH; k++) {
+      rhoMel = rhoMel + f.m[k];
+      s.a[k] = f.a[k];
+      double ein;
+      if constexpr (MODEL == MODEL_KAPILA) {
+        s.r[k] = f.m[k] / smax(f.a[k], eps);
+        ein = f.en[k] / smax(f.m[k], eps);
+      } else {
+        s.r[k] = f.m[k] / f.a[k];
+        ein = f.en[k] / f.m[k];
+      }
+      s.p[k] = eos_p(S.eos[k], s.r[k], ein);
+      verify_correct<MODEL, NPH>(S, s, k);
+    }
+#pragma unroll
+    for (int d = 0; d < 3; d++) s.u[d] = f.q[d] / rhoMel;
+    zero_small(s.u);
+    fulfill<MODEL, NPH>(S, s);
+    s.E = f.E / rhoMel;
+  }
+}
+
+// FluxMultiP::schemeCorrection (FluxMultiP.cpp:237-260); buf holds U^n.
+template <int MODEL, int NPH>
+ECO_D void scheme_correction(FluxVec<NPH> &f, const FluxVec<NPH> &buf) {
+  if constexpr (MODEL == MODEL_MULTIP) {
+    double rhoN = 0., rhoNp1 = 0., sum = 0., dek[NPH];
+#pragma unroll
+    for (int k = 0; k < NPH; k++) { rhoN += buf.m[k]; rhoNp1 += f.m[k]; dek[k] = f.en[k] - buf.en[k]; }
+#pragma unroll
+    for (int k = 0; k < NPH; k++) sum += dek[k];
+    double dE = f.E - buf.E;
+    double dRU2 = sqn(f.q) / rhoNp1 - sqn(buf.q) / rhoN;
+    if (fabs(sum) > 1.e-10 * f.E) {
+#pragma unroll
+      for (int k = 0; k < NPH; k++) f.en[k] = buf.en[k] + dek[k] / sum * (dE - 0.5 * dRU2);
+    }
+  }
+}
+
+// U <- U*mult + U^n (Flux::multiply then addFlux(1.), Cell.cpp:214-227).
+template <int MODEL, int NPH>
+ECO_D void scale_add(FluxVec<NPH> &U, double mult, const FluxVec<NPH> &buf) {
+  if constexpr (MODEL == MODEL_EULER) {
+    U.m[0] *= mult;
+  } else {
+#pragma unroll
+    for (int k = 0; k < NPH; k++) { U.a[k] *= mult; U.m[k] *= mult; U.en[k] *= mult; }
+  }
+#pragma unroll
+  for (int d = 0; d < 3; d++) U.q[d] *= mult;
+  U.E *= mult;
+  if constexpr (MODEL == MODEL_EULER) {
+    U.m[0] += 1. * buf.m[0];
+  } else {
+#pragma unroll
+    for (int k = 0; k < NPH; k++) { U.a[k] += 1. * buf.a[k]; U.m[k] += 1. * buf.m[k]; U.en[k] += 1. * buf.en[k]; }
+  }
+#pragma unroll
+  for (int d = 0; d < 3; d++) U.q[d] += buf.q[d] * 1.;
+  U.E += 1. * buf.E;
+}
+
+// RelaxationP::stiffRelaxation (RelaxationP.cpp:49-111).  Returns false when
+// the Newton iteration did not converge in 100 iterations (state untouched,
+// as in the reference).
+template <int MODEL, int NPH>
+ECO_D bool relax_p(const SchemeParams &S, CellState<NPH> &s) {
+  if (S.eps > 1.e-20) {
+#pragma unroll
+    for (int k = 0; k < NPH; k++) if (s.a[k] > (1. - 1.e-5)) return true;
+  }
+  double ak[NPH], pk[NPH], rk[NPH], rkS[NPH], akS[NPH];
+  double pStar = 0.;
+#pragma unroll
+  for (int k = 0; k < NPH; k++) {
+    verify_correct<MODEL, NPH>(S, s, k);
+    ak[k] = s.a[k]; pk[k] = s.p[k]; rk[k] = s.r[k];
+    pStar += ak[k] * pk[k];
+  }
+  int it = 0;
+  double f = 0., df = 1.;
+  do {
+    it++;
+    pStar -= f / df;
+#pragma unroll
+    for (int k = 0; k < NPH; k++) pStar = eos_fix_p(S.eos[k], pStar);
+    f = -1.; df = 0.;
+#pragma unroll
+    for (int k = 0; k < NPH; k++) {
+      double drho;
+      rkS[k] = eos_rho_pfinal(S.eos[k], pk[k], rk[k], pStar, drho, S.eps);
+      akS[k] = ak[k] * rk[k] / rkS[k];
+      double dalpha = ak[k] * rk[k] * drho / (rkS[k] * rkS[k]);
+      f += akS[k];
+      df -= dalpha;
+    }
+  } while (fabs(f) > 1e-10 && it < 100);
+  if (it < 100) {
+#pragma unroll
+    for (int k = 0; k < NPH; k++) { s.a[k] = akS[k]; s.r[k] = rkS[k]; s.p[k] = pStar; }
+    s.pm = pStar;
+    return true;
+  }
+  return false;
+}
+
+// FluxKapila::correctionEnergy (FluxKapila.cpp:239-268), after
+// Mixture::totalEnergyToInternalEnergy (MixKapila.cpp:188-195).  s.rho is the
+// mixture density cached by buildPrim (before relaxation).
+template <int MODEL, int NPH>
+ECO_D void correction_energy(const SchemeParams &S, CellState<NPH> &s) {
+  if constexpr (MODEL == MODEL_KAPILA) {
+    s.em = s.E - 0.5 * sqn(s.u);
+    double rhoe = s.rho * s.em;
+    double p = rhoe, denom = 0.;
+#pragma unroll
+    for (int k = 0; k < NPH; k++) {
+      p -= s.a[k] * (S.eos[k].gpg + s.r[k] * S.eos[k].eref);
+      denom += s.a[k] * S.eos[k].og;
+    }
+    p /= denom;
+#pragma unroll
+    for (int k = 0; k < NPH; k++) { s.p[k] = p; verify_correct<MODEL, NPH>(S, s, k); }
+  }
+}
+
+// ----------------------------------------------------------------------------
+// HLLC in the face frame
+// ----------------------------------------------------------------------------
+// ModKapila::solveRiemannIntern (ModKapila.cpp:107-218) == ModMultiP (ModMultiP.cpp:101-214).
+// em_* is the mixture internal energy the reference reads (ΣY_k e_k after
+// fulfillState, or E - |u|^2/2 for MultiP first-order interior faces).
+// Returns max(|sL|,|sR|) among speeds above the 1e-3 guard (0 if none).
+template <int MODEL, int NPH>
+ECO_D double hllc(const SchemeParams &S, const CellState<NPH> &L, const CellState<NPH> &R, FluxVec<NPH> &F) {
+  const double eps = S.eps;
+  if constexpr (MODEL == MODEL_EULER) {
+    // ModEuler::solveRiemannIntern (ModEuler.cpp:90-175)
+    double uL = L.u[0], vL = L.u[1], wL = L.u[2], pL = L.p[0], rhoL = L.r[0], cL = L.c[0], EL = L.E;
+    double uR = R.u[0], vR = R.u[1], wR = R.u[2], pR = R.p[0], rhoR = R.r[0], cR = R.c[0], ER = R.E;
+    double sL = smin(uL - cL, uR - cR);
+    double sR = smax(uR + cR, uL + cL);
+    double smx = 0.;
+    if (fabs(sL) > 1.e-3) smx = fabs(sL);
+    if (fabs(sR) > 1.e-3) smx = smax(smx, fabs(sR));
+    double mL = rhoL * (sL - uL), mR = rhoR * (sR - uR);
+    double sM = (pR - pL + mL * uL - mR * uR) / (mL - mR);
+    if (fabs(sM) < 1.e-8) sM = 0.;
+    if (sL > 0.) {
+      F.m[0] = rhoL * uL;
+      F.q[0] = rhoL * uL * uL + pL; F.q[1] = rhoL * vL * uL; F.q[2] = rhoL * wL * uL;
+      F.E = (rhoL * EL + pL) * uL;
+    } else if (sR < 0.) {
+      F.m[0] = rhoR * uR;
+      F.q[0] = rhoR * uR * uR + pR; F.q[1] = rhoR * vR * uR; F.q[2] = rhoR * wR * uR;
+      F.E = (rhoR * ER + pR) * uR;
+    } else {
+      const bool left = (sM >= 0.);
+      double mS = left ? mL : mR, uS = left ? uL : uR, pS = left ? pL : pR, sS = left ? sL : sR;
+      double ES = left ? EL : ER, vS = left ? vL : vR, wS = left ? wL : wR;
+      double pStar = mS * (sM - uS) + pS;
+      double rhoStar = mS / (sS - sM);
+      double Estar = ES + (sM - uS) * (sM + pS / mS);
+      F.m[0] = rhoStar * sM;
+      F.q[0] = rhoStar * sM * sM + pStar; F.q[1] = rhoStar * sM * vS; F.q[2] = rhoStar * sM * wS;
+      F.E = (rhoStar * Estar + pStar) * sM;
+    }
+    F.sM = sM;
+    return smx;
+  } else {
+    double uL = L.u[0], cL = L.cF, pL = L.pm, rhoL = L.rho;
+    double uR = R.u[0], cR = R.cF, pR = R.pm, rhoR = R.rho;
+    double sL = smin(uL - cL, uR - cR);
+    double sR = smax(uR + cR, uL + cL);
+    double smx = 0.;
+    if (fabs(sL) > 1.e-3) smx = fabs(sL);
+    if (fabs(sR) > 1.e-3) smx = smax(smx, fabs(sR));
+    double mL = rhoL * (sL - uL), mR = rhoR * (sR - uR);
+    double sM = (pR - pL + mL * uL - mR * uR) / (mL - mR);
+    if (fabs(sM) < 1.e-8) sM = 0.;
+    if (sL >= 0. || sR <= 0.) {
+      const bool left = (sL >= 0.);
+      const CellState<NPH> &X = left ? L : R;
+      double uS = left ? uL : uR, pS = left ? pL : pR, rhoS = left ? rhoL : rhoR;
+#pragma unroll
+      for (int k = 0; k < NPH; k++) {
+        F.a[k] = X.a[k] * sM;
+        F.m[k] = X.a[k] * X.r[k] * uS;
+        F.en[k] = X.a[k] * X.r[k] * X.e[k] * uS;
+      }
+      double Et = X.em + 0.5 * sqn(X.u);
+      F.q[0] = rhoS * uS * uS + pS;
+      F.q[1] = rhoS * X.u[1] * uS;
+      F.q[2] = rhoS * X.u[2] * uS;
+      F.E = (rhoS * Et + pS) * uS;
+    } else {
+      const bool left = (sM >= 0.);
+      const CellState<NPH> &X = left ? L : R;
+      double uS = left ? uL : uR, pS = left ? pL : pR, mS = left ? mL : mR, sS = left ? sL : sR;
+      double Et = X.em + 0.5 * sqn(X.u);
+      double rhoStar = mS / (sS - sM);
+      double EStar = Et + (sM - uS) * (sM + pS / mS);
+      double pStar = mS * (sM - uS) + pS;
+#pragma unroll
+      for (int k = 0; k < NPH; k++) {
+        double mk = X.r[k] * (sS - uS);
+        double rks = mk / (sS - sM);
+        double pks = eos_isentrope(S.eos[k], X.p[k], X.r[k], rks, eps);
+        double eks = eos_e(S.eos[k], rks, pks, eps);
+        F.a[k] = X.a[k] * sM;
+        F.m[k] = X.a[k] * rks * sM;
+        F.en[k] = X.a[k] * rks * eks * sM;
+      }
+      F.q[0] = rhoStar * sM * sM + pStar;
+      F.q[1] = rhoStar * X.u[1] * sM;
+      F.q[2] = rhoStar * X.u[2] * sM;
+      F.E = (rhoStar * EStar + pStar) * sM;
+    }
+    F.sM = sM;
+    return smx;
+  }
+}
+
+// ----------------------------------------------------------------------------
+// Face frames (src/Meshes/MeshCartesian.cpp:300-551, src/Maths/Coord.cpp:132-150)
+// side: 0 = interior face / upper boundary of direction dir, 1 = lower boundary.
+// ----------------------------------------------------------------------------
+ECO_D void to_face_frame(int dir, int lower, double *u) {
+  double a = u[0], b = u[1], c = u[2];
+  if (dir == 0) { if (lower) { u[0] = -a; u[1] = -b; u[2] = c; } }
+  else if (dir == 1) { if (lower) { u[0] = -b; u[1] = a; u[2] = c; } else { u[0] = b; u[1] = -a; u[2] = c; } }
+  else { if (lower) { u[0] = -c; u[1] = -a; u[2] = b; } else { u[0] = c; u[1] = a; u[2] = b; } }
+}
+ECO_D void from_face_frame(int dir, int lower, double *f) {
+  double a = f[0], b = f[1], c = f[2];
+  if (dir == 0) { if (lower) { f[0] = -a; f[1] = -b; f[2] = c; } }
+  else if (dir == 1) { if (lower) { f[0] = b; f[1] = -a; f[2] = c; } else { f[0] = -b; f[1] = a; f[2] = c; } }
+  else { if (lower) { f[0] = -b; f[1] = c; f[2] = -a; } else { f[0] = b; f[1] = c; f[2] = a; } }
+}
+
+}  // namespace eco

@@ -1,0 +1,555 @@
+#pragma once
+// gfx950 kernels of the explicit finite-volume step on Cartesian meshes
+// (template definitions; instantiated per model in eco_k_*.hip).
+//
+// One fused kernel per stage: the first-order step (Run::solveHyperbolic,
+// src/Run.cpp:396-412, then Run::solveRelaxations 457-474), the MUSCL
+// predictor (Run.cpp:347-364 with CellO2::predictionOrdre2, CellO2.cpp:376-397)
+// and the MUSCL corrector (Run.cpp:372-390 then solveRelaxations).
+//
+// Layout: field-major SoA in HBM, one double per cell per field.  Each block
+// owns a tile of cells; for every direction the block computes the Riemann
+// fluxes of the faces its tile touches (each interior face of the tile once),
+// stages them in LDS, and each thread accumulates its own cell's faces in the
+// reference order (x-, x+, y-, y+, z-, z+, then boundary faces X-, X+, Y-, Y+,
+// Z-, Z+) -- owner computes, no atomics on the state.  The cell update, the
+// buildPrim, the stiff pressure relaxation (FP64 Newton) and the energy
+// correction are fused behind the accumulation.  The CFL reduction keeps the
+// largest wave speed: wave shuffle, LDS across waves, one 64-bit atomic max
+// per block into a 3-slot ring of the control block (src/Run.cpp:225,270).
+#include <hip/hip_runtime.h>
+#include "eco_physics.hpp"
+#include "eco_launch.h"
+
+namespace eco {
+
+template <int MODEL, int NPH, int DIM>
+struct Layout {
+  static constexpr int NPRIM = (MODEL == MODEL_EULER) ? 2 + DIM : 3 * NPH + DIM;  // stored primitives
+  static constexpr int IE = NPRIM;                                                 // total energy field
+  static constexpr int NVAR = NPRIM;                                               // slope variables
+  static constexpr int NFV = (MODEL == MODEL_EULER) ? DIM + 3 : 3 * NPH + DIM + 2; // flux values + sM
+};
+
+template <int DIM> struct Tile;
+template <> struct Tile<1> { static constexpr int X = 256, Y = 1, Z = 1; };
+template <> struct Tile<2> { static constexpr int X = 64, Y = 4, Z = 1; };
+template <> struct Tile<3> { static constexpr int X = 32, Y = 4, Z = 2; };
+
+template <int DIM>
+struct TileFaces {
+  static constexpr int T[3] = {Tile<DIM>::X, Tile<DIM>::Y, Tile<DIM>::Z};
+  static constexpr int nf(int d) { return (T[0] + (d == 0)) * (T[1] + (d == 1)) * (T[2] + (d == 2)); }
+  static constexpr int MAXF = (nf(0) > nf(1) ? (nf(0) > nf(2) ? nf(0) : nf(2)) : (nf(1) > nf(2) ? nf(1) : nf(2)));
+};
+
+// ---------------------------------------------------------------------------
+// state access
+// ---------------------------------------------------------------------------
+template <int MODEL, int NPH, int DIM>
+__device__ __forceinline__ void load_prim(const double *__restrict__ W, long long nc, long long idx, CellState<NPH> &s) {
+  using Ly = Layout<MODEL, NPH, DIM>;
+  if constexpr (MODEL == MODEL_EULER) {
+    s.r[0] = W[idx];
+    s.p[0] = W[nc + idx];
+#pragma unroll
+    for (int d = 0; d < 3; d++) s.u[d] = (d < DIM) ? W[(2 + d) * nc + idx] : 0.;
+    s.a[0] = 1.;
+  } else {
+#pragma unroll
+    for (int k = 0; k < NPH; k++) {
+      s.a[k] = W[(3 * k) * nc + idx];
+      s.r[k] = W[(3 * k + 1) * nc + idx];
+      s.p[k] = W[(3 * k + 2) * nc + idx];
+    }
+#pragma unroll
+    for (int d = 0; d < 3; d++) s.u[d] = (d < DIM) ? W[(3 * NPH + d) * nc + idx] : 0.;
+  }
+  (void)Ly::NPRIM;
+}
+
+template <int MODEL, int NPH, int DIM>
+__device__ __forceinline__ void store_prim(double *__restrict__ W, long long nc, long long idx, const CellState<NPH> &s) {
+  if constexpr (MODEL == MODEL_EULER) {
+    W[idx] = s.r[0];
+    W[nc + idx] = s.p[0];
+#pragma unroll
+    for (int d = 0; d < DIM; d++) W[(2 + d) * nc + idx] = s.u[d];
+  } else {
+#pragma unroll
+    for (int k = 0; k < NPH; k++) {
+      W[(3 * k) * nc + idx] = s.a[k];
+      W[(3 * k + 1) * nc + idx] = s.r[k];
+      W[(3 * k + 2) * nc + idx] = s.p[k];
+    }
+#pragma unroll
+    for (int d = 0; d < DIM; d++) W[(3 * NPH + d) * nc + idx] = s.u[d];
+  }
+}
+
+// primitive vector used for slopes: (alpha,rho,p)_k, u_0..u_{DIM-1}  (Euler: rho, p, u)
+template <int MODEL, int NPH, int DIM>
+__device__ __forceinline__ void prim_to_vec(const CellState<NPH> &s, double *w) {
+  if constexpr (MODEL == MODEL_EULER) {
+    w[0] = s.r[0]; w[1] = s.p[0];
+#pragma unroll
+    for (int d = 0; d < DIM; d++) w[2 + d] = s.u[d];
+  } else {
+#pragma unroll
+    for (int k = 0; k < NPH; k++) { w[3 * k] = s.a[k]; w[3 * k + 1] = s.r[k]; w[3 * k + 2] = s.p[k]; }
+#pragma unroll
+    for (int d = 0; d < DIM; d++) w[3 * NPH + d] = s.u[d];
+  }
+}
+
+// CellO2::computeLocalSlopes limiter selection (CellO2.cpp:229-246) and
+// Phase*/Mix*::limitSlopes: alpha with the volume-fraction limiter, the rest
+// with the global/interface limiter.
+template <int MODEL, int NPH, int DIM>
+__device__ __forceinline__ void limit_vec(const SchemeParams &S, bool iface, const double *s1, const double *s2, double *out) {
+  using Ly = Layout<MODEL, NPH, DIM>;
+  const int lg = iface ? S.lim[1] : S.lim[0];
+  const int lv = iface ? S.lim[3] : S.lim[2];
+#pragma unroll
+  for (int v = 0; v < Ly::NVAR; v++) {
+    const bool isa = (MODEL != MODEL_EULER) && (v < 3 * NPH) && (v % 3 == 0);
+    out[v] = limiter(isa ? lv : lg, s1[v], s2[v]);
+  }
+}
+
+// Phase*::extrapolate + verifyAndCorrectPhase, Mix*::extrapolate
+// (CellInterfaceO2.cpp:219-226 / 250-257).  sign = -1 for the right state.
+template <int MODEL, int NPH, int DIM>
+__device__ __forceinline__ void extrapolate(const SchemeParams &S, CellState<NPH> &s, const double *sl, double dist, double sign) {
+  if constexpr (MODEL == MODEL_EULER) {
+    s.r[0] += (sign * sl[0]) * dist;
+    s.p[0] += (sign * sl[1]) * dist;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) s.u[d] = s.u[d] + (sign * sl[2 + d]) * dist;
+    verify_correct<MODEL, NPH>(S, s, 0);
+  } else {
+#pragma unroll
+    for (int k = 0; k < NPH; k++) {
+      s.a[k] += (sign * sl[3 * k]) * dist;
+      s.r[k] += (sign * sl[3 * k + 1]) * dist;
+      s.p[k] += (sign * sl[3 * k + 2]) * dist;
+      verify_correct<MODEL, NPH>(S, s, k);
+    }
+#pragma unroll
+    for (int d = 0; d < DIM; d++) s.u[d] = s.u[d] + (sign * sl[3 * NPH + d]) * dist;
+  }
+}
+
+// THINC volume-fraction reconstruction (CellInterfaceO2.cpp:228-247, 259-277).
+template <int NPH>
+__device__ __forceinline__ void thinc(CellState<NPH> &s, double acell, double a_hi, double a_lo, bool right_side) {
+  if constexpr (NPH == 2) {
+    const double beta = 1.6, epsI = 1.e-4;
+    if ((acell >= epsI) && (acell <= 1. - epsI) && ((a_hi - acell) * (acell - a_lo) >= 1.e-8)) {
+      double sign = (a_hi - a_lo > 0.) ? 1. : -1.;
+      double qmin = smin(a_hi, a_lo);
+      double qmax = smax(a_hi, a_lo) - qmin;
+      double C = (acell - qmin + 1.e-20) / (qmax + 1.e-20);
+      double B = exp(sign * beta * (2. * C - 1.));
+      double A = (B / cosh(beta) - 1.) / tanh(beta);
+      double na;
+      if (!right_side) na = qmin + 0.5 * qmax * (1. + sign * (tanh(beta) + A) / (1. + A * tanh(beta)));
+      else na = qmin + 0.5 * qmax * (1. + sign * A);
+      if (na < epsI) na = epsI;
+      if (na > 1. - epsI) na = 1. - epsI;
+      s.a[0] = na;
+      s.a[1] = 1. - na;
+    }
+  }
+}
+
+// pack/unpack the face flux through LDS (structure of arrays, one row per value)
+template <int MODEL, int NPH, int DIM, int MAXF>
+__device__ __forceinline__ void put_flux(double (*L)[MAXF], int f, const FluxVec<NPH> &F) {
+  int q = 0;
+  if constexpr (MODEL == MODEL_EULER) {
+    L[q++][f] = F.m[0];
+  } else {
+#pragma unroll
+    for (int k = 0; k < NPH; k++) { L[q++][f] = F.a[k]; L[q++][f] = F.m[k]; L[q++][f] = F.en[k]; }
+  }
+#pragma unroll
+  for (int d = 0; d < DIM; d++) L[q++][f] = F.q[d];
+  L[q++][f] = F.E;
+  L[q++][f] = F.sM;
+}
+
+template <int MODEL, int NPH, int DIM, int MAXF>
+__device__ __forceinline__ void get_flux(double (*L)[MAXF], int f, FluxVec<NPH> &F) {
+  int q = 0;
+  if constexpr (MODEL == MODEL_EULER) {
+    F.m[0] = L[q++][f];
+  } else {
+#pragma unroll
+    for (int k = 0; k < NPH; k++) { F.a[k] = L[q++][f]; F.m[k] = L[q++][f]; F.en[k] = L[q++][f]; }
+  }
+#pragma unroll
+  for (int d = 0; d < 3; d++) F.q[d] = (d < DIM) ? L[q++][f] : 0.;
+  F.E = L[q++][f];
+  F.sM = L[q++][f];
+}
+
+// CellInterface::addFlux/subtractFlux (CellInterface.cpp:150-180) with the
+// non-conservative terms of FluxKapila/FluxMultiP::addNonCons (owning cell's
+// time-n alpha_k, p_k).  sgn = +1 for the face on the cell's low side.
+template <int MODEL, int NPH>
+__device__ __forceinline__ void add_face(FluxVec<NPH> &U, const FluxVec<NPH> &F, double coefA, bool add,
+                                         const CellState<NPH> &own) {
+  if (add) {
+    if constexpr (MODEL == MODEL_EULER) {
+      U.m[0] += coefA * F.m[0];
+    } else {
+#pragma unroll
+      for (int k = 0; k < NPH; k++) { U.a[k] += coefA * F.a[k]; U.m[k] += coefA * F.m[k]; U.en[k] += coefA * F.en[k]; }
+    }
+#pragma unroll
+    for (int d = 0; d < 3; d++) U.q[d] += F.q[d] * coefA;
+    U.E += coefA * F.E;
+    if constexpr (MODEL != MODEL_EULER) {
+#pragma unroll
+      for (int k = 0; k < NPH; k++) {
+        U.a[k] += -coefA * own.a[k] * F.sM;
+        U.en[k] += coefA * own.a[k] * own.p[k] * F.sM;
+      }
+    }
+  } else {
+    if constexpr (MODEL == MODEL_EULER) {
+      U.m[0] -= coefA * F.m[0];
+    } else {
+#pragma unroll
+      for (int k = 0; k < NPH; k++) { U.a[k] -= coefA * F.a[k]; U.m[k] -= coefA * F.m[k]; U.en[k] -= coefA * F.en[k]; }
+    }
+#pragma unroll
+    for (int d = 0; d < 3; d++) U.q[d] -= F.q[d] * coefA;
+    U.E -= coefA * F.E;
+    if constexpr (MODEL != MODEL_EULER) {
+#pragma unroll
+      for (int k = 0; k < NPH; k++) {
+        U.a[k] -= -coefA * own.a[k] * F.sM;
+        U.en[k] -= coefA * own.a[k] * own.p[k] * F.sM;
+      }
+    }
+  }
+}
+
+template <int NPH>
+__device__ __forceinline__ void zero_flux(FluxVec<NPH> &U) {
+#pragma unroll
+  for (int k = 0; k < NPH; k++) { U.a[k] = 0.; U.m[k] = 0.; U.en[k] = 0.; }
+  U.q[0] = U.q[1] = U.q[2] = 0.;
+  U.E = 0.;
+  U.sM = 0.;
+}
+
+__device__ __forceinline__ double ctl_dt(const Control *ctl, const Geometry &G, double cfl, int step) {
+  if (step == 0) return 1.e-15;
+  unsigned long long bits = __atomic_load_n(&ctl->smax[(step - 1) % 3], __ATOMIC_RELAXED);
+  double sm = __longlong_as_double((long long)bits);
+  double dtMax = (sm > 0.) ? smin(1.e10, G.lcfl / sm) : 1.e10;
+  return cfl * dtMax;
+}
+
+// ---------------------------------------------------------------------------
+// the stage kernel
+// STAGE 0: first-order step; 1: MUSCL predictor (writes U*); 2: MUSCL corrector
+// ---------------------------------------------------------------------------
+template <int MODEL, int NPH, int DIM, int STAGE, bool SAMELIM>
+__global__ void __launch_bounds__(Tile<DIM>::X * Tile<DIM>::Y * Tile<DIM>::Z)
+k_stage(const StageArgs A) {
+  using Ly = Layout<MODEL, NPH, DIM>;
+  using TF = TileFaces<DIM>;
+  constexpr int TX = Tile<DIM>::X, TY = Tile<DIM>::Y, TZ = Tile<DIM>::Z;
+  constexpr int NT = TX * TY * TZ;
+  constexpr int MAXF = TF::MAXF;
+  constexpr bool O2 = (STAGE != 0);
+  __shared__ double sF[Ly::NFV][MAXF];
+  __shared__ double sRed[NT / 64];
+
+  const Geometry &G = A.geo;
+  const SchemeParams &S = A.sch;
+  Control *ctl = A.ctl;
+  const int step = A.step;
+  const int tid = threadIdx.x;
+  const bool first_kernel = (STAGE != 2);
+  const bool last_kernel = (STAGE != 1);
+
+  // run already finished: no-op step (the host may launch a few extra steps)
+  if (step > 0 && ctl->done[(step + 1) & 1]) {
+    if (last_kernel && blockIdx.x == 0 && tid == 0) {
+      ctl->done[step & 1] = 1;
+      ctl->t[step & 1] = ctl->t[(step + 1) & 1];
+    }
+    return;
+  }
+  const double dt = ctl_dt(ctl, G, S.cfl, step);
+  if (first_kernel && blockIdx.x == 0 && tid == 0) ctl->smax[(step + 1) % 3] = 0ull;
+
+  const long long nc = G.ncell;
+  const int nx = G.n[0], ny = G.n[1], nz = G.n[2];
+  // tile origin (blockIdx.x linearised over the tile grid, x fastest)
+  const int tgx = (nx + TX - 1) / TX, tgy = (ny + TY - 1) / TY;
+  int b = blockIdx.x;
+  const int bx = b % tgx; b /= tgx;
+  const int by = b % tgy; b /= tgy;
+  const int bz = b;
+  const int o[3] = {bx * TX, by * TY, bz * TZ};
+  const int lx = tid % TX, ly = (tid / TX) % TY, lz = tid / (TX * TY);
+  const int ci[3] = {o[0] + lx, o[1] + ly, o[2] + lz};
+  const bool active = ci[0] < nx && ci[1] < ny && ci[2] < nz;
+  const long long strd[3] = {1, (long long)nx, (long long)nx * ny};
+  const long long cidx = active ? ci[0] + strd[1] * ci[1] + strd[2] * ci[2] : 0;
+
+  const double *__restrict__ Wsrc = A.wsrc;  // state the fluxes are built from (U^n or U*)
+  const double *__restrict__ Wn = A.wn;      // time-n state
+
+  // own time-n state (non-conservative terms, buildCons)
+  CellState<NPH> own;
+  load_prim<MODEL, NPH, DIM>(Wn, nc, cidx, own);
+  own.E = Wn[Ly::IE * nc + cidx];
+
+  FluxVec<NPH> acc;
+  zero_flux(acc);
+  double smx = 0.;
+
+#pragma unroll
+  for (int dir = 0; dir < DIM; dir++) {
+    const int T0 = TX + (dir == 0), T1 = TY + (dir == 1);
+    const int nfd = T0 * T1 * (TZ + (dir == 2));
+    const int ndir = G.n[dir];
+    for (int f = tid; f < nfd; f += NT) {
+      const int fx = f % T0, fy = (f / T0) % T1, fz = f / (T0 * T1);
+      int g[3] = {o[0] + fx, o[1] + fy, o[2] + fz};  // cell on the high side of the face
+      if (g[0] >= nx + (dir == 0) || g[1] >= ny + (dir == 1) || g[2] >= nz + (dir == 2)) continue;
+      const int gi = g[dir];
+      if (gi < 1 || gi > ndir - 1) continue;  // boundary faces handled per cell below
+      const long long ihi = g[0] + strd[1] * g[1] + strd[2] * g[2];
+      const long long ilo = ihi - strd[dir];
+      const int i = gi - 1;  // index along dir of the low-side cell
+      CellState<NPH> L, R;
+      load_prim<MODEL, NPH, DIM>(Wsrc, nc, ilo, L);
+      load_prim<MODEL, NPH, DIM>(Wsrc, nc, ihi, R);
+      if constexpr (!O2) {
+        if constexpr (MODEL == MODEL_EULER) {
+          // first order, cached FluxEuler::buildPrim state: E stored, c from (rho, p)
+          L.E = Wsrc[Ly::IE * nc + ilo]; R.E = Wsrc[Ly::IE * nc + ihi];
+          L.c[0] = eos_c(S.eos[0], L.r[0], L.p[0], S.eps);
+          R.c[0] = eos_c(S.eos[0], R.r[0], R.p[0], S.eps);
+        } else {
+          fulfill<MODEL, NPH>(S, L);
+          fulfill<MODEL, NPH>(S, R);
+          if constexpr (MODEL == MODEL_MULTIP) {
+            // MultiP keeps the mixture energy of totalEnergyToInternalEnergy (Run.cpp:471)
+            L.em = Wsrc[Ly::IE * nc + ilo] - 0.5 * sqn(L.u);
+            R.em = Wsrc[Ly::IE * nc + ihi] - 0.5 * sqn(R.u);
+          }
+        }
+      } else {
+        // MUSCL: slopes of both cells from the faces i-1/2, i+1/2, i+3/2
+        double wlo[Ly::NVAR], whi[Ly::NVAR], wll[Ly::NVAR], whh[Ly::NVAR];
+        double sc[Ly::NVAR], sm[Ly::NVAR], sp[Ly::NVAR];
+        prim_to_vec<MODEL, NPH, DIM>(L, wlo);
+        prim_to_vec<MODEL, NPH, DIM>(R, whi);
+        const double dc = G.dcc[dir][i];
+#pragma unroll
+        for (int v = 0; v < Ly::NVAR; v++) sc[v] = (whi[v] - wlo[v]) / dc;
+        const bool has_m = (i >= 1), has_p = (i + 2 <= ndir - 1);
+        CellState<NPH> T;
+        if (has_m) {
+          load_prim<MODEL, NPH, DIM>(Wsrc, nc, ilo - strd[dir], T);
+          prim_to_vec<MODEL, NPH, DIM>(T, wll);
+          const double dm = G.dcc[dir][i - 1];
+#pragma unroll
+          for (int v = 0; v < Ly::NVAR; v++) sm[v] = (wlo[v] - wll[v]) / dm;
+        } else {
+#pragma unroll
+          for (int v = 0; v < Ly::NVAR; v++) sm[v] = 0.;
+        }
+        if (has_p) {
+          load_prim<MODEL, NPH, DIM>(Wsrc, nc, ihi + strd[dir], T);
+          prim_to_vec<MODEL, NPH, DIM>(T, whh);
+          const double dp = G.dcc[dir][i + 1];
+#pragma unroll
+          for (int v = 0; v < Ly::NVAR; v++) sp[v] = (whh[v] - whi[v]) / dp;
+        } else {
+#pragma unroll
+          for (int v = 0; v < Ly::NVAR; v++) sp[v] = 0.;
+        }
+        // interface detection: alpha of the face's cells from the stage state,
+        // alpha across the opposite face from the time-n state (CellO2.cpp:144-150)
+        double aL = 0., aR = 0., aLL = 0., aRR = 0.;
+        if constexpr (MODEL != MODEL_EULER) {
+          aL = L.a[0]; aR = R.a[0];
+          if (!SAMELIM || S.thinc) {
+            aLL = has_m ? Wn[ilo - strd[dir]] : 0.;
+            aRR = has_p ? Wn[ihi + strd[dir]] : 0.;
+          }
+        }
+        bool ifl = false, ifr = false;
+        if constexpr (!SAMELIM && MODEL != MODEL_EULER) {
+          ifl = (aL >= 1.e-4) && (aL <= 1. - 1.e-4) && ((aR - aL) * (aL - aLL) >= 1.e-8);
+          ifr = (aR >= 1.e-4) && (aR <= 1. - 1.e-4) && ((aL - aR) * (aR - aRR) >= 1.e-8);
+        }
+        double sl[Ly::NVAR], sr[Ly::NVAR];
+        limit_vec<MODEL, NPH, DIM>(S, ifl, sc, sm, sl);
+        limit_vec<MODEL, NPH, DIM>(S, ifr, sc, sp, sr);
+        extrapolate<MODEL, NPH, DIM>(S, L, sl, G.dfl[dir][i], 1.);
+        if (S.thinc) thinc<NPH>(L, aL, aR, aLL, false);
+        extrapolate<MODEL, NPH, DIM>(S, R, sr, G.dfr[dir][i], -1.);
+        if (S.thinc) thinc<NPH>(R, aR, aRR, aL, true);
+      }
+      to_face_frame(dir, 0, L.u);
+      to_face_frame(dir, 0, R.u);
+      if constexpr (O2) {
+        fulfill<MODEL, NPH>(S, L);
+        fulfill<MODEL, NPH>(S, R);
+      }
+      FluxVec<NPH> F;
+      smx = smax(smx, hllc<MODEL, NPH>(S, L, R, F));
+      from_face_frame(dir, 0, F.q);
+      put_flux<MODEL, NPH, DIM, MAXF>(sF, f, F);
+    }
+    __syncthreads();
+    if (active) {
+      const int lc[3] = {lx, ly, lz};
+      const int T0_ = TX + (dir == 0), T1_ = TY + (dir == 1);
+      int fl[3] = {lc[0], lc[1], lc[2]};
+      const int flo = fl[0] + T0_ * (fl[1] + T1_ * fl[2]);
+      fl[dir] += 1;
+      const int fhi = fl[0] + T0_ * (fl[1] + T1_ * fl[2]);
+      FluxVec<NPH> F;
+      if (ci[dir] >= 1) {
+        get_flux<MODEL, NPH, DIM, MAXF>(sF, flo, F);
+        add_face<MODEL, NPH>(acc, F, G.coefA[dir], true, own);
+      }
+      if (ci[dir] <= ndir - 2) {
+        get_flux<MODEL, NPH, DIM, MAXF>(sF, fhi, F);
+        add_face<MODEL, NPH>(acc, F, G.coefA[dir], false, own);
+      }
+    }
+    __syncthreads();
+  }
+
+  if (active) {
+    // transmissive boundary faces (BoundCond::computeFlux + BoundCondAbs, BoundCond.cpp:68-100)
+#pragma unroll
+    for (int dir = 0; dir < DIM; dir++) {
+#pragma unroll
+      for (int side = 0; side < 2; side++) {
+        const bool on = side == 0 ? (ci[dir] == 0) : (ci[dir] == G.n[dir] - 1);
+        if (!on) continue;
+        CellState<NPH> L;
+        load_prim<MODEL, NPH, DIM>(Wsrc, nc, cidx, L);
+        to_face_frame(dir, side == 0, L.u);
+        fulfill<MODEL, NPH>(S, L);
+        FluxVec<NPH> F;
+        smx = smax(smx, hllc<MODEL, NPH>(S, L, L, F));
+        from_face_frame(dir, side == 0, F.q);
+        add_face<MODEL, NPH>(acc, F, G.coefA[dir], false, own);
+      }
+    }
+
+    // cell update
+    CellState<NPH> sn = own;
+    if constexpr (MODEL != MODEL_EULER) {
+      sn.rho = 0.;
+#pragma unroll
+      for (int k = 0; k < NPH; k++) sn.rho += sn.a[k] * sn.r[k];
+    }
+    FluxVec<NPH> buf;
+    build_cons<MODEL, NPH>(S, sn, buf);
+    scale_add<MODEL, NPH>(acc, (STAGE == 1) ? 0.5 * dt : dt, buf);
+    scheme_correction<MODEL, NPH>(acc, buf);
+    CellState<NPH> s1;
+    build_prim<MODEL, NPH>(S, acc, s1);
+    bool ok = true;
+    if constexpr (MODEL != MODEL_EULER) {
+      if (S.relax) ok = relax_p<MODEL, NPH>(S, s1);
+      correction_energy<MODEL, NPH>(S, s1);
+    }
+    if (!ok) atomicOr(&ctl->err, 1u);
+    bool bad = false;
+    if constexpr (MODEL != MODEL_EULER) {
+#pragma unroll
+      for (int k = 0; k < NPH; k++) bad |= !(s1.p[k] == s1.p[k]) || !(s1.r[k] == s1.r[k]);
+    } else {
+      bad = !(s1.p[0] == s1.p[0]);
+    }
+    if (bad) atomicOr(&ctl->err, 2u);
+    store_prim<MODEL, NPH, DIM>(A.wout, nc, cidx, s1);
+    if constexpr (STAGE != 1) A.wout[Ly::IE * nc + cidx] = s1.E;
+  }
+
+  // CFL reduction: wave shuffle, LDS, one atomic per block
+  for (int off = 32; off > 0; off >>= 1) smx = smax(smx, __shfl_xor(smx, off));
+  if ((tid & 63) == 0) sRed[tid >> 6] = smx;
+  __syncthreads();
+  if (tid == 0) {
+    double m = sRed[0];
+#pragma unroll
+    for (int w = 1; w < NT / 64; w++) m = smax(m, sRed[w]);
+    if (m > 0.) atomicMax(&ctl->smax[step % 3], (unsigned long long)__double_as_longlong(m));
+    if (last_kernel && blockIdx.x == 0) {
+      double t = ((step == 0) ? 0. : ctl->t[(step + 1) & 1]) + dt;
+      ctl->t[step & 1] = t;
+      ctl->steps = step + 1;
+      ctl->dt_last = dt;
+      int done = (ctl->t_final < 0.) ? (step + 1 >= ctl->max_steps)
+                                      : (t >= (double)(float)ctl->t_final);
+      ctl->done[step & 1] = done;
+    }
+  }
+}
+
+// E = e + |u|^2/2 at t = 0 (Cell::completeFulfillState, Cell.cpp:271-278)
+template <int MODEL, int NPH, int DIM>
+__global__ void k_init(const StageArgs A) {
+  using Ly = Layout<MODEL, NPH, DIM>;
+  const long long nc = A.geo.ncell;
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= nc) return;
+  CellState<NPH> s;
+  load_prim<MODEL, NPH, DIM>(A.wsrc, nc, i, s);
+  fulfill<MODEL, NPH>(A.sch, s);
+  if constexpr (MODEL != MODEL_EULER) s.E = s.em + 0.5 * sqn(s.u);
+  A.wout[Ly::IE * nc + i] = s.E;
+}
+
+// ---------------------------------------------------------------------------
+// host-side dispatch
+// ---------------------------------------------------------------------------
+template <int MODEL, int NPH, int DIM>
+int launch_t(const StageArgs &A, int stage, hipStream_t st) {
+  constexpr int NT = Tile<DIM>::X * Tile<DIM>::Y * Tile<DIM>::Z;
+  const int tgx = (A.geo.n[0] + Tile<DIM>::X - 1) / Tile<DIM>::X;
+  const int tgy = (A.geo.n[1] + Tile<DIM>::Y - 1) / Tile<DIM>::Y;
+  const int tgz = (A.geo.n[2] + Tile<DIM>::Z - 1) / Tile<DIM>::Z;
+  dim3 grid((unsigned)(tgx * tgy * tgz)), block(NT);
+  const bool same = A.sch.same_lim != 0;
+  if (stage == 0) hipLaunchKernelGGL((k_stage<MODEL, NPH, DIM, 0, true>), grid, block, 0, st, A);
+  else if (stage == 1 && same) hipLaunchKernelGGL((k_stage<MODEL, NPH, DIM, 1, true>), grid, block, 0, st, A);
+  else if (stage == 1) hipLaunchKernelGGL((k_stage<MODEL, NPH, DIM, 1, false>), grid, block, 0, st, A);
+  else if (stage == 2 && same) hipLaunchKernelGGL((k_stage<MODEL, NPH, DIM, 2, true>), grid, block, 0, st, A);
+  else if (stage == 2) hipLaunchKernelGGL((k_stage<MODEL, NPH, DIM, 2, false>), grid, block, 0, st, A);
+  else if (stage == 3) {
+    const long long nc = A.geo.ncell;
+    hipLaunchKernelGGL((k_init<MODEL, NPH, DIM>), dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, st, A);
+  }
+  return (int)hipGetLastError();
+}
+
+template <int MODEL, int NPH>
+int launch_d(const StageArgs &A, int stage, hipStream_t st) {
+  switch (A.geo.dim) {
+    case 1: return launch_t<MODEL, NPH, 1>(A, stage, st);
+    case 2: return launch_t<MODEL, NPH, 2>(A, stage, st);
+    case 3: return launch_t<MODEL, NPH, 3>(A, stage, st);
+  }
+  return -1;
+}
+
+}  // namespace eco

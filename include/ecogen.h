@@ -1,0 +1,132 @@
+/*
+ * ecogen.h -- C ABI of the MI355X explicit finite-volume step.
+ *
+ * Drop-in boundary for the reference ECOGEN time step on Cartesian meshes.
+ * The reference has no FFI: its boundary is the C++ virtual API driven by
+ * Run::solver (src/Run.cpp:204-277).  Each entry point below names the
+ * reference routine(s) it replaces.  Plain pointers and sizes only; the host
+ * owns host arrays, a context owns its device memory.  Errors are returned as
+ * codes (never exit), soft numerical events as flags.
+ *
+ * State layout (host and device SoA, field-major, ncell = n[0]*n[1]*n[2]
+ * doubles per field, cell index i + j*n[0] + k*n[0]*n[1] as in
+ * MeshCartesian::construitIGlobal, src/Meshes/MeshCartesian.cpp:154-160):
+ *     for each phase k: alpha_k, rho_k, p_k     (Euler: 1, rho, p)
+ *     u, v, w
+ *     E  (mixture specific total energy; written by eco_get_state, ignored on input)
+ * => ECO_NFIELDS(nphase) = 3*nphase + 4 fields.
+ */
+#ifndef ECOGEN_H
+#define ECOGEN_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ECO_NFIELDS(nph) (3 * (nph) + 4)
+
+enum eco_model { ECO_MODEL_EULER = 0, ECO_MODEL_KAPILA = 1, ECO_MODEL_MULTIP = 2 };
+enum eco_limiter { ECO_LIM_NONE = 0, ECO_LIM_MINMOD = 1, ECO_LIM_VANLEER = 2, ECO_LIM_VANALBADA = 3,
+                   ECO_LIM_SUPERBEE = 4, ECO_LIM_MC = 5, ECO_LIM_THINC = 6 };
+enum eco_eos_type { ECO_EOS_IG = 0, ECO_EOS_SG = 1 };
+enum eco_precision { ECO_FP64 = 0, ECO_FP32S = 1 };
+enum eco_bc { ECO_BC_ABS = 0 };
+
+enum eco_status_code {
+  ECO_OK = 0,
+  ECO_ERR_INVALID = -1,      /* bad argument or configuration */
+  ECO_ERR_UNSUPPORTED = -2,  /* valid reference option not provided by the device path */
+  ECO_ERR_HIP = -3,          /* HIP runtime error (launch, allocation, copy) */
+  ECO_ERR_NOMEM = -4,        /* device allocation failed */
+  ECO_ERR_STATE = -5         /* call out of order (e.g. step before set_state) */
+};
+
+/* Soft-error flags (mirror the reference's non-fatal errors vector). */
+#define ECO_WARN_RELAX_NOT_CONVERGED 1u /* RelaxationP Newton hit 100 iterations (src/Relaxations/RelaxationP.cpp:92-109) */
+#define ECO_WARN_NAN 2u                 /* non-finite pressure or density produced */
+
+/* Equation of state (src/Eos/EosIG.h, src/Eos/EosSG.h; parameters of libEOS/*.xml). */
+typedef struct {
+  int type;                       /* ECO_EOS_IG | ECO_EOS_SG */
+  double gamma, pinf, cv, eref;   /* pinf ignored for IG */
+} eco_eos;
+
+/* Solve configuration: what Input::entreeMain/entreeMesh/entreeModel read
+   from the deck (src/InputOutput/Input.cpp:85-625) for this path. */
+typedef struct {
+  int model;        /* enum eco_model */
+  int nphase;       /* 1 for Euler, 2 for Kapila/MultiP on the device path */
+  int order;        /* 1 = FIRSTORDER, 2 = SECONDORDER (MUSCL predictor-corrector) */
+  int lim[4];       /* global, interface, global volume-fraction, interface volume-fraction limiter */
+  int alpha_null;   /* Kapila flowModel alphaNull attribute */
+  int relax_p;      /* MultiP <relaxation type="P"/> (Kapila always relaxes) */
+  int precision;    /* enum eco_precision */
+  int n[3];         /* numberCells x, y, z */
+  double L[3];      /* dimensions x, y, z (m) */
+  double cfl;       /* computationControl CFL */
+  int bc[6];        /* X-, X+, Y-, Y+, Z-, Z+ : ECO_BC_ABS */
+  eco_eos eos[4];   /* per phase */
+} eco_config;
+
+typedef struct eco_ctx eco_ctx;
+
+typedef struct {
+  int steps;          /* steps taken since eco_set_state */
+  double t;           /* physical time */
+  double dt_last;     /* dt of the last step taken */
+  double dt_next;     /* CFL*dtMax for the next step */
+  uint32_t flags;     /* ECO_WARN_* accumulated */
+  int done;           /* the last eco_run/eco_launch target was reached */
+} eco_status;
+
+/* Replaces Run::Run + Run::initialize's mesh/array allocation
+   (src/Run.cpp:43,55-108): allocates the device SoA state of one solve.
+   hip_stream: hipStream_t to run on (NULL = default stream). */
+int eco_create(const eco_config *cfg, void *hip_stream, eco_ctx **out);
+
+/* Replaces Cell::fill + Cell::completeFulfillState (src/Cell.cpp:127-155,
+   271-278): uploads the initial primitive state (layout above, E ignored),
+   resets time, step counter and the lagged dt (first step uses 1e-15). */
+int eco_set_state(eco_ctx *ctx, const double *host_state);
+
+/* Same from a device pointer (same layout), no host copy. */
+int eco_set_state_device(eco_ctx *ctx, const double *dev_state);
+
+/* Downloads the current state (layout above, E included). Synchronises. */
+int eco_get_state(eco_ctx *ctx, double *host_state);
+
+/* One full time step of Run::solver (src/Run.cpp:220-272: face fluxes,
+   cell update, relaxation, energy correction, dtMax reduction). Synchronises
+   and reports the status. */
+int eco_step(eco_ctx *ctx, eco_status *status);
+
+/* The time loop of Run::solver: nsteps > 0 runs exactly nsteps more steps
+   (iteration control), nsteps <= 0 runs until t >= t_final compared as float
+   (physicalTime control, src/Run.cpp:239-244, Input.cpp:163).  Synchronises. */
+int eco_run(eco_ctx *ctx, int nsteps, double t_final, eco_status *status);
+
+/* Asynchronous form: enqueue nsteps steps on the context stream, no host sync.
+   Steps past a reached t_final are no-ops on the device. */
+int eco_launch_steps(eco_ctx *ctx, int nsteps);
+
+/* Waits for the context stream and fills status. */
+int eco_sync(eco_ctx *ctx, eco_status *status);
+
+/* Device pointer of the current state (device layout: per phase alpha, rho, p,
+   then the nondegenerate velocity components, then E) for zero-copy access. */
+const double *eco_state_device(eco_ctx *ctx, int *nfields_device);
+
+/* Releases device memory (Run::finalize, src/Run.cpp:496-530). */
+void eco_destroy(eco_ctx *ctx);
+
+const char *eco_strerror(int code);
+
+/* Library version string. */
+const char *eco_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

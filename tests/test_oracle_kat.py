@@ -1,0 +1,335 @@
+"""Known-answer and invariant tests that pin the oracle without the reference.
+
+Each check is chosen so that a plausible slip in the oracle (a dropped term,
+a wrong sign or index, a swapped operand) fails it: closed forms of the limiter
+family, thermodynamic consistency of the EOS (sound speed = isentrope slope,
+derivative outputs = finite differences), the relaxation root against a brute
+force bisection, HLLC consistency and mirror symmetry, Sod's problem against
+the exact Riemann solution, Abgrall's uniform-pressure interface advection,
+and conservation on closed-in problems.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+from cases import AIR, WATER, _c1, _euler_1d
+
+L = O.lib()
+LIM = O.LIMITERS
+
+
+def lim(name, a, b):
+    return L.orc_limiter(LIM[name], a, b)
+
+
+# ----------------------------------------------------------------------------- limiters
+@pytest.mark.parametrize("name", ["minmod", "vanleer", "vanalbada", "superbee", "mc", "thinc"])
+def test_limiter_basic_properties(name):
+    for a in (0.3, 2.0, 17.5):
+        assert lim(name, a, a) == pytest.approx(a, rel=1e-15)
+        assert lim(name, -a, -a) == pytest.approx(-a, rel=1e-15)
+        assert lim(name, a, -a) == 0.0
+        assert lim(name, a, 0.0) == 0.0
+    for a, b in ((0.5, 3.0), (-2.0, -0.25), (7.0, 1.0)):
+        assert lim(name, a, b) == pytest.approx(lim(name, b, a), rel=1e-15)
+        # TVD region: between minmod and superbee
+        lo, hi = lim("minmod", a, b), lim("superbee", a, b)
+        v = lim(name, a, b)
+        assert min(lo, hi) - 1e-15 <= v <= max(lo, hi) + 1e-15
+
+
+def test_limiter_closed_forms():
+    assert lim("minmod", 1.0, 3.0) == 1.0
+    assert lim("vanleer", 1.0, 3.0) == pytest.approx(1.5)
+    assert lim("vanalbada", 1.0, 3.0) == pytest.approx(1.2)
+    assert lim("mc", 1.0, 3.0) == pytest.approx(2.0)
+    assert lim("superbee", 1.0, 3.0) == pytest.approx(2.0)
+    assert lim("superbee", 1.0, 1.5) == pytest.approx(1.5)
+
+
+def test_limiter_thresholds():
+    # products at or below the absolute thresholds give a zero slope
+    assert lim("vanleer", 1e-3, 1e-3) == 0.0          # product 1e-6, not > 1e-6
+    assert lim("vanleer", 2e-3, 1e-3) != 0.0
+    assert lim("minmod", 3e-5, 3e-5) == 0.0           # 9e-10 < 1e-9
+    assert lim("minmod", 4e-5, 4e-5) == pytest.approx(4e-5)
+    assert lim("superbee", 1e-3, 1e-3) == 0.0
+
+
+# ----------------------------------------------------------------------------- EOS
+def eos(e):
+    return O.make_eos(e)
+
+
+@pytest.mark.parametrize("e,rho,p", [(WATER, 1000.0, 1e5), (WATER, 1100.0, 1e9), (AIR, 50.0, 1e5), (AIR, 1.2, 6.5e5)])
+def test_eos_round_trip_and_sound_speed(e, rho, p):
+    q = eos(e)
+    ein = L.orc_eos_energy(C.byref(q), rho, p, 0.0)
+    assert L.orc_eos_pressure(C.byref(q), rho, ein) == pytest.approx(p, rel=1e-12, abs=1e-3)
+    # c^2 = dp/drho along the isentrope through (rho, p)
+    h = rho * 1e-6
+    dp = (L.orc_eos_isentrope(C.byref(q), p, rho, rho + h, 0.0) - L.orc_eos_isentrope(C.byref(q), p, rho, rho - h, 0.0)) / (2 * h)
+    c = L.orc_eos_sound(C.byref(q), rho, p, 0.0)
+    assert c * c == pytest.approx(dp, rel=1e-6)
+
+
+def test_eos_water_known_value():
+    q = eos(WATER)
+    # stiffened gas: e = (p + gamma*pinf)/((gamma-1)*rho); water rho=1000, p=1e5
+    assert L.orc_eos_energy(C.byref(q), 1000.0, 1e5, 0.0) == pytest.approx((1e5 + 4.4 * 6e8) / (3.4 * 1000.0), rel=1e-14)
+    # ideal gas air rho=1.2, p=1e5: c = sqrt(1.4e5/1.2)
+    assert L.orc_eos_sound(C.byref(eos(AIR)), 1.2, 1e5, 0.0) == pytest.approx(math.sqrt(1.4e5 / 1.2), rel=1e-15)
+
+
+def test_isentrope_first_law():
+    # along an isentrope de = p/rho^2 drho
+    for e, rho0, p0 in ((WATER, 1000.0, 1e8), (AIR, 2.0, 3e5)):
+        q = eos(e)
+        rhos = np.linspace(rho0, rho0 * 1.05, 2001)
+        ps = np.array([L.orc_eos_isentrope(C.byref(q), p0, rho0, r, 0.0) for r in rhos])
+        es = np.array([L.orc_eos_energy(C.byref(q), r, p, 0.0) for r, p in zip(rhos, ps)])
+        integral = np.trapezoid(ps / rhos ** 2, rhos)
+        assert es[-1] - es[0] == pytest.approx(integral, rel=1e-6)
+
+
+@pytest.mark.parametrize("e", [WATER, AIR])
+def test_rho_pfinal_derivative(e):
+    q = eos(e)
+    p0, rho0 = 2e6, (1000.0 if e is WATER else 20.0)
+    for pf in (1e5, 5e6, 3e7):
+        d = np.zeros(1)
+        r = L.orc_eos_rho_pfinal(C.byref(q), p0, rho0, pf, d.ctypes.data_as(C.POINTER(C.c_double)), 0.0)
+        h = pf * 1e-6
+        rp = L.orc_eos_rho_pfinal(C.byref(q), p0, rho0, pf + h, None, 0.0)
+        rm = L.orc_eos_rho_pfinal(C.byref(q), p0, rho0, pf - h, None, 0.0)
+        assert d[0] == pytest.approx((rp - rm) / (2 * h), rel=1e-6)
+        assert L.orc_eos_rho_pfinal(C.byref(q), p0, rho0, p0, None, 0.0) == pytest.approx(rho0, rel=1e-15)
+        assert r > 0
+
+
+# ----------------------------------------------------------------------------- relaxation
+def _relax(alpha, rho, p, model="kapila"):
+    case = dict(_c1(False), model=model, relax_p=True)
+    cfg = O.make_cfg(case)
+    a = np.array(alpha, dtype=np.float64)
+    r = np.array(rho, dtype=np.float64)
+    pp = np.array(p, dtype=np.float64)
+    dp = C.POINTER(C.c_double)
+    it = L.orc_relax_p(C.byref(cfg), a.ctypes.data_as(dp), r.ctypes.data_as(dp), pp.ctypes.data_as(dp))
+    return it, a, r, pp, cfg
+
+
+@pytest.mark.parametrize("alpha,rho,p", [
+    ([0.3, 0.7], [1.2, 1000.0], [1e5, 5e7]),
+    ([0.9, 0.1], [50.0, 1010.0], [2e9, 1e5]),
+    ([1e-3, 1 - 1e-3], [5.0, 990.0], [1e6, 1e6 + 1e3]),
+])
+def test_relaxation_root_and_invariants(alpha, rho, p):
+    it, a, r, pp, cfg = _relax(alpha, rho, p)
+    assert 0 < it < 100
+    assert pp[0] == pp[1]
+    assert abs(a.sum() - 1.0) <= 1e-10
+    for k in range(2):
+        assert a[k] * r[k] == pytest.approx(alpha[k] * rho[k], rel=1e-13)
+    # brute force: bisection on f(p) = sum alpha_k rho_k / rho_k*(p) - 1
+    eq = [O.make_eos(AIR), O.make_eos(WATER)]
+
+    def f(ps):
+        s = -1.0
+        for k in range(2):
+            rs = L.orc_eos_rho_pfinal(C.byref(eq[k]), p[k], rho[k], ps, None, 0.0)
+            s += alpha[k] * rho[k] / rs
+        return s
+    lo, hi = 1e-3, 1e10
+    assert f(lo) * f(hi) < 0
+    for _ in range(200):
+        mid = 0.5 * (lo + hi)
+        if f(lo) * f(mid) <= 0:
+            hi = mid
+        else:
+            lo = mid
+    assert pp[0] == pytest.approx(0.5 * (lo + hi), rel=1e-8)
+
+
+def test_relaxation_equilibrium_is_fixed_point():
+    it, a, r, pp, _ = _relax([0.4, 0.6], [1.1, 1000.0], [3e5, 3e5])
+    assert it == 1
+    assert pp[0] == 3e5 and a[0] == pytest.approx(0.4, rel=1e-15)
+
+
+# ----------------------------------------------------------------------------- HLLC
+def _hllc(case, wl, wr, dx=1e-3):
+    cfg = O.make_cfg(case)
+    nph = len(case["eos"])
+    nout = (3 * nph if case["model"] != "euler" else 1) + 5
+    out = np.zeros(nout)
+    dt = np.array([1e10])
+    dp = C.POINTER(C.c_double)
+    wl = np.asarray(wl, dtype=np.float64)
+    wr = np.asarray(wr, dtype=np.float64)
+    L.orc_hllc(C.byref(cfg), wl.ctypes.data_as(dp), wr.ctypes.data_as(dp), dx, dx, out.ctypes.data_as(dp),
+               dt.ctypes.data_as(dp))
+    return out, dt[0]
+
+
+@pytest.mark.parametrize("model", ["kapila", "multip"])
+@pytest.mark.parametrize("u", [0.0, 35.0, -120.0, 2500.0])
+def test_hllc_consistency(model, u):
+    case = dict(_c1(False), model=model)
+    a, r, p = [0.25, 0.75], [1.5, 1001.0], 2.5e6
+    w = [a[0], r[0], p, a[1], r[1], p, u, 3.0, -1.0]
+    F, dt = _hllc(case, w, w)
+    rho = a[0] * r[0] + a[1] * r[1]
+    e = [p / 0.4 / r[0], (p + 4.4 * 6e8) / 3.4 / r[1]]
+    Y = [a[k] * r[k] / rho for k in range(2)]
+    E = Y[0] * e[0] + Y[1] * e[1] + 0.5 * (u * u + 9.0 + 1.0)
+    tol = dict(rel=1e-12, abs=1e-9 * (1 + abs(u)) * rho)
+    for k in range(2):
+        assert F[k] == pytest.approx(a[k] * u, rel=1e-12, abs=1e-12)          # alpha flux (sM = u)
+        assert F[2 + k] == pytest.approx(a[k] * r[k] * u, **tol)               # mass
+        assert F[4 + k] == pytest.approx(a[k] * r[k] * e[k] * u, rel=1e-11, abs=1e-6 * (1 + abs(u)) * rho * e[k])
+    assert F[6] == pytest.approx(rho * u * u + p, rel=1e-12)                   # normal momentum
+    assert F[7] == pytest.approx(rho * 3.0 * u, **tol)
+    assert F[9] == pytest.approx((rho * E + p) * u, rel=1e-11, abs=1e-6 * (1 + abs(u)) * rho * E)
+    c = math.sqrt(Y[0] * 1.4 * p / r[0] + Y[1] * 4.4 * (p + 6e8) / r[1])
+    assert dt == pytest.approx(1e-3 / (abs(u) + c), rel=1e-12)
+
+
+def test_hllc_mirror_symmetry():
+    case = dict(_c1(False))
+    wl = [0.2, 1.3, 2e6, 0.8, 1000.0, 2e6, 40.0, 0.0, 0.0]
+    wr = [0.9, 1.1, 1e5, 0.1, 998.0, 1e5, -10.0, 0.0, 0.0]
+    F, _ = _hllc(case, wl, wr)
+    ml = list(wr); ml[6] = -wr[6]
+    mr = list(wl); mr[6] = -wl[6]
+    G, _ = _hllc(case, ml, mr)
+    for i in range(6):
+        assert G[i] == pytest.approx(-F[i], rel=1e-12, abs=1e-9)
+    assert G[6] == pytest.approx(F[6], rel=1e-12)
+    assert G[9] == pytest.approx(-F[9], rel=1e-12)
+
+
+# ----------------------------------------------------------------------------- whole step
+def exact_riemann_euler(rl, ul, pl, rr, ur, pr, g, xs, t, x0):
+    """Exact solution of the ideal-gas Riemann problem (Toro, ch. 4)."""
+    cl, cr = math.sqrt(g * pl / rl), math.sqrt(g * pr / rr)
+
+    def fk(p, rk, pk, ck):
+        if p > pk:
+            A, B = 2 / ((g + 1) * rk), (g - 1) / (g + 1) * pk
+            return (p - pk) * math.sqrt(A / (p + B))
+        return 2 * ck / (g - 1) * ((p / pk) ** ((g - 1) / (2 * g)) - 1)
+
+    lo, hi = 1e-6, 1e12
+    for _ in range(300):
+        mid = 0.5 * (lo + hi)
+        if fk(mid, rl, pl, cl) + fk(mid, rr, pr, cr) + (ur - ul) > 0:
+            hi = mid
+        else:
+            lo = mid
+    ps = 0.5 * (lo + hi)
+    us = 0.5 * (ul + ur) + 0.5 * (fk(ps, rr, pr, cr) - fk(ps, rl, pl, cl))
+    out = np.zeros_like(xs)
+    for i, x in enumerate(xs):
+        s = (x - x0) / t
+        if s < us:
+            if ps > pl:
+                rs = rl * ((ps / pl + (g - 1) / (g + 1)) / ((g - 1) / (g + 1) * ps / pl + 1))
+                sh = ul - cl * math.sqrt((g + 1) / (2 * g) * ps / pl + (g - 1) / (2 * g))
+                out[i] = rl if s < sh else rs
+            else:
+                rs = rl * (ps / pl) ** (1 / g)
+                csl = cl * (ps / pl) ** ((g - 1) / (2 * g))
+                if s < ul - cl:
+                    out[i] = rl
+                elif s > us - csl:
+                    out[i] = rs
+                else:
+                    out[i] = rl * (2 / (g + 1) + (g - 1) / ((g + 1) * cl) * (ul - s)) ** (2 / (g - 1))
+        else:
+            if ps > pr:
+                rs = rr * ((ps / pr + (g - 1) / (g + 1)) / ((g - 1) / (g + 1) * ps / pr + 1))
+                sh = ur + cr * math.sqrt((g + 1) / (2 * g) * ps / pr + (g - 1) / (2 * g))
+                out[i] = rr if s > sh else rs
+            else:
+                rs = rr * (ps / pr) ** (1 / g)
+                csr = cr * (ps / pr) ** ((g - 1) / (2 * g))
+                if s > ur + cr:
+                    out[i] = rr
+                elif s < us + csr:
+                    out[i] = rs
+                else:
+                    out[i] = rr * (2 / (g + 1) - (g - 1) / ((g + 1) * cr) * (ur - s)) ** (2 / (g - 1))
+    return out
+
+
+@pytest.mark.parametrize("order,tol", [(1, 0.035), (2, 0.015)])
+def test_euler_sod_converges_to_exact(order, tol):
+    n = 800
+    case = _euler_1d(order, ["vanleer"] * 4 if order == 2 else ["none"] * 4, 0, cells=n)
+    case["time_mode"] = "time"
+    x = (np.arange(n) + 0.5) / n
+    st = np.zeros((6, n))
+    st[0] = 1.0
+    st[1] = np.where(x <= 0.4, 100.0, 1.0)
+    st[2] = np.where(x <= 0.4, 1e7, 1e5)
+    out, it, t, _, _ = O.run(case, st, t_final=1.5e-4)
+    ex = exact_riemann_euler(100.0, 0.0, 1e7, 1.0, 0.0, 1e5, 1.4, x, t, 0.4)
+    err = np.abs(out[1] - ex).sum() / np.abs(ex).sum()
+    assert err < tol, err
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_abgrall_uniform_pressure_velocity(order):
+    """Interface advection at uniform p and u stays at uniform p and u."""
+    n = 200
+    case = _c1(False, cells=n)
+    case.update(order=order, limiters=["vanleer"] * 4 if order == 2 else ["none"] * 4,
+                time_mode="iterations", steps=100)
+    x = (np.arange(n) + 0.5) / n
+    air = (x > 0.3) & (x < 0.6)
+    st = np.zeros((9, n))
+    st[0] = np.where(air, 1 - 1e-6, 1e-6)
+    st[1] = 1.2
+    st[2] = 1e5
+    st[3] = 1 - st[0]
+    st[4] = 1000.0
+    st[5] = 1e5
+    st[6] = 100.0
+    out, it, t, _, _ = O.run(case, st)
+    pm = out[0] * out[2] + out[3] * out[5]
+    # the relaxation Newton stops at |sum alpha - 1| <= 1e-10, which leaves
+    # rho c^2 * 1e-10 ~ 0.3 Pa of pressure noise in water: ~3e-6 of 1e5 Pa
+    assert np.abs(pm / 1e5 - 1).max() < 3e-6
+    assert np.abs(out[6] / 100.0 - 1).max() < 1e-9
+    # the interface moved by u t
+    centre = (x * (1 - out[3])).sum() / (1 - out[3]).sum()
+    assert centre == pytest.approx(0.45 + 100.0 * t, abs=1.0 / n)
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_conservation_closed_problem(order):
+    """Waves stay inside the domain: per-phase mass, momentum and energy conserved to round-off."""
+    n = 400
+    case = _c1(False, t_final=4e-5, cells=n)
+    case.update(order=order, limiters=["vanleer"] * 4 if order == 2 else ["none"] * 4)
+    x = (np.arange(n) + 0.5) / n
+    hp = (x > 0.45) & (x < 0.55)
+    st = np.zeros((9, n))
+    st[0] = 1e-6
+    st[1] = 50.0
+    st[2] = np.where(hp, 1e9, 1e5)
+    st[3] = 1 - 1e-6
+    st[4] = 1000.0
+    st[5] = st[2]
+    out0, _, _, _, _ = O.run(case, st, nsteps=1, log_dt=False)  # gets E at t=0 via one tiny step
+    out, it, t, _, _ = O.run(case, st)
+    for k in range(2):
+        m0 = (st[3 * k] * st[3 * k + 1]).sum()
+        m1 = (out[3 * k] * out[3 * k + 1]).sum()
+        assert m1 == pytest.approx(m0, rel=1e-13)
+    rho = out[0] * out[1] + out[3] * out[4]
+    assert abs((rho * out[6]).sum()) < 1e-10 * (rho * np.abs(out[6])).sum()  # symmetric: zero net momentum
