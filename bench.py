@@ -1,0 +1,156 @@
+"""Benchmark of the explicit finite-volume step on one MI355X.
+
+Workload (default): the 2-D Kapila five-equation two-phase model, MUSCL with
+the van Leer limiter, FP64, 4096 x 4096 uniform Cartesian cells: a planar
+Mach 2.4 air shock hitting 32 water cylinders (radii uniform in [20, 80]
+cells, seed-placed without overlap), alpha floor 1e-6.  Synthetic initial
+state, built on the host and uploaded before timing; the timed region runs
+--steps full time steps (predictor + corrector kernels each) with the state
+resident in HBM.
+
+Prints one JSON line: cell-updates/s, the HBM roofline of the dominant kernel
+measured with HIP events over the timed region, and a CPU baseline (the FP64
+oracle, OpenMP on the host cores) on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "cell-updates/s & % HBM roofline; batched solves/s + p99 latency vs plain"
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=4096, help="cells per side")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-n", type=int, default=1024, help="cells per side of the CPU sample")
+    ap.add_argument("--cpu-steps", type=int, default=12)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run a plain copy kernel of known size (for HBM counter calibration)")
+    return ap.parse_args()
+
+
+def state_words(case):
+    nph = len(case["eos"])
+    dim = sum(1 for c in case["cells"] if c > 1)
+    return (3 * nph + dim + 1) if case["model"] != "euler" else dim + 2
+
+
+def copy_calibration(nbytes=1 << 30):
+    """Plain copy of nbytes through the library's streaming copy kernel."""
+    from ecogen_amd import lib
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    hip.hipMemset(a, 0, C.c_size_t(nbytes))
+    for _ in range(3):
+        assert lib().eco_copy_calibration(b, a, nbytes, None) == 0
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    return nbytes
+
+
+def cpu_baseline(args):
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle as O
+    from ecogen_amd import workloads as W
+    n = args.cpu_n
+    scale = n / 4096.0
+    case, st = W.shock_cylinders_2d(n=n, rmin=20.0 * scale, rmax=80.0 * scale, steps=args.cpu_steps)
+    threads = min(16, os.cpu_count() or 1)
+    t0 = time.perf_counter()
+    _, it, _, _, _ = O.run(case, st, nsteps=args.cpu_steps, nthreads=threads)
+    dt = time.perf_counter() - t0
+    return {"value": n * n * it / dt, "unit": "cell-updates/s", "cores": threads, "kind": "port",
+            "sample": f"same workload scaled to {n}x{n} cells (cylinder radii scaled), {it} steps, FP64 oracle "
+                      f"restatement of the reference step, OpenMP {threads} threads, {dt:.1f} s"}
+
+
+def traffic_from_profiles(workload_key):
+    p = ROOT / "profiles" / f"traffic_{workload_key}.json"
+    if p.exists():
+        try:
+            return json.loads(p.read_text()).get("bytes_per_launch")
+        except Exception:
+            return None
+    return None
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is designed (see the "
+                                   "design notes) but not built. Use --gpus 1."}))
+        sys.exit(2)
+    from ecogen_amd import Context
+    from ecogen_amd import workloads as W
+
+    case, st = W.shock_cylinders_2d(n=args.n, steps=args.steps)
+    ncell = int(np.prod(case["cells"]))
+    ctx = Context(case)
+    ctx.set_state(st)
+    del st
+    if args.warmup > 0:
+        ctx.run(nsteps=args.warmup)
+    ctx.sync()
+    t0 = time.perf_counter()
+    status, stage_ms, launches = ctx.time_steps(args.steps)
+    t1 = time.perf_counter()
+    wall = t1 - t0
+    if status.flags & 2:
+        raise SystemExit("non-finite state produced")
+    value = ncell * args.steps / wall
+
+    # roofline of the dominant kernel (algorithmic bytes: predictor reads U^n and
+    # writes U* = 2 S words per cell, corrector reads U^n, U* and writes U^n+1 = 3 S)
+    sw = state_words(case) * 8
+    per_cell = {1: 2 * sw, 2: 3 * sw, 0: 2 * sw}
+    names = {0: "first-order step", 1: "MUSCL predictor", 2: "MUSCL corrector"}
+    dom = max(range(3), key=lambda s: stage_ms[s])
+    avg_s = stage_ms[dom] / max(launches[dom], 1) / 1e3
+    alg_bytes = per_cell[dom] * ncell
+    achieved = alg_bytes / avg_s / 1e9
+    wkey = f"kapila2d_o2_vanleer_fp64_{args.n}"
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(wkey),
+            "kernel": names[dom], "kernel_ms": round(avg_s * 1e3, 4),
+            "alg_bytes_per_launch": alg_bytes,
+            "step_frac": round(ncell * 5 * sw * value / ncell / (HBM_PEAK_GBS * 1e9), 4)}
+    if args.calibrate_copy:
+        copy_calibration()
+    cpu = None if args.no_cpu else cpu_baseline(args)
+    line = {
+        "metric": METRIC, "value": round(value, 1), "unit": "cell-updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall / args.steps * 1e3, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"2-D Kapila 5-eq MUSCL vanLeer FP64 {args.n}x{args.n}, Mach 2.4 air shock on 32 "
+                               f"water cylinders", "cells": ncell, "model": "kapila", "order": 2,
+                   "limiter": "vanleer", "phases": 2, "parallelism": "single GPU",
+                   "t_end": status.t, "global_batch": 1, "seq_len": ncell},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

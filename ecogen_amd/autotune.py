@@ -1,0 +1,170 @@
+"""Tile-shape and occupancy autotuner for the stage kernels.
+
+Three steps, so that the compile happens on the development host and the
+timing on the GPU:
+
+    python -m ecogen_amd.autotune build            # compile every variant (hipcc, gfx950)
+    python -m ecogen_amd.autotune run  --out R.json # on the GPU: time each variant
+    python -m ecogen_amd.autotune apply R.json      # write the fastest into csrc/eco_tuning.h
+
+A variant is a set of build-time macros of ``csrc/eco_tuning.h`` (tile
+shape per dimension, minimum waves per SIMD requested from the compiler).
+Each is timed on the stage kernels of a fixed workload with HIP events
+(eco_time_steps), after a parity check of its result against the default
+build's result on the same state.
+"""
+from __future__ import annotations
+
+import argparse
+import itertools
+import json
+import os
+import re
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+from pathlib import Path
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parent
+TUNE_DIR = PKG / "_tune"
+CSRC = PKG / "csrc"
+
+SPACE = {
+    2: {"tiles": [(64, 4), (32, 8), (16, 16), (128, 2)], "waves": [1, 2, 3]},
+    3: {"tiles": [(16, 4, 4), (32, 4, 2), (8, 8, 4), (64, 2, 2)], "waves": [1, 2]},
+}
+
+
+def variants(dim):
+    out = []
+    sp = SPACE[dim]
+    for t, w in itertools.product(sp["tiles"], sp["waves"]):
+        if dim == 2:
+            name = f"d2_t{t[0]}x{t[1]}_w{w}"
+            defs = {"ECO_TILE2_X": t[0], "ECO_TILE2_Y": t[1], "ECO_MIN_WAVES_PER_EU": w}
+        else:
+            name = f"d3_t{t[0]}x{t[1]}x{t[2]}_w{w}"
+            defs = {"ECO_TILE3_X": t[0], "ECO_TILE3_Y": t[1], "ECO_TILE3_Z": t[2], "ECO_MIN_WAVES_PER_EU": w}
+        out.append((name, defs))
+    return out
+
+
+def _compile_variant(item):
+    from .build import COMMON, HIPCC, SOURCES, ARCH
+    name, defs = item
+    vdir = TUNE_DIR / name
+    vdir.mkdir(parents=True, exist_ok=True)
+    dflags = [f"-D{k}={v}" for k, v in defs.items()]
+    objs = []
+    for src in SOURCES:
+        obj = vdir / (Path(src).stem + ".o")
+        cmd = [HIPCC, *COMMON, *dflags, "-c", str(CSRC / src), "-o", str(obj)]
+        if src.endswith(".cpp"):
+            cmd[1:1] = ["-x", "hip"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            return name, f"compile failed: {r.stderr[-800:]}"
+        objs.append(str(obj))
+    lib = vdir / "libecogen_amd.so"
+    r = subprocess.run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", str(lib)],
+                       capture_output=True, text=True)
+    for o in objs:
+        os.remove(o)
+    return name, ("ok" if r.returncode == 0 else f"link failed: {r.stderr[-800:]}")
+
+
+def cmd_build(args):
+    items = [v for d in args.dims for v in variants(d)]
+    with ThreadPoolExecutor(max_workers=args.jobs) as ex:
+        for name, status in ex.map(_compile_variant, items):
+            print(f"{name}: {status}", flush=True)
+
+
+def _workload(dim, n):
+    from . import workloads as W
+    if dim == 2:
+        s = n / 4096.0
+        return W.shock_cylinders_2d(n=n, rmin=20.0 * s, rmax=80.0 * s)
+    s = n / 384.0
+    return W.bubbles_3d(n=n, rmin=max(1.5, 4.0 * s), rmax=24.0 * s)
+
+
+def _time_lib(lib_path, case, st, steps):
+    """Load one variant library in a fresh namespace and time its stage kernels."""
+    import importlib
+    from . import capi
+    capi._lib = None
+    capi.LIB_PATH = Path(lib_path)
+    importlib.reload  # keep the module object; only the handle changes
+    ctx = capi.Context(case)
+    ctx.set_state(st)
+    ctx.run(nsteps=2)
+    status, ms, cnt = ctx.time_steps(steps)
+    out = ctx.get_state()
+    ctx.close()
+    capi._lib = None
+    return float(ms.sum()) / steps, out
+
+
+def cmd_run(args):
+    results = []
+    for dim in args.dims:
+        case, st = _workload(dim, args.n2 if dim == 2 else args.n3)
+        base_ms, base = _time_lib(PKG / "libecogen_amd.so", case, st, args.steps)
+        print(f"dim {dim} default build: {base_ms:.3f} ms/step", flush=True)
+        for name, defs in variants(dim):
+            lib = TUNE_DIR / name / "libecogen_amd.so"
+            if not lib.exists():
+                continue
+            ms, out = _time_lib(lib, case, st, args.steps)
+            dev = float(np.abs(out - base).max())
+            results.append({"dim": dim, "variant": name, "defs": defs, "ms_per_step": ms, "max_abs_diff": dev})
+            print(f"  {name}: {ms:.3f} ms/step  (max |diff| vs default {dev:.2e})", flush=True)
+    Path(args.out).parent.mkdir(parents=True, exist_ok=True)
+    Path(args.out).write_text(json.dumps(results, indent=1))
+
+
+def cmd_apply(args):
+    res = json.loads(Path(args.results).read_text())
+    best = {}
+    for r in res:
+        if r["max_abs_diff"] != 0.0:
+            continue  # a tile shape must not change the result
+        if r["dim"] not in best or r["ms_per_step"] < best[r["dim"]]["ms_per_step"]:
+            best[r["dim"]] = r
+    hdr = (CSRC / "eco_tuning.h").read_text()
+    waves = None
+    for dim, r in sorted(best.items()):
+        for k, v in r["defs"].items():
+            if k == "ECO_MIN_WAVES_PER_EU":
+                waves = v if waves is None else min(waves, v)
+                continue
+            hdr = re.sub(rf"#define {k} \d+", f"#define {k} {v}", hdr)
+        print(f"dim {dim}: {r['variant']} {r['ms_per_step']:.3f} ms/step")
+    if waves is not None:
+        hdr = re.sub(r"#define ECO_MIN_WAVES_PER_EU \d+", f"#define ECO_MIN_WAVES_PER_EU {waves}", hdr)
+    (CSRC / "eco_tuning.h").write_text(hdr)
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    b = sub.add_parser("build")
+    b.add_argument("--dims", type=int, nargs="+", default=[2, 3])
+    b.add_argument("--jobs", type=int, default=min(8, os.cpu_count() or 1))
+    r = sub.add_parser("run")
+    r.add_argument("--dims", type=int, nargs="+", default=[2, 3])
+    r.add_argument("--n2", type=int, default=4096)
+    r.add_argument("--n3", type=int, default=256)
+    r.add_argument("--steps", type=int, default=10)
+    r.add_argument("--out", default="gpurun_out/autotune.json")
+    a = sub.add_parser("apply")
+    a.add_argument("results")
+    args = ap.parse_args(argv)
+    {"build": cmd_build, "run": cmd_run, "apply": cmd_apply}[args.cmd](args)
+
+
+if __name__ == "__main__":
+    main()

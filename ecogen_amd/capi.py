@@ -22,7 +22,7 @@ WARN_RELAX_NOT_CONVERGED = 1
 WARN_NAN = 2
 
 EXPORTS = ["eco_create", "eco_set_state", "eco_set_state_device", "eco_get_state", "eco_step", "eco_run",
-           "eco_launch_steps", "eco_sync", "eco_state_device", "eco_destroy", "eco_strerror", "eco_version"]
+           "eco_launch_steps", "eco_sync", "eco_time_steps", "eco_copy_calibration", "eco_state_device", "eco_destroy", "eco_strerror", "eco_version"]
 
 
 class EcoEos(C.Structure):
@@ -75,6 +75,10 @@ def lib():
         L.eco_launch_steps.restype = C.c_int
         L.eco_sync.argtypes = [vp, C.POINTER(EcoStatus)]
         L.eco_sync.restype = C.c_int
+        L.eco_time_steps.argtypes = [vp, C.c_int, dp, C.POINTER(C.c_int), C.POINTER(EcoStatus)]
+        L.eco_time_steps.restype = C.c_int
+        L.eco_copy_calibration.argtypes = [vp, vp, C.c_ulonglong, vp]
+        L.eco_copy_calibration.restype = C.c_int
         L.eco_state_device.argtypes = [vp, C.POINTER(C.c_int)]
         L.eco_state_device.restype = vp
         L.eco_destroy.argtypes = [vp]
@@ -186,6 +190,18 @@ class Context:
         st = EcoStatus()
         _check(lib().eco_sync(self._h, C.byref(st)), "eco_sync")
         return st
+
+    def time_steps(self, nsteps: int):
+        """Run nsteps steps with HIP events around each stage kernel.
+
+        Returns (status, stage_ms[3], launches[3]) for the first-order,
+        predictor and corrector kernels."""
+        ms = np.zeros(3)
+        cnt = (C.c_int * 3)()
+        st = EcoStatus()
+        _check(lib().eco_time_steps(self._h, int(nsteps), ms.ctypes.data_as(C.POINTER(C.c_double)), cnt,
+                                    C.byref(st)), "eco_time_steps")
+        return st, ms, list(cnt)
 
     def state_device(self):
         nf = C.c_int()

@@ -399,6 +399,67 @@ extern "C" int eco_run(eco_ctx *x, int nsteps, double t_final, eco_status *st) {
   return ECO_OK;
 }
 
+extern "C" int eco_time_steps(eco_ctx *x, int nsteps, double *stage_ms, int *launches, eco_status *st) {
+  if (!x || nsteps <= 0) return ECO_ERR_INVALID;
+  if (!x->have_state) return ECO_ERR_STATE;
+  int rc = set_target(x, x->step + nsteps, -1.);
+  if (rc) return rc;
+  const int nst = (x->cfg.order == 1) ? 1 : 2;
+  std::vector<hipEvent_t> ev((size_t)nsteps * nst * 2);
+  for (auto &e : ev)
+    if (hipEventCreate(&e) != hipSuccess) return ECO_ERR_HIP;
+  for (int i = 0; i < nsteps && !rc; i++) {
+    int k = x->step;
+    if (nst == 1) {
+      (void)hipEventRecord(ev[2 * i], x->stream);
+      rc = launch_stage(make_args(x, k), 0, x->stream);
+      (void)hipEventRecord(ev[2 * i + 1], x->stream);
+    } else {
+      StageArgs P = make_args(x, k);
+      P.wout = x->bstar;
+      (void)hipEventRecord(ev[4 * i], x->stream);
+      rc = launch_stage(P, 1, x->stream);
+      (void)hipEventRecord(ev[4 * i + 1], x->stream);
+      if (!rc) {
+        StageArgs Q = make_args(x, k);
+        Q.wsrc = x->bstar;
+        (void)hipEventRecord(ev[4 * i + 2], x->stream);
+        rc = launch_stage(Q, 2, x->stream);
+        (void)hipEventRecord(ev[4 * i + 3], x->stream);
+      }
+    }
+    if (rc) rc = rc < 0 ? ECO_ERR_UNSUPPORTED : ECO_ERR_HIP;
+    else x->step = k + 1;
+  }
+  if (!rc) rc = hipcode(hipStreamSynchronize(x->stream));
+  double ms[3] = {0., 0., 0.};
+  int cnt[3] = {0, 0, 0};
+  for (int i = 0; i < nsteps && !rc; i++) {
+    for (int s = 0; s < nst; s++) {
+      float t = 0.f;
+      if (hipEventElapsedTime(&t, ev[(size_t)(i * nst + s) * 2], ev[(size_t)(i * nst + s) * 2 + 1]) != hipSuccess) {
+        rc = ECO_ERR_HIP;
+        break;
+      }
+      int slot = (nst == 1) ? 0 : s + 1;
+      ms[slot] += t;
+      cnt[slot]++;
+    }
+  }
+  for (auto &e : ev) (void)hipEventDestroy(e);
+  if (rc) return rc;
+  if (stage_ms) for (int s = 0; s < 3; s++) stage_ms[s] = ms[s];
+  if (launches) for (int s = 0; s < 3; s++) launches[s] = cnt[s];
+  return eco_sync(x, st);
+}
+
+namespace eco { int launch_copy(void *dst, const void *src, unsigned long long nbytes, hipStream_t st); }
+
+extern "C" int eco_copy_calibration(void *dst, const void *src, unsigned long long nbytes, void *hip_stream) {
+  if (!dst || !src || (nbytes % 8)) return ECO_ERR_INVALID;
+  return launch_copy(dst, src, nbytes, (hipStream_t)hip_stream) ? ECO_ERR_HIP : ECO_OK;
+}
+
 extern "C" const double *eco_state_device(eco_ctx *x, int *nfields_device) {
   if (!x) return nullptr;
   if (nfields_device) *nfields_device = x->nprim + 1;

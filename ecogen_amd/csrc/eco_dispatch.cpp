@@ -2,6 +2,18 @@
 #include "eco_launch.h"
 
 namespace eco {
+__global__ void k_copy(double *__restrict__ d, const double *__restrict__ s, unsigned long long n) {
+  unsigned long long i = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x;
+  const unsigned long long stride = (unsigned long long)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) d[i] = s[i];
+}
+
+int launch_copy(void *dst, const void *src, unsigned long long nbytes, hipStream_t st) {
+  const unsigned long long n = nbytes / 8;
+  hipLaunchKernelGGL(k_copy, dim3(4096), dim3(256), 0, st, (double *)dst, (const double *)src, n);
+  return (int)hipGetLastError();
+}
+
 int launch_euler(const StageArgs &A, int stage, hipStream_t st);
 int launch_kapila(const StageArgs &A, int stage, hipStream_t st);
 int launch_multip(const StageArgs &A, int stage, hipStream_t st);

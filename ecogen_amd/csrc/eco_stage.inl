@@ -29,17 +29,21 @@ struct Layout {
   static constexpr int IE = NPRIM;                                                 // total energy field
   static constexpr int NVAR = NPRIM;                                               // slope variables
   static constexpr int NFV = (MODEL == MODEL_EULER) ? DIM + 3 : 3 * NPH + DIM + 2; // flux values + sM
+  static constexpr int NACC = NFV - 1;                                              // accumulated values
 };
 
+// Tile shapes and the occupancy request are build-time parameters chosen by
+// the autotuner (ecogen_amd/autotune.py writes eco_tuning.h).
+#include "eco_tuning.h"
 template <int DIM> struct Tile;
-template <> struct Tile<1> { static constexpr int X = 256, Y = 1, Z = 1; };
-template <> struct Tile<2> { static constexpr int X = 64, Y = 4, Z = 1; };
-template <> struct Tile<3> { static constexpr int X = 32, Y = 4, Z = 2; };
+template <> struct Tile<1> { static constexpr int X = ECO_TILE1_X, Y = 1, Z = 1; };
+template <> struct Tile<2> { static constexpr int X = ECO_TILE2_X, Y = ECO_TILE2_Y, Z = 1; };
+template <> struct Tile<3> { static constexpr int X = ECO_TILE3_X, Y = ECO_TILE3_Y, Z = ECO_TILE3_Z; };
 
 template <int DIM>
 struct TileFaces {
   static constexpr int T[3] = {Tile<DIM>::X, Tile<DIM>::Y, Tile<DIM>::Z};
-  static constexpr int nf(int d) { return (T[0] + (d == 0)) * (T[1] + (d == 1)) * (T[2] + (d == 2)); }
+  static constexpr int nf(int d) { return d < DIM ? (T[0] + (d == 0)) * (T[1] + (d == 1)) * (T[2] + (d == 2)) : 0; }
   static constexpr int MAXF = (nf(0) > nf(1) ? (nf(0) > nf(2) ? nf(0) : nf(2)) : (nf(1) > nf(2) ? nf(1) : nf(2)));
 };
 
@@ -102,41 +106,17 @@ __device__ __forceinline__ void prim_to_vec(const CellState<NPH> &s, double *w) 
   }
 }
 
-// CellO2::computeLocalSlopes limiter selection (CellO2.cpp:229-246) and
-// Phase*/Mix*::limitSlopes: alpha with the volume-fraction limiter, the rest
-// with the global/interface limiter.
 template <int MODEL, int NPH, int DIM>
-__device__ __forceinline__ void limit_vec(const SchemeParams &S, bool iface, const double *s1, const double *s2, double *out) {
-  using Ly = Layout<MODEL, NPH, DIM>;
-  const int lg = iface ? S.lim[1] : S.lim[0];
-  const int lv = iface ? S.lim[3] : S.lim[2];
-#pragma unroll
-  for (int v = 0; v < Ly::NVAR; v++) {
-    const bool isa = (MODEL != MODEL_EULER) && (v < 3 * NPH) && (v % 3 == 0);
-    out[v] = limiter(isa ? lv : lg, s1[v], s2[v]);
-  }
-}
-
-// Phase*::extrapolate + verifyAndCorrectPhase, Mix*::extrapolate
-// (CellInterfaceO2.cpp:219-226 / 250-257).  sign = -1 for the right state.
-template <int MODEL, int NPH, int DIM>
-__device__ __forceinline__ void extrapolate(const SchemeParams &S, CellState<NPH> &s, const double *sl, double dist, double sign) {
+__device__ __forceinline__ void vec_to_prim(const double *w, CellState<NPH> &s) {
   if constexpr (MODEL == MODEL_EULER) {
-    s.r[0] += (sign * sl[0]) * dist;
-    s.p[0] += (sign * sl[1]) * dist;
+    s.r[0] = w[0]; s.p[0] = w[1];
 #pragma unroll
-    for (int d = 0; d < DIM; d++) s.u[d] = s.u[d] + (sign * sl[2 + d]) * dist;
-    verify_correct<MODEL, NPH>(S, s, 0);
+    for (int d = 0; d < DIM; d++) s.u[d] = w[2 + d];
   } else {
 #pragma unroll
-    for (int k = 0; k < NPH; k++) {
-      s.a[k] += (sign * sl[3 * k]) * dist;
-      s.r[k] += (sign * sl[3 * k + 1]) * dist;
-      s.p[k] += (sign * sl[3 * k + 2]) * dist;
-      verify_correct<MODEL, NPH>(S, s, k);
-    }
+    for (int k = 0; k < NPH; k++) { s.a[k] = w[3 * k]; s.r[k] = w[3 * k + 1]; s.p[k] = w[3 * k + 2]; }
 #pragma unroll
-    for (int d = 0; d < DIM; d++) s.u[d] = s.u[d] + (sign * sl[3 * NPH + d]) * dist;
+    for (int d = 0; d < DIM; d++) s.u[d] = w[3 * NPH + d];
   }
 }
 
@@ -246,6 +226,45 @@ __device__ __forceinline__ void zero_flux(FluxVec<NPH> &U) {
   U.sM = 0.;
 }
 
+// per-thread accumulator parked in LDS between directions
+template <int MODEL, int NPH, int DIM, int NT>
+__device__ __forceinline__ void park_acc(double (*A)[NT], int t, const FluxVec<NPH> &U) {
+  int q = 0;
+  if constexpr (MODEL == MODEL_EULER) {
+    A[q++][t] = U.m[0];
+  } else {
+#pragma unroll
+    for (int k = 0; k < NPH; k++) { A[q++][t] = U.a[k]; A[q++][t] = U.m[k]; A[q++][t] = U.en[k]; }
+  }
+#pragma unroll
+  for (int d = 0; d < DIM; d++) A[q++][t] = U.q[d];
+  A[q++][t] = U.E;
+}
+
+template <int MODEL, int NPH, int DIM, int NT>
+__device__ __forceinline__ void unpark_acc(double (*A)[NT], int t, FluxVec<NPH> &U) {
+  int q = 0;
+  zero_flux(U);
+  if constexpr (MODEL == MODEL_EULER) {
+    U.m[0] = A[q++][t];
+  } else {
+#pragma unroll
+    for (int k = 0; k < NPH; k++) { U.a[k] = A[q++][t]; U.m[k] = A[q++][t]; U.en[k] = A[q++][t]; }
+  }
+#pragma unroll
+  for (int d = 0; d < DIM; d++) U.q[d] = A[q++][t];
+  U.E = A[q++][t];
+}
+
+// time-n alpha_k, p_k of one cell: the non-conservative terms' coefficients
+template <int MODEL, int NPH, int DIM>
+__device__ __forceinline__ void load_nc_coeffs(const double *__restrict__ W, long long nc, long long idx, CellState<NPH> &s) {
+  if constexpr (MODEL != MODEL_EULER) {
+#pragma unroll
+    for (int k = 0; k < NPH; k++) { s.a[k] = W[(3 * k) * nc + idx]; s.p[k] = W[(3 * k + 2) * nc + idx]; }
+  }
+}
+
 __device__ __forceinline__ double ctl_dt(const Control *ctl, const Geometry &G, double cfl, int step) {
   if (step == 0) return 1.e-15;
   unsigned long long bits = __atomic_load_n(&ctl->smax[(step - 1) % 3], __ATOMIC_RELAXED);
@@ -259,7 +278,7 @@ __device__ __forceinline__ double ctl_dt(const Control *ctl, const Geometry &G, 
 // STAGE 0: first-order step; 1: MUSCL predictor (writes U*); 2: MUSCL corrector
 // ---------------------------------------------------------------------------
 template <int MODEL, int NPH, int DIM, int STAGE, bool SAMELIM>
-__global__ void __launch_bounds__(Tile<DIM>::X * Tile<DIM>::Y * Tile<DIM>::Z)
+__global__ void __launch_bounds__(Tile<DIM>::X * Tile<DIM>::Y * Tile<DIM>::Z, ECO_MIN_WAVES_PER_EU)
 k_stage(const StageArgs A) {
   using Ly = Layout<MODEL, NPH, DIM>;
   using TF = TileFaces<DIM>;
@@ -268,6 +287,7 @@ k_stage(const StageArgs A) {
   constexpr int MAXF = TF::MAXF;
   constexpr bool O2 = (STAGE != 0);
   __shared__ double sF[Ly::NFV][MAXF];
+  __shared__ double sAcc[Ly::NACC][NT];
   __shared__ double sRed[NT / 64];
 
   const Geometry &G = A.geo;
@@ -307,11 +327,8 @@ k_stage(const StageArgs A) {
   const double *__restrict__ Wsrc = A.wsrc;  // state the fluxes are built from (U^n or U*)
   const double *__restrict__ Wn = A.wn;      // time-n state
 
-  // own time-n state (non-conservative terms, buildCons)
-  CellState<NPH> own;
-  load_prim<MODEL, NPH, DIM>(Wn, nc, cidx, own);
-  own.E = Wn[Ly::IE * nc + cidx];
-
+  // the accumulator stays in registers only during the accumulation steps and
+  // is parked in LDS while the block computes the next direction's faces
   FluxVec<NPH> acc;
   zero_flux(acc);
   double smx = 0.;
@@ -349,44 +366,19 @@ k_stage(const StageArgs A) {
           }
         }
       } else {
-        // MUSCL: slopes of both cells from the faces i-1/2, i+1/2, i+3/2
-        double wlo[Ly::NVAR], whi[Ly::NVAR], wll[Ly::NVAR], whh[Ly::NVAR];
-        double sc[Ly::NVAR], sm[Ly::NVAR], sp[Ly::NVAR];
-        prim_to_vec<MODEL, NPH, DIM>(L, wlo);
-        prim_to_vec<MODEL, NPH, DIM>(R, whi);
-        const double dc = G.dcc[dir][i];
-#pragma unroll
-        for (int v = 0; v < Ly::NVAR; v++) sc[v] = (whi[v] - wlo[v]) / dc;
+        // MUSCL: limited slopes of both cells from the face slopes at
+        // i-1/2, i+1/2, i+3/2 (CellInterfaceO2::computeSlopes + CellO2::
+        // computeLocalSlopes), then extrapolation to the face, one variable at a time.
         const bool has_m = (i >= 1), has_p = (i + 2 <= ndir - 1);
-        CellState<NPH> T;
-        if (has_m) {
-          load_prim<MODEL, NPH, DIM>(Wsrc, nc, ilo - strd[dir], T);
-          prim_to_vec<MODEL, NPH, DIM>(T, wll);
-          const double dm = G.dcc[dir][i - 1];
-#pragma unroll
-          for (int v = 0; v < Ly::NVAR; v++) sm[v] = (wlo[v] - wll[v]) / dm;
-        } else {
-#pragma unroll
-          for (int v = 0; v < Ly::NVAR; v++) sm[v] = 0.;
-        }
-        if (has_p) {
-          load_prim<MODEL, NPH, DIM>(Wsrc, nc, ihi + strd[dir], T);
-          prim_to_vec<MODEL, NPH, DIM>(T, whh);
-          const double dp = G.dcc[dir][i + 1];
-#pragma unroll
-          for (int v = 0; v < Ly::NVAR; v++) sp[v] = (whh[v] - whi[v]) / dp;
-        } else {
-#pragma unroll
-          for (int v = 0; v < Ly::NVAR; v++) sp[v] = 0.;
-        }
+        const long long sd = strd[dir];
         // interface detection: alpha of the face's cells from the stage state,
         // alpha across the opposite face from the time-n state (CellO2.cpp:144-150)
         double aL = 0., aR = 0., aLL = 0., aRR = 0.;
         if constexpr (MODEL != MODEL_EULER) {
           aL = L.a[0]; aR = R.a[0];
           if (!SAMELIM || S.thinc) {
-            aLL = has_m ? Wn[ilo - strd[dir]] : 0.;
-            aRR = has_p ? Wn[ihi + strd[dir]] : 0.;
+            aLL = has_m ? Wn[ilo - sd] : 0.;
+            aRR = has_p ? Wn[ihi + sd] : 0.;
           }
         }
         bool ifl = false, ifr = false;
@@ -394,13 +386,46 @@ k_stage(const StageArgs A) {
           ifl = (aL >= 1.e-4) && (aL <= 1. - 1.e-4) && ((aR - aL) * (aL - aLL) >= 1.e-8);
           ifr = (aR >= 1.e-4) && (aR <= 1. - 1.e-4) && ((aL - aR) * (aR - aRR) >= 1.e-8);
         }
-        double sl[Ly::NVAR], sr[Ly::NVAR];
-        limit_vec<MODEL, NPH, DIM>(S, ifl, sc, sm, sl);
-        limit_vec<MODEL, NPH, DIM>(S, ifr, sc, sp, sr);
-        extrapolate<MODEL, NPH, DIM>(S, L, sl, G.dfl[dir][i], 1.);
-        if (S.thinc) thinc<NPH>(L, aL, aR, aLL, false);
-        extrapolate<MODEL, NPH, DIM>(S, R, sr, G.dfr[dir][i], -1.);
-        if (S.thinc) thinc<NPH>(R, aR, aRR, aL, true);
+        const double dcc = G.dcc[dir][i];
+        const double dcm = has_m ? G.dcc[dir][i - 1] : 1.;
+        const double dcp = has_p ? G.dcc[dir][i + 1] : 1.;
+        const double dL = G.dfl[dir][i], dR = G.dfr[dir][i];
+        double wl_[Ly::NVAR], wr_[Ly::NVAR], wll[Ly::NVAR], whh[Ly::NVAR];
+        prim_to_vec<MODEL, NPH, DIM>(L, wl_);
+        prim_to_vec<MODEL, NPH, DIM>(R, wr_);
+        // all neighbour loads issued before any arithmetic
+#pragma unroll
+        for (int v = 0; v < Ly::NVAR; v++) {
+          const double *Fv = Wsrc + (long long)v * nc;
+          wll[v] = has_m ? Fv[ilo - sd] : 0.;
+          whh[v] = has_p ? Fv[ihi + sd] : 0.;
+        }
+        double xl[Ly::NVAR], xr[Ly::NVAR];
+#pragma unroll
+        for (int v = 0; v < Ly::NVAR; v++) {
+          const double wl = wl_[v], wr = wr_[v];
+          const double sc = (wr - wl) / dcc;
+          const double sm = has_m ? (wl - wll[v]) / dcm : 0.;
+          const double sp = has_p ? (whh[v] - wr) / dcp : 0.;
+          const bool isa = (MODEL != MODEL_EULER) && (v < 3 * NPH) && (v % 3 == 0);
+          const int llim = isa ? (ifl ? S.lim[3] : S.lim[2]) : (ifl ? S.lim[1] : S.lim[0]);
+          const int rlim = isa ? (ifr ? S.lim[3] : S.lim[2]) : (ifr ? S.lim[1] : S.lim[0]);
+          xl[v] = wl + limiter(llim, sc, sm) * dL;
+          xr[v] = wr + (-limiter(rlim, sc, sp)) * dR;
+        }
+        vec_to_prim<MODEL, NPH, DIM>(xl, L);
+        vec_to_prim<MODEL, NPH, DIM>(xr, R);
+        if constexpr (MODEL == MODEL_EULER) {
+          verify_correct<MODEL, NPH>(S, L, 0);
+          verify_correct<MODEL, NPH>(S, R, 0);
+        } else {
+#pragma unroll
+          for (int k = 0; k < NPH; k++) { verify_correct<MODEL, NPH>(S, L, k); verify_correct<MODEL, NPH>(S, R, k); }
+        }
+        if (S.thinc) {
+          thinc<NPH>(L, aL, aR, aLL, false);
+          thinc<NPH>(R, aR, aRR, aL, true);
+        }
       }
       to_face_frame(dir, 0, L.u);
       to_face_frame(dir, 0, R.u);
@@ -421,6 +446,9 @@ k_stage(const StageArgs A) {
       const int flo = fl[0] + T0_ * (fl[1] + T1_ * fl[2]);
       fl[dir] += 1;
       const int fhi = fl[0] + T0_ * (fl[1] + T1_ * fl[2]);
+      if (dir > 0) unpark_acc<MODEL, NPH, DIM, NT>(sAcc, tid, acc);
+      CellState<NPH> own;
+      load_nc_coeffs<MODEL, NPH, DIM>(Wn, nc, cidx, own);
       FluxVec<NPH> F;
       if (ci[dir] >= 1) {
         get_flux<MODEL, NPH, DIM, MAXF>(sF, flo, F);
@@ -430,11 +458,16 @@ k_stage(const StageArgs A) {
         get_flux<MODEL, NPH, DIM, MAXF>(sF, fhi, F);
         add_face<MODEL, NPH>(acc, F, G.coefA[dir], false, own);
       }
+      if (dir < DIM - 1) park_acc<MODEL, NPH, DIM, NT>(sAcc, tid, acc);
     }
     __syncthreads();
   }
 
   if (active) {
+    // own time-n state (non-conservative terms of the boundary faces, buildCons)
+    CellState<NPH> own;
+    load_prim<MODEL, NPH, DIM>(Wn, nc, cidx, own);
+    own.E = Wn[Ly::IE * nc + cidx];
     // transmissive boundary faces (BoundCond::computeFlux + BoundCondAbs, BoundCond.cpp:68-100)
 #pragma unroll
     for (int dir = 0; dir < DIM; dir++) {

@@ -1,0 +1,24 @@
+// Build-time kernel shape parameters (overridable with -D).  Chosen by the
+// tile/occupancy sweep of ecogen_amd/autotune.py on MI355X.
+#pragma once
+#ifndef ECO_TILE1_X
+#define ECO_TILE1_X 256
+#endif
+#ifndef ECO_TILE2_X
+#define ECO_TILE2_X 32
+#endif
+#ifndef ECO_TILE2_Y
+#define ECO_TILE2_Y 8
+#endif
+#ifndef ECO_TILE3_X
+#define ECO_TILE3_X 16
+#endif
+#ifndef ECO_TILE3_Y
+#define ECO_TILE3_Y 4
+#endif
+#ifndef ECO_TILE3_Z
+#define ECO_TILE3_Z 4
+#endif
+#ifndef ECO_MIN_WAVES_PER_EU
+#define ECO_MIN_WAVES_PER_EU 1
+#endif

@@ -1,0 +1,168 @@
+"""Synthetic initial states of the benchmark configurations.
+
+Initial conditions follow the reference's geometric-domain semantics
+(GeometricalDomain::fillIn, src/Geometries/GeometricalDomain.cpp:70-88: later
+domains override earlier ones; GDHalfSpace::belong ``x <= origin`` for
+direction negative, GDHalfSpace.cpp:74-96; GDDisc::belong / GDSphere::belong
+``sum of squared offsets <= r*r``, GDDisc.cpp:85-107, GDSphere.cpp:71-79) on
+the reference cell centres ``(i + 0.5) * L / n``.  Placement of cylinders and
+bubbles is seeded (splitmix64) and non-overlapping.
+
+State layout: ecogen.h host layout, field-major (alpha, rho, p) per phase,
+then u, v, w (E is computed on the device).
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+AIR = {"type": "IG", "name": "air", "gamma": 1.4, "pinf": 0.0, "cv": 800.0, "eref": 0.0}
+WATER = {"type": "SG", "name": "water", "gamma": 4.4, "pinf": 6.0e8, "cv": 1000.0, "eref": 0.0}
+ALPHA_FLOOR = 1e-6
+
+
+class SplitMix64:
+    def __init__(self, seed: int):
+        self.s = seed & 0xFFFFFFFFFFFFFFFF
+
+    def next(self) -> int:
+        self.s = (self.s + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+        z = self.s
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
+        return z ^ (z >> 31)
+
+    def uniform(self) -> float:
+        return (self.next() >> 11) * (1.0 / 9007199254740992.0)
+
+
+def place(seed, count, rmin_cells, rmax_cells, box, dx, log_radius=False, tries=200000):
+    """Non-overlapping discs/spheres: centres uniform inside ``box`` (list of (lo, hi)) keeping
+    the body inside, radii uniform (or log-uniform) in cells. Returns [(centre, radius_m)]."""
+    rng = SplitMix64(seed)
+    out = []
+    for _ in range(tries):
+        if len(out) == count:
+            break
+        u = rng.uniform()
+        if log_radius:
+            r = math.exp(math.log(rmin_cells) + u * (math.log(rmax_cells) - math.log(rmin_cells))) * dx
+        else:
+            r = (rmin_cells + u * (rmax_cells - rmin_cells)) * dx
+        c = [lo + r + rng.uniform() * (hi - lo - 2 * r) for lo, hi in box]
+        if all(sum((a - b) ** 2 for a, b in zip(c, q)) > (r + rq + dx) ** 2 for q, rq in out):
+            out.append((c, r))
+    if len(out) < count:
+        raise RuntimeError(f"could only place {len(out)} of {count} bodies")
+    return out
+
+
+def mach_shock_air(mach, p1=1e5, rho1=1.2, gamma=1.4):
+    """Post-shock state behind a normal shock running into air at rest (Rankine-Hugoniot)."""
+    m2 = mach * mach
+    p2 = p1 * (1.0 + 2.0 * gamma / (gamma + 1.0) * (m2 - 1.0))
+    rho2 = rho1 * (gamma + 1.0) * m2 / ((gamma - 1.0) * m2 + 2.0)
+    c1 = math.sqrt(gamma * p1 / rho1)
+    return p2, rho2, mach * c1 * (1.0 - rho1 / rho2)
+
+
+def _centres(n, L):
+    return [(np.arange(n[d]) + 0.5) * (L[d] / n[d]) for d in range(3)]
+
+
+def _vals(st, nph=2):
+    out = []
+    for k in range(nph):
+        out += [st["alpha"][k], st["rho"][k], st["p"][k]]
+    return out + list(st["u"])
+
+
+def _fill(state, mask_idx, st, nph=2):
+    for k in range(nph):
+        state[3 * k][mask_idx] = st["alpha"][k]
+        state[3 * k + 1][mask_idx] = st["rho"][k]
+        state[3 * k + 2][mask_idx] = st["p"][k]
+    for d, u in enumerate(st["u"]):
+        state[3 * nph + d][mask_idx] = u
+
+
+def shock_cylinders_2d(n=4096, ncyl=32, seed=0xEC0C0002, rmin=20.0, rmax=80.0, limiter="vanleer", steps=1000):
+    """Config C2: a planar Mach 2.4 air shock (x < 0.05) hits ``ncyl`` water cylinders."""
+    case = {
+        "model": "kapila", "alpha_null": False, "eos": [AIR, WATER],
+        "cells": [n, n, 1], "lengths": [1.0, 1.0, 1.0], "order": 2, "limiters": [limiter] * 4,
+        "cfl": 0.8, "time_mode": "iterations", "steps": steps,
+    }
+    L = case["lengths"]
+    xc, yc, _ = _centres(case["cells"], L)
+    ncell = n * n
+    st = np.empty((9, ncell))
+    p2, rho2, u2 = mach_shock_air(2.4)
+    air = {"alpha": [1 - ALPHA_FLOOR, ALPHA_FLOOR], "rho": [1.2, 1000.0], "p": [1e5, 1e5], "u": [0.0, 0.0, 0.0]}
+    shocked = {"alpha": [1 - ALPHA_FLOOR, ALPHA_FLOOR], "rho": [rho2, 1000.0], "p": [p2, p2], "u": [u2, 0.0, 0.0]}
+    water = {"alpha": [ALPHA_FLOOR, 1 - ALPHA_FLOOR], "rho": [1.2, 1000.0], "p": [1e5, 1e5], "u": [0.0, 0.0, 0.0]}
+    _fill(st, slice(None), air)
+    view = st.reshape(9, n, n)  # [field, j, i]
+    sx = xc <= 0.05
+    for f, v in enumerate(_vals(shocked)):
+        view[f][:, sx] = v
+    dx = L[0] / n
+    bodies = place(seed, ncyl, rmin, rmax, [(0.1, 0.95), (0.0, 1.0)], dx)
+    for (cx, cy), r in bodies:
+        i0, i1 = max(0, int((cx - r) / dx) - 1), min(n, int((cx + r) / dx) + 2)
+        j0, j1 = max(0, int((cy - r) / dx) - 1), min(n, int((cy + r) / dx) + 2)
+        X = xc[i0:i1][None, :]
+        Y = yc[j0:j1][:, None]
+        inside = (X - cx) ** 2 + (Y - cy) ** 2 <= r * r
+        for f, v in enumerate(_vals(water)):
+            blk = view[f][j0:j1, i0:i1]
+            blk[inside] = v
+    return case, st
+
+
+def bubbles_3d(n=384, nb=64, seed=0xEC0C0003, rmin=4.0, rmax=24.0, limiter="minmod", steps=500, precision="fp64"):
+    """Config C3: ``nb`` air bubbles in water at 5e7 Pa, pressure-disequilibrium model with relaxation."""
+    case = {
+        "model": "multip", "alpha_null": False, "eos": [AIR, WATER], "relax_p": True,
+        "cells": [n, n, n], "lengths": [1.0, 1.0, 1.0], "order": 2, "limiters": [limiter] * 4,
+        "cfl": 0.8, "time_mode": "iterations", "steps": steps, "precision": precision,
+    }
+    L = case["lengths"]
+    xc, yc, zc = _centres(case["cells"], L)
+    st = np.empty((9, n * n * n))
+    water = {"alpha": [ALPHA_FLOOR, 1 - ALPHA_FLOOR], "rho": [1.2, 1000.0], "p": [5e7, 5e7], "u": [0.0, 0.0, 0.0]}
+    bubble = [1 - ALPHA_FLOOR, 1.2, 1e5, ALPHA_FLOOR, 1000.0, 1e5, 0.0, 0.0, 0.0]
+    _fill(st, slice(None), water)
+    view = st.reshape(9, n, n, n)  # [field, k, j, i]
+    dx = L[0] / n
+    for (cx, cy, cz), r in place(seed, nb, rmin, rmax, [(0.0, 1.0)] * 3, dx, log_radius=True):
+        lo = [max(0, int((c - r) / dx) - 1) for c in (cx, cy, cz)]
+        hi = [min(n, int((c + r) / dx) + 2) for c in (cx, cy, cz)]
+        X = xc[lo[0]:hi[0]][None, None, :]
+        Y = yc[lo[1]:hi[1]][None, :, None]
+        Z = zc[lo[2]:hi[2]][:, None, None]
+        inside = (X - cx) ** 2 + (Y - cy) ** 2 + (Z - cz) ** 2 <= r * r
+        for f, v in enumerate(bubble):
+            blk = view[f][lo[2]:hi[2], lo[1]:hi[1], lo[0]:hi[0]]
+            blk[inside] = v
+    return case, st
+
+
+def shock_tube_1d(n=1000):
+    """Config C1: water (x <= 0.7, 1e9 Pa) / air (1e5 Pa) shock tube, first order, to 240 us."""
+    case = {
+        "model": "kapila", "alpha_null": False, "eos": [AIR, WATER],
+        "cells": [n, 1, 1], "lengths": [1.0, 0.1, 0.1], "order": 1, "limiters": ["none"] * 4,
+        "cfl": 0.8, "time_mode": "time", "t_final": 240e-6,
+    }
+    x = (np.arange(n) + 0.5) * (1.0 / n)
+    st = np.zeros((9, n))
+    hp = x <= 0.7
+    st[0] = np.where(hp, ALPHA_FLOOR, 1 - ALPHA_FLOOR)
+    st[1] = 50.0
+    st[2] = np.where(hp, 1e9, 1e5)
+    st[3] = 1 - st[0]
+    st[4] = 1000.0
+    st[5] = st[2]
+    return case, st

@@ -114,6 +114,17 @@ int eco_launch_steps(eco_ctx *ctx, int nsteps);
 /* Waits for the context stream and fills status. */
 int eco_sync(eco_ctx *ctx, eco_status *status);
 
+/* Measurement form of eco_run(nsteps > 0): the same launches, each stage
+   kernel bracketed by HIP events on the context stream.  stage_ms[3] receives
+   the summed kernel time of the first-order / predictor / corrector kernels,
+   launches[3] the number of launches of each.  Synchronises. */
+int eco_time_steps(eco_ctx *ctx, int nsteps, double *stage_ms, int *launches, eco_status *status);
+
+/* Measurement utility (no reference counterpart): a plain device copy of
+   nbytes (multiple of 8) with a streaming kernel, used to calibrate HBM byte
+   counters against a kernel of known traffic. */
+int eco_copy_calibration(void *dst, const void *src, unsigned long long nbytes, void *hip_stream);
+
 /* Device pointer of the current state (device layout: per phase alpha, rho, p,
    then the nondegenerate velocity components, then E) for zero-copy access. */
 const double *eco_state_device(eco_ctx *ctx, int *nfields_device);

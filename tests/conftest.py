@@ -36,6 +36,37 @@ def rel_l1(a, b):
     return float(np.abs(a - b).sum() / den)
 
 
+def parity_quantities(case, state):
+    """The quantities the FP64 parity bar is stated on: mixture density, velocity,
+    mixture pressure and per-phase volume fraction (dict name -> array)."""
+    nph = len(case["eos"])
+    dim = sum(1 for c in case["cells"] if c > 1)
+    out = {}
+    if case["model"] == "euler":
+        out["rho"] = state[1]
+        out["p"] = state[2]
+    else:
+        out["rho"] = sum(state[3 * k] * state[3 * k + 1] for k in range(nph))
+        out["p"] = sum(state[3 * k] * state[3 * k + 2] for k in range(nph))
+        for k in range(nph):
+            out[f"alpha{k}"] = state[3 * k]
+    for d in range(dim):
+        out["uvw"[d]] = state[3 * nph + d]
+    return out
+
+
+def phase_quantities(case, state):
+    """Per-phase density and pressure (multiphase models)."""
+    nph = len(case["eos"])
+    if case["model"] == "euler":
+        return {}
+    out = {}
+    for k in range(nph):
+        out[f"rho{k}"] = state[3 * k + 1]
+        out[f"p{k}"] = state[3 * k + 2]
+    return out
+
+
 def compared_fields(case):
     """(field index in the ecogen.h layout, name) pairs compared at parity."""
     nph = len(case["eos"])
