@@ -40,6 +40,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-n", type=int, default=1024, help="cells per side of the CPU sample")
     ap.add_argument("--cpu-steps", type=int, default=12)
+    ap.add_argument("--no-fp32s", action="store_true", help="skip the FP32-storage measurement")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain copy kernel of known size (for HBM counter calibration)")
     return ap.parse_args()
@@ -93,63 +94,102 @@ def traffic_from_profiles(workload_key):
     return None
 
 
-def main():
-    args = parse()
-    if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is designed (see the "
-                                   "design notes) but not built. Use --gpus 1."}))
-        sys.exit(2)
+def run_precision(case, st, args, precision):
+    """Upload, warm up and time ``args.steps`` steps; returns the measurements."""
     from ecogen_amd import Context
-    from ecogen_amd import workloads as W
-
-    case, st = W.shock_cylinders_2d(n=args.n, steps=args.steps)
-    ncell = int(np.prod(case["cells"]))
-    ctx = Context(case)
+    c = dict(case, precision=precision)
+    ncell = int(np.prod(c["cells"]))
+    ctx = Context(c)
     ctx.set_state(st)
-    del st
     if args.warmup > 0:
         ctx.run(nsteps=args.warmup)
     ctx.sync()
     t0 = time.perf_counter()
     status, stage_ms, launches = ctx.time_steps(args.steps)
     t1 = time.perf_counter()
-    wall = t1 - t0
     if status.flags & 2:
-        raise SystemExit("non-finite state produced")
-    value = ncell * args.steps / wall
+        raise SystemExit(f"non-finite state produced ({precision})")
+    out = ctx.get_state()
+    ctx.close()
+    wall = t1 - t0
+    return {"value": ncell * args.steps / wall, "wall": wall, "stage_ms": stage_ms, "launches": launches,
+            "status": status, "state": out}
 
-    # roofline of the dominant kernel (algorithmic bytes: predictor reads U^n and
-    # writes U* = 2 S words per cell, corrector reads U^n, U* and writes U^n+1 = 3 S)
-    sw = state_words(case) * 8
-    per_cell = {1: 2 * sw, 2: 3 * sw, 0: 2 * sw}
+
+def roofline(case, r, word_bytes):
+    """Dominant stage kernel: algorithmic bytes (predictor reads U^n, writes U*:
+    2 S words per cell; corrector reads U^n and U*, writes U^n+1: 3 S words)
+    over its mean HIP-event duration in the timed region."""
+    ncell = int(np.prod(case["cells"]))
+    sw = state_words(case) * word_bytes
+    per_cell = {0: 2 * sw, 1: 2 * sw, 2: 3 * sw}
     names = {0: "first-order step", 1: "MUSCL predictor", 2: "MUSCL corrector"}
-    dom = max(range(3), key=lambda s: stage_ms[s])
-    avg_s = stage_ms[dom] / max(launches[dom], 1) / 1e3
-    alg_bytes = per_cell[dom] * ncell
-    achieved = alg_bytes / avg_s / 1e9
-    wkey = f"kapila2d_o2_vanleer_fp64_{args.n}"
-    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(wkey),
-            "kernel": names[dom], "kernel_ms": round(avg_s * 1e3, 4),
-            "alg_bytes_per_launch": alg_bytes,
-            "step_frac": round(ncell * 5 * sw * value / ncell / (HBM_PEAK_GBS * 1e9), 4)}
+    ms, cnt = r["stage_ms"], r["launches"]
+    dom = max(range(3), key=lambda s: ms[s])
+    avg_s = ms[dom] / max(cnt[dom], 1) / 1e3
+    alg = per_cell[dom] * ncell
+    ach = alg / avg_s / 1e9
+    return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 4), "kernel": names[dom], "kernel_ms": round(avg_s * 1e3, 4),
+            "alg_bytes_per_launch": alg,
+            "step_frac": round(5 * sw * r["value"] / (HBM_PEAK_GBS * 1e9), 4)}
+
+
+def l1_errors(case, a, b):
+    nph = len(case["eos"])
+    dim = sum(1 for c in case["cells"] if c > 1)
+    q = {}
+    for name, fa, fb in (("rho", sum(a[3 * k] * a[3 * k + 1] for k in range(nph)),
+                                 sum(b[3 * k] * b[3 * k + 1] for k in range(nph))),
+                         ("p", sum(a[3 * k] * a[3 * k + 2] for k in range(nph)),
+                               sum(b[3 * k] * b[3 * k + 2] for k in range(nph))),
+                         ("velocity", np.concatenate([a[3 * nph + d] for d in range(dim)]),
+                                      np.concatenate([b[3 * nph + d] for d in range(dim)]))):
+        q[name] = float(np.abs(fa - fb).sum() / np.abs(fb).sum())
+    for k in range(nph):
+        q[f"alpha{k}"] = float(np.abs(a[3 * k] - b[3 * k]).sum() / np.abs(b[3 * k]).sum())
+    return q
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is designed (see the "
+                                   "design notes) but not built. Use --gpus 1."}))
+        sys.exit(2)
+    from ecogen_amd import workloads as W
+
+    case, st = W.shock_cylinders_2d(n=args.n, steps=args.steps)
+    ncell = int(np.prod(case["cells"]))
+    r64 = run_precision(case, st, args, "fp64")
+    roof = roofline(case, r64, 8)
+    roof["traffic"] = traffic_from_profiles(f"kapila2d_o2_vanleer_fp64_{args.n}")
+    fp32s = None
+    if not args.no_fp32s:
+        r32 = run_precision(case, st, args, "fp32s")
+        fp32s = {"value": round(r32["value"], 1), "speedup_vs_fp64": round(r32["value"] / r64["value"], 3),
+                 "ms_per_step": round(r32["wall"] / args.steps * 1e3, 4),
+                 "l1_vs_fp64": {k: float(f"{v:.3e}") for k, v in l1_errors(case, r32["state"], r64["state"]).items()},
+                 "roofline": roofline(case, r32, 4),
+                 "storage": "alpha_k, rho_k, p_k, u, reduced energy in FP32; fluxes, update, relaxation in FP64"}
+    del st
     if args.calibrate_copy:
         copy_calibration()
     cpu = None if args.no_cpu else cpu_baseline(args)
     line = {
-        "metric": METRIC, "value": round(value, 1), "unit": "cell-updates/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall / args.steps * 1e3, 4),
+        "metric": METRIC, "value": round(r64["value"], 1), "unit": "cell-updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(r64["wall"] / args.steps * 1e3, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": f"2-D Kapila 5-eq MUSCL vanLeer FP64 {args.n}x{args.n}, Mach 2.4 air shock on 32 "
                                f"water cylinders", "cells": ncell, "model": "kapila", "order": 2,
                    "limiter": "vanleer", "phases": 2, "parallelism": "single GPU",
-                   "t_end": status.t, "global_batch": 1, "seq_len": ncell},
+                   "t_end": r64["status"].t},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "fp32_storage": fp32s,
     }
     print(json.dumps(line))
-    ctx.close()
 
 
 if __name__ == "__main__":

@@ -43,10 +43,10 @@ def variants(dim):
     for t, w in itertools.product(sp["tiles"], sp["waves"]):
         if dim == 2:
             name = f"d2_t{t[0]}x{t[1]}_w{w}"
-            defs = {"ECO_TILE2_X": t[0], "ECO_TILE2_Y": t[1], "ECO_MIN_WAVES_PER_EU": w}
+            defs = {"ECO_TILE2_X": t[0], "ECO_TILE2_Y": t[1], "ECO_WAVES2": w}
         else:
             name = f"d3_t{t[0]}x{t[1]}x{t[2]}_w{w}"
-            defs = {"ECO_TILE3_X": t[0], "ECO_TILE3_Y": t[1], "ECO_TILE3_Z": t[2], "ECO_MIN_WAVES_PER_EU": w}
+            defs = {"ECO_TILE3_X": t[0], "ECO_TILE3_Y": t[1], "ECO_TILE3_Z": t[2], "ECO_WAVES3": w}
         out.append((name, defs))
     return out
 
@@ -93,11 +93,9 @@ def _workload(dim, n):
 
 def _time_lib(lib_path, case, st, steps):
     """Load one variant library in a fresh namespace and time its stage kernels."""
-    import importlib
     from . import capi
     capi._lib = None
     capi.LIB_PATH = Path(lib_path)
-    importlib.reload  # keep the module object; only the handle changes
     ctx = capi.Context(case)
     ctx.set_state(st)
     ctx.run(nsteps=2)
@@ -135,16 +133,11 @@ def cmd_apply(args):
         if r["dim"] not in best or r["ms_per_step"] < best[r["dim"]]["ms_per_step"]:
             best[r["dim"]] = r
     hdr = (CSRC / "eco_tuning.h").read_text()
-    waves = None
     for dim, r in sorted(best.items()):
         for k, v in r["defs"].items():
-            if k == "ECO_MIN_WAVES_PER_EU":
-                waves = v if waves is None else min(waves, v)
-                continue
-            hdr = re.sub(rf"#define {k} \d+", f"#define {k} {v}", hdr)
+            k = {"ECO_MIN_WAVES_PER_EU": f"ECO_WAVES{dim}"}.get(k, k)
+            hdr = re.sub(rf"#define {k} \w+", f"#define {k} {v}", hdr)
         print(f"dim {dim}: {r['variant']} {r['ms_per_step']:.3f} ms/step")
-    if waves is not None:
-        hdr = re.sub(r"#define ECO_MIN_WAVES_PER_EU \d+", f"#define ECO_MIN_WAVES_PER_EU {waves}", hdr)
     (CSRC / "eco_tuning.h").write_text(hdr)
 
 

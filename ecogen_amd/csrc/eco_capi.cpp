@@ -27,6 +27,8 @@ struct eco_ctx {
   int nprim;            // device primitive fields
   double *buf[2];       // time levels, nprim+1 fields each
   double *bstar;        // U*, nprim fields
+  double *tmp64;        // FP64 staging of set/get_state in the FP32-storage form
+  size_t es;            // bytes per stored value (8 or 4)
   double *tables;       // geometry tables
   Control *ctl;
   int step;             // next step index to launch
@@ -93,7 +95,9 @@ static int validate(const eco_config *c) {
   // MultiP first order with relaxation reads per-cell cached phase energies and
   // sound speeds from before the relaxation; the device state does not keep them.
   if (c->model == ECO_MODEL_MULTIP && c->order == 1 && c->relax_p) return ECO_ERR_UNSUPPORTED;
-  if (c->precision != ECO_FP64) return ECO_ERR_UNSUPPORTED;
+  if (c->precision != ECO_FP64 && c->precision != ECO_FP32S) return ECO_ERR_INVALID;
+  // FP32 storage: MUSCL multiphase path (the batched/served configurations)
+  if (c->precision == ECO_FP32S && (c->order != 2 || c->model == ECO_MODEL_EULER)) return ECO_ERR_UNSUPPORTED;
   return ECO_OK;
 }
 
@@ -154,6 +158,7 @@ extern "C" int eco_create(const eco_config *cfg, void *hip_stream, eco_ctx **out
   S.eps = (c.model == ECO_MODEL_KAPILA && c.alpha_null) ? 1.e-15 : 0.;
   S.relax = (c.model == ECO_MODEL_KAPILA) || (c.model == ECO_MODEL_MULTIP && c.relax_p);
   S.cfl = c.cfl;
+  S.prec = c.precision;
   for (int k = 0; k < c.nphase && k < kMaxPhases; k++) lower_eos(c.eos[k], S.eos[k]);
   x->nprim = (c.model == ECO_MODEL_EULER) ? 2 + dim : 3 * c.nphase + dim;
 
@@ -164,11 +169,13 @@ extern "C" int eco_create(const eco_config *cfg, void *hip_stream, eco_ctx **out
   // device memory
   size_t nc = (size_t)G.ncell;
   hipError_t e = hipSuccess;
-  x->buf[0] = x->buf[1] = x->bstar = x->tables = nullptr;
+  x->buf[0] = x->buf[1] = x->bstar = x->tables = x->tmp64 = nullptr;
   x->ctl = nullptr;
-  e = hipMalloc(&x->buf[0], nc * (x->nprim + 1) * sizeof(double));
-  if (e == hipSuccess) e = hipMalloc(&x->buf[1], nc * (x->nprim + 1) * sizeof(double));
-  if (e == hipSuccess && c.order == 2) e = hipMalloc(&x->bstar, nc * x->nprim * sizeof(double));
+  x->es = (c.precision == ECO_FP32S) ? sizeof(float) : sizeof(double);
+  e = hipMalloc(&x->buf[0], nc * (x->nprim + 1) * x->es);
+  if (e == hipSuccess) e = hipMalloc(&x->buf[1], nc * (x->nprim + 1) * x->es);
+  if (e == hipSuccess && c.order == 2) e = hipMalloc(&x->bstar, nc * x->nprim * x->es);
+  if (e == hipSuccess && c.precision == ECO_FP32S) e = hipMalloc(&x->tmp64, nc * (x->nprim + 1) * sizeof(double));
   if (e == hipSuccess) e = hipMalloc(&x->tables, ntab * sizeof(double));
   if (e == hipSuccess) e = hipMalloc(&x->ctl, sizeof(Control));
   if (e != hipSuccess) { eco_destroy(x); return ECO_ERR_NOMEM; }
@@ -219,13 +226,22 @@ static void host_to_dev_fields(const eco_ctx *x, std::vector<int> &map) {
   }
 }
 
+// device buffer the FP64 device-layout input is staged in
+static double *staging(eco_ctx *x) { return x->tmp64 ? x->tmp64 : x->buf[0]; }
+
 static int finish_set_state(eco_ctx *x) {
   int rc = reset_control(x);
   if (rc) return rc;
   StageArgs A = make_args(x, 0);
-  A.wout = x->buf[0];
+  A.wsrc = A.wn = staging(x);
+  A.wout = staging(x);
   rc = launch_stage(A, 3, x->stream);
   if (rc) return rc < 0 ? ECO_ERR_UNSUPPORTED : ECO_ERR_HIP;
+  if (x->tmp64) {
+    A.wout = x->buf[0];
+    rc = launch_convert(A, 1, x->tmp64, x->stream);
+    if (rc) return rc < 0 ? ECO_ERR_UNSUPPORTED : ECO_ERR_HIP;
+  }
   x->step = 0;
   x->have_state = 1;
   return hipcode(hipStreamSynchronize(x->stream));
@@ -237,7 +253,7 @@ extern "C" int eco_set_state(eco_ctx *x, const double *host_state) {
   host_to_dev_fields(x, map);
   size_t nc = (size_t)x->geo.ncell;
   for (size_t f = 0; f < map.size(); f++) {
-    hipError_t e = hipMemcpyAsync(x->buf[0] + f * nc, host_state + (size_t)map[f] * nc, nc * sizeof(double),
+    hipError_t e = hipMemcpyAsync(staging(x) + f * nc, host_state + (size_t)map[f] * nc, nc * sizeof(double),
                                   hipMemcpyHostToDevice, x->stream);
     if (e != hipSuccess) return ECO_ERR_HIP;
   }
@@ -250,7 +266,7 @@ extern "C" int eco_set_state_device(eco_ctx *x, const double *dev_state) {
   host_to_dev_fields(x, map);
   size_t nc = (size_t)x->geo.ncell;
   for (size_t f = 0; f < map.size(); f++) {
-    hipError_t e = hipMemcpyAsync(x->buf[0] + f * nc, dev_state + (size_t)map[f] * nc, nc * sizeof(double),
+    hipError_t e = hipMemcpyAsync(staging(x) + f * nc, dev_state + (size_t)map[f] * nc, nc * sizeof(double),
                                   hipMemcpyDeviceToDevice, x->stream);
     if (e != hipSuccess) return ECO_ERR_HIP;
   }
@@ -292,6 +308,13 @@ extern "C" int eco_get_state(eco_ctx *x, double *host_state) {
   const eco_config &c = x->cfg;
   size_t nc = (size_t)x->geo.ncell;
   const double *src = x->buf[h.steps & 1];
+  if (x->tmp64) {
+    StageArgs A = make_args(x, h.steps);
+    A.wn = x->buf[h.steps & 1];
+    int rc2 = launch_convert(A, 0, x->tmp64, x->stream);
+    if (rc2) return rc2 < 0 ? ECO_ERR_UNSUPPORTED : ECO_ERR_HIP;
+    src = x->tmp64;
+  }
   std::vector<int> map;
   host_to_dev_fields(x, map);
   int nf = ECO_NFIELDS(c.nphase);
@@ -473,5 +496,6 @@ extern "C" void eco_destroy(eco_ctx *x) {
   if (x->bstar) (void)hipFree(x->bstar);
   if (x->tables) (void)hipFree(x->tables);
   if (x->ctl) (void)hipFree(x->ctl);
+  if (x->tmp64) (void)hipFree(x->tmp64);
   delete x;
 }

@@ -17,12 +17,25 @@ int launch_copy(void *dst, const void *src, unsigned long long nbytes, hipStream
 int launch_euler(const StageArgs &A, int stage, hipStream_t st);
 int launch_kapila(const StageArgs &A, int stage, hipStream_t st);
 int launch_multip(const StageArgs &A, int stage, hipStream_t st);
+int launch_kapila32(const StageArgs &A, int stage, hipStream_t st, double *tmp);
+int launch_multip32(const StageArgs &A, int stage, hipStream_t st, double *tmp);
 
 int launch_stage(const StageArgs &A, int stage, hipStream_t st) {
+  if (A.sch.prec == PREC_FP32S && stage != 3) {
+    if (A.sch.model == MODEL_KAPILA) return launch_kapila32(A, stage, st, nullptr);
+    if (A.sch.model == MODEL_MULTIP) return launch_multip32(A, stage, st, nullptr);
+    return -1;
+  }
   if (A.sch.model == MODEL_EULER) return launch_euler(A, stage, st);
   if (A.sch.nph != 2) return -1;
   if (A.sch.model == MODEL_KAPILA) return launch_kapila(A, stage, st);
   if (A.sch.model == MODEL_MULTIP) return launch_multip(A, stage, st);
+  return -1;
+}
+int launch_convert(const StageArgs &A, int to_f32, double *tmp, hipStream_t st) {
+  const int stage = to_f32 ? 4 : 5;
+  if (A.sch.model == MODEL_KAPILA) return launch_kapila32(A, stage, st, tmp);
+  if (A.sch.model == MODEL_MULTIP) return launch_multip32(A, stage, st, tmp);
   return -1;
 }
 }  // namespace eco

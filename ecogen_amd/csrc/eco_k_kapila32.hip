@@ -1,0 +1,8 @@
+// Instantiation of the FP32-storage stage kernels for the kapila model.
+#include "eco_stage32.inl"
+
+namespace eco {
+int launch_kapila32(const StageArgs &A, int stage, hipStream_t st, double *tmp) {
+  return launch_d32<MODEL_KAPILA, 2>(A, stage, st, tmp);
+}
+}  // namespace eco

@@ -50,6 +50,7 @@ struct SchemeParams {
   double eps;        // 1e-15 when alphaNull, else 0 (src/Eos/Eos.cpp:114-122)
   int relax;         // stiff pressure relaxation applied
   double cfl;
+  int prec;          // PREC_FP64 or PREC_FP32S (storage)
   EosParams eos[kMaxPhases];
 };
 

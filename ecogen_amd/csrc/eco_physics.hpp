@@ -472,6 +472,107 @@ ECO_D double hllc(const SchemeParams &S, const CellState<NPH> &L, const CellStat
 }
 
 // ----------------------------------------------------------------------------
+// Mixed-precision HLLC of the FP32-storage path (multiphase models).
+// The wave-speed estimates sL, sR come in from the FP32 reconstruction; every
+// flux component is formed in FP64 from the (FP32-rounded) face states, so
+// that the volume-fraction, mass and energy fluxes stay mutually consistent
+// to FP64 precision (the stiffened-gas energy offset gamma*pinf/(gamma-1)
+// multiplies them).  The isentropic star pressure
+//   p_k* = (p_k + pinf)(rho*/rho)^gamma - pinf = p_k + (p_k + pinf) expm1(gamma log1p(delta)),
+// delta = (sM - uS)/(sS - sM), evaluates the power factor in FP32 without the
+// (p + pinf) - pinf cancellation.
+// ----------------------------------------------------------------------------
+template <int MODEL, int NPH>
+ECO_D void hllc_mixed(const SchemeParams &S, const CellState<NPH> &L, const CellState<NPH> &R, double sL, double sR,
+                      FluxVec<NPH> &F) {
+  const double eps = S.eps;
+  double uL = L.u[0], pL = L.pm, rhoL = L.rho;
+  double uR = R.u[0], pR = R.pm, rhoR = R.rho;
+  double mL = rhoL * (sL - uL), mR = rhoR * (sR - uR);
+  double sM = (pR - pL + mL * uL - mR * uR) / (mL - mR);
+  if (fabs(sM) < 1.e-8) sM = 0.;
+  if (sL >= 0. || sR <= 0.) {
+    const bool left = (sL >= 0.);
+    const CellState<NPH> &X = left ? L : R;
+    double uS = left ? uL : uR, pS = left ? pL : pR, rhoS = left ? rhoL : rhoR;
+#pragma unroll
+    for (int k = 0; k < NPH; k++) {
+      F.a[k] = X.a[k] * sM;
+      F.m[k] = X.a[k] * X.r[k] * uS;
+      F.en[k] = X.a[k] * X.r[k] * X.e[k] * uS;
+    }
+    double Et = X.em + 0.5 * sqn(X.u);
+    F.q[0] = rhoS * uS * uS + pS;
+    F.q[1] = rhoS * X.u[1] * uS;
+    F.q[2] = rhoS * X.u[2] * uS;
+    F.E = (rhoS * Et + pS) * uS;
+  } else {
+    const bool left = (sM >= 0.);
+    const CellState<NPH> &X = left ? L : R;
+    double uS = left ? uL : uR, pS = left ? pL : pR, mS = left ? mL : mR, sS = left ? sL : sR;
+    double Et = X.em + 0.5 * sqn(X.u);
+    const double iss = 1. / (sS - sM);
+    double rhoStar = mS * iss;
+    double EStar = Et + (sM - uS) * (sM + pS / mS);
+    double pStar = mS * (sM - uS) + pS;
+    const double ratio = (sS - uS) * iss;
+    const float delta = (float)((sM - uS) * iss);
+    const float l1p = log1pf(delta);
+#pragma unroll
+    for (int k = 0; k < NPH; k++) {
+      const EosParams &q = S.eos[k];
+      double rks = X.r[k] * ratio;
+      double g = (double)expm1f((float)q.gamma * l1p);
+      double pks = X.p[k] + (X.p[k] + q.pinf) * g;
+      double eks = (pks + q.gpinf) * q.og * (1. / smax(rks, eps)) + q.eref;
+      F.a[k] = X.a[k] * sM;
+      F.m[k] = X.a[k] * rks * sM;
+      F.en[k] = X.a[k] * rks * eks * sM;
+    }
+    F.q[0] = rhoStar * sM * sM + pStar;
+    F.q[1] = rhoStar * X.u[1] * sM;
+    F.q[2] = rhoStar * X.u[2] * sM;
+    F.E = (rhoStar * EStar + pStar) * sM;
+  }
+  F.sM = sM;
+}
+
+// FP32 limiter (same thresholds and branch structure as limiter()).
+ECO_D float limiter_f(int lim, float s1, float s2) {
+  float slope = 0.f, prod = s1 * s2;
+  switch (lim) {
+    case LIM_MINMOD:
+    case LIM_THINC:
+      if (prod > 1e-9f) { slope = fminf(fabsf(s1), fabsf(s2)); if (s1 < 0.f) slope = -slope; }
+      return slope;
+    case LIM_VANLEER: {
+      float sum = s1 + s2;
+      if ((fabsf(sum) > 1e-6f) && (prod > 1e-6f)) slope = 2.f * prod / sum;
+      return slope;
+    }
+    case LIM_VANALBADA: {
+      float sum = s1 + s2;
+      if ((fabsf(s1) > 1e-6f) && (fabsf(s2) > 1e-6f) && (fabsf(sum) > 1e-6f) && (prod > 1e-6f))
+        slope = prod * sum / (s1 * s1 + s2 * s2);
+      return slope;
+    }
+    case LIM_MC:
+      if (prod > 1e-9f) {
+        slope = fminf(2.f * fabsf(s1), 2.f * fabsf(s2));
+        slope = fminf(slope, 0.5f * (fabsf(s1) + fabsf(s2)));
+      }
+      if (s1 < 0.f) slope = -slope;
+      return slope;
+    case LIM_SUPERBEE:
+      if (prod > 1e-6f) slope = fmaxf(fminf(2.f * fabsf(s1), fabsf(s2)), fminf(fabsf(s1), 2.f * fabsf(s2)));
+      if (s1 < 0.f) slope = -slope;
+      return slope;
+    default:
+      return 0.f;
+  }
+}
+
+// ----------------------------------------------------------------------------
 // Face frames (src/Meshes/MeshCartesian.cpp:300-551, src/Maths/Coord.cpp:132-150)
 // side: 0 = interior face / upper boundary of direction dir, 1 = lower boundary.
 // ----------------------------------------------------------------------------

@@ -36,9 +36,9 @@ struct Layout {
 // the autotuner (ecogen_amd/autotune.py writes eco_tuning.h).
 #include "eco_tuning.h"
 template <int DIM> struct Tile;
-template <> struct Tile<1> { static constexpr int X = ECO_TILE1_X, Y = 1, Z = 1; };
-template <> struct Tile<2> { static constexpr int X = ECO_TILE2_X, Y = ECO_TILE2_Y, Z = 1; };
-template <> struct Tile<3> { static constexpr int X = ECO_TILE3_X, Y = ECO_TILE3_Y, Z = ECO_TILE3_Z; };
+template <> struct Tile<1> { static constexpr int X = ECO_TILE1_X, Y = 1, Z = 1, W = ECO_WAVES1; };
+template <> struct Tile<2> { static constexpr int X = ECO_TILE2_X, Y = ECO_TILE2_Y, Z = 1, W = ECO_WAVES2; };
+template <> struct Tile<3> { static constexpr int X = ECO_TILE3_X, Y = ECO_TILE3_Y, Z = ECO_TILE3_Z, W = ECO_WAVES3; };
 
 template <int DIM>
 struct TileFaces {
@@ -273,21 +273,217 @@ __device__ __forceinline__ double ctl_dt(const Control *ctl, const Geometry &G, 
   return cfl * dtMax;
 }
 
+template <int MODEL, int NPH, int DIM>
+__device__ __forceinline__ void put_flux_s(double *L, int stride, int f, const FluxVec<NPH> &F) {
+  int q = 0;
+  if constexpr (MODEL == MODEL_EULER) {
+    L[(q++) * stride + f] = F.m[0];
+  } else {
+#pragma unroll
+    for (int k = 0; k < NPH; k++) {
+      L[(q++) * stride + f] = F.a[k]; L[(q++) * stride + f] = F.m[k]; L[(q++) * stride + f] = F.en[k];
+    }
+  }
+#pragma unroll
+  for (int d = 0; d < DIM; d++) L[(q++) * stride + f] = F.q[d];
+  L[(q++) * stride + f] = F.E;
+  L[(q++) * stride + f] = F.sM;
+}
+
+template <int MODEL, int NPH, int DIM>
+__device__ __forceinline__ void get_flux_s(const double *L, int stride, int f, FluxVec<NPH> &F) {
+  int q = 0;
+  if constexpr (MODEL == MODEL_EULER) {
+    F.m[0] = L[(q++) * stride + f];
+  } else {
+#pragma unroll
+    for (int k = 0; k < NPH; k++) {
+      F.a[k] = L[(q++) * stride + f]; F.m[k] = L[(q++) * stride + f]; F.en[k] = L[(q++) * stride + f];
+    }
+  }
+#pragma unroll
+  for (int d = 0; d < 3; d++) F.q[d] = (d < DIM) ? L[(q++) * stride + f] : 0.;
+  F.E = L[(q++) * stride + f];
+  F.sM = L[(q++) * stride + f];
+}
+
+// the cell's two faces of one direction from the LDS flux rows (low face added,
+// high face subtracted, in that order: CellInterface.cpp:100-116)
+template <int MODEL, int NPH, int DIM, int dir>
+__device__ __forceinline__ void accumulate_dir(FluxVec<NPH> &acc, const double *sF, int stride, int base, int lx, int ly,
+                                               int lz, const int *ci, const Geometry &G, const CellState<NPH> &own) {
+  constexpr int TX = Tile<DIM>::X, TY = Tile<DIM>::Y;
+  const int T0 = TX + (dir == 0), T1 = TY + (dir == 1);
+  int fl[3] = {lx, ly, lz};
+  const int flo = fl[0] + T0 * (fl[1] + T1 * fl[2]);
+  fl[dir] += 1;
+  const int fhi = fl[0] + T0 * (fl[1] + T1 * fl[2]);
+  FluxVec<NPH> F;
+  if (ci[dir] >= 1) {
+    get_flux_s<MODEL, NPH, DIM>(sF, stride, base + flo, F);
+    add_face<MODEL, NPH>(acc, F, G.coefA[dir], true, own);
+  }
+  if (ci[dir] <= G.n[dir] - 2) {
+    get_flux_s<MODEL, NPH, DIM>(sF, stride, base + fhi, F);
+    add_face<MODEL, NPH>(acc, F, G.coefA[dir], false, own);
+  }
+}
+
+// One interior face of the tile: reconstruction (MUSCL) or cached cell states
+// (first order), HLLC, flux to LDS slot ``slot`` (row stride ``stride``).
+// Returns the largest wave speed of the Riemann problem (0 if no face).
+template <int MODEL, int NPH, int DIM, int STAGE, bool SAMELIM, int dir>
+__device__ __forceinline__ double face_fp64(const StageArgs &A, const int *o, const long long *strd, int f,
+                                            double *sF, int stride, int slot) {
+  using Ly = Layout<MODEL, NPH, DIM>;
+  constexpr int TX = Tile<DIM>::X, TY = Tile<DIM>::Y;
+  constexpr bool O2 = (STAGE != 0);
+  const Geometry &G = A.geo;
+  const SchemeParams &S = A.sch;
+  const long long nc = G.ncell;
+  const int nx = G.n[0], ny = G.n[1], nz = G.n[2];
+  const double *__restrict__ Wsrc = A.wsrc;
+  const double *__restrict__ Wn = A.wn;
+  const int T0 = TX + (dir == 0), T1 = TY + (dir == 1);
+  const int ndir = G.n[dir];
+    const int fx = f % T0, fy = (f / T0) % T1, fz = f / (T0 * T1);
+    int g[3] = {o[0] + fx, o[1] + fy, o[2] + fz};  // cell on the high side of the face
+    if (g[0] >= nx + (dir == 0) || g[1] >= ny + (dir == 1) || g[2] >= nz + (dir == 2)) return 0.;
+    const int gi = g[dir];
+    if (gi < 1 || gi > ndir - 1) return 0.;  // boundary faces handled per cell below
+    const long long ihi = g[0] + strd[1] * g[1] + strd[2] * g[2];
+    const long long ilo = ihi - strd[dir];
+    const int i = gi - 1;  // index along dir of the low-side cell
+    CellState<NPH> L, R;
+    load_prim<MODEL, NPH, DIM>(Wsrc, nc, ilo, L);
+    load_prim<MODEL, NPH, DIM>(Wsrc, nc, ihi, R);
+    if constexpr (!O2) {
+      if constexpr (MODEL == MODEL_EULER) {
+        // first order, cached FluxEuler::buildPrim state: E stored, c from (rho, p)
+        L.E = Wsrc[Ly::IE * nc + ilo]; R.E = Wsrc[Ly::IE * nc + ihi];
+        L.c[0] = eos_c(S.eos[0], L.r[0], L.p[0], S.eps);
+        R.c[0] = eos_c(S.eos[0], R.r[0], R.p[0], S.eps);
+      } else {
+        fulfill<MODEL, NPH>(S, L);
+        fulfill<MODEL, NPH>(S, R);
+        if constexpr (MODEL == MODEL_MULTIP) {
+          // MultiP keeps the mixture energy of totalEnergyToInternalEnergy (Run.cpp:471)
+          L.em = Wsrc[Ly::IE * nc + ilo] - 0.5 * sqn(L.u);
+          R.em = Wsrc[Ly::IE * nc + ihi] - 0.5 * sqn(R.u);
+        }
+      }
+    } else {
+      // MUSCL: limited slopes of both cells from the face slopes at
+      // i-1/2, i+1/2, i+3/2 (CellInterfaceO2::computeSlopes + CellO2::
+      // computeLocalSlopes), then extrapolation to the face, one variable at a time.
+      const bool has_m = (i >= 1), has_p = (i + 2 <= ndir - 1);
+      const long long sd = strd[dir];
+      // interface detection: alpha of the face's cells from the stage state,
+      // alpha across the opposite face from the time-n state (CellO2.cpp:144-150)
+      double aL = 0., aR = 0., aLL = 0., aRR = 0.;
+      if constexpr (MODEL != MODEL_EULER) {
+        aL = L.a[0]; aR = R.a[0];
+        if (!SAMELIM || S.thinc) {
+          aLL = has_m ? Wn[ilo - sd] : 0.;
+          aRR = has_p ? Wn[ihi + sd] : 0.;
+        }
+      }
+      bool ifl = false, ifr = false;
+      if constexpr (!SAMELIM && MODEL != MODEL_EULER) {
+        ifl = (aL >= 1.e-4) && (aL <= 1. - 1.e-4) && ((aR - aL) * (aL - aLL) >= 1.e-8);
+        ifr = (aR >= 1.e-4) && (aR <= 1. - 1.e-4) && ((aL - aR) * (aR - aRR) >= 1.e-8);
+      }
+      const double dcc = G.dcc[dir][i];
+      const double dcm = has_m ? G.dcc[dir][i - 1] : 1.;
+      const double dcp = has_p ? G.dcc[dir][i + 1] : 1.;
+      const double dL = G.dfl[dir][i], dR = G.dfr[dir][i];
+      double wl_[Ly::NVAR], wr_[Ly::NVAR], wll[Ly::NVAR], whh[Ly::NVAR];
+      prim_to_vec<MODEL, NPH, DIM>(L, wl_);
+      prim_to_vec<MODEL, NPH, DIM>(R, wr_);
+      // all neighbour loads issued before any arithmetic
+#pragma unroll
+      for (int v = 0; v < Ly::NVAR; v++) {
+        const double *Fv = Wsrc + (long long)v * nc;
+        wll[v] = has_m ? Fv[ilo - sd] : 0.;
+        whh[v] = has_p ? Fv[ihi + sd] : 0.;
+      }
+      double xl[Ly::NVAR], xr[Ly::NVAR];
+#pragma unroll
+      for (int v = 0; v < Ly::NVAR; v++) {
+        const double wl = wl_[v], wr = wr_[v];
+        const double sc = (wr - wl) / dcc;
+        const double sm = has_m ? (wl - wll[v]) / dcm : 0.;
+        const double sp = has_p ? (whh[v] - wr) / dcp : 0.;
+        const bool isa = (MODEL != MODEL_EULER) && (v < 3 * NPH) && (v % 3 == 0);
+        const int llim = isa ? (ifl ? S.lim[3] : S.lim[2]) : (ifl ? S.lim[1] : S.lim[0]);
+        const int rlim = isa ? (ifr ? S.lim[3] : S.lim[2]) : (ifr ? S.lim[1] : S.lim[0]);
+        xl[v] = wl + limiter(llim, sc, sm) * dL;
+        xr[v] = wr + (-limiter(rlim, sc, sp)) * dR;
+      }
+      vec_to_prim<MODEL, NPH, DIM>(xl, L);
+      vec_to_prim<MODEL, NPH, DIM>(xr, R);
+      if constexpr (MODEL == MODEL_EULER) {
+        verify_correct<MODEL, NPH>(S, L, 0);
+        verify_correct<MODEL, NPH>(S, R, 0);
+      } else {
+#pragma unroll
+        for (int k = 0; k < NPH; k++) { verify_correct<MODEL, NPH>(S, L, k); verify_correct<MODEL, NPH>(S, R, k); }
+      }
+      if (S.thinc) {
+        thinc<NPH>(L, aL, aR, aLL, false);
+        thinc<NPH>(R, aR, aRR, aL, true);
+      }
+    }
+    to_face_frame(dir, 0, L.u);
+    to_face_frame(dir, 0, R.u);
+    if constexpr (O2) {
+      fulfill<MODEL, NPH>(S, L);
+      fulfill<MODEL, NPH>(S, R);
+    }
+    FluxVec<NPH> F;
+    const double smx = hllc<MODEL, NPH>(S, L, R, F);
+    from_face_frame(dir, 0, F.q);
+    put_flux_s<MODEL, NPH, DIM>(sF, stride, slot, F);
+    return smx;
+}
+
+template <int MODEL, int NPH, int DIM, int STAGE, bool SAMELIM, int dir, int NT>
+__device__ __forceinline__ void stage_dir_3d(const StageArgs &A, const int *o, const long long *strd, int tid, bool active,
+                                             int lx, int ly, int lz, const int *ci, long long cidx, double *sF, int stride,
+                                             double (*sAcc)[NT], FluxVec<NPH> &acc, double &smx) {
+  if constexpr (dir < DIM) {
+    constexpr int TX = Tile<DIM>::X, TY = Tile<DIM>::Y, TZ = Tile<DIM>::Z;
+    const int nfd = (TX + (dir == 0)) * (TY + (dir == 1)) * (TZ + (dir == 2));
+    for (int f = tid; f < nfd; f += NT) smx = smax(smx, face_fp64<MODEL, NPH, DIM, STAGE, SAMELIM, dir>(A, o, strd, f, sF, stride, f));
+    __syncthreads();
+    if (active) {
+      if (dir > 0) unpark_acc<MODEL, NPH, DIM, NT>(sAcc, tid, acc);
+      CellState<NPH> own;
+      load_nc_coeffs<MODEL, NPH, DIM>(A.wn, A.geo.ncell, cidx, own);
+      accumulate_dir<MODEL, NPH, DIM, dir>(acc, sF, stride, 0, lx, ly, lz, ci, A.geo, own);
+      if (dir < DIM - 1) park_acc<MODEL, NPH, DIM, NT>(sAcc, tid, acc);
+    }
+    __syncthreads();
+  }
+}
+
 // ---------------------------------------------------------------------------
 // the stage kernel
 // STAGE 0: first-order step; 1: MUSCL predictor (writes U*); 2: MUSCL corrector
 // ---------------------------------------------------------------------------
 template <int MODEL, int NPH, int DIM, int STAGE, bool SAMELIM>
-__global__ void __launch_bounds__(Tile<DIM>::X * Tile<DIM>::Y * Tile<DIM>::Z, ECO_MIN_WAVES_PER_EU)
+__global__ void __launch_bounds__(Tile<DIM>::X * Tile<DIM>::Y * Tile<DIM>::Z, Tile<DIM>::W)
 k_stage(const StageArgs A) {
   using Ly = Layout<MODEL, NPH, DIM>;
   using TF = TileFaces<DIM>;
   constexpr int TX = Tile<DIM>::X, TY = Tile<DIM>::Y, TZ = Tile<DIM>::Z;
   constexpr int NT = TX * TY * TZ;
   constexpr int MAXF = TF::MAXF;
-  constexpr bool O2 = (STAGE != 0);
-  __shared__ double sF[Ly::NFV][MAXF];
-  __shared__ double sAcc[Ly::NACC][NT];
+  constexpr bool MERGED = (DIM <= 2);
+  constexpr int NF0 = TF::nf(0), NF1 = TF::nf(1);
+  constexpr int NFT = MERGED ? NF0 + NF1 : MAXF;  // LDS flux slots
+  __shared__ double sF[Ly::NFV][NFT];
+  __shared__ double sAcc[MERGED ? 1 : Ly::NACC][NT];
   __shared__ double sRed[NT / 64];
 
   const Geometry &G = A.geo;
@@ -327,140 +523,29 @@ k_stage(const StageArgs A) {
   const double *__restrict__ Wsrc = A.wsrc;  // state the fluxes are built from (U^n or U*)
   const double *__restrict__ Wn = A.wn;      // time-n state
 
-  // the accumulator stays in registers only during the accumulation steps and
-  // is parked in LDS while the block computes the next direction's faces
   FluxVec<NPH> acc;
   zero_flux(acc);
   double smx = 0.;
 
-#pragma unroll
-  for (int dir = 0; dir < DIM; dir++) {
-    const int T0 = TX + (dir == 0), T1 = TY + (dir == 1);
-    const int nfd = T0 * T1 * (TZ + (dir == 2));
-    const int ndir = G.n[dir];
-    for (int f = tid; f < nfd; f += NT) {
-      const int fx = f % T0, fy = (f / T0) % T1, fz = f / (T0 * T1);
-      int g[3] = {o[0] + fx, o[1] + fy, o[2] + fz};  // cell on the high side of the face
-      if (g[0] >= nx + (dir == 0) || g[1] >= ny + (dir == 1) || g[2] >= nz + (dir == 2)) continue;
-      const int gi = g[dir];
-      if (gi < 1 || gi > ndir - 1) continue;  // boundary faces handled per cell below
-      const long long ihi = g[0] + strd[1] * g[1] + strd[2] * g[2];
-      const long long ilo = ihi - strd[dir];
-      const int i = gi - 1;  // index along dir of the low-side cell
-      CellState<NPH> L, R;
-      load_prim<MODEL, NPH, DIM>(Wsrc, nc, ilo, L);
-      load_prim<MODEL, NPH, DIM>(Wsrc, nc, ihi, R);
-      if constexpr (!O2) {
-        if constexpr (MODEL == MODEL_EULER) {
-          // first order, cached FluxEuler::buildPrim state: E stored, c from (rho, p)
-          L.E = Wsrc[Ly::IE * nc + ilo]; R.E = Wsrc[Ly::IE * nc + ihi];
-          L.c[0] = eos_c(S.eos[0], L.r[0], L.p[0], S.eps);
-          R.c[0] = eos_c(S.eos[0], R.r[0], R.p[0], S.eps);
-        } else {
-          fulfill<MODEL, NPH>(S, L);
-          fulfill<MODEL, NPH>(S, R);
-          if constexpr (MODEL == MODEL_MULTIP) {
-            // MultiP keeps the mixture energy of totalEnergyToInternalEnergy (Run.cpp:471)
-            L.em = Wsrc[Ly::IE * nc + ilo] - 0.5 * sqn(L.u);
-            R.em = Wsrc[Ly::IE * nc + ihi] - 0.5 * sqn(R.u);
-          }
-        }
-      } else {
-        // MUSCL: limited slopes of both cells from the face slopes at
-        // i-1/2, i+1/2, i+3/2 (CellInterfaceO2::computeSlopes + CellO2::
-        // computeLocalSlopes), then extrapolation to the face, one variable at a time.
-        const bool has_m = (i >= 1), has_p = (i + 2 <= ndir - 1);
-        const long long sd = strd[dir];
-        // interface detection: alpha of the face's cells from the stage state,
-        // alpha across the opposite face from the time-n state (CellO2.cpp:144-150)
-        double aL = 0., aR = 0., aLL = 0., aRR = 0.;
-        if constexpr (MODEL != MODEL_EULER) {
-          aL = L.a[0]; aR = R.a[0];
-          if (!SAMELIM || S.thinc) {
-            aLL = has_m ? Wn[ilo - sd] : 0.;
-            aRR = has_p ? Wn[ihi + sd] : 0.;
-          }
-        }
-        bool ifl = false, ifr = false;
-        if constexpr (!SAMELIM && MODEL != MODEL_EULER) {
-          ifl = (aL >= 1.e-4) && (aL <= 1. - 1.e-4) && ((aR - aL) * (aL - aLL) >= 1.e-8);
-          ifr = (aR >= 1.e-4) && (aR <= 1. - 1.e-4) && ((aL - aR) * (aR - aRR) >= 1.e-8);
-        }
-        const double dcc = G.dcc[dir][i];
-        const double dcm = has_m ? G.dcc[dir][i - 1] : 1.;
-        const double dcp = has_p ? G.dcc[dir][i + 1] : 1.;
-        const double dL = G.dfl[dir][i], dR = G.dfr[dir][i];
-        double wl_[Ly::NVAR], wr_[Ly::NVAR], wll[Ly::NVAR], whh[Ly::NVAR];
-        prim_to_vec<MODEL, NPH, DIM>(L, wl_);
-        prim_to_vec<MODEL, NPH, DIM>(R, wr_);
-        // all neighbour loads issued before any arithmetic
-#pragma unroll
-        for (int v = 0; v < Ly::NVAR; v++) {
-          const double *Fv = Wsrc + (long long)v * nc;
-          wll[v] = has_m ? Fv[ilo - sd] : 0.;
-          whh[v] = has_p ? Fv[ihi + sd] : 0.;
-        }
-        double xl[Ly::NVAR], xr[Ly::NVAR];
-#pragma unroll
-        for (int v = 0; v < Ly::NVAR; v++) {
-          const double wl = wl_[v], wr = wr_[v];
-          const double sc = (wr - wl) / dcc;
-          const double sm = has_m ? (wl - wll[v]) / dcm : 0.;
-          const double sp = has_p ? (whh[v] - wr) / dcp : 0.;
-          const bool isa = (MODEL != MODEL_EULER) && (v < 3 * NPH) && (v % 3 == 0);
-          const int llim = isa ? (ifl ? S.lim[3] : S.lim[2]) : (ifl ? S.lim[1] : S.lim[0]);
-          const int rlim = isa ? (ifr ? S.lim[3] : S.lim[2]) : (ifr ? S.lim[1] : S.lim[0]);
-          xl[v] = wl + limiter(llim, sc, sm) * dL;
-          xr[v] = wr + (-limiter(rlim, sc, sp)) * dR;
-        }
-        vec_to_prim<MODEL, NPH, DIM>(xl, L);
-        vec_to_prim<MODEL, NPH, DIM>(xr, R);
-        if constexpr (MODEL == MODEL_EULER) {
-          verify_correct<MODEL, NPH>(S, L, 0);
-          verify_correct<MODEL, NPH>(S, R, 0);
-        } else {
-#pragma unroll
-          for (int k = 0; k < NPH; k++) { verify_correct<MODEL, NPH>(S, L, k); verify_correct<MODEL, NPH>(S, R, k); }
-        }
-        if (S.thinc) {
-          thinc<NPH>(L, aL, aR, aLL, false);
-          thinc<NPH>(R, aR, aRR, aL, true);
-        }
-      }
-      to_face_frame(dir, 0, L.u);
-      to_face_frame(dir, 0, R.u);
-      if constexpr (O2) {
-        fulfill<MODEL, NPH>(S, L);
-        fulfill<MODEL, NPH>(S, R);
-      }
-      FluxVec<NPH> F;
-      smx = smax(smx, hllc<MODEL, NPH>(S, L, R, F));
-      from_face_frame(dir, 0, F.q);
-      put_flux<MODEL, NPH, DIM, MAXF>(sF, f, F);
+  if constexpr (MERGED) {
+    // all faces of the tile (both directions) in one pass, one barrier
+    for (int w = tid; w < NF0 + NF1; w += NT) {
+      if (w < NF0) smx = smax(smx, face_fp64<MODEL, NPH, DIM, STAGE, SAMELIM, 0>(A, o, strd, w, &sF[0][0], NFT, w));
+      else if constexpr (DIM > 1)
+        smx = smax(smx, face_fp64<MODEL, NPH, DIM, STAGE, SAMELIM, 1>(A, o, strd, w - NF0, &sF[0][0], NFT, w));
     }
     __syncthreads();
     if (active) {
-      const int lc[3] = {lx, ly, lz};
-      const int T0_ = TX + (dir == 0), T1_ = TY + (dir == 1);
-      int fl[3] = {lc[0], lc[1], lc[2]};
-      const int flo = fl[0] + T0_ * (fl[1] + T1_ * fl[2]);
-      fl[dir] += 1;
-      const int fhi = fl[0] + T0_ * (fl[1] + T1_ * fl[2]);
-      if (dir > 0) unpark_acc<MODEL, NPH, DIM, NT>(sAcc, tid, acc);
       CellState<NPH> own;
       load_nc_coeffs<MODEL, NPH, DIM>(Wn, nc, cidx, own);
-      FluxVec<NPH> F;
-      if (ci[dir] >= 1) {
-        get_flux<MODEL, NPH, DIM, MAXF>(sF, flo, F);
-        add_face<MODEL, NPH>(acc, F, G.coefA[dir], true, own);
-      }
-      if (ci[dir] <= ndir - 2) {
-        get_flux<MODEL, NPH, DIM, MAXF>(sF, fhi, F);
-        add_face<MODEL, NPH>(acc, F, G.coefA[dir], false, own);
-      }
-      if (dir < DIM - 1) park_acc<MODEL, NPH, DIM, NT>(sAcc, tid, acc);
+      accumulate_dir<MODEL, NPH, DIM, 0>(acc, &sF[0][0], NFT, 0, lx, ly, lz, ci, G, own);
+      if constexpr (DIM > 1) accumulate_dir<MODEL, NPH, DIM, 1>(acc, &sF[0][0], NFT, NF0, lx, ly, lz, ci, G, own);
     }
-    __syncthreads();
+  } else {
+    // 3-D: one direction at a time, the accumulator parked in LDS in between
+    stage_dir_3d<MODEL, NPH, DIM, STAGE, SAMELIM, 0>(A, o, strd, tid, active, lx, ly, lz, ci, cidx, &sF[0][0], NFT, sAcc, acc, smx);
+    stage_dir_3d<MODEL, NPH, DIM, STAGE, SAMELIM, 1>(A, o, strd, tid, active, lx, ly, lz, ci, cidx, &sF[0][0], NFT, sAcc, acc, smx);
+    stage_dir_3d<MODEL, NPH, DIM, STAGE, SAMELIM, 2>(A, o, strd, tid, active, lx, ly, lz, ci, cidx, &sF[0][0], NFT, sAcc, acc, smx);
   }
 
   if (active) {

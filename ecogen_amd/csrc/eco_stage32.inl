@@ -1,0 +1,525 @@
+// FP32-storage form of the MUSCL stage kernels (multiphase models).
+//
+// State in HBM: alpha_k, rho_k, p_k and the velocity in FP32, plus the reduced
+// mixture energy R = rho*E - sum_k alpha_k (gamma_k pinf_k/(gamma_k-1) + rho_k eref_k)
+// in FP32.  R is a pressure-scale quantity: the stiffened-gas offset is
+// re-added in FP64 from the stored alpha_k, rho_k, so the (alpha, E) pair
+// stays self-consistent after rounding and the mixture pressure recovered by
+// the energy correction keeps its accuracy.
+//
+// Compute: wave-speed estimates in FP32; slopes, limiters, extrapolation,
+// every flux component, the accumulation, buildPrim, the stiff pressure
+// relaxation (Newton) and the energy correction in FP64.
+#pragma once
+#include "eco_stage.inl"
+
+namespace eco {
+
+template <int MODEL, int NPH, int DIM>
+__device__ __forceinline__ void load_prim_f(const float *__restrict__ W, long long nc, long long idx, CellState<NPH> &s) {
+#pragma unroll
+  for (int k = 0; k < NPH; k++) {
+    s.a[k] = (double)W[(3 * k) * nc + idx];
+    s.r[k] = (double)W[(3 * k + 1) * nc + idx];
+    s.p[k] = (double)W[(3 * k + 2) * nc + idx];
+  }
+#pragma unroll
+  for (int d = 0; d < 3; d++) s.u[d] = (d < DIM) ? (double)W[(3 * NPH + d) * nc + idx] : 0.;
+}
+
+template <int MODEL, int NPH, int DIM>
+__device__ __forceinline__ void store_prim_f(float *__restrict__ W, long long nc, long long idx, const CellState<NPH> &s) {
+#pragma unroll
+  for (int k = 0; k < NPH; k++) {
+    W[(3 * k) * nc + idx] = (float)s.a[k];
+    W[(3 * k + 1) * nc + idx] = (float)s.r[k];
+    W[(3 * k + 2) * nc + idx] = (float)s.p[k];
+  }
+#pragma unroll
+  for (int d = 0; d < DIM; d++) W[(3 * NPH + d) * nc + idx] = (float)s.u[d];
+}
+
+// energy offset sum_k alpha_k (gamma pinf/(gamma-1) + rho_k eref) of a state
+template <int NPH>
+__device__ __forceinline__ double energy_offset(const SchemeParams &S, const CellState<NPH> &s) {
+  double o = 0.;
+#pragma unroll
+  for (int k = 0; k < NPH; k++) o += s.a[k] * (S.eos[k].gpg + s.r[k] * S.eos[k].eref);
+  return o;
+}
+
+// the FP32-rounded state as stored (for the offset of R)
+template <int NPH>
+__device__ __forceinline__ void round_state(CellState<NPH> &s) {
+#pragma unroll
+  for (int k = 0; k < NPH; k++) {
+    s.a[k] = (double)(float)s.a[k];
+    s.r[k] = (double)(float)s.r[k];
+    s.p[k] = (double)(float)s.p[k];
+  }
+}
+
+// Specific internal energy with one reciprocal: the FP32-storage path needs
+// the fluxes mutually consistent in FP64, not the reference's rounding.
+__device__ __forceinline__ double eos_e_fast(const EosParams &q, double rho, double p, double eps) {
+  return (p + q.gpinf) * q.og * (1. / smax(rho, eps)) + q.eref;
+}
+
+// energies and mixture variables in FP64 (no sound speeds): the part of
+// fulfillState the fluxes need at full precision
+template <int MODEL, int NPH>
+__device__ __forceinline__ void fulfill_energy(const SchemeParams &S, CellState<NPH> &s) {
+  const double eps = S.eps;
+#pragma unroll
+  for (int k = 0; k < NPH; k++) s.e[k] = eos_e_fast(S.eos[k], s.r[k], s.p[k], eps);
+  s.rho = 0.; s.pm = 0.;
+#pragma unroll
+  for (int k = 0; k < NPH; k++) { s.rho += s.a[k] * s.r[k]; s.pm += s.a[k] * s.p[k]; }
+  const double irho = 1. / ((MODEL == MODEL_KAPILA) ? smax(s.rho, eps) : s.rho);
+#pragma unroll
+  for (int k = 0; k < NPH; k++) s.Y[k] = s.a[k] * s.r[k] * irho;
+  s.em = 0.;
+#pragma unroll
+  for (int k = 0; k < NPH; k++) s.em += s.Y[k] * s.e[k];
+}
+
+// buildCons with one reciprocal per phase
+template <int MODEL, int NPH>
+__device__ __forceinline__ void build_cons_fast(const SchemeParams &S, const CellState<NPH> &s, FluxVec<NPH> &f) {
+#pragma unroll
+  for (int k = 0; k < NPH; k++) {
+    f.a[k] = s.a[k];
+    f.m[k] = s.a[k] * s.r[k];
+    f.en[k] = s.a[k] * (s.p[k] + S.eos[k].gpinf) * S.eos[k].og + f.m[k] * S.eos[k].eref;
+  }
+#pragma unroll
+  for (int d = 0; d < 3; d++) f.q[d] = s.u[d] * s.rho;
+  f.E = s.rho * s.E;
+}
+
+// RelaxationP::stiffRelaxation (RelaxationP.cpp:49-111) with the Newton
+// residual in the form sum ak denom/num - 1 (one division per phase per
+// iteration), same skip rule, floors, tolerance and 100-iteration cap.
+template <int MODEL, int NPH>
+__device__ __forceinline__ bool relax_p_fast(const SchemeParams &S, CellState<NPH> &s) {
+  if (S.eps > 1.e-20) {
+#pragma unroll
+    for (int k = 0; k < NPH; k++) if (s.a[k] > (1. - 1.e-5)) return true;
+  }
+  double ak[NPH], pk[NPH], rk[NPH];
+  double pStar = 0.;
+#pragma unroll
+  for (int k = 0; k < NPH; k++) {
+    verify_correct<MODEL, NPH>(S, s, k);
+    ak[k] = s.a[k]; pk[k] = s.p[k]; rk[k] = s.r[k];
+    pStar += ak[k] * pk[k];
+  }
+  int it = 0;
+  double f = 0., df = 1.;
+  do {
+    it++;
+    pStar -= f / df;
+#pragma unroll
+    for (int k = 0; k < NPH; k++) pStar = eos_fix_p(S.eos[k], pStar);
+    f = -1.; df = 0.;
+#pragma unroll
+    for (int k = 0; k < NPH; k++) {
+      const EosParams &q = S.eos[k];
+      const double num = q.gamma * (pStar + q.pinf);
+      const double inum = 1. / num;
+      f += ak[k] * (num + pk[k] - pStar) * inum;
+      df -= ak[k] * q.gamma * (pk[k] + q.pinf) * inum * inum;
+    }
+  } while (fabs(f) > 1e-10 && it < 100);
+  if (it >= 100) return false;
+#pragma unroll
+  for (int k = 0; k < NPH; k++) {
+    const EosParams &q = S.eos[k];
+    const double num = q.gamma * (pStar + q.pinf);
+    const double denom = num + pk[k] - pStar;
+    s.a[k] = ak[k] * denom / num;
+    s.r[k] = rk[k] * num / denom;
+    s.p[k] = pStar;
+  }
+  s.pm = pStar;
+  return true;
+}
+
+__device__ __forceinline__ float sminf_(float a, float b) { return (b < a) ? b : a; }
+__device__ __forceinline__ float smaxf_(float a, float b) { return (a < b) ? b : a; }
+
+// frozen sound speed in FP32 (MixKapila.cpp:147-172)
+template <int NPH>
+__device__ __forceinline__ float frozen_c_f(const SchemeParams &S, const CellState<NPH> &s) {
+  float rho = 0.f, ck2[NPH];
+#pragma unroll
+  for (int k = 0; k < NPH; k++) rho += (float)s.a[k] * (float)s.r[k];
+  float c2 = 0.f;
+#pragma unroll
+  for (int k = 0; k < NPH; k++) {
+    const EosParams &q = S.eos[k];
+    const float r = (float)s.r[k], p = (float)s.p[k];
+    ck2[k] = (q.type == EOS_SG) ? (float)q.gamma * (p + (float)q.pinf) / r : (float)q.gamma * p / r;
+    c2 += ((float)s.a[k] * r / rho) * ck2[k];
+  }
+  return sqrtf(c2);
+}
+
+// Davis estimates (ModKapila.cpp:117-121) in FP32; returns max |s| above the guard
+template <int NPH>
+__device__ __forceinline__ float speeds_f(const SchemeParams &S, const CellState<NPH> &L, const CellState<NPH> &R,
+                                          double &sL, double &sR) {
+  const float cL = frozen_c_f<NPH>(S, L), cR = frozen_c_f<NPH>(S, R);
+  const float uL = (float)L.u[0], uR = (float)R.u[0];
+  const float a = sminf_(uL - cL, uR - cR);
+  const float b = smaxf_(uR + cR, uL + cL);
+  float smx = 0.f;
+  if (fabsf(a) > 1.e-3f) smx = fabsf(a);
+  if (fabsf(b) > 1.e-3f) smx = smaxf_(smx, fabsf(b));
+  sL = (double)a;
+  sR = (double)b;
+  return smx;
+}
+
+// time-n alpha_k, p_k of the owning cell (non-conservative terms)
+template <int NPH>
+__device__ __forceinline__ void load_nc_coeffs_f(const float *__restrict__ Wn, long long nc, long long cidx,
+                                                 CellState<NPH> &own) {
+#pragma unroll
+  for (int k = 0; k < NPH; k++) {
+    own.a[k] = (double)Wn[(3 * k) * nc + cidx];
+    own.p[k] = (double)Wn[(3 * k + 2) * nc + cidx];
+  }
+}
+
+// One interior face of the tile in the FP32-storage form: FP32 loads,
+// reconstruction and limiting in FP64 (slopes through one reciprocal per
+// cell distance), FP32 wave-speed estimates, FP64 HLLC fluxes to LDS slot
+// ``slot``.  Returns the largest wave speed above the guard (0 if no face).
+template <int MODEL, int NPH, int DIM, bool SAMELIM, int dir>
+__device__ __forceinline__ float face_fp32s(const StageArgs &A, const int *o, const long long *strd, int f, double *sF,
+                                            int stride, int slot) {
+  using Ly = Layout<MODEL, NPH, DIM>;
+  constexpr int TX = Tile<DIM>::X, TY = Tile<DIM>::Y;
+  const Geometry &G = A.geo;
+  const SchemeParams &S = A.sch;
+  const long long nc = G.ncell;
+  const int nx = G.n[0], ny = G.n[1], nz = G.n[2];
+  const float *__restrict__ Wsrc = (const float *)A.wsrc;
+  const float *__restrict__ Wn = (const float *)A.wn;
+  const int T0 = TX + (dir == 0), T1 = TY + (dir == 1);
+  const int ndir = G.n[dir];
+  const int fx = f % T0, fy = (f / T0) % T1, fz = f / (T0 * T1);
+  int g[3] = {o[0] + fx, o[1] + fy, o[2] + fz};
+  if (g[0] >= nx + (dir == 0) || g[1] >= ny + (dir == 1) || g[2] >= nz + (dir == 2)) return 0.f;
+  const int gi = g[dir];
+  if (gi < 1 || gi > ndir - 1) return 0.f;
+  const long long ihi = g[0] + strd[1] * g[1] + strd[2] * g[2];
+  const long long ilo = ihi - strd[dir];
+  const long long sd = strd[dir];
+  const int i = gi - 1;
+  const bool has_m = (i >= 1), has_p = (i + 2 <= ndir - 1);
+  float wl_[Ly::NVAR], wr_[Ly::NVAR], wll[Ly::NVAR], whh[Ly::NVAR];
+#pragma unroll
+  for (int v = 0; v < Ly::NVAR; v++) {
+    const float *Fv = Wsrc + (long long)v * nc;
+    wl_[v] = Fv[ilo];
+    wr_[v] = Fv[ihi];
+    wll[v] = has_m ? Fv[ilo - sd] : 0.f;
+    whh[v] = has_p ? Fv[ihi + sd] : 0.f;
+  }
+  const double aL = wl_[0], aR = wr_[0];
+  double aLL = 0., aRR = 0.;
+  if (!SAMELIM || S.thinc) {
+    aLL = has_m ? (double)Wn[ilo - sd] : 0.;
+    aRR = has_p ? (double)Wn[ihi + sd] : 0.;
+  }
+  bool ifl = false, ifr = false;
+  if constexpr (!SAMELIM) {
+    ifl = (aL >= 1.e-4) && (aL <= 1. - 1.e-4) && ((aR - aL) * (aL - aLL) >= 1.e-8);
+    ifr = (aR >= 1.e-4) && (aR <= 1. - 1.e-4) && ((aL - aR) * (aR - aRR) >= 1.e-8);
+  }
+  const double idc = 1. / G.dcc[dir][i];
+  const double idm = has_m ? 1. / G.dcc[dir][i - 1] : 1.;
+  const double idp = has_p ? 1. / G.dcc[dir][i + 1] : 1.;
+  const double dL = G.dfl[dir][i], dR = G.dfr[dir][i];
+  double xl[Ly::NVAR], xr[Ly::NVAR];
+#pragma unroll
+  for (int v = 0; v < Ly::NVAR; v++) {
+    const double wl = wl_[v], wr = wr_[v];
+    const double sc = (wr - wl) * idc;
+    const double sm = has_m ? (wl - (double)wll[v]) * idm : 0.;
+    const double sp = has_p ? ((double)whh[v] - wr) * idp : 0.;
+    const bool isa = (v < 3 * NPH) && (v % 3 == 0);
+    const int llim = isa ? (ifl ? S.lim[3] : S.lim[2]) : (ifl ? S.lim[1] : S.lim[0]);
+    const int rlim = isa ? (ifr ? S.lim[3] : S.lim[2]) : (ifr ? S.lim[1] : S.lim[0]);
+    xl[v] = wl + limiter(llim, sc, sm) * dL;
+    xr[v] = wr + (-limiter(rlim, sc, sp)) * dR;
+  }
+  CellState<NPH> L, R;
+  vec_to_prim<MODEL, NPH, DIM>(xl, L);
+  vec_to_prim<MODEL, NPH, DIM>(xr, R);
+#pragma unroll
+  for (int d = DIM; d < 3; d++) { L.u[d] = 0.; R.u[d] = 0.; }
+#pragma unroll
+  for (int k = 0; k < NPH; k++) { verify_correct<MODEL, NPH>(S, L, k); verify_correct<MODEL, NPH>(S, R, k); }
+  if (S.thinc) {
+    thinc<NPH>(L, aL, aR, aLL, false);
+    thinc<NPH>(R, aR, aRR, aL, true);
+  }
+  to_face_frame(dir, 0, L.u);
+  to_face_frame(dir, 0, R.u);
+  double sL, sR;
+  const float smx = speeds_f<NPH>(S, L, R, sL, sR);
+  fulfill_energy<MODEL, NPH>(S, L);
+  fulfill_energy<MODEL, NPH>(S, R);
+  FluxVec<NPH> F;
+  hllc_mixed<MODEL, NPH>(S, L, R, sL, sR, F);
+  from_face_frame(dir, 0, F.q);
+  put_flux_s<MODEL, NPH, DIM>(sF, stride, slot, F);
+  return smx;
+}
+
+template <int MODEL, int NPH, int DIM, bool SAMELIM, int dir, int NT>
+__device__ __forceinline__ void stage32_dir_3d(const StageArgs &A, const int *o, const long long *strd, int tid,
+                                               bool active, int lx, int ly, int lz, const int *ci, long long cidx,
+                                               double *sF, int stride, double (*sAcc)[NT], FluxVec<NPH> &acc,
+                                               float &smx) {
+  if constexpr (dir < DIM) {
+    constexpr int TX = Tile<DIM>::X, TY = Tile<DIM>::Y, TZ = Tile<DIM>::Z;
+    const int nfd = (TX + (dir == 0)) * (TY + (dir == 1)) * (TZ + (dir == 2));
+    for (int f = tid; f < nfd; f += NT) smx = smaxf_(smx, face_fp32s<MODEL, NPH, DIM, SAMELIM, dir>(A, o, strd, f, sF, stride, f));
+    __syncthreads();
+    if (active) {
+      if (dir > 0) unpark_acc<MODEL, NPH, DIM, NT>(sAcc, tid, acc);
+      CellState<NPH> own;
+      load_nc_coeffs_f<NPH>((const float *)A.wn, A.geo.ncell, cidx, own);
+      accumulate_dir<MODEL, NPH, DIM, dir>(acc, sF, stride, 0, lx, ly, lz, ci, A.geo, own);
+      if (dir < DIM - 1) park_acc<MODEL, NPH, DIM, NT>(sAcc, tid, acc);
+    }
+    __syncthreads();
+  }
+}
+
+template <int MODEL, int NPH, int DIM, int STAGE, bool SAMELIM>
+__global__ void __launch_bounds__(Tile<DIM>::X * Tile<DIM>::Y * Tile<DIM>::Z, Tile<DIM>::W)
+k_stage32(const StageArgs A) {
+  using Ly = Layout<MODEL, NPH, DIM>;
+  using TF = TileFaces<DIM>;
+  constexpr int TX = Tile<DIM>::X, TY = Tile<DIM>::Y, TZ = Tile<DIM>::Z;
+  constexpr int NT = TX * TY * TZ;
+  constexpr int MAXF = TF::MAXF;
+  constexpr bool MERGED = (DIM <= 2);
+  constexpr int NF0 = TF::nf(0), NF1 = TF::nf(1);
+  constexpr int NFT = MERGED ? NF0 + NF1 : MAXF;  // LDS flux slots
+  __shared__ double sF[Ly::NFV][NFT];
+  __shared__ double sAcc[MERGED ? 1 : Ly::NACC][NT];
+  __shared__ double sRed[NT / 64];
+
+  const Geometry &G = A.geo;
+  const SchemeParams &S = A.sch;
+  Control *ctl = A.ctl;
+  const int step = A.step;
+  const int tid = threadIdx.x;
+  const bool first_kernel = (STAGE == 1);
+  const bool last_kernel = (STAGE == 2);
+
+  if (step > 0 && ctl->done[(step + 1) & 1]) {
+    if (last_kernel && blockIdx.x == 0 && tid == 0) {
+      ctl->done[step & 1] = 1;
+      ctl->t[step & 1] = ctl->t[(step + 1) & 1];
+    }
+    return;
+  }
+  const double dt = ctl_dt(ctl, G, S.cfl, step);
+  if (first_kernel && blockIdx.x == 0 && tid == 0) ctl->smax[(step + 1) % 3] = 0ull;
+
+  const long long nc = G.ncell;
+  const int nx = G.n[0], ny = G.n[1], nz = G.n[2];
+  const int tgx = (nx + TX - 1) / TX, tgy = (ny + TY - 1) / TY;
+  int b = blockIdx.x;
+  const int bx = b % tgx; b /= tgx;
+  const int by = b % tgy; b /= tgy;
+  const int bz = b;
+  const int o[3] = {bx * TX, by * TY, bz * TZ};
+  const int lx = tid % TX, ly = (tid / TX) % TY, lz = tid / (TX * TY);
+  const int ci[3] = {o[0] + lx, o[1] + ly, o[2] + lz};
+  const bool active = ci[0] < nx && ci[1] < ny && ci[2] < nz;
+  const long long strd[3] = {1, (long long)nx, (long long)nx * ny};
+  const long long cidx = active ? ci[0] + strd[1] * ci[1] + strd[2] * ci[2] : 0;
+
+  const float *__restrict__ Wsrc = (const float *)A.wsrc;
+  const float *__restrict__ Wn = (const float *)A.wn;
+
+  FluxVec<NPH> acc;
+  zero_flux(acc);
+  float smx = 0.f;
+
+  if constexpr (MERGED) {
+    // all faces of the tile (both directions) in one pass, one barrier
+    for (int w = tid; w < NF0 + NF1; w += NT) {
+      if (w < NF0) smx = smaxf_(smx, face_fp32s<MODEL, NPH, DIM, SAMELIM, 0>(A, o, strd, w, &sF[0][0], NFT, w));
+      else if constexpr (DIM > 1)
+        smx = smaxf_(smx, face_fp32s<MODEL, NPH, DIM, SAMELIM, 1>(A, o, strd, w - NF0, &sF[0][0], NFT, w));
+    }
+    __syncthreads();
+    if (active) {
+      CellState<NPH> own;
+      load_nc_coeffs_f<NPH>(Wn, nc, cidx, own);
+      accumulate_dir<MODEL, NPH, DIM, 0>(acc, &sF[0][0], NFT, 0, lx, ly, lz, ci, G, own);
+      if constexpr (DIM > 1) accumulate_dir<MODEL, NPH, DIM, 1>(acc, &sF[0][0], NFT, NF0, lx, ly, lz, ci, G, own);
+    }
+  } else {
+    stage32_dir_3d<MODEL, NPH, DIM, SAMELIM, 0>(A, o, strd, tid, active, lx, ly, lz, ci, cidx, &sF[0][0], NFT, sAcc, acc, smx);
+    stage32_dir_3d<MODEL, NPH, DIM, SAMELIM, 1>(A, o, strd, tid, active, lx, ly, lz, ci, cidx, &sF[0][0], NFT, sAcc, acc, smx);
+    stage32_dir_3d<MODEL, NPH, DIM, SAMELIM, 2>(A, o, strd, tid, active, lx, ly, lz, ci, cidx, &sF[0][0], NFT, sAcc, acc, smx);
+  }
+
+  if (active) {
+    CellState<NPH> own;
+    load_prim_f<MODEL, NPH, DIM>(Wn, nc, cidx, own);
+    const double rhoE = (double)Wn[Ly::IE * nc + cidx] + energy_offset<NPH>(S, own);
+    // transmissive boundary faces on the stage state
+#pragma unroll
+    for (int dir = 0; dir < DIM; dir++) {
+#pragma unroll
+      for (int side = 0; side < 2; side++) {
+        const bool on = side == 0 ? (ci[dir] == 0) : (ci[dir] == G.n[dir] - 1);
+        if (!on) continue;
+        CellState<NPH> L;
+        load_prim_f<MODEL, NPH, DIM>(Wsrc, nc, cidx, L);
+        to_face_frame(dir, side == 0, L.u);
+        double sL, sR;
+        smx = smaxf_(smx, speeds_f<NPH>(S, L, L, sL, sR));
+        fulfill_energy<MODEL, NPH>(S, L);
+        FluxVec<NPH> F;
+        hllc_mixed<MODEL, NPH>(S, L, L, sL, sR, F);
+        from_face_frame(dir, side == 0, F.q);
+        add_face<MODEL, NPH>(acc, F, G.coefA[dir], false, own);
+      }
+    }
+    // cell update in FP64
+    CellState<NPH> sn = own;
+    sn.rho = 0.;
+#pragma unroll
+    for (int k = 0; k < NPH; k++) sn.rho += sn.a[k] * sn.r[k];
+    sn.E = rhoE / sn.rho;
+    FluxVec<NPH> buf;
+    build_cons_fast<MODEL, NPH>(S, sn, buf);
+    buf.E = rhoE;
+    scale_add<MODEL, NPH>(acc, (STAGE == 1) ? 0.5 * dt : dt, buf);
+    scheme_correction<MODEL, NPH>(acc, buf);
+    const double rhoE_new = acc.E;
+    CellState<NPH> s1;
+    build_prim<MODEL, NPH>(S, acc, s1);
+    bool ok = true;
+    if (S.relax) ok = relax_p_fast<MODEL, NPH>(S, s1);
+    correction_energy<MODEL, NPH>(S, s1);
+    if (!ok) atomicOr(&ctl->err, 1u);
+    bool bad = false;
+#pragma unroll
+    for (int k = 0; k < NPH; k++) bad |= !(s1.p[k] == s1.p[k]) || !(s1.r[k] == s1.r[k]);
+    if (bad) atomicOr(&ctl->err, 2u);
+    float *W = (float *)A.wout;
+    store_prim_f<MODEL, NPH, DIM>(W, nc, cidx, s1);
+    if constexpr (STAGE == 2) {
+      CellState<NPH> st = s1;
+      round_state<NPH>(st);
+      W[Ly::IE * nc + cidx] = (float)(rhoE_new - energy_offset<NPH>(S, st));
+    }
+  }
+
+  float m = smx;
+  for (int off = 32; off > 0; off >>= 1) m = smaxf_(m, __shfl_xor(m, off));
+  if ((tid & 63) == 0) sRed[tid >> 6] = (double)m;
+  __syncthreads();
+  if (tid == 0) {
+    double mm = sRed[0];
+#pragma unroll
+    for (int w = 1; w < NT / 64; w++) mm = smax(mm, sRed[w]);
+    if (mm > 0.) atomicMax(&ctl->smax[step % 3], (unsigned long long)__double_as_longlong(mm));
+    if (last_kernel && blockIdx.x == 0) {
+      double t = ((step == 0) ? 0. : ctl->t[(step + 1) & 1]) + dt;
+      ctl->t[step & 1] = t;
+      ctl->steps = step + 1;
+      ctl->dt_last = dt;
+      int done = (ctl->t_final < 0.) ? (step + 1 >= ctl->max_steps) : (t >= (double)(float)ctl->t_final);
+      ctl->done[step & 1] = done;
+    }
+  }
+}
+
+// FP64 device state (nprim fields + E) -> FP32 storage (nprim fields + R)
+template <int MODEL, int NPH, int DIM>
+__global__ void k_to_f32(const StageArgs A, const double *__restrict__ src) {
+  using Ly = Layout<MODEL, NPH, DIM>;
+  const long long nc = A.geo.ncell;
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= nc) return;
+  CellState<NPH> s;
+  load_prim<MODEL, NPH, DIM>(src, nc, i, s);
+  double rho = 0.;
+#pragma unroll
+  for (int k = 0; k < NPH; k++) rho += s.a[k] * s.r[k];
+  const double rhoE = rho * src[Ly::IE * nc + i];
+  float *W = (float *)A.wout;
+  store_prim_f<MODEL, NPH, DIM>(W, nc, i, s);
+  round_state<NPH>(s);
+  W[Ly::IE * nc + i] = (float)(rhoE - energy_offset<NPH>(A.sch, s));
+}
+
+// FP32 storage -> FP64 device layout with the specific total energy E
+template <int MODEL, int NPH, int DIM>
+__global__ void k_from_f32(const StageArgs A, double *__restrict__ dst) {
+  using Ly = Layout<MODEL, NPH, DIM>;
+  const long long nc = A.geo.ncell;
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= nc) return;
+  const float *W = (const float *)A.wn;
+  CellState<NPH> s;
+  load_prim_f<MODEL, NPH, DIM>(W, nc, i, s);
+  double rho = 0.;
+#pragma unroll
+  for (int k = 0; k < NPH; k++) rho += s.a[k] * s.r[k];
+  const double rhoE = (double)W[Ly::IE * nc + i] + energy_offset<NPH>(A.sch, s);
+  store_prim<MODEL, NPH, DIM>(dst, nc, i, s);
+  dst[Ly::IE * nc + i] = rhoE / rho;
+}
+
+template <int MODEL, int NPH, int DIM>
+int launch_t32(const StageArgs &A, int stage, hipStream_t st, double *tmp) {
+  constexpr int NT = Tile<DIM>::X * Tile<DIM>::Y * Tile<DIM>::Z;
+  const int tgx = (A.geo.n[0] + Tile<DIM>::X - 1) / Tile<DIM>::X;
+  const int tgy = (A.geo.n[1] + Tile<DIM>::Y - 1) / Tile<DIM>::Y;
+  const int tgz = (A.geo.n[2] + Tile<DIM>::Z - 1) / Tile<DIM>::Z;
+  dim3 grid((unsigned)(tgx * tgy * tgz)), block(NT);
+  const bool same = A.sch.same_lim != 0;
+  const long long nc = A.geo.ncell;
+  const dim3 g1((unsigned)((nc + 255) / 256)), b1(256);
+  switch (stage) {
+    case 1:
+      if (same) hipLaunchKernelGGL((k_stage32<MODEL, NPH, DIM, 1, true>), grid, block, 0, st, A);
+      else hipLaunchKernelGGL((k_stage32<MODEL, NPH, DIM, 1, false>), grid, block, 0, st, A);
+      break;
+    case 2:
+      if (same) hipLaunchKernelGGL((k_stage32<MODEL, NPH, DIM, 2, true>), grid, block, 0, st, A);
+      else hipLaunchKernelGGL((k_stage32<MODEL, NPH, DIM, 2, false>), grid, block, 0, st, A);
+      break;
+    case 4: hipLaunchKernelGGL((k_to_f32<MODEL, NPH, DIM>), g1, b1, 0, st, A, (const double *)tmp); break;
+    case 5: hipLaunchKernelGGL((k_from_f32<MODEL, NPH, DIM>), g1, b1, 0, st, A, tmp); break;
+    default: return -1;
+  }
+  return (int)hipGetLastError();
+}
+
+template <int MODEL, int NPH>
+int launch_d32(const StageArgs &A, int stage, hipStream_t st, double *tmp) {
+  switch (A.geo.dim) {
+    case 1: return launch_t32<MODEL, NPH, 1>(A, stage, st, tmp);
+    case 2: return launch_t32<MODEL, NPH, 2>(A, stage, st, tmp);
+    case 3: return launch_t32<MODEL, NPH, 3>(A, stage, st, tmp);
+  }
+  return -1;
+}
+
+}  // namespace eco

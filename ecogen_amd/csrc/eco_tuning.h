@@ -22,3 +22,12 @@
 #ifndef ECO_MIN_WAVES_PER_EU
 #define ECO_MIN_WAVES_PER_EU 1
 #endif
+#ifndef ECO_WAVES1
+#define ECO_WAVES1 ECO_MIN_WAVES_PER_EU
+#endif
+#ifndef ECO_WAVES2
+#define ECO_WAVES2 3
+#endif
+#ifndef ECO_WAVES3
+#define ECO_WAVES3 1
+#endif

@@ -37,8 +37,9 @@ def rel_l1(a, b):
 
 
 def parity_quantities(case, state):
-    """The quantities the FP64 parity bar is stated on: mixture density, velocity,
-    mixture pressure and per-phase volume fraction (dict name -> array)."""
+    """The quantities the parity bars are stated on: mixture density, velocity
+    (one vector field: the L1 norm sums the components), mixture pressure and
+    per-phase volume fraction (dict name -> array)."""
     nph = len(case["eos"])
     dim = sum(1 for c in case["cells"] if c > 1)
     out = {}
@@ -50,8 +51,7 @@ def parity_quantities(case, state):
         out["p"] = sum(state[3 * k] * state[3 * k + 2] for k in range(nph))
         for k in range(nph):
             out[f"alpha{k}"] = state[3 * k]
-    for d in range(dim):
-        out["uvw"[d]] = state[3 * nph + d]
+    out["velocity"] = np.concatenate([state[3 * nph + d] for d in range(dim)])
     return out
 
 
