@@ -1,6 +1,6 @@
 """Benchmark of the explicit finite-volume step on one MI355X.
 
-Workload (default): the 2-D Kapila five-equation two-phase model, MUSCL with
+Headline workload: the 2-D Kapila five-equation two-phase model, MUSCL with
 the van Leer limiter, FP64, 4096 x 4096 uniform Cartesian cells: a planar
 Mach 2.4 air shock hitting 32 water cylinders (radii uniform in [20, 80]
 cells, seed-placed without overlap), alpha floor 1e-6.  Synthetic initial
@@ -9,8 +9,13 @@ state, built on the host and uploaded before timing; the timed region runs
 resident in HBM.
 
 Prints one JSON line: cell-updates/s, the HBM roofline of the dominant kernel
-measured with HIP events over the timed region, and a CPU baseline (the FP64
-oracle, OpenMP on the host cores) on a bounded sample of the same workload.
+measured with HIP events over the timed region, a CPU baseline (the FP64
+oracle, OpenMP on the host cores) on a bounded sample of the same workload,
+and the secondary measurements: FP32 storage on the same workload, the
+batched pool (2048 water/air Riemann problems, FP32 storage) against serial
+plain FP64 solves, the served request stream at three arrival rates, the
+1-D first-order shock-tube latency and the 3-D pressure-disequilibrium
+bubble workload.
 """
 from __future__ import annotations
 
@@ -43,7 +48,20 @@ def parse():
     ap.add_argument("--no-fp32s", action="store_true", help="skip the FP32-storage measurement")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain copy kernel of known size (for HBM counter calibration)")
+    ap.add_argument("--batch-cases", type=int, default=2048, help="cases of the batched leg (0: skip)")
+    ap.add_argument("--serial-stride", type=int, default=16,
+                    help="serial plain FP64 comparator: every k-th case of the batch")
+    ap.add_argument("--serve-requests", type=int, default=120, help="requests per arrival rate (0: skip)")
+    ap.add_argument("--serve-rates", type=float, nargs="+", default=[50.0, 200.0, 800.0])
+    ap.add_argument("--c3-n", type=int, default=384)
+    ap.add_argument("--c3-steps", type=int, default=20, help="steps of the 3-D leg (0: skip)")
+    ap.add_argument("--only-main", action="store_true", help="headline workload only")
     return ap.parse_args()
+
+
+def log(msg):
+    """Progress on stderr (stdout carries only the JSON line)."""
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
 
 
 def state_words(case):
@@ -151,6 +169,177 @@ def l1_errors(case, a, b):
     return q
 
 
+def _pct(x, q):
+    return float(np.percentile(np.asarray(x), q)) if len(x) else None
+
+
+def _fp32s_vs_fp64(case, a32, a64):
+    """rel L1 of FP32 storage against FP64 in mixture density, mixture pressure,
+    velocity and volume fractions (one case)."""
+    return l1_errors(case, a32, a64)
+
+
+def leg_batched(args):
+    """2048 independent water/air Riemann problems (70 % 1-D, 30 % 2-D), Kapila
+    MUSCL van Leer, FP32 storage + FP64 relaxation, all admitted to one pool and
+    advanced by persistent batched kernels replayed as hipGraph chunks; against
+    the same cases solved one after another with the plain FP64 path."""
+    from ecogen_amd import Context
+    from ecogen_amd import workloads as W
+    from ecogen_amd.pool import Pool
+    specs = W.batch_specs(args.batch_cases)
+    cases = [W.riemann_case(*sp, precision="fp32s") for sp in specs]
+    cells = np.array([int(np.prod(c["cells"])) for c, _, _ in cases])
+    sample = list(range(0, len(cases), max(1, args.serial_stride)))
+    finish = {}
+    with Pool(hbm_budget=48 << 30, steps_per_chunk=24) as pool:
+        t0 = time.perf_counter()
+        ids = [pool.admit(c, st, t_final=tf) for c, st, tf in cases]
+        if any(i is None for i in ids):
+            raise SystemExit("batched leg: cases exceed the pool budget")
+        while pool.stats().running > 0:
+            pool.advance()
+            for cid in pool.poll(keep=1):
+                finish[cid] = time.perf_counter() - t0
+        pool.poll(keep=0)
+        wall = time.perf_counter() - t0
+        for cid in pool.finished:
+            finish.setdefault(cid, wall)
+        stats = pool.stats()
+        res = [pool.result(i) for i in ids]
+    steps = np.array([r[1].steps for r in res])
+    flags = np.array([r[1].flags for r in res])
+    cu = float((cells * steps).sum())
+    lat = [finish[i] for i in ids]
+    # serial plain FP64 solves of the sample (create, upload, run to t_final, download, destroy)
+    ser_t, ser_cu, l1s, l1_skipped = 0.0, 0.0, [], 0
+    for k in sample:
+        c, st, tf = cases[k]
+        c64 = dict(c, precision="fp64")
+        t1 = time.perf_counter()
+        with Context(c64) as ctx:
+            ctx.set_state(st)
+            s64 = ctx.run(t_final=tf)
+            a64 = ctx.get_state()
+        ser_t += time.perf_counter() - t1
+        ser_cu += cells[k] * s64.steps
+        if s64.steps == steps[k] and not (s64.flags & 1) and not (flags[k] & 1):
+            l1s.append(_fp32s_vs_fp64(c, res[k][0], a64))
+        else:
+            l1_skipped += 1
+    ser_solves = len(sample) / ser_t
+    ser_cus = ser_cu / ser_t
+    l1 = None
+    if l1s:
+        l1 = {"median": {q: float(f"{np.median([e[q] for e in l1s]):.3e}") for q in l1s[0]},
+              "worst": {q: float(f"{max(e[q] for e in l1s):.3e}") for q in l1s[0]},
+              "cases_within_1e-5": int(sum(1 for e in l1s if max(e.values()) <= 1e-5))}
+    return {
+        "cases": len(cases), "cells": int(cells.sum()), "cell_updates": cu, "wall_s": round(wall, 3),
+        "solves_per_s": round(len(cases) / wall, 2), "cell_updates_per_s": round(cu / wall, 1),
+        "latency_s": {"p50": round(_pct(lat, 50), 3), "p99": round(_pct(lat, 99), 3), "max": round(max(lat), 3)},
+        "pool": {"steps_per_chunk": 24, "graph_captures": stats.graph_captures, "steps_launched": stats.steps_launched,
+                 "tiles_launched": stats.tiles_launched},
+        "relax_not_converged_cases": int((flags & 1).astype(bool).sum()),
+        "serial_fp64": {"sample": f"every {args.serial_stride}th case ({len(sample)} cases), one solve at a time",
+                        "solves_per_s": round(ser_solves, 2), "cell_updates_per_s": round(ser_cus, 1)},
+        "speedup_cell_updates_vs_serial_fp64": round((cu / wall) / ser_cus, 2),
+        "speedup_solves_vs_serial_fp64": round((len(cases) / wall) / ser_solves, 2),
+        "fp32s_l1_vs_fp64": l1,
+        "fp32s_l1_cases_compared": len(l1s), "fp32s_l1_cases_skipped": l1_skipped,
+        "storage": "FP32 state (alpha_k, rho_k, p_k, u, reduced energy); FP64 fluxes, update, relaxation",
+    }
+
+
+def leg_served(args):
+    """Poisson request streams (same requests at every rate) through the C++
+    serving loop: continuous batching, admission by a 200 GB HBM budget;
+    against a serial plain FP64 server (measured service time of every request,
+    first-in first-out)."""
+    from ecogen_amd import Context
+    from ecogen_amd import workloads as W
+    from ecogen_amd.pool import Pool
+    base = W.request_stream(1.0, args.serve_requests)
+    reqcases = [W.request_case(kind, prm, precision="fp32s") for _, kind, prm in base]
+    cells = np.array([int(np.prod(c["cells"])) for c, _, _ in reqcases])
+    # serial plain FP64 service times
+    service = []
+    for c, st, tf in reqcases:
+        t1 = time.perf_counter()
+        with Context(dict(c, precision="fp64")) as ctx:
+            ctx.set_state(st)
+            ctx.run(t_final=tf)
+            ctx.get_state()
+        service.append(time.perf_counter() - t1)
+    out = {"requests_per_rate": len(reqcases), "bubble_3d_requests": sum(1 for _, k, _ in base if k == "bubbles"),
+           "hbm_budget_gb": 200, "rates": {}}
+    for lam in args.serve_rates:
+        arrivals = [a / lam for a, _, _ in base]
+        reqs = [{"case": c, "state": st, "t_final": tf, "arrival_s": arr}
+                for (c, st, tf), arr in zip(reqcases, arrivals)]
+        with Pool(hbm_budget=200 << 30, steps_per_chunk=24) as pool:
+            t0 = time.perf_counter()
+            res, _ = pool.serve(reqs)
+            wall = time.perf_counter() - t0
+        tts = res[:, 1] - np.array(arrivals)
+        cu = float((cells * res[:, 2]).sum())
+        # FIFO serial FP64 server on the same arrivals
+        fin, ser = 0.0, []
+        for arr, sv in zip(arrivals, service):
+            fin = max(fin, arr) + sv
+            ser.append(fin - arr)
+        out["rates"][str(int(lam))] = {
+            "tts_s": {"p50": round(_pct(tts, 50), 3), "p99": round(_pct(tts, 99), 3)},
+            "solves_per_s": round(len(reqs) / wall, 2), "cell_updates_per_s": round(cu / wall, 1),
+            "wall_s": round(wall, 3),
+            "serial_fp64_tts_s": {"p50": round(_pct(ser, 50), 3), "p99": round(_pct(ser, 99), 3)},
+            "serial_fp64_solves_per_s": round(len(reqs) / fin, 2),
+        }
+    return out
+
+
+def leg_latency_1d():
+    """1-D Kapila first-order water/air shock tube, 1000 cells, FP64, to 240 us:
+    time to solution of one solve (upload, run, download) and per step."""
+    from ecogen_amd import Context
+    from ecogen_amd import workloads as W
+    case, st = W.shock_tube_1d()
+    with Context(case) as ctx:  # warm-up (module load, first launches)
+        ctx.set_state(st)
+        ctx.run(t_final=case["t_final"])
+    t0 = time.perf_counter()
+    with Context(case) as ctx:
+        ctx.set_state(st)
+        s = ctx.run(t_final=case["t_final"])
+        ctx.get_state()
+    wall = time.perf_counter() - t0
+    return {"workload": "1-D Kapila FO shock tube, 1000 cells, FP64, t=240 us", "steps": s.steps,
+            "time_to_solution_s": round(wall, 4), "us_per_step": round(wall / s.steps * 1e6, 2),
+            "cell_updates_per_s": round(1000 * s.steps / wall, 1)}
+
+
+def leg_c3(args):
+    """3-D pressure-disequilibrium model with stiff relaxation, MUSCL minmod,
+    64 air bubbles in water at 5e7 Pa: FP64 and FP32 storage."""
+    from ecogen_amd import workloads as W
+    n = args.c3_n
+    s = n / 384.0
+    case, st = W.bubbles_3d(n=n, rmin=4.0 * s, rmax=24.0 * s)
+    sub = argparse.Namespace(warmup=2, steps=args.c3_steps)
+    r64 = run_precision(case, st, sub, "fp64")
+    r32 = run_precision(case, st, sub, "fp32s")
+    del st
+    ncell = n ** 3
+    return {"workload": f"3-D MultiP + relaxation, MUSCL minmod, {n}^3, 64 bubbles", "steps": args.c3_steps,
+            "fp64": {"cell_updates_per_s": round(r64["value"], 1), "ms_per_step": round(r64["wall"] / args.c3_steps * 1e3, 3),
+                     "roofline": roofline(case, r64, 8)},
+            "fp32s": {"cell_updates_per_s": round(r32["value"], 1), "ms_per_step": round(r32["wall"] / args.c3_steps * 1e3, 3),
+                      "roofline": roofline(case, r32, 4)},
+            "fp32s_speedup": round(r32["value"] / r64["value"], 3),
+            "fp32s_l1_vs_fp64": {k: float(f"{v:.3e}") for k, v in l1_errors(case, r32["state"], r64["state"]).items()},
+            "cells": ncell}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -159,9 +348,11 @@ def main():
         sys.exit(2)
     from ecogen_amd import workloads as W
 
+    log(f"headline workload {args.n}^2, {args.steps} steps")
     case, st = W.shock_cylinders_2d(n=args.n, steps=args.steps)
     ncell = int(np.prod(case["cells"]))
     r64 = run_precision(case, st, args, "fp64")
+    log(f"fp64: {r64['wall'] / args.steps * 1e3:.3f} ms/step")
     roof = roofline(case, r64, 8)
     roof["traffic"] = traffic_from_profiles(f"kapila2d_o2_vanleer_fp64_{args.n}")
     fp32s = None
@@ -175,6 +366,22 @@ def main():
     del st
     if args.calibrate_copy:
         copy_calibration()
+    extra = {}
+    if not args.only_main:
+        log("1-D shock-tube latency")
+        extra["latency_1d"] = leg_latency_1d()
+        if args.batch_cases > 0:
+            log(f"batched pool: {args.batch_cases} cases")
+            extra["batched"] = leg_batched(args)
+            log(f"batched: {extra['batched']['wall_s']} s")
+        if args.serve_requests > 0:
+            log(f"served stream: {args.serve_requests} requests x {len(args.serve_rates)} rates")
+            extra["served"] = leg_served(args)
+        if args.c3_steps > 0:
+            log(f"3-D bubbles {args.c3_n}^3, {args.c3_steps} steps")
+            extra["c3"] = leg_c3(args)
+    if not args.no_cpu:
+        log("CPU baseline")
     cpu = None if args.no_cpu else cpu_baseline(args)
     line = {
         "metric": METRIC, "value": round(r64["value"], 1), "unit": "cell-updates/s", "n_gpus": 1,
@@ -189,6 +396,7 @@ def main():
         "cpu_baseline": cpu,
         "fp32_storage": fp32s,
     }
+    line.update(extra)
     print(json.dumps(line))
 
 

@@ -8,7 +8,8 @@ timing on the GPU:
     python -m ecogen_amd.autotune apply R.json      # write the fastest into csrc/eco_tuning.h
 
 A variant is a set of build-time macros of ``csrc/eco_tuning.h`` (tile
-shape per dimension, minimum waves per SIMD requested from the compiler).
+shape per dimension, minimum waves per SIMD requested from the compiler,
+compile-time or runtime limiter form).
 Each is timed on the stage kernels of a fixed workload with HIP events
 (eco_time_steps), after a parity check of its result against the default
 build's result on the same state.
@@ -32,21 +33,22 @@ TUNE_DIR = PKG / "_tune"
 CSRC = PKG / "csrc"
 
 SPACE = {
-    2: {"tiles": [(64, 4), (32, 8), (16, 16), (128, 2)], "waves": [1, 2, 3]},
-    3: {"tiles": [(16, 4, 4), (32, 4, 2), (8, 8, 4), (64, 2, 2)], "waves": [1, 2]},
+    2: {"tiles": [(32, 8), (64, 4)], "waves": [2, 3, 4], "limspec": [0, 1]},
+    3: {"tiles": [(16, 4, 4), (32, 4, 2)], "waves": [1, 2], "limspec": [0, 1]},
 }
 
 
 def variants(dim):
     out = []
     sp = SPACE[dim]
-    for t, w in itertools.product(sp["tiles"], sp["waves"]):
+    for t, w, ls in itertools.product(sp["tiles"], sp["waves"], sp["limspec"]):
         if dim == 2:
-            name = f"d2_t{t[0]}x{t[1]}_w{w}"
-            defs = {"ECO_TILE2_X": t[0], "ECO_TILE2_Y": t[1], "ECO_WAVES2": w}
+            name = f"d2_t{t[0]}x{t[1]}_w{w}_ls{ls}"
+            defs = {"ECO_TILE2_X": t[0], "ECO_TILE2_Y": t[1], "ECO_WAVES2": w, "ECO_LIMSPEC2": ls}
         else:
-            name = f"d3_t{t[0]}x{t[1]}x{t[2]}_w{w}"
-            defs = {"ECO_TILE3_X": t[0], "ECO_TILE3_Y": t[1], "ECO_TILE3_Z": t[2], "ECO_WAVES3": w}
+            name = f"d3_t{t[0]}x{t[1]}x{t[2]}_w{w}_ls{ls}"
+            defs = {"ECO_TILE3_X": t[0], "ECO_TILE3_Y": t[1], "ECO_TILE3_Z": t[2], "ECO_WAVES3": w,
+                    "ECO_LIMSPEC3": ls}
         out.append((name, defs))
     return out
 

@@ -15,6 +15,7 @@
 
 #include "../../include/ecogen.h"
 #include "eco_launch.h"
+#include "eco_host.h"
 
 using namespace eco;
 
@@ -65,7 +66,7 @@ static void lower_eos(const eco_eos &e, EosParams &q) {
   q.pfloor = (e.type == ECO_EOS_SG) ? -(1. - 1.e-15) * e.pinf + 1.e-15 : 1.e-15;
 }
 
-static int validate(const eco_config *c) {
+int eco::validate_config(const eco_config *c) {
   if (!c) return ECO_ERR_INVALID;
   if (c->model < 0 || c->model > 2) return ECO_ERR_INVALID;
   if (c->order != 1 && c->order != 2) return ECO_ERR_INVALID;
@@ -101,17 +102,10 @@ static int validate(const eco_config *c) {
   return ECO_OK;
 }
 
-extern "C" int eco_create(const eco_config *cfg, void *hip_stream, eco_ctx **out) {
-  if (!out) return ECO_ERR_INVALID;
-  *out = nullptr;
-  int rc = validate(cfg);
-  if (rc) return rc;
-  eco_ctx *x = new (std::nothrow) eco_ctx();
-  if (!x) return ECO_ERR_NOMEM;
-  x->cfg = *cfg;
-  const eco_config &c = *cfg;
+void eco::lower_config(const eco_config &c, Lowered &L) {
   // geometry (src/Meshes/MeshCartesian.cpp:180-262, 300-397)
-  Geometry &G = x->geo;
+  Geometry &G = L.geo;
+  std::memset(&G, 0, sizeof(G));
   int dim = 0;
   for (int d = 0; d < 3; d++) { G.n[d] = c.n[d]; dim += (c.n[d] > 1); }
   G.dim = dim;
@@ -128,25 +122,24 @@ extern "C" int eco_create(const eco_config *cfg, void *hip_stream, eco_ctx **out
   // per-direction distance tables (cell-centre and face positions as the reference builds them)
   size_t ntab = 0;
   for (int d = 0; d < 3; d++) ntab += 3 * (size_t)std::max(c.n[d], 1);
-  std::vector<double> tab(ntab, 0.);
-  size_t off[3];
+  L.tab.assign(ntab, 0.);
   size_t o = 0;
   for (int d = 0; d < 3; d++) {
-    off[d] = o;
+    L.off[d] = o;
     int n = c.n[d];
     for (int i = 0; i + 1 < n; i++) {
       double xi = (i + 0.5) * dd[d], xj = (i + 1 + 0.5) * dd[d];
       double xf = xi + 0.5 * dd[d];
       double v = xi - xj;
-      tab[o + i] = std::sqrt(v * v);                   // |x_i - x_{i+1}| (Element::distance)
+      L.tab[o + i] = std::sqrt(v * v);                   // |x_i - x_{i+1}| (Element::distance)
       double a = xf - xi, b = xf - xj;
-      tab[o + n + i] = std::sqrt(a * a);               // face - left cell (Face::distance)
-      tab[o + 2 * n + i] = std::sqrt(b * b);           // face - right cell
+      L.tab[o + n + i] = std::sqrt(a * a);               // face - left cell (Face::distance)
+      L.tab[o + 2 * n + i] = std::sqrt(b * b);           // face - right cell
     }
     o += 3 * (size_t)n;
   }
   // scheme
-  SchemeParams &S = x->sch;
+  SchemeParams &S = L.sch;
   std::memset(&S, 0, sizeof(S));
   S.model = c.model;
   S.nph = c.nphase;
@@ -160,18 +153,57 @@ extern "C" int eco_create(const eco_config *cfg, void *hip_stream, eco_ctx **out
   S.cfl = c.cfl;
   S.prec = c.precision;
   for (int k = 0; k < c.nphase && k < kMaxPhases; k++) lower_eos(c.eos[k], S.eos[k]);
-  x->nprim = (c.model == ECO_MODEL_EULER) ? 2 + dim : 3 * c.nphase + dim;
+  L.nprim = (c.model == ECO_MODEL_EULER) ? 2 + dim : 3 * c.nphase + dim;
+  L.es = (c.precision == ECO_FP32S) ? sizeof(float) : sizeof(double);
+}
+
+void eco::bind_tables(const eco_config &c, Lowered &L, double *dev_tables) {
+  for (int d = 0; d < 3; d++) {
+    int n = c.n[d];
+    L.geo.dcc[d] = dev_tables + L.off[d];
+    L.geo.dfl[d] = dev_tables + L.off[d] + n;
+    L.geo.dfr[d] = dev_tables + L.off[d] + 2 * n;
+  }
+}
+
+// host layout (3N+4 fields, u,v,w) -> device layout (nprim fields)
+void eco::host_field_map(const eco_config &c, int dim, std::vector<int> &map) {
+  map.clear();
+  if (c.model == ECO_MODEL_EULER) {
+    map.push_back(1); map.push_back(2);
+    for (int d = 0; d < dim; d++) map.push_back(3 + d);
+  } else {
+    for (int k = 0; k < c.nphase; k++) { map.push_back(3 * k); map.push_back(3 * k + 1); map.push_back(3 * k + 2); }
+    for (int d = 0; d < dim; d++) map.push_back(3 * c.nphase + d);
+  }
+}
+
+extern "C" int eco_create(const eco_config *cfg, void *hip_stream, eco_ctx **out) {
+  if (!out) return ECO_ERR_INVALID;
+  *out = nullptr;
+  int rc = validate_config(cfg);
+  if (rc) return rc;
+  eco_ctx *x = new (std::nothrow) eco_ctx();
+  if (!x) return ECO_ERR_NOMEM;
+  x->cfg = *cfg;
+  const eco_config &c = *cfg;
+  Lowered L;
+  lower_config(c, L);
+  x->geo = L.geo;
+  x->sch = L.sch;
+  x->nprim = L.nprim;
 
   // stream
   if (hip_stream) { x->stream = (hipStream_t)hip_stream; x->own_stream = 0; }
   else { x->stream = nullptr; x->own_stream = 0; }
 
   // device memory
-  size_t nc = (size_t)G.ncell;
+  const size_t nc = (size_t)x->geo.ncell;
+  const size_t ntab = L.tab.size();
   hipError_t e = hipSuccess;
   x->buf[0] = x->buf[1] = x->bstar = x->tables = x->tmp64 = nullptr;
   x->ctl = nullptr;
-  x->es = (c.precision == ECO_FP32S) ? sizeof(float) : sizeof(double);
+  x->es = L.es;
   e = hipMalloc(&x->buf[0], nc * (x->nprim + 1) * x->es);
   if (e == hipSuccess) e = hipMalloc(&x->buf[1], nc * (x->nprim + 1) * x->es);
   if (e == hipSuccess && c.order == 2) e = hipMalloc(&x->bstar, nc * x->nprim * x->es);
@@ -179,14 +211,10 @@ extern "C" int eco_create(const eco_config *cfg, void *hip_stream, eco_ctx **out
   if (e == hipSuccess) e = hipMalloc(&x->tables, ntab * sizeof(double));
   if (e == hipSuccess) e = hipMalloc(&x->ctl, sizeof(Control));
   if (e != hipSuccess) { eco_destroy(x); return ECO_ERR_NOMEM; }
-  e = hipMemcpy(x->tables, tab.data(), ntab * sizeof(double), hipMemcpyHostToDevice);
+  e = hipMemcpy(x->tables, L.tab.data(), ntab * sizeof(double), hipMemcpyHostToDevice);
   if (e != hipSuccess) { eco_destroy(x); return ECO_ERR_HIP; }
-  for (int d = 0; d < 3; d++) {
-    int n = c.n[d];
-    G.dcc[d] = x->tables + off[d];
-    G.dfl[d] = x->tables + off[d] + n;
-    G.dfr[d] = x->tables + off[d] + 2 * n;
-  }
+  bind_tables(c, L, x->tables);
+  x->geo = L.geo;
   x->step = 0;
   x->have_state = 0;
   *out = x;
@@ -213,18 +241,7 @@ static StageArgs make_args(eco_ctx *x, int step) {
   return A;
 }
 
-// host layout (3N+4 fields, u,v,w) -> device layout (nprim fields)
-static void host_to_dev_fields(const eco_ctx *x, std::vector<int> &map) {
-  const eco_config &c = x->cfg;
-  map.clear();
-  if (c.model == ECO_MODEL_EULER) {
-    map.push_back(1); map.push_back(2);
-    for (int d = 0; d < x->geo.dim; d++) map.push_back(3 + d);
-  } else {
-    for (int k = 0; k < c.nphase; k++) { map.push_back(3 * k); map.push_back(3 * k + 1); map.push_back(3 * k + 2); }
-    for (int d = 0; d < x->geo.dim; d++) map.push_back(3 * c.nphase + d);
-  }
-}
+static void host_to_dev_fields(const eco_ctx *x, std::vector<int> &map) { host_field_map(x->cfg, x->geo.dim, map); }
 
 // device buffer the FP64 device-layout input is staged in
 static double *staging(eco_ctx *x) { return x->tmp64 ? x->tmp64 : x->buf[0]; }
@@ -281,10 +298,11 @@ static int read_status(eco_ctx *x, eco_status *st, Control *hc) {
   if (hc) *hc = h;
   if (st) {
     st->steps = h.steps;
-    st->t = (h.steps > 0) ? h.t[(h.steps - 1) & 1] : 0.;
+    st->t = (h.steps > 0) ? h.t_now : 0.;
     st->dt_last = h.dt_last;
     double dtn = 1.e-15;
     if (h.steps > 0) {
+      // the context's step k runs at launch slot k
       unsigned long long b = h.smax[(h.steps - 1) % 3];
       double sm;
       std::memcpy(&sm, &b, sizeof(sm));
@@ -293,7 +311,7 @@ static int read_status(eco_ctx *x, eco_status *st, Control *hc) {
     }
     st->dt_next = dtn;
     st->flags = h.err;
-    st->done = (h.steps > 0) ? h.done[(h.steps - 1) & 1] : 0;
+    st->done = h.finished;
   }
   return ECO_OK;
 }
@@ -352,12 +370,12 @@ static int launch_one(eco_ctx *x) {
 }
 
 static int set_target(eco_ctx *x, int max_steps, double t_final) {
-  // Control::t_final / max_steps / done[] are plain fields written before the launches.
-  struct { double t_final; int max_steps; } tgt = {t_final, max_steps};
-  hipError_t e = hipMemcpyAsync(&x->ctl->t_final, &tgt.t_final, sizeof(double), hipMemcpyHostToDevice, x->stream);
-  if (e == hipSuccess) e = hipMemcpyAsync(&x->ctl->max_steps, &tgt.max_steps, sizeof(int), hipMemcpyHostToDevice, x->stream);
+  // Control::t_final, max_steps, done[] and finished are plain fields written before the launches.
+  hipError_t e = hipMemcpyAsync(&x->ctl->t_final, &t_final, sizeof(double), hipMemcpyHostToDevice, x->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(&x->ctl->max_steps, &max_steps, sizeof(int), hipMemcpyHostToDevice, x->stream);
   int zero[2] = {0, 0};
   if (e == hipSuccess) e = hipMemcpyAsync(&x->ctl->done[0], zero, sizeof(zero), hipMemcpyHostToDevice, x->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(&x->ctl->finished, zero, sizeof(int), hipMemcpyHostToDevice, x->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(x->stream);
   return hipcode(e);
 }

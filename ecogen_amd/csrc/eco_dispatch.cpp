@@ -1,5 +1,6 @@
 // Model dispatch of the stage kernels.
 #include "eco_launch.h"
+#include "eco_tuning.h"
 
 namespace eco {
 __global__ void k_copy(double *__restrict__ d, const double *__restrict__ s, unsigned long long n) {
@@ -32,6 +33,30 @@ int launch_stage(const StageArgs &A, int stage, hipStream_t st) {
   if (A.sch.model == MODEL_MULTIP) return launch_multip(A, stage, st);
   return -1;
 }
+int batch_euler(const BatchLaunch &L, hipStream_t st);
+int batch_kapila(const BatchLaunch &L, hipStream_t st);
+int batch_multip(const BatchLaunch &L, hipStream_t st);
+int batch_kapila32(const BatchLaunch &L, hipStream_t st);
+int batch_multip32(const BatchLaunch &L, hipStream_t st);
+
+int launch_batch(int model, int prec, const BatchLaunch &L, hipStream_t st) {
+  if (prec == PREC_FP32S) {
+    if (model == MODEL_KAPILA) return batch_kapila32(L, st);
+    if (model == MODEL_MULTIP) return batch_multip32(L, st);
+    return -1;
+  }
+  if (model == MODEL_EULER) return batch_euler(L, st);
+  if (model == MODEL_KAPILA) return batch_kapila(L, st);
+  if (model == MODEL_MULTIP) return batch_multip(L, st);
+  return -1;
+}
+
+void tile_shape(int dim, int t[3]) {
+  if (dim == 1) { t[0] = ECO_TILE1_X; t[1] = 1; t[2] = 1; }
+  else if (dim == 2) { t[0] = ECO_TILE2_X; t[1] = ECO_TILE2_Y; t[2] = 1; }
+  else { t[0] = ECO_TILE3_X; t[1] = ECO_TILE3_Y; t[2] = ECO_TILE3_Z; }
+}
+
 int launch_convert(const StageArgs &A, int to_f32, double *tmp, hipStream_t st) {
   const int stage = to_f32 ? 4 : 5;
   if (A.sch.model == MODEL_KAPILA) return launch_kapila32(A, stage, st, tmp);

@@ -17,6 +17,33 @@ struct StageArgs {
   int step;            // 0-based step index of the solve
 };
 
+// One tile of one case in a pool's persistent launch.
+struct TileRef {
+  int c;    // case slot in the class's argument table
+  int blk;  // tile index within the case (x fastest)
+};
+
+// Device-resident descriptor of one kernel class of a pool: per case four
+// argument blocks ([slot parity][first/second kernel of the step]), the tile
+// table and its length.
+struct BatchDesc {
+  const StageArgs *args;
+  const TileRef *tiles;
+  int ntiles;
+};
+
+// Launch of a pool's persistent stage kernel for one class.
+struct BatchLaunch {
+  int dim, stage, same;   // class within the model (stage 0, 1, 2 as launch_stage)
+  const BatchDesc *desc;  // device pointer
+  int slot, grid;
+  int *occupancy;         // non-null: report workgroups per CU for this kernel, no launch
+};
+int launch_batch(int model, int prec, const BatchLaunch &L, hipStream_t stream);
+
+// Tile shape (cells per workgroup along x, y, z) of the stage kernels for a dimension.
+void tile_shape(int dim, int t[3]);
+
 // stage: 0 first-order step, 1 MUSCL predictor, 2 MUSCL corrector, 3 initial total energy.
 // Returns a hipError_t value (0 on success) or -1 for an unsupported configuration.
 int launch_stage(const StageArgs &A, int stage, hipStream_t stream);

@@ -54,20 +54,25 @@ struct SchemeParams {
   EosParams eos[kMaxPhases];
 };
 
-// Device-resident control block of one solve (one per context / batch case).
-// dt handling reproduces Run::solver (src/Run.cpp:204-277): dt of step 0 is
-// 1e-15, dt of step k is cfl * dtMax(k-1); dtMax is min over all Riemann
-// problems of lcfl/|s|, kept here as max |s| (division is monotonic).
+// Device-resident control block of one solve (one per context / pool case).
+// dt handling reproduces Run::solver (src/Run.cpp:204-277): dt of a case's
+// first step is 1e-15, dt of step k is cfl * dtMax(k-1); dtMax is min over
+// all Riemann problems of lcfl/|s|, kept here as max |s| (division is
+// monotonic).  Entries are indexed by the launch slot (ping-pong slot&1, ring
+// slot%3), so a case only needs to see consecutive slots; the number of steps
+// a case has taken is cnt[], independent of the slot it started at.
 struct Control {
-  unsigned long long smax[3];  // max wave speed bits, slot k%3 for step k
-  double t[2];                 // physical time after step k in t[k&1]
-  int done[2];                 // run finished after step k in done[k&1]
-  int steps;                   // steps taken
+  unsigned long long smax[3];  // max wave speed bits, ring entry slot%3
+  double t[2];                 // physical time after the step of slot s in t[s&1]
+  int done[2];                 // target reached after the step of slot s in done[s&1]
+  int cnt[2];                  // steps taken after the step of slot s in cnt[s&1]
+  int steps;                   // steps taken (host view)
+  int finished;                // target reached (host view; cleared by the host)
   unsigned int err;            // soft-error flags (ECO_WARN_*)
-  double dt_last;              // dt used by the last step taken
-  double t_final;              // <0: iteration mode
   int max_steps;               // iteration-mode step count
-  int pad;
+  double dt_last;              // dt used by the last step taken
+  double t_now;                // physical time after the last step taken
+  double t_final;              // <0: iteration mode
 };
 
 }  // namespace eco

@@ -60,6 +60,40 @@ ECO_D double eos_fix_p(const EosParams &q, double p) {
 // ----------------------------------------------------------------------------
 // Limiters: src/Ordre2/Limiter*.cpp:49-59
 // ----------------------------------------------------------------------------
+// Branch-free forms (every operand computed, the result selected), so that
+// the limited slopes of all variables of a face schedule as one basic block.
+// The selected value is the reference's bit for bit.
+template <int LIM>
+ECO_D double limiter_k(double s1, double s2) {
+  const double prod = s1 * s2;
+  if constexpr (LIM == LIM_MINMOD || LIM == LIM_THINC) {
+    const double m = smin(fabs(s1), fabs(s2));
+    return (prod > 1e-9) ? ((s1 < 0) ? -m : m) : 0.;
+  } else if constexpr (LIM == LIM_VANLEER) {
+    const double sum = s1 + s2;
+    const double q = 2. * prod / sum;
+    return ((fabs(sum) > 1e-6) && (prod > 1e-6)) ? q : 0.;
+  } else if constexpr (LIM == LIM_VANALBADA) {
+    const double sum = s1 + s2;
+    const double q = prod * sum / (s1 * s1 + s2 * s2);
+    return ((fabs(s1) > 1e-6) && (fabs(s2) > 1e-6) && (fabs(sum) > 1e-6) && (prod > 1e-6)) ? q : 0.;
+  } else if constexpr (LIM == LIM_MC) {
+    double m = smin(2. * fabs(s1), 2. * fabs(s2));
+    m = smin(m, 0.5 * (fabs(s1) + fabs(s2)));
+    const double slope = (prod > 1e-9) ? m : 0.;
+    return (s1 < 0.) ? -slope : slope;
+  } else if constexpr (LIM == LIM_SUPERBEE) {
+    const double m = smax(smin(2. * fabs(s1), fabs(s2)), smin(fabs(s1), 2. * fabs(s2)));
+    const double slope = (prod > 1e-6) ? m : 0.;
+    return (s1 < 0.) ? -slope : slope;
+  } else {
+    return 0.;
+  }
+}
+
+// runtime limiter choice (LimiterMinmod/VanLeer/VanAlbada/Mc/Superbee
+// ::limiteSlope, src/Ordre2/Limiter*.cpp:49-59; THINC limits with minmod),
+// with the reference's branches (the division only where it is used)
 ECO_D double limiter(int lim, double s1, double s2) {
   double slope = 0., prod = s1 * s2;
   switch (lim) {

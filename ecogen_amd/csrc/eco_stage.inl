@@ -36,9 +36,13 @@ struct Layout {
 // the autotuner (ecogen_amd/autotune.py writes eco_tuning.h).
 #include "eco_tuning.h"
 template <int DIM> struct Tile;
-template <> struct Tile<1> { static constexpr int X = ECO_TILE1_X, Y = 1, Z = 1, W = ECO_WAVES1; };
-template <> struct Tile<2> { static constexpr int X = ECO_TILE2_X, Y = ECO_TILE2_Y, Z = 1, W = ECO_WAVES2; };
-template <> struct Tile<3> { static constexpr int X = ECO_TILE3_X, Y = ECO_TILE3_Y, Z = ECO_TILE3_Z, W = ECO_WAVES3; };
+template <> struct Tile<1> { static constexpr int X = ECO_TILE1_X, Y = 1, Z = 1, W = ECO_WAVES1, LS = ECO_LIMSPEC1; };
+template <> struct Tile<2> {
+  static constexpr int X = ECO_TILE2_X, Y = ECO_TILE2_Y, Z = 1, W = ECO_WAVES2, LS = ECO_LIMSPEC2;
+};
+template <> struct Tile<3> {
+  static constexpr int X = ECO_TILE3_X, Y = ECO_TILE3_Y, Z = ECO_TILE3_Z, W = ECO_WAVES3, LS = ECO_LIMSPEC3;
+};
 
 template <int DIM>
 struct TileFaces {
@@ -265,12 +269,58 @@ __device__ __forceinline__ void load_nc_coeffs(const double *__restrict__ W, lon
   }
 }
 
-__device__ __forceinline__ double ctl_dt(const Control *ctl, const Geometry &G, double cfl, int step) {
-  if (step == 0) return 1.e-15;
-  unsigned long long bits = __atomic_load_n(&ctl->smax[(step - 1) % 3], __ATOMIC_RELAXED);
-  double sm = __longlong_as_double((long long)bits);
-  double dtMax = (sm > 0.) ? smin(1.e10, G.lcfl / sm) : 1.e10;
-  return cfl * dtMax;
+// Step bookkeeping of Run::solver (src/Run.cpp:220-272) for one launch slot.
+struct StepInfo {
+  double dt;
+  int cnt;  // steps the case has taken before this one
+};
+
+// Returns false when the case already reached its target (the launch is a
+// no-op for it; its last kernel carries time, count and flag to this slot).
+__device__ __forceinline__ bool step_begin(Control *ctl, const Geometry &G, double cfl, int slot, int blk,
+                                           bool first_kernel, bool last_kernel, StepInfo &si) {
+  const int prv = (slot + 1) & 1;
+  const int cnt = ctl->cnt[prv];
+  if (ctl->done[prv]) {
+    if (last_kernel && blk == 0 && threadIdx.x == 0) {
+      const int cur = slot & 1;
+      ctl->done[cur] = 1;
+      ctl->t[cur] = ctl->t[prv];
+      ctl->cnt[cur] = cnt;
+    }
+    return false;
+  }
+  double dt = 1.e-15;  // m_dt of Run::Run (src/Run.cpp:43)
+  if (cnt > 0) {
+    unsigned long long bits = __atomic_load_n(&ctl->smax[(slot + 2) % 3], __ATOMIC_RELAXED);
+    double sm = __longlong_as_double((long long)bits);
+    double dtMax = (sm > 0.) ? smin(1.e10, G.lcfl / sm) : 1.e10;
+    dt = cfl * dtMax;
+  }
+  if (first_kernel && blk == 0 && threadIdx.x == 0) ctl->smax[(slot + 1) % 3] = 0ull;
+  si.dt = dt;
+  si.cnt = cnt;
+  return true;
+}
+
+// Called by one thread per tile with the tile's largest wave speed m.
+__device__ __forceinline__ void step_end(Control *ctl, double m, int slot, int blk, bool last_kernel,
+                                         const StepInfo &si) {
+  if (m > 0.) atomicMax(&ctl->smax[slot % 3], (unsigned long long)__double_as_longlong(m));
+  if (last_kernel && blk == 0) {
+    const int cur = slot & 1, prv = cur ^ 1;
+    const double t = ((si.cnt == 0) ? 0. : ctl->t[prv]) + si.dt;
+    const int n = si.cnt + 1;
+    ctl->t[cur] = t;
+    ctl->cnt[cur] = n;
+    ctl->steps = n;
+    ctl->dt_last = si.dt;
+    ctl->t_now = t;
+    // iteration or physical-time control (src/Run.cpp:239-244, tFinal is a float)
+    const int done = (ctl->t_final < 0.) ? (n >= ctl->max_steps) : (t >= (double)(float)ctl->t_final);
+    ctl->done[cur] = done;
+    if (done) ctl->finished = 1;
+  }
 }
 
 template <int MODEL, int NPH, int DIM>
@@ -327,6 +377,82 @@ __device__ __forceinline__ void accumulate_dir(FluxVec<NPH> &acc, const double *
     get_flux_s<MODEL, NPH, DIM>(sF, stride, base + fhi, F);
     add_face<MODEL, NPH>(acc, F, G.coefA[dir], false, own);
   }
+}
+
+// Limited extrapolation of every variable to the face (CellInterfaceO2::
+// computeSlopes, CellO2::computeLocalSlopes, CellO2.cpp:120-175): slopes at
+// i-1/2, i+1/2, i+3/2, the low cell limited from (i+1/2, i-1/2), the high
+// cell from (i+1/2, i+3/2).  LV/LA: compile-time limiter of the non-alpha /
+// alpha variables, or -1 for the runtime choice from S.lim and the interface
+// flags.  RECIP: slopes through reciprocals of the cell distances (the
+// FP32-storage form; the FP64 form divides like the reference).
+template <int MODEL, int NPH, int DIM, int LV, int LA, bool RECIP, typename TW>
+__device__ __forceinline__ void muscl_vars(const SchemeParams &S, const TW *wl_, const TW *wr_, const TW *wll,
+                                           const TW *whh, bool has_m, bool has_p, double dcc, double dcm, double dcp,
+                                           double dL, double dR, bool ifl, bool ifr, double *xl, double *xr) {
+  using Ly = Layout<MODEL, NPH, DIM>;
+  double idc = 0., idm = 0., idp = 0.;
+  if constexpr (RECIP) { idc = 1. / dcc; idm = 1. / dcm; idp = 1. / dcp; }
+#pragma unroll
+  for (int v = 0; v < Ly::NVAR; v++) {
+    const double wl = wl_[v], wr = wr_[v];
+    double sc, sm, sp;
+    if constexpr (RECIP) {
+      sc = (wr - wl) * idc;
+      sm = has_m ? (wl - (double)wll[v]) * idm : 0.;
+      sp = has_p ? ((double)whh[v] - wr) * idp : 0.;
+    } else {
+      sc = (wr - wl) / dcc;
+      sm = has_m ? (wl - (double)wll[v]) / dcm : 0.;
+      sp = has_p ? ((double)whh[v] - wr) / dcp : 0.;
+    }
+    const bool isa = (MODEL != MODEL_EULER) && (v < 3 * NPH) && (v % 3 == 0);
+    double gl, gr;
+    if constexpr (LV >= 0) {
+      gl = isa ? limiter_k<LA>(sc, sm) : limiter_k<LV>(sc, sm);
+      gr = isa ? limiter_k<LA>(sc, sp) : limiter_k<LV>(sc, sp);
+    } else {
+      const int llim = isa ? (ifl ? S.lim[3] : S.lim[2]) : (ifl ? S.lim[1] : S.lim[0]);
+      const int rlim = isa ? (ifr ? S.lim[3] : S.lim[2]) : (ifr ? S.lim[1] : S.lim[0]);
+      gl = limiter(llim, sc, sm);
+      gr = limiter(rlim, sc, sp);
+    }
+    xl[v] = wl + gl * dL;
+    xr[v] = wr + (-gr) * dR;
+  }
+}
+
+// one limiter for the non-alpha variables and one for alpha (no interface
+// detection): compile-time limiter pair for the common choices
+template <int MODEL, int NPH, int DIM, bool SAMELIM, bool RECIP, typename TW>
+__device__ __forceinline__ void muscl_dispatch(const SchemeParams &S, const TW *wl_, const TW *wr_, const TW *wll,
+                                               const TW *whh, bool has_m, bool has_p, double dcc, double dcm,
+                                               double dcp, double dL, double dR, bool ifl, bool ifr, double *xl,
+                                               double *xr) {
+#define ECO_MUSCL(LV, LA) \
+  muscl_vars<MODEL, NPH, DIM, LV, LA, RECIP>(S, wl_, wr_, wll, whh, has_m, has_p, dcc, dcm, dcp, dL, dR, ifl, ifr, xl, xr)
+  if constexpr (SAMELIM && Tile<DIM>::LS != 0) {
+    const int lv = S.lim[0];
+    const int la = (MODEL == MODEL_EULER) ? lv : ((S.lim[2] == LIM_THINC) ? LIM_MINMOD : S.lim[2]);
+    if (lv == la) {
+      switch (lv) {
+        case LIM_MINMOD: ECO_MUSCL(LIM_MINMOD, LIM_MINMOD); return;
+        case LIM_VANLEER: ECO_MUSCL(LIM_VANLEER, LIM_VANLEER); return;
+        case LIM_VANALBADA: ECO_MUSCL(LIM_VANALBADA, LIM_VANALBADA); return;
+        case LIM_MC: ECO_MUSCL(LIM_MC, LIM_MC); return;
+        case LIM_SUPERBEE: ECO_MUSCL(LIM_SUPERBEE, LIM_SUPERBEE); return;
+        default: break;
+      }
+    } else if (la == LIM_MINMOD) {
+      switch (lv) {
+        case LIM_VANLEER: ECO_MUSCL(LIM_VANLEER, LIM_MINMOD); return;
+        case LIM_SUPERBEE: ECO_MUSCL(LIM_SUPERBEE, LIM_MINMOD); return;
+        default: break;
+      }
+    }
+  }
+  ECO_MUSCL(-1, -1);
+#undef ECO_MUSCL
 }
 
 // One interior face of the tile: reconstruction (MUSCL) or cached cell states
@@ -408,18 +534,8 @@ __device__ __forceinline__ double face_fp64(const StageArgs &A, const int *o, co
         whh[v] = has_p ? Fv[ihi + sd] : 0.;
       }
       double xl[Ly::NVAR], xr[Ly::NVAR];
-#pragma unroll
-      for (int v = 0; v < Ly::NVAR; v++) {
-        const double wl = wl_[v], wr = wr_[v];
-        const double sc = (wr - wl) / dcc;
-        const double sm = has_m ? (wl - wll[v]) / dcm : 0.;
-        const double sp = has_p ? (whh[v] - wr) / dcp : 0.;
-        const bool isa = (MODEL != MODEL_EULER) && (v < 3 * NPH) && (v % 3 == 0);
-        const int llim = isa ? (ifl ? S.lim[3] : S.lim[2]) : (ifl ? S.lim[1] : S.lim[0]);
-        const int rlim = isa ? (ifr ? S.lim[3] : S.lim[2]) : (ifr ? S.lim[1] : S.lim[0]);
-        xl[v] = wl + limiter(llim, sc, sm) * dL;
-        xr[v] = wr + (-limiter(rlim, sc, sp)) * dR;
-      }
+      muscl_dispatch<MODEL, NPH, DIM, SAMELIM, false>(S, wl_, wr_, wll, whh, has_m, has_p, dcc, dcm, dcp, dL, dR, ifl,
+                                                      ifr, xl, xr);
       vec_to_prim<MODEL, NPH, DIM>(xl, L);
       vec_to_prim<MODEL, NPH, DIM>(xr, R);
       if constexpr (MODEL == MODEL_EULER) {
@@ -468,48 +584,53 @@ __device__ __forceinline__ void stage_dir_3d(const StageArgs &A, const int *o, c
 }
 
 // ---------------------------------------------------------------------------
-// the stage kernel
+// one tile of a stage
 // STAGE 0: first-order step; 1: MUSCL predictor (writes U*); 2: MUSCL corrector
 // ---------------------------------------------------------------------------
-template <int MODEL, int NPH, int DIM, int STAGE, bool SAMELIM>
-__global__ void __launch_bounds__(Tile<DIM>::X * Tile<DIM>::Y * Tile<DIM>::Z, Tile<DIM>::W)
-k_stage(const StageArgs A) {
+template <int MODEL, int NPH, int DIM>
+struct StageLds {
   using Ly = Layout<MODEL, NPH, DIM>;
   using TF = TileFaces<DIM>;
+  static constexpr int NT = Tile<DIM>::X * Tile<DIM>::Y * Tile<DIM>::Z;
+  static constexpr bool MERGED = (DIM <= 2);
+  static constexpr int NF0 = TF::nf(0), NF1 = TF::nf(1);
+  static constexpr int NFT = MERGED ? NF0 + NF1 : TF::MAXF;  // LDS flux slots
+  double sF[Ly::NFV][NFT];
+  double sAcc[MERGED ? 1 : Ly::NACC][NT];
+  double sRed[NT / 64];
+};
+
+// Tile `blk` of the case described by A, launch slot `slot`.
+template <int MODEL, int NPH, int DIM, int STAGE, bool SAMELIM>
+__device__ __forceinline__ void stage_tile(const StageArgs &A, const int slot, const int blk,
+                                           StageLds<MODEL, NPH, DIM> &lds) {
+  using Ly = Layout<MODEL, NPH, DIM>;
+  using LD = StageLds<MODEL, NPH, DIM>;
   constexpr int TX = Tile<DIM>::X, TY = Tile<DIM>::Y, TZ = Tile<DIM>::Z;
-  constexpr int NT = TX * TY * TZ;
-  constexpr int MAXF = TF::MAXF;
-  constexpr bool MERGED = (DIM <= 2);
-  constexpr int NF0 = TF::nf(0), NF1 = TF::nf(1);
-  constexpr int NFT = MERGED ? NF0 + NF1 : MAXF;  // LDS flux slots
-  __shared__ double sF[Ly::NFV][NFT];
-  __shared__ double sAcc[MERGED ? 1 : Ly::NACC][NT];
-  __shared__ double sRed[NT / 64];
+  constexpr int NT = LD::NT;
+  constexpr bool MERGED = LD::MERGED;
+  constexpr int NF0 = LD::NF0, NF1 = LD::NF1;
+  constexpr int NFT = LD::NFT;
+  auto &sF = lds.sF;
+  auto &sAcc = lds.sAcc;
+  auto &sRed = lds.sRed;
 
   const Geometry &G = A.geo;
   const SchemeParams &S = A.sch;
   Control *ctl = A.ctl;
-  const int step = A.step;
   const int tid = threadIdx.x;
   const bool first_kernel = (STAGE != 2);
   const bool last_kernel = (STAGE != 1);
 
-  // run already finished: no-op step (the host may launch a few extra steps)
-  if (step > 0 && ctl->done[(step + 1) & 1]) {
-    if (last_kernel && blockIdx.x == 0 && tid == 0) {
-      ctl->done[step & 1] = 1;
-      ctl->t[step & 1] = ctl->t[(step + 1) & 1];
-    }
-    return;
-  }
-  const double dt = ctl_dt(ctl, G, S.cfl, step);
-  if (first_kernel && blockIdx.x == 0 && tid == 0) ctl->smax[(step + 1) % 3] = 0ull;
+  StepInfo si;
+  if (!step_begin(ctl, G, S.cfl, slot, blk, first_kernel, last_kernel, si)) return;
+  const double dt = si.dt;
 
   const long long nc = G.ncell;
   const int nx = G.n[0], ny = G.n[1], nz = G.n[2];
-  // tile origin (blockIdx.x linearised over the tile grid, x fastest)
+  // tile origin (tile index linearised over the tile grid, x fastest)
   const int tgx = (nx + TX - 1) / TX, tgy = (ny + TY - 1) / TY;
-  int b = blockIdx.x;
+  int b = blk;
   const int bx = b % tgx; b /= tgx;
   const int by = b % tgy; b /= tgy;
   const int bz = b;
@@ -610,16 +731,32 @@ k_stage(const StageArgs A) {
     double m = sRed[0];
 #pragma unroll
     for (int w = 1; w < NT / 64; w++) m = smax(m, sRed[w]);
-    if (m > 0.) atomicMax(&ctl->smax[step % 3], (unsigned long long)__double_as_longlong(m));
-    if (last_kernel && blockIdx.x == 0) {
-      double t = ((step == 0) ? 0. : ctl->t[(step + 1) & 1]) + dt;
-      ctl->t[step & 1] = t;
-      ctl->steps = step + 1;
-      ctl->dt_last = dt;
-      int done = (ctl->t_final < 0.) ? (step + 1 >= ctl->max_steps)
-                                      : (t >= (double)(float)ctl->t_final);
-      ctl->done[step & 1] = done;
-    }
+    step_end(ctl, m, slot, blk, last_kernel, si);
+  }
+}
+
+template <int MODEL, int NPH, int DIM, int STAGE, bool SAMELIM>
+__global__ void __launch_bounds__(Tile<DIM>::X * Tile<DIM>::Y * Tile<DIM>::Z, Tile<DIM>::W)
+k_stage(const StageArgs A) {
+  __shared__ StageLds<MODEL, NPH, DIM> lds;
+  stage_tile<MODEL, NPH, DIM, STAGE, SAMELIM>(A, A.step, blockIdx.x, lds);
+}
+
+// Persistent form over a pool: every workgroup walks the class's tile table
+// (stride gridDim.x); each entry names a case and a tile of it.  The tile
+// count is read from device memory, so a captured launch stays valid while
+// the table changes between replays.
+template <int MODEL, int NPH, int DIM, int STAGE, bool SAMELIM>
+__global__ void __launch_bounds__(Tile<DIM>::X * Tile<DIM>::Y * Tile<DIM>::Z, Tile<DIM>::W)
+k_stage_batch(const BatchDesc *__restrict__ D, const int slot) {
+  __shared__ StageLds<MODEL, NPH, DIM> lds;
+  const int ntiles = D->ntiles;
+  const int kidx = (STAGE == 2) ? 1 : 0;
+  for (int w = blockIdx.x; w < ntiles; w += gridDim.x) {
+    const TileRef t = D->tiles[w];
+    const StageArgs &A = D->args[(size_t)t.c * 4 + (slot & 1) * 2 + kidx];
+    stage_tile<MODEL, NPH, DIM, STAGE, SAMELIM>(A, slot, t.blk, lds);
+    __syncthreads();  // LDS reuse by the next tile
   }
 }
 
@@ -666,6 +803,37 @@ int launch_d(const StageArgs &A, int stage, hipStream_t st) {
     case 1: return launch_t<MODEL, NPH, 1>(A, stage, st);
     case 2: return launch_t<MODEL, NPH, 2>(A, stage, st);
     case 3: return launch_t<MODEL, NPH, 3>(A, stage, st);
+  }
+  return -1;
+}
+
+template <int MODEL, int NPH, int DIM, int STAGE, bool SAME>
+int launch_tb1(const BatchLaunch &L, hipStream_t st) {
+  constexpr int NT = Tile<DIM>::X * Tile<DIM>::Y * Tile<DIM>::Z;
+  if (L.occupancy)
+    return (int)hipOccupancyMaxActiveBlocksPerMultiprocessor(L.occupancy, k_stage_batch<MODEL, NPH, DIM, STAGE, SAME>,
+                                                             NT, 0);
+  hipLaunchKernelGGL((k_stage_batch<MODEL, NPH, DIM, STAGE, SAME>), dim3((unsigned)L.grid), dim3(NT), 0, st, L.desc,
+                     L.slot);
+  return (int)hipGetLastError();
+}
+
+template <int MODEL, int NPH, int DIM>
+int launch_tb(const BatchLaunch &L, hipStream_t st) {
+  switch (L.stage) {
+    case 0: return launch_tb1<MODEL, NPH, DIM, 0, true>(L, st);
+    case 1: return L.same ? launch_tb1<MODEL, NPH, DIM, 1, true>(L, st) : launch_tb1<MODEL, NPH, DIM, 1, false>(L, st);
+    case 2: return L.same ? launch_tb1<MODEL, NPH, DIM, 2, true>(L, st) : launch_tb1<MODEL, NPH, DIM, 2, false>(L, st);
+  }
+  return -1;
+}
+
+template <int MODEL, int NPH>
+int launch_db(const BatchLaunch &L, hipStream_t st) {
+  switch (L.dim) {
+    case 1: return launch_tb<MODEL, NPH, 1>(L, st);
+    case 2: return launch_tb<MODEL, NPH, 2>(L, st);
+    case 3: return launch_tb<MODEL, NPH, 3>(L, st);
   }
   return -1;
 }

@@ -15,11 +15,22 @@
 
 namespace eco {
 
+// Two-phase volume fractions from their FP32 copies: the smaller one is
+// authoritative (FP32 keeps its relative precision), the larger is its FP64
+// complement.  A majority fraction 1 - 1e-6 stored in FP32 alone would carry
+// ~6e-8 absolute error, which the stiffened-gas energy alpha*gamma*pinf/(gamma-1)
+// turns into pressure errors of order 1e2 Pa.
+__device__ __forceinline__ void decode_alpha(float f0, float f1, double &a0, double &a1) {
+  if (f0 <= f1) { a0 = (double)f0; a1 = 1. - a0; }
+  else { a1 = (double)f1; a0 = 1. - a1; }
+}
+
 template <int MODEL, int NPH, int DIM>
 __device__ __forceinline__ void load_prim_f(const float *__restrict__ W, long long nc, long long idx, CellState<NPH> &s) {
+  static_assert(NPH == 2, "FP32 storage holds two-phase states");
+  decode_alpha(W[idx], W[3 * nc + idx], s.a[0], s.a[1]);
 #pragma unroll
   for (int k = 0; k < NPH; k++) {
-    s.a[k] = (double)W[(3 * k) * nc + idx];
     s.r[k] = (double)W[(3 * k + 1) * nc + idx];
     s.p[k] = (double)W[(3 * k + 2) * nc + idx];
   }
@@ -51,9 +62,9 @@ __device__ __forceinline__ double energy_offset(const SchemeParams &S, const Cel
 // the FP32-rounded state as stored (for the offset of R)
 template <int NPH>
 __device__ __forceinline__ void round_state(CellState<NPH> &s) {
+  decode_alpha((float)s.a[0], (float)s.a[1], s.a[0], s.a[1]);
 #pragma unroll
   for (int k = 0; k < NPH; k++) {
-    s.a[k] = (double)(float)s.a[k];
     s.r[k] = (double)(float)s.r[k];
     s.p[k] = (double)(float)s.p[k];
   }
@@ -185,11 +196,9 @@ __device__ __forceinline__ float speeds_f(const SchemeParams &S, const CellState
 template <int NPH>
 __device__ __forceinline__ void load_nc_coeffs_f(const float *__restrict__ Wn, long long nc, long long cidx,
                                                  CellState<NPH> &own) {
+  decode_alpha(Wn[cidx], Wn[3 * nc + cidx], own.a[0], own.a[1]);
 #pragma unroll
-  for (int k = 0; k < NPH; k++) {
-    own.a[k] = (double)Wn[(3 * k) * nc + cidx];
-    own.p[k] = (double)Wn[(3 * k + 2) * nc + cidx];
-  }
+  for (int k = 0; k < NPH; k++) own.p[k] = (double)Wn[(3 * k + 2) * nc + cidx];
 }
 
 // One interior face of the tile in the FP32-storage form: FP32 loads,
@@ -228,34 +237,31 @@ __device__ __forceinline__ float face_fp32s(const StageArgs &A, const int *o, co
     wll[v] = has_m ? Fv[ilo - sd] : 0.f;
     whh[v] = has_p ? Fv[ihi + sd] : 0.f;
   }
-  const double aL = wl_[0], aR = wr_[0];
-  double aLL = 0., aRR = 0.;
+  double wlv[Ly::NVAR], wrv[Ly::NVAR], wlm[Ly::NVAR], whp[Ly::NVAR];
+#pragma unroll
+  for (int v = 0; v < Ly::NVAR; v++) { wlv[v] = wl_[v]; wrv[v] = wr_[v]; wlm[v] = wll[v]; whp[v] = whh[v]; }
+  decode_alpha(wl_[0], wl_[3], wlv[0], wlv[3]);
+  decode_alpha(wr_[0], wr_[3], wrv[0], wrv[3]);
+  if (has_m) decode_alpha(wll[0], wll[3], wlm[0], wlm[3]);
+  if (has_p) decode_alpha(whh[0], whh[3], whp[0], whp[3]);
+  const double aL = wlv[0], aR = wrv[0];
+  double aLL = 0., aRR = 0., tmp;
   if (!SAMELIM || S.thinc) {
-    aLL = has_m ? (double)Wn[ilo - sd] : 0.;
-    aRR = has_p ? (double)Wn[ihi + sd] : 0.;
+    if (has_m) decode_alpha(Wn[ilo - sd], Wn[3 * nc + ilo - sd], aLL, tmp);
+    if (has_p) decode_alpha(Wn[ihi + sd], Wn[3 * nc + ihi + sd], aRR, tmp);
   }
   bool ifl = false, ifr = false;
   if constexpr (!SAMELIM) {
     ifl = (aL >= 1.e-4) && (aL <= 1. - 1.e-4) && ((aR - aL) * (aL - aLL) >= 1.e-8);
     ifr = (aR >= 1.e-4) && (aR <= 1. - 1.e-4) && ((aL - aR) * (aR - aRR) >= 1.e-8);
   }
-  const double idc = 1. / G.dcc[dir][i];
-  const double idm = has_m ? 1. / G.dcc[dir][i - 1] : 1.;
-  const double idp = has_p ? 1. / G.dcc[dir][i + 1] : 1.;
+  const double dcc = G.dcc[dir][i];
+  const double dcm = has_m ? G.dcc[dir][i - 1] : 1.;
+  const double dcp = has_p ? G.dcc[dir][i + 1] : 1.;
   const double dL = G.dfl[dir][i], dR = G.dfr[dir][i];
   double xl[Ly::NVAR], xr[Ly::NVAR];
-#pragma unroll
-  for (int v = 0; v < Ly::NVAR; v++) {
-    const double wl = wl_[v], wr = wr_[v];
-    const double sc = (wr - wl) * idc;
-    const double sm = has_m ? (wl - (double)wll[v]) * idm : 0.;
-    const double sp = has_p ? ((double)whh[v] - wr) * idp : 0.;
-    const bool isa = (v < 3 * NPH) && (v % 3 == 0);
-    const int llim = isa ? (ifl ? S.lim[3] : S.lim[2]) : (ifl ? S.lim[1] : S.lim[0]);
-    const int rlim = isa ? (ifr ? S.lim[3] : S.lim[2]) : (ifr ? S.lim[1] : S.lim[0]);
-    xl[v] = wl + limiter(llim, sc, sm) * dL;
-    xr[v] = wr + (-limiter(rlim, sc, sp)) * dR;
-  }
+  muscl_dispatch<MODEL, NPH, DIM, SAMELIM, true>(S, wlv, wrv, wlm, whp, has_m, has_p, dcc, dcm, dcp, dL, dR, ifl, ifr,
+                                                 xl, xr);
   CellState<NPH> L, R;
   vec_to_prim<MODEL, NPH, DIM>(xl, L);
   vec_to_prim<MODEL, NPH, DIM>(xr, R);
@@ -301,43 +307,37 @@ __device__ __forceinline__ void stage32_dir_3d(const StageArgs &A, const int *o,
   }
 }
 
+// Tile `blk` of a MUSCL stage in the FP32-storage form (STAGE 1 predictor,
+// 2 corrector), launch slot `slot`.
 template <int MODEL, int NPH, int DIM, int STAGE, bool SAMELIM>
-__global__ void __launch_bounds__(Tile<DIM>::X * Tile<DIM>::Y * Tile<DIM>::Z, Tile<DIM>::W)
-k_stage32(const StageArgs A) {
+__device__ __forceinline__ void stage32_tile(const StageArgs &A, const int slot, const int blk,
+                                             StageLds<MODEL, NPH, DIM> &lds) {
   using Ly = Layout<MODEL, NPH, DIM>;
-  using TF = TileFaces<DIM>;
+  using LD = StageLds<MODEL, NPH, DIM>;
   constexpr int TX = Tile<DIM>::X, TY = Tile<DIM>::Y, TZ = Tile<DIM>::Z;
-  constexpr int NT = TX * TY * TZ;
-  constexpr int MAXF = TF::MAXF;
-  constexpr bool MERGED = (DIM <= 2);
-  constexpr int NF0 = TF::nf(0), NF1 = TF::nf(1);
-  constexpr int NFT = MERGED ? NF0 + NF1 : MAXF;  // LDS flux slots
-  __shared__ double sF[Ly::NFV][NFT];
-  __shared__ double sAcc[MERGED ? 1 : Ly::NACC][NT];
-  __shared__ double sRed[NT / 64];
+  constexpr int NT = LD::NT;
+  constexpr bool MERGED = LD::MERGED;
+  constexpr int NF0 = LD::NF0, NF1 = LD::NF1;
+  constexpr int NFT = LD::NFT;
+  auto &sF = lds.sF;
+  auto &sAcc = lds.sAcc;
+  auto &sRed = lds.sRed;
 
   const Geometry &G = A.geo;
   const SchemeParams &S = A.sch;
   Control *ctl = A.ctl;
-  const int step = A.step;
   const int tid = threadIdx.x;
   const bool first_kernel = (STAGE == 1);
   const bool last_kernel = (STAGE == 2);
 
-  if (step > 0 && ctl->done[(step + 1) & 1]) {
-    if (last_kernel && blockIdx.x == 0 && tid == 0) {
-      ctl->done[step & 1] = 1;
-      ctl->t[step & 1] = ctl->t[(step + 1) & 1];
-    }
-    return;
-  }
-  const double dt = ctl_dt(ctl, G, S.cfl, step);
-  if (first_kernel && blockIdx.x == 0 && tid == 0) ctl->smax[(step + 1) % 3] = 0ull;
+  StepInfo si;
+  if (!step_begin(ctl, G, S.cfl, slot, blk, first_kernel, last_kernel, si)) return;
+  const double dt = si.dt;
 
   const long long nc = G.ncell;
   const int nx = G.n[0], ny = G.n[1], nz = G.n[2];
   const int tgx = (nx + TX - 1) / TX, tgy = (ny + TY - 1) / TY;
-  int b = blockIdx.x;
+  int b = blk;
   const int bx = b % tgx; b /= tgx;
   const int by = b % tgy; b /= tgy;
   const int bz = b;
@@ -437,15 +437,29 @@ k_stage32(const StageArgs A) {
     double mm = sRed[0];
 #pragma unroll
     for (int w = 1; w < NT / 64; w++) mm = smax(mm, sRed[w]);
-    if (mm > 0.) atomicMax(&ctl->smax[step % 3], (unsigned long long)__double_as_longlong(mm));
-    if (last_kernel && blockIdx.x == 0) {
-      double t = ((step == 0) ? 0. : ctl->t[(step + 1) & 1]) + dt;
-      ctl->t[step & 1] = t;
-      ctl->steps = step + 1;
-      ctl->dt_last = dt;
-      int done = (ctl->t_final < 0.) ? (step + 1 >= ctl->max_steps) : (t >= (double)(float)ctl->t_final);
-      ctl->done[step & 1] = done;
-    }
+    step_end(ctl, mm, slot, blk, last_kernel, si);
+  }
+}
+
+template <int MODEL, int NPH, int DIM, int STAGE, bool SAMELIM>
+__global__ void __launch_bounds__(Tile<DIM>::X * Tile<DIM>::Y * Tile<DIM>::Z, Tile<DIM>::W)
+k_stage32(const StageArgs A) {
+  __shared__ StageLds<MODEL, NPH, DIM> lds;
+  stage32_tile<MODEL, NPH, DIM, STAGE, SAMELIM>(A, A.step, blockIdx.x, lds);
+}
+
+// persistent form over a pool (as k_stage_batch)
+template <int MODEL, int NPH, int DIM, int STAGE, bool SAMELIM>
+__global__ void __launch_bounds__(Tile<DIM>::X * Tile<DIM>::Y * Tile<DIM>::Z, Tile<DIM>::W)
+k_stage32_batch(const BatchDesc *__restrict__ D, const int slot) {
+  __shared__ StageLds<MODEL, NPH, DIM> lds;
+  const int ntiles = D->ntiles;
+  const int kidx = (STAGE == 2) ? 1 : 0;
+  for (int w = blockIdx.x; w < ntiles; w += gridDim.x) {
+    const TileRef t = D->tiles[w];
+    const StageArgs &A = D->args[(size_t)t.c * 4 + (slot & 1) * 2 + kidx];
+    stage32_tile<MODEL, NPH, DIM, STAGE, SAMELIM>(A, slot, t.blk, lds);
+    __syncthreads();  // LDS reuse by the next tile
   }
 }
 
@@ -518,6 +532,38 @@ int launch_d32(const StageArgs &A, int stage, hipStream_t st, double *tmp) {
     case 1: return launch_t32<MODEL, NPH, 1>(A, stage, st, tmp);
     case 2: return launch_t32<MODEL, NPH, 2>(A, stage, st, tmp);
     case 3: return launch_t32<MODEL, NPH, 3>(A, stage, st, tmp);
+  }
+  return -1;
+}
+
+template <int MODEL, int NPH, int DIM, int STAGE, bool SAME>
+int launch_tb32_1(const BatchLaunch &L, hipStream_t st) {
+  constexpr int NT = Tile<DIM>::X * Tile<DIM>::Y * Tile<DIM>::Z;
+  if (L.occupancy)
+    return (int)hipOccupancyMaxActiveBlocksPerMultiprocessor(L.occupancy,
+                                                             k_stage32_batch<MODEL, NPH, DIM, STAGE, SAME>, NT, 0);
+  hipLaunchKernelGGL((k_stage32_batch<MODEL, NPH, DIM, STAGE, SAME>), dim3((unsigned)L.grid), dim3(NT), 0, st, L.desc,
+                     L.slot);
+  return (int)hipGetLastError();
+}
+
+template <int MODEL, int NPH, int DIM>
+int launch_tb32(const BatchLaunch &L, hipStream_t st) {
+  switch (L.stage) {
+    case 1:
+      return L.same ? launch_tb32_1<MODEL, NPH, DIM, 1, true>(L, st) : launch_tb32_1<MODEL, NPH, DIM, 1, false>(L, st);
+    case 2:
+      return L.same ? launch_tb32_1<MODEL, NPH, DIM, 2, true>(L, st) : launch_tb32_1<MODEL, NPH, DIM, 2, false>(L, st);
+  }
+  return -1;
+}
+
+template <int MODEL, int NPH>
+int launch_db32(const BatchLaunch &L, hipStream_t st) {
+  switch (L.dim) {
+    case 1: return launch_tb32<MODEL, NPH, 1>(L, st);
+    case 2: return launch_tb32<MODEL, NPH, 2>(L, st);
+    case 3: return launch_tb32<MODEL, NPH, 3>(L, st);
   }
   return -1;
 }

@@ -26,8 +26,18 @@
 #define ECO_WAVES1 ECO_MIN_WAVES_PER_EU
 #endif
 #ifndef ECO_WAVES2
-#define ECO_WAVES2 3
+#define ECO_WAVES2 2
 #endif
 #ifndef ECO_WAVES3
 #define ECO_WAVES3 1
+#endif
+// compile-time (branch-free) limiter for the single-limiter configurations
+#ifndef ECO_LIMSPEC1
+#define ECO_LIMSPEC1 1
+#endif
+#ifndef ECO_LIMSPEC2
+#define ECO_LIMSPEC2 1
+#endif
+#ifndef ECO_LIMSPEC3
+#define ECO_LIMSPEC3 1
 #endif

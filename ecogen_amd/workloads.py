@@ -166,3 +166,105 @@ def shock_tube_1d(n=1000):
     st[4] = 1000.0
     st[5] = st[2]
     return case, st
+
+
+# ---------------------------------------------------------------------------
+# Batched and served workloads: independent two-phase Riemann problems
+# ---------------------------------------------------------------------------
+P_RIGHT = 1e5
+RHO_AIR_TUBE = 50.0   # air density of the water/air shock tube (as shock_tube_1d)
+
+
+def frozen_sound_speed(state_vals, eos=(AIR, WATER)):
+    """Frozen mixture sound speed sqrt(sum_k Y_k c_k^2) of one cell (MixKapila.cpp:166-169)."""
+    a = [state_vals[3 * k] for k in range(2)]
+    r = [state_vals[3 * k + 1] for k in range(2)]
+    p = [state_vals[3 * k + 2] for k in range(2)]
+    rho = sum(a[k] * r[k] for k in range(2))
+    c2 = 0.0
+    for k, e in enumerate(eos):
+        ck2 = e["gamma"] * (p[k] + e.get("pinf", 0.0)) / r[k]
+        c2 += a[k] * r[k] / rho * ck2
+    return math.sqrt(c2)
+
+
+def riemann_case(dim, n, ratio, pos, precision="fp32s"):
+    """Water (high pressure p = ratio * 1e5) against air (1e5 Pa), both at rest.
+
+    1-D: water for x <= pos on [0, 1].  2-D: a water column of radius pos/2
+    centred in [0, 1]^2.  Kapila model, MUSCL van Leer, transmissive
+    boundaries, alpha floor 1e-6, CFL 0.8.  Runs to t_final = 0.3 * L / a_max,
+    a_max the largest initial wave speed (frozen sound speed, fluid at rest).
+    Returns (case, state, t_final)."""
+    cells = [n, 1, 1] if dim == 1 else [n, n, 1]
+    lengths = [1.0, 0.1, 0.1] if dim == 1 else [1.0, 1.0, 1.0]
+    case = {"model": "kapila", "alpha_null": False, "eos": [AIR, WATER], "cells": cells, "lengths": lengths,
+            "order": 2, "limiters": ["vanleer"] * 4, "cfl": 0.8, "precision": precision}
+    pl = ratio * P_RIGHT
+    left = [ALPHA_FLOOR, RHO_AIR_TUBE, pl, 1 - ALPHA_FLOOR, 1000.0, pl, 0.0, 0.0, 0.0]
+    right = [1 - ALPHA_FLOOR, RHO_AIR_TUBE, P_RIGHT, ALPHA_FLOOR, 1000.0, P_RIGHT, 0.0, 0.0, 0.0]
+    xc, yc, _ = _centres(cells, lengths)
+    if dim == 1:
+        inside = xc <= pos
+    else:
+        r = 0.5 * pos
+        inside = ((xc[None, :] - 0.5) ** 2 + (yc[:, None] - 0.5) ** 2 <= r * r).ravel()
+    st = np.empty((9, inside.size))
+    for f in range(9):
+        st[f] = np.where(inside, left[f], right[f])
+    a_max = max(frozen_sound_speed(left), frozen_sound_speed(right))
+    return case, st, 0.3 * lengths[0] / a_max
+
+
+def batch_specs(count=2048, seed=0xEC0C0004, frac_1d=0.7, n1=(200, 50000), s2=(64, 768), ratio=(10.0, 1e4),
+                pos=(0.2, 0.8)):
+    """Parameters of the batched configuration: (dim, n, ratio, pos) per case,
+    drawn with splitmix64 (log-uniform sizes and pressure ratios)."""
+    rng = SplitMix64(seed)
+    out = []
+    for _ in range(count):
+        dim = 1 if rng.uniform() < frac_1d else 2
+        lo, hi = n1 if dim == 1 else s2
+        n = int(round(math.exp(math.log(lo) + rng.uniform() * (math.log(hi) - math.log(lo)))))
+        r = math.exp(math.log(ratio[0]) + rng.uniform() * (math.log(ratio[1]) - math.log(ratio[0])))
+        x = pos[0] + rng.uniform() * (pos[1] - pos[0])
+        out.append((dim, n, r, x))
+    return out
+
+
+def bubble_request(seed, n=128, nb=8, rmin=4.0, rmax=16.0, precision="fp32s"):
+    """3-D MultiP bubble case of the served stream (physics of bubbles_3d):
+    ``nb`` air bubbles in water at 5e7 Pa, MUSCL minmod, stiff relaxation.
+    Runs to t_final = 0.3 * L / a_max.  Returns (case, state, t_final)."""
+    case, st = bubbles_3d(n=n, nb=nb, seed=seed, rmin=rmin, rmax=rmax, precision=precision)
+    case.pop("time_mode", None)
+    case.pop("steps", None)
+    water = [ALPHA_FLOOR, 1.2, 5e7, 1 - ALPHA_FLOOR, 1000.0, 5e7]
+    return case, st, 0.3 * case["lengths"][0] / frozen_sound_speed(water)
+
+
+def request_stream(lam, count, seed=0xEC0C0005, frac_3d=0.05):
+    """Poisson arrivals at rate ``lam`` per second: list of (arrival_s, kind, params).
+
+    The sequence of requests depends on ``seed`` only; ``lam`` scales the
+    exponential inter-arrival gaps, so streams at different rates carry the
+    same requests.  kind "riemann": params (dim, n, ratio, pos) from the
+    batched distribution; kind "bubbles": params = seed of a 3-D bubble case."""
+    rng = SplitMix64(seed)
+    gaps = SplitMix64(seed ^ 0x5EED0A11)
+    t = 0.0
+    out = []
+    for i in range(count):
+        t += -math.log(1.0 - gaps.uniform()) / lam
+        if rng.uniform() < frac_3d:
+            out.append((t, "bubbles", (seed + 7919 * (i + 1)) & 0xFFFFFFFF))
+        else:
+            out.append((t, "riemann", batch_specs(1, seed=rng.next())[0]))
+    return out
+
+
+def request_case(kind, params, precision="fp32s"):
+    """(case, state, t_final) of one request of request_stream."""
+    if kind == "riemann":
+        return riemann_case(*params, precision=precision)
+    return bubble_request(params, precision=precision)
