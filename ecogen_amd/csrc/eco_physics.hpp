@@ -18,12 +18,25 @@ namespace eco {
 ECO_D double smin(double a, double b) { return (b < a) ? b : a; }
 ECO_D double smax(double a, double b) { return (a < b) ? b : a; }
 
+// a / b, correctly rounded, from y = RN(1/b): q = RN(a*y) is within one ulp
+// of a/b, the residual r = a - b*q is exact in one fma, and RN(q + r*y) is
+// the correctly rounded quotient (Markstein's theorem; P. Markstein, "IA-64
+// and elementary functions", 2000, Thm. 8.10).  Three operations instead of
+// the ten of a division when several numerators share b; bit-identical to
+// a / b for finite, normal operands.
+ECO_D double qdiv(double a, double b, double y) {
+  const double q = a * y;
+  const double r = fma(-b, q, a);
+  return fma(r, y, q);
+}
+
 // ----------------------------------------------------------------------------
 // EOS: src/Eos/EosSG.cpp:93-254, src/Eos/EosIG.cpp:90-251
 // ----------------------------------------------------------------------------
 ECO_D double eos_e(const EosParams &q, double rho, double p, double eps) {
-  if (q.type == EOS_SG) return (p + q.gpinf) / q.gm1 / smax(rho, eps) + q.eref;
-  return p / q.gm1 / smax(rho, eps) + q.eref;
+  // og = RN(1/(gamma-1)) (host division): the first quotient through qdiv
+  if (q.type == EOS_SG) return qdiv(p + q.gpinf, q.gm1, q.og) / smax(rho, eps) + q.eref;
+  return qdiv(p, q.gm1, q.og) / smax(rho, eps) + q.eref;
 }
 ECO_D double eos_p(const EosParams &q, double rho, double e) {
   if (q.type == EOS_SG) return q.gm1 * rho * (e - q.eref) - q.gpinf;
@@ -197,9 +210,10 @@ ECO_D void fulfill(const SchemeParams &S, CellState<NPH> &s) {
     s.rho = 0.; s.pm = 0.;
 #pragma unroll
     for (int k = 0; k < NPH; k++) { s.rho += s.a[k] * s.r[k]; s.pm += s.a[k] * s.p[k]; }
+    const double rd = (MODEL == MODEL_KAPILA) ? smax(s.rho, eps) : s.rho;
+    const double ird = 1. / rd;
 #pragma unroll
-    for (int k = 0; k < NPH; k++)
-      s.Y[k] = (MODEL == MODEL_KAPILA) ? s.a[k] * s.r[k] / smax(s.rho, eps) : s.a[k] * s.r[k] / s.rho;
+    for (int k = 0; k < NPH; k++) s.Y[k] = qdiv(s.a[k] * s.r[k], rd, ird);
     s.em = 0.; s.cF = 0.;
 #pragma unroll
     for (int k = 0; k < NPH; k++) { s.em += s.Y[k] * s.e[k]; s.cF += s.Y[k] * s.c[k] * s.c[k]; }
@@ -244,10 +258,11 @@ ECO_D void build_prim(const SchemeParams &S, FluxVec<NPH> &f, CellState<NPH> &s)
   const double eps = S.eps;
   if constexpr (MODEL == MODEL_EULER) {
     s.r[0] = f.m[0];
+    const double irm = 1. / f.m[0];
 #pragma unroll
-    for (int d = 0; d < 3; d++) s.u[d] = f.q[d] / f.m[0];
+    for (int d = 0; d < 3; d++) s.u[d] = qdiv(f.q[d], f.m[0], irm);
     zero_small(s.u);
-    s.E = f.E / f.m[0];
+    s.E = qdiv(f.E, f.m[0], irm);
     s.e[0] = s.E - 0.5 * sqn(s.u);
     s.p[0] = eos_p(S.eos[0], f.m[0], s.e[0]);
     s.c[0] = eos_c(S.eos[0], f.m[0], s.p[0], eps);
@@ -277,8 +292,9 @@ ECO_D void build_prim(const SchemeParams &S, FluxVec<NPH> &f, CellState<NPH> &s)
         un += f.a[k];
       }
     }
+    const double iun = 1. / un;
 #pragma unroll
-    for (int k = 0; k < NPH; k++) f.a[k] /= un;
+    for (int k = 0; k < NPH; k++) f.a[k] = qdiv(f.a[k], un, iun);
     double rhoMel = 0.;
 #pragma unroll
     for (int k = 0; k < NPH; k++) {
@@ -295,11 +311,12 @@ ECO_D void build_prim(const SchemeParams &S, FluxVec<NPH> &f, CellState<NPH> &s)
       s.p[k] = eos_p(S.eos[k], s.r[k], ein);
       verify_correct<MODEL, NPH>(S, s, k);
     }
+    const double irm = 1. / rhoMel;
 #pragma unroll
-    for (int d = 0; d < 3; d++) s.u[d] = f.q[d] / rhoMel;
+    for (int d = 0; d < 3; d++) s.u[d] = qdiv(f.q[d], rhoMel, irm);
     zero_small(s.u);
     fulfill<MODEL, NPH>(S, s);
-    s.E = f.E / rhoMel;
+    s.E = qdiv(f.E, rhoMel, irm);
   }
 }
 
@@ -482,13 +499,14 @@ ECO_D double hllc(const SchemeParams &S, const CellState<NPH> &L, const CellStat
       const CellState<NPH> &X = left ? L : R;
       double uS = left ? uL : uR, pS = left ? pL : pR, mS = left ? mL : mR, sS = left ? sL : sR;
       double Et = X.em + 0.5 * sqn(X.u);
-      double rhoStar = mS / (sS - sM);
+      const double dS = sS - sM, idS = 1. / dS;
+      double rhoStar = qdiv(mS, dS, idS);
       double EStar = Et + (sM - uS) * (sM + pS / mS);
       double pStar = mS * (sM - uS) + pS;
 #pragma unroll
       for (int k = 0; k < NPH; k++) {
         double mk = X.r[k] * (sS - uS);
-        double rks = mk / (sS - sM);
+        double rks = qdiv(mk, dS, idS);
         double pks = eos_isentrope(S.eos[k], X.p[k], X.r[k], rks, eps);
         double eks = eos_e(S.eos[k], rks, pks, eps);
         F.a[k] = X.a[k] * sM;

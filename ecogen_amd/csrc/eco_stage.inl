@@ -385,14 +385,16 @@ __device__ __forceinline__ void accumulate_dir(FluxVec<NPH> &acc, const double *
 // cell from (i+1/2, i+3/2).  LV/LA: compile-time limiter of the non-alpha /
 // alpha variables, or -1 for the runtime choice from S.lim and the interface
 // flags.  RECIP: slopes through reciprocals of the cell distances (the
-// FP32-storage form; the FP64 form divides like the reference).
+// FP32-storage form); otherwise the reference's correctly rounded quotients,
+// formed through one reciprocal per distance (qdiv).
 template <int MODEL, int NPH, int DIM, int LV, int LA, bool RECIP, typename TW>
 __device__ __forceinline__ void muscl_vars(const SchemeParams &S, const TW *wl_, const TW *wr_, const TW *wll,
                                            const TW *whh, bool has_m, bool has_p, double dcc, double dcm, double dcp,
                                            double dL, double dR, bool ifl, bool ifr, double *xl, double *xr) {
   using Ly = Layout<MODEL, NPH, DIM>;
-  double idc = 0., idm = 0., idp = 0.;
+  double idc = 0., idm = 0., idp = 0., ydc = 0., ydm = 0., ydp = 0.;
   if constexpr (RECIP) { idc = 1. / dcc; idm = 1. / dcm; idp = 1. / dcp; }
+  else { ydc = 1. / dcc; ydm = 1. / dcm; ydp = 1. / dcp; }
 #pragma unroll
   for (int v = 0; v < Ly::NVAR; v++) {
     const double wl = wl_[v], wr = wr_[v];
@@ -402,9 +404,10 @@ __device__ __forceinline__ void muscl_vars(const SchemeParams &S, const TW *wl_,
       sm = has_m ? (wl - (double)wll[v]) * idm : 0.;
       sp = has_p ? ((double)whh[v] - wr) * idp : 0.;
     } else {
-      sc = (wr - wl) / dcc;
-      sm = has_m ? (wl - (double)wll[v]) / dcm : 0.;
-      sp = has_p ? ((double)whh[v] - wr) / dcp : 0.;
+      // correctly rounded quotients through the shared reciprocals
+      sc = qdiv(wr - wl, dcc, ydc);
+      sm = has_m ? qdiv(wl - (double)wll[v], dcm, ydm) : 0.;
+      sp = has_p ? qdiv((double)whh[v] - wr, dcp, ydp) : 0.;
     }
     const bool isa = (MODEL != MODEL_EULER) && (v < 3 * NPH) && (v % 3 == 0);
     double gl, gr;
