@@ -102,14 +102,17 @@ def cpu_baseline(args):
                       f"restatement of the reference step, OpenMP {threads} threads, {dt:.1f} s"}
 
 
-def traffic_from_profiles(workload_key):
-    p = ROOT / "profiles" / f"traffic_{workload_key}.json"
-    if p.exists():
-        try:
-            return json.loads(p.read_text()).get("bytes_per_launch")
-        except Exception:
-            return None
-    return None
+def traffic_from_profiles(n, precision, kernel):
+    """HBM bytes per launch of a stage kernel from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE passes (scaled by the copy-kernel calibration), or
+    None when no profile of this workload size is committed."""
+    p = ROOT / "profiles" / f"traffic_kapila2d_o2_vanleer_{n}.json"
+    if not p.exists():
+        return None
+    try:
+        return json.loads(p.read_text())["kernels"][precision][kernel]["bytes_per_launch"]
+    except (KeyError, ValueError):
+        return None
 
 
 def run_precision(case, st, args, precision):
@@ -354,14 +357,16 @@ def main():
     r64 = run_precision(case, st, args, "fp64")
     log(f"fp64: {r64['wall'] / args.steps * 1e3:.3f} ms/step")
     roof = roofline(case, r64, 8)
-    roof["traffic"] = traffic_from_profiles(f"kapila2d_o2_vanleer_fp64_{args.n}")
+    roof["traffic"] = traffic_from_profiles(args.n, "fp64", roof["kernel"])
     fp32s = None
     if not args.no_fp32s:
         r32 = run_precision(case, st, args, "fp32s")
+        roof32 = roofline(case, r32, 4)
+        roof32["traffic"] = traffic_from_profiles(args.n, "fp32s", roof32["kernel"])
         fp32s = {"value": round(r32["value"], 1), "speedup_vs_fp64": round(r32["value"] / r64["value"], 3),
                  "ms_per_step": round(r32["wall"] / args.steps * 1e3, 4),
                  "l1_vs_fp64": {k: float(f"{v:.3e}") for k, v in l1_errors(case, r32["state"], r64["state"]).items()},
-                 "roofline": roofline(case, r32, 4),
+                 "roofline": roof32,
                  "storage": "alpha_k, rho_k, p_k, u, reduced energy in FP32; fluxes, update, relaxation in FP64"}
     del st
     if args.calibrate_copy:

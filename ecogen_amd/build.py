@@ -44,6 +44,20 @@ def _compile(src: str) -> Path:
     return obj
 
 
+API_CHECK_SRC = PKG.parent / "tests" / "cpp" / "api_check.cpp"
+API_CHECK = PKG.parent / "tests" / "cpp" / "api_check"
+
+
+def build_api_check() -> Path:
+    """Host test program of the C++ class API, linked against the library."""
+    cmd = ["g++", "-O2", "-std=c++17", "-I", str(PKG.parent / "include"), str(API_CHECK_SRC), str(OUT),
+           f"-Wl,-rpath,{PKG}", "-Wl,-rpath,/opt/rocm/lib", "-o", str(API_CHECK)]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"api_check build failed:\n{r.stderr[-3000:]}")
+    return API_CHECK
+
+
 def build(verbose: bool = True) -> Path:
     OBJ.mkdir(exist_ok=True)
     with ThreadPoolExecutor(max_workers=min(len(SOURCES), os.cpu_count() or 4)) as ex:
@@ -54,6 +68,8 @@ def build(verbose: bool = True) -> Path:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr[-4000:]}")
     os.replace(tmp, OUT)
+    if API_CHECK_SRC.exists():
+        build_api_check()
     if verbose:
         print(f"built {OUT}")
     return OUT
