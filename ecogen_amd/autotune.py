@@ -33,8 +33,8 @@ TUNE_DIR = PKG / "_tune"
 CSRC = PKG / "csrc"
 
 SPACE = {
-    2: {"tiles": [(32, 8), (64, 4)], "waves": [2, 3, 4], "limspec": [0, 1]},
-    3: {"tiles": [(16, 4, 4), (32, 4, 2)], "waves": [1, 2], "limspec": [0, 1]},
+    2: {"tiles": [(32, 8), (64, 4), (16, 16)], "waves": [1, 2, 3, 4], "limspec": [1]},
+    3: {"tiles": [(16, 4, 4), (32, 4, 2), (8, 8, 4)], "waves": [1, 2], "limspec": [1]},
 }
 
 

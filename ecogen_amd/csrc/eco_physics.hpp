@@ -3,12 +3,14 @@
 // Every function restates one reference routine (file:line relative to the
 // ECOGEN source tree) with the same evaluation order, clamp constants and
 // std::min/std::max semantics, so that the only arithmetic differences from
-// the reference are those of the device pow/exp (not correctly rounded).
+// the reference are those of the power in the isentrope (pow_pos, a few
+// ulp; the reference's C library pow is close to correctly rounded).
 // Derived quantities the reference caches per cell (phase energies, sound
 // speeds, mixture density/pressure/energy, frozen sound speed) are recomputed
 // from the stored primitive state (alpha_k, rho_k, p_k, u, E) when needed.
 #pragma once
 #include <hip/hip_runtime.h>
+#include "eco_math.h"
 #include "eco_params.h"
 
 namespace eco {
@@ -17,18 +19,6 @@ namespace eco {
 
 ECO_D double smin(double a, double b) { return (b < a) ? b : a; }
 ECO_D double smax(double a, double b) { return (a < b) ? b : a; }
-
-// a / b, correctly rounded, from y = RN(1/b): q = RN(a*y) is within one ulp
-// of a/b, the residual r = a - b*q is exact in one fma, and RN(q + r*y) is
-// the correctly rounded quotient (Markstein's theorem; P. Markstein, "IA-64
-// and elementary functions", 2000, Thm. 8.10).  Three operations instead of
-// the ten of a division when several numerators share b; bit-identical to
-// a / b for finite, normal operands.
-ECO_D double qdiv(double a, double b, double y) {
-  const double q = a * y;
-  const double r = fma(-b, q, a);
-  return fma(r, y, q);
-}
 
 // ----------------------------------------------------------------------------
 // EOS: src/Eos/EosSG.cpp:93-254, src/Eos/EosIG.cpp:90-251
@@ -47,8 +37,8 @@ ECO_D double eos_c(const EosParams &q, double rho, double p, double eps) {
   return sqrt(q.gamma * p / smax(rho, eps));
 }
 ECO_D double eos_isentrope(const EosParams &q, double p0, double rho0, double rho1, double eps) {
-  if (q.type == EOS_SG) return (p0 + q.pinf) * pow(rho1 / smax(rho0, eps), q.gamma) - q.pinf;
-  return p0 * pow(rho1 / smax(rho0, eps), q.gamma);
+  if (q.type == EOS_SG) return (p0 + q.pinf) * pow_pos(rho1 / smax(rho0, eps), q.gamma) - q.pinf;
+  return p0 * pow_pos(rho1 / smax(rho0, eps), q.gamma);
 }
 ECO_D double eos_rho_pfinal(const EosParams &q, double p0, double rho0, double pf, double &drho, double eps) {
   if (q.type == EOS_SG) {

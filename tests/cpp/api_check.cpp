@@ -77,9 +77,9 @@ static int run_shock_tube(const char *out) {
   for (int i = 0; i < n; i++) {
     const double x = (i + 0.5) * (1. / n);
     const bool hp = x <= 0.7;
-    const double aw = hp ? 1. - 1e-6 : 1e-6, p = hp ? 1e9 : 1e5;
-    st[0 * n + i] = 1. - aw; st[1 * n + i] = 50.;   st[2 * n + i] = p;
-    st[3 * n + i] = aw;      st[4 * n + i] = 1000.; st[5 * n + i] = p;
+    const double aa = hp ? 1e-6 : 1. - 1e-6, p = hp ? 1e9 : 1e5;  // air fraction
+    st[0 * n + i] = aa;      st[1 * n + i] = 50.;   st[2 * n + i] = p;
+    st[3 * n + i] = 1. - aa; st[4 * n + i] = 1000.; st[5 * n + i] = p;
   }
   run.setState(st);
   RunStatus s = run.solverTime(240e-6);

@@ -1,0 +1,93 @@
+// Host build of the kernels' arithmetic helpers (ecogen_amd/csrc/eco_math.h)
+// checked against the C library: qdiv bit for bit against IEEE division,
+// pow_pos within its stated ulp bound against pow.
+// Build: g++ -O2 -ffp-contract=off -std=c++17.  Usage:
+//   math_check qdiv <pairs> <seed>          prints the mismatch count
+//   math_check pow <samples> <gamma> <seed> prints max and mean ulp error
+//   math_check plain <pairs> <seed>         mismatches of a*RN(1/b) alone
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+
+#include "../ecogen_amd/csrc/eco_math.h"
+
+static uint64_t st;
+static uint64_t next64() {
+  st += 0x9E3779B97F4A7C15ull;
+  uint64_t z = st;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+static double u01() { return (double)(next64() >> 11) * (1.0 / 9007199254740992.0); }
+static double draw(int emin, int emax) {
+  uint64_t m = next64() & 0xFFFFFFFFFFFFFull;
+  int e = emin + (int)(next64() % (uint64_t)(emax - emin + 1));
+  uint64_t b = ((uint64_t)(e + 1023) << 52) | m;
+  if (next64() & 1) b |= 1ull << 63;
+  double d;
+  std::memcpy(&d, &b, 8);
+  return d;
+}
+static double set_low_bits(double x, uint64_t mask) {
+  uint64_t b;
+  std::memcpy(&b, &x, 8);
+  b |= mask;
+  std::memcpy(&x, &b, 8);
+  return x;
+}
+
+static long check_div(long n, bool plain) {
+  long bad = 0;
+  for (long i = 0; i < n; i++) {
+    double b = draw(-40, 40), a = draw(-60, 60);
+    if (i % 4 == 1) b = set_low_bits(b, 0xFFFFFFFFFFF00ull);  // near all-ones significands
+    if (i % 4 == 2) a = set_low_bits(a, 0xFFFFFFFFFFFF0ull);
+    if (i % 8 == 3) b = std::ldexp(1.0, (int)(next64() % 40) - 20) * (1.0 + std::ldexp((double)(next64() % 16), -52));
+    volatile double y = 1.0 / b;
+    const double t = a / b;
+    const double q = plain ? a * y : eco::qdiv(a, b, y);
+    if (std::memcmp(&t, &q, 8) != 0) bad++;
+  }
+  return bad;
+}
+
+static void check_pow(long n, double g) {
+  double mx = 0., sum = 0.;
+  for (long i = 0; i < n; i++) {
+    double x = std::exp((u01() * 2. - 1.) * std::log(1e3));  // log-uniform in [1e-3, 1e3]
+    if (i % 4 == 0) x = 1. + (u01() * 2. - 1.) * 0.3;         // weak waves
+    const double a = eco::pow_pos(x, g), r = std::pow(x, g);
+    const double ulp = std::nextafter(std::fabs(r), INFINITY) - std::fabs(r);
+    const double e = std::fabs(a - r) / ulp;
+    mx = e > mx ? e : mx;
+    sum += e;
+  }
+  std::printf("%.3f %.4f\n", mx, sum / n);
+}
+
+int main(int argc, char **argv) {
+  if (argc < 2) return 1;
+  if (!std::strcmp(argv[1], "qdiv") || !std::strcmp(argv[1], "plain")) {
+    const long n = argc > 2 ? std::atol(argv[2]) : 1000000;
+    st = argc > 3 ? std::strtoull(argv[3], nullptr, 0) : 1;
+    std::printf("%ld\n", check_div(n, argv[1][0] == 'p'));
+    return 0;
+  }
+  if (!std::strcmp(argv[1], "pow")) {
+    const long n = argc > 2 ? std::atol(argv[2]) : 1000000;
+    const double g = argc > 3 ? std::atof(argv[3]) : 1.4;
+    st = argc > 4 ? std::strtoull(argv[4], nullptr, 0) : 1;
+    check_pow(n, g);
+    // special values follow the library pow
+    const double sp[4] = {0., -1., INFINITY, NAN};
+    for (double x : sp) {
+      const double a = eco::pow_pos(x, g), r = std::pow(x, g);
+      if (!((a == r) || (std::isnan(a) && std::isnan(r)))) { std::printf("special %g\n", x); return 2; }
+    }
+    return 0;
+  }
+  return 1;
+}
