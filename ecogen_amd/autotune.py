@@ -8,8 +8,8 @@ timing on the GPU:
     python -m ecogen_amd.autotune apply R.json      # write the fastest into csrc/eco_tuning.h
 
 A variant is a set of build-time macros of ``csrc/eco_tuning.h`` (tile
-shape per dimension, minimum waves per SIMD requested from the compiler,
-compile-time or runtime limiter form).
+shape and workgroup size per dimension, minimum waves per SIMD requested
+from the compiler).
 Each is timed on the stage kernels of a fixed workload with HIP events
 (eco_time_steps), after a parity check of its result against the default
 build's result on the same state.
@@ -32,35 +32,45 @@ PKG = Path(__file__).resolve().parent
 TUNE_DIR = PKG / "_tune"
 CSRC = PKG / "csrc"
 
+# (tile cells per dimension, workgroup threads): a workgroup may be larger
+# than its tile so that the face pass ends in fewer iterations
 SPACE = {
-    2: {"tiles": [(32, 8), (64, 4), (16, 16)], "waves": [1, 2, 3, 4], "limspec": [1]},
-    3: {"tiles": [(16, 4, 4), (32, 4, 2), (8, 8, 4)], "waves": [1, 2], "limspec": [1]},
+    1: {"shapes": [((256,), 256), ((248,), 256), ((256,), 320), ((504,), 512)], "waves": [1, 2]},
+    2: {"shapes": [((32, 8), 256), ((16, 15), 256), ((32, 7), 256), ((16, 16), 320), ((20, 12), 256)],
+        "waves": [2, 3]},
+    3: {"shapes": [((16, 4, 4), 256), ((16, 4, 4), 320), ((8, 8, 4), 320), ((16, 4, 3), 256)], "waves": [1, 2]},
 }
+AXES = "XYZ"
 
 
 def variants(dim):
     out = []
     sp = SPACE[dim]
-    for t, w, ls in itertools.product(sp["tiles"], sp["waves"], sp["limspec"]):
-        if dim == 2:
-            name = f"d2_t{t[0]}x{t[1]}_w{w}_ls{ls}"
-            defs = {"ECO_TILE2_X": t[0], "ECO_TILE2_Y": t[1], "ECO_WAVES2": w, "ECO_LIMSPEC2": ls}
-        else:
-            name = f"d3_t{t[0]}x{t[1]}x{t[2]}_w{w}_ls{ls}"
-            defs = {"ECO_TILE3_X": t[0], "ECO_TILE3_Y": t[1], "ECO_TILE3_Z": t[2], "ECO_WAVES3": w,
-                    "ECO_LIMSPEC3": ls}
+    for (t, nt), w in itertools.product(sp["shapes"], sp["waves"]):
+        name = f"d{dim}_t{'x'.join(map(str, t))}_n{nt}_w{w}"
+        defs = {f"ECO_TILE{dim}_{AXES[a]}": t[a] for a in range(dim)}
+        defs.update({f"ECO_THREADS{dim}": nt, f"ECO_WAVES{dim}": w, f"ECO_LIMSPEC{dim}": 1})
         out.append((name, defs))
     return out
 
 
+# the translation unit whose kernels a dimension's workload runs; the other
+# objects of a variant are those of the main build (python -m ecogen_amd.build)
+TU_FOR_DIM = {1: "eco_k_kapila32.hip", 2: "eco_k_kapila.hip", 3: "eco_k_multip.hip"}
+
+
 def _compile_variant(item):
-    from .build import COMMON, HIPCC, SOURCES, ARCH
+    from .build import ARCH, COMMON, HIPCC, OBJ, SOURCES
     name, defs = item
+    dim = int(name[1])
     vdir = TUNE_DIR / name
     vdir.mkdir(parents=True, exist_ok=True)
     dflags = [f"-D{k}={v}" for k, v in defs.items()]
-    objs = []
+    objs, fresh = [], []
     for src in SOURCES:
+        if src not in (TU_FOR_DIM[dim], "eco_dispatch.cpp"):
+            objs.append(str(OBJ / (Path(src).stem + ".o")))
+            continue
         obj = vdir / (Path(src).stem + ".o")
         cmd = [HIPCC, *COMMON, *dflags, "-c", str(CSRC / src), "-o", str(obj)]
         if src.endswith(".cpp"):
@@ -69,10 +79,11 @@ def _compile_variant(item):
         if r.returncode != 0:
             return name, f"compile failed: {r.stderr[-800:]}"
         objs.append(str(obj))
+        fresh.append(obj)
     lib = vdir / "libecogen_amd.so"
     r = subprocess.run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", str(lib)],
                        capture_output=True, text=True)
-    for o in objs:
+    for o in fresh:
         os.remove(o)
     return name, ("ok" if r.returncode == 0 else f"link failed: {r.stderr[-800:]}")
 
@@ -86,6 +97,9 @@ def cmd_build(args):
 
 def _workload(dim, n):
     from . import workloads as W
+    if dim == 1:
+        case, st, _ = W.riemann_case(1, n, 100.0, 0.5, precision="fp32s")
+        return case, st
     if dim == 2:
         s = n / 4096.0
         return W.shock_cylinders_2d(n=n, rmin=20.0 * s, rmax=80.0 * s)
@@ -111,7 +125,7 @@ def _time_lib(lib_path, case, st, steps):
 def cmd_run(args):
     results = []
     for dim in args.dims:
-        case, st = _workload(dim, args.n2 if dim == 2 else args.n3)
+        case, st = _workload(dim, {1: args.n1, 2: args.n2, 3: args.n3}[dim])
         base_ms, base = _time_lib(PKG / "libecogen_amd.so", case, st, args.steps)
         print(f"dim {dim} default build: {base_ms:.3f} ms/step", flush=True)
         for name, defs in variants(dim):
@@ -137,7 +151,6 @@ def cmd_apply(args):
     hdr = (CSRC / "eco_tuning.h").read_text()
     for dim, r in sorted(best.items()):
         for k, v in r["defs"].items():
-            k = {"ECO_MIN_WAVES_PER_EU": f"ECO_WAVES{dim}"}.get(k, k)
             hdr = re.sub(rf"#define {k} \w+", f"#define {k} {v}", hdr)
         print(f"dim {dim}: {r['variant']} {r['ms_per_step']:.3f} ms/step")
     (CSRC / "eco_tuning.h").write_text(hdr)
@@ -147,10 +160,11 @@ def main(argv=None):
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
     b = sub.add_parser("build")
-    b.add_argument("--dims", type=int, nargs="+", default=[2, 3])
+    b.add_argument("--dims", type=int, nargs="+", default=[1, 2, 3])
     b.add_argument("--jobs", type=int, default=min(8, os.cpu_count() or 1))
     r = sub.add_parser("run")
-    r.add_argument("--dims", type=int, nargs="+", default=[2, 3])
+    r.add_argument("--dims", type=int, nargs="+", default=[1, 2, 3])
+    r.add_argument("--n1", type=int, default=1 << 22)
     r.add_argument("--n2", type=int, default=4096)
     r.add_argument("--n3", type=int, default=256)
     r.add_argument("--steps", type=int, default=10)

@@ -36,13 +36,21 @@ struct Layout {
 // the autotuner (ecogen_amd/autotune.py writes eco_tuning.h).
 #include "eco_tuning.h"
 template <int DIM> struct Tile;
-template <> struct Tile<1> { static constexpr int X = ECO_TILE1_X, Y = 1, Z = 1, W = ECO_WAVES1, LS = ECO_LIMSPEC1; };
+template <> struct Tile<1> {
+  static constexpr int X = ECO_TILE1_X, Y = 1, Z = 1, W = ECO_WAVES1, LS = ECO_LIMSPEC1;
+  static constexpr int C = X * Y * Z, T = ECO_THREADS1 > 0 ? ECO_THREADS1 : C;
+};
 template <> struct Tile<2> {
   static constexpr int X = ECO_TILE2_X, Y = ECO_TILE2_Y, Z = 1, W = ECO_WAVES2, LS = ECO_LIMSPEC2;
+  static constexpr int C = X * Y * Z, T = ECO_THREADS2 > 0 ? ECO_THREADS2 : C;
 };
 template <> struct Tile<3> {
   static constexpr int X = ECO_TILE3_X, Y = ECO_TILE3_Y, Z = ECO_TILE3_Z, W = ECO_WAVES3, LS = ECO_LIMSPEC3;
+  static constexpr int C = X * Y * Z, T = ECO_THREADS3 > 0 ? ECO_THREADS3 : C;
 };
+static_assert(Tile<1>::T >= Tile<1>::C && Tile<1>::T % 64 == 0, "1-D workgroup");
+static_assert(Tile<2>::T >= Tile<2>::C && Tile<2>::T % 64 == 0, "2-D workgroup");
+static_assert(Tile<3>::T >= Tile<3>::C && Tile<3>::T % 64 == 0, "3-D workgroup");
 
 template <int DIM>
 struct TileFaces {
@@ -306,7 +314,14 @@ __device__ __forceinline__ bool step_begin(Control *ctl, const Geometry &G, doub
 // Called by one thread per tile with the tile's largest wave speed m.
 __device__ __forceinline__ void step_end(Control *ctl, double m, int slot, int blk, bool last_kernel,
                                          const StepInfo &si) {
-  if (m > 0.) atomicMax(&ctl->smax[slot % 3], (unsigned long long)__double_as_longlong(m));
+  // Positive doubles order like their bit patterns.  The running maximum only
+  // grows during a step, so a tile whose value does not exceed what a relaxed
+  // read already shows leaves it unchanged: the atomic (one per tile, all on
+  // one address) is issued only by tiles that can raise it.
+  if (m > 0.) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(m);
+    if (b > __atomic_load_n(&ctl->smax[slot % 3], __ATOMIC_RELAXED)) atomicMax(&ctl->smax[slot % 3], b);
+  }
   if (last_kernel && blk == 0) {
     const int cur = slot & 1, prv = cur ^ 1;
     const double t = ((si.cnt == 0) ? 0. : ctl->t[prv]) + si.dt;
@@ -594,7 +609,7 @@ template <int MODEL, int NPH, int DIM>
 struct StageLds {
   using Ly = Layout<MODEL, NPH, DIM>;
   using TF = TileFaces<DIM>;
-  static constexpr int NT = Tile<DIM>::X * Tile<DIM>::Y * Tile<DIM>::Z;
+  static constexpr int NT = Tile<DIM>::T;  // threads
   static constexpr bool MERGED = (DIM <= 2);
   static constexpr int NF0 = TF::nf(0), NF1 = TF::nf(1);
   static constexpr int NFT = MERGED ? NF0 + NF1 : TF::MAXF;  // LDS flux slots
@@ -640,7 +655,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int slot, c
   const int o[3] = {bx * TX, by * TY, bz * TZ};
   const int lx = tid % TX, ly = (tid / TX) % TY, lz = tid / (TX * TY);
   const int ci[3] = {o[0] + lx, o[1] + ly, o[2] + lz};
-  const bool active = ci[0] < nx && ci[1] < ny && ci[2] < nz;
+  const bool active = tid < Tile<DIM>::C && ci[0] < nx && ci[1] < ny && ci[2] < nz;
   const long long strd[3] = {1, (long long)nx, (long long)nx * ny};
   const long long cidx = active ? ci[0] + strd[1] * ci[1] + strd[2] * ci[2] : 0;
 
@@ -739,7 +754,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int slot, c
 }
 
 template <int MODEL, int NPH, int DIM, int STAGE, bool SAMELIM>
-__global__ void __launch_bounds__(Tile<DIM>::X * Tile<DIM>::Y * Tile<DIM>::Z, Tile<DIM>::W)
+__global__ void __launch_bounds__(Tile<DIM>::T, Tile<DIM>::W)
 k_stage(const StageArgs A) {
   __shared__ StageLds<MODEL, NPH, DIM> lds;
   stage_tile<MODEL, NPH, DIM, STAGE, SAMELIM>(A, A.step, blockIdx.x, lds);
@@ -750,7 +765,7 @@ k_stage(const StageArgs A) {
 // count is read from device memory, so a captured launch stays valid while
 // the table changes between replays.
 template <int MODEL, int NPH, int DIM, int STAGE, bool SAMELIM>
-__global__ void __launch_bounds__(Tile<DIM>::X * Tile<DIM>::Y * Tile<DIM>::Z, Tile<DIM>::W)
+__global__ void __launch_bounds__(Tile<DIM>::T, Tile<DIM>::W)
 k_stage_batch(const BatchDesc *__restrict__ D, const int slot) {
   __shared__ StageLds<MODEL, NPH, DIM> lds;
   const int ntiles = D->ntiles;
@@ -782,7 +797,7 @@ __global__ void k_init(const StageArgs A) {
 // ---------------------------------------------------------------------------
 template <int MODEL, int NPH, int DIM>
 int launch_t(const StageArgs &A, int stage, hipStream_t st) {
-  constexpr int NT = Tile<DIM>::X * Tile<DIM>::Y * Tile<DIM>::Z;
+  constexpr int NT = Tile<DIM>::T;
   const int tgx = (A.geo.n[0] + Tile<DIM>::X - 1) / Tile<DIM>::X;
   const int tgy = (A.geo.n[1] + Tile<DIM>::Y - 1) / Tile<DIM>::Y;
   const int tgz = (A.geo.n[2] + Tile<DIM>::Z - 1) / Tile<DIM>::Z;
@@ -812,7 +827,7 @@ int launch_d(const StageArgs &A, int stage, hipStream_t st) {
 
 template <int MODEL, int NPH, int DIM, int STAGE, bool SAME>
 int launch_tb1(const BatchLaunch &L, hipStream_t st) {
-  constexpr int NT = Tile<DIM>::X * Tile<DIM>::Y * Tile<DIM>::Z;
+  constexpr int NT = Tile<DIM>::T;
   if (L.occupancy)
     return (int)hipOccupancyMaxActiveBlocksPerMultiprocessor(L.occupancy, k_stage_batch<MODEL, NPH, DIM, STAGE, SAME>,
                                                              NT, 0);

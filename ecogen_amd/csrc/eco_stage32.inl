@@ -344,7 +344,7 @@ __device__ __forceinline__ void stage32_tile(const StageArgs &A, const int slot,
   const int o[3] = {bx * TX, by * TY, bz * TZ};
   const int lx = tid % TX, ly = (tid / TX) % TY, lz = tid / (TX * TY);
   const int ci[3] = {o[0] + lx, o[1] + ly, o[2] + lz};
-  const bool active = ci[0] < nx && ci[1] < ny && ci[2] < nz;
+  const bool active = tid < Tile<DIM>::C && ci[0] < nx && ci[1] < ny && ci[2] < nz;
   const long long strd[3] = {1, (long long)nx, (long long)nx * ny};
   const long long cidx = active ? ci[0] + strd[1] * ci[1] + strd[2] * ci[2] : 0;
 
@@ -442,7 +442,7 @@ __device__ __forceinline__ void stage32_tile(const StageArgs &A, const int slot,
 }
 
 template <int MODEL, int NPH, int DIM, int STAGE, bool SAMELIM>
-__global__ void __launch_bounds__(Tile<DIM>::X * Tile<DIM>::Y * Tile<DIM>::Z, Tile<DIM>::W)
+__global__ void __launch_bounds__(Tile<DIM>::T, Tile<DIM>::W)
 k_stage32(const StageArgs A) {
   __shared__ StageLds<MODEL, NPH, DIM> lds;
   stage32_tile<MODEL, NPH, DIM, STAGE, SAMELIM>(A, A.step, blockIdx.x, lds);
@@ -450,7 +450,7 @@ k_stage32(const StageArgs A) {
 
 // persistent form over a pool (as k_stage_batch)
 template <int MODEL, int NPH, int DIM, int STAGE, bool SAMELIM>
-__global__ void __launch_bounds__(Tile<DIM>::X * Tile<DIM>::Y * Tile<DIM>::Z, Tile<DIM>::W)
+__global__ void __launch_bounds__(Tile<DIM>::T, Tile<DIM>::W)
 k_stage32_batch(const BatchDesc *__restrict__ D, const int slot) {
   __shared__ StageLds<MODEL, NPH, DIM> lds;
   const int ntiles = D->ntiles;
@@ -502,7 +502,7 @@ __global__ void k_from_f32(const StageArgs A, double *__restrict__ dst) {
 
 template <int MODEL, int NPH, int DIM>
 int launch_t32(const StageArgs &A, int stage, hipStream_t st, double *tmp) {
-  constexpr int NT = Tile<DIM>::X * Tile<DIM>::Y * Tile<DIM>::Z;
+  constexpr int NT = Tile<DIM>::T;
   const int tgx = (A.geo.n[0] + Tile<DIM>::X - 1) / Tile<DIM>::X;
   const int tgy = (A.geo.n[1] + Tile<DIM>::Y - 1) / Tile<DIM>::Y;
   const int tgz = (A.geo.n[2] + Tile<DIM>::Z - 1) / Tile<DIM>::Z;
@@ -538,7 +538,7 @@ int launch_d32(const StageArgs &A, int stage, hipStream_t st, double *tmp) {
 
 template <int MODEL, int NPH, int DIM, int STAGE, bool SAME>
 int launch_tb32_1(const BatchLaunch &L, hipStream_t st) {
-  constexpr int NT = Tile<DIM>::X * Tile<DIM>::Y * Tile<DIM>::Z;
+  constexpr int NT = Tile<DIM>::T;
   if (L.occupancy)
     return (int)hipOccupancyMaxActiveBlocksPerMultiprocessor(L.occupancy,
                                                              k_stage32_batch<MODEL, NPH, DIM, STAGE, SAME>, NT, 0);

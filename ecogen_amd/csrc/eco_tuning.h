@@ -2,7 +2,7 @@
 // tile/occupancy sweep of ecogen_amd/autotune.py on MI355X.
 #pragma once
 #ifndef ECO_TILE1_X
-#define ECO_TILE1_X 256
+#define ECO_TILE1_X 504
 #endif
 #ifndef ECO_TILE2_X
 #define ECO_TILE2_X 32
@@ -17,19 +17,19 @@
 #define ECO_TILE3_Y 4
 #endif
 #ifndef ECO_TILE3_Z
-#define ECO_TILE3_Z 4
+#define ECO_TILE3_Z 3
 #endif
 #ifndef ECO_MIN_WAVES_PER_EU
 #define ECO_MIN_WAVES_PER_EU 1
 #endif
 #ifndef ECO_WAVES1
-#define ECO_WAVES1 ECO_MIN_WAVES_PER_EU
+#define ECO_WAVES1 1
 #endif
 #ifndef ECO_WAVES2
 #define ECO_WAVES2 2
 #endif
 #ifndef ECO_WAVES3
-#define ECO_WAVES3 1
+#define ECO_WAVES3 2
 #endif
 // compile-time (branch-free) limiter for the single-limiter configurations
 #ifndef ECO_LIMSPEC1
@@ -40,4 +40,16 @@
 #endif
 #ifndef ECO_LIMSPEC3
 #define ECO_LIMSPEC3 1
+#endif
+// workgroup size (threads) per dimension; 0 = one thread per tile cell.  A
+// workgroup larger than its tile lets the face pass (more faces than cells)
+// finish in fewer iterations; the extra threads idle in the cell update.
+#ifndef ECO_THREADS1
+#define ECO_THREADS1 512
+#endif
+#ifndef ECO_THREADS2
+#define ECO_THREADS2 256
+#endif
+#ifndef ECO_THREADS3
+#define ECO_THREADS3 256
 #endif
