@@ -300,7 +300,8 @@ __device__ __forceinline__ bool step_begin(Control *ctl, const Geometry &G, doub
   }
   double dt = 1.e-15;  // m_dt of Run::Run (src/Run.cpp:43)
   if (cnt > 0) {
-    unsigned long long bits = __atomic_load_n(&ctl->smax[(slot + 2) % 3], __ATOMIC_RELAXED);
+    // written by the previous step's kernels: visible at this launch
+    const unsigned long long bits = ctl->smax[(slot + 2) % 3];
     double sm = __longlong_as_double((long long)bits);
     double dtMax = (sm > 0.) ? smin(1.e10, G.lcfl / sm) : 1.e10;
     dt = cfl * dtMax;
@@ -544,12 +545,14 @@ __device__ __forceinline__ double face_fp64(const StageArgs &A, const int *o, co
       double wl_[Ly::NVAR], wr_[Ly::NVAR], wll[Ly::NVAR], whh[Ly::NVAR];
       prim_to_vec<MODEL, NPH, DIM>(L, wl_);
       prim_to_vec<MODEL, NPH, DIM>(R, wr_);
-      // all neighbour loads issued before any arithmetic
+      // all neighbour loads issued before any arithmetic, unconditionally (a
+      // missing neighbour reads the face's own cell, in bounds; its slope is 0)
+      const long long jm = has_m ? ilo - sd : ilo, jp = has_p ? ihi + sd : ihi;
 #pragma unroll
       for (int v = 0; v < Ly::NVAR; v++) {
         const double *Fv = Wsrc + (long long)v * nc;
-        wll[v] = has_m ? Fv[ilo - sd] : 0.;
-        whh[v] = has_p ? Fv[ihi + sd] : 0.;
+        wll[v] = Fv[jm];
+        whh[v] = Fv[jp];
       }
       double xl[Ly::NVAR], xr[Ly::NVAR];
       muscl_dispatch<MODEL, NPH, DIM, SAMELIM, false>(S, wl_, wr_, wll, whh, has_m, has_p, dcc, dcm, dcp, dL, dR, ifl,

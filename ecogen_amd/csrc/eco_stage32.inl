@@ -228,22 +228,25 @@ __device__ __forceinline__ float face_fp32s(const StageArgs &A, const int *o, co
   const long long sd = strd[dir];
   const int i = gi - 1;
   const bool has_m = (i >= 1), has_p = (i + 2 <= ndir - 1);
+  // every load unconditional (a missing neighbour reads the face's own cell,
+  // in bounds, and its slope is replaced by 0): no branch, no wait between loads
+  const long long jm = has_m ? ilo - sd : ilo, jp = has_p ? ihi + sd : ihi;
   float wl_[Ly::NVAR], wr_[Ly::NVAR], wll[Ly::NVAR], whh[Ly::NVAR];
 #pragma unroll
   for (int v = 0; v < Ly::NVAR; v++) {
     const float *Fv = Wsrc + (long long)v * nc;
     wl_[v] = Fv[ilo];
     wr_[v] = Fv[ihi];
-    wll[v] = has_m ? Fv[ilo - sd] : 0.f;
-    whh[v] = has_p ? Fv[ihi + sd] : 0.f;
+    wll[v] = Fv[jm];
+    whh[v] = Fv[jp];
   }
   double wlv[Ly::NVAR], wrv[Ly::NVAR], wlm[Ly::NVAR], whp[Ly::NVAR];
 #pragma unroll
   for (int v = 0; v < Ly::NVAR; v++) { wlv[v] = wl_[v]; wrv[v] = wr_[v]; wlm[v] = wll[v]; whp[v] = whh[v]; }
   decode_alpha(wl_[0], wl_[3], wlv[0], wlv[3]);
   decode_alpha(wr_[0], wr_[3], wrv[0], wrv[3]);
-  if (has_m) decode_alpha(wll[0], wll[3], wlm[0], wlm[3]);
-  if (has_p) decode_alpha(whh[0], whh[3], whp[0], whp[3]);
+  decode_alpha(wll[0], wll[3], wlm[0], wlm[3]);
+  decode_alpha(whh[0], whh[3], whp[0], whp[3]);
   const double aL = wlv[0], aR = wrv[0];
   double aLL = 0., aRR = 0., tmp;
   if (!SAMELIM || S.thinc) {
