@@ -265,17 +265,26 @@ def leg_served(args):
     base = W.request_stream(1.0, args.serve_requests)
     reqcases = [W.request_case(kind, prm, precision="fp32s") for _, kind, prm in base]
     cells = np.array([int(np.prod(c["cells"])) for c, _, _ in reqcases])
-    # serial plain FP64 service times
-    service = []
+    # serial plain FP64 service times.  Each solve is capped at four times its
+    # CFL step estimate (t_final = 0.3 L / a_max: about 0.3 n / (cfl * lcfl
+    # factor) steps) plus 200: a request whose FP64 trajectory collapses dt (a
+    # chaotic 3-D bubble collapse can) then counts with a lower bound of its
+    # service time, which favours the serial comparator.
+    service, capped = [], 0
     for c, st, tf in reqcases:
+        dim = sum(1 for n in c["cells"] if n > 1)
+        est = 0.3 * max(c["cells"]) / (c["cfl"] * (0.6 if dim > 1 else 1.0))
         t1 = time.perf_counter()
         with Context(dict(c, precision="fp64")) as ctx:
             ctx.set_state(st)
-            ctx.run(t_final=tf)
+            s64 = ctx.run(t_final=tf, max_steps=int(4 * est) + 200)
             ctx.get_state()
         service.append(time.perf_counter() - t1)
+        capped += int(s64.t < float(np.float32(tf)))
     out = {"requests_per_rate": len(reqcases), "bubble_3d_requests": sum(1 for _, k, _ in base if k == "bubbles"),
-           "hbm_budget_gb": 200, "rates": {}}
+           "hbm_budget_gb": 200, "serial_fp64_capped_requests": capped,
+           "serial_fp64_step_cap": "4x the CFL step estimate + 200",
+           "rates": {}}
     for lam in args.serve_rates:
         arrivals = [a / lam for a, _, _ in base]
         reqs = [{"case": c, "state": st, "t_final": tf, "arrival_s": arr}

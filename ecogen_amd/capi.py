@@ -178,7 +178,11 @@ class Context:
         _check(lib().eco_step(self._h, C.byref(st)), "eco_step")
         return st
 
-    def run(self, nsteps: int = 0, t_final: float = 0.0) -> EcoStatus:
+    def run(self, nsteps: int = 0, t_final: float = 0.0, max_steps: int = 0) -> EcoStatus:
+        """nsteps > 0: exactly nsteps steps; otherwise until t >= t_final (as
+        float), stopping after max_steps steps if max_steps > 0."""
+        if nsteps <= 0 and max_steps > 0:
+            nsteps = -int(max_steps)
         st = EcoStatus()
         _check(lib().eco_run(self._h, int(nsteps), float(t_final), C.byref(st)), "eco_run")
         return st

@@ -421,7 +421,7 @@ extern "C" int eco_run(eco_ctx *x, int nsteps, double t_final, eco_status *st) {
     }
     return eco_sync(x, st);
   }
-  rc = set_target(x, 0, t_final);
+  rc = set_target(x, nsteps < 0 ? x->step - nsteps : 0, t_final);
   if (rc) return rc;
   // Time control: enqueue chunks, check the device done flag between chunks.
   int chunk = 16;

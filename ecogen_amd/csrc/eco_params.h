@@ -69,7 +69,7 @@ struct Control {
   int steps;                   // steps taken (host view)
   int finished;                // target reached (host view; cleared by the host)
   unsigned int err;            // soft-error flags (ECO_WARN_*)
-  int max_steps;               // iteration-mode step count
+  int max_steps;               // iteration mode: step count; time mode: cap (0 = none)
   double dt_last;              // dt used by the last step taken
   double t_now;                // physical time after the last step taken
   double t_final;              // <0: iteration mode

@@ -332,8 +332,10 @@ __device__ __forceinline__ void step_end(Control *ctl, double m, int slot, int b
     ctl->steps = n;
     ctl->dt_last = si.dt;
     ctl->t_now = t;
-    // iteration or physical-time control (src/Run.cpp:239-244, tFinal is a float)
-    const int done = (ctl->t_final < 0.) ? (n >= ctl->max_steps) : (t >= (double)(float)ctl->t_final);
+    // iteration or physical-time control (src/Run.cpp:239-244, tFinal is a
+    // float); a time-controlled run may also carry a step cap (max_steps > 0)
+    const int done = (ctl->t_final < 0.) ? (n >= ctl->max_steps)
+                                          : (t >= (double)(float)ctl->t_final || (ctl->max_steps > 0 && n >= ctl->max_steps));
     ctl->done[cur] = done;
     if (done) ctl->finished = 1;
   }

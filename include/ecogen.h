@@ -104,7 +104,9 @@ int eco_step(eco_ctx *ctx, eco_status *status);
 
 /* The time loop of Run::solver: nsteps > 0 runs exactly nsteps more steps
    (iteration control), nsteps <= 0 runs until t >= t_final compared as float
-   (physicalTime control, src/Run.cpp:239-244, Input.cpp:163).  Synchronises. */
+   (physicalTime control, src/Run.cpp:239-244, Input.cpp:163); nsteps < 0
+   also stops after -nsteps more steps (a guard the reference does not have;
+   status->t tells whether t_final was reached).  Synchronises. */
 int eco_run(eco_ctx *ctx, int nsteps, double t_final, eco_status *status);
 
 /* Asynchronous form: enqueue nsteps steps on the context stream, no host sync.
