@@ -35,10 +35,10 @@ CSRC = PKG / "csrc"
 # (tile cells per dimension, workgroup threads): a workgroup may be larger
 # than its tile so that the face pass ends in fewer iterations
 SPACE = {
-    1: {"shapes": [((256,), 256), ((248,), 256), ((256,), 320), ((504,), 512)], "waves": [1, 2]},
+    1: {"shapes": [((504,), 512), ((248,), 256), ((1016,), 1024), ((256,), 256)], "waves": [1, 2]},
     2: {"shapes": [((32, 8), 256), ((16, 15), 256), ((32, 7), 256), ((16, 16), 320), ((20, 12), 256)],
-        "waves": [2, 3]},
-    3: {"shapes": [((16, 4, 4), 256), ((16, 4, 4), 320), ((8, 8, 4), 320), ((16, 4, 3), 256)], "waves": [1, 2]},
+        "waves": [2]},
+    3: {"shapes": [((16, 4, 3), 256), ((16, 4, 4), 256), ((16, 4, 4), 320), ((8, 8, 4), 320)], "waves": [2]},
 }
 AXES = "XYZ"
 
