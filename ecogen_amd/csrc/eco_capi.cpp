@@ -34,7 +34,14 @@ struct eco_ctx {
   Control *ctl;
   int step;             // next step index to launch
   int have_state;
+  hipStream_t cap;      // private stream for graph capture
+  hipGraphExec_t gexec[6];  // kGraphSteps steps from a start slot of each residue mod 6
 };
+
+// Steps per captured graph: a multiple of 6, so that the slot-indexed ring
+// (slot % 3) and ping-pong (slot & 1) entries of a replay repeat those of
+// the capture whatever the start slot with the same residue.
+static const int kGraphSteps = 48;
 
 static int hipcode(hipError_t e) { return e == hipSuccess ? ECO_OK : ECO_ERR_HIP; }
 
@@ -380,6 +387,49 @@ static int set_target(eco_ctx *x, int max_steps, double t_final) {
   return hipcode(e);
 }
 
+// Graph of kGraphSteps steps starting at a slot congruent to r (mod 6),
+// captured on first use.  Its kernel arguments depend on the slot only
+// through its parity (buffer roles) and its residues mod 2 and 3.
+static int graph_for(eco_ctx *x, int r, hipGraphExec_t *out) {
+  if (x->gexec[r]) { *out = x->gexec[r]; return ECO_OK; }
+  if (!x->cap && hipStreamCreateWithFlags(&x->cap, hipStreamNonBlocking) != hipSuccess) return ECO_ERR_HIP;
+  if (hipStreamBeginCapture(x->cap, hipStreamCaptureModeThreadLocal) != hipSuccess) return ECO_ERR_HIP;
+  const int save_step = x->step;
+  hipStream_t save_stream = x->stream;
+  x->step = r;
+  x->stream = x->cap;
+  int rc = ECO_OK;
+  for (int i = 0; i < kGraphSteps && rc == ECO_OK; i++) rc = launch_one(x);
+  x->stream = save_stream;
+  x->step = save_step;
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(x->cap, &g);
+  if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+  if (e != hipSuccess) return ECO_ERR_HIP;
+  e = hipGraphInstantiate(&x->gexec[r], g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) { x->gexec[r] = nullptr; return ECO_ERR_HIP; }
+  *out = x->gexec[r];
+  return ECO_OK;
+}
+
+// Enqueues nsteps steps: whole graphs of kGraphSteps, then single launches.
+static int launch_steps(eco_ctx *x, int nsteps) {
+  while (nsteps >= kGraphSteps) {
+    hipGraphExec_t ge;
+    int rc = graph_for(x, x->step % 6, &ge);
+    if (rc) return rc;
+    if (hipGraphLaunch(ge, x->stream) != hipSuccess) return ECO_ERR_HIP;
+    x->step += kGraphSteps;
+    nsteps -= kGraphSteps;
+  }
+  for (; nsteps > 0; nsteps--) {
+    int rc = launch_one(x);
+    if (rc) return rc;
+  }
+  return ECO_OK;
+}
+
 extern "C" int eco_launch_steps(eco_ctx *x, int nsteps) {
   if (!x || nsteps < 0) return ECO_ERR_INVALID;
   if (!x->have_state) return ECO_ERR_STATE;
@@ -414,27 +464,23 @@ extern "C" int eco_run(eco_ctx *x, int nsteps, double t_final, eco_status *st) {
   int rc;
   if (nsteps > 0) {
     rc = set_target(x, x->step + nsteps, -1.);
+    if (!rc) rc = launch_steps(x, nsteps);
     if (rc) return rc;
-    for (int i = 0; i < nsteps; i++) {
-      rc = launch_one(x);
-      if (rc) return rc;
-    }
     return eco_sync(x, st);
   }
   rc = set_target(x, nsteps < 0 ? x->step - nsteps : 0, t_final);
   if (rc) return rc;
-  // Time control: enqueue chunks, check the device done flag between chunks.
-  int chunk = 16;
+  // Time control: enqueue chunks of captured steps, check the device done
+  // flag between chunks (steps past the target are no-ops on the device).
+  int chunk = kGraphSteps;
   eco_status s;
   for (;;) {
-    for (int i = 0; i < chunk; i++) {
-      rc = launch_one(x);
-      if (rc) return rc;
-    }
+    rc = launch_steps(x, chunk);
+    if (rc) return rc;
     rc = eco_sync(x, &s);
     if (rc) return rc;
     if (s.done) break;
-    chunk = std::min(chunk * 2, 512);
+    chunk = std::min(chunk * 2, 4 * kGraphSteps);
   }
   if (st) *st = s;
   return ECO_OK;
@@ -509,6 +555,9 @@ extern "C" const double *eco_state_device(eco_ctx *x, int *nfields_device) {
 
 extern "C" void eco_destroy(eco_ctx *x) {
   if (!x) return;
+  for (auto &g : x->gexec)
+    if (g) (void)hipGraphExecDestroy(g);
+  if (x->cap) (void)hipStreamDestroy(x->cap);
   if (x->buf[0]) (void)hipFree(x->buf[0]);
   if (x->buf[1]) (void)hipFree(x->buf[1]);
   if (x->bstar) (void)hipFree(x->bstar);
