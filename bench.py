@@ -43,8 +43,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=4096, help="cells per side")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-n", type=int, default=1024, help="cells per side of the CPU sample")
-    ap.add_argument("--cpu-steps", type=int, default=12)
+    ap.add_argument("--cpu-n", type=int, default=2048, help="cells per side of the CPU sample")
+    ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--no-fp32s", action="store_true", help="skip the FP32-storage measurement")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain copy kernel of known size (for HBM counter calibration)")

@@ -32,12 +32,16 @@ struct BatchDesc {
   int ntiles;
 };
 
-// Launch of a pool's persistent stage kernel for one class.
+// Launch of a pool's persistent stage kernel for one class.  The tables are
+// passed as kernel pointer arguments (device pointers): the backend then
+// knows them, and the state pointers read from them, to be global memory.
 struct BatchLaunch {
-  int dim, stage, same;   // class within the model (stage 0, 1, 2 as launch_stage)
-  const BatchDesc *desc;  // device pointer
+  int dim, stage, same;      // class within the model (stage 0, 1, 2 as launch_stage)
+  const StageArgs *args;     // four per case (BatchDesc::args)
+  const TileRef *tiles;      // BatchDesc::tiles
+  const int *ntiles;         // &BatchDesc::ntiles on the device
   int slot, grid;
-  int *occupancy;         // non-null: report workgroups per CU for this kernel, no launch
+  int *occupancy;            // non-null: report workgroups per CU for this kernel, no launch
 };
 int launch_batch(int model, int prec, const BatchLaunch &L, hipStream_t stream);
 

@@ -268,7 +268,7 @@ static int class_for(eco_pool *p, const ClassKey &key) {
   for (int i = p->max_cases - 1; i >= 0; i--) c.free_argslots.push_back(i);
   // persistent grid: resident workgroups of the heavier (second) kernel of the step
   int occ = 0;
-  BatchLaunch L{key.dim, key.order == 1 ? 0 : 2, key.same, nullptr, 0, 0, &occ};
+  BatchLaunch L{key.dim, key.order == 1 ? 0 : 2, key.same, nullptr, nullptr, nullptr, 0, 0, &occ};
   if (launch_batch(key.model, key.prec, L, p->cs) != 0 || occ <= 0) occ = 1;
   c.grid = p->pc.grid_blocks > 0 ? p->pc.grid_blocks : occ * p->cus;
   p->classes.push_back(std::move(c));
@@ -405,6 +405,9 @@ static int upload_tables(eco_pool *p) {
       if (cl.d_tiles) (void)hipFree(cl.d_tiles);
       cl.tiles_cap = std::max(nt, cl.tiles_cap * 2);
       if (hipMalloc(&cl.d_tiles, cl.tiles_cap * sizeof(TileRef)) != hipSuccess) { cl.d_tiles = nullptr; return ECO_ERR_NOMEM; }
+      // the captured launches hold the table pointer: capture again
+      if (p->gexec) { (void)hipGraphExecDestroy(p->gexec); p->gexec = nullptr; }
+      if (p->graph) { (void)hipGraphDestroy(p->graph); p->graph = nullptr; }
     }
     const size_t bytes = nt * sizeof(TileRef) + sizeof(BatchDesc);
     char *h = (char *)pinned_get(p, bytes);
@@ -439,7 +442,8 @@ static int enqueue_steps(eco_pool *p, const std::vector<int> &cls, long long bas
         Class &cl = p->classes[c];
         const int stage = (cl.key.order == 1) ? (pass == 0 ? 0 : -1) : pass + 1;
         if (stage < 0) continue;
-        BatchLaunch L{cl.key.dim, stage, cl.key.same, cl.d_desc, sl, cl.grid, nullptr};
+        BatchLaunch L{cl.key.dim, stage, cl.key.same, cl.d_args, cl.d_tiles, &cl.d_desc->ntiles, sl, cl.grid,
+                      nullptr};
         int rc = launch_batch(cl.key.model, cl.key.prec, L, p->cs);
         if (rc) return rc < 0 ? ECO_ERR_UNSUPPORTED : ECO_ERR_HIP;
       }

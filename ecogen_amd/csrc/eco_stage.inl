@@ -32,6 +32,18 @@ struct Layout {
   static constexpr int NACC = NFV - 1;                                              // accumulated values
 };
 
+// Device-memory pointers taken from an argument table are generic pointers
+// to the compiler (only kernel arguments are known to be global), and
+// accesses through them become flat memory operations.  Every state and
+// table pointer of the stage kernels is global memory: the kernels carry it
+// in pointers of the global address space, so that loads and stores are
+// global ones.
+#define ECO_GLOBAL __attribute__((address_space(1)))
+template <typename T>
+__device__ __forceinline__ ECO_GLOBAL T *gmem(T *p) {
+  return (ECO_GLOBAL T *)p;
+}
+
 // Tile shapes and the occupancy request are build-time parameters chosen by
 // the autotuner (ecogen_amd/autotune.py writes eco_tuning.h).
 #include "eco_tuning.h"
@@ -39,14 +51,17 @@ template <int DIM> struct Tile;
 template <> struct Tile<1> {
   static constexpr int X = ECO_TILE1_X, Y = 1, Z = 1, W = ECO_WAVES1, LS = ECO_LIMSPEC1;
   static constexpr int C = X * Y * Z, T = ECO_THREADS1 > 0 ? ECO_THREADS1 : C;
+  static constexpr int WB = ECO_WAVESB1 > 0 ? ECO_WAVESB1 : W;
 };
 template <> struct Tile<2> {
   static constexpr int X = ECO_TILE2_X, Y = ECO_TILE2_Y, Z = 1, W = ECO_WAVES2, LS = ECO_LIMSPEC2;
   static constexpr int C = X * Y * Z, T = ECO_THREADS2 > 0 ? ECO_THREADS2 : C;
+  static constexpr int WB = ECO_WAVESB2 > 0 ? ECO_WAVESB2 : W;
 };
 template <> struct Tile<3> {
   static constexpr int X = ECO_TILE3_X, Y = ECO_TILE3_Y, Z = ECO_TILE3_Z, W = ECO_WAVES3, LS = ECO_LIMSPEC3;
   static constexpr int C = X * Y * Z, T = ECO_THREADS3 > 0 ? ECO_THREADS3 : C;
+  static constexpr int WB = ECO_WAVESB3 > 0 ? ECO_WAVESB3 : W;
 };
 static_assert(Tile<1>::T >= Tile<1>::C && Tile<1>::T % 64 == 0, "1-D workgroup");
 static_assert(Tile<2>::T >= Tile<2>::C && Tile<2>::T % 64 == 0, "2-D workgroup");
@@ -62,8 +77,8 @@ struct TileFaces {
 // ---------------------------------------------------------------------------
 // state access
 // ---------------------------------------------------------------------------
-template <int MODEL, int NPH, int DIM>
-__device__ __forceinline__ void load_prim(const double *__restrict__ W, long long nc, long long idx, CellState<NPH> &s) {
+template <int MODEL, int NPH, int DIM, typename PW>
+__device__ __forceinline__ void load_prim(PW W, long long nc, long long idx, CellState<NPH> &s) {
   using Ly = Layout<MODEL, NPH, DIM>;
   if constexpr (MODEL == MODEL_EULER) {
     s.r[0] = W[idx];
@@ -84,8 +99,8 @@ __device__ __forceinline__ void load_prim(const double *__restrict__ W, long lon
   (void)Ly::NPRIM;
 }
 
-template <int MODEL, int NPH, int DIM>
-__device__ __forceinline__ void store_prim(double *__restrict__ W, long long nc, long long idx, const CellState<NPH> &s) {
+template <int MODEL, int NPH, int DIM, typename PW>
+__device__ __forceinline__ void store_prim(PW W, long long nc, long long idx, const CellState<NPH> &s) {
   if constexpr (MODEL == MODEL_EULER) {
     W[idx] = s.r[0];
     W[nc + idx] = s.p[0];
@@ -269,8 +284,8 @@ __device__ __forceinline__ void unpark_acc(double (*A)[NT], int t, FluxVec<NPH> 
 }
 
 // time-n alpha_k, p_k of one cell: the non-conservative terms' coefficients
-template <int MODEL, int NPH, int DIM>
-__device__ __forceinline__ void load_nc_coeffs(const double *__restrict__ W, long long nc, long long idx, CellState<NPH> &s) {
+template <int MODEL, int NPH, int DIM, typename PW>
+__device__ __forceinline__ void load_nc_coeffs(PW W, long long nc, long long idx, CellState<NPH> &s) {
   if constexpr (MODEL != MODEL_EULER) {
 #pragma unroll
     for (int k = 0; k < NPH; k++) { s.a[k] = W[(3 * k) * nc + idx]; s.p[k] = W[(3 * k + 2) * nc + idx]; }
@@ -489,8 +504,8 @@ __device__ __forceinline__ double face_fp64(const StageArgs &A, const int *o, co
   const SchemeParams &S = A.sch;
   const long long nc = G.ncell;
   const int nx = G.n[0], ny = G.n[1], nz = G.n[2];
-  const double *__restrict__ Wsrc = A.wsrc;
-  const double *__restrict__ Wn = A.wn;
+  ECO_GLOBAL const double *__restrict__ Wsrc = gmem(A.wsrc);
+  ECO_GLOBAL const double *__restrict__ Wn = gmem(A.wn);
   const int T0 = TX + (dir == 0), T1 = TY + (dir == 1);
   const int ndir = G.n[dir];
     const int fx = f % T0, fy = (f / T0) % T1, fz = f / (T0 * T1);
@@ -540,10 +555,11 @@ __device__ __forceinline__ double face_fp64(const StageArgs &A, const int *o, co
         ifl = (aL >= 1.e-4) && (aL <= 1. - 1.e-4) && ((aR - aL) * (aL - aLL) >= 1.e-8);
         ifr = (aR >= 1.e-4) && (aR <= 1. - 1.e-4) && ((aL - aR) * (aR - aRR) >= 1.e-8);
       }
-      const double dcc = G.dcc[dir][i];
-      const double dcm = has_m ? G.dcc[dir][i - 1] : 1.;
-      const double dcp = has_p ? G.dcc[dir][i + 1] : 1.;
-      const double dL = G.dfl[dir][i], dR = G.dfr[dir][i];
+      ECO_GLOBAL const double *Gc = gmem(G.dcc[dir]);
+      const double dcc = Gc[i];
+      const double dcm = has_m ? Gc[i - 1] : 1.;
+      const double dcp = has_p ? Gc[i + 1] : 1.;
+      const double dL = gmem(G.dfl[dir])[i], dR = gmem(G.dfr[dir])[i];
       double wl_[Ly::NVAR], wr_[Ly::NVAR], wll[Ly::NVAR], whh[Ly::NVAR];
       prim_to_vec<MODEL, NPH, DIM>(L, wl_);
       prim_to_vec<MODEL, NPH, DIM>(R, wr_);
@@ -552,7 +568,7 @@ __device__ __forceinline__ double face_fp64(const StageArgs &A, const int *o, co
       const long long jm = has_m ? ilo - sd : ilo, jp = has_p ? ihi + sd : ihi;
 #pragma unroll
       for (int v = 0; v < Ly::NVAR; v++) {
-        const double *Fv = Wsrc + (long long)v * nc;
+        ECO_GLOBAL const double *Fv = Wsrc + (long long)v * nc;
         wll[v] = Fv[jm];
         whh[v] = Fv[jp];
       }
@@ -598,7 +614,7 @@ __device__ __forceinline__ void stage_dir_3d(const StageArgs &A, const int *o, c
     if (active) {
       if (dir > 0) unpark_acc<MODEL, NPH, DIM, NT>(sAcc, tid, acc);
       CellState<NPH> own;
-      load_nc_coeffs<MODEL, NPH, DIM>(A.wn, A.geo.ncell, cidx, own);
+      load_nc_coeffs<MODEL, NPH, DIM>(gmem(A.wn), A.geo.ncell, cidx, own);
       accumulate_dir<MODEL, NPH, DIM, dir>(acc, sF, stride, 0, lx, ly, lz, ci, A.geo, own);
       if (dir < DIM - 1) park_acc<MODEL, NPH, DIM, NT>(sAcc, tid, acc);
     }
@@ -664,8 +680,9 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int slot, c
   const long long strd[3] = {1, (long long)nx, (long long)nx * ny};
   const long long cidx = active ? ci[0] + strd[1] * ci[1] + strd[2] * ci[2] : 0;
 
-  const double *__restrict__ Wsrc = A.wsrc;  // state the fluxes are built from (U^n or U*)
-  const double *__restrict__ Wn = A.wn;      // time-n state
+  ECO_GLOBAL const double *__restrict__ Wsrc = gmem(A.wsrc);  // state the fluxes are built from (U^n or U*)
+  ECO_GLOBAL const double *__restrict__ Wn = gmem(A.wn);      // time-n state
+  ECO_GLOBAL double *__restrict__ Wout = gmem(A.wout);
 
   FluxVec<NPH> acc;
   zero_flux(acc);
@@ -742,8 +759,8 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int slot, c
       bad = !(s1.p[0] == s1.p[0]);
     }
     if (bad) atomicOr(&ctl->err, 2u);
-    store_prim<MODEL, NPH, DIM>(A.wout, nc, cidx, s1);
-    if constexpr (STAGE != 1) A.wout[Ly::IE * nc + cidx] = s1.E;
+    store_prim<MODEL, NPH, DIM>(Wout, nc, cidx, s1);
+    if constexpr (STAGE != 1) Wout[Ly::IE * nc + cidx] = s1.E;
   }
 
   // CFL reduction: wave shuffle, LDS, one atomic per block
@@ -770,14 +787,19 @@ k_stage(const StageArgs A) {
 // count is read from device memory, so a captured launch stays valid while
 // the table changes between replays.
 template <int MODEL, int NPH, int DIM, int STAGE, bool SAMELIM>
-__global__ void __launch_bounds__(Tile<DIM>::T, Tile<DIM>::W)
-k_stage_batch(const BatchDesc *__restrict__ D, const int slot) {
+__global__ void __launch_bounds__(Tile<DIM>::T, Tile<DIM>::WB)
+k_stage_batch(const StageArgs *__restrict__ args, const TileRef *__restrict__ tiles,
+                                   const int *__restrict__ ntile, const int slot) {
   __shared__ StageLds<MODEL, NPH, DIM> lds;
-  const int ntiles = D->ntiles;
+  const int ntiles = __builtin_amdgcn_readfirstlane(*ntile);
   const int kidx = (STAGE == 2) ? 1 : 0;
   for (int w = blockIdx.x; w < ntiles; w += gridDim.x) {
-    const TileRef t = D->tiles[w];
-    const StageArgs &A = D->args[(size_t)t.c * 4 + (slot & 1) * 2 + kidx];
+    // uniform by construction; readfirstlane keeps the entry (and every
+    // address derived from it) in scalar registers
+    TileRef t = tiles[w];
+    t.c = __builtin_amdgcn_readfirstlane(t.c);
+    t.blk = __builtin_amdgcn_readfirstlane(t.blk);
+    const StageArgs &A = args[(size_t)t.c * 4 + (slot & 1) * 2 + kidx];
     stage_tile<MODEL, NPH, DIM, STAGE, SAMELIM>(A, slot, t.blk, lds);
     __syncthreads();  // LDS reuse by the next tile
   }
@@ -836,8 +858,8 @@ int launch_tb1(const BatchLaunch &L, hipStream_t st) {
   if (L.occupancy)
     return (int)hipOccupancyMaxActiveBlocksPerMultiprocessor(L.occupancy, k_stage_batch<MODEL, NPH, DIM, STAGE, SAME>,
                                                              NT, 0);
-  hipLaunchKernelGGL((k_stage_batch<MODEL, NPH, DIM, STAGE, SAME>), dim3((unsigned)L.grid), dim3(NT), 0, st, L.desc,
-                     L.slot);
+  hipLaunchKernelGGL((k_stage_batch<MODEL, NPH, DIM, STAGE, SAME>), dim3((unsigned)L.grid), dim3(NT), 0, st, L.args,
+                     L.tiles, L.ntiles, L.slot);
   return (int)hipGetLastError();
 }
 

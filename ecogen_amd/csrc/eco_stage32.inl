@@ -25,8 +25,8 @@ __device__ __forceinline__ void decode_alpha(float f0, float f1, double &a0, dou
   else { a1 = (double)f1; a0 = 1. - a1; }
 }
 
-template <int MODEL, int NPH, int DIM>
-__device__ __forceinline__ void load_prim_f(const float *__restrict__ W, long long nc, long long idx, CellState<NPH> &s) {
+template <int MODEL, int NPH, int DIM, typename PW>
+__device__ __forceinline__ void load_prim_f(PW W, long long nc, long long idx, CellState<NPH> &s) {
   static_assert(NPH == 2, "FP32 storage holds two-phase states");
   decode_alpha(W[idx], W[3 * nc + idx], s.a[0], s.a[1]);
 #pragma unroll
@@ -38,8 +38,8 @@ __device__ __forceinline__ void load_prim_f(const float *__restrict__ W, long lo
   for (int d = 0; d < 3; d++) s.u[d] = (d < DIM) ? (double)W[(3 * NPH + d) * nc + idx] : 0.;
 }
 
-template <int MODEL, int NPH, int DIM>
-__device__ __forceinline__ void store_prim_f(float *__restrict__ W, long long nc, long long idx, const CellState<NPH> &s) {
+template <int MODEL, int NPH, int DIM, typename PW>
+__device__ __forceinline__ void store_prim_f(PW W, long long nc, long long idx, const CellState<NPH> &s) {
 #pragma unroll
   for (int k = 0; k < NPH; k++) {
     W[(3 * k) * nc + idx] = (float)s.a[k];
@@ -193,9 +193,8 @@ __device__ __forceinline__ float speeds_f(const SchemeParams &S, const CellState
 }
 
 // time-n alpha_k, p_k of the owning cell (non-conservative terms)
-template <int NPH>
-__device__ __forceinline__ void load_nc_coeffs_f(const float *__restrict__ Wn, long long nc, long long cidx,
-                                                 CellState<NPH> &own) {
+template <int NPH, typename PW>
+__device__ __forceinline__ void load_nc_coeffs_f(PW Wn, long long nc, long long cidx, CellState<NPH> &own) {
   decode_alpha(Wn[cidx], Wn[3 * nc + cidx], own.a[0], own.a[1]);
 #pragma unroll
   for (int k = 0; k < NPH; k++) own.p[k] = (double)Wn[(3 * k + 2) * nc + cidx];
@@ -214,8 +213,8 @@ __device__ __forceinline__ float face_fp32s(const StageArgs &A, const int *o, co
   const SchemeParams &S = A.sch;
   const long long nc = G.ncell;
   const int nx = G.n[0], ny = G.n[1], nz = G.n[2];
-  const float *__restrict__ Wsrc = (const float *)A.wsrc;
-  const float *__restrict__ Wn = (const float *)A.wn;
+  ECO_GLOBAL const float *__restrict__ Wsrc = (ECO_GLOBAL const float *)gmem(A.wsrc);
+  ECO_GLOBAL const float *__restrict__ Wn = (ECO_GLOBAL const float *)gmem(A.wn);
   const int T0 = TX + (dir == 0), T1 = TY + (dir == 1);
   const int ndir = G.n[dir];
   const int fx = f % T0, fy = (f / T0) % T1, fz = f / (T0 * T1);
@@ -234,7 +233,7 @@ __device__ __forceinline__ float face_fp32s(const StageArgs &A, const int *o, co
   float wl_[Ly::NVAR], wr_[Ly::NVAR], wll[Ly::NVAR], whh[Ly::NVAR];
 #pragma unroll
   for (int v = 0; v < Ly::NVAR; v++) {
-    const float *Fv = Wsrc + (long long)v * nc;
+    ECO_GLOBAL const float *Fv = Wsrc + (long long)v * nc;
     wl_[v] = Fv[ilo];
     wr_[v] = Fv[ihi];
     wll[v] = Fv[jm];
@@ -258,10 +257,11 @@ __device__ __forceinline__ float face_fp32s(const StageArgs &A, const int *o, co
     ifl = (aL >= 1.e-4) && (aL <= 1. - 1.e-4) && ((aR - aL) * (aL - aLL) >= 1.e-8);
     ifr = (aR >= 1.e-4) && (aR <= 1. - 1.e-4) && ((aL - aR) * (aR - aRR) >= 1.e-8);
   }
-  const double dcc = G.dcc[dir][i];
-  const double dcm = has_m ? G.dcc[dir][i - 1] : 1.;
-  const double dcp = has_p ? G.dcc[dir][i + 1] : 1.;
-  const double dL = G.dfl[dir][i], dR = G.dfr[dir][i];
+  ECO_GLOBAL const double *Gc = gmem(G.dcc[dir]);
+  const double dcc = Gc[i];
+  const double dcm = has_m ? Gc[i - 1] : 1.;
+  const double dcp = has_p ? Gc[i + 1] : 1.;
+  const double dL = gmem(G.dfl[dir])[i], dR = gmem(G.dfr[dir])[i];
   double xl[Ly::NVAR], xr[Ly::NVAR];
   muscl_dispatch<MODEL, NPH, DIM, SAMELIM, true>(S, wlv, wrv, wlm, whp, has_m, has_p, dcc, dcm, dcp, dL, dR, ifl, ifr,
                                                  xl, xr);
@@ -302,7 +302,7 @@ __device__ __forceinline__ void stage32_dir_3d(const StageArgs &A, const int *o,
     if (active) {
       if (dir > 0) unpark_acc<MODEL, NPH, DIM, NT>(sAcc, tid, acc);
       CellState<NPH> own;
-      load_nc_coeffs_f<NPH>((const float *)A.wn, A.geo.ncell, cidx, own);
+      load_nc_coeffs_f<NPH>((ECO_GLOBAL const float *)gmem(A.wn), A.geo.ncell, cidx, own);
       accumulate_dir<MODEL, NPH, DIM, dir>(acc, sF, stride, 0, lx, ly, lz, ci, A.geo, own);
       if (dir < DIM - 1) park_acc<MODEL, NPH, DIM, NT>(sAcc, tid, acc);
     }
@@ -351,8 +351,8 @@ __device__ __forceinline__ void stage32_tile(const StageArgs &A, const int slot,
   const long long strd[3] = {1, (long long)nx, (long long)nx * ny};
   const long long cidx = active ? ci[0] + strd[1] * ci[1] + strd[2] * ci[2] : 0;
 
-  const float *__restrict__ Wsrc = (const float *)A.wsrc;
-  const float *__restrict__ Wn = (const float *)A.wn;
+  ECO_GLOBAL const float *__restrict__ Wsrc = (ECO_GLOBAL const float *)gmem(A.wsrc);
+  ECO_GLOBAL const float *__restrict__ Wn = (ECO_GLOBAL const float *)gmem(A.wn);
 
   FluxVec<NPH> acc;
   zero_flux(acc);
@@ -423,7 +423,7 @@ __device__ __forceinline__ void stage32_tile(const StageArgs &A, const int slot,
 #pragma unroll
     for (int k = 0; k < NPH; k++) bad |= !(s1.p[k] == s1.p[k]) || !(s1.r[k] == s1.r[k]);
     if (bad) atomicOr(&ctl->err, 2u);
-    float *W = (float *)A.wout;
+    ECO_GLOBAL float *W = (ECO_GLOBAL float *)gmem(A.wout);
     store_prim_f<MODEL, NPH, DIM>(W, nc, cidx, s1);
     if constexpr (STAGE == 2) {
       CellState<NPH> st = s1;
@@ -453,14 +453,19 @@ k_stage32(const StageArgs A) {
 
 // persistent form over a pool (as k_stage_batch)
 template <int MODEL, int NPH, int DIM, int STAGE, bool SAMELIM>
-__global__ void __launch_bounds__(Tile<DIM>::T, Tile<DIM>::W)
-k_stage32_batch(const BatchDesc *__restrict__ D, const int slot) {
+__global__ void __launch_bounds__(Tile<DIM>::T, Tile<DIM>::WB)
+k_stage32_batch(const StageArgs *__restrict__ args, const TileRef *__restrict__ tiles,
+                                   const int *__restrict__ ntile, const int slot) {
   __shared__ StageLds<MODEL, NPH, DIM> lds;
-  const int ntiles = D->ntiles;
+  const int ntiles = __builtin_amdgcn_readfirstlane(*ntile);
   const int kidx = (STAGE == 2) ? 1 : 0;
   for (int w = blockIdx.x; w < ntiles; w += gridDim.x) {
-    const TileRef t = D->tiles[w];
-    const StageArgs &A = D->args[(size_t)t.c * 4 + (slot & 1) * 2 + kidx];
+    // uniform by construction; readfirstlane keeps the entry (and every
+    // address derived from it) in scalar registers
+    TileRef t = tiles[w];
+    t.c = __builtin_amdgcn_readfirstlane(t.c);
+    t.blk = __builtin_amdgcn_readfirstlane(t.blk);
+    const StageArgs &A = args[(size_t)t.c * 4 + (slot & 1) * 2 + kidx];
     stage32_tile<MODEL, NPH, DIM, STAGE, SAMELIM>(A, slot, t.blk, lds);
     __syncthreads();  // LDS reuse by the next tile
   }
@@ -545,8 +550,8 @@ int launch_tb32_1(const BatchLaunch &L, hipStream_t st) {
   if (L.occupancy)
     return (int)hipOccupancyMaxActiveBlocksPerMultiprocessor(L.occupancy,
                                                              k_stage32_batch<MODEL, NPH, DIM, STAGE, SAME>, NT, 0);
-  hipLaunchKernelGGL((k_stage32_batch<MODEL, NPH, DIM, STAGE, SAME>), dim3((unsigned)L.grid), dim3(NT), 0, st, L.desc,
-                     L.slot);
+  hipLaunchKernelGGL((k_stage32_batch<MODEL, NPH, DIM, STAGE, SAME>), dim3((unsigned)L.grid), dim3(NT), 0, st, L.args,
+                     L.tiles, L.ntiles, L.slot);
   return (int)hipGetLastError();
 }
 

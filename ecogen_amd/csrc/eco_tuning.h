@@ -53,3 +53,14 @@
 #ifndef ECO_THREADS3
 #define ECO_THREADS3 256
 #endif
+// minimum waves per SIMD requested for the persistent pool kernels (0 = as
+// the single-solve kernels)
+#ifndef ECO_WAVESB1
+#define ECO_WAVESB1 0
+#endif
+#ifndef ECO_WAVESB2
+#define ECO_WAVESB2 0
+#endif
+#ifndef ECO_WAVESB3
+#define ECO_WAVESB3 0
+#endif
