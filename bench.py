@@ -9,8 +9,9 @@ state, built on the host and uploaded before timing; the timed region runs
 resident in HBM.
 
 Prints one JSON line: cell-updates/s, the HBM roofline of the dominant kernel
-measured with HIP events over the timed region, a CPU baseline (the FP64
-oracle, OpenMP on the host cores) on a bounded sample of the same workload,
+measured with HIP events over the timed region, a CPU baseline (the
+reference executable built from source, MPI ranks on the host cores, or the
+FP64 oracle restatement) on a bounded sample of the same workload,
 and the secondary measurements: FP32 storage on the same workload, the
 batched pool (2048 water/air Riemann problems, FP32 storage) against serial
 plain FP64 solves, the served request stream at three arrival rates, the
@@ -43,8 +44,12 @@ def parse():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=4096, help="cells per side")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-n", type=int, default=2048, help="cells per side of the CPU sample")
-    ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--cpu-kind", choices=["auto", "reference", "port"], default="auto",
+                    help="CPU baseline: the reference executable (auto: when built) or the oracle restatement")
+    ap.add_argument("--cpu-n", type=int, default=2048, help="cells per side of the oracle (port) sample")
+    ap.add_argument("--cpu-steps", type=int, default=20, help="steps of the oracle (port) sample")
+    ap.add_argument("--ref-n", type=int, default=1024, help="cells per side of the reference sample")
+    ap.add_argument("--ref-steps", type=int, default=10, help="measured steps of the reference sample")
     ap.add_argument("--no-fp32s", action="store_true", help="skip the FP32-storage measurement")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain copy kernel of known size (for HBM counter calibration)")
@@ -86,20 +91,74 @@ def copy_calibration(nbytes=1 << 30):
     return nbytes
 
 
+REF_BIN = ROOT / "oracle" / "_ref" / "ECOGEN"   # the reference, built from its sources by oracle/ref.mk
+MPIEXEC = Path("/opt/conda/bin/mpiexec")         # MPICH of the image, which the reference links
+
+
+def _ref_deck_case(n, steps):
+    """Deck of the headline workload at n x n cells (cylinder radii scaled)."""
+    from ecogen_amd import workloads as W
+    sys.path.insert(0, str(ROOT / "oracle"))
+    from deck import OUTPUT_BINARY
+    scale = n / 4096.0
+    air, shocked, water = W.shock_cylinders_2d_states()
+    doms = [{"type": "entireDomain", "state": "air"},
+            {"type": "halfSpace", "state": "shocked", "axe": "x", "origin": 0.05, "direction": "negative"}]
+    doms += [{"type": "disc", "state": "water", "radius": r, "center": c}
+             for c, r in W.cylinder_bodies(n, 32, 0xEC0C0002, 20.0 * scale, 80.0 * scale)]
+    return {"name": "bench", "model": "kapila", "alpha_null": False, "eos": [W.AIR, W.WATER],
+            "cells": [n, n, 1], "lengths": [1.0, 1.0, 1.0], "order": 2, "limiters": ["vanleer"] * 4,
+            "cfl": 0.8, "time_mode": "iterations", "steps": steps, "output": OUTPUT_BINARY,
+            "domains": doms, "states": [dict(air, u=air["u"][:2]), dict(shocked, u=shocked["u"][:2]),
+                                        dict(water, u=water["u"][:2])]}
+
+
+def _run_reference(n, steps, ranks):
+    import shutil
+    import subprocess
+    import tempfile
+    sys.path.insert(0, str(ROOT / "oracle"))
+    from deck import write_deck
+    tmp = Path(tempfile.mkdtemp(prefix="eco_ref_"))
+    try:
+        write_deck(_ref_deck_case(n, steps), tmp)
+        t0 = time.perf_counter()
+        subprocess.run([str(MPIEXEC), "-n", str(ranks), str(REF_BIN)], cwd=tmp, capture_output=True, text=True,
+                       timeout=900, check=True)
+        return time.perf_counter() - t0
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
 def cpu_baseline(args):
+    """The reference's own CPU path (its executable built from source, MPI ranks
+    on the host cores) when it is present, else the FP64 oracle restatement
+    (OpenMP).  The reference is timed at two step counts so that start-up,
+    mesh construction and output cancel: rate = cells x (s2 - s1) / (T2 - T1)."""
+    n = args.cpu_n
+    cores = min(16, os.cpu_count() or 1)
+    have_ref = REF_BIN.exists() and MPIEXEC.exists()
+    if args.cpu_kind == "reference" or (args.cpu_kind == "auto" and have_ref):
+        n_ref = args.ref_n
+        s1, s2 = 2, 2 + args.ref_steps
+        t1 = _run_reference(n_ref, s1, cores)
+        t2 = _run_reference(n_ref, s2, cores)
+        return {"value": n_ref * n_ref * (s2 - s1) / (t2 - t1), "unit": "cell-updates/s", "cores": cores,
+                "kind": "reference",
+                "sample": f"headline workload at {n_ref}x{n_ref} cells (cylinder radii scaled), reference ECOGEN "
+                          f"built from source, mpiexec -n {cores} (MPICH 3.3.2); {s2} minus {s1} steps, wall "
+                          f"{t2:.1f} s - {t1:.1f} s"}
     sys.path.insert(0, str(ROOT / "oracle"))
     import oracle as O
     from ecogen_amd import workloads as W
-    n = args.cpu_n
     scale = n / 4096.0
     case, st = W.shock_cylinders_2d(n=n, rmin=20.0 * scale, rmax=80.0 * scale, steps=args.cpu_steps)
-    threads = min(16, os.cpu_count() or 1)
     t0 = time.perf_counter()
-    _, it, _, _, _ = O.run(case, st, nsteps=args.cpu_steps, nthreads=threads)
+    _, it, _, _, _ = O.run(case, st, nsteps=args.cpu_steps, nthreads=cores)
     dt = time.perf_counter() - t0
-    return {"value": n * n * it / dt, "unit": "cell-updates/s", "cores": threads, "kind": "port",
+    return {"value": n * n * it / dt, "unit": "cell-updates/s", "cores": cores, "kind": "port",
             "sample": f"same workload scaled to {n}x{n} cells (cylinder radii scaled), {it} steps, FP64 oracle "
-                      f"restatement of the reference step, OpenMP {threads} threads, {dt:.1f} s"}
+                      f"restatement of the reference step, OpenMP {cores} threads, {dt:.1f} s"}
 
 
 def traffic_from_profiles(n, precision, kernel):

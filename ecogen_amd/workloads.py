@@ -87,6 +87,24 @@ def _fill(state, mask_idx, st, nph=2):
         state[3 * nph + d][mask_idx] = u
 
 
+def cylinder_bodies(n=4096, ncyl=32, seed=0xEC0C0002, rmin=20.0, rmax=80.0):
+    """Water cylinders of shock_cylinders_2d: ((cx, cy), r), non-overlapping,
+    centres in x in [0.1, 0.95], radii uniform in [rmin, rmax] cells."""
+    return place(seed, ncyl, rmin, rmax, [(0.1, 0.95), (0.0, 1.0)], 1.0 / n)
+
+
+def shock_cylinders_2d_states():
+    """(air, shocked air, water) states of shock_cylinders_2d."""
+    p2, rho2, u2 = mach_shock_air(2.4)
+    air = {"name": "air", "alpha": [1 - ALPHA_FLOOR, ALPHA_FLOOR], "rho": [1.2, 1000.0], "p": [1e5, 1e5],
+           "u": [0.0, 0.0, 0.0]}
+    shocked = {"name": "shocked", "alpha": [1 - ALPHA_FLOOR, ALPHA_FLOOR], "rho": [rho2, 1000.0], "p": [p2, p2],
+               "u": [u2, 0.0, 0.0]}
+    water = {"name": "water", "alpha": [ALPHA_FLOOR, 1 - ALPHA_FLOOR], "rho": [1.2, 1000.0], "p": [1e5, 1e5],
+             "u": [0.0, 0.0, 0.0]}
+    return air, shocked, water
+
+
 def shock_cylinders_2d(n=4096, ncyl=32, seed=0xEC0C0002, rmin=20.0, rmax=80.0, limiter="vanleer", steps=1000):
     """Config C2: a planar Mach 2.4 air shock (x < 0.05) hits ``ncyl`` water cylinders."""
     case = {
@@ -98,18 +116,14 @@ def shock_cylinders_2d(n=4096, ncyl=32, seed=0xEC0C0002, rmin=20.0, rmax=80.0, l
     xc, yc, _ = _centres(case["cells"], L)
     ncell = n * n
     st = np.empty((9, ncell))
-    p2, rho2, u2 = mach_shock_air(2.4)
-    air = {"alpha": [1 - ALPHA_FLOOR, ALPHA_FLOOR], "rho": [1.2, 1000.0], "p": [1e5, 1e5], "u": [0.0, 0.0, 0.0]}
-    shocked = {"alpha": [1 - ALPHA_FLOOR, ALPHA_FLOOR], "rho": [rho2, 1000.0], "p": [p2, p2], "u": [u2, 0.0, 0.0]}
-    water = {"alpha": [ALPHA_FLOOR, 1 - ALPHA_FLOOR], "rho": [1.2, 1000.0], "p": [1e5, 1e5], "u": [0.0, 0.0, 0.0]}
+    air, shocked, water = shock_cylinders_2d_states()
     _fill(st, slice(None), air)
     view = st.reshape(9, n, n)  # [field, j, i]
     sx = xc <= 0.05
     for f, v in enumerate(_vals(shocked)):
         view[f][:, sx] = v
     dx = L[0] / n
-    bodies = place(seed, ncyl, rmin, rmax, [(0.1, 0.95), (0.0, 1.0)], dx)
-    for (cx, cy), r in bodies:
+    for (cx, cy), r in cylinder_bodies(n, ncyl, seed, rmin, rmax):
         i0, i1 = max(0, int((cx - r) / dx) - 1), min(n, int((cx + r) / dx) + 2)
         j0, j1 = max(0, int((cy - r) / dx) - 1), min(n, int((cy + r) / dx) + 2)
         X = xc[i0:i1][None, :]

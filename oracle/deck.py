@@ -1,0 +1,133 @@
+"""Input decks of the reference ECOGEN executable -- TEST INFRASTRUCTURE ONLY.
+
+Writes a run directory (ECOGEN.xml, libEOS/, case/*.xml) in the reference's
+input schema (libTests/manual*.xml) for a case description with geometric
+domains and states.  Used by tests/golden/make_golden.py (golden vectors)
+and by the CPU-baseline leg of bench.py (the reference timed on the host).
+EOS files carry the parameters of libEOS/SG_water.xml and libEOS/IG_air.xml.
+"""
+from __future__ import annotations
+
+from pathlib import Path
+
+EOS_XML = {
+    "IG": '<?xml version="1.0" encoding="UTF-8" standalone="yes"?>\n<parametersEOS>\n'
+          '  <EOS type="IG"/>\n  <parameters gamma="{gamma!r}" cv="{cv!r}" energyRef="{eref!r}" entropyRef="{sref!r}"/>\n'
+          '</parametersEOS>\n',
+    "SG": '<?xml version="1.0" encoding="UTF-8" standalone="yes"?>\n<parametersEOS>\n'
+          '  <EOS type="SG"/>\n  <parameters gamma="{gamma!r}" pInf="{pinf!r}" cv="{cv!r}" energyRef="{eref!r}" entropyRef="{sref!r}"/>\n'
+          '</parametersEOS>\n',
+}
+
+
+# full-precision ASCII output (golden vectors); binary output is Float32
+OUTPUT_ASCII17 = 'format="XML" binary="false" precision="17"'
+OUTPUT_BINARY = 'format="XML" binary="true"'
+
+
+def _eos_file(e):
+    return f"{e['type']}_{e['name']}.xml"
+
+
+def _state_xml(case, st):
+    model = case["model"]
+    out = [f'  <state name="{st["name"]}">']
+    eos = case["eos"]
+    if model == "euler":
+        e = eos[0]
+        out.append(f'    <material type="fluide" EOS="{_eos_file(e)}">')
+        out.append(f'      <dataFluid density="{st["rho"][0]!r}" pressure="{st["p"][0]!r}">')
+        u = list(st["u"]) + [0.0] * (3 - len(st["u"]))
+        out.append(f'        <velocity x="{u[0]!r}" y="{u[1]!r}" z="{u[2]!r}"/>')
+        out.append('      </dataFluid>')
+        out.append('    </material>')
+    else:
+        for k, e in enumerate(eos):
+            out.append(f'    <material type="fluide" EOS="{_eos_file(e)}">')
+            if model == "kapila":
+                out.append(f'      <dataFluid alpha="{st["alpha"][k]!r}" density="{st["rho"][k]!r}"/>')
+            else:
+                out.append(f'      <dataFluid alpha="{st["alpha"][k]!r}" density="{st["rho"][k]!r}" pressure="{st["p"][k]!r}"/>')
+            out.append('    </material>')
+        u = list(st["u"]) + [0.0] * (3 - len(st["u"]))
+        out.append('    <mixture>')
+        if model == "kapila":
+            out.append(f'      <dataMix pressure="{st["p"][0]!r}"/>')
+        out.append(f'      <velocity x="{u[0]!r}" y="{u[1]!r}" z="{u[2]!r}"/>')
+        out.append('    </mixture>')
+    out.append('  </state>')
+    return "\n".join(out)
+
+
+def _domain_xml(i, d):
+    t = d["type"]
+    head = f'    <domain name="d{i}" state="{d["state"]}" type="{t}"'
+    if t == "entireDomain":
+        return head + "/>"
+    if t == "halfSpace":
+        return (head + ">\n" + f'      <dataHalfSpace axe="{d["axe"]}" origin="{d["origin"]!r}" direction="{d["direction"]}"/>\n'
+                "    </domain>")
+    if t == "disc":
+        c = d["center"]
+        return (head + ">\n" + f'      <dataDisc axe1="x" axe2="y" radius="{d["radius"]!r}">\n'
+                f'        <center x="{c[0]!r}" y="{c[1]!r}" z="0."/>\n      </dataDisc>\n    </domain>')
+    if t == "sphere":
+        c = d["center"]
+        return (head + ">\n" + f'      <dataSphere radius="{d["radius"]!r}">\n'
+                f'        <center x="{c[0]!r}" y="{c[1]!r}" z="{c[2]!r}"/>\n      </dataSphere>\n    </domain>')
+    raise ValueError(t)
+
+
+def write_deck(case, root: Path):
+    run = case["name"]
+    cdir = root / "case"
+    cdir.mkdir(parents=True)
+    (root / "libEOS").mkdir()
+    for e in case["eos"]:
+        (root / "libEOS" / _eos_file(e)).write_text(EOS_XML[e["type"]].format(**{"sref": 0.0, "pinf": 0.0, **e}))
+    (root / "ECOGEN.xml").write_text(
+        '<?xml version="1.0" encoding="UTF-8" standalone="yes"?>\n<ecogen>\n'
+        '  <testCase>./case/</testCase>\n</ecogen>\n')
+    if case["time_mode"] == "iterations":
+        tc = (f'  <timeControlMode iterations="true">\n    <iterations number="{case["steps"]}" iterFreq="{case["steps"]}"/>\n'
+              f'    <physicalTime totalTime="1." timeFreq="1."/>\n  </timeControlMode>')
+    else:
+        tf = case["t_final"]
+        tc = (f'  <timeControlMode iterations="false">\n    <iterations number="1" iterFreq="1"/>\n'
+              f'    <physicalTime totalTime="{tf!r}" timeFreq="{tf!r}"/>\n  </timeControlMode>')
+    so = ""
+    if case["order"] == 2:
+        lim = case["limiters"]
+        so = ("  <secondOrder>\n"
+              f"    <globalLimiter>{lim[0]}</globalLimiter>\n"
+              f"    <interfaceLimiter>{lim[1]}</interfaceLimiter>\n"
+              f"    <globalVolumeFractionLimiter>{lim[2]}</globalVolumeFractionLimiter>\n"
+              f"    <interfaceVolumeFractionLimiter>{lim[3]}</interfaceVolumeFractionLimiter>\n"
+              "  </secondOrder>\n")
+    (cdir / "mainV5.xml").write_text(
+        '<?xml version="1.0" encoding="UTF-8" standalone="yes"?>\n<computationParam>\n'
+        f'  <run>{run}</run>\n  <outputMode {case.get("output", OUTPUT_ASCII17)}/>\n'
+        f'{tc}\n  <computationControl CFL="{case["cfl"]!r}"/>\n{so}</computationParam>\n')
+    L = case["lengths"]
+    n = case["cells"]
+    (cdir / "meshV5.xml").write_text(
+        '<?xml version="1.0" encoding="UTF-8" standalone="yes"?>\n<mesh>\n  <type structure="cartesian"/>\n'
+        f'  <cartesianMesh>\n    <dimensions x="{L[0]!r}" y="{L[1]!r}" z="{L[2]!r}"/>\n'
+        f'    <numberCells x="{n[0]}" y="{n[1]}" z="{n[2]}"/>\n  </cartesianMesh>\n</mesh>\n')
+    model = case["model"]
+    fm = {"euler": '<flowModel name="Euler"/>',
+          "kapila": f'<flowModel name="Kapila" numberPhases="{len(case["eos"])}" alphaNull="{"true" if case["alpha_null"] else "false"}"/>',
+          "multip": f'<flowModel name="MultiP" numberPhases="{len(case["eos"])}"/>'}[model]
+    eosl = "\n".join(f'  <EOS name="{_eos_file(e)}"/>' for e in case["eos"])
+    relax = '  <relaxation type="P"/>\n' if case.get("relax_p") and model == "multip" else ""
+    (cdir / "modelV4.xml").write_text(
+        '<?xml version="1.0" encoding="UTF-8" standalone="yes"?>\n<model>\n'
+        f'  {fm}\n{eosl}\n{relax}</model>\n')
+    doms = "\n".join(_domain_xml(i, d) for i, d in enumerate(case["domains"]))
+    names = ["CLXm", "CLXp", "CLYm", "CLYp", "CLZm", "CLZp"]
+    dim = sum(1 for c in n if c != 1)
+    bcs = "\n".join(f'    <boundCond name="{names[i]}" type="abs" number="{i + 1}"/>' for i in range(2 * dim))
+    states = "\n".join(_state_xml(case, st) for st in case["states"])
+    (cdir / "initialConditionsV4.xml").write_text(
+        '<?xml version="1.0" encoding="UTF-8" standalone="yes"?>\n<CI>\n  <physicalDomains>\n'
+        f'{doms}\n  </physicalDomains>\n  <boundaryConditions>\n{bcs}\n  </boundaryConditions>\n{states}\n</CI>\n')
