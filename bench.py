@@ -61,6 +61,11 @@ def parse():
     ap.add_argument("--c3-n", type=int, default=384)
     ap.add_argument("--c3-steps", type=int, default=20, help="steps of the 3-D leg (0: skip)")
     ap.add_argument("--only-main", action="store_true", help="headline workload only")
+    ap.add_argument("--legs", nargs="+", choices=["latency", "batched", "served", "c3"],
+                    default=["latency", "batched", "served", "c3"], help="secondary measurements to run")
+    ap.add_argument("--batch-direct", action="store_true",
+                    help="batched leg with direct launches instead of hipGraph replay (for kernel tracing)")
+    ap.add_argument("--no-serial", action="store_true", help="batched leg without the serial FP64 comparator")
     return ap.parse_args()
 
 
@@ -252,9 +257,9 @@ def leg_batched(args):
     specs = W.batch_specs(args.batch_cases)
     cases = [W.riemann_case(*sp, precision="fp32s") for sp in specs]
     cells = np.array([int(np.prod(c["cells"])) for c, _, _ in cases])
-    sample = list(range(0, len(cases), max(1, args.serial_stride)))
+    sample = [] if args.no_serial else list(range(0, len(cases), max(1, args.serial_stride)))
     finish = {}
-    with Pool(hbm_budget=48 << 30, steps_per_chunk=24) as pool:
+    with Pool(hbm_budget=48 << 30, steps_per_chunk=24, use_graph=not args.batch_direct) as pool:
         t0 = time.perf_counter()
         ids = [pool.admit(c, st, t_final=tf) for c, st, tf in cases]
         if any(i is None for i in ids):
@@ -289,8 +294,8 @@ def leg_batched(args):
             l1s.append(_fp32s_vs_fp64(c, res[k][0], a64))
         else:
             l1_skipped += 1
-    ser_solves = len(sample) / ser_t
-    ser_cus = ser_cu / ser_t
+    ser_solves = len(sample) / ser_t if sample else float("nan")
+    ser_cus = ser_cu / ser_t if sample else float("nan")
     l1 = None
     if l1s:
         l1 = {"median": {q: float(f"{np.median([e[q] for e in l1s]):.3e}") for q in l1s[0]},
@@ -304,9 +309,10 @@ def leg_batched(args):
                  "tiles_launched": stats.tiles_launched},
         "relax_not_converged_cases": int((flags & 1).astype(bool).sum()),
         "serial_fp64": {"sample": f"every {args.serial_stride}th case ({len(sample)} cases), one solve at a time",
-                        "solves_per_s": round(ser_solves, 2), "cell_updates_per_s": round(ser_cus, 1)},
-        "speedup_cell_updates_vs_serial_fp64": round((cu / wall) / ser_cus, 2),
-        "speedup_solves_vs_serial_fp64": round((len(cases) / wall) / ser_solves, 2),
+                        "solves_per_s": round(ser_solves, 2) if sample else None,
+                        "cell_updates_per_s": round(ser_cus, 1) if sample else None},
+        "speedup_cell_updates_vs_serial_fp64": round((cu / wall) / ser_cus, 2) if sample else None,
+        "speedup_solves_vs_serial_fp64": round((len(cases) / wall) / ser_solves, 2) if sample else None,
         "fp32s_l1_vs_fp64": l1,
         "fp32s_l1_cases_compared": len(l1s), "fp32s_l1_cases_skipped": l1_skipped,
         "storage": "FP32 state (alpha_k, rho_k, p_k, u, reduced energy); FP64 fluxes, update, relaxation",
@@ -440,17 +446,19 @@ def main():
     if args.calibrate_copy:
         copy_calibration()
     extra = {}
-    if not args.only_main:
-        log("1-D shock-tube latency")
-        extra["latency_1d"] = leg_latency_1d()
-        if args.batch_cases > 0:
+    legs = set() if args.only_main else set(args.legs)
+    if legs:
+        if "latency" in legs:
+            log("1-D shock-tube latency")
+            extra["latency_1d"] = leg_latency_1d()
+        if "batched" in legs and args.batch_cases > 0:
             log(f"batched pool: {args.batch_cases} cases")
             extra["batched"] = leg_batched(args)
             log(f"batched: {extra['batched']['wall_s']} s")
-        if args.serve_requests > 0:
+        if "served" in legs and args.serve_requests > 0:
             log(f"served stream: {args.serve_requests} requests x {len(args.serve_rates)} rates")
             extra["served"] = leg_served(args)
-        if args.c3_steps > 0:
+        if "c3" in legs and args.c3_steps > 0:
             log(f"3-D bubbles {args.c3_n}^3, {args.c3_steps} steps")
             extra["c3"] = leg_c3(args)
     if not args.no_cpu:
