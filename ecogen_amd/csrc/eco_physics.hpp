@@ -66,32 +66,39 @@ ECO_D double eos_fix_p(const EosParams &q, double p) {
 // Branch-free forms (every operand computed, the result selected), so that
 // the limited slopes of all variables of a face schedule as one basic block.
 // The selected value is the reference's bit for bit.
+template <int LIM, typename T>
+ECO_D T limiter_kt(T s1, T s2) {
+  auto mn = [](T a, T b) { return (b < a) ? b : a; };
+  auto mx = [](T a, T b) { return (a < b) ? b : a; };
+  const T prod = s1 * s2;
+  if constexpr (LIM == LIM_MINMOD || LIM == LIM_THINC) {
+    const T m = mn(fabs(s1), fabs(s2));
+    return (prod > T(1e-9)) ? ((s1 < 0) ? -m : m) : T(0);
+  } else if constexpr (LIM == LIM_VANLEER) {
+    const T sum = s1 + s2;
+    const T q = T(2) * prod / sum;
+    return ((fabs(sum) > T(1e-6)) && (prod > T(1e-6))) ? q : T(0);
+  } else if constexpr (LIM == LIM_VANALBADA) {
+    const T sum = s1 + s2;
+    const T q = prod * sum / (s1 * s1 + s2 * s2);
+    return ((fabs(s1) > T(1e-6)) && (fabs(s2) > T(1e-6)) && (fabs(sum) > T(1e-6)) && (prod > T(1e-6))) ? q : T(0);
+  } else if constexpr (LIM == LIM_MC) {
+    T m = mn(T(2) * fabs(s1), T(2) * fabs(s2));
+    m = mn(m, T(0.5) * (fabs(s1) + fabs(s2)));
+    const T slope = (prod > T(1e-9)) ? m : T(0);
+    return (s1 < 0) ? -slope : slope;
+  } else if constexpr (LIM == LIM_SUPERBEE) {
+    const T m = mx(mn(T(2) * fabs(s1), fabs(s2)), mn(fabs(s1), T(2) * fabs(s2)));
+    const T slope = (prod > T(1e-6)) ? m : T(0);
+    return (s1 < 0) ? -slope : slope;
+  } else {
+    return T(0);
+  }
+}
+
 template <int LIM>
 ECO_D double limiter_k(double s1, double s2) {
-  const double prod = s1 * s2;
-  if constexpr (LIM == LIM_MINMOD || LIM == LIM_THINC) {
-    const double m = smin(fabs(s1), fabs(s2));
-    return (prod > 1e-9) ? ((s1 < 0) ? -m : m) : 0.;
-  } else if constexpr (LIM == LIM_VANLEER) {
-    const double sum = s1 + s2;
-    const double q = 2. * prod / sum;
-    return ((fabs(sum) > 1e-6) && (prod > 1e-6)) ? q : 0.;
-  } else if constexpr (LIM == LIM_VANALBADA) {
-    const double sum = s1 + s2;
-    const double q = prod * sum / (s1 * s1 + s2 * s2);
-    return ((fabs(s1) > 1e-6) && (fabs(s2) > 1e-6) && (fabs(sum) > 1e-6) && (prod > 1e-6)) ? q : 0.;
-  } else if constexpr (LIM == LIM_MC) {
-    double m = smin(2. * fabs(s1), 2. * fabs(s2));
-    m = smin(m, 0.5 * (fabs(s1) + fabs(s2)));
-    const double slope = (prod > 1e-9) ? m : 0.;
-    return (s1 < 0.) ? -slope : slope;
-  } else if constexpr (LIM == LIM_SUPERBEE) {
-    const double m = smax(smin(2. * fabs(s1), fabs(s2)), smin(fabs(s1), 2. * fabs(s2)));
-    const double slope = (prod > 1e-6) ? m : 0.;
-    return (s1 < 0.) ? -slope : slope;
-  } else {
-    return 0.;
-  }
+  return limiter_kt<LIM, double>(s1, s2);
 }
 
 // runtime limiter choice (LimiterMinmod/VanLeer/VanAlbada/Mc/Superbee

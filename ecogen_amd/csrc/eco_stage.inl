@@ -417,44 +417,62 @@ __device__ __forceinline__ void accumulate_dir(FluxVec<NPH> &acc, const double *
 // i-1/2, i+1/2, i+3/2, the low cell limited from (i+1/2, i-1/2), the high
 // cell from (i+1/2, i+3/2).  LV/LA: compile-time limiter of the non-alpha /
 // alpha variables, or -1 for the runtime choice from S.lim and the interface
-// flags.  RECIP: slopes through reciprocals of the cell distances (the
-// FP32-storage form); otherwise the reference's correctly rounded quotients,
-// formed through one reciprocal per distance (qdiv).
+// flags.  RECIP: the FP32-storage form (slopes and limiters in FP32);
+// otherwise the reference's arithmetic in FP64.
 template <int MODEL, int NPH, int DIM, int LV, int LA, bool RECIP, typename TW>
 __device__ __forceinline__ void muscl_vars(const SchemeParams &S, const TW *wl_, const TW *wr_, const TW *wll,
                                            const TW *whh, bool has_m, bool has_p, double dcc, double dcm, double dcp,
                                            double dL, double dR, bool ifl, bool ifr, double *xl, double *xr) {
   using Ly = Layout<MODEL, NPH, DIM>;
-  double idc = 0., idm = 0., idp = 0., ydc = 0., ydm = 0., ydp = 0.;
-  if constexpr (RECIP) { idc = 1. / dcc; idm = 1. / dcm; idp = 1. / dcp; }
-  else { ydc = 1. / dcc; ydm = 1. / dcm; ydp = 1. / dcp; }
+  if constexpr (RECIP) {
+    // FP32-storage form: differences of the (decoded) cell values in FP64,
+    // slopes, limiters and the extrapolation increment in FP32, added to the
+    // FP64 cell value
+    const float idc = 1.f / (float)dcc, idm = 1.f / (float)dcm, idp = 1.f / (float)dcp;
+    const float fL = (float)dL, fR = (float)dR;
 #pragma unroll
-  for (int v = 0; v < Ly::NVAR; v++) {
-    const double wl = wl_[v], wr = wr_[v];
-    double sc, sm, sp;
-    if constexpr (RECIP) {
-      sc = (wr - wl) * idc;
-      sm = has_m ? (wl - (double)wll[v]) * idm : 0.;
-      sp = has_p ? ((double)whh[v] - wr) * idp : 0.;
-    } else {
-      // correctly rounded quotients through the shared reciprocals
-      sc = qdiv(wr - wl, dcc, ydc);
-      sm = has_m ? qdiv(wl - (double)wll[v], dcm, ydm) : 0.;
-      sp = has_p ? qdiv((double)whh[v] - wr, dcp, ydp) : 0.;
+    for (int v = 0; v < Ly::NVAR; v++) {
+      const double wl = wl_[v], wr = wr_[v];
+      const float sc = (float)(wr - wl) * idc;
+      const float sm = has_m ? (float)(wl - (double)wll[v]) * idm : 0.f;
+      const float sp = has_p ? (float)((double)whh[v] - wr) * idp : 0.f;
+      const bool isa = (MODEL != MODEL_EULER) && (v < 3 * NPH) && (v % 3 == 0);
+      float gl, gr;
+      if constexpr (LV >= 0) {
+        gl = isa ? limiter_kt<LA, float>(sc, sm) : limiter_kt<LV, float>(sc, sm);
+        gr = isa ? limiter_kt<LA, float>(sc, sp) : limiter_kt<LV, float>(sc, sp);
+      } else {
+        const int llim = isa ? (ifl ? S.lim[3] : S.lim[2]) : (ifl ? S.lim[1] : S.lim[0]);
+        const int rlim = isa ? (ifr ? S.lim[3] : S.lim[2]) : (ifr ? S.lim[1] : S.lim[0]);
+        gl = limiter_f(llim, sc, sm);
+        gr = limiter_f(rlim, sc, sp);
+      }
+      xl[v] = wl + (double)(gl * fL);
+      xr[v] = wr + (double)(-gr * fR);
     }
-    const bool isa = (MODEL != MODEL_EULER) && (v < 3 * NPH) && (v % 3 == 0);
-    double gl, gr;
-    if constexpr (LV >= 0) {
-      gl = isa ? limiter_k<LA>(sc, sm) : limiter_k<LV>(sc, sm);
-      gr = isa ? limiter_k<LA>(sc, sp) : limiter_k<LV>(sc, sp);
-    } else {
-      const int llim = isa ? (ifl ? S.lim[3] : S.lim[2]) : (ifl ? S.lim[1] : S.lim[0]);
-      const int rlim = isa ? (ifr ? S.lim[3] : S.lim[2]) : (ifr ? S.lim[1] : S.lim[0]);
-      gl = limiter(llim, sc, sm);
-      gr = limiter(rlim, sc, sp);
+  } else {
+    // the reference's correctly rounded quotients through shared reciprocals
+    const double ydc = 1. / dcc, ydm = 1. / dcm, ydp = 1. / dcp;
+#pragma unroll
+    for (int v = 0; v < Ly::NVAR; v++) {
+      const double wl = wl_[v], wr = wr_[v];
+      const double sc = qdiv(wr - wl, dcc, ydc);
+      const double sm = has_m ? qdiv(wl - (double)wll[v], dcm, ydm) : 0.;
+      const double sp = has_p ? qdiv((double)whh[v] - wr, dcp, ydp) : 0.;
+      const bool isa = (MODEL != MODEL_EULER) && (v < 3 * NPH) && (v % 3 == 0);
+      double gl, gr;
+      if constexpr (LV >= 0) {
+        gl = isa ? limiter_k<LA>(sc, sm) : limiter_k<LV>(sc, sm);
+        gr = isa ? limiter_k<LA>(sc, sp) : limiter_k<LV>(sc, sp);
+      } else {
+        const int llim = isa ? (ifl ? S.lim[3] : S.lim[2]) : (ifl ? S.lim[1] : S.lim[0]);
+        const int rlim = isa ? (ifr ? S.lim[3] : S.lim[2]) : (ifr ? S.lim[1] : S.lim[0]);
+        gl = limiter(llim, sc, sm);
+        gr = limiter(rlim, sc, sp);
+      }
+      xl[v] = wl + gl * dL;
+      xr[v] = wr + (-gr) * dR;
     }
-    xl[v] = wl + gl * dL;
-    xr[v] = wr + (-gr) * dR;
   }
 }
 
