@@ -1,6 +1,7 @@
 // FP32-storage form of the MUSCL stage kernels (multiphase models).
 //
-// State in HBM: alpha_k, rho_k, p_k and the velocity in FP32, plus the reduced
+// State in HBM: the volume-fraction pair in two 32-bit words (one FP64
+// value, below), rho_k, p_k and the velocity in FP32, plus the reduced
 // mixture energy R = rho*E - sum_k alpha_k (gamma_k pinf_k/(gamma_k-1) + rho_k eref_k)
 // in FP32.  R is a pressure-scale quantity: the stiffened-gas offset is
 // re-added in FP64 from the stored alpha_k, rho_k, so the (alpha, E) pair
@@ -15,14 +16,25 @@
 
 namespace eco {
 
-// Two-phase volume fractions from their FP32 copies: the smaller one is
-// authoritative (FP32 keeps its relative precision), the larger is its FP64
-// complement.  A majority fraction 1 - 1e-6 stored in FP32 alone would carry
-// ~6e-8 absolute error, which the stiffened-gas energy alpha*gamma*pinf/(gamma-1)
-// turns into pressure errors of order 1e2 Pa.
-__device__ __forceinline__ void decode_alpha(float f0, float f1, double &a0, double &a1) {
-  if (f0 <= f1) { a0 = (double)f0; a1 = 1. - a0; }
-  else { a1 = (double)f1; a0 = 1. - a1; }
+// Two-phase volume fractions in the two 32-bit words of the alpha_0 and
+// alpha_1 planes: the smaller fraction as one FP64 value (high word in the
+// alpha_0 plane, low word in the alpha_1 plane), negated when it is phase
+// 1's; the larger is its FP64 complement.  The pair costs the bytes of two
+// FP32 values and stores no rounding: alpha rounded to FP32 every step
+// (6e-8 of the smaller fraction) was the largest source of FP32-storage
+// error on long runs, through the stiffened-gas energy
+// alpha*gamma*pinf/(gamma-1) and the pressure equilibrium.  The words are
+// only moved as bits, never used as FP32 numbers.
+__device__ __forceinline__ void decode_alpha(float w0, float w1, double &a0, double &a1) {
+  const double d = __hiloint2double(__float_as_int(w0), __float_as_int(w1));
+  if (__builtin_signbit(d)) { a1 = -d; a0 = 1. - a1; }
+  else { a0 = d; a1 = 1. - a0; }
+}
+
+__device__ __forceinline__ void encode_alpha(double a0, double a1, float &w0, float &w1) {
+  const double d = (a0 <= a1) ? a0 : -a1;
+  w0 = __int_as_float(__double2hiint(d));
+  w1 = __int_as_float(__double2loint(d));
 }
 
 template <int MODEL, int NPH, int DIM, typename PW>
@@ -40,9 +52,12 @@ __device__ __forceinline__ void load_prim_f(PW W, long long nc, long long idx, C
 
 template <int MODEL, int NPH, int DIM, typename PW>
 __device__ __forceinline__ void store_prim_f(PW W, long long nc, long long idx, const CellState<NPH> &s) {
+  float w0, w1;
+  encode_alpha(s.a[0], s.a[1], w0, w1);
+  W[idx] = w0;
+  W[3 * nc + idx] = w1;
 #pragma unroll
   for (int k = 0; k < NPH; k++) {
-    W[(3 * k) * nc + idx] = (float)s.a[k];
     W[(3 * k + 1) * nc + idx] = (float)s.r[k];
     W[(3 * k + 2) * nc + idx] = (float)s.p[k];
   }
@@ -62,7 +77,9 @@ __device__ __forceinline__ double energy_offset(const SchemeParams &S, const Cel
 // the FP32-rounded state as stored (for the offset of R)
 template <int NPH>
 __device__ __forceinline__ void round_state(CellState<NPH> &s) {
-  decode_alpha((float)s.a[0], (float)s.a[1], s.a[0], s.a[1]);
+  float w0, w1;
+  encode_alpha(s.a[0], s.a[1], w0, w1);
+  decode_alpha(w0, w1, s.a[0], s.a[1]);
 #pragma unroll
   for (int k = 0; k < NPH; k++) {
     s.r[k] = (double)(float)s.r[k];

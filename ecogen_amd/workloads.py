@@ -135,8 +135,10 @@ def shock_cylinders_2d(n=4096, ncyl=32, seed=0xEC0C0002, rmin=20.0, rmax=80.0, l
     return case, st
 
 
-def bubbles_3d(n=384, nb=64, seed=0xEC0C0003, rmin=4.0, rmax=24.0, limiter="minmod", steps=500, precision="fp64"):
-    """Config C3: ``nb`` air bubbles in water at 5e7 Pa, pressure-disequilibrium model with relaxation."""
+def bubbles_3d(n=384, nb=64, seed=0xEC0C0003, rmin=4.0, rmax=24.0, limiter="minmod", steps=500, precision="fp64",
+               box=(0.0, 1.0)):
+    """Config C3: ``nb`` air bubbles in water at 5e7 Pa, pressure-disequilibrium model with relaxation.
+    The bubbles lie inside ``box`` along every axis (the whole domain by default)."""
     case = {
         "model": "multip", "alpha_null": False, "eos": [AIR, WATER], "relax_p": True,
         "cells": [n, n, n], "lengths": [1.0, 1.0, 1.0], "order": 2, "limiters": [limiter] * 4,
@@ -150,7 +152,7 @@ def bubbles_3d(n=384, nb=64, seed=0xEC0C0003, rmin=4.0, rmax=24.0, limiter="minm
     _fill(st, slice(None), water)
     view = st.reshape(9, n, n, n)  # [field, k, j, i]
     dx = L[0] / n
-    for (cx, cy, cz), r in place(seed, nb, rmin, rmax, [(0.0, 1.0)] * 3, dx, log_radius=True):
+    for (cx, cy, cz), r in place(seed, nb, rmin, rmax, [tuple(box)] * 3, dx, log_radius=True):
         lo = [max(0, int((c - r) / dx) - 1) for c in (cx, cy, cz)]
         hi = [min(n, int((c + r) / dx) + 2) for c in (cx, cy, cz)]
         X = xc[lo[0]:hi[0]][None, None, :]

@@ -80,3 +80,43 @@ def compared_fields(case):
     for d in range(dim):
         out.append((3 * nph + d, "uvw"[d]))
     return out
+
+
+def conserved_totals(case, state):
+    """Per-phase mass, mixture momentum, sum rho|u| per axis and mixture total
+    energy of a two-phase state, each summed over the cells times the cell volume."""
+    nph = 2
+    dV = float(np.prod([case["lengths"][d] / case["cells"][d] for d in range(3)]))
+    rho = state[0] * state[1] + state[3] * state[4]
+    mass = np.array([(state[3 * k] * state[3 * k + 1]).sum() for k in range(nph)]) * dV
+    mom = np.array([(rho * state[3 * nph + d]).sum() for d in range(3)]) * dV
+    amom = np.array([(rho * np.abs(state[3 * nph + d])).sum() for d in range(3)]) * dV
+    energy = float((rho * state[3 * nph + 3]).sum()) * dV
+    return mass, mom, amom, energy
+
+
+def shock_inflow_errors(case, a, t1, b, t2):
+    """Balance of the planar-shock inflow problem (post-shock state on the
+    left boundary column, supersonic inflow, air at rest at 1e5 Pa on the
+    right, nothing yet at the other boundaries) between states ``a`` at time
+    ``t1`` and ``b`` at ``t2``: relative errors of per-phase mass, mixture
+    energy and x-momentum against what enters through x = 0, and the y-momentum
+    over sum rho|u|."""
+    n = case["cells"][0]
+    s = a[:, 0]
+    # the boundary column keeps the post-shock state (alpha, rho, p, u; rows
+    # away from the corners): its physical flux is what crosses x = 0
+    for st in (a, b):
+        assert np.allclose(st[:7, 2 * n:-2 * n:n], s[:7, None], rtol=1e-13, atol=0.0)
+    rho_s = s[0] * s[1] + s[3] * s[4]
+    u_s, p_s, E_s = s[6], s[0] * s[2] + s[3] * s[5], s[9]
+    Ly, dt = case["lengths"][1], t2 - t1
+    m0, q0, _, e0 = conserved_totals(case, a)
+    m1, q1, aq1, e1 = conserved_totals(case, b)
+    flux_m = np.array([s[3 * k] * s[3 * k + 1] * u_s for k in range(2)])
+    return {
+        "mass": np.abs(m1 - m0 - dt * Ly * flux_m) / m0,
+        "energy": abs(e1 - e0 - dt * Ly * (rho_s * E_s + p_s) * u_s) / e0,
+        "xmom": abs(q1[0] - q0[0] - dt * Ly * (rho_s * u_s * u_s + p_s - 1e5)) / q1[0],
+        "ymom": abs(q1[1]) / aq1[0],
+    }

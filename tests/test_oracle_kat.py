@@ -333,3 +333,40 @@ def test_conservation_closed_problem(order):
         assert m1 == pytest.approx(m0, rel=1e-13)
     rho = out[0] * out[1] + out[3] * out[4]
     assert abs((rho * out[6]).sum()) < 1e-10 * (rho * np.abs(out[6])).sum()  # symmetric: zero net momentum
+
+
+def test_conservation_shock_inflow_2d():
+    """The 2-D shock/cylinders problem at 128^2, 10 steps: per-phase mass,
+    energy and x-momentum change by exactly what the supersonic post-shock
+    inflow carries through x = 0, and no y-momentum appears (the GPU test of
+    the same balance runs at 4096^2)."""
+    from conftest import shock_inflow_errors
+    from ecogen_amd import workloads as W
+    n = 128
+    case, st = W.shock_cylinders_2d(n=n, rmin=20 * n / 4096, rmax=80 * n / 4096, steps=10)
+    a, _, t1, _, _ = O.run(case, st, nsteps=1)
+    b, it, t2, _, _ = O.run(case, st, nsteps=11)
+    assert it == 11 and t1 == 1e-15
+    err = shock_inflow_errors(case, a, t1, b, t2)
+    assert np.all(err["mass"] < 1e-14) and err["energy"] < 1e-14
+    assert err["xmom"] < 1e-12 and err["ymom"] == 0.0
+    # the balance sees a wrong inflow: one percent more boundary flux fails it
+    # by orders of magnitude (the water inflow is at the 1e-6 volume fraction)
+    err2 = shock_inflow_errors(case, a, t1, b, t1 + 1.01 * (t2 - t1))
+    assert err2["mass"][0] > 1e-5 and err2["mass"][1] > 1e-10 and err2["energy"] > 1e-6
+
+
+def test_conservation_closed_bubbles_3d():
+    """3-D MultiP bubbles with relaxation, kept 0.1 from the faces, 32^3, 3
+    steps: nothing crosses the boundary, so per-phase mass and mixture energy
+    are conserved and the total momentum stays zero to round-off."""
+    from conftest import conserved_totals
+    from ecogen_amd import workloads as W
+    case, st = W.bubbles_3d(n=32, nb=4, rmin=1.5, rmax=3.0, steps=3, box=(0.25, 0.75))
+    a, _, _, _, _ = O.run(case, st, nsteps=1)
+    b, _, _, _, _ = O.run(case, st, nsteps=4)
+    m0, _, _, e0 = conserved_totals(case, a)
+    m1, q1, aq1, e1 = conserved_totals(case, b)
+    assert np.all(np.abs(m1 - m0) / m0 < 1e-14)
+    assert abs(e1 - e0) / abs(e0) < 1e-14
+    assert np.all(np.abs(q1) < 1e-10 * aq1) and np.all(aq1 > 0)
