@@ -9,10 +9,11 @@ timing on the GPU:
 
 A variant is a set of build-time macros of ``csrc/eco_tuning.h`` (tile
 shape and workgroup size per dimension, minimum waves per SIMD requested
-from the compiler).
+from the compiler for the single-solve and the pool kernels).
 Each is timed on the stage kernels of a fixed workload with HIP events
-(eco_time_steps), after a parity check of its result against the default
-build's result on the same state.
+(eco_time_steps), or, for the pool kernels, by the wall time of a pool
+running the batched Riemann workload; its results must equal the default
+build's on the same states.
 """
 from __future__ import annotations
 
@@ -41,9 +42,16 @@ SPACE = {
     3: {"shapes": [((16, 4, 3), 256), ((16, 4, 4), 256), ((16, 4, 4), 320), ((8, 8, 4), 320)], "waves": [2]},
 }
 AXES = "XYZ"
+# "dimension" 0: the persistent pool kernels of the FP32-storage form, timed
+# on the batched Riemann workload (minimum waves per SIMD of the 1-D and
+# 2-D pool kernels; 0 keeps the single-solve request)
+POOL_SPACE = {"b1": [0, 4], "b2": [0, 3]}
 
 
 def variants(dim):
+    if dim == 0:
+        return [(f"pool_b1{w1}_b2{w2}", {"ECO_WAVESB32_1": w1, "ECO_WAVESB32_2": w2})
+                for w1, w2 in itertools.product(POOL_SPACE["b1"], POOL_SPACE["b2"])]
     out = []
     sp = SPACE[dim]
     for (t, nt), w in itertools.product(sp["shapes"], sp["waves"]):
@@ -56,13 +64,12 @@ def variants(dim):
 
 # the translation unit whose kernels a dimension's workload runs; the other
 # objects of a variant are those of the main build (python -m ecogen_amd.build)
-TU_FOR_DIM = {1: "eco_k_kapila32.hip", 2: "eco_k_kapila.hip", 3: "eco_k_multip.hip"}
+TU_FOR_DIM = {0: "eco_k_kapila32.hip", 1: "eco_k_kapila32.hip", 2: "eco_k_kapila.hip", 3: "eco_k_multip.hip"}
 
 
 def _compile_variant(item):
     from .build import ARCH, COMMON, HIPCC, OBJ, SOURCES
-    name, defs = item
-    dim = int(name[1])
+    dim, name, defs = item
     vdir = TUNE_DIR / name
     vdir.mkdir(parents=True, exist_ok=True)
     dflags = [f"-D{k}={v}" for k, v in defs.items()]
@@ -89,7 +96,7 @@ def _compile_variant(item):
 
 
 def cmd_build(args):
-    items = [v for d in args.dims for v in variants(d)]
+    items = [(d, name, defs) for d in args.dims for name, defs in variants(d)]
     with ThreadPoolExecutor(max_workers=args.jobs) as ex:
         for name, status in ex.map(_compile_variant, items):
             print(f"{name}: {status}", flush=True)
@@ -122,9 +129,45 @@ def _time_lib(lib_path, case, st, steps):
     return float(ms.sum()) / steps, out
 
 
+def _time_pool(lib_path, count):
+    """Load one variant library and run ``count`` cases of the batched
+    Riemann workload through a pool: (wall seconds, per-case results)."""
+    import time
+    from . import capi
+    from . import pool as P
+    from . import workloads as W
+    capi._lib, P._bound = None, False
+    capi.LIB_PATH = Path(lib_path)
+    cases = [W.riemann_case(*sp, precision="fp32s") for sp in W.batch_specs(count)]
+    with P.Pool(hbm_budget=48 << 30, steps_per_chunk=24) as pool:
+        ids = [pool.admit(c, st, t_final=tf) for c, st, tf in cases]
+        t0 = time.perf_counter()
+        pool.run_all()
+        wall = time.perf_counter() - t0
+        outs = [pool.result(i)[0] for i in ids]
+    capi._lib, P._bound = None, False
+    return wall, outs
+
+
+def _run_pool(args, results):
+    base_s, base = _time_pool(PKG / "libecogen_amd.so", args.pool_cases)
+    print(f"pool default build: {base_s:.2f} s for {args.pool_cases} cases", flush=True)
+    for name, defs in variants(0):
+        lib = TUNE_DIR / name / "libecogen_amd.so"
+        if not lib.exists():
+            continue
+        wall, outs = _time_pool(lib, args.pool_cases)
+        dev = max(float(np.abs(a - b).max()) for a, b in zip(outs, base))
+        results.append({"dim": 0, "variant": name, "defs": defs, "wall_ms": wall * 1e3, "max_abs_diff": dev})
+        print(f"  {name}: {wall:.2f} s  (max |diff| vs default {dev:.2e})", flush=True)
+
+
 def cmd_run(args):
     results = []
     for dim in args.dims:
+        if dim == 0:
+            _run_pool(args, results)
+            continue
         case, st = _workload(dim, {1: args.n1, 2: args.n2, 3: args.n3}[dim])
         base_ms, base = _time_lib(PKG / "libecogen_amd.so", case, st, args.steps)
         print(f"dim {dim} default build: {base_ms:.3f} ms/step", flush=True)
@@ -140,19 +183,23 @@ def cmd_run(args):
     Path(args.out).write_text(json.dumps(results, indent=1))
 
 
+def _cost(r):
+    return r["ms_per_step"] if "ms_per_step" in r else r["wall_ms"]
+
+
 def cmd_apply(args):
     res = json.loads(Path(args.results).read_text())
     best = {}
     for r in res:
         if r["max_abs_diff"] != 0.0:
             continue  # a tile shape must not change the result
-        if r["dim"] not in best or r["ms_per_step"] < best[r["dim"]]["ms_per_step"]:
+        if r["dim"] not in best or _cost(r) < _cost(best[r["dim"]]):
             best[r["dim"]] = r
     hdr = (CSRC / "eco_tuning.h").read_text()
     for dim, r in sorted(best.items()):
         for k, v in r["defs"].items():
             hdr = re.sub(rf"#define {k} \w+", f"#define {k} {v}", hdr)
-        print(f"dim {dim}: {r['variant']} {r['ms_per_step']:.3f} ms/step")
+        print(f"dim {dim}: {r['variant']} {_cost(r):.3f} ms")
     (CSRC / "eco_tuning.h").write_text(hdr)
 
 
@@ -160,14 +207,15 @@ def main(argv=None):
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
     b = sub.add_parser("build")
-    b.add_argument("--dims", type=int, nargs="+", default=[1, 2, 3])
+    b.add_argument("--dims", type=int, nargs="+", default=[0, 1, 2, 3])
     b.add_argument("--jobs", type=int, default=min(8, os.cpu_count() or 1))
     r = sub.add_parser("run")
-    r.add_argument("--dims", type=int, nargs="+", default=[1, 2, 3])
+    r.add_argument("--dims", type=int, nargs="+", default=[0, 1, 2, 3])
     r.add_argument("--n1", type=int, default=1 << 22)
     r.add_argument("--n2", type=int, default=4096)
     r.add_argument("--n3", type=int, default=256)
     r.add_argument("--steps", type=int, default=10)
+    r.add_argument("--pool-cases", type=int, default=512)
     r.add_argument("--out", default="gpurun_out/autotune.json")
     a = sub.add_parser("apply")
     a.add_argument("results")

@@ -52,16 +52,19 @@ template <> struct Tile<1> {
   static constexpr int X = ECO_TILE1_X, Y = 1, Z = 1, W = ECO_WAVES1, LS = ECO_LIMSPEC1;
   static constexpr int C = X * Y * Z, T = ECO_THREADS1 > 0 ? ECO_THREADS1 : C;
   static constexpr int WB = ECO_WAVESB1 > 0 ? ECO_WAVESB1 : W;
+  static constexpr int WB32 = ECO_WAVESB32_1 > 0 ? ECO_WAVESB32_1 : WB;
 };
 template <> struct Tile<2> {
   static constexpr int X = ECO_TILE2_X, Y = ECO_TILE2_Y, Z = 1, W = ECO_WAVES2, LS = ECO_LIMSPEC2;
   static constexpr int C = X * Y * Z, T = ECO_THREADS2 > 0 ? ECO_THREADS2 : C;
   static constexpr int WB = ECO_WAVESB2 > 0 ? ECO_WAVESB2 : W;
+  static constexpr int WB32 = ECO_WAVESB32_2 > 0 ? ECO_WAVESB32_2 : WB;
 };
 template <> struct Tile<3> {
   static constexpr int X = ECO_TILE3_X, Y = ECO_TILE3_Y, Z = ECO_TILE3_Z, W = ECO_WAVES3, LS = ECO_LIMSPEC3;
   static constexpr int C = X * Y * Z, T = ECO_THREADS3 > 0 ? ECO_THREADS3 : C;
   static constexpr int WB = ECO_WAVESB3 > 0 ? ECO_WAVESB3 : W;
+  static constexpr int WB32 = ECO_WAVESB32_3 > 0 ? ECO_WAVESB32_3 : WB;
 };
 static_assert(Tile<1>::T >= Tile<1>::C && Tile<1>::T % 64 == 0, "1-D workgroup");
 static_assert(Tile<2>::T >= Tile<2>::C && Tile<2>::T % 64 == 0, "2-D workgroup");

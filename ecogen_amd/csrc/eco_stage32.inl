@@ -470,7 +470,7 @@ k_stage32(const StageArgs A) {
 
 // persistent form over a pool (as k_stage_batch)
 template <int MODEL, int NPH, int DIM, int STAGE, bool SAMELIM>
-__global__ void __launch_bounds__(Tile<DIM>::T, Tile<DIM>::WB)
+__global__ void __launch_bounds__(Tile<DIM>::T, Tile<DIM>::WB32)
 k_stage32_batch(const StageArgs *__restrict__ args, const TileRef *__restrict__ tiles,
                                    const int *__restrict__ ntile, const int slot) {
   __shared__ StageLds<MODEL, NPH, DIM> lds;

@@ -64,3 +64,19 @@
 #ifndef ECO_WAVESB3
 #define ECO_WAVESB3 0
 #endif
+// minimum waves per SIMD requested for the FP32-storage pool kernels (0 = as
+// the FP64 pool kernels), chosen by the autotuner's pool target.  At their
+// natural register counts (179 and 196 VGPRs) the 1-D and 2-D kernels run
+// one 512-thread or two 256-thread workgroups per CU; capping them at 128
+// and 168 VGPRs (a few spilled values) doubles that and took the 2048-case
+// batched run from 32.2 to about 28 s on MI355X.  The FP64 pool kernels
+// would spill 300 bytes per lane at those caps and keep their own.
+#ifndef ECO_WAVESB32_1
+#define ECO_WAVESB32_1 4
+#endif
+#ifndef ECO_WAVESB32_2
+#define ECO_WAVESB32_2 3
+#endif
+#ifndef ECO_WAVESB32_3
+#define ECO_WAVESB32_3 0
+#endif
