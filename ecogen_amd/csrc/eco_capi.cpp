@@ -548,7 +548,10 @@ extern "C" int eco_copy_calibration(void *dst, const void *src, unsigned long lo
 }
 
 extern "C" const double *eco_state_device(eco_ctx *x, int *nfields_device) {
-  if (!x) return nullptr;
+  if (nfields_device) *nfields_device = 0;
+  // FP32-storage contexts hold a packed format (FP32 fields, the alpha pair
+  // as one FP64 value in two 32-bit words, a reduced energy): no FP64 view
+  if (!x || x->es != sizeof(double)) return nullptr;
   if (nfields_device) *nfields_device = x->nprim + 1;
   return x->buf[x->step & 1];
 }

@@ -128,7 +128,9 @@ int eco_time_steps(eco_ctx *ctx, int nsteps, double *stage_ms, int *launches, ec
 int eco_copy_calibration(void *dst, const void *src, unsigned long long nbytes, void *hip_stream);
 
 /* Device pointer of the current state (device layout: per phase alpha, rho, p,
-   then the nondegenerate velocity components, then E) for zero-copy access. */
+   then the nondegenerate velocity components, then E) for zero-copy access.
+   FP64 contexts only: an ECO_FP32S context keeps a packed storage format and
+   returns NULL with *nfields_device = 0 (eco_get_state converts it). */
 const double *eco_state_device(eco_ctx *ctx, int *nfields_device);
 
 /* Releases device memory (Run::finalize, src/Run.cpp:496-530). */

@@ -63,3 +63,25 @@ def test_continuation_and_async_noops(tube):
         s = ctx.sync()
         assert s.steps == full.steps and s.t == full.t
         assert np.array_equal(ctx.get_state(), ref)
+
+
+def test_zero_copy_state_view(tube):
+    """eco_state_device exposes the FP64 device state (set_state_device of
+    that view reproduces it) and refuses the packed FP32-storage format."""
+    case, st = tube
+    with Context(case) as ctx:
+        ctx.set_state(st)
+        ctx.run(nsteps=10)
+        ref = ctx.get_state()
+        ptr, nf = ctx.state_device()
+        assert ptr and nf == ref.shape[0] - 2  # device layout: 1-D, one velocity component
+        with Context(case) as other:
+            other.set_state_device(ptr)
+            got = other.get_state()
+    assert np.array_equal(got[:7], ref[:7])
+    from ecogen_amd import workloads as W2
+    c32, s32, _ = W2.riemann_case(1, 200, 10.0, 0.5, precision="fp32s")
+    with Context(c32) as ctx:
+        ctx.set_state(s32)
+        ptr, nf = ctx.state_device()
+        assert not ptr and nf == 0
