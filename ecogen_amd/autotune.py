@@ -37,8 +37,8 @@ CSRC = PKG / "csrc"
 # than its tile so that the face pass ends in fewer iterations
 SPACE = {
     1: {"shapes": [((504,), 512), ((248,), 256), ((1016,), 1024), ((256,), 256)], "waves": [1, 2]},
-    2: {"shapes": [((32, 8), 256), ((16, 15), 256), ((32, 7), 256), ((16, 16), 320), ((20, 12), 256)],
-        "waves": [2]},
+    2: {"shapes": [((32, 8), 256), ((64, 4), 256), ((16, 8), 128), ((32, 4), 128), ((16, 15), 256)],
+        "waves": [2, 3]},
     3: {"shapes": [((16, 4, 3), 256), ((16, 4, 4), 256), ((16, 4, 4), 320), ((8, 8, 4), 320)], "waves": [2]},
 }
 AXES = "XYZ"

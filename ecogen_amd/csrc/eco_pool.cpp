@@ -262,9 +262,14 @@ static int class_for(eco_pool *p, const ClassKey &key) {
     if (p->classes[i].key == key) return (int)i;
   Class c;
   c.key = key;
-  if (hipMalloc(&c.d_args, sizeof(StageArgs) * 4 * (size_t)p->max_cases) != hipSuccess) return -1;
-  if (hipHostMalloc(&c.h_args, sizeof(StageArgs) * 4 * (size_t)p->max_cases, 0) != hipSuccess) return -1;
-  if (hipMalloc(&c.d_desc, sizeof(BatchDesc)) != hipSuccess) return -1;
+  if (hipMalloc(&c.d_args, sizeof(StageArgs) * 4 * (size_t)p->max_cases) != hipSuccess ||
+      hipHostMalloc(&c.h_args, sizeof(StageArgs) * 4 * (size_t)p->max_cases, 0) != hipSuccess ||
+      hipMalloc(&c.d_desc, sizeof(BatchDesc)) != hipSuccess) {
+    if (c.d_args) (void)hipFree(c.d_args);
+    if (c.h_args) (void)hipHostFree(c.h_args);
+    if (c.d_desc) (void)hipFree(c.d_desc);
+    return -1;
+  }
   for (int i = p->max_cases - 1; i >= 0; i--) c.free_argslots.push_back(i);
   // persistent grid: resident workgroups of the heavier (second) kernel of the step
   int occ = 0;
