@@ -17,6 +17,7 @@ MODEL = {"euler": 0, "kapila": 1, "multip": 2}
 LIMITER = {"none": 0, "minmod": 1, "vanleer": 2, "vanalbada": 3, "superbee": 4, "mc": 5, "thinc": 6}
 EOS_TYPE = {"IG": 0, "SG": 1}
 PRECISION = {"fp64": 0, "fp32s": 1}
+BC = {"abs": 0, "wall": 1}  # per side X-, X+, Y-, Y+, Z-, Z+; "wall" at first order only
 
 WARN_RELAX_NOT_CONVERGED = 1
 WARN_NAN = 2
@@ -118,8 +119,8 @@ def make_config(case) -> EcoConfig:
         c.n[d] = case["cells"][d]
         c.L[d] = case["lengths"][d]
     c.cfl = case["cfl"]
-    for b in range(6):
-        c.bc[b] = 0
+    for b, name in enumerate(case.get("bc", ["abs"] * 6)):
+        c.bc[b] = BC[name]
     for k, e in enumerate(case["eos"]):
         c.eos[k] = EcoEos(EOS_TYPE[e["type"]], e["gamma"], e.get("pinf", 0.0), e["cv"], e.get("eref", 0.0))
     return c

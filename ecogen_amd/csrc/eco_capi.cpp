@@ -85,7 +85,11 @@ int eco::validate_config(const eco_config *c) {
     if (c->lim[0] == ECO_LIM_THINC || c->lim[1] == ECO_LIM_THINC) return ECO_ERR_INVALID;
     for (int i = 0; i < 4; i++) if (c->lim[i] == ECO_LIM_NONE) return ECO_ERR_INVALID;
   }
-  for (int i = 0; i < 6; i++) if (c->bc[i] != ECO_BC_ABS) return ECO_ERR_UNSUPPORTED;
+  for (int i = 0; i < 6; i++) {
+    if (c->bc[i] != ECO_BC_ABS && c->bc[i] != ECO_BC_WALL) return ECO_ERR_INVALID;
+    // a second-order WALL deck is BoundCondWallO2 (mirrored velocity slopes), not built
+    if (c->bc[i] == ECO_BC_WALL && c->order != 1) return ECO_ERR_UNSUPPORTED;
+  }
   if (c->model == ECO_MODEL_EULER) {
     if (c->nphase != 1) return ECO_ERR_INVALID;
   } else {
@@ -159,6 +163,8 @@ void eco::lower_config(const eco_config &c, Lowered &L) {
   S.relax = (c.model == ECO_MODEL_KAPILA) || (c.model == ECO_MODEL_MULTIP && c.relax_p);
   S.cfl = c.cfl;
   S.prec = c.precision;
+  S.bc_wall = 0;
+  for (int b = 0; b < 6; b++) if (c.bc[b] == ECO_BC_WALL) S.bc_wall |= 1 << b;
   for (int k = 0; k < c.nphase && k < kMaxPhases; k++) lower_eos(c.eos[k], S.eos[k]);
   L.nprim = (c.model == ECO_MODEL_EULER) ? 2 + dim : 3 * c.nphase + dim;
   L.es = (c.precision == ECO_FP32S) ? sizeof(float) : sizeof(double);

@@ -51,6 +51,7 @@ struct SchemeParams {
   int relax;         // stiff pressure relaxation applied
   double cfl;
   int prec;          // PREC_FP64 or PREC_FP32S (storage)
+  int bc_wall;       // bit 2*dir+side set: wall boundary (BoundCondWall), else transmissive
   EosParams eos[kMaxPhases];
 };
 

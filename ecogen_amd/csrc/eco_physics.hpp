@@ -429,6 +429,29 @@ ECO_D void correction_energy(const SchemeParams &S, CellState<NPH> &s) {
 // em_* is the mixture internal energy the reference reads (ΣY_k e_k after
 // fulfillState, or E - |u|^2/2 for MultiP first-order interior faces).
 // Returns max(|sL|,|sR|) among speeds above the 1e-3 guard (0 if none).
+// The wall's half Riemann problem in the face frame: ModKapila::solveRiemannWall
+// (ModKapila.cpp:224-249), ModMultiP::solveRiemannWall (ModMultiP.cpp:220-245),
+// ModEuler::solveRiemannWall (ModEuler.cpp:181-210).  Only the normal momentum
+// flux p* is nonzero and sM = 0; the products keep each model's operand order.
+// Returns |sL| when above the 1e-3 guard of the dt estimate, else 0.
+template <int MODEL, int NPH>
+ECO_D double wall_flux(const CellState<NPH> &L, FluxVec<NPH> &F) {
+#pragma unroll
+  for (int k = 0; k < NPH; k++) { F.a[k] = 0.; F.m[k] = 0.; F.en[k] = 0.; }
+  F.q[1] = 0.; F.q[2] = 0.; F.E = 0.; F.sM = 0.;
+  double sL;
+  if constexpr (MODEL == MODEL_EULER) {
+    const double uL = L.u[0], pL = L.p[0], rhoL = L.r[0], cL = L.c[0];
+    sL = smin(uL - cL, -uL - cL);
+    F.q[0] = rhoL * uL * (uL - sL) + pL;
+  } else {
+    const double uL = L.u[0], pL = L.pm, rhoL = L.rho, cL = L.cF;
+    sL = smin(uL - cL, -uL - cL);
+    F.q[0] = rhoL * (uL - sL) * uL + pL;
+  }
+  return (fabs(sL) > 1.e-3) ? fabs(sL) : 0.;
+}
+
 template <int MODEL, int NPH>
 ECO_D double hllc(const SchemeParams &S, const CellState<NPH> &L, const CellState<NPH> &R, FluxVec<NPH> &F) {
   const double eps = S.eps;

@@ -735,7 +735,8 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int slot, c
     CellState<NPH> own;
     load_prim<MODEL, NPH, DIM>(Wn, nc, cidx, own);
     own.E = Wn[Ly::IE * nc + cidx];
-    // transmissive boundary faces (BoundCond::computeFlux + BoundCondAbs, BoundCond.cpp:68-100)
+    // boundary faces (BoundCond::computeFlux, BoundCond.cpp:68-100): transmissive
+    // (BoundCondAbs, the interior solver on (L, L)) or wall (BoundCondWall)
 #pragma unroll
     for (int dir = 0; dir < DIM; dir++) {
 #pragma unroll
@@ -747,7 +748,8 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int slot, c
         to_face_frame(dir, side == 0, L.u);
         fulfill<MODEL, NPH>(S, L);
         FluxVec<NPH> F;
-        smx = smax(smx, hllc<MODEL, NPH>(S, L, L, F));
+        if ((S.bc_wall >> (2 * dir + side)) & 1) smx = smax(smx, wall_flux<MODEL, NPH>(L, F));
+        else smx = smax(smx, hllc<MODEL, NPH>(S, L, L, F));
         from_face_frame(dir, side == 0, F.q);
         add_face<MODEL, NPH>(acc, F, G.coefA[dir], false, own);
       }

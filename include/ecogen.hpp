@@ -163,6 +163,9 @@ class ModMultiP : public Model {
 struct CartesianMesh {
   int numberCells[3] = {1, 1, 1};
   double dimensions[3] = {1., 1., 1.};
+  // boundary condition per side X-, X+, Y-, Y+, Z-, Z+ (initialConditions
+  // <boundCond type="abs|wall">; a wall needs first order)
+  int boundCond[6] = {ECO_BC_ABS, ECO_BC_ABS, ECO_BC_ABS, ECO_BC_ABS, ECO_BC_ABS, ECO_BC_ABS};
 };
 
 struct RunStatus {

@@ -126,7 +126,8 @@ def write_deck(case, root: Path):
     doms = "\n".join(_domain_xml(i, d) for i, d in enumerate(case["domains"]))
     names = ["CLXm", "CLXp", "CLYm", "CLYp", "CLZm", "CLZp"]
     dim = sum(1 for c in n if c != 1)
-    bcs = "\n".join(f'    <boundCond name="{names[i]}" type="abs" number="{i + 1}"/>' for i in range(2 * dim))
+    kinds = case.get("bc", ["abs"] * 6)
+    bcs = "\n".join(f'    <boundCond name="{names[i]}" type="{kinds[i]}" number="{i + 1}"/>' for i in range(2 * dim))
     states = "\n".join(_state_xml(case, st) for st in case["states"])
     (cdir / "initialConditionsV4.xml").write_text(
         '<?xml version="1.0" encoding="UTF-8" standalone="yes"?>\n<CI>\n  <physicalDomains>\n'

@@ -515,6 +515,26 @@ static void riemann(const Ctx *c, const State *L, const State *R, double dxL, do
   else hllc_multi(c, L, R, dxL, dxR, dtMax, F);
 }
 
+/* The wall's half Riemann problem in the face frame: ModKapila::solveRiemannWall
+   (ModKapila.cpp:224-249), ModMultiP::solveRiemannWall (ModMultiP.cpp:220-245),
+   ModEuler::solveRiemannWall (ModEuler.cpp:181-210).  Only the normal
+   momentum flux p* is nonzero and sM = 0 (F arrives zeroed).  The two
+   products keep the reference's operand order, which differs by model. */
+static void riemann_wall(const Ctx *c, const State *L, double dxL, double *dtMax, Flux *F) {
+  double uL, cL, pL, rhoL, sL;
+  if (c->cfg->model == ORC_EULER) {
+    uL = L->ph[0].u[0]; pL = L->ph[0].p; rhoL = L->ph[0].rho; cL = L->ph[0].c;
+    sL = smin(uL - cL, -uL - cL);
+    if (fabs(sL) > 1.e-3) *dtMax = smin(*dtMax, dxL / fabs(sL));
+    F->qdm[0] = rhoL * uL * (uL - sL) + pL;
+  } else {
+    uL = L->mx.u[0]; cL = L->mx.cF; pL = L->mx.p; rhoL = L->mx.rho;
+    sL = smin(uL - cL, -uL - cL);
+    if (fabs(sL) > 1.e-3) *dtMax = smin(*dtMax, dxL / fabs(sL));
+    F->qdm[0] = rhoL * (uL - sL) * uL + pL;
+  }
+}
+
 /* ------------------------------------------------------------------------ */
 /* Mesh: src/Meshes/MeshCartesian.cpp:154-160, 180-552                       */
 /* ------------------------------------------------------------------------ */
@@ -526,6 +546,7 @@ typedef struct {
   double pos[3];
   int opp_left;       /* face opposite to this one across the left cell (-1: none/boundary) */
   int opp_right;      /* face opposite across the right cell */
+  int side;           /* boundary faces: 0..5 = X-, X+, Y-, Y+, Z-, Z+ (interior: -1) */
 } Face;
 
 typedef struct {
@@ -607,13 +628,13 @@ static int mesh_build(Mesh *m, const orc_cfg *cfg) {
   if (nx != 1) {
     for (int iy = 0; iy < ny; iy++) for (int iz = 0; iz < nz; iz++) {
       Face *F = &m->faces[f++];
-      F->left = cidx(m, 0, iy, iz); F->right = -1; F->dir = 0;
+      F->left = cidx(m, 0, iy, iz); F->right = -1; F->dir = 0; F->side = 0;
       set3(F->tan, 0., -1., 0.); set3(F->nrm, -1., 0., 0.); set3(F->bin, 0., 0., 1.);
       F->surface = dY * dZ; set3(F->pos, 0., m->posy[iy], m->posz[iz]);
     }
     for (int iy = 0; iy < ny; iy++) for (int iz = 0; iz < nz; iz++) {
       Face *F = &m->faces[f++];
-      F->left = cidx(m, nx - 1, iy, iz); F->right = -1; F->dir = 0;
+      F->left = cidx(m, nx - 1, iy, iz); F->right = -1; F->dir = 0; F->side = 1;
       set3(F->tan, 0., 1., 0.); set3(F->nrm, 1., 0., 0.); set3(F->bin, 0., 0., 1.);
       F->surface = dY * dZ; set3(F->pos, m->posx[nx - 1] + 0.5 * dX, m->posy[iy], m->posz[iz]);
     }
@@ -621,13 +642,13 @@ static int mesh_build(Mesh *m, const orc_cfg *cfg) {
   if (ny != 1) {
     for (int ix = 0; ix < nx; ix++) for (int iz = 0; iz < nz; iz++) {
       Face *F = &m->faces[f++];
-      F->left = cidx(m, ix, 0, iz); F->right = -1; F->dir = 1;
+      F->left = cidx(m, ix, 0, iz); F->right = -1; F->dir = 1; F->side = 2;
       set3(F->tan, 1., 0., 0.); set3(F->nrm, 0., -1., 0.); set3(F->bin, 0., 0., 1.);
       F->surface = dX * dZ; set3(F->pos, m->posx[ix], 0., m->posz[iz]);
     }
     for (int ix = 0; ix < nx; ix++) for (int iz = 0; iz < nz; iz++) {
       Face *F = &m->faces[f++];
-      F->left = cidx(m, ix, ny - 1, iz); F->right = -1; F->dir = 1;
+      F->left = cidx(m, ix, ny - 1, iz); F->right = -1; F->dir = 1; F->side = 3;
       set3(F->tan, -1., 0., 0.); set3(F->nrm, 0., 1., 0.); set3(F->bin, 0., 0., 1.);
       F->surface = dX * dZ; set3(F->pos, m->posx[ix], m->posy[ny - 1] + 0.5 * dY, m->posz[iz]);
     }
@@ -635,13 +656,13 @@ static int mesh_build(Mesh *m, const orc_cfg *cfg) {
   if (nz != 1) {
     for (int ix = 0; ix < nx; ix++) for (int iy = 0; iy < ny; iy++) {
       Face *F = &m->faces[f++];
-      F->left = cidx(m, ix, iy, 0); F->right = -1; F->dir = 2;
+      F->left = cidx(m, ix, iy, 0); F->right = -1; F->dir = 2; F->side = 4;
       set3(F->tan, -1., 0., 0.); set3(F->nrm, 0., 0., -1.); set3(F->bin, 0., 1., 0.);
       F->surface = dX * dY; set3(F->pos, m->posx[ix], m->posy[iy], 0.);
     }
     for (int ix = 0; ix < nx; ix++) for (int iy = 0; iy < ny; iy++) {
       Face *F = &m->faces[f++];
-      F->left = cidx(m, ix, iy, nz - 1); F->right = -1; F->dir = 2;
+      F->left = cidx(m, ix, iy, nz - 1); F->right = -1; F->dir = 2; F->side = 5;
       set3(F->tan, 1., 0., 0.); set3(F->nrm, 0., 0., 1.); set3(F->bin, 0., 1., 0.);
       F->surface = dX * dY; set3(F->pos, m->posx[ix], m->posy[iy], m->posz[nz - 1] + 0.5 * dZ);
     }
@@ -842,11 +863,13 @@ static void face_fluxes(Solver *S, State *st, int o2) {
     Flux *FL = &S->fflux[fi];
     memset(FL, 0, sizeof(Flux));
     if (F->right < 0) {
-      /* BoundCond::solveRiemann (BoundCond.cpp:82-100) + BoundCondAbs (BoundCondAbs.cpp:70-73) */
+      /* BoundCond::solveRiemann (BoundCond.cpp:82-100) + BoundCondAbs (BoundCondAbs.cpp:70-73)
+         or BoundCondWall (BoundCondWall.cpp:68-71) */
       State L = st[F->left];
       proj_state(c, &L, F);
       fulfill(c, &L);
-      riemann(c, &L, &L, lcfl, lcfl, &dtm, FL);
+      if (c->cfg->bc[F->side] == ORC_BC_WALL) riemann_wall(c, &L, lcfl, &dtm, FL);
+      else riemann(c, &L, &L, lcfl, lcfl, &dtm, FL);
     } else if (!o2) {
       /* CellInterface::solveRiemann (CellInterface.cpp:127-146): the cells' own cached state */
       State L = st[F->left], R = st[F->right];
@@ -1087,6 +1110,10 @@ int orc_run(const orc_cfg *cfg, const double *state_in, double *state_out,
   if (!cfg || cfg->nph < 1 || cfg->nph > MAXPH) return -1;
   if (cfg->model == ORC_EULER && cfg->nph != 1) return -1;
   if (is_thinc(cfg) && cfg->nph != 2) return -1;
+  for (int b = 0; b < 6; b++) {
+    if (cfg->bc[b] != ORC_BC_ABS && cfg->bc[b] != ORC_BC_WALL) return -1;
+    if (cfg->bc[b] == ORC_BC_WALL && cfg->order != 1) return -1; /* second order: BoundCondWallO2, not restated */
+  }
 #ifdef _OPENMP
   if (nthreads > 0) omp_set_num_threads(nthreads);
 #else

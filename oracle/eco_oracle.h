@@ -26,6 +26,10 @@ enum { ORC_LIM_NONE = 0, ORC_LIM_MINMOD = 1, ORC_LIM_VANLEER = 2, ORC_LIM_VANALB
 enum { ORC_EOS_IG = 0, ORC_EOS_SG = 1 };
 #define ORC_MAXPH 4
 
+/* boundary conditions: BoundCondAbs, and BoundCondWall (the reference's first-order
+   WALL and SYMMETRY, src/InputOutput/Input.cpp:850-852) */
+enum { ORC_BC_ABS = 0, ORC_BC_WALL = 1 };
+
 typedef struct {
   int type;
   double gamma, pinf, cv, eref;
@@ -42,6 +46,7 @@ typedef struct {
   double L[3];
   double cfl;
   orc_eos eos[ORC_MAXPH];
+  int bc[6];         /* X-, X+, Y-, Y+, Z-, Z+: ORC_BC_ABS or ORC_BC_WALL (first order) */
 } orc_cfg;
 
 /*

@@ -20,6 +20,7 @@ LIB_PATH = HERE / "libeco_oracle.so"
 MODELS = {"euler": 0, "kapila": 1, "multip": 2}
 LIMITERS = {"none": 0, "minmod": 1, "vanleer": 2, "vanalbada": 3, "superbee": 4, "mc": 5, "thinc": 6}
 EOS_TYPES = {"IG": 0, "SG": 1}
+BCS = {"abs": 0, "wall": 1}  # per side X-, X+, Y-, Y+, Z-, Z+; "wall" is first order only
 MAXPH = 4
 
 
@@ -31,7 +32,7 @@ class OrcEos(C.Structure):
 class OrcCfg(C.Structure):
     _fields_ = [("model", C.c_int), ("nph", C.c_int), ("order", C.c_int), ("lim", C.c_int * 4),
                 ("alpha_null", C.c_int), ("relax_p", C.c_int), ("n", C.c_int * 3),
-                ("L", C.c_double * 3), ("cfl", C.c_double), ("eos", OrcEos * MAXPH)]
+                ("L", C.c_double * 3), ("cfl", C.c_double), ("eos", OrcEos * MAXPH), ("bc", C.c_int * 6)]
 
 
 _lib = None
@@ -92,6 +93,8 @@ def make_cfg(case) -> OrcCfg:
     cfg.cfl = case["cfl"]
     for k, e in enumerate(case["eos"]):
         cfg.eos[k] = make_eos(e)
+    for b, name in enumerate(case.get("bc", ["abs"] * 6)):
+        cfg.bc[b] = BCS[name]
     return cfg
 
 

@@ -160,4 +160,11 @@ CASES = {
     "c2_kapila_2d_fo": dict(_shock_cylinders_2d(n=40, steps=40), order=1, limiters=["none"] * 4),
     "c3_multip_3d_o2": _bubbles_3d(),
     "kapila_3d_o2": _bubbles_3d(n=12, steps=10, model="kapila", relax=False, limiters=("vanleer",) * 4),
+    # Wall boundaries (first order: BoundCondWall, also the reference's first-order SYMMETRY):
+    # the C1 tube closed at both ends, run until the air shock has reflected off x = 1;
+    # MultiP and Euler tubes with one wall; the 2-D cylinders in a closed box.
+    "c1_kapila_fo_wall": dict(_c1(False, t_final=600e-6, cells=400), bc=["wall"] * 6),
+    "multip_1d_fo_wall": dict(_shock_tube_1d("multip", 1, ["none"] * 4, 300, cells=200), bc=["wall", "abs"] + ["abs"] * 4),
+    "euler_1d_fo_wall": dict(_euler_1d(1, ["none"] * 4, 300), bc=["abs", "wall"] + ["abs"] * 4),
+    "kapila_2d_fo_wall": dict(_shock_cylinders_2d(n=40, steps=80), order=1, limiters=["none"] * 4, bc=["wall"] * 6),
 }

@@ -62,3 +62,15 @@ def test_gpu_matches_oracle_and_reference(name):
     for nm in gp:
         e = rel_l1(gp[nm], wp[nm])
         assert e <= TOL_PHASE, f"{nm}: rel L1 vs oracle {e:.3e}"
+
+
+def test_second_order_wall_is_refused():
+    """A second-order WALL deck is BoundCondWallO2 in the reference (mirrored
+    velocity slopes), which the device path does not build: refused, not
+    silently run as a first-order wall."""
+    from ecogen_amd import Context, EcoError
+    case, _ = load_golden("kapila_1d_o2_vanleer")
+    with pytest.raises(EcoError):
+        Context(dict(case, bc=["wall"] * 6))
+    case, _ = load_golden("c1_kapila_fo_wall")
+    Context(case).close()
