@@ -1,0 +1,307 @@
+"""Reader of the reference's input decks, so that an ECOGEN run directory runs
+unchanged on the device path.
+
+A run directory holds ``ECOGEN.xml`` (the test-case folder), the case files
+``mainV5.xml``, ``meshV5.xml``, ``modelV4.xml`` and ``initialConditionsV4.xml``,
+and the EOS files under ``libEOS/``.  ``load`` turns it into the case
+description of :class:`ecogen_amd.Context` and the initial state in the
+``ecogen.h`` host layout, following ``Input::entreeMain``, ``entreeMesh``,
+``entreeModel``, ``entreeEOS`` and ``entreeConditionsInitiales``
+(``src/InputOutput/Input.cpp:85-864``) for what the device path runs:
+
+* Cartesian meshes without AMR;
+* the Euler, Kapila and MultiP flow models, ideal- and stiffened-gas EOS, the
+  MultiP pressure relaxation;
+* first order, or MUSCL with the four limiters and their defaults
+  (``Input.cpp:185-249``);
+* iteration or physical-time control and the CFL number;
+* the geometric domains entireDomain, halfSpace, disc, sphere and rectangle,
+  applied in deck order (``Cell::fill``, ``src/Cell.cpp:130-137``; the
+  ``belong`` tests of ``src/Geometries/GD*.cpp``);
+* ``abs`` boundaries, and ``wall`` / ``symmetry`` at first order (the
+  reference makes both a ``BoundCondWall`` there, ``Input.cpp:849-852``).
+
+Anything else raises ``DeckError`` naming the element.
+
+Command line (runs the deck on the GPU and saves the final state)::
+
+    python -m ecogen_amd.deck RUNDIR [--out final.npz] [--precision fp64|fp32s]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import xml.etree.ElementTree as ET
+from pathlib import Path
+
+import numpy as np
+
+LIMITERS = {"MINMOD": "minmod", "VANLEER": "vanleer", "VANALBADA": "vanalbada", "SUPERBEE": "superbee",
+            "MC": "mc", "THINC": "thinc"}
+MODELS = {"EULER": "euler", "KAPILA": "kapila", "MULTIP": "multip"}
+AXES = {"X": 0, "Y": 1, "Z": 2}
+
+
+class DeckError(ValueError):
+    """The deck uses something the device path does not run, or is malformed."""
+
+
+def _xml(path: Path):
+    if not path.exists():
+        raise DeckError(f"missing {path}")
+    return ET.parse(path).getroot()
+
+
+def _child(node, tag, where):
+    c = node.find(tag)
+    if c is None:
+        raise DeckError(f"{where}: missing <{tag}>")
+    return c
+
+
+def _f(node, attr, where):
+    v = node.get(attr)
+    if v is None:
+        raise DeckError(f"{where}: <{node.tag}> lacks {attr}")
+    return float(v)
+
+
+def _true(v):
+    return v is not None and v.strip().lower() == "true"
+
+
+def _eos(path: Path):
+    """libEOS file: EosIG / EosSG parameters (Eos*::assignParametersFromXML)."""
+    root = _xml(path)
+    kind = _child(root, "EOS", path.name).get("type", "").upper()
+    par = _child(root, "parameters", path.name)
+    if kind == "IG":
+        return {"type": "IG", "gamma": _f(par, "gamma", path.name), "pinf": 0.0, "cv": _f(par, "cv", path.name),
+                "eref": float(par.get("energyRef", 0.0))}
+    if kind == "SG":
+        return {"type": "SG", "gamma": _f(par, "gamma", path.name), "pinf": _f(par, "pInf", path.name),
+                "cv": _f(par, "cv", path.name), "eref": float(par.get("energyRef", 0.0))}
+    raise DeckError(f"{path.name}: EOS type {kind!r} is not IG or SG")
+
+
+def _limiters(so):
+    """Input.cpp:185-249: interface defaults to global, global volume fraction
+    to global, interface volume fraction to interface."""
+    def text(tag):
+        e = so.find(tag)
+        if e is None:
+            return None
+        if e.text is None:
+            raise DeckError(f"secondOrder: empty <{tag}>")
+        return e.text.strip().upper()
+    g = text("globalLimiter")
+    if g is None:
+        raise DeckError("secondOrder: missing <globalLimiter>")
+    i = text("interfaceLimiter") or g
+    gv = text("globalVolumeFractionLimiter") or g
+    iv = text("interfaceVolumeFractionLimiter") or i
+    for name, v in (("globalLimiter", g), ("interfaceLimiter", i)):
+        if v == "THINC":
+            raise DeckError(f"{name}: THINC can only be a volume-fraction limiter")
+    out = []
+    for v in (g, i, gv, iv):
+        if v not in LIMITERS:
+            raise DeckError(f"secondOrder: unknown limiter {v!r}")
+        out.append(LIMITERS[v])
+    return out
+
+
+def _centres(cells, lengths):
+    """Cell centres (i + 0.5) * L / n per axis (MeshCartesian)."""
+    return [(np.arange(cells[d]) + 0.5) * (lengths[d] / cells[d]) for d in range(3)]
+
+
+def _domain_mask(dom, X, Y, Z, where):
+    """GeometricalDomain::belong of one domain on the cell-centre grids."""
+    kind = dom.get("type", "")
+    P = (X, Y, Z)
+    if kind == "entireDomain":
+        return np.ones(np.broadcast(X, Y, Z).shape, dtype=bool)
+    if kind == "halfSpace":  # GDHalfSpace.cpp:74-96
+        d = _child(dom, "dataHalfSpace", where)
+        ax = AXES[d.get("axe", "").upper()]
+        org = _f(d, "origin", where)
+        direction = d.get("direction", "").lower()
+        if direction not in ("positive", "negative"):
+            raise DeckError(f"{where}: halfSpace direction {direction!r}")
+        p = P[ax]
+        m = (p >= org) if direction == "positive" else (p <= org)
+        return np.broadcast_to(m, np.broadcast(X, Y, Z).shape)
+    if kind == "disc":  # GDDisc.cpp:85-107
+        d = _child(dom, "dataDisc", where)
+        c = _child(d, "center", where)
+        cen = [float(c.get(a, 0.0)) for a in "xyz"]
+        r = _f(d, "radius", where)
+        s = 0.0
+        for a in (d.get("axe1", ""), d.get("axe2", "")):
+            k = AXES[a.upper()]
+            s = s + (P[k] - cen[k]) ** 2
+        return np.broadcast_to(s <= r * r, np.broadcast(X, Y, Z).shape)
+    if kind == "sphere":  # GDSphere.cpp:71-79
+        d = _child(dom, "dataSphere", where)
+        c = _child(d, "center", where)
+        cen = [float(c.get(a, 0.0)) for a in "xyz"]
+        r = _f(d, "radius", where)
+        s = (X - cen[0]) ** 2 + (Y - cen[1]) ** 2 + (Z - cen[2]) ** 2
+        return s <= r * r
+    if kind == "rectangle":  # GDRectangle.cpp:88-120
+        d = _child(dom, "dataRectangle", where)
+        lb = _child(d, "posInferiorVertex", where)
+        lo = [float(lb.get(a, 0.0)) for a in "xyz"]
+        m = np.ones(np.broadcast(X, Y, Z).shape, dtype=bool)
+        for a, la in ((d.get("axe1", ""), "lAxe1"), (d.get("axe2", ""), "lAxe2")):
+            k = AXES[a.upper()]
+            off = P[k] - lo[k]
+            m = m & (off >= 0) & (off <= _f(d, la, where))
+        return m
+    raise DeckError(f"{where}: geometric domain type {kind!r} is not read")
+
+
+def _state_values(st, model, nph, where):
+    """(alpha_k, rho_k, p_k per phase, u, v, w) of one <state>."""
+    mats = st.findall("material")
+    if len(mats) != nph:
+        raise DeckError(f"{where}: {len(mats)} materials for {nph} phases")
+    if model == "euler":
+        fl = _child(mats[0], "dataFluid", where)
+        v = _child(fl, "velocity", where)
+        return [1.0, _f(fl, "density", where), _f(fl, "pressure", where)] + [float(v.get(a, 0.0)) for a in "xyz"]
+    mix = _child(st, "mixture", where)
+    vel = _child(mix, "velocity", where)
+    out = []
+    pm = _f(_child(mix, "dataMix", where), "pressure", where) if model == "kapila" else None
+    for m in mats:
+        fl = _child(m, "dataFluid", where)
+        p = pm if model == "kapila" else _f(fl, "pressure", where)
+        out += [_f(fl, "alpha", where), _f(fl, "density", where), p]
+    return out + [float(vel.get(a, 0.0)) for a in "xyz"]
+
+
+def load(rundir):
+    """(case, state) of a reference run directory.  ``state`` is the t = 0
+    field-major array of the ecogen.h host layout without E."""
+    rundir = Path(rundir)
+    top = _xml(rundir / "ECOGEN.xml")
+    tc = top.find("testCase")
+    if tc is None or not (tc.text or "").strip():
+        raise DeckError("ECOGEN.xml: missing <testCase>")
+    cdir = (rundir / tc.text.strip()).resolve()
+    main = _xml(cdir / "mainV5.xml")
+    mesh = _xml(cdir / "meshV5.xml")
+    model_x = _xml(cdir / "modelV4.xml")
+    ci = _xml(cdir / "initialConditionsV4.xml")
+
+    case = {"name": (main.findtext("run") or "deck").strip()}
+    # run control (Input::entreeMain)
+    tcm = _child(main, "timeControlMode", "mainV5.xml")
+    if _true(tcm.get("iterations")):
+        case["time_mode"] = "iterations"
+        case["steps"] = int(_f(_child(tcm, "iterations", "mainV5.xml"), "number", "mainV5.xml"))
+    else:
+        case["time_mode"] = "time"
+        case["t_final"] = _f(_child(tcm, "physicalTime", "mainV5.xml"), "totalTime", "mainV5.xml")
+    case["cfl"] = _f(_child(main, "computationControl", "mainV5.xml"), "CFL", "mainV5.xml")
+    so = main.find("secondOrder")
+    if so is None:
+        case["order"], case["limiters"] = 1, ["none"] * 4
+    else:
+        case["order"], case["limiters"] = 2, _limiters(so)
+
+    # mesh (Input::entreeMesh)
+    st_ = _child(mesh, "type", "meshV5.xml").get("structure", "").lower()
+    if st_ != "cartesian":
+        raise DeckError(f"meshV5.xml: mesh structure {st_!r} (only cartesian)")
+    cm = _child(mesh, "cartesianMesh", "meshV5.xml")
+    amr = cm.find("AMR")
+    if amr is not None and int(float(amr.get("lvlMax", "0"))) > 0:
+        raise DeckError("meshV5.xml: AMR stays on the reference")
+    dims = _child(cm, "dimensions", "meshV5.xml")
+    ncel = _child(cm, "numberCells", "meshV5.xml")
+    case["lengths"] = [_f(dims, a, "meshV5.xml") for a in "xyz"]
+    case["cells"] = [int(_f(ncel, a, "meshV5.xml")) for a in "xyz"]
+
+    # model and EOS (Input::entreeModel / entreeEOS)
+    fm = _child(model_x, "flowModel", "modelV4.xml")
+    name = fm.get("name", "").upper()
+    if name not in MODELS:
+        raise DeckError(f"modelV4.xml: flow model {fm.get('name')!r} is not Euler, Kapila or MultiP")
+    model = MODELS[name]
+    case["model"] = model
+    case["alpha_null"] = _true(fm.get("alphaNull")) if model == "kapila" else False
+    eos_files = [e.get("name", "") for e in model_x.findall("EOS")]
+    nph = 1 if model == "euler" else int(_f(fm, "numberPhases", "modelV4.xml"))
+    if len(eos_files) != nph:
+        raise DeckError(f"modelV4.xml: {len(eos_files)} EOS for {nph} phases")
+    case["eos"] = [dict(_eos(rundir / "libEOS" / f), name=Path(f).stem) for f in eos_files]
+    relax = [r.get("type", "").upper() for r in model_x.findall("relaxation")]
+    if any(r != "P" for r in relax):
+        raise DeckError(f"modelV4.xml: relaxation {relax} (only P)")
+    case["relax_p"] = model == "multip" and "P" in relax
+
+    # initial conditions (Input::entreeConditionsInitiales)
+    states = {s.get("name"): s for s in ci.findall("state")}
+    bcs = ["abs"] * 6
+    bnode = ci.find("boundaryConditions")
+    for b in (bnode.findall("boundCond") if bnode is not None else []):
+        kind = b.get("type", "").lower()
+        num = int(float(b.get("number", "0")))
+        if not 1 <= num <= 6:
+            raise DeckError(f"boundCond number {num}")
+        if kind == "abs":
+            bcs[num - 1] = "abs"
+        elif kind in ("wall", "symmetry"):
+            if case["order"] != 1:
+                raise DeckError(f"boundCond {kind} at second order (BoundCondWallO2 / SymmetryO2) is not built")
+            bcs[num - 1] = "wall"
+        else:
+            raise DeckError(f"boundCond type {kind!r} is not read")
+    case["bc"] = bcs
+
+    X, Y, Z = _centres(case["cells"], case["lengths"])
+    Xg, Yg, Zg = X[None, None, :], Y[None, :, None], Z[:, None, None]
+    nf = 3 * nph + 3 if model != "euler" else 6
+    grid = np.zeros((nf, case["cells"][2], case["cells"][1], case["cells"][0]))
+    filled = np.zeros(grid.shape[1:], dtype=bool)
+    pdoms = ci.find("physicalDomains")
+    for k, dom in enumerate(pdoms.findall("domain") if pdoms is not None else []):
+        where = f"domain {dom.get('name', k)}"
+        sname = dom.get("state")
+        if sname not in states:
+            raise DeckError(f"{where}: unknown state {sname!r}")
+        vals = _state_values(states[sname], model, nph, where)
+        m = _domain_mask(dom, Xg, Yg, Zg, where)
+        for f, v in enumerate(vals):
+            grid[f][m] = v
+        filled |= m
+    if not filled.all():
+        raise DeckError("initialConditionsV4.xml: the domains leave cells without a state")
+    return case, grid.reshape(nf, -1)
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("rundir")
+    ap.add_argument("--out", default="final.npz")
+    ap.add_argument("--precision", choices=["fp64", "fp32s"], default="fp64")
+    args = ap.parse_args(argv)
+    from .capi import Context
+    case, state = load(args.rundir)
+    case["precision"] = args.precision
+    with Context(case) as ctx:
+        ctx.set_state(state)
+        if case["time_mode"] == "iterations":
+            st = ctx.run(nsteps=case["steps"])
+        else:
+            st = ctx.run(t_final=case["t_final"])
+        out = ctx.get_state()
+    np.savez(args.out, state=out, steps=st.steps, t=st.t, case_json=json.dumps(case))
+    print(f"{case['name']}: {st.steps} steps, t = {st.t!r} -> {args.out}")
+
+
+if __name__ == "__main__":
+    main()
