@@ -23,9 +23,11 @@ description of :class:`ecogen_amd.Context` and the initial state in the
 
 Anything else raises ``DeckError`` naming the element.
 
-Command line (runs the deck on the GPU and saves the final state)::
+Command line (runs the deck on the GPU and saves the final state; ``--vtk``
+also writes the reference's VTK result files at the deck's output frequency,
+``ecogen_amd/output.py``)::
 
-    python -m ecogen_amd.deck RUNDIR [--out final.npz] [--precision fp64|fp32s]
+    python -m ecogen_amd.deck RUNDIR [--out final.npz] [--precision fp64|fp32s] [--vtk]
 """
 from __future__ import annotations
 
@@ -206,6 +208,15 @@ def load(rundir):
         case["time_mode"] = "time"
         case["t_final"] = _f(_child(tcm, "physicalTime", "mainV5.xml"), "totalTime", "mainV5.xml")
     case["cfl"] = _f(_child(main, "computationControl", "mainV5.xml"), "CFL", "mainV5.xml")
+    # output cadence and format (Input::entreeMain; OutputXML)
+    it_x, pt_x = tcm.find("iterations"), tcm.find("physicalTime")
+    om = main.find("outputMode")
+    case["output"] = {
+        "iter_freq": int(float(it_x.get("iterFreq", "0"))) if it_x is not None else 0,
+        "time_freq": float(pt_x.get("timeFreq", "0")) if pt_x is not None else 0.0,
+        "binary": _true(om.get("binary")) if om is not None else False,
+        "precision": int(float(om.get("precision", "0"))) if om is not None else 0,
+    }
     so = main.find("secondOrder")
     if so is None:
         case["order"], case["limiters"] = 1, ["none"] * 4
@@ -237,7 +248,7 @@ def load(rundir):
     nph = 1 if model == "euler" else int(_f(fm, "numberPhases", "modelV4.xml"))
     if len(eos_files) != nph:
         raise DeckError(f"modelV4.xml: {len(eos_files)} EOS for {nph} phases")
-    case["eos"] = [dict(_eos(rundir / "libEOS" / f), name=Path(f).stem) for f in eos_files]
+    case["eos"] = [dict(_eos(rundir / "libEOS" / f), name=Path(f).stem, file=f) for f in eos_files]
     relax = [r.get("type", "").upper() for r in model_x.findall("relaxation")]
     if any(r != "P" for r in relax):
         raise DeckError(f"modelV4.xml: relaxation {relax} (only P)")
@@ -283,18 +294,61 @@ def load(rundir):
     return case, grid.reshape(nf, -1)
 
 
+def _run_with_output(ctx, case, outdir):
+    """March to the deck's end, writing a result file at t = 0 and after every
+    iterFreq steps (iteration control) or at the first step reaching each
+    multiple of timeFreq (physical-time control), as the reference does, plus
+    collection.pvd."""
+    from . import output
+    o = case["output"]
+    outdir.mkdir(parents=True, exist_ok=True)
+    prec = o["precision"] or 6  # the stream's default precision when the deck sets none
+    files = []
+
+    def dump():
+        f = outdir / f"result_CPU0_AMR0_TIME{len(files)}.vtr"
+        output.write_vtr(f, case, ctx.get_state(), binary=o["binary"], precision=prec)
+        files.append(f)
+    dump()
+    if case["time_mode"] == "iterations":
+        step = o["iter_freq"] if o["iter_freq"] > 0 else case["steps"]
+        done = 0
+        while done < case["steps"]:
+            n = min(step, case["steps"] - done)
+            st = ctx.run(nsteps=n)
+            done += n
+            dump()
+    else:
+        tf = case["t_final"]
+        freq = o["time_freq"] if o["time_freq"] > 0 else tf
+        k = 1
+        while True:
+            target = min(k * freq, tf)
+            st = ctx.run(t_final=target)
+            dump()
+            if st.t >= float(np.float32(tf)) or target >= tf:
+                break
+            k += 1
+    output.write_collection(outdir / "collection.pvd", files)
+    return st
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("rundir")
     ap.add_argument("--out", default="final.npz")
     ap.add_argument("--precision", choices=["fp64", "fp32s"], default="fp64")
+    ap.add_argument("--vtk", action="store_true",
+                    help="write VTK result files at the deck's output frequency under RUNDIR/results/<run>/")
     args = ap.parse_args(argv)
     from .capi import Context
     case, state = load(args.rundir)
     case["precision"] = args.precision
     with Context(case) as ctx:
         ctx.set_state(state)
-        if case["time_mode"] == "iterations":
+        if args.vtk:
+            st = _run_with_output(ctx, case, Path(args.rundir) / "results" / case["name"])
+        elif case["time_mode"] == "iterations":
             st = ctx.run(nsteps=case["steps"])
         else:
             st = ctx.run(t_final=case["t_final"])

@@ -1,0 +1,188 @@
+"""Results in the reference's VTK XML format, so that device results load in
+the same post-processing as the reference's own.
+
+``write_vtr`` writes one RectilinearGrid file of a Cartesian state as
+``OutputXML::ecritSolutionXML`` does (``src/InputOutput/OutputXML.cpp:496-560``
+for the mesh, ``:353-480`` for the cell data): node coordinates, then per phase
+the scalars of ``Phase*::returnNameScalar`` named ``F<k>_<name>_<EOS file>``,
+the mixture scalars and velocity (multiphase models), and ``gradRho``
+(``Cell::getGradient``, ``src/Cell.cpp:525-682, 880-887``).  Arrays are typed
+Float32 as in the reference; ASCII output prints the double values with the
+deck's precision, binary output is base64 (a 32-bit byte count, then the
+little-endian Float32 data, ``Output::ecritJeuDonnees``, ``Output.cpp:219-266``).
+``write_collection`` writes the ``collection.pvd`` time series.
+
+Every array is computed from the stored state.  With MultiP pressure
+relaxation the reference prints instead the phase temperatures and the
+mixture pressure it cached before its last relaxation (the relaxation does
+not refresh them); there the values differ by what the relaxation moved.
+"""
+from __future__ import annotations
+
+import base64
+import struct
+from pathlib import Path
+
+import numpy as np
+
+
+def _eos_file(e):
+    return e.get("file") or f"{e['type']}_{e['name']}.xml"
+
+
+def _temperature(e, rho, p, eps):
+    """Eos{IG,SG}::computeTemperature (EosIG.cpp:90-93, EosSG.cpp:93-96)."""
+    r = np.maximum(rho, eps)
+    if e["type"] == "SG":
+        return (p + e["pinf"]) / (e["gamma"] - 1.0) / r / e["cv"]
+    return p / (e["gamma"] - 1.0) / r / e["cv"]
+
+
+def _positions(case):
+    """Cell centres (i + 0.5) dX and widths dX per axis (MeshCartesian)."""
+    n, L = case["cells"], case["lengths"]
+    d = [L[a] / n[a] for a in range(3)]
+    return [(np.arange(n[a]) + 0.5) * d[a] for a in range(3)], d
+
+
+def grad_rho(case, rho):
+    """|grad rho| of Cell::getGradient on a Cartesian grid: per interior face
+    (cd - cg)/|xR - xL| times the normal, weighted by the centre distance along
+    each axis; every boundary face (any type, for the density) adds twice the
+    centre-to-face distance to the weights only."""
+    n = case["cells"]
+    dim = sum(1 for c in n if c > 1)
+    pos, dx = _positions(case)
+    r = rho.reshape(n[2], n[1], n[0])
+    g = [np.zeros_like(r) for _ in range(3)]
+    w = [np.zeros_like(r) for _ in range(3)]
+    axis_of = {0: 2, 1: 1, 2: 0}  # array axis of each direction in [k, j, i]
+    for d in range(dim):
+        ax = axis_of[d]
+        p = pos[d]
+        dist = np.abs(p[:-1] - p[1:])        # |xL - xR| of each interior face (Element::distance)
+        shape = [1, 1, 1]
+        shape[ax] = -1
+        dist = dist.reshape(shape)
+        lo = [slice(None)] * 3
+        hi = [slice(None)] * 3
+        lo[ax] = slice(0, -1)
+        hi[ax] = slice(1, None)
+        lo, hi = tuple(lo), tuple(hi)
+        face = ((r[hi] - r[lo]) / dist) * 1.0 * dist   # normal component 1, times |distance along d|
+        # accumulation order of a cell's faces: the minus face, then the plus face
+        gm = np.zeros_like(r)
+        gp = np.zeros_like(r)
+        gp[lo] = face                      # the plus face of the left cell
+        gm[hi] = face                      # the minus face of the right cell
+        g[d] = (g[d] + gm) + gp
+        wm = np.zeros_like(r)
+        wp = np.zeros_like(r)
+        wp[lo] = np.broadcast_to(dist, face.shape)
+        wm[hi] = np.broadcast_to(dist, face.shape)
+        w[d] = (w[d] + wm) + wp
+    # boundary faces after the interior ones (X-, X+, Y-, ...): twice the
+    # centre-to-face distance, the X+ face lying at x_{n-1} + dX/2
+    for d in range(dim):
+        ax = axis_of[d]
+        p = pos[d]
+        first = [slice(None)] * 3
+        last = [slice(None)] * 3
+        first[ax] = slice(0, 1)
+        last[ax] = slice(-1, None)
+        w[d][tuple(first)] += np.abs(p[0] - 0.0) * 2.0
+        w[d][tuple(last)] += np.abs(p[-1] - (p[-1] + 0.5 * dx[d])) * 2.0
+    comp = []
+    for d in range(3):
+        ww = np.where(w[d] <= 1e-12, 1.0, w[d])
+        comp.append(g[d] / ww)
+    return np.sqrt(comp[0] * comp[0] + comp[1] * comp[1] + comp[2] * comp[2]).reshape(-1)
+
+
+def cell_data(case, state):
+    """[(name, components, values)] of the reference's output for ``state``
+    (ecogen.h host layout: per phase alpha, rho, p; u, v, w; E optional)."""
+    model = case["model"]
+    eos = case["eos"]
+    eps = 1e-15 if (model == "kapila" and case.get("alpha_null")) else 0.0
+    out = []
+    if model == "euler":
+        e = eos[0]
+        rho, p = state[1], state[2]
+        vel = np.stack([state[3], state[4], state[5]], axis=1)
+        f = _eos_file(e)
+        out += [(f"F0_Density_{f}", 1, rho), (f"F0_Pressure_{f}", 1, p),
+                (f"F0_Temperature_{f}", 1, _temperature(e, rho, p, eps)), (f"F0_Velocity_{f}", 3, vel.reshape(-1))]
+        out.append(("gradRho", 1, grad_rho(case, rho)))
+        return out
+    nph = len(eos)
+    rho_m = np.zeros(state.shape[1])
+    p_m = np.zeros(state.shape[1])
+    for k in range(nph):
+        rho_m = rho_m + state[3 * k] * state[3 * k + 1]
+        p_m = p_m + state[3 * k] * state[3 * k + 2]
+    for k, e in enumerate(eos):
+        a, r, p = state[3 * k], state[3 * k + 1], state[3 * k + 2]
+        f = _eos_file(e)
+        T = _temperature(e, r, p, eps)
+        if model == "kapila":
+            Y = a * r / np.maximum(rho_m, eps)
+            # PhaseKapila::returnScalar (PhaseKapila.cpp:115-138) clamps for output
+            out += [(f"F{k}_Alpha_{f}", 1, np.where(a < 1e-20, 0.0, a)),
+                    (f"F{k}_Density_{f}", 1, np.where(r < 1e-20, 1e-20, r)),
+                    (f"F{k}_Temperature_{f}", 1, np.where(T < 1e-20, 1e-20, T)),
+                    (f"F{k}_Mass fraction_{f}", 1, np.where(Y < 1e-20, 0.0, Y))]
+        else:
+            Y = a * r / rho_m
+            out += [(f"F{k}_Alpha_{f}", 1, a), (f"F{k}_Density_{f}", 1, r), (f"F{k}_Temperature_{f}", 1, T),
+                    (f"F{k}_Mass fraction_{f}", 1, Y), (f"F{k}_Pressure_{f}", 1, p)]
+    vel = np.stack([state[3 * nph], state[3 * nph + 1], state[3 * nph + 2]], axis=1)
+    out += [("Density_Mixture", 1, rho_m), ("Pressure_Mixture", 1, p_m), ("Velocity_Mixture", 3, vel.reshape(-1))]
+    out.append(("gradRho", 1, grad_rho(case, rho_m)))
+    return out
+
+
+def _ascii(values, precision):
+    return "".join(f"{v:.{precision}g} " for v in np.asarray(values, dtype=np.float64))
+
+
+def _binary(values):
+    data = np.asarray(values, dtype="<f4").tobytes()
+    return (base64.b64encode(struct.pack("<i", len(data))) + base64.b64encode(data)).decode()
+
+
+def write_vtr(path, case, state, binary=False, precision=17):
+    """One result file of ``state`` (host layout) in the reference's format."""
+    n = case["cells"]
+    pos, dx = _positions(case)
+    fmt = "binary" if binary else "ascii"
+
+    def enc(v):
+        return _binary(v) if binary else _ascii(v, precision)
+    lines = ['<?xml version="1.0"?>',
+             '<VTKFile type="RectilinearGrid" version="0.1" byte_order="LittleEndian">',
+             f'  <RectilinearGrid WholeExtent="0 {n[0]} 0 {n[1]} 0 {n[2]}">',
+             f'    <Piece Extent="0 {n[0]} 0 {n[1]} 0 {n[2]}">',
+             "      <Coordinates>"]
+    for a in range(3):
+        # MeshCartesian::recupereCoord: x_i - dX/2 for every cell, then x_{n-1} + dX/2
+        nodes = np.append(pos[a] - 0.5 * dx[a], pos[a][-1] + 0.5 * dx[a])
+        lead = "\n          " if not binary else "\n"
+        lines.append(f'        <DataArray type="Float32" format="{fmt}">{lead}{enc(nodes)}')
+        lines.append("        </DataArray>")
+    lines += ["      </Coordinates>", "      <CellData>"]
+    for name, nc, vals in cell_data(case, np.asarray(state, dtype=np.float64)):
+        comp = f' NumberOfComponents="{nc}"' if nc > 1 else ""
+        lines.append(f'        <DataArray type="Float32" Name="{name}"{comp} format="{fmt}">\n{enc(vals)}')
+        lines.append("        </DataArray>")
+    lines += ["      </CellData>", "    </Piece>", "  </RectilinearGrid>", "</VTKFile>", ""]
+    Path(path).write_text("\n".join(lines))
+
+
+def write_collection(path, files):
+    """collection.pvd: one DataSet per written file, timestep = its index
+    (OutputXML::ecritCollectionXML, OutputXML.cpp:323-350)."""
+    rows = "".join(f'        <DataSet timestep="{t}" part="0" file="{Path(f).name}"/>\n' for t, f in enumerate(files))
+    Path(path).write_text('<?xml version="1.0"?>\n<VTKFile type="Collection" version="0.1" byte_order="LittleEndian" '
+                          'compressor="vtkZLibDataCompressor">\n    <Collection>\n' + rows +
+                          "    </Collection>\n</VTKFile>\n")

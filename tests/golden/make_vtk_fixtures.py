@@ -1,0 +1,58 @@
+"""Reference VTK output files for the output-writer tests.
+
+Runs the reference executable (``oracle/_ref/ECOGEN``, built by
+``oracle/ref.mk``) on small decks and keeps its last result file (ASCII,
+17 digits), its ``collection.pvd`` and the case description, under
+``tests/golden/vtk/<case>/``.  The files are the reference's outputs: data.
+
+    python tests/golden/make_vtk_fixtures.py
+"""
+from __future__ import annotations
+
+import json
+import shutil
+import subprocess
+import sys
+import tempfile
+from pathlib import Path
+
+HERE = Path(__file__).resolve().parent
+REPO = HERE.parent.parent
+sys.path.insert(0, str(HERE))
+sys.path.insert(0, str(REPO / "oracle"))
+from cases import CASES, _bubbles_3d, _euler_1d, _shock_tube_1d  # noqa: E402
+from deck import write_deck  # noqa: E402
+
+REF_BIN = REPO / "oracle" / "_ref" / "ECOGEN"
+
+FIXTURES = {
+    "kapila_2d_fo_wall_6x4": dict(CASES["kapila_2d_fo_wall"], cells=[6, 4, 1], steps=3),
+    "kapila_1d_o2_alphanull_30": _shock_tube_1d("kapila", 2, ["vanleer"] * 4, 12, cells=30, alpha_null=True),
+    "multip_1d_o2_relax_40": _shock_tube_1d("multip", 2, ["minmod"] * 4, 15, cells=40, relax=True),
+    "euler_1d_fo_30": _euler_1d(1, ["none"] * 4, 20, cells=30),
+    "kapila_3d_o2_6": _bubbles_3d(n=6, steps=3, nb=1, model="kapila", relax=False, limiters=("vanleer",) * 4),
+}
+
+
+def main():
+    for name, case in FIXTURES.items():
+        tmp = Path(tempfile.mkdtemp(prefix="ecogen_vtk_"))
+        try:
+            write_deck(dict(case, name=name), tmp)
+            r = subprocess.run([str(REF_BIN)], cwd=tmp, capture_output=True, text=True, timeout=600)
+            if r.returncode != 0:
+                raise RuntimeError(r.stdout[-2000:] + r.stderr[-2000:])
+            rdir = tmp / "results" / name
+            vtrs = sorted(rdir.glob("result_CPU0_AMR0_TIME*.vtr"), key=lambda p: int(p.stem.rsplit("TIME", 1)[1]))
+            out = HERE / "vtk" / name
+            out.mkdir(parents=True, exist_ok=True)
+            shutil.copy(vtrs[-1], out / "final.vtr")
+            shutil.copy(rdir / "collection.pvd", out / "collection.pvd")
+            (out / "case.json").write_text(json.dumps(case))
+            print(f"{name}: {vtrs[-1].name} -> {out}")
+        finally:
+            shutil.rmtree(tmp, ignore_errors=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -70,3 +70,31 @@ def test_deck_runs_on_the_device(name, tmp_path):
         else:
             e = rel_l1(got[nm], want[nm])
         assert e <= 1e-12, f"{nm}: {e:.3e}"
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["kapila_2d_fo_wall_6x4", "euler_1d_fo_30"])
+def test_deck_vtk_results_match_reference_files(name, tmp_path):
+    """python -m ecogen_amd.deck --vtk writes the reference's result files:
+    the collection file character for character, and the last result file
+    with the reference's arrays and header, values within 1e-10 relative of
+    the reference's own file (device and reference differ in last bits)."""
+    import json
+    from pathlib import Path
+    from ecogen_amd import deck
+    from test_output import parse
+    d = Path(__file__).resolve().parent / "golden" / "vtk" / name
+    case = json.loads((d / "case.json").read_text())
+    print(f"case {name} deck --vtk", flush=True)
+    write_deck(dict(case, name=name, output='format="XML" binary="false" precision="17"'), tmp_path / "run")
+    deck.main([str(tmp_path / "run"), "--out", str(tmp_path / "f.npz"), "--vtk"])
+    res = tmp_path / "run" / "results" / name
+    assert (res / "collection.pvd").read_text() == (d / "collection.pvd").read_text()
+    ours_text = (res / "result_CPU0_AMR0_TIME1.vtr").read_text()
+    ref_text = (d / "final.vtr").read_text()
+    assert ours_text[: ours_text.index("<CellData>")] == ref_text[: ref_text.index("<CellData>")]
+    ours, ref = parse(ours_text), parse(ref_text)
+    assert [(n, c) for n, c, _ in ours] == [(n, c) for n, c, _ in ref]
+    for (n, _, a), (_, _, b) in zip(ours, ref):
+        scale = max(np.abs(b).max(), 1e-300)
+        assert np.abs(a - b).max() <= 1e-10 * scale, n
