@@ -17,7 +17,7 @@ MODEL = {"euler": 0, "kapila": 1, "multip": 2}
 LIMITER = {"none": 0, "minmod": 1, "vanleer": 2, "vanalbada": 3, "superbee": 4, "mc": 5, "thinc": 6}
 EOS_TYPE = {"IG": 0, "SG": 1}
 PRECISION = {"fp64": 0, "fp32s": 1}
-BC = {"abs": 0, "wall": 1}  # per side X-, X+, Y-, Y+, Z-, Z+; "wall" at first order only
+BC = {"abs": 0, "wall": 1}  # per side X-, X+, Y-, Y+, Z-, Z+; "wall": FP64 only
 
 WARN_RELAX_NOT_CONVERGED = 1
 WARN_NAN = 2

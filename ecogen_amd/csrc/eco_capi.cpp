@@ -87,8 +87,8 @@ int eco::validate_config(const eco_config *c) {
   }
   for (int i = 0; i < 6; i++) {
     if (c->bc[i] != ECO_BC_ABS && c->bc[i] != ECO_BC_WALL) return ECO_ERR_INVALID;
-    // a second-order WALL deck is BoundCondWallO2 (mirrored velocity slopes), not built
-    if (c->bc[i] == ECO_BC_WALL && c->order != 1) return ECO_ERR_UNSUPPORTED;
+    // walls: FP64 path (the FP32-storage kernels keep transmissive boundaries)
+    if (c->bc[i] == ECO_BC_WALL && c->precision == ECO_FP32S) return ECO_ERR_UNSUPPORTED;
   }
   if (c->model == ECO_MODEL_EULER) {
     if (c->nphase != 1) return ECO_ERR_INVALID;
@@ -148,6 +148,10 @@ void eco::lower_config(const eco_config &c, Lowered &L) {
       L.tab[o + 2 * n + i] = std::sqrt(b * b);           // face - right cell
     }
     o += 3 * (size_t)n;
+    // boundary faces: the minus face at 0, the plus face at x_{n-1} + dX/2
+    const double x0 = 0.5 * dd[d], xl = (n - 1 + 0.5) * dd[d];
+    G.dwall[d][0] = x0 - 0.;
+    G.dwall[d][1] = xl - (xl + 0.5 * dd[d]);
   }
   // scheme
   SchemeParams &S = L.sch;

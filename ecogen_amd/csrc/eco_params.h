@@ -39,6 +39,7 @@ struct Geometry {
   const double *dcc[3];
   const double *dfl[3];
   const double *dfr[3];
+  double dwall[3][2];  // signed centre-to-face distance (cell - face) of the boundary cells, minus / plus side
 };
 
 struct SchemeParams {

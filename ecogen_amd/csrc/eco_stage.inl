@@ -425,7 +425,8 @@ __device__ __forceinline__ void accumulate_dir(FluxVec<NPH> &acc, const double *
 template <int MODEL, int NPH, int DIM, int LV, int LA, bool RECIP, typename TW>
 __device__ __forceinline__ void muscl_vars(const SchemeParams &S, const TW *wl_, const TW *wr_, const TW *wll,
                                            const TW *whh, bool has_m, bool has_p, double dcc, double dcm, double dcp,
-                                           double dL, double dR, bool ifl, bool ifr, double *xl, double *xr) {
+                                           double dL, double dR, bool ifl, bool ifr, double *xl, double *xr,
+                                           int iuw = -1, double swm = 0., double swp = 0.) {
   using Ly = Layout<MODEL, NPH, DIM>;
   if constexpr (RECIP) {
     // FP32-storage form: differences of the (decoded) cell values in FP64,
@@ -460,8 +461,10 @@ __device__ __forceinline__ void muscl_vars(const SchemeParams &S, const TW *wl_,
     for (int v = 0; v < Ly::NVAR; v++) {
       const double wl = wl_[v], wr = wr_[v];
       const double sc = qdiv(wr - wl, dcc, ydc);
-      const double sm = has_m ? qdiv(wl - (double)wll[v], dcm, ydm) : 0.;
-      const double sp = has_p ? qdiv((double)whh[v] - wr, dcp, ydp) : 0.;
+      // a missing neighbour gives slope 0, or behind a second-order wall the
+      // wall face's slope for the normal velocity (variable iuw)
+      const double sm = has_m ? qdiv(wl - (double)wll[v], dcm, ydm) : ((v == iuw) ? swm : 0.);
+      const double sp = has_p ? qdiv((double)whh[v] - wr, dcp, ydp) : ((v == iuw) ? swp : 0.);
       const bool isa = (MODEL != MODEL_EULER) && (v < 3 * NPH) && (v % 3 == 0);
       double gl, gr;
       if constexpr (LV >= 0) {
@@ -485,9 +488,10 @@ template <int MODEL, int NPH, int DIM, bool SAMELIM, bool RECIP, typename TW>
 __device__ __forceinline__ void muscl_dispatch(const SchemeParams &S, const TW *wl_, const TW *wr_, const TW *wll,
                                                const TW *whh, bool has_m, bool has_p, double dcc, double dcm,
                                                double dcp, double dL, double dR, bool ifl, bool ifr, double *xl,
-                                               double *xr) {
+                                               double *xr, int iuw = -1, double swm = 0., double swp = 0.) {
 #define ECO_MUSCL(LV, LA) \
-  muscl_vars<MODEL, NPH, DIM, LV, LA, RECIP>(S, wl_, wr_, wll, whh, has_m, has_p, dcc, dcm, dcp, dL, dR, ifl, ifr, xl, xr)
+  muscl_vars<MODEL, NPH, DIM, LV, LA, RECIP>(S, wl_, wr_, wll, whh, has_m, has_p, dcc, dcm, dcp, dL, dR, ifl, ifr, xl, xr, \
+                                             iuw, swm, swp)
   if constexpr (SAMELIM && Tile<DIM>::LS != 0) {
     const int lv = S.lim[0];
     const int la = (MODEL == MODEL_EULER) ? lv : ((S.lim[2] == LIM_THINC) ? LIM_MINMOD : S.lim[2]);
@@ -510,6 +514,46 @@ __device__ __forceinline__ void muscl_dispatch(const SchemeParams &S, const TW *
   }
   ECO_MUSCL(-1, -1);
 #undef ECO_MUSCL
+}
+
+// BoundCondWallO2::solveRiemann (BoundCondWallO2.cpp:115-150) before the
+// wall's Riemann problem: the boundary cell extrapolated to the wall, with
+// slopes limited between the wall's own slope (normal velocity over the
+// signed centre-to-wall distance, every other slope 0) and the internal face
+// across the cell (CellO2::computeLocalSlopesLimite, CellO2.cpp:250-354:
+// interface limiters when the cell's time-n alpha lies in [1e-5, 1-1e-5]).
+// No phase verification follows the extrapolation there.
+template <int MODEL, int NPH, int DIM, typename PW>
+__device__ __forceinline__ void wall_state_o2(const SchemeParams &S, const Geometry &G, PW Wsrc, PW Wn,
+                                              long long nc, long long cidx, long long sd, int dir, int side,
+                                              CellState<NPH> &L) {
+  using Ly = Layout<MODEL, NPH, DIM>;
+  const int iu = ((MODEL == MODEL_EULER) ? 2 : 3 * NPH) + dir;
+  CellState<NPH> N;
+  const long long in = side == 0 ? cidx + sd : cidx - sd;  // the cell across the internal face
+  load_prim<MODEL, NPH, DIM>(Wsrc, nc, in, N);
+  double w[Ly::NVAR], wn[Ly::NVAR];
+  prim_to_vec<MODEL, NPH, DIM>(L, w);
+  prim_to_vec<MODEL, NPH, DIM>(N, wn);
+  const int i = side == 0 ? 0 : G.n[dir] - 2;  // index of the internal face along dir
+  const double dcc = gmem(G.dcc[dir])[i];
+  bool iface = false;
+  if constexpr (MODEL != MODEL_EULER) {
+    const double a = Wn[cidx];
+    iface = (a >= 1.e-5) && (a <= 1. - 1.e-5);
+  }
+  const int lg = iface ? S.lim[1] : S.lim[0];
+  const int lv = iface ? S.lim[3] : S.lim[2];
+  const double dw = G.dwall[dir][side];
+  const double dg = side == 0 ? -fabs(dw) : fabs(dw);
+#pragma unroll
+  for (int v = 0; v < Ly::NVAR; v++) {
+    const double s1 = (v == iu) ? w[v] / dw : 0.;
+    const double s2 = side == 0 ? (wn[v] - w[v]) / dcc : (w[v] - wn[v]) / dcc;
+    const bool isa = (MODEL != MODEL_EULER) && (v < 3 * NPH) && (v % 3 == 0);
+    w[v] = w[v] + limiter(isa ? lv : lg, s1, s2) * dg;
+  }
+  vec_to_prim<MODEL, NPH, DIM>(w, L);
 }
 
 // One interior face of the tile: reconstruction (MUSCL) or cached cell states
@@ -561,14 +605,19 @@ __device__ __forceinline__ double face_fp64(const StageArgs &A, const int *o, co
       // computeLocalSlopes), then extrapolation to the face, one variable at a time.
       const bool has_m = (i >= 1), has_p = (i + 2 <= ndir - 1);
       const long long sd = strd[dir];
+      // a second-order wall behind a boundary cell (BoundCondWallO2) carries
+      // the slope u_n / (x_cell - x_wall) of the normal velocity and counts
+      // the cell itself as the cell across it (CellO2.cpp:121-205)
+      const bool wm = !has_m && ((S.bc_wall >> (2 * dir)) & 1);
+      const bool wp = !has_p && ((S.bc_wall >> (2 * dir + 1)) & 1);
       // interface detection: alpha of the face's cells from the stage state,
       // alpha across the opposite face from the time-n state (CellO2.cpp:144-150)
       double aL = 0., aR = 0., aLL = 0., aRR = 0.;
       if constexpr (MODEL != MODEL_EULER) {
         aL = L.a[0]; aR = R.a[0];
         if (!SAMELIM || S.thinc) {
-          aLL = has_m ? Wn[ilo - sd] : 0.;
-          aRR = has_p ? Wn[ihi + sd] : 0.;
+          aLL = has_m ? Wn[ilo - sd] : (wm ? Wn[ilo] : 0.);
+          aRR = has_p ? Wn[ihi + sd] : (wp ? Wn[ihi] : 0.);
         }
       }
       bool ifl = false, ifr = false;
@@ -593,9 +642,13 @@ __device__ __forceinline__ double face_fp64(const StageArgs &A, const int *o, co
         wll[v] = Fv[jm];
         whh[v] = Fv[jp];
       }
+      constexpr int IU = (MODEL == MODEL_EULER) ? 2 + dir : 3 * NPH + dir;  // normal velocity in the slope vector
+      double swm = 0., swp = 0.;
+      if (wm) swm = wl_[IU] / G.dwall[dir][0];
+      if (wp) swp = wr_[IU] / G.dwall[dir][1];
       double xl[Ly::NVAR], xr[Ly::NVAR];
       muscl_dispatch<MODEL, NPH, DIM, SAMELIM, false>(S, wl_, wr_, wll, whh, has_m, has_p, dcc, dcm, dcp, dL, dR, ifl,
-                                                      ifr, xl, xr);
+                                                      ifr, xl, xr, IU, swm, swp);
       vec_to_prim<MODEL, NPH, DIM>(xl, L);
       vec_to_prim<MODEL, NPH, DIM>(xr, R);
       if constexpr (MODEL == MODEL_EULER) {
@@ -745,10 +798,14 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int slot, c
         if (!on) continue;
         CellState<NPH> L;
         load_prim<MODEL, NPH, DIM>(Wsrc, nc, cidx, L);
+        const bool wall = (S.bc_wall >> (2 * dir + side)) & 1;
+        if constexpr (STAGE != 0) {
+          if (wall) wall_state_o2<MODEL, NPH, DIM>(S, G, Wsrc, Wn, nc, cidx, strd[dir], dir, side, L);
+        }
         to_face_frame(dir, side == 0, L.u);
         fulfill<MODEL, NPH>(S, L);
         FluxVec<NPH> F;
-        if ((S.bc_wall >> (2 * dir + side)) & 1) smx = smax(smx, wall_flux<MODEL, NPH>(L, F));
+        if (wall) smx = smax(smx, wall_flux<MODEL, NPH>(L, F));
         else smx = smax(smx, hllc<MODEL, NPH>(S, L, L, F));
         from_face_frame(dir, side == 0, F.q);
         add_face<MODEL, NPH>(acc, F, G.coefA[dir], false, own);

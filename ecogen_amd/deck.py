@@ -18,8 +18,9 @@ description of :class:`ecogen_amd.Context` and the initial state in the
 * the geometric domains entireDomain, halfSpace, disc, sphere and rectangle,
   applied in deck order (``Cell::fill``, ``src/Cell.cpp:130-137``; the
   ``belong`` tests of ``src/Geometries/GD*.cpp``);
-* ``abs`` boundaries, and ``wall`` / ``symmetry`` at first order (the
-  reference makes both a ``BoundCondWall`` there, ``Input.cpp:849-852``).
+* ``abs`` and ``wall`` boundaries (``BoundCondWall`` at first order,
+  ``BoundCondWallO2`` at second order), and ``symmetry`` at first order,
+  where the reference makes it a ``BoundCondWall`` (``Input.cpp:849-852``).
 
 Anything else raises ``DeckError`` naming the element.
 
@@ -265,10 +266,10 @@ def load(rundir):
             raise DeckError(f"boundCond number {num}")
         if kind == "abs":
             bcs[num - 1] = "abs"
-        elif kind in ("wall", "symmetry"):
-            if case["order"] != 1:
-                raise DeckError(f"boundCond {kind} at second order (BoundCondWallO2 / SymmetryO2) is not built")
+        elif kind == "wall" or (kind == "symmetry" and case["order"] == 1):
             bcs[num - 1] = "wall"
+        elif kind == "symmetry":
+            raise DeckError("boundCond symmetry at second order (BoundCondSymmetryO2) is not built")
         else:
             raise DeckError(f"boundCond type {kind!r} is not read")
     case["bc"] = bcs

@@ -32,9 +32,10 @@ enum eco_limiter { ECO_LIM_NONE = 0, ECO_LIM_MINMOD = 1, ECO_LIM_VANLEER = 2, EC
                    ECO_LIM_SUPERBEE = 4, ECO_LIM_MC = 5, ECO_LIM_THINC = 6 };
 enum eco_eos_type { ECO_EOS_IG = 0, ECO_EOS_SG = 1 };
 enum eco_precision { ECO_FP64 = 0, ECO_FP32S = 1 };
-/* Boundary conditions: transmissive (BoundCondAbs) and, at first order, wall
-   (BoundCondWall, which the reference also uses for a first-order SYMMETRY
-   boundary; src/InputOutput/Input.cpp:849-852). */
+/* Boundary conditions: transmissive (BoundCondAbs) and wall (BoundCondWall at
+   first order, which the reference also uses for a first-order SYMMETRY
+   boundary; BoundCondWallO2 at second order; src/InputOutput/Input.cpp:849-852).
+   Walls run in the FP64 path only. */
 enum eco_bc { ECO_BC_ABS = 0, ECO_BC_WALL = 1 };
 
 enum eco_status_code {
@@ -69,7 +70,7 @@ typedef struct {
   int n[3];         /* numberCells x, y, z */
   double L[3];      /* dimensions x, y, z (m) */
   double cfl;       /* computationControl CFL */
-  int bc[6];        /* X-, X+, Y-, Y+, Z-, Z+ : enum eco_bc (ECO_BC_WALL: order 1 only) */
+  int bc[6];        /* X-, X+, Y-, Y+, Z-, Z+ : enum eco_bc (ECO_BC_WALL: FP64 only) */
   eco_eos eos[4];   /* per phase */
 } eco_config;
 

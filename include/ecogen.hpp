@@ -164,7 +164,7 @@ struct CartesianMesh {
   int numberCells[3] = {1, 1, 1};
   double dimensions[3] = {1., 1., 1.};
   // boundary condition per side X-, X+, Y-, Y+, Z-, Z+ (initialConditions
-  // <boundCond type="abs|wall">; a wall needs first order)
+  // <boundCond type="abs|wall">; walls run in FP64)
   int boundCond[6] = {ECO_BC_ABS, ECO_BC_ABS, ECO_BC_ABS, ECO_BC_ABS, ECO_BC_ABS, ECO_BC_ABS};
 };
 

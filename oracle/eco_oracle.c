@@ -547,6 +547,9 @@ typedef struct {
   int opp_left;       /* face opposite to this one across the left cell (-1: none/boundary) */
   int opp_right;      /* face opposite across the right cell */
   int side;           /* boundary faces: 0..5 = X-, X+, Y-, Y+, Z-, Z+ (interior: -1) */
+  int bnd_left;       /* boundary face on the other side of the left cell (-1: none) */
+  int bnd_right;      /* boundary face on the other side of the right cell */
+  int opp_int;        /* boundary faces: the internal face across the cell (-1: none) */
 } Face;
 
 typedef struct {
@@ -688,6 +691,20 @@ static int mesh_build(Mesh *m, const orc_cfg *cfg) {
       if (g != fi && g < m->nint && m->faces[g].dir == F->dir) F->opp_right = g;
     }
   }
+  /* boundary faces parallel to an internal face across its cell (walls at second order) */
+  for (int fi = 0; fi < m->nint; fi++) { m->faces[fi].bnd_left = -1; m->faces[fi].bnd_right = -1; }
+  for (int fi = m->nint; fi < nf; fi++) {
+    Face *B = &m->faces[fi];
+    B->opp_int = -1;
+    for (int q = 0; q < m->cell_nf[B->left]; q++) {
+      int g = m->cell_faces[6 * B->left + q];
+      if (g < m->nint && m->faces[g].dir == B->dir) {
+        B->opp_int = g;
+        if (m->faces[g].left == B->left) m->faces[g].bnd_left = fi;
+        else m->faces[g].bnd_right = fi;
+      }
+    }
+  }
   return 0;
 }
 
@@ -763,6 +780,24 @@ static void compute_face_slopes(Solver *S, State *st) {
     prim_vec(&S->c, &st[F->right], wr);
     for (int v = 0; v < nv; v++) S->fslope[(size_t)fi * nv + v] = (wr[v] - wl[v]) / dist;
   }
+  /* BoundCondWallO2::computeSlopes (BoundCondWallO2.cpp:81-111): a wall face's
+     slope is the normal velocity over the signed centre-to-face distance
+     (the mirror image), every other slope 0 */
+  for (int fi = m->nint; fi < m->nface; fi++) {
+    Face *F = &m->faces[fi];
+    double *sl = &S->fslope[(size_t)fi * nv];
+    for (int v = 0; v < nv; v++) sl[v] = 0.;
+    if (S->c.cfg->bc[F->side] != ORC_BC_WALL) continue;
+    double pl[3];
+    cell_pos(m, F->left, pl);
+    double dd = pl[F->dir] - F->pos[F->dir];
+    if (fabs(dd) > 1.e-8) {
+      int iu = (S->c.cfg->model == ORC_EULER) ? 2 + F->dir : 3 * S->c.cfg->nph + F->dir;
+      double w[16];
+      prim_vec(&S->c, &st[F->left], w);
+      sl[iu] = w[iu] / dd;
+    }
+  }
 }
 
 /* CellO2::computeLocalSlopes (CellO2.cpp:101-247) on a Cartesian mesh: slope1 is
@@ -777,12 +812,17 @@ static void local_slopes(Solver *S, int cell, int fref, int fopp, int cell_is_le
   const double *s1 = &S->fslope[(size_t)fref * nv];
   double s2[16];
   double aafter = 0.;
+  const Face *Fr = &S->m.faces[fref];
+  int fb = cell_is_left ? Fr->bnd_left : Fr->bnd_right;
   if (fopp >= 0) {
     for (int v = 0; v < nv; v++) s2[v] = S->fslope[(size_t)fopp * nv + v];
     const Face *G = &S->m.faces[fopp];
     int other = (G->left == cell) ? G->right : G->left;
-    (void)cell_is_left;
     aafter = (cfg->model == ORC_EULER) ? 0. : S->sn[other].ph[0].alpha;
+  } else if (fb >= 0 && cfg->bc[S->m.faces[fb].side] == ORC_BC_WALL) {
+    /* a BoundCondWallO2 face carries slopes; its "cell across" is the cell itself */
+    for (int v = 0; v < nv; v++) s2[v] = S->fslope[(size_t)fb * nv + v];
+    aafter = (cfg->model == ORC_EULER) ? 0. : S->sn[cell].ph[0].alpha;
   } else {
     for (int v = 0; v < nv; v++) s2[v] = 0.;
   }
@@ -803,6 +843,54 @@ static void local_slopes(Solver *S, int cell, int fref, int fopp, int cell_is_le
     out[q] = limiter(lg, s1[q], s2[q]); q++;
   }
   for (int d = 0; d < 3; d++, q++) out[q] = limiter(lg, s1[q], s2[q]);
+}
+
+/* CellO2::computeLocalSlopesLimite (CellO2.cpp:250-354) for a wall face:
+   slope1 is the wall's own slope, slope2 the internal face across the cell
+   (0 if none); the interface test uses the cell's phase-0 alpha only. */
+static void local_slopes_limite(Solver *S, int cell, int fwall, int fopp, double *out) {
+  const orc_cfg *cfg = S->c.cfg;
+  int nv = S->nvar;
+  const double *s1 = &S->fslope[(size_t)fwall * nv];
+  double s2[16];
+  for (int v = 0; v < nv; v++) s2[v] = (fopp >= 0) ? S->fslope[(size_t)fopp * nv + v] : 0.;
+  int iface = 0;
+  if (cfg->model != ORC_EULER) {
+    double a = S->sn[cell].ph[0].alpha;
+    iface = (a >= 1.e-5) && (a <= 1. - 1.e-5);
+  }
+  int lg = iface ? cfg->lim[1] : cfg->lim[0];
+  int lv = iface ? cfg->lim[3] : cfg->lim[2];
+  if (cfg->model == ORC_EULER) {
+    for (int v = 0; v < nv; v++) out[v] = limiter(lg, s1[v], s2[v]);
+    return;
+  }
+  int q = 0;
+  for (int k = 0; k < cfg->nph; k++) {
+    out[q] = limiter(lv, s1[q], s2[q]); q++;
+    out[q] = limiter(lg, s1[q], s2[q]); q++;
+    out[q] = limiter(lg, s1[q], s2[q]); q++;
+  }
+  for (int d = 0; d < 3; d++, q++) out[q] = limiter(lg, s1[q], s2[q]);
+}
+
+/* Phase*::extrapolate / Mix*::extrapolate without the phase verification
+   (BoundCondWallO2::solveRiemann, BoundCondWallO2.cpp:115-150, calls none) */
+static void extrapolate_raw(const Ctx *c, State *s, const double *sl, double dist) {
+  int q = 0;
+  if (c->cfg->model == ORC_EULER) {
+    Phase *p = &s->ph[0];
+    p->rho += sl[0] * dist; p->p += sl[1] * dist;
+    for (int d = 0; d < 3; d++) p->u[d] = p->u[d] + sl[2 + d] * dist;
+    return;
+  }
+  for (int k = 0; k < c->cfg->nph; k++) {
+    Phase *p = &s->ph[k];
+    p->alpha += sl[q] * dist; q++;
+    p->rho += sl[q] * dist; q++;
+    p->p += sl[q] * dist; q++;
+  }
+  for (int d = 0; d < 3; d++, q++) s->mx.u[d] = s->mx.u[d] + sl[q] * dist;
 }
 
 static void extrapolate(const Ctx *c, State *s, const double *sl, double dist, double sign) {
@@ -866,6 +954,16 @@ static void face_fluxes(Solver *S, State *st, int o2) {
       /* BoundCond::solveRiemann (BoundCond.cpp:82-100) + BoundCondAbs (BoundCondAbs.cpp:70-73)
          or BoundCondWall (BoundCondWall.cpp:68-71) */
       State L = st[F->left];
+      if (o2 && c->cfg->bc[F->side] == ORC_BC_WALL) {
+        /* BoundCondWallO2::solveRiemann: the cell extrapolated to the wall
+           (distance negated when the normal points to -x, -y or -z) */
+        double sl[16], pl[3];
+        local_slopes_limite(S, F->left, fi, F->opp_int, sl);
+        cell_pos(m, F->left, pl);
+        double dg = dist3(F->pos, pl);
+        if (F->nrm[0] < 0. || F->nrm[1] < 0. || F->nrm[2] < 0.) dg = -dg;
+        extrapolate_raw(c, &L, sl, dg);
+      }
       proj_state(c, &L, F);
       fulfill(c, &L);
       if (c->cfg->bc[F->side] == ORC_BC_WALL) riemann_wall(c, &L, lcfl, &dtm, FL);
@@ -1112,7 +1210,7 @@ int orc_run(const orc_cfg *cfg, const double *state_in, double *state_out,
   if (is_thinc(cfg) && cfg->nph != 2) return -1;
   for (int b = 0; b < 6; b++) {
     if (cfg->bc[b] != ORC_BC_ABS && cfg->bc[b] != ORC_BC_WALL) return -1;
-    if (cfg->bc[b] == ORC_BC_WALL && cfg->order != 1) return -1; /* second order: BoundCondWallO2, not restated */
+
   }
 #ifdef _OPENMP
   if (nthreads > 0) omp_set_num_threads(nthreads);
@@ -1129,7 +1227,7 @@ int orc_run(const orc_cfg *cfg, const double *state_in, double *state_out,
   S.s2 = calloc(S.m.ncell, sizeof(State));
   S.cons = calloc(S.m.ncell, sizeof(Flux));
   S.fflux = calloc(S.m.nface, sizeof(Flux));
-  S.fslope = calloc((size_t)S.m.nint * S.nvar + 1, sizeof(double));
+  S.fslope = calloc((size_t)S.m.nface * S.nvar + 1, sizeof(double));
   S.fdt = calloc(S.m.nface, sizeof(double));
   if (!S.sn || !S.s2 || !S.cons || !S.fflux || !S.fslope || !S.fdt) return -2;
   init_state(&S, state_in);

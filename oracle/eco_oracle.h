@@ -26,8 +26,9 @@ enum { ORC_LIM_NONE = 0, ORC_LIM_MINMOD = 1, ORC_LIM_VANLEER = 2, ORC_LIM_VANALB
 enum { ORC_EOS_IG = 0, ORC_EOS_SG = 1 };
 #define ORC_MAXPH 4
 
-/* boundary conditions: BoundCondAbs, and BoundCondWall (the reference's first-order
-   WALL and SYMMETRY, src/InputOutput/Input.cpp:850-852) */
+/* boundary conditions: BoundCondAbs, and the wall: BoundCondWall at first
+   order (also the reference's first-order SYMMETRY), BoundCondWallO2 at second
+   order (src/InputOutput/Input.cpp:850-852) */
 enum { ORC_BC_ABS = 0, ORC_BC_WALL = 1 };
 
 typedef struct {

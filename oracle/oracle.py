@@ -20,7 +20,7 @@ LIB_PATH = HERE / "libeco_oracle.so"
 MODELS = {"euler": 0, "kapila": 1, "multip": 2}
 LIMITERS = {"none": 0, "minmod": 1, "vanleer": 2, "vanalbada": 3, "superbee": 4, "mc": 5, "thinc": 6}
 EOS_TYPES = {"IG": 0, "SG": 1}
-BCS = {"abs": 0, "wall": 1}  # per side X-, X+, Y-, Y+, Z-, Z+; "wall" is first order only
+BCS = {"abs": 0, "wall": 1}  # per side X-, X+, Y-, Y+, Z-, Z+ (BoundCondWall / BoundCondWallO2)
 MAXPH = 4
 
 

@@ -167,4 +167,18 @@ CASES = {
     "multip_1d_fo_wall": dict(_shock_tube_1d("multip", 1, ["none"] * 4, 300, cells=200), bc=["wall", "abs"] + ["abs"] * 4),
     "euler_1d_fo_wall": dict(_euler_1d(1, ["none"] * 4, 300), bc=["abs", "wall"] + ["abs"] * 4),
     "kapila_2d_fo_wall": dict(_shock_cylinders_2d(n=40, steps=80), order=1, limiters=["none"] * 4, bc=["wall"] * 6),
+    # Walls at second order (BoundCondWallO2: mirrored normal-velocity slopes, the cell
+    # extrapolated to the wall), with the interface limiters and THINC next to a wall.
+    "kapila_1d_o2_wall": dict(_shock_tube_1d("kapila", 2, ["vanleer"] * 4, 400, cells=200), bc=["wall"] * 6),
+    # (interface limiters next to walls: minmod / van Leer; the minmod / superbee pair is
+    # chaotic over this length -- a 2e-16 change of the initial pressure moves the
+    # result by 6e-2 in the oracle itself -- and so cannot pin 1e-12 parity)
+    "kapila_1d_o2_minmod_vanleer_wall": dict(
+        _shock_tube_1d("kapila", 2, ["minmod", "vanleer", "minmod", "vanleer"], 200, cells=100), bc=["wall"] * 6),
+    "kapila_1d_o2_thinc_wall": dict(
+        _shock_tube_1d("kapila", 2, ["vanleer", "vanleer", "thinc", "thinc"], 400, cells=200), bc=["wall"] * 6),
+    "multip_1d_o2_relax_wall": dict(_shock_tube_1d("multip", 2, ["minmod"] * 4, 160, cells=100, relax=True),
+                                    bc=["wall", "abs"] + ["abs"] * 4),
+    "euler_1d_o2_wall": dict(_euler_1d(2, ["vanleer"] * 4, 300), bc=["abs", "wall"] + ["abs"] * 4),
+    "kapila_2d_o2_wall": dict(_shock_cylinders_2d(n=40, steps=50), bc=["wall"] * 6),
 }

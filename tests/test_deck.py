@@ -38,7 +38,7 @@ def test_deck_reads_back_the_reference_inputs(name, tmp_path):
 def test_deck_refusals(tmp_path):
     from ecogen_amd import deck
     case, _ = load_golden("kapila_1d_o2_vanleer")
-    write_deck(dict(case, name="x", bc=["wall"] * 6), tmp_path / "a")
+    write_deck(dict(case, name="x", bc=["symmetry"] * 6), tmp_path / "a")
     with pytest.raises(deck.DeckError, match="second order"):
         deck.load(tmp_path / "a")
     (tmp_path / "a" / "case" / "meshV5.xml").write_text(

@@ -22,7 +22,8 @@ def _cases():
     out = []
     for n in golden_names():
         case, _ = load_golden(n)
-        if case["order"] == 2 and case["model"] != "euler":
+        # walls run in the FP64 path only (the FP32-storage kernels refuse them)
+        if case["order"] == 2 and case["model"] != "euler" and "wall" not in case.get("bc", []):
             out.append(n)
     return out
 

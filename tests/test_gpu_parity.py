@@ -64,13 +64,11 @@ def test_gpu_matches_oracle_and_reference(name):
         assert e <= TOL_PHASE, f"{nm}: rel L1 vs oracle {e:.3e}"
 
 
-def test_second_order_wall_is_refused():
-    """A second-order WALL deck is BoundCondWallO2 in the reference (mirrored
-    velocity slopes), which the device path does not build: refused, not
-    silently run as a first-order wall."""
+def test_walls_are_refused_in_fp32_storage():
+    """The FP32-storage kernels keep transmissive boundaries: a wall there is
+    refused, not run as a transmissive boundary."""
     from ecogen_amd import Context, EcoError
-    case, _ = load_golden("kapila_1d_o2_vanleer")
+    case, _ = load_golden("kapila_1d_o2_wall")
     with pytest.raises(EcoError):
-        Context(dict(case, bc=["wall"] * 6))
-    case, _ = load_golden("c1_kapila_fo_wall")
+        Context(dict(case, precision="fp32s"))
     Context(case).close()
