@@ -18,9 +18,10 @@ description of :class:`ecogen_amd.Context` and the initial state in the
 * the geometric domains entireDomain, halfSpace, disc, sphere and rectangle,
   applied in deck order (``Cell::fill``, ``src/Cell.cpp:130-137``; the
   ``belong`` tests of ``src/Geometries/GD*.cpp``);
-* ``abs`` and ``wall`` boundaries (``BoundCondWall`` at first order,
-  ``BoundCondWallO2`` at second order), and ``symmetry`` at first order,
-  where the reference makes it a ``BoundCondWall`` (``Input.cpp:849-852``).
+* ``abs``, ``wall`` and ``symmetry`` boundaries: ``BoundCondWall`` at first
+  order for both wall and symmetry (``Input.cpp:849-852``), ``BoundCondWallO2``
+  and ``BoundCondSymmetryO2`` at second order, which differ only in name
+  (``src/BoundConds/BoundCondSymmetryO2.h:41``).
 
 Anything else raises ``DeckError`` naming the element.
 
@@ -266,10 +267,10 @@ def load(rundir):
             raise DeckError(f"boundCond number {num}")
         if kind == "abs":
             bcs[num - 1] = "abs"
-        elif kind == "wall" or (kind == "symmetry" and case["order"] == 1):
+        elif kind in ("wall", "symmetry"):
+            # BoundCondWall at first order for both; at second order
+            # BoundCondSymmetryO2 is BoundCondWallO2 in all but the name
             bcs[num - 1] = "wall"
-        elif kind == "symmetry":
-            raise DeckError("boundCond symmetry at second order (BoundCondSymmetryO2) is not built")
         else:
             raise DeckError(f"boundCond type {kind!r} is not read")
     case["bc"] = bcs

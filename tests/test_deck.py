@@ -38,9 +38,11 @@ def test_deck_reads_back_the_reference_inputs(name, tmp_path):
 def test_deck_refusals(tmp_path):
     from ecogen_amd import deck
     case, _ = load_golden("kapila_1d_o2_vanleer")
-    write_deck(dict(case, name="x", bc=["symmetry"] * 6), tmp_path / "a")
-    with pytest.raises(deck.DeckError, match="second order"):
+    write_deck(dict(case, name="x", bc=["tank"] * 6), tmp_path / "a")
+    with pytest.raises(deck.DeckError, match="tank"):
         deck.load(tmp_path / "a")
+    write_deck(dict(case, name="x", bc=["symmetry"] * 6), tmp_path / "b")
+    assert deck.load(tmp_path / "b")[0]["bc"] == ["wall"] * 6
     (tmp_path / "a" / "case" / "meshV5.xml").write_text(
         '<?xml version="1.0"?><mesh><type structure="unstructured"/></mesh>')
     with pytest.raises(deck.DeckError, match="cartesian"):
