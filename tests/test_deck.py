@@ -42,7 +42,7 @@ def test_deck_refusals(tmp_path):
     with pytest.raises(deck.DeckError, match="tank"):
         deck.load(tmp_path / "a")
     write_deck(dict(case, name="x", bc=["symmetry"] * 6), tmp_path / "b")
-    assert deck.load(tmp_path / "b")[0]["bc"] == ["wall"] * 6
+    assert deck.load(tmp_path / "b")[0]["bc"][:2] == ["wall"] * 2  # a 1-D deck names X- and X+ only
     (tmp_path / "a" / "case" / "meshV5.xml").write_text(
         '<?xml version="1.0"?><mesh><type structure="unstructured"/></mesh>')
     with pytest.raises(deck.DeckError, match="cartesian"):
