@@ -300,16 +300,26 @@ def _run_with_output(ctx, case, outdir):
     """March to the deck's end, writing a result file at t = 0 and after every
     iterFreq steps (iteration control) or at the first step reaching each
     multiple of timeFreq (physical-time control), as the reference does, plus
-    collection.pvd."""
+    collection.pvd and infoCalcul.out (Output::saveInfos, Output.cpp:337-364:
+    the CPU count, then per result file its number, the iteration, the
+    physical time, the computation time in ms and the dt of the last step,
+    1e-15 at t = 0: Run::solver writes before it updates dt, Run.cpp:234-270)."""
+    import time
     from . import output
     o = case["output"]
     outdir.mkdir(parents=True, exist_ok=True)
     prec = o["precision"] or 6  # the stream's default precision when the deck sets none
     files = []
+    t0 = time.perf_counter()
+    info = open(outdir / "infoCalcul.out", "w")
+    info.write("1\n")
 
-    def dump():
+    def dump(st=None):
         f = outdir / f"result_CPU0_AMR0_TIME{len(files)}.vtr"
         output.write_vtr(f, case, ctx.get_state(), binary=o["binary"], precision=prec)
+        it, t, dt = (0, 0.0, 1.e-15) if st is None else (st.steps, st.t, st.dt_last)
+        ms = int((time.perf_counter() - t0) * 1e3) if st is not None else 0
+        info.write(f"{len(files)} {it} {t:.{prec}g} {ms} {dt:.{prec}g}\n")
         files.append(f)
     dump()
     if case["time_mode"] == "iterations":
@@ -319,7 +329,7 @@ def _run_with_output(ctx, case, outdir):
             n = min(step, case["steps"] - done)
             st = ctx.run(nsteps=n)
             done += n
-            dump()
+            dump(st)
     else:
         tf = case["t_final"]
         freq = o["time_freq"] if o["time_freq"] > 0 else tf
@@ -327,11 +337,12 @@ def _run_with_output(ctx, case, outdir):
         while True:
             target = min(k * freq, tf)
             st = ctx.run(t_final=target)
-            dump()
+            dump(st)
             if st.t >= float(np.float32(tf)) or target >= tf:
                 break
             k += 1
     output.write_collection(outdir / "collection.pvd", files)
+    info.close()
     return st
 
 

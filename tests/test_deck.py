@@ -100,3 +100,24 @@ def test_deck_vtk_results_match_reference_files(name, tmp_path):
     for (n, _, a), (_, _, b) in zip(ours, ref):
         scale = max(np.abs(b).max(), 1e-300)
         assert np.abs(a - b).max() <= 1e-10 * scale, n
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["c1_kapila_fo_wall", "kapila_1d_o2_wall"])
+def test_deck_info_file_matches_reference(name, tmp_path):
+    """infoCalcul.out of a --vtk run against the reference's own (kept in the
+    golden): result number, iteration, physical time and last dt per output
+    (the computation-time column is a wall-clock time and is not compared)."""
+    from ecogen_amd import deck
+    case, d = load_golden(name)
+    print(f"case {name} deck --vtk info", flush=True)
+    write_deck(dict(case, name=name), tmp_path / "run")
+    deck.main([str(tmp_path / "run"), "--out", str(tmp_path / "f.npz"), "--vtk"])
+    lines = (tmp_path / "run" / "results" / name / "infoCalcul.out").read_text().split("\n")
+    assert lines[0] == "1"
+    ours = np.array([[float(x) for x in r.split()] for r in lines[1:] if r.strip()])
+    ref = d["info"]
+    assert ours.shape == ref.shape
+    assert np.array_equal(ours[:, :2], ref[:, :2])
+    for col in (2, 4):
+        assert np.allclose(ours[:, col], ref[:, col], rtol=1e-12, atol=0.0), col
