@@ -559,7 +559,7 @@ __device__ __forceinline__ void wall_state_o2(const SchemeParams &S, const Geome
 // One interior face of the tile: reconstruction (MUSCL) or cached cell states
 // (first order), HLLC, flux to LDS slot ``slot`` (row stride ``stride``).
 // Returns the largest wave speed of the Riemann problem (0 if no face).
-template <int MODEL, int NPH, int DIM, int STAGE, bool SAMELIM, int dir>
+template <int MODEL, int NPH, int DIM, int STAGE, bool SAMELIM, int dir, bool WALL = true>
 __device__ __forceinline__ double face_fp64(const StageArgs &A, const int *o, const long long *strd, int f,
                                             double *sF, int stride, int slot) {
   using Ly = Layout<MODEL, NPH, DIM>;
@@ -608,8 +608,8 @@ __device__ __forceinline__ double face_fp64(const StageArgs &A, const int *o, co
       // a second-order wall behind a boundary cell (BoundCondWallO2) carries
       // the slope u_n / (x_cell - x_wall) of the normal velocity and counts
       // the cell itself as the cell across it (CellO2.cpp:121-205)
-      const bool wm = !has_m && ((S.bc_wall >> (2 * dir)) & 1);
-      const bool wp = !has_p && ((S.bc_wall >> (2 * dir + 1)) & 1);
+      const bool wm = WALL && !has_m && ((S.bc_wall >> (2 * dir)) & 1);
+      const bool wp = WALL && !has_p && ((S.bc_wall >> (2 * dir + 1)) & 1);
       // interface detection: alpha of the face's cells from the stage state,
       // alpha across the opposite face from the time-n state (CellO2.cpp:144-150)
       double aL = 0., aR = 0., aLL = 0., aRR = 0.;
@@ -676,14 +676,15 @@ __device__ __forceinline__ double face_fp64(const StageArgs &A, const int *o, co
     return smx;
 }
 
-template <int MODEL, int NPH, int DIM, int STAGE, bool SAMELIM, int dir, int NT>
+template <int MODEL, int NPH, int DIM, int STAGE, bool SAMELIM, int dir, int NT, bool WALL = true>
 __device__ __forceinline__ void stage_dir_3d(const StageArgs &A, const int *o, const long long *strd, int tid, bool active,
                                              int lx, int ly, int lz, const int *ci, long long cidx, double *sF, int stride,
                                              double (*sAcc)[NT], FluxVec<NPH> &acc, double &smx) {
   if constexpr (dir < DIM) {
     constexpr int TX = Tile<DIM>::X, TY = Tile<DIM>::Y, TZ = Tile<DIM>::Z;
     const int nfd = (TX + (dir == 0)) * (TY + (dir == 1)) * (TZ + (dir == 2));
-    for (int f = tid; f < nfd; f += NT) smx = smax(smx, face_fp64<MODEL, NPH, DIM, STAGE, SAMELIM, dir>(A, o, strd, f, sF, stride, f));
+    for (int f = tid; f < nfd; f += NT)
+      smx = smax(smx, face_fp64<MODEL, NPH, DIM, STAGE, SAMELIM, dir, WALL>(A, o, strd, f, sF, stride, f));
     __syncthreads();
     if (active) {
       if (dir > 0) unpark_acc<MODEL, NPH, DIM, NT>(sAcc, tid, acc);
@@ -714,7 +715,7 @@ struct StageLds {
 };
 
 // Tile `blk` of the case described by A, launch slot `slot`.
-template <int MODEL, int NPH, int DIM, int STAGE, bool SAMELIM>
+template <int MODEL, int NPH, int DIM, int STAGE, bool SAMELIM, bool WALL = true>
 __device__ __forceinline__ void stage_tile(const StageArgs &A, const int slot, const int blk,
                                            StageLds<MODEL, NPH, DIM> &lds) {
   using Ly = Layout<MODEL, NPH, DIM>;
@@ -765,9 +766,9 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int slot, c
   if constexpr (MERGED) {
     // all faces of the tile (both directions) in one pass, one barrier
     for (int w = tid; w < NF0 + NF1; w += NT) {
-      if (w < NF0) smx = smax(smx, face_fp64<MODEL, NPH, DIM, STAGE, SAMELIM, 0>(A, o, strd, w, &sF[0][0], NFT, w));
+      if (w < NF0) smx = smax(smx, face_fp64<MODEL, NPH, DIM, STAGE, SAMELIM, 0, WALL>(A, o, strd, w, &sF[0][0], NFT, w));
       else if constexpr (DIM > 1)
-        smx = smax(smx, face_fp64<MODEL, NPH, DIM, STAGE, SAMELIM, 1>(A, o, strd, w - NF0, &sF[0][0], NFT, w));
+        smx = smax(smx, face_fp64<MODEL, NPH, DIM, STAGE, SAMELIM, 1, WALL>(A, o, strd, w - NF0, &sF[0][0], NFT, w));
     }
     __syncthreads();
     if (active) {
@@ -778,9 +779,9 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int slot, c
     }
   } else {
     // 3-D: one direction at a time, the accumulator parked in LDS in between
-    stage_dir_3d<MODEL, NPH, DIM, STAGE, SAMELIM, 0>(A, o, strd, tid, active, lx, ly, lz, ci, cidx, &sF[0][0], NFT, sAcc, acc, smx);
-    stage_dir_3d<MODEL, NPH, DIM, STAGE, SAMELIM, 1>(A, o, strd, tid, active, lx, ly, lz, ci, cidx, &sF[0][0], NFT, sAcc, acc, smx);
-    stage_dir_3d<MODEL, NPH, DIM, STAGE, SAMELIM, 2>(A, o, strd, tid, active, lx, ly, lz, ci, cidx, &sF[0][0], NFT, sAcc, acc, smx);
+    stage_dir_3d<MODEL, NPH, DIM, STAGE, SAMELIM, 0, NT, WALL>(A, o, strd, tid, active, lx, ly, lz, ci, cidx, &sF[0][0], NFT, sAcc, acc, smx);
+    stage_dir_3d<MODEL, NPH, DIM, STAGE, SAMELIM, 1, NT, WALL>(A, o, strd, tid, active, lx, ly, lz, ci, cidx, &sF[0][0], NFT, sAcc, acc, smx);
+    stage_dir_3d<MODEL, NPH, DIM, STAGE, SAMELIM, 2, NT, WALL>(A, o, strd, tid, active, lx, ly, lz, ci, cidx, &sF[0][0], NFT, sAcc, acc, smx);
   }
 
   if (active) {
@@ -798,8 +799,8 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int slot, c
         if (!on) continue;
         CellState<NPH> L;
         load_prim<MODEL, NPH, DIM>(Wsrc, nc, cidx, L);
-        const bool wall = (S.bc_wall >> (2 * dir + side)) & 1;
-        if constexpr (STAGE != 0) {
+        const bool wall = WALL && ((S.bc_wall >> (2 * dir + side)) & 1);
+        if constexpr (STAGE != 0 && WALL) {
           if (wall) wall_state_o2<MODEL, NPH, DIM>(S, G, Wsrc, Wn, nc, cidx, strd[dir], dir, side, L);
         }
         to_face_frame(dir, side == 0, L.u);
@@ -855,11 +856,14 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int slot, c
   }
 }
 
-template <int MODEL, int NPH, int DIM, int STAGE, bool SAMELIM>
+// WALL = false: the MUSCL stages of a solve without walls (no wall code in
+// the face pass or the boundary faces; the register allocation of the hot
+// path is the transmissive one)
+template <int MODEL, int NPH, int DIM, int STAGE, bool SAMELIM, bool WALL = true>
 __global__ void __launch_bounds__(Tile<DIM>::T, Tile<DIM>::W)
 k_stage(const StageArgs A) {
   __shared__ StageLds<MODEL, NPH, DIM> lds;
-  stage_tile<MODEL, NPH, DIM, STAGE, SAMELIM>(A, A.step, blockIdx.x, lds);
+  stage_tile<MODEL, NPH, DIM, STAGE, SAMELIM, WALL>(A, A.step, blockIdx.x, lds);
 }
 
 // Persistent form over a pool: every workgroup walks the class's tile table
@@ -910,11 +914,16 @@ int launch_t(const StageArgs &A, int stage, hipStream_t st) {
   const int tgz = (A.geo.n[2] + Tile<DIM>::Z - 1) / Tile<DIM>::Z;
   dim3 grid((unsigned)(tgx * tgy * tgz)), block(NT);
   const bool same = A.sch.same_lim != 0;
+  const bool wall = A.sch.bc_wall != 0;
   if (stage == 0) hipLaunchKernelGGL((k_stage<MODEL, NPH, DIM, 0, true>), grid, block, 0, st, A);
-  else if (stage == 1 && same) hipLaunchKernelGGL((k_stage<MODEL, NPH, DIM, 1, true>), grid, block, 0, st, A);
-  else if (stage == 1) hipLaunchKernelGGL((k_stage<MODEL, NPH, DIM, 1, false>), grid, block, 0, st, A);
-  else if (stage == 2 && same) hipLaunchKernelGGL((k_stage<MODEL, NPH, DIM, 2, true>), grid, block, 0, st, A);
-  else if (stage == 2) hipLaunchKernelGGL((k_stage<MODEL, NPH, DIM, 2, false>), grid, block, 0, st, A);
+  else if (stage == 1 && same && wall) hipLaunchKernelGGL((k_stage<MODEL, NPH, DIM, 1, true>), grid, block, 0, st, A);
+  else if (stage == 1 && same) hipLaunchKernelGGL((k_stage<MODEL, NPH, DIM, 1, true, false>), grid, block, 0, st, A);
+  else if (stage == 1 && wall) hipLaunchKernelGGL((k_stage<MODEL, NPH, DIM, 1, false>), grid, block, 0, st, A);
+  else if (stage == 1) hipLaunchKernelGGL((k_stage<MODEL, NPH, DIM, 1, false, false>), grid, block, 0, st, A);
+  else if (stage == 2 && same && wall) hipLaunchKernelGGL((k_stage<MODEL, NPH, DIM, 2, true>), grid, block, 0, st, A);
+  else if (stage == 2 && same) hipLaunchKernelGGL((k_stage<MODEL, NPH, DIM, 2, true, false>), grid, block, 0, st, A);
+  else if (stage == 2 && wall) hipLaunchKernelGGL((k_stage<MODEL, NPH, DIM, 2, false>), grid, block, 0, st, A);
+  else if (stage == 2) hipLaunchKernelGGL((k_stage<MODEL, NPH, DIM, 2, false, false>), grid, block, 0, st, A);
   else if (stage == 3) {
     const long long nc = A.geo.ncell;
     hipLaunchKernelGGL((k_init<MODEL, NPH, DIM>), dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, st, A);
