@@ -17,7 +17,7 @@ MODEL = {"euler": 0, "kapila": 1, "multip": 2}
 LIMITER = {"none": 0, "minmod": 1, "vanleer": 2, "vanalbada": 3, "superbee": 4, "mc": 5, "thinc": 6}
 EOS_TYPE = {"IG": 0, "SG": 1}
 PRECISION = {"fp64": 0, "fp32s": 1}
-BC = {"abs": 0, "wall": 1}  # per side X-, X+, Y-, Y+, Z-, Z+; "wall": FP64 only
+BC = {"abs": 0, "wall": 1, "outflow": 2}  # per side X-, X+, Y-, Y+, Z-, Z+; wall and outflow: FP64 only
 
 WARN_RELAX_NOT_CONVERGED = 1
 WARN_NAN = 2
@@ -34,7 +34,8 @@ class EcoEos(C.Structure):
 class EcoConfig(C.Structure):
     _fields_ = [("model", C.c_int), ("nphase", C.c_int), ("order", C.c_int), ("lim", C.c_int * 4),
                 ("alpha_null", C.c_int), ("relax_p", C.c_int), ("precision", C.c_int), ("n", C.c_int * 3),
-                ("L", C.c_double * 3), ("cfl", C.c_double), ("bc", C.c_int * 6), ("eos", EcoEos * 4)]
+                ("L", C.c_double * 3), ("cfl", C.c_double), ("bc", C.c_int * 6), ("eos", EcoEos * 4),
+                ("bc_p0", C.c_double * 6)]
 
 
 class EcoStatus(C.Structure):
@@ -121,6 +122,8 @@ def make_config(case) -> EcoConfig:
     c.cfl = case["cfl"]
     for b, name in enumerate(case.get("bc", ["abs"] * 6)):
         c.bc[b] = BC[name]
+    for b, p0 in enumerate(case.get("bc_p0", [0.0] * 6)):
+        c.bc_p0[b] = p0
     for k, e in enumerate(case["eos"]):
         c.eos[k] = EcoEos(EOS_TYPE[e["type"]], e["gamma"], e.get("pinf", 0.0), e["cv"], e.get("eref", 0.0))
     return c

@@ -86,9 +86,9 @@ int eco::validate_config(const eco_config *c) {
     for (int i = 0; i < 4; i++) if (c->lim[i] == ECO_LIM_NONE) return ECO_ERR_INVALID;
   }
   for (int i = 0; i < 6; i++) {
-    if (c->bc[i] != ECO_BC_ABS && c->bc[i] != ECO_BC_WALL) return ECO_ERR_INVALID;
-    // walls: FP64 path (the FP32-storage kernels keep transmissive boundaries)
-    if (c->bc[i] == ECO_BC_WALL && c->precision == ECO_FP32S) return ECO_ERR_UNSUPPORTED;
+    if (c->bc[i] != ECO_BC_ABS && c->bc[i] != ECO_BC_WALL && c->bc[i] != ECO_BC_OUTFLOW) return ECO_ERR_INVALID;
+    // walls and outflows: FP64 path (the FP32-storage kernels keep transmissive boundaries)
+    if (c->bc[i] != ECO_BC_ABS && c->precision == ECO_FP32S) return ECO_ERR_UNSUPPORTED;
   }
   if (c->model == ECO_MODEL_EULER) {
     if (c->nphase != 1) return ECO_ERR_INVALID;
@@ -168,7 +168,12 @@ void eco::lower_config(const eco_config &c, Lowered &L) {
   S.cfl = c.cfl;
   S.prec = c.precision;
   S.bc_wall = 0;
-  for (int b = 0; b < 6; b++) if (c.bc[b] == ECO_BC_WALL) S.bc_wall |= 1 << b;
+  S.bc_out = 0;
+  for (int b = 0; b < 6; b++) {
+    if (c.bc[b] == ECO_BC_WALL) S.bc_wall |= 1 << b;
+    if (c.bc[b] == ECO_BC_OUTFLOW) S.bc_out |= 1 << b;
+    S.bc_p0[b] = c.bc_p0[b];
+  }
   for (int k = 0; k < c.nphase && k < kMaxPhases; k++) lower_eos(c.eos[k], S.eos[k]);
   L.nprim = (c.model == ECO_MODEL_EULER) ? 2 + dim : 3 * c.nphase + dim;
   L.es = (c.precision == ECO_FP32S) ? sizeof(float) : sizeof(double);

@@ -53,6 +53,8 @@ struct SchemeParams {
   double cfl;
   int prec;          // PREC_FP64 or PREC_FP32S (storage)
   int bc_wall;       // bit 2*dir+side set: wall boundary (BoundCondWall), else transmissive
+  int bc_out;        // bit 2*dir+side set: outflow boundary (BoundCondOutflow) at bc_p0[2*dir+side]
+  double bc_p0[6];
   EosParams eos[kMaxPhases];
 };
 

@@ -429,6 +429,96 @@ ECO_D void correction_energy(const SchemeParams &S, CellState<NPH> &s) {
 // em_* is the mixture internal energy the reference reads (ΣY_k e_k after
 // fulfillState, or E - |u|^2/2 for MultiP first-order interior faces).
 // Returns max(|sL|,|sR|) among speeds above the 1e-3 guard (0 if none).
+// Eos{IG,SG}::computeDensityIsentropic (EosIG.cpp:146-151, EosSG.cpp:149-154)
+// and computeDensityHugoniot (EosIG.cpp:155-162, EosSG.cpp:158-165)
+ECO_D double eos_rho_isentropic(const EosParams &q, double p0, double rho0, double pf, double eps) {
+  if (q.type == EOS_SG) return rho0 * pow_pos((pf + q.pinf) / smax((p0 + q.pinf), eps), 1. / q.gamma);
+  return rho0 * pow_pos(pf / smax(p0, eps), 1. / q.gamma);
+}
+ECO_D double eos_rho_hugoniot(const EosParams &q, double p0, double rho0, double pf, double eps) {
+  double num, denom;
+  if (q.type == EOS_SG) {
+    num = (q.gamma + 1.) * (pf + q.pinf) + (q.gamma - 1.) * (p0 + q.pinf);
+    denom = (q.gamma - 1.) * (pf + q.pinf) + (q.gamma + 1.) * (p0 + q.pinf);
+  } else {
+    num = (q.gamma + 1.) * pf + (q.gamma - 1.) * p0;
+    denom = (q.gamma - 1.) * pf + (q.gamma + 1.) * p0;
+  }
+  return rho0 * num / smax(denom, eps);
+}
+
+// The outflow's half Riemann problem (imposed pressure p0) in the face frame:
+// ModKapila::solveRiemannOutflow (ModKapila.cpp:479-549, isentropic left
+// wave), ModMultiP::solveRiemannOutflow (ModMultiP.cpp:468-538, Hugoniot left
+// wave), ModEuler::solveRiemannOutflow (ModEuler.cpp:427-475).  Where the
+// MultiP form leaves mL unset (|v* - 1/rhoL| <= 1e-10) it is rhoL cL, as in
+// the Kapila form.  Returns |sL| when above the 1e-3 guard, else 0.
+template <int MODEL, int NPH>
+ECO_D double outflow_flux(const SchemeParams &S, const CellState<NPH> &L, double p0, FluxVec<NPH> &F) {
+  const double eps = S.eps;
+#pragma unroll
+  for (int k = 0; k < NPH; k++) { F.a[k] = 0.; F.m[k] = 0.; F.en[k] = 0.; }
+  double sL;
+  if constexpr (MODEL == MODEL_EULER) {
+    const double uL = L.u[0], vL = L.u[1], wL = L.u[2], pL = L.p[0], rhoL = L.r[0], cL = L.c[0];
+    const double zL = rhoL * cL;
+    sL = uL - cL;
+    double pStar = p0;
+    double rhoStar = eos_rho_hugoniot(S.eos[0], pL, rhoL, pStar, 0.);
+    double uStar = (pL + zL * uL - pStar) / zL;
+    if (sL >= 0.) { uStar = uL; pStar = pL; rhoStar = rhoL; }
+    else if (uStar < 0) { rhoStar = rhoL; }
+    const double eStar = eos_e(S.eos[0], rhoStar, pStar, 0.);
+    F.m[0] = rhoStar * uStar;
+    F.q[0] = rhoStar * uStar * uStar + pStar; F.q[1] = rhoStar * uStar * vL; F.q[2] = rhoStar * uStar * wL;
+    F.E = (rhoStar * (eStar + 0.5 * (uStar * uStar + vL * vL + wL * wL)) + pStar) * uStar;
+    F.sM = uStar;
+  } else {
+    constexpr bool KAP = (MODEL == MODEL_KAPILA);
+    const double uL = L.u[0], cL = L.cF, pL = L.pm, rhoL = L.rho;
+    const double uyL = L.u[1], uzL = L.u[2], EL = L.em + 0.5 * sqn(L.u);
+    const double zL = rhoL * cL;
+    double pStar = p0, EStar, vStar = 0., uStar, rk[NPH];
+#pragma unroll
+    for (int k = 0; k < NPH; k++) {
+      if constexpr (KAP) {
+        rk[k] = eos_rho_isentropic(S.eos[k], pL, L.r[k], pStar, eps);
+        vStar += L.a[k] * L.r[k] / rhoL / smax(rk[k], eps);
+      } else {
+        rk[k] = eos_rho_hugoniot(S.eos[k], pL, L.r[k], pStar, eps);
+        vStar += L.a[k] * L.r[k] / rhoL / rk[k];
+      }
+    }
+    const double vSmvL = vStar - 1. / rhoL;
+    double mL = zL;
+    if (fabs(vSmvL) > 1e-10) mL = sqrt((pL - pStar) / vSmvL);
+    sL = uL - mL / rhoL;
+    const double sM = uL + mL * vSmvL;
+    if (sL >= 0. || sM < 0) {  // supersonic outflow, or inflow: the left state
+      uStar = uL; pStar = pL;
+#pragma unroll
+      for (int k = 0; k < NPH; k++) rk[k] = L.r[k];
+      vStar = 1. / rhoL;
+      EStar = EL;
+    } else {
+      uStar = sM;
+      EStar = EL + (uStar - uL) * (uStar - pL / mL);
+    }
+#pragma unroll
+    for (int k = 0; k < NPH; k++) {
+      const double YkL = L.a[k] * L.r[k] / rhoL;
+      const double ekStar = eos_e(S.eos[k], rk[k], pStar, eps);
+      F.a[k] = KAP ? YkL / smax(rk[k], eps) / vStar * uStar : YkL / rk[k] / vStar * uStar;
+      F.m[k] = F.a[k] * rk[k];
+      F.en[k] = F.m[k] * ekStar;
+    }
+    F.q[0] = uStar * uStar / vStar + pStar; F.q[1] = uStar * uyL / vStar; F.q[2] = uStar * uzL / vStar;
+    F.E = (EStar / vStar + pStar) * uStar;
+    F.sM = uStar;
+  }
+  return (fabs(sL) > 1.e-3) ? fabs(sL) : 0.;
+}
+
 // The wall's half Riemann problem in the face frame: ModKapila::solveRiemannWall
 // (ModKapila.cpp:224-249), ModMultiP::solveRiemannWall (ModMultiP.cpp:220-245),
 // ModEuler::solveRiemannWall (ModEuler.cpp:181-210).  Only the normal momentum

@@ -18,7 +18,8 @@ description of :class:`ecogen_amd.Context` and the initial state in the
 * the geometric domains entireDomain, halfSpace, disc, sphere and rectangle,
   applied in deck order (``Cell::fill``, ``src/Cell.cpp:130-137``; the
   ``belong`` tests of ``src/Geometries/GD*.cpp``);
-* ``abs``, ``wall`` and ``symmetry`` boundaries: ``BoundCondWall`` at first
+* ``abs``, ``outflow`` (``BoundCondOutflow``, the exit pressure ``p0``),
+  ``wall`` and ``symmetry`` boundaries: ``BoundCondWall`` at first
   order for both wall and symmetry (``Input.cpp:849-852``), ``BoundCondWallO2``
   and ``BoundCondSymmetryO2`` at second order, which differ only in name
   (``src/BoundConds/BoundCondSymmetryO2.h:41``).
@@ -259,6 +260,7 @@ def load(rundir):
     # initial conditions (Input::entreeConditionsInitiales)
     states = {s.get("name"): s for s in ci.findall("state")}
     bcs = ["abs"] * 6
+    p0s = [0.0] * 6
     bnode = ci.find("boundaryConditions")
     for b in (bnode.findall("boundCond") if bnode is not None else []):
         kind = b.get("type", "").lower()
@@ -271,9 +273,13 @@ def load(rundir):
             # BoundCondWall at first order for both; at second order
             # BoundCondSymmetryO2 is BoundCondWallO2 in all but the name
             bcs[num - 1] = "wall"
+        elif kind == "outflow":  # BoundCondOutflow.cpp:41-52: <dataOutflow p0="..."/>
+            bcs[num - 1] = "outflow"
+            p0s[num - 1] = _f(_child(b, "dataOutflow", "boundCond outflow"), "p0", "boundCond outflow")
         else:
             raise DeckError(f"boundCond type {kind!r} is not read")
     case["bc"] = bcs
+    case["bc_p0"] = p0s
 
     X, Y, Z = _centres(case["cells"], case["lengths"])
     Xg, Yg, Zg = X[None, None, :], Y[None, :, None], Z[:, None, None]

@@ -49,7 +49,7 @@ def grad_rho(case, rho):
     """|grad rho| of Cell::getGradient on a Cartesian grid: per interior face
     (cd - cg)/|xR - xL| times the normal, weighted by the centre distance along
     each axis; every boundary face (any type, for the density) adds twice the
-    centre-to-face distance to the weights only."""
+    centre-to-face distance to the weights only (an outflow face nothing)."""
     n = case["cells"]
     dim = sum(1 for c in n if c > 1)
     pos, dx = _positions(case)
@@ -82,7 +82,9 @@ def grad_rho(case, rho):
         wm[hi] = np.broadcast_to(dist, face.shape)
         w[d] = (w[d] + wm) + wp
     # boundary faces after the interior ones (X-, X+, Y-, ...): twice the
-    # centre-to-face distance, the X+ face lying at x_{n-1} + dX/2
+    # centre-to-face distance, the X+ face lying at x_{n-1} + dX/2; an outflow
+    # face (whoAmI 3) has no branch there and adds nothing
+    bcs = case.get("bc", ["abs"] * 6)
     for d in range(dim):
         ax = axis_of[d]
         p = pos[d]
@@ -90,8 +92,10 @@ def grad_rho(case, rho):
         last = [slice(None)] * 3
         first[ax] = slice(0, 1)
         last[ax] = slice(-1, None)
-        w[d][tuple(first)] += np.abs(p[0] - 0.0) * 2.0
-        w[d][tuple(last)] += np.abs(p[-1] - (p[-1] + 0.5 * dx[d])) * 2.0
+        if bcs[2 * d] != "outflow":
+            w[d][tuple(first)] += np.abs(p[0] - 0.0) * 2.0
+        if bcs[2 * d + 1] != "outflow":
+            w[d][tuple(last)] += np.abs(p[-1] - (p[-1] + 0.5 * dx[d])) * 2.0
     comp = []
     for d in range(3):
         ww = np.where(w[d] <= 1e-12, 1.0, w[d])

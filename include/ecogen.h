@@ -34,9 +34,10 @@ enum eco_eos_type { ECO_EOS_IG = 0, ECO_EOS_SG = 1 };
 enum eco_precision { ECO_FP64 = 0, ECO_FP32S = 1 };
 /* Boundary conditions: transmissive (BoundCondAbs) and wall (BoundCondWall at
    first order, which the reference also uses for a first-order SYMMETRY
-   boundary; BoundCondWallO2 at second order; src/InputOutput/Input.cpp:849-852).
-   Walls run in the FP64 path only. */
-enum eco_bc { ECO_BC_ABS = 0, ECO_BC_WALL = 1 };
+   boundary; BoundCondWallO2 at second order; src/InputOutput/Input.cpp:849-852),
+   outflow at an imposed pressure (BoundCondOutflow, at both orders; the
+   pressure in bc_p0).  Walls and outflows run in the FP64 path only. */
+enum eco_bc { ECO_BC_ABS = 0, ECO_BC_WALL = 1, ECO_BC_OUTFLOW = 2 };
 
 enum eco_status_code {
   ECO_OK = 0,
@@ -70,8 +71,9 @@ typedef struct {
   int n[3];         /* numberCells x, y, z */
   double L[3];      /* dimensions x, y, z (m) */
   double cfl;       /* computationControl CFL */
-  int bc[6];        /* X-, X+, Y-, Y+, Z-, Z+ : enum eco_bc (ECO_BC_WALL: FP64 only) */
+  int bc[6];        /* X-, X+, Y-, Y+, Z-, Z+ : enum eco_bc (ECO_BC_WALL, ECO_BC_OUTFLOW: FP64 only) */
   eco_eos eos[4];   /* per phase */
+  double bc_p0[6];  /* <dataOutflow p0> of each ECO_BC_OUTFLOW side (Pa) */
 } eco_config;
 
 typedef struct eco_ctx eco_ctx;

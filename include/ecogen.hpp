@@ -164,8 +164,9 @@ struct CartesianMesh {
   int numberCells[3] = {1, 1, 1};
   double dimensions[3] = {1., 1., 1.};
   // boundary condition per side X-, X+, Y-, Y+, Z-, Z+ (initialConditions
-  // <boundCond type="abs|wall">; walls run in FP64)
+  // <boundCond type="abs|wall|outflow">; walls and outflows run in FP64)
   int boundCond[6] = {ECO_BC_ABS, ECO_BC_ABS, ECO_BC_ABS, ECO_BC_ABS, ECO_BC_ABS, ECO_BC_ABS};
+  double outflowPressure[6] = {0., 0., 0., 0., 0., 0.};  // <dataOutflow p0> of ECO_BC_OUTFLOW sides
 };
 
 struct RunStatus {

@@ -127,7 +127,14 @@ def write_deck(case, root: Path):
     names = ["CLXm", "CLXp", "CLYm", "CLYp", "CLZm", "CLZp"]
     dim = sum(1 for c in n if c != 1)
     kinds = case.get("bc", ["abs"] * 6)
-    bcs = "\n".join(f'    <boundCond name="{names[i]}" type="{kinds[i]}" number="{i + 1}"/>' for i in range(2 * dim))
+    p0s = case.get("bc_p0", [0.0] * 6)
+
+    def bc_xml(i):
+        head = f'    <boundCond name="{names[i]}" type="{kinds[i]}" number="{i + 1}"'
+        if kinds[i] == "outflow":
+            return head + f'>\n      <dataOutflow p0="{p0s[i]!r}"/>\n    </boundCond>'
+        return head + "/>"
+    bcs = "\n".join(bc_xml(i) for i in range(2 * dim))
     states = "\n".join(_state_xml(case, st) for st in case["states"])
     (cdir / "initialConditionsV4.xml").write_text(
         '<?xml version="1.0" encoding="UTF-8" standalone="yes"?>\n<CI>\n  <physicalDomains>\n'

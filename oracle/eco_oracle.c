@@ -94,6 +94,23 @@ static double eos_rho_pfinal(const orc_eos *q, double p0, double rho0, double pf
   if (drho) *drho = rho0 * q->gamma * p0 / smax((denom * denom), eps);
   return rf;
 }
+/* computeDensityIsentropic: EosSG.cpp:149-154, EosIG.cpp:146-151 */
+static double eos_rho_isentropic(const orc_eos *q, double p0, double rho0, double pf, double eps) {
+  if (q->type == ORC_EOS_SG) return rho0 * pow((pf + q->pinf) / smax((p0 + q->pinf), eps), 1. / q->gamma);
+  return rho0 * pow(pf / smax(p0, eps), 1. / q->gamma);
+}
+/* computeDensityHugoniot: EosSG.cpp:158-165, EosIG.cpp:155-162 */
+static double eos_rho_hugoniot(const orc_eos *q, double p0, double rho0, double pf, double eps) {
+  double num, denom;
+  if (q->type == ORC_EOS_SG) {
+    num = (q->gamma + 1.) * (pf + q->pinf) + (q->gamma - 1.) * (p0 + q->pinf);
+    denom = (q->gamma - 1.) * (pf + q->pinf) + (q->gamma + 1.) * (p0 + q->pinf);
+  } else {
+    num = (q->gamma + 1.) * pf + (q->gamma - 1.) * p0;
+    denom = (q->gamma - 1.) * pf + (q->gamma + 1.) * p0;
+  }
+  return rho0 * num / smax(denom, eps);
+}
 /* verifyAndModifyPressure: EosSG.cpp:243-247, EosIG.cpp:241-244 */
 static void eos_fix_p(const orc_eos *q, double *p) {
   if (q->type == ORC_EOS_SG) {
@@ -513,6 +530,75 @@ static void riemann(const Ctx *c, const State *L, const State *R, double dxL, do
                     double *dtMax, Flux *F) {
   if (c->cfg->model == ORC_EULER) hllc_euler(c, L, R, dxL, dxR, dtMax, F);
   else hllc_multi(c, L, R, dxL, dxR, dtMax, F);
+}
+
+/* The outflow's half Riemann problem (imposed pressure p0) in the face frame:
+   ModKapila::solveRiemannOutflow (ModKapila.cpp:479-549, isentropic left
+   wave), ModMultiP::solveRiemannOutflow (ModMultiP.cpp:468-538, Hugoniot
+   left wave), ModEuler::solveRiemannOutflow (ModEuler.cpp:427-475).  MultiP
+   leaves mL unset when |v* - 1/rhoL| <= 1e-10; here it is rhoL cL, as in
+   the Kapila form. */
+static void riemann_outflow(const Ctx *c, const State *L, double dxL, double *dtMax, double p0, Flux *F) {
+  const orc_cfg *cfg = c->cfg;
+  double eps = c->epsilon;
+  if (cfg->model == ORC_EULER) {
+    const Phase *P = &L->ph[0];
+    double uL = P->u[0], vL = P->u[1], wL = P->u[2], pL = P->p, rhoL = P->rho, cL = P->c;
+    double zL = rhoL * cL;
+    double sL = uL - cL;
+    if (fabs(sL) > 1.e-3) *dtMax = smin(*dtMax, dxL / fabs(sL));
+    double pStar = p0;
+    double rhoStar = eos_rho_hugoniot(&cfg->eos[0], pL, rhoL, pStar, 0.);
+    double uStar = (pL + zL * uL - pStar) / zL;
+    if (sL >= 0.) { uStar = uL; pStar = pL; rhoStar = rhoL; }
+    else if (uStar < 0) { rhoStar = rhoL; }
+    double eStar = eos_e(&cfg->eos[0], rhoStar, pStar, 0.);
+    F->masse[0] = rhoStar * uStar;
+    F->qdm[0] = rhoStar * uStar * uStar + pStar; F->qdm[1] = rhoStar * uStar * vL; F->qdm[2] = rhoStar * uStar * wL;
+    F->energMix = (rhoStar * (eStar + 0.5 * (uStar * uStar + vL * vL + wL * wL)) + pStar) * uStar;
+    F->sM = uStar;
+    return;
+  }
+  int n = cfg->nph, kap = (cfg->model == ORC_KAPILA);
+  double uL = L->mx.u[0], cL = L->mx.cF, pL = L->mx.p, rhoL = L->mx.rho;
+  double uyL = L->mx.u[1], uzL = L->mx.u[2], EL = L->mx.e + 0.5 * sqnorm(L->mx.u);
+  double zL = rhoL * cL;
+  double pStar = p0, EStar = 0., vStar = 0., uStar = 0., rk[ORC_MAXPH];
+  for (int k = 0; k < n; k++) {
+    const Phase *P = &L->ph[k];
+    if (kap) {
+      rk[k] = eos_rho_isentropic(&cfg->eos[k], pL, P->rho, pStar, eps);
+      vStar += P->alpha * P->rho / rhoL / smax(rk[k], eps);
+    } else {
+      rk[k] = eos_rho_hugoniot(&cfg->eos[k], pL, P->rho, pStar, eps);
+      vStar += P->alpha * P->rho / rhoL / rk[k];
+    }
+  }
+  double vSmvL = vStar - 1. / rhoL, mL = zL;
+  if (fabs(vSmvL) > 1e-10) mL = sqrt((pL - pStar) / vSmvL);
+  double sL = uL - mL / rhoL;
+  if (fabs(sL) > 1.e-3) *dtMax = smin(*dtMax, dxL / fabs(sL));
+  double sM = uL + mL * vSmvL;
+  if (sL >= 0. || sM < 0) {  /* supersonic outflow, or inflow: the left state */
+    uStar = uL; pStar = pL;
+    for (int k = 0; k < n; k++) rk[k] = L->ph[k].rho;
+    vStar = 1. / rhoL;
+    EStar = EL;
+  } else {
+    uStar = sM;
+    EStar = EL + (uStar - uL) * (uStar - pL / mL);
+  }
+  for (int k = 0; k < n; k++) {
+    const Phase *P = &L->ph[k];
+    double YkL = P->alpha * P->rho / rhoL;
+    double ekStar = eos_e(&cfg->eos[k], rk[k], pStar, eps);
+    F->alpha[k] = kap ? YkL / smax(rk[k], eps) / vStar * uStar : YkL / rk[k] / vStar * uStar;
+    F->masse[k] = F->alpha[k] * rk[k];
+    F->energ[k] = F->masse[k] * ekStar;
+  }
+  F->qdm[0] = uStar * uStar / vStar + pStar; F->qdm[1] = uStar * uyL / vStar; F->qdm[2] = uStar * uzL / vStar;
+  F->energMix = (EStar / vStar + pStar) * uStar;
+  F->sM = uStar;
 }
 
 /* The wall's half Riemann problem in the face frame: ModKapila::solveRiemannWall
@@ -967,6 +1053,7 @@ static void face_fluxes(Solver *S, State *st, int o2) {
       proj_state(c, &L, F);
       fulfill(c, &L);
       if (c->cfg->bc[F->side] == ORC_BC_WALL) riemann_wall(c, &L, lcfl, &dtm, FL);
+      else if (c->cfg->bc[F->side] == ORC_BC_OUTFLOW) riemann_outflow(c, &L, lcfl, &dtm, c->cfg->bc_p0[F->side], FL);
       else riemann(c, &L, &L, lcfl, lcfl, &dtm, FL);
     } else if (!o2) {
       /* CellInterface::solveRiemann (CellInterface.cpp:127-146): the cells' own cached state */
@@ -1209,7 +1296,7 @@ int orc_run(const orc_cfg *cfg, const double *state_in, double *state_out,
   if (cfg->model == ORC_EULER && cfg->nph != 1) return -1;
   if (is_thinc(cfg) && cfg->nph != 2) return -1;
   for (int b = 0; b < 6; b++) {
-    if (cfg->bc[b] != ORC_BC_ABS && cfg->bc[b] != ORC_BC_WALL) return -1;
+    if (cfg->bc[b] != ORC_BC_ABS && cfg->bc[b] != ORC_BC_WALL && cfg->bc[b] != ORC_BC_OUTFLOW) return -1;
 
   }
 #ifdef _OPENMP

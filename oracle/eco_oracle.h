@@ -29,7 +29,7 @@ enum { ORC_EOS_IG = 0, ORC_EOS_SG = 1 };
 /* boundary conditions: BoundCondAbs, and the wall: BoundCondWall at first
    order (also the reference's first-order SYMMETRY), BoundCondWallO2 at second
    order (src/InputOutput/Input.cpp:850-852) */
-enum { ORC_BC_ABS = 0, ORC_BC_WALL = 1 };
+enum { ORC_BC_ABS = 0, ORC_BC_WALL = 1, ORC_BC_OUTFLOW = 2 };  /* outflow: BoundCondOutflow, imposed p0 */
 
 typedef struct {
   int type;
@@ -47,7 +47,8 @@ typedef struct {
   double L[3];
   double cfl;
   orc_eos eos[ORC_MAXPH];
-  int bc[6];         /* X-, X+, Y-, Y+, Z-, Z+: ORC_BC_ABS or ORC_BC_WALL (first order) */
+  int bc[6];         /* X-, X+, Y-, Y+, Z-, Z+: ORC_BC_ABS, ORC_BC_WALL or ORC_BC_OUTFLOW */
+  double bc_p0[6];   /* outflow pressure per side */
 } orc_cfg;
 
 /*

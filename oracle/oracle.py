@@ -20,7 +20,7 @@ LIB_PATH = HERE / "libeco_oracle.so"
 MODELS = {"euler": 0, "kapila": 1, "multip": 2}
 LIMITERS = {"none": 0, "minmod": 1, "vanleer": 2, "vanalbada": 3, "superbee": 4, "mc": 5, "thinc": 6}
 EOS_TYPES = {"IG": 0, "SG": 1}
-BCS = {"abs": 0, "wall": 1}  # per side X-, X+, Y-, Y+, Z-, Z+ (BoundCondWall / BoundCondWallO2)
+BCS = {"abs": 0, "wall": 1, "outflow": 2}  # per side X-, X+, Y-, Y+, Z-, Z+; outflow pressures in case["bc_p0"]
 MAXPH = 4
 
 
@@ -32,7 +32,8 @@ class OrcEos(C.Structure):
 class OrcCfg(C.Structure):
     _fields_ = [("model", C.c_int), ("nph", C.c_int), ("order", C.c_int), ("lim", C.c_int * 4),
                 ("alpha_null", C.c_int), ("relax_p", C.c_int), ("n", C.c_int * 3),
-                ("L", C.c_double * 3), ("cfl", C.c_double), ("eos", OrcEos * MAXPH), ("bc", C.c_int * 6)]
+                ("L", C.c_double * 3), ("cfl", C.c_double), ("eos", OrcEos * MAXPH), ("bc", C.c_int * 6),
+                ("bc_p0", C.c_double * 6)]
 
 
 _lib = None
@@ -95,6 +96,8 @@ def make_cfg(case) -> OrcCfg:
         cfg.eos[k] = make_eos(e)
     for b, name in enumerate(case.get("bc", ["abs"] * 6)):
         cfg.bc[b] = BCS[name]
+    for b, p0 in enumerate(case.get("bc_p0", [0.0] * 6)):
+        cfg.bc_p0[b] = p0
     return cfg
 
 

@@ -181,4 +181,17 @@ CASES = {
                                     bc=["wall", "abs"] + ["abs"] * 4),
     "euler_1d_o2_wall": dict(_euler_1d(2, ["vanleer"] * 4, 300), bc=["abs", "wall"] + ["abs"] * 4),
     "kapila_2d_o2_wall": dict(_shock_cylinders_2d(n=40, steps=50), bc=["wall"] * 6),
+    # Outflow boundaries (BoundCondOutflow, imposed exit pressure; no second-order
+    # variant): the tubes exit through x+ into a pressure below the air's, and the
+    # Euler tube through x- against the expansion.
+    "kapila_1d_fo_outflow": dict(_shock_tube_1d("kapila", 1, ["none"] * 4, 300, cells=200),
+                                 bc=["abs", "outflow"] + ["abs"] * 4, bc_p0=[0.0, 5e4] + [0.0] * 4),
+    "kapila_1d_o2_outflow": dict(_shock_tube_1d("kapila", 2, ["vanleer"] * 4, 300, cells=200),
+                                 bc=["outflow", "outflow"] + ["abs"] * 4, bc_p0=[2e9, 5e4] + [0.0] * 4),
+    "multip_1d_o2_relax_outflow": dict(_shock_tube_1d("multip", 2, ["minmod"] * 4, 160, cells=100, relax=True),
+                                       bc=["abs", "outflow"] + ["abs"] * 4, bc_p0=[0.0, 5e4] + [0.0] * 4),
+    "euler_1d_fo_outflow": dict(_euler_1d(1, ["none"] * 4, 300), bc=["outflow", "outflow"] + ["abs"] * 4,
+                                bc_p0=[2e7, 5e4] + [0.0] * 4),
+    "kapila_2d_fo_outflow": dict(_shock_cylinders_2d(n=40, steps=80), order=1, limiters=["none"] * 4,
+                                 bc=["abs", "outflow", "wall", "wall", "abs", "abs"], bc_p0=[0.0, 8e4] + [0.0] * 4),
 }

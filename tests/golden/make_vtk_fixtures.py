@@ -31,6 +31,8 @@ FIXTURES = {
     "multip_1d_o2_relax_40": _shock_tube_1d("multip", 2, ["minmod"] * 4, 15, cells=40, relax=True),
     "euler_1d_fo_30": _euler_1d(1, ["none"] * 4, 20, cells=30),
     "kapila_3d_o2_6": _bubbles_3d(n=6, steps=3, nb=1, model="kapila", relax=False, limiters=("vanleer",) * 4),
+    "kapila_1d_fo_outflow_30": dict(_shock_tube_1d("kapila", 1, ["none"] * 4, 20, cells=30),
+                                    bc=["outflow", "outflow"] + ["abs"] * 4, bc_p0=[2e9, 5e4] + [0.0] * 4),
 }
 
 
