@@ -17,10 +17,14 @@ MODEL = {"euler": 0, "kapila": 1, "multip": 2}
 LIMITER = {"none": 0, "minmod": 1, "vanleer": 2, "vanalbada": 3, "superbee": 4, "mc": 5, "thinc": 6}
 EOS_TYPE = {"IG": 0, "SG": 1}
 PRECISION = {"fp64": 0, "fp32s": 1}
-BC = {"abs": 0, "wall": 1, "outflow": 2}  # per side X-, X+, Y-, Y+, Z-, Z+; wall and outflow: FP64 only
+# per side X-, X+, Y-, Y+, Z-, Z+; all but abs: FP64 only.  Outflow pressures in
+# case["bc_p0"]; injections in case["bc_inj"][side] = {"m0": mass flow into the
+# domain, "alpha", "rho", "p": the injected fluid per phase}
+BC = {"abs": 0, "wall": 1, "outflow": 2, "injection": 3}
 
 WARN_RELAX_NOT_CONVERGED = 1
 WARN_NAN = 2
+WARN_INFLOW_NOT_CONVERGED = 4
 
 EXPORTS = ["eco_create", "eco_set_state", "eco_set_state_device", "eco_get_state", "eco_step", "eco_run",
            "eco_launch_steps", "eco_sync", "eco_time_steps", "eco_copy_calibration", "eco_state_device", "eco_destroy", "eco_strerror", "eco_version"]
@@ -35,7 +39,9 @@ class EcoConfig(C.Structure):
     _fields_ = [("model", C.c_int), ("nphase", C.c_int), ("order", C.c_int), ("lim", C.c_int * 4),
                 ("alpha_null", C.c_int), ("relax_p", C.c_int), ("precision", C.c_int), ("n", C.c_int * 3),
                 ("L", C.c_double * 3), ("cfl", C.c_double), ("bc", C.c_int * 6), ("eos", EcoEos * 4),
-                ("bc_p0", C.c_double * 6)]
+                ("bc_p0", C.c_double * 6), ("bc_inj_m0", C.c_double * 6),
+                ("bc_inj_alpha", (C.c_double * 4) * 6), ("bc_inj_rho", (C.c_double * 4) * 6),
+                ("bc_inj_p", (C.c_double * 4) * 6)]
 
 
 class EcoStatus(C.Structure):
@@ -124,6 +130,13 @@ def make_config(case) -> EcoConfig:
         c.bc[b] = BC[name]
     for b, p0 in enumerate(case.get("bc_p0", [0.0] * 6)):
         c.bc_p0[b] = p0
+    for b, inj in enumerate(case.get("bc_inj") or [None] * 6):
+        if inj:
+            c.bc_inj_m0[b] = inj["m0"]
+            for k in range(len(inj["rho"])):
+                c.bc_inj_alpha[b][k] = inj["alpha"][k]
+                c.bc_inj_rho[b][k] = inj["rho"][k]
+                c.bc_inj_p[b][k] = inj["p"][k]
     for k, e in enumerate(case["eos"]):
         c.eos[k] = EcoEos(EOS_TYPE[e["type"]], e["gamma"], e.get("pinf", 0.0), e["cv"], e.get("eref", 0.0))
     return c

@@ -67,7 +67,16 @@ eco_config lowerConfig(const Model &model, const std::vector<std::unique_ptr<Eos
   c.precision = precision;
   for (int d = 0; d < 3; d++) { c.n[d] = mesh.numberCells[d]; c.L[d] = mesh.dimensions[d]; }
   c.cfl = cfl;
-  for (int b = 0; b < 6; b++) { c.bc[b] = mesh.boundCond[b]; c.bc_p0[b] = mesh.outflowPressure[b]; }
+  for (int b = 0; b < 6; b++) {
+    c.bc[b] = mesh.boundCond[b];
+    c.bc_p0[b] = mesh.outflowPressure[b];
+    c.bc_inj_m0[b] = mesh.injection[b].m0;
+    for (int k = 0; k < 4; k++) {
+      c.bc_inj_alpha[b][k] = mesh.injection[b].alpha[k];
+      c.bc_inj_rho[b][k] = mesh.injection[b].density[k];
+      c.bc_inj_p[b][k] = mesh.injection[b].pressure[k];
+    }
+  }
   for (size_t k = 0; k < eos.size() && k < 4; k++) c.eos[k] = eos[k]->lower();
   return c;
 }

@@ -73,6 +73,40 @@ static void lower_eos(const eco_eos &e, EosParams &q) {
   q.pfloor = (e.type == ECO_EOS_SG) ? -(1. - 1.e-15) * e.pinf + 1.e-15 : 1.e-15;
 }
 
+// The constants of one injection side, as the first lines of
+// Mod{Kapila,MultiP}::solveRiemannInflow (ModKapila.cpp:259-266) and
+// ModEuler::solveRiemannInflow (ModEuler.cpp:238-239) compute them on each
+// call: the signed flow (BoundCondInj.cpp:37), the injected density
+// (Mix*::computeDensity), Y_k0 and H_k0 = e_k(rho_k0, p_k0) + p_k0/rho_k0 + u0^2/2
+// (Eos::computeTotalEnthalpy, Eos.cpp:99-102).
+static void lower_injection(const eco_config &c, int b, SchemeParams &S) {
+  const double m0 = -c.bc_inj_m0[b];
+  const double eps = S.eps;
+  auto energy = [&](const eco_eos &e, double rho, double p) {  // Eos{IG,SG}::computeEnergy
+    const double num = (e.type == ECO_EOS_SG) ? p + e.gamma * e.pinf : p;
+    return num / (e.gamma - 1.) / std::max(rho, eps) + e.eref;
+  };
+  S.inj_m0[b] = m0;
+  if (c.model == ECO_MODEL_EULER) {
+    const double rho = c.bc_inj_rho[b][0], p = c.bc_inj_p[b][0];
+    const double u0 = m0 / rho;
+    S.inj_Y[b][0] = 1.;
+    S.inj_H[b][0] = energy(c.eos[0], rho, p) + p / rho + 0.5 * u0 * u0;
+    return;
+  }
+  // the reader divides the fractions by their sum (BoundCondInj.cpp:82-86)
+  double ak0[4], sum = 0., rho0 = 0.;
+  for (int k = 0; k < c.nphase; k++) sum += c.bc_inj_alpha[b][k];
+  for (int k = 0; k < c.nphase; k++) ak0[k] = c.bc_inj_alpha[b][k] / sum;
+  for (int k = 0; k < c.nphase; k++) rho0 += ak0[k] * c.bc_inj_rho[b][k];
+  const double u0 = m0 / rho0;
+  for (int k = 0; k < c.nphase && k < kMaxPhases; k++) {
+    const double rho = c.bc_inj_rho[b][k], p = c.bc_inj_p[b][k];
+    S.inj_H[b][k] = energy(c.eos[k], rho, p) + p / rho + 0.5 * u0 * u0;
+    S.inj_Y[b][k] = ak0[k] * rho / rho0;
+  }
+}
+
 int eco::validate_config(const eco_config *c) {
   if (!c) return ECO_ERR_INVALID;
   if (c->model < 0 || c->model > 2) return ECO_ERR_INVALID;
@@ -86,8 +120,20 @@ int eco::validate_config(const eco_config *c) {
     for (int i = 0; i < 4; i++) if (c->lim[i] == ECO_LIM_NONE) return ECO_ERR_INVALID;
   }
   for (int i = 0; i < 6; i++) {
-    if (c->bc[i] != ECO_BC_ABS && c->bc[i] != ECO_BC_WALL && c->bc[i] != ECO_BC_OUTFLOW) return ECO_ERR_INVALID;
-    // walls and outflows: FP64 path (the FP32-storage kernels keep transmissive boundaries)
+    if (c->bc[i] < ECO_BC_ABS || c->bc[i] > ECO_BC_INJECTION) return ECO_ERR_INVALID;
+    if (c->bc[i] == ECO_BC_INJECTION) {
+      // BoundCondInj.cpp:30-90 reads m0 and, per phase, alpha in [0, 1] summing to 1
+      if (!(c->bc_inj_m0[i] != 0.)) return ECO_ERR_INVALID;
+      const int np = (c->model == ECO_MODEL_EULER) ? 1 : c->nphase;
+      double sum = 0.;
+      for (int k = 0; k < np && k < 4; k++) {
+        if (!(c->bc_inj_rho[i][k] > 0.)) return ECO_ERR_INVALID;
+        if (np > 1 && !(c->bc_inj_alpha[i][k] >= 0. && c->bc_inj_alpha[i][k] <= 1.)) return ECO_ERR_INVALID;
+        sum += c->bc_inj_alpha[i][k];
+      }
+      if (np > 1 && !(std::fabs(sum - 1.) <= 1.e-6)) return ECO_ERR_INVALID;
+    }
+    // walls, outflows and injections: FP64 path (the FP32-storage kernels keep transmissive boundaries)
     if (c->bc[i] != ECO_BC_ABS && c->precision == ECO_FP32S) return ECO_ERR_UNSUPPORTED;
   }
   if (c->model == ECO_MODEL_EULER) {
@@ -169,12 +215,15 @@ void eco::lower_config(const eco_config &c, Lowered &L) {
   S.prec = c.precision;
   S.bc_wall = 0;
   S.bc_out = 0;
+  S.bc_inj = 0;
   for (int b = 0; b < 6; b++) {
     if (c.bc[b] == ECO_BC_WALL) S.bc_wall |= 1 << b;
     if (c.bc[b] == ECO_BC_OUTFLOW) S.bc_out |= 1 << b;
+    if (c.bc[b] == ECO_BC_INJECTION) S.bc_inj |= 1 << b;
     S.bc_p0[b] = c.bc_p0[b];
   }
   for (int k = 0; k < c.nphase && k < kMaxPhases; k++) lower_eos(c.eos[k], S.eos[k]);
+  for (int b = 0; b < 6; b++) if (c.bc[b] == ECO_BC_INJECTION) lower_injection(c, b, S);
   L.nprim = (c.model == ECO_MODEL_EULER) ? 2 + dim : 3 * c.nphase + dim;
   L.es = (c.precision == ECO_FP32S) ? sizeof(float) : sizeof(double);
 }

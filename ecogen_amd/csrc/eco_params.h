@@ -54,7 +54,14 @@ struct SchemeParams {
   int prec;          // PREC_FP64 or PREC_FP32S (storage)
   int bc_wall;       // bit 2*dir+side set: wall boundary (BoundCondWall), else transmissive
   int bc_out;        // bit 2*dir+side set: outflow boundary (BoundCondOutflow) at bc_p0[2*dir+side]
+  int bc_inj;        // bit 2*dir+side set: injection boundary (BoundCondInj)
   double bc_p0[6];
+  // injection constants per side (BoundCondInj + the first lines of
+  // Mod*::solveRiemannInflow): signed mass flow (negative: into the domain),
+  // injected mass fractions Y_k0 and phase total enthalpies H_k0
+  double inj_m0[6];
+  double inj_Y[6][kMaxPhases];
+  double inj_H[6][kMaxPhases];
   EosParams eos[kMaxPhases];
 };
 

@@ -519,6 +519,91 @@ ECO_D double outflow_flux(const SchemeParams &S, const CellState<NPH> &L, double
   return (fabs(sL) > 1.e-3) ? fabs(sL) : 0.;
 }
 
+// Eos{IG,SG}::vfpfh, dvdpch, dvdhcp (EosIG.cpp:220-239, EosSG.cpp:223-242):
+// specific volume from pressure and enthalpy, and its two partial derivatives.
+ECO_D double eos_vfpfh(const EosParams &q, double p, double h, double eps) {
+  const double pp = (q.type == EOS_SG) ? q.gamma * (p + q.pinf) : q.gamma * p;
+  return q.gm1 * (h - q.eref) / smax(pp, eps);
+}
+ECO_D double eos_dvdpch(const EosParams &q, double p, double h, double eps) {
+  const double pp = (q.type == EOS_SG) ? (p + q.pinf) * (p + q.pinf) : p * p;
+  return (1. - q.gamma) / q.gamma * (h - q.eref) / smax(pp, eps);
+}
+ECO_D double eos_dvdhcp(const EosParams &q, double p, double eps) {
+  const double pp = (q.type == EOS_SG) ? (p + q.pinf) : p;
+  return q.gm1 / q.gamma / smax(pp, eps);
+}
+
+// The injection's half Riemann problem in the face frame (BoundCondInj): a
+// fluid of phase total enthalpies H_k0 and mass fractions Y_k0 enters at the
+// mass flow m0 per unit area (negative: into the domain, the reference's sign
+// after reading, BoundCondInj.cpp:33-37); the left wave is acoustic.
+// ModKapila::solveRiemannInflow (ModKapila.cpp:253-322) == ModMultiP
+// (ModMultiP.cpp:249-318); ModEuler::solveRiemannInflow (ModEuler.cpp:214-303)
+// is the one-phase form with H0 in H[0].  Newton on the pressure from pL until
+// |f| <= 1e-10; where the reference stops after 50 iterations, the 50th
+// iterate is kept and `nonconv` set.  Returns |sL| above the 1e-3 guard, else 0.
+template <int MODEL, int NPH>
+ECO_D double inflow_flux(const SchemeParams &S, const CellState<NPH> &L, double m0, const double *Y,
+                         const double *H, FluxVec<NPH> &F, bool &nonconv) {
+  const double eps = S.eps;
+#pragma unroll
+  for (int k = 0; k < NPH; k++) { F.a[k] = 0.; F.m[k] = 0.; F.en[k] = 0.; }
+  double uL, vL, wL, pL, rhoL, cL;
+  if constexpr (MODEL == MODEL_EULER) {
+    uL = L.u[0]; vL = L.u[1]; wL = L.u[2]; pL = L.p[0]; rhoL = L.r[0]; cL = L.c[0];
+  } else {
+    uL = L.u[0]; vL = L.u[1]; wL = L.u[2]; pL = L.pm; rhoL = L.rho; cL = L.cF;
+  }
+  const double sL = uL - cL;
+  const double zL = rhoL * cL;
+  const double e = (MODEL == MODEL_EULER) ? 0. : eps;
+  double p = pL, f = 0., df = 1., u = 0., v = 0., vk[NPH];
+  int it = 0;
+  do {
+    p -= f / df;
+    if (++it > 50) { nonconv = true; break; }
+#pragma unroll
+    for (int k = 0; k < NPH; k++) p = eos_fix_p(S.eos[k], p);
+    u = uL + (pL - p) / zL;
+    if (u >= -1e-6) u = -1e-6;
+    const double du = -1. / zL;
+    v = u / m0;
+    f = v;
+    df = du / m0;
+#pragma unroll
+    for (int k = 0; k < NPH; k++) {
+      const double hk = H[k] - 0.5 * u * u;
+      vk[k] = eos_vfpfh(S.eos[k], p, hk, e);
+      const double dvk = eos_dvdpch(S.eos[k], p, hk, e) - eos_dvdhcp(S.eos[k], p, e) * u * du;
+      if constexpr (MODEL == MODEL_EULER) { f -= vk[k]; df -= dvk; }
+      else { f -= Y[k] * vk[k]; df -= Y[k] * dvk; }
+    }
+  } while (fabs(f) > 1e-10);
+  if constexpr (MODEL == MODEL_EULER) {
+    const double rhoStar = m0 / u;
+    const double eStar = eos_e(S.eos[0], rhoStar, p, 0.);
+    F.m[0] = rhoStar * u;
+    F.q[0] = rhoStar * u * u + p; F.q[1] = rhoStar * u * vL; F.q[2] = rhoStar * u * wL;
+    F.E = (rhoStar * (eStar + 0.5 * (u * u + vL * vL + wL * wL)) + p) * u;
+  } else {
+    double Estar = 0.5 * (u * u + vL * vL + wL * wL);
+#pragma unroll
+    for (int k = 0; k < NPH; k++) {
+      const double rhok = 1. / vk[k];
+      const double ek = eos_e(S.eos[k], rhok, p, eps);
+      Estar += Y[k] * ek;
+      F.a[k] = Y[k] * vk[k] / v * u;
+      F.m[k] = F.a[k] * rhok;
+      F.en[k] = F.a[k] * rhok * ek;
+    }
+    F.q[0] = u * u / v + p; F.q[1] = u * vL / v; F.q[2] = u * wL / v;
+    F.E = (Estar / v + p) * u;
+  }
+  F.sM = u;
+  return (fabs(sL) > 1.e-3) ? fabs(sL) : 0.;
+}
+
 // The wall's half Riemann problem in the face frame: ModKapila::solveRiemannWall
 // (ModKapila.cpp:224-249), ModMultiP::solveRiemannWall (ModMultiP.cpp:220-245),
 // ModEuler::solveRiemannWall (ModEuler.cpp:181-210).  Only the normal momentum

@@ -791,7 +791,8 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int slot, c
     own.E = Wn[Ly::IE * nc + cidx];
     // boundary faces (BoundCond::computeFlux, BoundCond.cpp:68-100): transmissive
     // (BoundCondAbs, the interior solver on (L, L)), wall (BoundCondWall /
-    // BoundCondWallO2) or outflow (BoundCondOutflow, the cell's own state)
+    // BoundCondWallO2), outflow (BoundCondOutflow) or injection (BoundCondInj),
+    // the last two on the cell's own state at both orders
 #pragma unroll
     for (int dir = 0; dir < DIM; dir++) {
 #pragma unroll
@@ -802,6 +803,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int slot, c
         load_prim<MODEL, NPH, DIM>(Wsrc, nc, cidx, L);
         const bool wall = WALL && ((S.bc_wall >> (2 * dir + side)) & 1);
         const bool outf = WALL && ((S.bc_out >> (2 * dir + side)) & 1);
+        const bool inj = WALL && ((S.bc_inj >> (2 * dir + side)) & 1);
         if constexpr (STAGE != 0 && WALL) {
           if (wall) wall_state_o2<MODEL, NPH, DIM>(S, G, Wsrc, Wn, nc, cidx, strd[dir], dir, side, L);
         }
@@ -810,6 +812,12 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int slot, c
         FluxVec<NPH> F;
         if (wall) smx = smax(smx, wall_flux<MODEL, NPH>(L, F));
         else if (outf) smx = smax(smx, outflow_flux<MODEL, NPH>(S, L, S.bc_p0[2 * dir + side], F));
+        else if (inj) {
+          const int b = 2 * dir + side;
+          bool nonconv = false;
+          smx = smax(smx, inflow_flux<MODEL, NPH>(S, L, S.inj_m0[b], S.inj_Y[b], S.inj_H[b], F, nonconv));
+          if (nonconv) atomicOr(&ctl->err, 4u);
+        }
         else smx = smax(smx, hllc<MODEL, NPH>(S, L, L, F));
         from_face_frame(dir, side == 0, F.q);
         add_face<MODEL, NPH>(acc, F, G.coefA[dir], false, own);
@@ -917,7 +925,7 @@ int launch_t(const StageArgs &A, int stage, hipStream_t st) {
   const int tgz = (A.geo.n[2] + Tile<DIM>::Z - 1) / Tile<DIM>::Z;
   dim3 grid((unsigned)(tgx * tgy * tgz)), block(NT);
   const bool same = A.sch.same_lim != 0;
-  const bool wall = A.sch.bc_wall != 0 || A.sch.bc_out != 0;  // kernels with the wall and outflow code
+  const bool wall = A.sch.bc_wall != 0 || A.sch.bc_out != 0 || A.sch.bc_inj != 0;  // kernels with boundary code
   if (stage == 0) hipLaunchKernelGGL((k_stage<MODEL, NPH, DIM, 0, true>), grid, block, 0, st, A);
   else if (stage == 1 && same && wall) hipLaunchKernelGGL((k_stage<MODEL, NPH, DIM, 1, true>), grid, block, 0, st, A);
   else if (stage == 1 && same) hipLaunchKernelGGL((k_stage<MODEL, NPH, DIM, 1, true, false>), grid, block, 0, st, A);

@@ -19,6 +19,7 @@ description of :class:`ecogen_amd.Context` and the initial state in the
   applied in deck order (``Cell::fill``, ``src/Cell.cpp:130-137``; the
   ``belong`` tests of ``src/Geometries/GD*.cpp``);
 * ``abs``, ``outflow`` (``BoundCondOutflow``, the exit pressure ``p0``),
+  ``injection`` (``BoundCondInj``, the mass flow ``m0`` and the injected fluid),
   ``wall`` and ``symmetry`` boundaries: ``BoundCondWall`` at first
   order for both wall and symmetry (``Input.cpp:849-852``), ``BoundCondWallO2``
   and ``BoundCondSymmetryO2`` at second order, which differ only in name
@@ -261,6 +262,7 @@ def load(rundir):
     states = {s.get("name"): s for s in ci.findall("state")}
     bcs = ["abs"] * 6
     p0s = [0.0] * 6
+    injs = [None] * 6
     bnode = ci.find("boundaryConditions")
     for b in (bnode.findall("boundCond") if bnode is not None else []):
         kind = b.get("type", "").lower()
@@ -276,10 +278,15 @@ def load(rundir):
         elif kind == "outflow":  # BoundCondOutflow.cpp:41-52: <dataOutflow p0="..."/>
             bcs[num - 1] = "outflow"
             p0s[num - 1] = _f(_child(b, "dataOutflow", "boundCond outflow"), "p0", "boundCond outflow")
+        elif kind == "injection":
+            bcs[num - 1] = "injection"
+            injs[num - 1] = _injection(b, case["eos"])
         else:
             raise DeckError(f"boundCond type {kind!r} is not read")
     case["bc"] = bcs
     case["bc_p0"] = p0s
+    if any(injs):
+        case["bc_inj"] = injs
 
     X, Y, Z = _centres(case["cells"], case["lengths"])
     Xg, Yg, Zg = X[None, None, :], Y[None, :, None], Z[:, None, None]
@@ -300,6 +307,35 @@ def load(rundir):
     if not filled.all():
         raise DeckError("initialConditionsV4.xml: the domains leave cells without a state")
     return case, grid.reshape(nf, -1)
+
+
+def _injection(b, eos):
+    """BoundCondInj's reader (BoundCondInj.cpp:17-90): <dataInjection m0>, then
+    one <dataFluid density pressure> for one phase, or per phase a <dataFluid
+    EOS alpha density pressure> under <fluidsProp>, the fractions in [0, 1]
+    and summing to 1 within 1e-6 (they are divided by their sum when lowered)."""
+    where = "boundCond injection"
+    m0 = _f(_child(b, "dataInjection", where), "m0", where)
+    nph = len(eos)
+    if nph == 1:
+        fl = _child(b, "dataFluid", where)
+        return {"m0": m0, "alpha": [1.0], "rho": [_f(fl, "density", where)], "p": [_f(fl, "pressure", where)]}
+    alpha, rho, p = [None] * nph, [None] * nph, [None] * nph
+    fluids = _child(b, "fluidsProp", where).findall("dataFluid")
+    names = [e.get("file") or f"{e['type']}_{e['name']}.xml" for e in eos]
+    for fl in fluids:
+        name = fl.get("EOS")
+        if name not in names:
+            raise DeckError(f"{where}: unknown EOS {name!r}")
+        k = names.index(name)
+        alpha[k], rho[k], p[k] = _f(fl, "alpha", where), _f(fl, "density", where), _f(fl, "pressure", where)
+    if len(fluids) != nph or None in alpha:
+        raise DeckError(f"{where}: {len(fluids)} fluids for {nph} phases")
+    if any(a < 0.0 or a > 1.0 for a in alpha):
+        raise DeckError(f"{where}: alpha should be in [0,1]")
+    if abs(sum(alpha) - 1.0) > 1e-6:
+        raise DeckError(f"{where}: sum of alpha should be 1")
+    return {"m0": m0, "alpha": alpha, "rho": rho, "p": p}
 
 
 def _run_with_output(ctx, case, outdir):

@@ -49,7 +49,8 @@ def grad_rho(case, rho):
     """|grad rho| of Cell::getGradient on a Cartesian grid: per interior face
     (cd - cg)/|xR - xL| times the normal, weighted by the centre distance along
     each axis; every boundary face (any type, for the density) adds twice the
-    centre-to-face distance to the weights only (an outflow face nothing)."""
+    centre-to-face distance to the weights only (outflow and injection faces
+    nothing)."""
     n = case["cells"]
     dim = sum(1 for c in n if c > 1)
     pos, dx = _positions(case)
@@ -82,9 +83,10 @@ def grad_rho(case, rho):
         wm[hi] = np.broadcast_to(dist, face.shape)
         w[d] = (w[d] + wm) + wp
     # boundary faces after the interior ones (X-, X+, Y-, ...): twice the
-    # centre-to-face distance, the X+ face lying at x_{n-1} + dX/2; an outflow
-    # face (whoAmI 3) has no branch there and adds nothing
+    # centre-to-face distance, the X+ face lying at x_{n-1} + dX/2; outflow and
+    # injection faces (whoAmI 3 and 4) have no branch there and add nothing
     bcs = case.get("bc", ["abs"] * 6)
+    silent = ("outflow", "injection")
     for d in range(dim):
         ax = axis_of[d]
         p = pos[d]
@@ -92,9 +94,9 @@ def grad_rho(case, rho):
         last = [slice(None)] * 3
         first[ax] = slice(0, 1)
         last[ax] = slice(-1, None)
-        if bcs[2 * d] != "outflow":
+        if bcs[2 * d] not in silent:
             w[d][tuple(first)] += np.abs(p[0] - 0.0) * 2.0
-        if bcs[2 * d + 1] != "outflow":
+        if bcs[2 * d + 1] not in silent:
             w[d][tuple(last)] += np.abs(p[-1] - (p[-1] + 0.5 * dx[d])) * 2.0
     comp = []
     for d in range(3):

@@ -36,8 +36,10 @@ enum eco_precision { ECO_FP64 = 0, ECO_FP32S = 1 };
    first order, which the reference also uses for a first-order SYMMETRY
    boundary; BoundCondWallO2 at second order; src/InputOutput/Input.cpp:849-852),
    outflow at an imposed pressure (BoundCondOutflow, at both orders; the
-   pressure in bc_p0).  Walls and outflows run in the FP64 path only. */
-enum eco_bc { ECO_BC_ABS = 0, ECO_BC_WALL = 1, ECO_BC_OUTFLOW = 2 };
+   pressure in bc_p0), injection at an imposed mass flow (BoundCondInj, at both
+   orders; the flow and the injected fluid in bc_inj_*; Input.cpp:853).  Walls,
+   outflows and injections run in the FP64 path only. */
+enum eco_bc { ECO_BC_ABS = 0, ECO_BC_WALL = 1, ECO_BC_OUTFLOW = 2, ECO_BC_INJECTION = 3 };
 
 enum eco_status_code {
   ECO_OK = 0,
@@ -51,6 +53,8 @@ enum eco_status_code {
 /* Soft-error flags (mirror the reference's non-fatal errors vector). */
 #define ECO_WARN_RELAX_NOT_CONVERGED 1u /* RelaxationP Newton hit 100 iterations (src/Relaxations/RelaxationP.cpp:92-109) */
 #define ECO_WARN_NAN 2u                 /* non-finite pressure or density produced */
+#define ECO_WARN_INFLOW_NOT_CONVERGED 4u /* injection Newton hit 50 iterations; the reference
+                                            stops there (src/Models/Kapila/ModKapila.cpp:283) */
 
 /* Equation of state (src/Eos/EosIG.h, src/Eos/EosSG.h; parameters of libEOS/*.xml). */
 typedef struct {
@@ -71,9 +75,14 @@ typedef struct {
   int n[3];         /* numberCells x, y, z */
   double L[3];      /* dimensions x, y, z (m) */
   double cfl;       /* computationControl CFL */
-  int bc[6];        /* X-, X+, Y-, Y+, Z-, Z+ : enum eco_bc (ECO_BC_WALL, ECO_BC_OUTFLOW: FP64 only) */
+  int bc[6];        /* X-, X+, Y-, Y+, Z-, Z+ : enum eco_bc (all but ECO_BC_ABS: FP64 only) */
   eco_eos eos[4];   /* per phase */
   double bc_p0[6];  /* <dataOutflow p0> of each ECO_BC_OUTFLOW side (Pa) */
+  /* each ECO_BC_INJECTION side (BoundCondInj.cpp:17-90): <dataInjection m0>,
+     the mass flow per unit area into the domain (kg/s/m^2, as in the deck),
+     and the injected fluid's alpha_k, rho_k, p_k (Euler: rho and p in [0]) */
+  double bc_inj_m0[6];
+  double bc_inj_alpha[6][4], bc_inj_rho[6][4], bc_inj_p[6][4];
 } eco_config;
 
 typedef struct eco_ctx eco_ctx;

@@ -133,6 +133,16 @@ def write_deck(case, root: Path):
         head = f'    <boundCond name="{names[i]}" type="{kinds[i]}" number="{i + 1}"'
         if kinds[i] == "outflow":
             return head + f'>\n      <dataOutflow p0="{p0s[i]!r}"/>\n    </boundCond>'
+        if kinds[i] == "injection":
+            inj = case["bc_inj"][i]
+            body = f'>\n      <dataInjection m0="{inj["m0"]!r}"/>\n'
+            if len(case["eos"]) == 1:
+                body += f'      <dataFluid density="{inj["rho"][0]!r}" pressure="{inj["p"][0]!r}"/>\n'
+            else:
+                body += "      <fluidsProp>\n" + "".join(
+                    f'        <dataFluid EOS="{_eos_file(e)}" alpha="{inj["alpha"][k]!r}" density="{inj["rho"][k]!r}" '
+                    f'pressure="{inj["p"][k]!r}"/>\n' for k, e in enumerate(case["eos"])) + "      </fluidsProp>\n"
+            return head + body + "    </boundCond>"
         return head + "/>"
     bcs = "\n".join(bc_xml(i) for i in range(2 * dim))
     states = "\n".join(_state_xml(case, st) for st in case["states"])

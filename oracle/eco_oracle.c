@@ -601,6 +601,117 @@ static void riemann_outflow(const Ctx *c, const State *L, double dxL, double *dt
   F->sM = uStar;
 }
 
+/* Eos{IG,SG}::vfpfh, dvdpch, dvdhcp (EosIG.cpp:220-239, EosSG.cpp:223-242):
+   specific volume from pressure and enthalpy and its two partial derivatives */
+static double eos_vfpfh(const orc_eos *q, double p, double h, double eps) {
+  double pp = (q->type == ORC_EOS_SG) ? q->gamma * (p + q->pinf) : q->gamma * p;
+  return (q->gamma - 1.) * (h - q->eref) / smax(pp, eps);
+}
+static double eos_dvdpch(const orc_eos *q, double p, double h, double eps) {
+  double pp = (q->type == ORC_EOS_SG) ? (p + q->pinf) * (p + q->pinf) : p * p;
+  return (1. - q->gamma) / q->gamma * (h - q->eref) / smax(pp, eps);
+}
+static double eos_dvdhcp(const orc_eos *q, double p, double eps) {
+  double pp = (q->type == ORC_EOS_SG) ? (p + q->pinf) : p;
+  return (q->gamma - 1.) / q->gamma / smax(pp, eps);
+}
+
+/* The injection's half Riemann problem in the face frame (BoundCondInj,
+   src/BoundConds/BoundCondInj.cpp:33-37, 190-193): a fluid of total enthalpy
+   H_k0 (Eos::computeTotalEnthalpy, Eos.cpp:99-102) enters at the mass flow m0
+   per unit area; the left wave is acoustic.  ModKapila::solveRiemannInflow
+   (ModKapila.cpp:253-322) and ModMultiP::solveRiemannInflow
+   (ModMultiP.cpp:249-318) are the same; ModEuler::solveRiemannInflow
+   (ModEuler.cpp:214-303) is their one-phase form.  m0 is the deck's value; the
+   reference changes its sign on reading.  Newton on the pressure from pL until
+   |f| <= 1e-10; the reference aborts after 50 iterations, here the 50th
+   iterate is kept and *nonconv set. */
+static void riemann_inflow(const Ctx *c, const State *L, double dxL, double *dtMax, int side, Flux *F, int *nonconv) {
+  const orc_cfg *cfg = c->cfg;
+  double eps = c->epsilon;
+  double m0 = -cfg->bc_m0[side];
+  const double *ak0 = cfg->bc_ak0[side], *rhok0 = cfg->bc_rhok0[side], *pk0 = cfg->bc_pk0[side];
+  if (cfg->model == ORC_EULER) {
+    const orc_eos *q = &cfg->eos[0];
+    const Phase *P = &L->ph[0];
+    double uL = P->u[0], vL = P->u[1], wL = P->u[2], pL = P->p, rhoL = P->rho, cL = P->c;
+    double zL = rhoL * cL;
+    double sL = uL - cL;
+    if (fabs(sL) > 1.e-3) *dtMax = smin(*dtMax, dxL / fabs(sL));
+    double u0 = m0 / rhok0[0];
+    double H0 = eos_e(q, rhok0[0], pk0[0], 0.) + pk0[0] / rhok0[0] + 0.5 * u0 * u0;
+    int it = 0;
+    double p = pL, f = 0., df = 1., u, du, v, dv, h;
+    do {
+      p -= f / df; it++;
+      if (it > 50) { *nonconv = 1; break; }
+      eos_fix_p(q, &p);
+      u = uL + (pL - p) / zL;
+      if (u >= -1e-6) u = -1e-6;
+      du = -1. / zL;
+      v = u / m0; dv = du / m0;
+      f = v; df = dv;
+      h = H0 - 0.5 * u * u;
+      v = eos_vfpfh(q, p, h, 0.);
+      dv = eos_dvdpch(q, p, h, 0.) - eos_dvdhcp(q, p, 0.) * u * du;
+      f -= v; df -= dv;
+    } while (fabs(f) > 1e-10);
+    double uStar = u, pStar = p, rhoStar = m0 / uStar;
+    double eStar = eos_e(q, rhoStar, pStar, 0.);
+    F->masse[0] = rhoStar * uStar;
+    F->qdm[0] = rhoStar * uStar * uStar + pStar; F->qdm[1] = rhoStar * uStar * vL; F->qdm[2] = rhoStar * uStar * wL;
+    F->energMix = (rhoStar * (eStar + 0.5 * (uStar * uStar + vL * vL + wL * wL)) + pStar) * uStar;
+    F->sM = uStar;
+    return;
+  }
+  int n = cfg->nph;
+  double uL = L->mx.u[0], cL = L->mx.cF, pL = L->mx.p, rhoL = L->mx.rho;
+  double vL = L->mx.u[1], wL = L->mx.u[2];
+  double rho0 = 0., Hk0[MAXPH], Yk0[MAXPH], vk[MAXPH], a0[MAXPH], sum = 0.;
+  for (int k = 0; k < n; k++) sum += ak0[k];                 /* BoundCondInj.cpp:82-86 */
+  for (int k = 0; k < n; k++) a0[k] = ak0[k] / sum;
+  for (int k = 0; k < n; k++) rho0 += a0[k] * rhok0[k];     /* Mix*::computeDensity */
+  double u0 = m0 / rho0;
+  for (int k = 0; k < n; k++) {
+    Hk0[k] = eos_e(&cfg->eos[k], rhok0[k], pk0[k], eps) + pk0[k] / rhok0[k] + 0.5 * u0 * u0;
+    Yk0[k] = a0[k] * rhok0[k] / rho0;
+  }
+  double sL = uL - cL;
+  if (fabs(sL) > 1.e-3) *dtMax = smin(*dtMax, dxL / fabs(sL));
+  double zL = rhoL * cL;
+  int it = 0;
+  double pStar = pL, f = 0., df = 1., u, du, v, dv, hk;
+  do {
+    pStar -= f / df; it++;
+    if (it > 50) { *nonconv = 1; break; }
+    for (int k = 0; k < n; k++) eos_fix_p(&cfg->eos[k], &pStar);
+    u = uL + (pL - pStar) / zL;
+    if (u >= -1e-6) u = -1e-6;
+    du = -1. / zL;
+    v = u / m0; dv = du / m0;
+    f = v; df = dv;
+    for (int k = 0; k < n; k++) {
+      hk = Hk0[k] - 0.5 * u * u;
+      vk[k] = eos_vfpfh(&cfg->eos[k], pStar, hk, eps);
+      double dvk = eos_dvdpch(&cfg->eos[k], pStar, hk, eps) - eos_dvdhcp(&cfg->eos[k], pStar, eps) * u * du;
+      f -= Yk0[k] * vk[k];
+      df -= Yk0[k] * dvk;
+    }
+  } while (fabs(f) > 1e-10);
+  double Estar = 0.5 * (u * u + vL * vL + wL * wL);
+  for (int k = 0; k < n; k++) {
+    double rhok = 1. / vk[k];
+    double ek = eos_e(&cfg->eos[k], rhok, pStar, eps);
+    Estar += Yk0[k] * ek;
+    F->alpha[k] = Yk0[k] * vk[k] / v * u;
+    F->masse[k] = F->alpha[k] * rhok;
+    F->energ[k] = F->alpha[k] * rhok * ek;
+  }
+  F->qdm[0] = u * u / v + pStar; F->qdm[1] = u * vL / v; F->qdm[2] = u * wL / v;
+  F->energMix = (Estar / v + pStar) * u;
+  F->sM = u;
+}
+
 /* The wall's half Riemann problem in the face frame: ModKapila::solveRiemannWall
    (ModKapila.cpp:224-249), ModMultiP::solveRiemannWall (ModMultiP.cpp:220-245),
    ModEuler::solveRiemannWall (ModEuler.cpp:181-210).  Only the normal
@@ -838,6 +949,7 @@ typedef struct {
   double *fslope;     /* per-face primitive slopes (nvar per face) */
   int nvar;
   double *fdt;        /* per-face dtMax contributions */
+  int inflow_nonconv; /* an injection Newton solve hit its 50 iterations */
 } Solver;
 
 /* Primitive slope vector: phases (alpha, rho, p) or Euler (rho, p, u, v, w); mixture (u, v, w). */
@@ -1054,6 +1166,14 @@ static void face_fluxes(Solver *S, State *st, int o2) {
       fulfill(c, &L);
       if (c->cfg->bc[F->side] == ORC_BC_WALL) riemann_wall(c, &L, lcfl, &dtm, FL);
       else if (c->cfg->bc[F->side] == ORC_BC_OUTFLOW) riemann_outflow(c, &L, lcfl, &dtm, c->cfg->bc_p0[F->side], FL);
+      else if (c->cfg->bc[F->side] == ORC_BC_INJ) {
+        int nc = 0;
+        riemann_inflow(c, &L, lcfl, &dtm, F->side, FL, &nc);
+        if (nc) {
+#pragma omp atomic write
+          S->inflow_nonconv = 1;
+        }
+      }
       else riemann(c, &L, &L, lcfl, lcfl, &dtm, FL);
     } else if (!o2) {
       /* CellInterface::solveRiemann (CellInterface.cpp:127-146): the cells' own cached state */
@@ -1296,7 +1416,7 @@ int orc_run(const orc_cfg *cfg, const double *state_in, double *state_out,
   if (cfg->model == ORC_EULER && cfg->nph != 1) return -1;
   if (is_thinc(cfg) && cfg->nph != 2) return -1;
   for (int b = 0; b < 6; b++) {
-    if (cfg->bc[b] != ORC_BC_ABS && cfg->bc[b] != ORC_BC_WALL && cfg->bc[b] != ORC_BC_OUTFLOW) return -1;
+    if (cfg->bc[b] < ORC_BC_ABS || cfg->bc[b] > ORC_BC_INJ) return -1;
 
   }
 #ifdef _OPENMP
@@ -1336,7 +1456,8 @@ int orc_run(const orc_cfg *cfg, const double *state_in, double *state_out,
   if (dt_next_out) *dt_next_out = dt;
   free(S.sn); free(S.s2); free(S.cons); free(S.fflux); free(S.fslope); free(S.fdt);
   mesh_free(&S.m);
-  return it;
+  /* the reference stops with "solveRiemannInflow not converged" */
+  return S.inflow_nonconv ? -3 : it;
 }
 
 /* ------------------------------------------------------------------------ */

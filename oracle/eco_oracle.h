@@ -29,7 +29,9 @@ enum { ORC_EOS_IG = 0, ORC_EOS_SG = 1 };
 /* boundary conditions: BoundCondAbs, and the wall: BoundCondWall at first
    order (also the reference's first-order SYMMETRY), BoundCondWallO2 at second
    order (src/InputOutput/Input.cpp:850-852) */
-enum { ORC_BC_ABS = 0, ORC_BC_WALL = 1, ORC_BC_OUTFLOW = 2 };  /* outflow: BoundCondOutflow, imposed p0 */
+enum { ORC_BC_ABS = 0, ORC_BC_WALL = 1, ORC_BC_OUTFLOW = 2, ORC_BC_INJ = 3 };
+/* outflow: BoundCondOutflow, imposed p0; injection: BoundCondInj, imposed mass
+   flow m0 of a fluid of given alpha_k, rho_k, p_k */
 
 typedef struct {
   int type;
@@ -49,6 +51,8 @@ typedef struct {
   orc_eos eos[ORC_MAXPH];
   int bc[6];         /* X-, X+, Y-, Y+, Z-, Z+: ORC_BC_ABS, ORC_BC_WALL or ORC_BC_OUTFLOW */
   double bc_p0[6];   /* outflow pressure per side */
+  double bc_m0[6];   /* injection: <dataInjection m0>, mass flow per unit area into the domain */
+  double bc_ak0[6][ORC_MAXPH], bc_rhok0[6][ORC_MAXPH], bc_pk0[6][ORC_MAXPH];  /* injected fluid per phase */
 } orc_cfg;
 
 /*
@@ -64,7 +68,9 @@ typedef struct {
  * steps; nsteps <= 0: time-controlled run to t_final (compared as float, as
  * the reference reads totalTime with QueryFloatAttribute).  dt_log (optional,
  * length >= steps taken) receives the dt used by each step.
- * Returns the number of steps taken, or a negative error code.
+ * Returns the number of steps taken, or a negative error code (-1 invalid
+ * configuration, -2 allocation, -3 an injection Newton solve did not converge
+ * in 50 iterations, where the reference stops).
  */
 int orc_run(const orc_cfg *cfg, const double *state_in, double *state_out,
             int nsteps, double t_final, int max_steps,

@@ -20,7 +20,9 @@ LIB_PATH = HERE / "libeco_oracle.so"
 MODELS = {"euler": 0, "kapila": 1, "multip": 2}
 LIMITERS = {"none": 0, "minmod": 1, "vanleer": 2, "vanalbada": 3, "superbee": 4, "mc": 5, "thinc": 6}
 EOS_TYPES = {"IG": 0, "SG": 1}
-BCS = {"abs": 0, "wall": 1, "outflow": 2}  # per side X-, X+, Y-, Y+, Z-, Z+; outflow pressures in case["bc_p0"]
+# per side X-, X+, Y-, Y+, Z-, Z+; outflow pressures in case["bc_p0"]; injections in
+# case["bc_inj"][side] = {"m0": mass flow into the domain, "alpha", "rho", "p": per phase}
+BCS = {"abs": 0, "wall": 1, "outflow": 2, "injection": 3}
 MAXPH = 4
 
 
@@ -33,7 +35,9 @@ class OrcCfg(C.Structure):
     _fields_ = [("model", C.c_int), ("nph", C.c_int), ("order", C.c_int), ("lim", C.c_int * 4),
                 ("alpha_null", C.c_int), ("relax_p", C.c_int), ("n", C.c_int * 3),
                 ("L", C.c_double * 3), ("cfl", C.c_double), ("eos", OrcEos * MAXPH), ("bc", C.c_int * 6),
-                ("bc_p0", C.c_double * 6)]
+                ("bc_p0", C.c_double * 6), ("bc_m0", C.c_double * 6),
+                ("bc_ak0", (C.c_double * MAXPH) * 6), ("bc_rhok0", (C.c_double * MAXPH) * 6),
+                ("bc_pk0", (C.c_double * MAXPH) * 6)]
 
 
 _lib = None
@@ -98,6 +102,13 @@ def make_cfg(case) -> OrcCfg:
         cfg.bc[b] = BCS[name]
     for b, p0 in enumerate(case.get("bc_p0", [0.0] * 6)):
         cfg.bc_p0[b] = p0
+    for b, inj in enumerate(case.get("bc_inj", [None] * 6)):
+        if inj:
+            cfg.bc_m0[b] = inj["m0"]
+            for k in range(cfg.nph):
+                cfg.bc_ak0[b][k] = inj["alpha"][k]
+                cfg.bc_rhok0[b][k] = inj["rho"][k]
+                cfg.bc_pk0[b][k] = inj["p"][k]
     return cfg
 
 

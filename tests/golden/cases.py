@@ -91,6 +91,22 @@ def _euler_1d(order, limiters, steps, cells=200):
     }
 
 
+def _inj(nph, m0, axis, x_side, mix=False, rho=None, p=None):
+    """bc_inj list with one injection (BoundCondInj) on side ``2*(axis-1)+x_side``
+    of axis 1..3: mass flow ``m0`` per unit area into the domain of air at
+    1e5 Pa (density 1.2, or 50 in the tubes' two-phase decks), or of an
+    air/water mixture (alpha 0.9/0.1) when ``mix``."""
+    if nph == 1:
+        inj = {"m0": m0, "alpha": [1.0], "rho": [rho or 1.2], "p": [p or 1e5]}
+    elif mix:
+        inj = {"m0": m0, "alpha": [0.9, 0.1], "rho": [50.0, 1000.0], "p": [1e5, 1e5]}
+    else:
+        inj = {"m0": m0, "alpha": [1.0 - 1e-6, 1e-6], "rho": [50.0 if axis == 1 else 1.2, 1000.0], "p": [1e5, 1e5]}
+    out = [None] * 6
+    out[2 * (axis - 1) + x_side] = inj
+    return out
+
+
 def mach_shock_air(mach, p1=1e5, rho1=1.2, gamma=1.4):
     """Post-shock state of a normal shock moving into air at rest (Rankine-Hugoniot)."""
     import math
@@ -194,4 +210,20 @@ CASES = {
                                 bc_p0=[2e7, 5e4] + [0.0] * 4),
     "kapila_2d_fo_outflow": dict(_shock_cylinders_2d(n=40, steps=80), order=1, limiters=["none"] * 4,
                                  bc=["abs", "outflow", "wall", "wall", "abs", "abs"], bc_p0=[0.0, 8e4] + [0.0] * 4),
+    # Injection boundaries (BoundCondInj, imposed mass flow of a given fluid; no
+    # second-order variant): air, or an air/water mixture, enters the tubes'
+    # low-pressure end, and air enters the 2-D box through its bottom face.
+    "kapila_1d_fo_injection": dict(_shock_tube_1d("kapila", 1, ["none"] * 4, 300, cells=200),
+                                   bc=["abs", "injection"] + ["abs"] * 4, bc_inj=_inj(2, 500.0, 1, x_side=1)),
+    "kapila_1d_o2_injection": dict(_shock_tube_1d("kapila", 2, ["vanleer"] * 4, 300, cells=200),
+                                   bc=["wall", "injection"] + ["abs"] * 4, bc_inj=_inj(2, 2000.0, 1, x_side=1, mix=True)),
+    "multip_1d_o2_relax_injection": dict(_shock_tube_1d("multip", 2, ["minmod"] * 4, 160, cells=100, relax=True),
+                                         bc=["abs", "injection"] + ["abs"] * 4, bc_inj=_inj(2, 800.0, 1, x_side=1)),
+    "euler_1d_fo_injection": dict(_euler_1d(1, ["none"] * 4, 300), bc=["abs", "injection"] + ["abs"] * 4,
+                                  bc_inj=_inj(1, 150.0, 1, x_side=1)),
+    "euler_1d_o2_injection": dict(_euler_1d(2, ["vanleer"] * 4, 300), bc=["injection", "outflow"] + ["abs"] * 4,
+                                  bc_p0=[0.0, 5e4] + [0.0] * 4, bc_inj=_inj(1, 3000.0, 1, x_side=0, rho=80.0, p=1e7)),
+    "kapila_2d_fo_injection": dict(_shock_cylinders_2d(n=40, steps=80), order=1, limiters=["none"] * 4,
+                                   bc=["abs", "outflow", "injection", "wall", "abs", "abs"],
+                                   bc_p0=[0.0, 8e4] + [0.0] * 4, bc_inj=_inj(2, 300.0, 2, x_side=0)),
 }

@@ -97,3 +97,25 @@ def test_riemann_case_t_final():
     assert abs(tf - 0.3 / c_w) / tf < 1e-3  # water mass fraction ~1 - 5e-8
     assert st.shape == (9, 1000)
     assert st[2][0] == 1e7 and st[2][-1] == 1e5
+
+
+def test_injection_configs_are_validated_without_device(lib):
+    """eco_create checks a configuration before any HIP call: an injection
+    side needs a nonzero flow, positive densities and fractions summing to 1
+    (BoundCondInj.cpp:30-90), and runs in FP64 only."""
+    import ctypes as C
+    from ecogen_amd import capi
+    from cases import CASES
+    case = CASES["kapila_1d_o2_injection"]
+
+    def create(c):
+        ctx = C.c_void_p()
+        return lib.eco_create(C.byref(capi.make_config(c)), None, C.byref(ctx))
+    inj = case["bc_inj"][1]
+    assert create(dict(case, precision="fp32s")) == -2
+    bad_sum = [None, dict(inj, alpha=[0.5, 0.4])] + [None] * 4
+    assert create(dict(case, bc_inj=bad_sum)) == -1
+    no_flow = [None, dict(inj, m0=0.0)] + [None] * 4
+    assert create(dict(case, bc_inj=no_flow)) == -1
+    no_rho = [None, dict(inj, rho=[50.0, 0.0])] + [None] * 4
+    assert create(dict(case, bc_inj=no_rho)) == -1

@@ -25,6 +25,11 @@ def test_deck_reads_back_the_reference_inputs(name, tmp_path):
     assert got["relax_p"] == bool(case.get("relax_p", False) and case["model"] == "multip")
     dim = sum(1 for c in case["cells"] if c > 1)
     assert got["bc"][:2 * dim] == case.get("bc", ["abs"] * 6)[:2 * dim]
+    for i, kind in enumerate(got["bc"][:2 * dim]):
+        if kind == "outflow":
+            assert got["bc_p0"][i] == case["bc_p0"][i]
+        if kind == "injection":
+            assert got["bc_inj"][i] == case["bc_inj"][i]
     for a, b in zip(got["eos"], case["eos"]):
         for k in ("type", "gamma", "cv"):
             assert a[k] == b[k]
@@ -43,6 +48,10 @@ def test_deck_refusals(tmp_path):
         deck.load(tmp_path / "a")
     write_deck(dict(case, name="x", bc=["symmetry"] * 6), tmp_path / "b")
     assert deck.load(tmp_path / "b")[0]["bc"][:2] == ["wall"] * 2  # a 1-D deck names X- and X+ only
+    inj = {"m0": 10.0, "alpha": [0.5, 0.4], "rho": [1.0, 1000.0], "p": [1e5, 1e5]}
+    write_deck(dict(case, name="x", bc=["injection"] * 6, bc_inj=[inj] * 6), tmp_path / "c")
+    with pytest.raises(deck.DeckError, match="sum of alpha"):
+        deck.load(tmp_path / "c")
     (tmp_path / "a" / "case" / "meshV5.xml").write_text(
         '<?xml version="1.0"?><mesh><type structure="unstructured"/></mesh>')
     with pytest.raises(deck.DeckError, match="cartesian"):
@@ -50,7 +59,8 @@ def test_deck_refusals(tmp_path):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", ["c1_kapila_fo_wall", "kapila_2d_fo_wall", "multip_1d_o2_relax"])
+@pytest.mark.parametrize("name", ["c1_kapila_fo_wall", "kapila_2d_fo_wall", "multip_1d_o2_relax",
+                                  "kapila_1d_o2_injection"])
 def test_deck_runs_on_the_device(name, tmp_path):
     """python -m ecogen_amd.deck on a reference deck: the final state matches
     the reference's output (1e-12 relative L1, the parity bar)."""

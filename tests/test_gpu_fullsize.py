@@ -108,3 +108,30 @@ def test_c3_full_size_conservation():
     assert np.all(err_m <= TOL_CONS), err_m
     assert err_e <= TOL_CONS, err_e
     assert np.all(np.abs(q1) <= TOL_ZERO * aq1)
+
+
+def test_c2_full_size_injection_mass_inflow():
+    """4096^2, first order: air enters the closed box through its bottom face
+    (BoundCondInj, m0 = 300 kg/s/m^2).  The injection face's mass flux is m0
+    up to round-off and every other side is a wall, so phase k's mass grows by
+    Y_k0 m0 S t exactly (S = 1 m^2, t the time marched)."""
+    from ecogen_amd import Context
+    from ecogen_amd import workloads as W
+    case, st = W.shock_cylinders_2d(n=4096, steps=10)
+    inj = {"m0": 300.0, "alpha": [1.0 - 1e-6, 1e-6], "rho": [1.2, 1000.0], "p": [1e5, 1e5]}
+    case = dict(case, order=1, limiters=["none"] * 4, bc=["wall", "wall", "injection", "wall", "abs", "abs"],
+                bc_inj=[None, None, inj, None, None, None])
+    print("case shock_cylinders_2d 4096^2 fp64 first order, injection at y-, walls elsewhere, 10 steps", flush=True)
+    vol = (1.0 / 4096) ** 2
+    m_start = [float((st[3 * k] * st[3 * k + 1]).sum()) * vol for k in range(2)]
+    with Context(case) as ctx:
+        ctx.set_state(st)
+        s = ctx.run(nsteps=10)
+        assert s.steps == 10 and s.flags == 0
+        out = ctx.get_state()
+    rho0 = inj["alpha"][0] * inj["rho"][0] + inj["alpha"][1] * inj["rho"][1]
+    for k in range(2):
+        m_end = float((out[3 * k] * out[3 * k + 1]).sum()) * vol
+        want = inj["alpha"][k] * inj["rho"][k] / rho0 * inj["m0"] * 1.0 * s.t
+        print(f"  phase {k}: mass gained {m_end - m_start[k]:.6e}, m0 Y S t {want:.6e}", flush=True)
+        assert abs((m_end - m_start[k]) - want) <= 1e-12 * m_end + 1e-9 * want

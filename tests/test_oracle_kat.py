@@ -421,3 +421,37 @@ def test_walls_close_the_box(model):
         rest[2] = 1e5
     c, _, _, _, _ = O.run(case, rest, nsteps=50)
     assert np.allclose(c[:st.shape[0]], rest, rtol=2e-16, atol=0.0)  # the relaxation may move p by an ulp
+
+
+@pytest.mark.parametrize("name", ["kapila_1d_fo_injection", "kapila_1d_o2_injection",
+                                  "multip_1d_o2_relax_injection", "euler_1d_fo_injection",
+                                  "kapila_2d_fo_injection"])
+def test_injection_mass_inflow(name):
+    """An injection face (BoundCondInj) carries the mass flux u*/v* with
+    v* = u*/m0, that is m0 itself up to round-off whatever the Newton iterate,
+    split between the phases by the injected mass fractions Y_k0 =
+    alpha_k0 rho_k0 / rho_0.  With every other side closed by a wall, phase
+    k's mass therefore grows by exactly Y_k0 m0 S t (S the face area, t the
+    time marched, the first 1e-15 step included).  The reference's own
+    injection goldens pin the energy and momentum parts of the flux."""
+    from conftest import load_golden
+    case, d = load_golden(name)
+    case["bc"] = ["injection" if b == "injection" else "wall" for b in case["bc"]]
+    side = case["bc"].index("injection")
+    inj = case["bc_inj"][side]
+    n, L = case["cells"], case["lengths"]
+    vol = (L[0] / n[0]) * (L[1] / n[1]) * (L[2] / n[2])
+    axis = side // 2
+    area = float(np.prod([L[a] for a in range(3) if a != axis]))  # the whole side
+    nph = len(case["eos"])
+    st = O.state_from_golden(case, d)
+    out, it, t, _, _ = O.run(case, st)
+    assert it == case["steps"]
+    a0 = np.array(inj["alpha"]) / sum(inj["alpha"])
+    rho0 = float(np.dot(a0, inj["rho"]))
+    for k in range(nph):
+        y = a0[k] * inj["rho"][k] / rho0
+        mk = (lambda s: (s[3 * k] * s[3 * k + 1]).sum()) if nph > 1 else (lambda s: s[1].sum())
+        m_start, m_end = mk(st) * vol, mk(out) * vol
+        want = y * inj["m0"] * area * t
+        assert abs((m_end - m_start) - want) <= 1e-12 * m_end + 1e-10 * want, (k, m_end - m_start, want)
