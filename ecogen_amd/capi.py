@@ -19,8 +19,9 @@ EOS_TYPE = {"IG": 0, "SG": 1}
 PRECISION = {"fp64": 0, "fp32s": 1}
 # per side X-, X+, Y-, Y+, Z-, Z+; all but abs: FP64 only.  Outflow pressures in
 # case["bc_p0"]; injections in case["bc_inj"][side] = {"m0": mass flow into the
-# domain, "alpha", "rho", "p": the injected fluid per phase}
-BC = {"abs": 0, "wall": 1, "outflow": 2, "injection": 3}
+# domain, "alpha", "rho", "p": the injected fluid per phase}; tanks in
+# case["bc_tank"][side] = {"p0", "T0", and per phase "alpha" or "massFrac"}
+BC = {"abs": 0, "wall": 1, "outflow": 2, "injection": 3, "tank": 4}
 
 WARN_RELAX_NOT_CONVERGED = 1
 WARN_NAN = 2
@@ -41,7 +42,9 @@ class EcoConfig(C.Structure):
                 ("L", C.c_double * 3), ("cfl", C.c_double), ("bc", C.c_int * 6), ("eos", EcoEos * 4),
                 ("bc_p0", C.c_double * 6), ("bc_inj_m0", C.c_double * 6),
                 ("bc_inj_alpha", (C.c_double * 4) * 6), ("bc_inj_rho", (C.c_double * 4) * 6),
-                ("bc_inj_p", (C.c_double * 4) * 6)]
+                ("bc_inj_p", (C.c_double * 4) * 6), ("bc_tank_p0", C.c_double * 6),
+                ("bc_tank_T0", C.c_double * 6), ("bc_tank_frac", (C.c_double * 4) * 6),
+                ("bc_tank_massfrac", C.c_int * 6)]
 
 
 class EcoStatus(C.Structure):
@@ -137,6 +140,12 @@ def make_config(case) -> EcoConfig:
                 c.bc_inj_alpha[b][k] = inj["alpha"][k]
                 c.bc_inj_rho[b][k] = inj["rho"][k]
                 c.bc_inj_p[b][k] = inj["p"][k]
+    for b, tk in enumerate(case.get("bc_tank") or [None] * 6):
+        if tk:
+            c.bc_tank_p0[b], c.bc_tank_T0[b] = tk["p0"], tk["T0"]
+            c.bc_tank_massfrac[b] = int("massFrac" in tk)
+            for k, v in enumerate(tk.get("massFrac") or tk.get("alpha") or [1.0]):
+                c.bc_tank_frac[b][k] = v
     for k, e in enumerate(case["eos"]):
         c.eos[k] = EcoEos(EOS_TYPE[e["type"]], e["gamma"], e.get("pinf", 0.0), e["cv"], e.get("eref", 0.0))
     return c

@@ -75,7 +75,11 @@ eco_config lowerConfig(const Model &model, const std::vector<std::unique_ptr<Eos
       c.bc_inj_alpha[b][k] = mesh.injection[b].alpha[k];
       c.bc_inj_rho[b][k] = mesh.injection[b].density[k];
       c.bc_inj_p[b][k] = mesh.injection[b].pressure[k];
+      c.bc_tank_frac[b][k] = mesh.tank[b].fraction[k];
     }
+    c.bc_tank_p0[b] = mesh.tank[b].p0;
+    c.bc_tank_T0[b] = mesh.tank[b].T0;
+    c.bc_tank_massfrac[b] = mesh.tank[b].massFractions ? 1 : 0;
   }
   for (size_t k = 0; k < eos.size() && k < 4; k++) c.eos[k] = eos[k]->lower();
   return c;

@@ -73,10 +73,60 @@ static void lower_eos(const eco_eos &e, EosParams &q) {
   q.pfloor = (e.type == ECO_EOS_SG) ? -(1. - 1.e-15) * e.pinf + 1.e-15 : 1.e-15;
 }
 
+// The constants of one tank side: rho_k0 = rho_k(p0, T0) (Eos*::computeDensity,
+// EosIG.cpp:111-114, EosSG.cpp:114-117), the fractions divided by their sum,
+// volume fractions from mass fractions through 1/rho = sum Y_k/rho_k0
+// (BoundCondTank.cpp:46-176), then, as the tank branch of
+// Mod{Kapila,MultiP}::solveRiemannTank computes them (ModKapila.cpp:386-390),
+// Y_k0 = alpha_k0 rho_k0 / rho0 and H0 = sum Y_k0 (e_k + p0/rho_k0) of the
+// fluid at rest; ModEuler::solveRiemannTank (ModEuler.cpp:358-360): H0 of
+// the one phase.
+static void lower_tank(const eco_config &c, int b, SchemeParams &S) {
+  const double p0 = c.bc_tank_p0[b], T0 = c.bc_tank_T0[b];
+  const double eps = S.eps;
+  const int n = c.nphase;
+  double rk[4], ak[4];
+  for (int k = 0; k < n; k++) {
+    const eco_eos &e = c.eos[k];
+    const double num = (e.type == ECO_EOS_SG) ? p0 + e.pinf : p0;
+    rk[k] = num / std::max((e.gamma - 1.) * e.cv * T0, eps);
+  }
+  auto energy = [&](const eco_eos &e, double rho, double p) {  // Eos{IG,SG}::computeEnergy
+    const double num = (e.type == ECO_EOS_SG) ? p + e.gamma * e.pinf : p;
+    return num / (e.gamma - 1.) / std::max(rho, eps) + e.eref;
+  };
+  S.tank_p0[b] = p0;
+  if (c.model == ECO_MODEL_EULER) {
+    S.tank_rho[b][0] = rk[0];
+    S.tank_Y[b][0] = 1.;
+    S.tank_H0[b] = energy(c.eos[0], rk[0], p0) + p0 / rk[0] + 0.5 * 0. * 0.;
+    return;
+  }
+  double fr[4], sum = 0.;
+  for (int k = 0; k < n; k++) sum += c.bc_tank_frac[b][k];
+  for (int k = 0; k < n; k++) fr[k] = c.bc_tank_frac[b][k] / sum;
+  if (!c.bc_tank_massfrac[b]) {
+    for (int k = 0; k < n; k++) ak[k] = fr[k];
+  } else {
+    double rm = 0.;
+    for (int k = 0; k < n; k++) rm += fr[k] / rk[k];
+    rm = 1.0 / rm;
+    for (int k = 0; k < n; k++) ak[k] = rm * fr[k] / rk[k];
+  }
+  double rho0 = 0., H0 = 0.;
+  for (int k = 0; k < n; k++) rho0 += ak[k] * rk[k];
+  for (int k = 0; k < n && k < kMaxPhases; k++) {
+    S.tank_rho[b][k] = rk[k];
+    S.tank_Y[b][k] = ak[k] * rk[k] / rho0;
+    H0 += S.tank_Y[b][k] * (energy(c.eos[k], rk[k], p0) + p0 / rk[k] + 0.5 * 0. * 0.);
+  }
+  S.tank_H0[b] = H0;
+}
+
 // The constants of one injection side, as the first lines of
 // Mod{Kapila,MultiP}::solveRiemannInflow (ModKapila.cpp:259-266) and
-// ModEuler::solveRiemannInflow (ModEuler.cpp:238-239) compute them on each
-// call: the signed flow (BoundCondInj.cpp:37), the injected density
+// ModEuler::solveRiemannInflow (ModEuler.cpp:240-241) compute them on each
+// call: the signed flow (BoundCondInj.cpp:62), the injected density
 // (Mix*::computeDensity), Y_k0 and H_k0 = e_k(rho_k0, p_k0) + p_k0/rho_k0 + u0^2/2
 // (Eos::computeTotalEnthalpy, Eos.cpp:99-102).
 static void lower_injection(const eco_config &c, int b, SchemeParams &S) {
@@ -94,7 +144,7 @@ static void lower_injection(const eco_config &c, int b, SchemeParams &S) {
     S.inj_H[b][0] = energy(c.eos[0], rho, p) + p / rho + 0.5 * u0 * u0;
     return;
   }
-  // the reader divides the fractions by their sum (BoundCondInj.cpp:82-86)
+  // the reader divides the fractions by their sum (BoundCondInj.cpp:108-117)
   double ak0[4], sum = 0., rho0 = 0.;
   for (int k = 0; k < c.nphase; k++) sum += c.bc_inj_alpha[b][k];
   for (int k = 0; k < c.nphase; k++) ak0[k] = c.bc_inj_alpha[b][k] / sum;
@@ -120,9 +170,21 @@ int eco::validate_config(const eco_config *c) {
     for (int i = 0; i < 4; i++) if (c->lim[i] == ECO_LIM_NONE) return ECO_ERR_INVALID;
   }
   for (int i = 0; i < 6; i++) {
-    if (c->bc[i] < ECO_BC_ABS || c->bc[i] > ECO_BC_INJECTION) return ECO_ERR_INVALID;
+    if (c->bc[i] < ECO_BC_ABS || c->bc[i] > ECO_BC_TANK) return ECO_ERR_INVALID;
+    if (c->bc[i] == ECO_BC_TANK) {
+      // BoundCondTank.cpp:46-144: p0 and T0; per phase a fraction in [0, 1], summing to 1
+      if (!(c->bc_tank_p0[i] > 0.) || !(c->bc_tank_T0[i] > 0.)) return ECO_ERR_INVALID;
+      if (c->model != ECO_MODEL_EULER) {
+        double sum = 0.;
+        for (int k = 0; k < c->nphase && k < 4; k++) {
+          if (!(c->bc_tank_frac[i][k] >= 0. && c->bc_tank_frac[i][k] <= 1.)) return ECO_ERR_INVALID;
+          sum += c->bc_tank_frac[i][k];
+        }
+        if (!(std::fabs(sum - 1.) <= 1.e-6)) return ECO_ERR_INVALID;
+      }
+    }
     if (c->bc[i] == ECO_BC_INJECTION) {
-      // BoundCondInj.cpp:30-90 reads m0 and, per phase, alpha in [0, 1] summing to 1
+      // BoundCondInj.cpp:46-150 reads m0 and, per phase, alpha in [0, 1] summing to 1
       if (!(c->bc_inj_m0[i] != 0.)) return ECO_ERR_INVALID;
       const int np = (c->model == ECO_MODEL_EULER) ? 1 : c->nphase;
       double sum = 0.;
@@ -216,14 +278,17 @@ void eco::lower_config(const eco_config &c, Lowered &L) {
   S.bc_wall = 0;
   S.bc_out = 0;
   S.bc_inj = 0;
+  S.bc_tank = 0;
   for (int b = 0; b < 6; b++) {
     if (c.bc[b] == ECO_BC_WALL) S.bc_wall |= 1 << b;
     if (c.bc[b] == ECO_BC_OUTFLOW) S.bc_out |= 1 << b;
     if (c.bc[b] == ECO_BC_INJECTION) S.bc_inj |= 1 << b;
+    if (c.bc[b] == ECO_BC_TANK) S.bc_tank |= 1 << b;
     S.bc_p0[b] = c.bc_p0[b];
   }
   for (int k = 0; k < c.nphase && k < kMaxPhases; k++) lower_eos(c.eos[k], S.eos[k]);
   for (int b = 0; b < 6; b++) if (c.bc[b] == ECO_BC_INJECTION) lower_injection(c, b, S);
+  for (int b = 0; b < 6; b++) if (c.bc[b] == ECO_BC_TANK) lower_tank(c, b, S);
   L.nprim = (c.model == ECO_MODEL_EULER) ? 2 + dim : 3 * c.nphase + dim;
   L.es = (c.precision == ECO_FP32S) ? sizeof(float) : sizeof(double);
 }

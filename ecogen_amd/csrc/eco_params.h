@@ -62,6 +62,14 @@ struct SchemeParams {
   double inj_m0[6];
   double inj_Y[6][kMaxPhases];
   double inj_H[6][kMaxPhases];
+  int bc_tank;       // bit 2*dir+side set: tank boundary (BoundCondTank)
+  // tank constants per side (BoundCondTank.cpp:46-176 and the tank branch of
+  // Mod*::solveRiemannTank): pressure, phase densities rho_k(p0, T0), mass
+  // fractions Y_k0 and the total enthalpy H0 of the fluid at rest
+  double tank_p0[6];
+  double tank_rho[6][kMaxPhases];
+  double tank_Y[6][kMaxPhases];
+  double tank_H0[6];
   EosParams eos[kMaxPhases];
 };
 

@@ -537,9 +537,9 @@ ECO_D double eos_dvdhcp(const EosParams &q, double p, double eps) {
 // The injection's half Riemann problem in the face frame (BoundCondInj): a
 // fluid of phase total enthalpies H_k0 and mass fractions Y_k0 enters at the
 // mass flow m0 per unit area (negative: into the domain, the reference's sign
-// after reading, BoundCondInj.cpp:33-37); the left wave is acoustic.
+// after reading, BoundCondInj.cpp:62); the left wave is acoustic.
 // ModKapila::solveRiemannInflow (ModKapila.cpp:253-322) == ModMultiP
-// (ModMultiP.cpp:249-318); ModEuler::solveRiemannInflow (ModEuler.cpp:214-303)
+// (ModMultiP.cpp:249-318); ModEuler::solveRiemannInflow (ModEuler.cpp:215-303)
 // is the one-phase form with H0 in H[0].  Newton on the pressure from pL until
 // |f| <= 1e-10; where the reference stops after 50 iterations, the 50th
 // iterate is kept and `nonconv` set.  Returns |sL| above the 1e-3 guard, else 0.
@@ -602,6 +602,198 @@ ECO_D double inflow_flux(const SchemeParams &S, const CellState<NPH> &L, double 
   }
   F.sM = u;
   return (fabs(sL) > 1.e-3) ? fabs(sL) : 0.;
+}
+
+// computeDensityIsentropic / computeDensityHugoniot with drho/dp
+// (EosIG.cpp:146-162, EosSG.cpp:149-165) and computeEnthalpyIsentropic with
+// dh/dp (EosIG.cpp:177-184, EosSG.cpp:180-187).
+ECO_D double eos_rho_isentropic_d(const EosParams &q, double p0, double rho0, double pf, double eps, double &drho) {
+  const double rf = eos_rho_isentropic(q, p0, rho0, pf, eps);
+  drho = rf / smax(q.gamma * (pf + q.pinf), eps);
+  return rf;
+}
+ECO_D double eos_rho_hugoniot_d(const EosParams &q, double p0, double rho0, double pf, double eps, double &drho) {
+  const double denom = (q.gamma - 1.) * (pf + q.pinf) + (q.gamma + 1.) * (p0 + q.pinf);
+  drho = rho0 * 4. * q.gamma * (p0 + q.pinf) / smax(denom * denom, eps);
+  return eos_rho_hugoniot(q, p0, rho0, pf, eps);
+}
+ECO_D double eos_h_isentropic(const EosParams &q, double p0, double rho0, double pf, double eps, double &dh) {
+  double drho;
+  const double rf = eos_rho_isentropic_d(q, p0, rho0, pf, eps, drho);
+  const double pp = pf + q.pinf;
+  dh = q.gamma / q.gm1 * (rf - pp * drho) / smax(rf * rf, eps);
+  return q.gamma * pp / q.gm1 / smax(rf, eps) + q.eref;
+}
+
+// The tank's half Riemann problem in the face frame (BoundCondTank): from the
+// left wave at p* = p0, supersonic outflow keeps the left state and subsonic
+// outflow the left star state at p0; otherwise fluid flows in from the tank
+// (at rest, total enthalpy H0, isentropic) and a Newton solve on p* matches
+// the tank-side velocity to the left star velocity.
+// ModKapila::solveRiemannTank (ModKapila.cpp:326-475: Hugoniot left wave,
+// |f| <= 1e-2), ModMultiP::solveRiemannTank (ModMultiP.cpp:322-465: |f| <=
+// 1e-3, no epsilon guards, the first estimate's sM as the contact speed),
+// ModEuler::solveRiemannTank (ModEuler.cpp:305-423: isentropic left wave,
+// |f| <= 1e-3).  Every iterate's left wave enters the dt bound; where the
+// reference stops after 50 iterations, the last iterate is kept and `nonconv`
+// set.  The tank's rho_k0, Y_k0 and H0 come from the host.  Returns the
+// largest |sL| above the 1e-3 guard, else 0.
+template <int MODEL, int NPH>
+ECO_D double tank_flux(const SchemeParams &S, const CellState<NPH> &L, double p0, const double *rhok0,
+                       const double *Yk0, double H0, FluxVec<NPH> &F, bool &nonconv) {
+#pragma unroll
+  for (int k = 0; k < NPH; k++) { F.a[k] = 0.; F.m[k] = 0.; F.en[k] = 0.; }
+  double smx = 0.;
+  auto wave = [&](double sL) { if (fabs(sL) > 1.e-3) smx = smax(smx, fabs(sL)); };
+  if constexpr (MODEL == MODEL_EULER) {
+    const EosParams &q = S.eos[0];
+    const double uL = L.u[0], vL = L.u[1], wL = L.u[2], pL = L.p[0], rhoL = L.r[0], cL = L.c[0];
+    const double zL = rhoL * cL;
+    double uStar, rhoStar, pStar = p0, vStar, wStar, mL, drho;
+    double v = 1. / eos_rho_isentropic(q, pL, rhoL, pStar, 0.);
+    double vmv0 = v - 1. / rhoL;
+    mL = (fabs(vmv0) > 1e-10) ? sqrt((pL - p0) / vmv0) : zL;
+    double sL = uL - mL / rhoL;
+    wave(sL);
+    const double u = uL + mL * vmv0;
+    if (sL >= 0.) { uStar = uL; pStar = pL; rhoStar = rhoL; vStar = vL; wStar = wL; }
+    else if (u >= -1e-3) { uStar = u; pStar = p0; rhoStar = 1. / v; vStar = vL; wStar = wL; }
+    else {
+      int it = 0;
+      double p = 0.5 * p0, f = 0., df = 1., dh;
+      double uStarR = 0., uStarL = 0.;
+      rhoStar = 0.;
+      do {
+        p -= f / df;
+        if (++it > 50) { nonconv = true; break; }
+        p = eos_fix_p(q, p);
+        if (p > p0) p = p0 - 1e-6;
+        rhoStar = eos_rho_isentropic(q, p0, rhok0[0], p, 0.);
+        const double h = eos_h_isentropic(q, p0, rhok0[0], p, 0., dh);
+        uStarR = -sqrt(2. * (H0 - h));
+        const double duStarR = -dh / uStarR;
+        v = 1.0 / eos_rho_isentropic_d(q, pL, rhoL, p, 0., drho);
+        const double dv = -v * v * drho;
+        vmv0 = v - 1. / rhoL;
+        double dmL = 0.;
+        if (fabs(vmv0) > 1e-10) {
+          mL = sqrt((pL - p) / vmv0);
+          dmL = 0.5 * (-vmv0 + (p - pL) * dv) / (vmv0 * vmv0) / mL;
+        } else {
+          mL = zL;
+        }
+        sL = uL - mL / rhoL;
+        wave(sL);
+        uStarL = uL + mL * vmv0;
+        const double duStarL = dmL * vmv0 + mL * dv;
+        f = uStarR - uStarL;
+        df = duStarR - duStarL;
+      } while (fabs(f) > 1e-3);
+      pStar = p;
+      uStar = 0.5 * (uStarL + uStarR);
+      vStar = 0.; wStar = 0.;
+    }
+    const double eStar = eos_e(q, rhoStar, pStar, 0.);
+    F.m[0] = rhoStar * uStar;
+    F.q[0] = rhoStar * uStar * uStar + pStar; F.q[1] = rhoStar * uStar * vStar; F.q[2] = rhoStar * uStar * wStar;
+    F.E = (rhoStar * (eStar + 0.5 * (uStar * uStar + vStar * vStar + wStar * wStar)) + pStar) * uStar;
+    F.sM = uStar;
+  } else {
+    constexpr bool KAP = (MODEL == MODEL_KAPILA);
+    const double eps = S.eps;
+    auto gd = [&](double x) { return KAP ? smax(x, eps) : x; };  // the Kapila form's epsilon guards
+    const double uL = L.u[0], cL = L.cF, pL = L.pm, rhoL = L.rho;
+    const double uyL = L.u[1], uzL = L.u[2];
+    const double zL = rhoL * cL;
+    double rkS[NPH], YkS[NPH], pStar = p0, vStar = 0., uStar, rhoStar, uyStar, uzStar, mL;
+#pragma unroll
+    for (int k = 0; k < NPH; k++) {
+      rkS[k] = eos_rho_hugoniot(S.eos[k], L.p[k], L.r[k], pStar, eps);
+      vStar += L.a[k] * L.r[k] / rhoL / gd(rkS[k]);
+    }
+    double vmv0 = vStar - 1. / rhoL;
+    mL = (fabs(vmv0) > 1e-10) ? sqrt((pL - pStar) / vmv0) : zL;
+    double sL = uL - mL / rhoL;
+    wave(sL);
+    const double sM = uL + mL * vmv0;
+    if (sL >= 0.) {
+      uStar = uL; pStar = pL;
+#pragma unroll
+      for (int k = 0; k < NPH; k++) { rkS[k] = L.r[k]; YkS[k] = L.a[k] * L.r[k] / rhoL; }
+      rhoStar = rhoL; uyStar = uyL; uzStar = uzL;
+    } else if (sM >= -1e-3) {
+      uStar = sM; pStar = p0;
+#pragma unroll
+      for (int k = 0; k < NPH; k++) YkS[k] = L.a[k] * L.r[k] / rhoL;
+      rhoStar = 1. / vStar; uyStar = uyL; uzStar = uzL;
+    } else {
+      int it = 0;
+      double p = 0.5 * p0, f = 0., df = 1.;
+      double uStarR = 0., uStarL = 0.;
+      do {
+        p -= f / df;
+        if (++it > 50) { nonconv = true; break; }
+#pragma unroll
+        for (int k = 0; k < NPH; k++) p = eos_fix_p(S.eos[k], p);
+        if (p > p0) p = p0 - 1e-6;
+        uStarR = H0;
+        double duStarR = 0.;
+#pragma unroll
+        for (int k = 0; k < NPH; k++) {
+          double dhk;
+          rkS[k] = eos_rho_isentropic(S.eos[k], p0, rhok0[k], p, eps);
+          const double hk = eos_h_isentropic(S.eos[k], p0, rhok0[k], p, eps, dhk);
+          uStarR -= Yk0[k] * hk;
+          duStarR -= Yk0[k] * dhk;
+        }
+        uStarR = -sqrt(2. * uStarR);
+        duStarR = duStarR / uStarR;
+        double vStarL = 0., dvStarL = 0.;
+#pragma unroll
+        for (int k = 0; k < NPH; k++) {
+          double drhok;
+          const double rhok = eos_rho_hugoniot_d(S.eos[k], L.p[k], L.r[k], p, eps, drhok);
+          const double YkL = L.a[k] * L.r[k] / rhoL;
+          vStarL += YkL / gd(rhok);
+          dvStarL -= YkL / gd(rhok * rhok) * drhok;
+        }
+        vmv0 = vStarL - 1. / rhoL;
+        double dmL = 0.;
+        if (fabs(vmv0) > 1e-10) {
+          mL = sqrt((pL - p) / vmv0);
+          dmL = 0.5 * (-vmv0 + (p - pL) * dvStarL) / (vmv0 * vmv0) / mL;
+        } else {
+          mL = zL;
+        }
+        sL = uL - mL / rhoL;
+        wave(sL);
+        uStarL = uL + mL * vmv0;
+        const double duStarL = dmL * vmv0 + mL * dvStarL;
+        f = uStarR - uStarL;
+        df = duStarR - duStarL;
+      } while (fabs(f) > (KAP ? 1e-2 : 1e-3));
+      pStar = p;
+      uStar = 0.5 * (uStarL + uStarR);
+      rhoStar = 0.;
+#pragma unroll
+      for (int k = 0; k < NPH; k++) { YkS[k] = Yk0[k]; rhoStar += YkS[k] / gd(rkS[k]); }
+      rhoStar = 1. / rhoStar;
+      uyStar = 0.; uzStar = 0.;
+    }
+    double EStar = 0.5 * (uStar * uStar + uyStar * uyStar + uzStar * uzStar);
+#pragma unroll
+    for (int k = 0; k < NPH; k++) {
+      const double ek = eos_e(S.eos[k], rkS[k], pStar, eps);
+      EStar += YkS[k] * ek;
+      F.a[k] = YkS[k] * rhoStar / gd(rkS[k]) * uStar;
+      F.m[k] = F.a[k] * rkS[k];
+      F.en[k] = F.m[k] * ek;
+    }
+    F.q[0] = rhoStar * uStar * uStar + pStar; F.q[1] = rhoStar * uStar * uyStar; F.q[2] = rhoStar * uStar * uzStar;
+    F.E = (rhoStar * EStar + pStar) * uStar;
+    F.sM = KAP ? uStar : sM;
+  }
+  return smx;
 }
 
 // The wall's half Riemann problem in the face frame: ModKapila::solveRiemannWall

@@ -791,8 +791,8 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int slot, c
     own.E = Wn[Ly::IE * nc + cidx];
     // boundary faces (BoundCond::computeFlux, BoundCond.cpp:68-100): transmissive
     // (BoundCondAbs, the interior solver on (L, L)), wall (BoundCondWall /
-    // BoundCondWallO2), outflow (BoundCondOutflow) or injection (BoundCondInj),
-    // the last two on the cell's own state at both orders
+    // BoundCondWallO2), outflow (BoundCondOutflow), injection (BoundCondInj)
+    // or tank (BoundCondTank), the last three on the cell's own state at both orders
 #pragma unroll
     for (int dir = 0; dir < DIM; dir++) {
 #pragma unroll
@@ -804,6 +804,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int slot, c
         const bool wall = WALL && ((S.bc_wall >> (2 * dir + side)) & 1);
         const bool outf = WALL && ((S.bc_out >> (2 * dir + side)) & 1);
         const bool inj = WALL && ((S.bc_inj >> (2 * dir + side)) & 1);
+        const bool tank = WALL && ((S.bc_tank >> (2 * dir + side)) & 1);
         if constexpr (STAGE != 0 && WALL) {
           if (wall) wall_state_o2<MODEL, NPH, DIM>(S, G, Wsrc, Wn, nc, cidx, strd[dir], dir, side, L);
         }
@@ -816,6 +817,11 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int slot, c
           const int b = 2 * dir + side;
           bool nonconv = false;
           smx = smax(smx, inflow_flux<MODEL, NPH>(S, L, S.inj_m0[b], S.inj_Y[b], S.inj_H[b], F, nonconv));
+          if (nonconv) atomicOr(&ctl->err, 4u);
+        } else if (tank) {
+          const int b = 2 * dir + side;
+          bool nonconv = false;
+          smx = smax(smx, tank_flux<MODEL, NPH>(S, L, S.tank_p0[b], S.tank_rho[b], S.tank_Y[b], S.tank_H0[b], F, nonconv));
           if (nonconv) atomicOr(&ctl->err, 4u);
         }
         else smx = smax(smx, hllc<MODEL, NPH>(S, L, L, F));
@@ -925,7 +931,7 @@ int launch_t(const StageArgs &A, int stage, hipStream_t st) {
   const int tgz = (A.geo.n[2] + Tile<DIM>::Z - 1) / Tile<DIM>::Z;
   dim3 grid((unsigned)(tgx * tgy * tgz)), block(NT);
   const bool same = A.sch.same_lim != 0;
-  const bool wall = A.sch.bc_wall != 0 || A.sch.bc_out != 0 || A.sch.bc_inj != 0;  // kernels with boundary code
+  const bool wall = (A.sch.bc_wall | A.sch.bc_out | A.sch.bc_inj | A.sch.bc_tank) != 0;  // kernels with boundary code
   if (stage == 0) hipLaunchKernelGGL((k_stage<MODEL, NPH, DIM, 0, true>), grid, block, 0, st, A);
   else if (stage == 1 && same && wall) hipLaunchKernelGGL((k_stage<MODEL, NPH, DIM, 1, true>), grid, block, 0, st, A);
   else if (stage == 1 && same) hipLaunchKernelGGL((k_stage<MODEL, NPH, DIM, 1, true, false>), grid, block, 0, st, A);

@@ -20,6 +20,7 @@ description of :class:`ecogen_amd.Context` and the initial state in the
   ``belong`` tests of ``src/Geometries/GD*.cpp``);
 * ``abs``, ``outflow`` (``BoundCondOutflow``, the exit pressure ``p0``),
   ``injection`` (``BoundCondInj``, the mass flow ``m0`` and the injected fluid),
+  ``tank`` (``BoundCondTank``, a reservoir at ``p0``, ``T0``),
   ``wall`` and ``symmetry`` boundaries: ``BoundCondWall`` at first
   order for both wall and symmetry (``Input.cpp:849-852``), ``BoundCondWallO2``
   and ``BoundCondSymmetryO2`` at second order, which differ only in name
@@ -263,6 +264,7 @@ def load(rundir):
     bcs = ["abs"] * 6
     p0s = [0.0] * 6
     injs = [None] * 6
+    tanks = [None] * 6
     bnode = ci.find("boundaryConditions")
     for b in (bnode.findall("boundCond") if bnode is not None else []):
         kind = b.get("type", "").lower()
@@ -281,12 +283,17 @@ def load(rundir):
         elif kind == "injection":
             bcs[num - 1] = "injection"
             injs[num - 1] = _injection(b, case["eos"])
+        elif kind == "tank":
+            bcs[num - 1] = "tank"
+            tanks[num - 1] = _tank(b, case["eos"])
         else:
             raise DeckError(f"boundCond type {kind!r} is not read")
     case["bc"] = bcs
     case["bc_p0"] = p0s
     if any(injs):
         case["bc_inj"] = injs
+    if any(tanks):
+        case["bc_tank"] = tanks
 
     X, Y, Z = _centres(case["cells"], case["lengths"])
     Xg, Yg, Zg = X[None, None, :], Y[None, :, None], Z[:, None, None]
@@ -310,7 +317,7 @@ def load(rundir):
 
 
 def _injection(b, eos):
-    """BoundCondInj's reader (BoundCondInj.cpp:17-90): <dataInjection m0>, then
+    """BoundCondInj's reader (BoundCondInj.cpp:46-150): <dataInjection m0>, then
     one <dataFluid density pressure> for one phase, or per phase a <dataFluid
     EOS alpha density pressure> under <fluidsProp>, the fractions in [0, 1]
     and summing to 1 within 1e-6 (they are divided by their sum when lowered)."""
@@ -336,6 +343,44 @@ def _injection(b, eos):
     if abs(sum(alpha) - 1.0) > 1e-6:
         raise DeckError(f"{where}: sum of alpha should be 1")
     return {"m0": m0, "alpha": alpha, "rho": rho, "p": p}
+
+
+def _tank(b, eos):
+    """BoundCondTank's reader (BoundCondTank.cpp:46-144): <dataTank p0 T0>,
+    then for several phases one <dataFluid EOS alpha|massFrac> per phase under
+    <fluidsProp>, all alpha or all massFrac, each in [0, 1] and summing to 1
+    within 1e-6 (divided by their sum when lowered)."""
+    where = "boundCond tank"
+    dt = _child(b, "dataTank", where)
+    tk = {"p0": _f(dt, "p0", where), "T0": _f(dt, "T0", where)}
+    nph = len(eos)
+    if nph == 1:
+        return tk
+    names = [e.get("file") or f"{e['type']}_{e['name']}.xml" for e in eos]
+    fluids = _child(b, "fluidsProp", where).findall("dataFluid")
+    if len(fluids) != nph:
+        raise DeckError(f"{where}: {len(fluids)} fluids for {nph} phases")
+    kinds = {("massFrac" if fl.get("massFrac") is not None else "alpha") for fl in fluids}
+    for fl in fluids:
+        if (fl.get("alpha") is None) == (fl.get("massFrac") is None):
+            raise DeckError(f"{where}: one of alpha, massFrac is required")
+    if len(kinds) != 1:
+        raise DeckError(f"{where}: only one of alpha, massFrac is required")
+    key = kinds.pop()
+    frac = [None] * nph
+    for fl in fluids:
+        name = fl.get("EOS")
+        if name not in names:
+            raise DeckError(f"{where}: unknown EOS {name!r}")
+        frac[names.index(name)] = _f(fl, key, where)
+    if None in frac:
+        raise DeckError(f"{where}: {nph} phases need one fluid each")
+    if any(v < 0.0 or v > 1.0 for v in frac):
+        raise DeckError(f"{where}: {key} should be in [0,1]")
+    if abs(sum(frac) - 1.0) > 1e-6:
+        raise DeckError(f"{where}: sum of {key} should be 1")
+    tk[key] = frac
+    return tk
 
 
 def _run_with_output(ctx, case, outdir):

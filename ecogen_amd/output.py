@@ -49,8 +49,8 @@ def grad_rho(case, rho):
     """|grad rho| of Cell::getGradient on a Cartesian grid: per interior face
     (cd - cg)/|xR - xL| times the normal, weighted by the centre distance along
     each axis; every boundary face (any type, for the density) adds twice the
-    centre-to-face distance to the weights only (outflow and injection faces
-    nothing)."""
+    centre-to-face distance to the weights only (outflow, injection and tank
+    faces nothing)."""
     n = case["cells"]
     dim = sum(1 for c in n if c > 1)
     pos, dx = _positions(case)
@@ -83,10 +83,10 @@ def grad_rho(case, rho):
         wm[hi] = np.broadcast_to(dist, face.shape)
         w[d] = (w[d] + wm) + wp
     # boundary faces after the interior ones (X-, X+, Y-, ...): twice the
-    # centre-to-face distance, the X+ face lying at x_{n-1} + dX/2; outflow and
-    # injection faces (whoAmI 3 and 4) have no branch there and add nothing
+    # centre-to-face distance, the X+ face lying at x_{n-1} + dX/2; outflow,
+    # injection and tank faces (whoAmI 3, 4, 5) have no branch there and add nothing
     bcs = case.get("bc", ["abs"] * 6)
-    silent = ("outflow", "injection")
+    silent = ("outflow", "injection", "tank")
     for d in range(dim):
         ax = axis_of[d]
         p = pos[d]

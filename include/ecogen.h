@@ -37,9 +37,11 @@ enum eco_precision { ECO_FP64 = 0, ECO_FP32S = 1 };
    boundary; BoundCondWallO2 at second order; src/InputOutput/Input.cpp:849-852),
    outflow at an imposed pressure (BoundCondOutflow, at both orders; the
    pressure in bc_p0), injection at an imposed mass flow (BoundCondInj, at both
-   orders; the flow and the injected fluid in bc_inj_*; Input.cpp:853).  Walls,
-   outflows and injections run in the FP64 path only. */
-enum eco_bc { ECO_BC_ABS = 0, ECO_BC_WALL = 1, ECO_BC_OUTFLOW = 2, ECO_BC_INJECTION = 3 };
+   orders; the flow and the injected fluid in bc_inj_*; Input.cpp:853) and a
+   tank at rest (BoundCondTank, at both orders; its pressure, temperature and
+   fractions in bc_tank_*; Input.cpp:854).  All but ECO_BC_ABS run in the FP64
+   path only. */
+enum eco_bc { ECO_BC_ABS = 0, ECO_BC_WALL = 1, ECO_BC_OUTFLOW = 2, ECO_BC_INJECTION = 3, ECO_BC_TANK = 4 };
 
 enum eco_status_code {
   ECO_OK = 0,
@@ -53,8 +55,8 @@ enum eco_status_code {
 /* Soft-error flags (mirror the reference's non-fatal errors vector). */
 #define ECO_WARN_RELAX_NOT_CONVERGED 1u /* RelaxationP Newton hit 100 iterations (src/Relaxations/RelaxationP.cpp:92-109) */
 #define ECO_WARN_NAN 2u                 /* non-finite pressure or density produced */
-#define ECO_WARN_INFLOW_NOT_CONVERGED 4u /* injection Newton hit 50 iterations; the reference
-                                            stops there (src/Models/Kapila/ModKapila.cpp:283) */
+#define ECO_WARN_INFLOW_NOT_CONVERGED 4u /* injection or tank Newton hit 50 iterations; the reference
+                                            stops there (src/Models/Kapila/ModKapila.cpp:283, 405) */
 
 /* Equation of state (src/Eos/EosIG.h, src/Eos/EosSG.h; parameters of libEOS/*.xml). */
 typedef struct {
@@ -78,11 +80,17 @@ typedef struct {
   int bc[6];        /* X-, X+, Y-, Y+, Z-, Z+ : enum eco_bc (all but ECO_BC_ABS: FP64 only) */
   eco_eos eos[4];   /* per phase */
   double bc_p0[6];  /* <dataOutflow p0> of each ECO_BC_OUTFLOW side (Pa) */
-  /* each ECO_BC_INJECTION side (BoundCondInj.cpp:17-90): <dataInjection m0>,
+  /* each ECO_BC_INJECTION side (BoundCondInj.cpp:46-150): <dataInjection m0>,
      the mass flow per unit area into the domain (kg/s/m^2, as in the deck),
      and the injected fluid's alpha_k, rho_k, p_k (Euler: rho and p in [0]) */
   double bc_inj_m0[6];
   double bc_inj_alpha[6][4], bc_inj_rho[6][4], bc_inj_p[6][4];
+  /* each ECO_BC_TANK side (BoundCondTank.cpp:46-176): <dataTank p0 T0> and, for
+     several phases, the tank's volume fractions (or, with bc_tank_massfrac
+     set, mass fractions) per phase */
+  double bc_tank_p0[6], bc_tank_T0[6];
+  double bc_tank_frac[6][4];
+  int bc_tank_massfrac[6];
 } eco_config;
 
 typedef struct eco_ctx eco_ctx;

@@ -164,7 +164,7 @@ struct CartesianMesh {
   int numberCells[3] = {1, 1, 1};
   double dimensions[3] = {1., 1., 1.};
   // boundary condition per side X-, X+, Y-, Y+, Z-, Z+ (initialConditions
-  // <boundCond type="abs|wall|outflow|injection">; all but abs run in FP64)
+  // <boundCond type="abs|wall|outflow|injection|tank">; all but abs run in FP64)
   int boundCond[6] = {ECO_BC_ABS, ECO_BC_ABS, ECO_BC_ABS, ECO_BC_ABS, ECO_BC_ABS, ECO_BC_ABS};
   double outflowPressure[6] = {0., 0., 0., 0., 0., 0.};  // <dataOutflow p0> of ECO_BC_OUTFLOW sides
   // ECO_BC_INJECTION sides (BoundCondInj): <dataInjection m0> and the injected
@@ -173,6 +173,13 @@ struct CartesianMesh {
     double m0 = 0.;
     double alpha[4] = {1., 0., 0., 0.}, density[4] = {0., 0., 0., 0.}, pressure[4] = {0., 0., 0., 0.};
   } injection[6];
+  // ECO_BC_TANK sides (BoundCondTank): <dataTank p0 T0> and, for several
+  // phases, the volume fractions (or mass fractions) per phase
+  struct Tank {
+    double p0 = 0., T0 = 0.;
+    double fraction[4] = {1., 0., 0., 0.};
+    bool massFractions = false;
+  } tank[6];
 };
 
 struct RunStatus {

@@ -143,6 +143,15 @@ def write_deck(case, root: Path):
                     f'        <dataFluid EOS="{_eos_file(e)}" alpha="{inj["alpha"][k]!r}" density="{inj["rho"][k]!r}" '
                     f'pressure="{inj["p"][k]!r}"/>\n' for k, e in enumerate(case["eos"])) + "      </fluidsProp>\n"
             return head + body + "    </boundCond>"
+        if kinds[i] == "tank":
+            tk = case["bc_tank"][i]
+            body = f'>\n      <dataTank p0="{tk["p0"]!r}" T0="{tk["T0"]!r}"/>\n'
+            if len(case["eos"]) > 1:
+                key = "massFrac" if "massFrac" in tk else "alpha"
+                body += "      <fluidsProp>\n" + "".join(
+                    f'        <dataFluid EOS="{_eos_file(e)}" {key}="{tk[key][k]!r}"/>\n'
+                    for k, e in enumerate(case["eos"])) + "      </fluidsProp>\n"
+            return head + body + "    </boundCond>"
         return head + "/>"
     bcs = "\n".join(bc_xml(i) for i in range(2 * dim))
     states = "\n".join(_state_xml(case, st) for st in case["states"])

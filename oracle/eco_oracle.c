@@ -617,12 +617,12 @@ static double eos_dvdhcp(const orc_eos *q, double p, double eps) {
 }
 
 /* The injection's half Riemann problem in the face frame (BoundCondInj,
-   src/BoundConds/BoundCondInj.cpp:33-37, 190-193): a fluid of total enthalpy
+   src/BoundConds/BoundCondInj.cpp:62, 198-201): a fluid of total enthalpy
    H_k0 (Eos::computeTotalEnthalpy, Eos.cpp:99-102) enters at the mass flow m0
    per unit area; the left wave is acoustic.  ModKapila::solveRiemannInflow
    (ModKapila.cpp:253-322) and ModMultiP::solveRiemannInflow
    (ModMultiP.cpp:249-318) are the same; ModEuler::solveRiemannInflow
-   (ModEuler.cpp:214-303) is their one-phase form.  m0 is the deck's value; the
+   (ModEuler.cpp:215-303) is their one-phase form.  m0 is the deck's value; the
    reference changes its sign on reading.  Newton on the pressure from pL until
    |f| <= 1e-10; the reference aborts after 50 iterations, here the 50th
    iterate is kept and *nonconv set. */
@@ -668,7 +668,7 @@ static void riemann_inflow(const Ctx *c, const State *L, double dxL, double *dtM
   double uL = L->mx.u[0], cL = L->mx.cF, pL = L->mx.p, rhoL = L->mx.rho;
   double vL = L->mx.u[1], wL = L->mx.u[2];
   double rho0 = 0., Hk0[MAXPH], Yk0[MAXPH], vk[MAXPH], a0[MAXPH], sum = 0.;
-  for (int k = 0; k < n; k++) sum += ak0[k];                 /* BoundCondInj.cpp:82-86 */
+  for (int k = 0; k < n; k++) sum += ak0[k];                 /* BoundCondInj.cpp:108-117 */
   for (int k = 0; k < n; k++) a0[k] = ak0[k] / sum;
   for (int k = 0; k < n; k++) rho0 += a0[k] * rhok0[k];     /* Mix*::computeDensity */
   double u0 = m0 / rho0;
@@ -710,6 +710,228 @@ static void riemann_inflow(const Ctx *c, const State *L, double dxL, double *dtM
   F->qdm[0] = u * u / v + pStar; F->qdm[1] = u * vL / v; F->qdm[2] = u * wL / v;
   F->energMix = (Estar / v + pStar) * u;
   F->sM = u;
+}
+
+/* computeDensityIsentropic and computeDensityHugoniot with their derivative
+   drho/dp (EosIG.cpp:146-162, EosSG.cpp:149-165), computeEnthalpyIsentropic
+   (EosIG.cpp:177-184, EosSG.cpp:180-187) and computeDensity(p, T)
+   (EosIG.cpp:111-114, EosSG.cpp:114-117) */
+static double eos_rho_isentropic_d(const orc_eos *q, double p0, double rho0, double pf, double eps, double *drho) {
+  double pi = (q->type == ORC_EOS_SG) ? q->pinf : 0.;
+  double rf = eos_rho_isentropic(q, p0, rho0, pf, eps);
+  *drho = rf / smax(q->gamma * (pf + pi), eps);
+  return rf;
+}
+static double eos_rho_hugoniot_d(const orc_eos *q, double p0, double rho0, double pf, double eps, double *drho) {
+  double pi = (q->type == ORC_EOS_SG) ? q->pinf : 0.;
+  double denom = (q->type == ORC_EOS_SG) ? (q->gamma - 1.) * (pf + pi) + (q->gamma + 1.) * (p0 + pi)
+                                         : (q->gamma - 1.) * pf + (q->gamma + 1.) * p0;
+  *drho = (q->type == ORC_EOS_SG) ? rho0 * 4. * q->gamma * (p0 + pi) / smax(denom * denom, eps)
+                                  : rho0 * 4. * q->gamma * p0 / smax(denom * denom, eps);
+  return eos_rho_hugoniot(q, p0, rho0, pf, eps);
+}
+static double eos_h_isentropic(const orc_eos *q, double p0, double rho0, double pf, double eps, double *dh) {
+  double drho, rf = eos_rho_isentropic_d(q, p0, rho0, pf, eps, &drho);
+  double pp = (q->type == ORC_EOS_SG) ? pf + q->pinf : pf;
+  *dh = q->gamma / (q->gamma - 1.) * (rf - pp * drho) / smax(rf * rf, eps);
+  return q->gamma * pp / (q->gamma - 1.) / smax(rf, eps) + q->eref;
+}
+static double eos_rho_pt(const orc_eos *q, double p, double T, double eps) {
+  double num = (q->type == ORC_EOS_SG) ? p + q->pinf : p;
+  return num / smax((q->gamma - 1.) * q->cv * T, eps);
+}
+
+/* The tank's state (BoundCondTank.cpp:46-176): rho_k0 = rho_k(p0, T0); the
+   fractions divided by their sum; from volume fractions the mass fractions
+   alpha_k0 rho_k0 / rho, from mass fractions the volume fractions
+   rho Y_k / rho_k0 with 1/rho = sum Y_k / rho_k0.  One phase: alpha 1. */
+static void tank_state(const Ctx *c, int side, double *ak0, double *rhok0) {
+  const orc_cfg *cfg = c->cfg;
+  int n = cfg->nph;
+  double p0 = cfg->bc_tank_p0[side], T0 = cfg->bc_tank_T0[side];
+  for (int k = 0; k < n; k++) rhok0[k] = eos_rho_pt(&cfg->eos[k], p0, T0, c->epsilon);
+  if (n == 1) { ak0[0] = 1.; return; }
+  double fr[MAXPH], sum = 0.;
+  for (int k = 0; k < n; k++) sum += cfg->bc_tank_frac[side][k];
+  for (int k = 0; k < n; k++) fr[k] = cfg->bc_tank_frac[side][k] / sum;
+  if (!cfg->bc_tank_massfrac[side]) {
+    for (int k = 0; k < n; k++) ak0[k] = fr[k];
+  } else {
+    double rm = 0.;
+    for (int k = 0; k < n; k++) rm += fr[k] / rhok0[k];
+    rm = 1.0 / rm;
+    for (int k = 0; k < n; k++) ak0[k] = rm * fr[k] / rhok0[k];
+  }
+}
+
+/* The tank's half Riemann problem in the face frame (BoundCondTank): from
+   the left wave at p* = p0, supersonic outflow keeps the left state and
+   subsonic outflow the left star state at p0; otherwise fluid flows in from
+   the tank, at rest with total enthalpy H0 and isentropic, and a Newton
+   solve on p* matches the tank-side velocity to the left star velocity.
+   ModKapila::solveRiemannTank (ModKapila.cpp:326-475, Hugoniot left wave,
+   |f| <= 1e-2), ModMultiP::solveRiemannTank (ModMultiP.cpp:322-465, the same
+   with |f| <= 1e-3, no epsilon guards, and sM of the first estimate as the
+   contact speed), ModEuler::solveRiemannTank (ModEuler.cpp:305-423,
+   isentropic left wave, |f| <= 1e-3).  Every Newton iterate's left wave
+   enters dtMax, as in the reference; the reference stops after 50 iterations,
+   here the last iterate is kept and *nonconv set. */
+static void riemann_tank(const Ctx *c, const State *L, double dxL, double *dtMax, int side, Flux *F, int *nonconv) {
+  const orc_cfg *cfg = c->cfg;
+  double eps = c->epsilon;
+  double p0 = cfg->bc_tank_p0[side];
+  double ak0[MAXPH], rhok0[MAXPH];
+  tank_state(c, side, ak0, rhok0);
+  if (cfg->model == ORC_EULER) {
+    const orc_eos *q = &cfg->eos[0];
+    const Phase *P = &L->ph[0];
+    double uL = P->u[0], vL = P->u[1], wL = P->u[2], pL = P->p, rhoL = P->rho, cL = P->c;
+    double zL = rhoL * cL;
+    double uStar, rhoStar, pStar = p0, vStar, wStar, mL, drho;
+    double v = 1. / eos_rho_isentropic(q, pL, rhoL, pStar, 0.);
+    double vmv0 = v - 1. / rhoL;
+    if (fabs(vmv0) > 1e-10) mL = sqrt((pL - p0) / vmv0);
+    else mL = zL;
+    double sL = uL - mL / rhoL;
+    if (fabs(sL) > 1.e-3) *dtMax = smin(*dtMax, dxL / fabs(sL));
+    double u = uL + mL * vmv0;
+    if (sL >= 0.) { uStar = uL; pStar = pL; rhoStar = rhoL; vStar = vL; wStar = wL; }
+    else if (u >= -1e-3) { uStar = u; pStar = p0; rhoStar = 1. / v; vStar = vL; wStar = wL; }
+    else {
+      double H0 = eos_e(q, rhok0[0], p0, 0.) + p0 / rhok0[0] + 0.5 * 0. * 0.;
+      int it = 0;
+      double p = 0.5 * p0, f = 0., df = 1., dv, h, dh;
+      double uStarR = 0., duStarR = 0., uStarL = 0., duStarL = 0.;
+      rhoStar = 0.;
+      do {
+        p -= f / df; it++;
+        if (it > 50) { *nonconv = 1; break; }
+        eos_fix_p(q, &p);
+        if (p > p0) p = p0 - 1e-6;
+        rhoStar = eos_rho_isentropic(q, p0, rhok0[0], p, 0.);
+        h = eos_h_isentropic(q, p0, rhok0[0], p, 0., &dh);
+        uStarR = -sqrt(2. * (H0 - h));
+        duStarR = -dh / uStarR;
+        double dmL;
+        v = 1.0 / eos_rho_isentropic_d(q, pL, rhoL, p, 0., &drho);
+        dv = -v * v * drho;
+        vmv0 = v - 1. / rhoL;
+        if (fabs(vmv0) > 1e-10) {
+          mL = sqrt((pL - p) / vmv0);
+          dmL = 0.5 * (-vmv0 + (p - pL) * dv) / (vmv0 * vmv0) / mL;
+        } else { mL = zL; dmL = 0.; }
+        sL = uL - mL / rhoL;
+        if (fabs(sL) > 1.e-3) *dtMax = smin(*dtMax, dxL / fabs(sL));
+        uStarL = uL + mL * vmv0;
+        duStarL = dmL * vmv0 + mL * dv;
+        f = uStarR - uStarL;
+        df = duStarR - duStarL;
+      } while (fabs(f) > 1e-3);
+      pStar = p;
+      uStar = 0.5 * (uStarL + uStarR);
+      vStar = 0.; wStar = 0.;
+    }
+    double eStar = eos_e(q, rhoStar, pStar, 0.);
+    F->masse[0] = rhoStar * uStar;
+    F->qdm[0] = rhoStar * uStar * uStar + pStar; F->qdm[1] = rhoStar * uStar * vStar; F->qdm[2] = rhoStar * uStar * wStar;
+    F->energMix = (rhoStar * (eStar + 0.5 * (uStar * uStar + vStar * vStar + wStar * wStar)) + pStar) * uStar;
+    F->sM = uStar;
+    return;
+  }
+  int n = cfg->nph, kap = (cfg->model == ORC_KAPILA);
+  double uL = L->mx.u[0], cL = L->mx.cF, pL = L->mx.p, rhoL = L->mx.rho;
+  double uyL = L->mx.u[1], uzL = L->mx.u[2];
+  double zL = rhoL * cL;
+  double rkS[MAXPH], YkS[MAXPH], pStar = p0, vStar = 0., uStar, rhoStar, uyStar, uzStar, mL;
+  for (int k = 0; k < n; k++) {
+    const Phase *P = &L->ph[k];
+    rkS[k] = eos_rho_hugoniot(&cfg->eos[k], P->p, P->rho, pStar, eps);
+    vStar += kap ? P->alpha * P->rho / rhoL / smax(rkS[k], eps) : P->alpha * P->rho / rhoL / rkS[k];
+  }
+  double vmv0 = vStar - 1. / rhoL;
+  if (fabs(vmv0) > 1e-10) mL = sqrt((pL - pStar) / vmv0);
+  else mL = zL;
+  double sL = uL - mL / rhoL;
+  if (fabs(sL) > 1.e-3) *dtMax = smin(*dtMax, dxL / fabs(sL));
+  double sM = uL + mL * vmv0;
+  if (sL >= 0.) {
+    uStar = uL; pStar = pL;
+    for (int k = 0; k < n; k++) { rkS[k] = L->ph[k].rho; YkS[k] = L->ph[k].alpha * L->ph[k].rho / rhoL; }
+    rhoStar = rhoL; uyStar = uyL; uzStar = uzL;
+  } else if (sM >= -1e-3) {
+    uStar = sM; pStar = p0;
+    for (int k = 0; k < n; k++) YkS[k] = L->ph[k].alpha * L->ph[k].rho / rhoL;
+    rhoStar = 1. / vStar; uyStar = uyL; uzStar = uzL;
+  } else {
+    double H0 = 0., Yk0[MAXPH], rho0 = 0.;
+    for (int k = 0; k < n; k++) rho0 += ak0[k] * rhok0[k];   /* Mix*::computeDensity */
+    for (int k = 0; k < n; k++) {
+      Yk0[k] = ak0[k] * rhok0[k] / rho0;
+      H0 += Yk0[k] * (eos_e(&cfg->eos[k], rhok0[k], p0, eps) + p0 / rhok0[k] + 0.5 * 0. * 0.);
+    }
+    int it = 0;
+    double p = 0.5 * p0, f = 0., df = 1., dhk, drhok;
+    double uStarR = 0., duStarR = 0., uStarL = 0., duStarL = 0.;
+    do {
+      p -= f / df; it++;
+      if (it > 50) { *nonconv = 1; break; }
+      for (int k = 0; k < n; k++) eos_fix_p(&cfg->eos[k], &p);
+      if (p > p0) p = p0 - 1e-6;
+      uStarR = H0; duStarR = 0.;
+      for (int k = 0; k < n; k++) {
+        rkS[k] = eos_rho_isentropic(&cfg->eos[k], p0, rhok0[k], p, eps);
+        double hk = eos_h_isentropic(&cfg->eos[k], p0, rhok0[k], p, eps, &dhk);
+        uStarR -= Yk0[k] * hk;
+        duStarR -= Yk0[k] * dhk;
+      }
+      uStarR = -sqrt(2. * uStarR);
+      duStarR = duStarR / uStarR;
+      double vStarL = 0., dvStarL = 0., dmL;
+      for (int k = 0; k < n; k++) {
+        const Phase *P = &L->ph[k];
+        double rhok = eos_rho_hugoniot_d(&cfg->eos[k], P->p, P->rho, p, eps, &drhok);
+        double YkL = P->alpha * P->rho / rhoL;
+        if (kap) {
+          vStarL += YkL / smax(rhok, eps);
+          dvStarL -= YkL / smax(rhok * rhok, eps) * drhok;
+        } else {
+          vStarL += YkL / rhok;
+          dvStarL -= YkL / (rhok * rhok) * drhok;
+        }
+      }
+      vmv0 = vStarL - 1. / rhoL;
+      if (fabs(vmv0) > 1e-10) {
+        mL = sqrt((pL - p) / vmv0);
+        dmL = 0.5 * (-vmv0 + (p - pL) * dvStarL) / (vmv0 * vmv0) / mL;
+      } else { mL = zL; dmL = 0.; }
+      sL = uL - mL / rhoL;
+      if (fabs(sL) > 1.e-3) *dtMax = smin(*dtMax, dxL / fabs(sL));
+      uStarL = uL + mL * vmv0;
+      duStarL = dmL * vmv0 + mL * dvStarL;
+      f = uStarR - uStarL;
+      df = duStarR - duStarL;
+    } while (fabs(f) > (kap ? 1e-2 : 1e-3));
+    pStar = p;
+    uStar = 0.5 * (uStarL + uStarR);
+    rhoStar = 0.;
+    for (int k = 0; k < n; k++) {
+      YkS[k] = Yk0[k];
+      rhoStar += kap ? YkS[k] / smax(rkS[k], eps) : YkS[k] / rkS[k];
+    }
+    rhoStar = 1. / rhoStar;
+    uyStar = 0.; uzStar = 0.;
+  }
+  double EStar = 0.5 * (uStar * uStar + uyStar * uyStar + uzStar * uzStar);
+  for (int k = 0; k < n; k++) {
+    double ek = eos_e(&cfg->eos[k], rkS[k], pStar, eps);
+    EStar += YkS[k] * ek;
+    F->alpha[k] = kap ? YkS[k] * rhoStar / smax(rkS[k], eps) * uStar : YkS[k] * rhoStar / rkS[k] * uStar;
+    F->masse[k] = F->alpha[k] * rkS[k];
+    F->energ[k] = F->masse[k] * ek;
+  }
+  F->qdm[0] = rhoStar * uStar * uStar + pStar; F->qdm[1] = rhoStar * uStar * uyStar; F->qdm[2] = rhoStar * uStar * uzStar;
+  F->energMix = (rhoStar * EStar + pStar) * uStar;
+  F->sM = kap ? uStar : sM;
 }
 
 /* The wall's half Riemann problem in the face frame: ModKapila::solveRiemannWall
@@ -949,7 +1171,7 @@ typedef struct {
   double *fslope;     /* per-face primitive slopes (nvar per face) */
   int nvar;
   double *fdt;        /* per-face dtMax contributions */
-  int inflow_nonconv; /* an injection Newton solve hit its 50 iterations */
+  int inflow_nonconv; /* an injection or tank Newton solve hit its 50 iterations */
 } Solver;
 
 /* Primitive slope vector: phases (alpha, rho, p) or Euler (rho, p, u, v, w); mixture (u, v, w). */
@@ -1166,9 +1388,10 @@ static void face_fluxes(Solver *S, State *st, int o2) {
       fulfill(c, &L);
       if (c->cfg->bc[F->side] == ORC_BC_WALL) riemann_wall(c, &L, lcfl, &dtm, FL);
       else if (c->cfg->bc[F->side] == ORC_BC_OUTFLOW) riemann_outflow(c, &L, lcfl, &dtm, c->cfg->bc_p0[F->side], FL);
-      else if (c->cfg->bc[F->side] == ORC_BC_INJ) {
+      else if (c->cfg->bc[F->side] == ORC_BC_INJ || c->cfg->bc[F->side] == ORC_BC_TANK) {
         int nc = 0;
-        riemann_inflow(c, &L, lcfl, &dtm, F->side, FL, &nc);
+        if (c->cfg->bc[F->side] == ORC_BC_INJ) riemann_inflow(c, &L, lcfl, &dtm, F->side, FL, &nc);
+        else riemann_tank(c, &L, lcfl, &dtm, F->side, FL, &nc);
         if (nc) {
 #pragma omp atomic write
           S->inflow_nonconv = 1;
@@ -1416,7 +1639,7 @@ int orc_run(const orc_cfg *cfg, const double *state_in, double *state_out,
   if (cfg->model == ORC_EULER && cfg->nph != 1) return -1;
   if (is_thinc(cfg) && cfg->nph != 2) return -1;
   for (int b = 0; b < 6; b++) {
-    if (cfg->bc[b] < ORC_BC_ABS || cfg->bc[b] > ORC_BC_INJ) return -1;
+    if (cfg->bc[b] < ORC_BC_ABS || cfg->bc[b] > ORC_BC_TANK) return -1;
 
   }
 #ifdef _OPENMP
@@ -1456,7 +1679,7 @@ int orc_run(const orc_cfg *cfg, const double *state_in, double *state_out,
   if (dt_next_out) *dt_next_out = dt;
   free(S.sn); free(S.s2); free(S.cons); free(S.fflux); free(S.fslope); free(S.fdt);
   mesh_free(&S.m);
-  /* the reference stops with "solveRiemannInflow not converged" */
+  /* the reference stops with "solveRiemannInflow (or Tank) not converged" */
   return S.inflow_nonconv ? -3 : it;
 }
 

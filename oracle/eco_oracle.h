@@ -29,9 +29,11 @@ enum { ORC_EOS_IG = 0, ORC_EOS_SG = 1 };
 /* boundary conditions: BoundCondAbs, and the wall: BoundCondWall at first
    order (also the reference's first-order SYMMETRY), BoundCondWallO2 at second
    order (src/InputOutput/Input.cpp:850-852) */
-enum { ORC_BC_ABS = 0, ORC_BC_WALL = 1, ORC_BC_OUTFLOW = 2, ORC_BC_INJ = 3 };
+enum { ORC_BC_ABS = 0, ORC_BC_WALL = 1, ORC_BC_OUTFLOW = 2, ORC_BC_INJ = 3, ORC_BC_TANK = 4 };
 /* outflow: BoundCondOutflow, imposed p0; injection: BoundCondInj, imposed mass
-   flow m0 of a fluid of given alpha_k, rho_k, p_k */
+   flow m0 of a fluid of given alpha_k, rho_k, p_k; tank: BoundCondTank, a
+   reservoir at rest at p0, T0 holding the phases in given volume (or mass)
+   fractions */
 
 typedef struct {
   int type;
@@ -53,6 +55,9 @@ typedef struct {
   double bc_p0[6];   /* outflow pressure per side */
   double bc_m0[6];   /* injection: <dataInjection m0>, mass flow per unit area into the domain */
   double bc_ak0[6][ORC_MAXPH], bc_rhok0[6][ORC_MAXPH], bc_pk0[6][ORC_MAXPH];  /* injected fluid per phase */
+  double bc_tank_p0[6], bc_tank_T0[6];   /* tank: <dataTank p0 T0> */
+  double bc_tank_frac[6][ORC_MAXPH];     /* tank: <dataFluid alpha> (or massFrac) per phase */
+  int bc_tank_massfrac[6];               /* tank: 1 if bc_tank_frac holds mass fractions */
 } orc_cfg;
 
 /*
@@ -69,8 +74,8 @@ typedef struct {
  * the reference reads totalTime with QueryFloatAttribute).  dt_log (optional,
  * length >= steps taken) receives the dt used by each step.
  * Returns the number of steps taken, or a negative error code (-1 invalid
- * configuration, -2 allocation, -3 an injection Newton solve did not converge
- * in 50 iterations, where the reference stops).
+ * configuration, -2 allocation, -3 an injection or tank Newton solve did not
+ * converge in 50 iterations, where the reference stops).
  */
 int orc_run(const orc_cfg *cfg, const double *state_in, double *state_out,
             int nsteps, double t_final, int max_steps,

@@ -22,7 +22,8 @@ LIMITERS = {"none": 0, "minmod": 1, "vanleer": 2, "vanalbada": 3, "superbee": 4,
 EOS_TYPES = {"IG": 0, "SG": 1}
 # per side X-, X+, Y-, Y+, Z-, Z+; outflow pressures in case["bc_p0"]; injections in
 # case["bc_inj"][side] = {"m0": mass flow into the domain, "alpha", "rho", "p": per phase}
-BCS = {"abs": 0, "wall": 1, "outflow": 2, "injection": 3}
+# case["bc_tank"][side] = {"p0", "T0", and per phase "alpha" or "massFrac"}
+BCS = {"abs": 0, "wall": 1, "outflow": 2, "injection": 3, "tank": 4}
 MAXPH = 4
 
 
@@ -37,7 +38,9 @@ class OrcCfg(C.Structure):
                 ("L", C.c_double * 3), ("cfl", C.c_double), ("eos", OrcEos * MAXPH), ("bc", C.c_int * 6),
                 ("bc_p0", C.c_double * 6), ("bc_m0", C.c_double * 6),
                 ("bc_ak0", (C.c_double * MAXPH) * 6), ("bc_rhok0", (C.c_double * MAXPH) * 6),
-                ("bc_pk0", (C.c_double * MAXPH) * 6)]
+                ("bc_pk0", (C.c_double * MAXPH) * 6), ("bc_tank_p0", C.c_double * 6),
+                ("bc_tank_T0", C.c_double * 6), ("bc_tank_frac", (C.c_double * MAXPH) * 6),
+                ("bc_tank_massfrac", C.c_int * 6)]
 
 
 _lib = None
@@ -109,6 +112,13 @@ def make_cfg(case) -> OrcCfg:
                 cfg.bc_ak0[b][k] = inj["alpha"][k]
                 cfg.bc_rhok0[b][k] = inj["rho"][k]
                 cfg.bc_pk0[b][k] = inj["p"][k]
+    for b, tk in enumerate(case.get("bc_tank", [None] * 6)):
+        if tk:
+            cfg.bc_tank_p0[b], cfg.bc_tank_T0[b] = tk["p0"], tk["T0"]
+            frac = tk.get("massFrac") or tk.get("alpha") or [1.0]
+            cfg.bc_tank_massfrac[b] = int("massFrac" in tk)
+            for k, v in enumerate(frac):
+                cfg.bc_tank_frac[b][k] = v
     return cfg
 
 

@@ -107,6 +107,21 @@ def _inj(nph, m0, axis, x_side, mix=False, rho=None, p=None):
     return out
 
 
+def _tank(nph, p0, T0, x_side, water=False, mass=False):
+    """bc_tank list with one tank (BoundCondTank) on X side ``x_side``: a
+    reservoir at p0, T0 holding air (alpha 1 - 1e-6, or mass fraction 0.99 when
+    ``mass``), or water when ``water``."""
+    tk = {"p0": p0, "T0": T0}
+    if nph > 1:
+        if mass:
+            tk["massFrac"] = [0.99, 0.01]
+        else:
+            tk["alpha"] = [1e-6, 1.0 - 1e-6] if water else [1.0 - 1e-6, 1e-6]
+    out = [None] * 6
+    out[x_side] = tk
+    return out
+
+
 def mach_shock_air(mach, p1=1e5, rho1=1.2, gamma=1.4):
     """Post-shock state of a normal shock moving into air at rest (Rankine-Hugoniot)."""
     import math
@@ -223,6 +238,23 @@ CASES = {
                                   bc_inj=_inj(1, 150.0, 1, x_side=1)),
     "euler_1d_o2_injection": dict(_euler_1d(2, ["vanleer"] * 4, 300), bc=["injection", "outflow"] + ["abs"] * 4,
                                   bc_p0=[0.0, 5e4] + [0.0] * 4, bc_inj=_inj(1, 3000.0, 1, x_side=0, rho=80.0, p=1e7)),
+    # Tank boundaries (BoundCondTank, a reservoir at rest at p0, T0; no
+    # second-order variant): air flows in from tanks above the tubes' air
+    # pressure, and the tubes' high-pressure water flows out into tanks below
+    # it (the subsonic-outflow branch); one MultiP tank is given by mass fractions.
+    "kapila_1d_fo_tank": dict(_shock_tube_1d("kapila", 1, ["none"] * 4, 300, cells=200),
+                              bc=["abs", "tank"] + ["abs"] * 4, bc_tank=_tank(2, 3e5, 300.0, 1)),
+    "kapila_1d_o2_tank": dict(_shock_tube_1d("kapila", 2, ["vanleer"] * 4, 300, cells=200), bc=["tank", "tank"] + ["abs"] * 4,
+                              bc_tank=[_tank(2, 5e8, 300.0, 0, water=True)[0], _tank(2, 3e5, 300.0, 1)[1]] + [None] * 4),
+    "multip_1d_o2_relax_tank": dict(_shock_tube_1d("multip", 2, ["minmod"] * 4, 160, cells=100, relax=True),
+                                    bc=["abs", "tank"] + ["abs"] * 4, bc_tank=_tank(2, 4e5, 350.0, 1, mass=True)),
+    "euler_1d_fo_tank": dict(_euler_1d(1, ["none"] * 4, 300), bc=["abs", "tank"] + ["abs"] * 4,
+                             bc_tank=_tank(1, 3e5, 300.0, 1)),
+    "euler_1d_o2_tank": dict(_euler_1d(2, ["vanleer"] * 4, 300), bc=["tank", "tank"] + ["abs"] * 4,
+                             bc_tank=[_tank(1, 5e6, 300.0, 0)[0], _tank(1, 2e5, 300.0, 1)[1]] + [None] * 4),
+    "kapila_2d_fo_tank": dict(_shock_cylinders_2d(n=40, steps=80), order=1, limiters=["none"] * 4,
+                              bc=["abs", "outflow", "tank", "wall", "abs", "abs"],
+                              bc_p0=[0.0, 8e4] + [0.0] * 4, bc_tank=[None, None] + _tank(2, 3e5, 300.0, 0)[:1] + [None] * 3),
     "kapila_2d_fo_injection": dict(_shock_cylinders_2d(n=40, steps=80), order=1, limiters=["none"] * 4,
                                    bc=["abs", "outflow", "injection", "wall", "abs", "abs"],
                                    bc_p0=[0.0, 8e4] + [0.0] * 4, bc_inj=_inj(2, 300.0, 2, x_side=0)),

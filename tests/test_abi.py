@@ -99,10 +99,11 @@ def test_riemann_case_t_final():
     assert st[2][0] == 1e7 and st[2][-1] == 1e5
 
 
-def test_injection_configs_are_validated_without_device(lib):
+def test_boundary_configs_are_validated_without_device(lib):
     """eco_create checks a configuration before any HIP call: an injection
     side needs a nonzero flow, positive densities and fractions summing to 1
-    (BoundCondInj.cpp:30-90), and runs in FP64 only."""
+    (BoundCondInj.cpp:46-150), a tank positive p0 and T0 and fractions summing
+    to 1 (BoundCondTank.cpp:46-144); both run in FP64 only."""
     import ctypes as C
     from ecogen_amd import capi
     from cases import CASES
@@ -119,3 +120,8 @@ def test_injection_configs_are_validated_without_device(lib):
     assert create(dict(case, bc_inj=no_flow)) == -1
     no_rho = [None, dict(inj, rho=[50.0, 0.0])] + [None] * 4
     assert create(dict(case, bc_inj=no_rho)) == -1
+    tcase = CASES["multip_1d_o2_relax_tank"]
+    tk = tcase["bc_tank"][1]
+    assert create(dict(tcase, precision="fp32s")) == -2
+    assert create(dict(tcase, bc_tank=[None, dict(tk, massFrac=[0.5, 0.6])] + [None] * 4)) == -1
+    assert create(dict(tcase, bc_tank=[None, dict(tk, T0=0.0)] + [None] * 4)) == -1

@@ -30,6 +30,8 @@ def test_deck_reads_back_the_reference_inputs(name, tmp_path):
             assert got["bc_p0"][i] == case["bc_p0"][i]
         if kind == "injection":
             assert got["bc_inj"][i] == case["bc_inj"][i]
+        if kind == "tank":
+            assert got["bc_tank"][i] == case["bc_tank"][i]
     for a, b in zip(got["eos"], case["eos"]):
         for k in ("type", "gamma", "cv"):
             assert a[k] == b[k]
@@ -43,8 +45,9 @@ def test_deck_reads_back_the_reference_inputs(name, tmp_path):
 def test_deck_refusals(tmp_path):
     from ecogen_amd import deck
     case, _ = load_golden("kapila_1d_o2_vanleer")
-    write_deck(dict(case, name="x", bc=["tank"] * 6), tmp_path / "a")
-    with pytest.raises(deck.DeckError, match="tank"):
+    # every type Input.cpp:849-856 reads is read; others are refused as there
+    write_deck(dict(case, name="x", bc=["periodic"] * 6), tmp_path / "a")
+    with pytest.raises(deck.DeckError, match="periodic"):
         deck.load(tmp_path / "a")
     write_deck(dict(case, name="x", bc=["symmetry"] * 6), tmp_path / "b")
     assert deck.load(tmp_path / "b")[0]["bc"][:2] == ["wall"] * 2  # a 1-D deck names X- and X+ only
@@ -52,6 +55,10 @@ def test_deck_refusals(tmp_path):
     write_deck(dict(case, name="x", bc=["injection"] * 6, bc_inj=[inj] * 6), tmp_path / "c")
     with pytest.raises(deck.DeckError, match="sum of alpha"):
         deck.load(tmp_path / "c")
+    tank = {"p0": 2e5, "T0": 300.0, "massFrac": [0.5, 0.6]}
+    write_deck(dict(case, name="x", bc=["tank"] * 6, bc_tank=[tank] * 6), tmp_path / "d")
+    with pytest.raises(deck.DeckError, match="sum of massFrac"):
+        deck.load(tmp_path / "d")
     (tmp_path / "a" / "case" / "meshV5.xml").write_text(
         '<?xml version="1.0"?><mesh><type structure="unstructured"/></mesh>')
     with pytest.raises(deck.DeckError, match="cartesian"):
@@ -60,7 +67,7 @@ def test_deck_refusals(tmp_path):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("name", ["c1_kapila_fo_wall", "kapila_2d_fo_wall", "multip_1d_o2_relax",
-                                  "kapila_1d_o2_injection"])
+                                  "kapila_1d_o2_injection", "multip_1d_o2_relax_tank"])
 def test_deck_runs_on_the_device(name, tmp_path):
     """python -m ecogen_amd.deck on a reference deck: the final state matches
     the reference's output (1e-12 relative L1, the parity bar)."""

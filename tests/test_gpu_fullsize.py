@@ -135,3 +135,31 @@ def test_c2_full_size_injection_mass_inflow():
         want = inj["alpha"][k] * inj["rho"][k] / rho0 * inj["m0"] * 1.0 * s.t
         print(f"  phase {k}: mass gained {m_end - m_start[k]:.6e}, m0 Y S t {want:.6e}", flush=True)
         assert abs((m_end - m_start[k]) - want) <= 1e-12 * m_end + 1e-9 * want
+
+
+def test_c2_full_size_tank_equilibrium():
+    """4096^2, first order, a tank (BoundCondTank) at p0 = 2e5 on every side
+    of a box of air/water mixture at rest at 2e5: the tanks meet the subsonic
+    outflow branch with p* = p0 and a contact speed of round-off size, so the
+    box stays at rest and at p0 (the oracle shows the same on a tube)."""
+    from cases import AIR, WATER
+    from ecogen_amd import Context
+    n = 4096
+    tank = {"p0": 2e5, "T0": 300.0, "alpha": [0.7, 0.3]}
+    case = {"model": "kapila", "alpha_null": False, "eos": [AIR, WATER], "cells": [n, n, 1],
+            "lengths": [1.0, 1.0, 1.0], "order": 1, "limiters": ["none"] * 4, "cfl": 0.8,
+            "time_mode": "iterations", "steps": 10, "bc": ["tank"] * 4 + ["abs"] * 2,
+            "bc_tank": [tank] * 4 + [None] * 2}
+    st = np.zeros((10, n * n))
+    st[0], st[1], st[3], st[4] = 0.7, 2.3, 0.3, 1000.0
+    st[2] = st[5] = 2e5
+    print("case 4096^2 mixture at rest, tanks at p0 on all four sides, fp64 first order, 10 steps", flush=True)
+    with Context(case) as ctx:
+        ctx.set_state(st)
+        s = ctx.run(nsteps=10)
+        assert s.steps == 10 and s.flags == 0
+        out = ctx.get_state()
+    umax = float(np.max(np.abs(out[6:8])))
+    dp = float(np.max(np.abs(out[2] - 2e5)))
+    print(f"  max |u| {umax:.2e}, max |p - p0| {dp:.2e}", flush=True)
+    assert umax < 1e-9 and dp < 1e-9 * 2e5

@@ -455,3 +455,41 @@ def test_injection_mass_inflow(name):
         m_start, m_end = mk(st) * vol, mk(out) * vol
         want = y * inj["m0"] * area * t
         assert abs((m_end - m_start) - want) <= 1e-12 * m_end + 1e-10 * want, (k, m_end - m_start, want)
+
+
+@pytest.mark.parametrize("model", ["kapila", "multip", "euler"])
+def test_tank_equilibrium_and_direction(model):
+    """A tank (BoundCondTank) is a reservoir at rest at p0.  A fluid at rest
+    at p0 in a tube between two such tanks meets the subsonic-outflow branch
+    with p* = p0 and a contact speed of round-off size, so it stays at rest
+    and at p0.  A tank above the tube's pressure takes the inflow branch and
+    mass enters through both ends; a tank below it draws mass out.  The
+    reference's own tank goldens pin the flux values."""
+    n = 50
+    if model == "euler":
+        case = _euler_1d(1, ["none"] * 4, 100, cells=n)
+        st = np.zeros((6, n))
+        st[0], st[1], st[2] = 1.0, 1.2, 2e5
+        mass = lambda s: s[1].sum()
+    else:
+        case = dict(_c1(False, cells=n), model=model, relax_p=(model == "multip"), time_mode="iterations", steps=100)
+        st = np.zeros((9, n))
+        st[0], st[1], st[3], st[4] = 0.7, 2.3, 0.3, 1000.0
+        st[2] = st[5] = 2e5
+        mass = lambda s: (s[0] * s[1] + s[3] * s[4]).sum()
+    case["bc"] = ["tank", "tank", "abs", "abs", "abs", "abs"]
+    frac = {} if model == "euler" else {"alpha": [0.7, 0.3]}
+    for p0, sign in ((2e5, 0), (3e5, 1), (1.5e5, -1)):
+        case["bc_tank"] = [dict(p0=p0, T0=300.0, **frac)] * 2 + [None] * 4
+        steps = 100 if sign == 0 else 10   # the direction before the end waves meet
+        out, it, _, _, _ = O.run(case, st, nsteps=steps)
+        assert it == steps
+        u = out[6] if model != "euler" else out[3]
+        p = out[2]
+        if sign == 0:
+            assert np.max(np.abs(u)) < 1e-9
+            assert np.max(np.abs(p - 2e5)) < 1e-9 * 2e5
+        else:
+            dm = mass(out) - mass(st)
+            assert sign * dm > 1e-6 * mass(st), (p0, dm)
+            assert sign * u[-1] < 0 < sign * u[0]  # the flow runs inward (or outward) at both ends
