@@ -932,7 +932,8 @@ int launch_t(const StageArgs &A, int stage, hipStream_t st) {
   dim3 grid((unsigned)(tgx * tgy * tgz)), block(NT);
   const bool same = A.sch.same_lim != 0;
   const bool wall = (A.sch.bc_wall | A.sch.bc_out | A.sch.bc_inj | A.sch.bc_tank) != 0;  // kernels with boundary code
-  if (stage == 0) hipLaunchKernelGGL((k_stage<MODEL, NPH, DIM, 0, true>), grid, block, 0, st, A);
+  if (stage == 0 && wall) hipLaunchKernelGGL((k_stage<MODEL, NPH, DIM, 0, true>), grid, block, 0, st, A);
+  else if (stage == 0) hipLaunchKernelGGL((k_stage<MODEL, NPH, DIM, 0, true, false>), grid, block, 0, st, A);
   else if (stage == 1 && same && wall) hipLaunchKernelGGL((k_stage<MODEL, NPH, DIM, 1, true>), grid, block, 0, st, A);
   else if (stage == 1 && same) hipLaunchKernelGGL((k_stage<MODEL, NPH, DIM, 1, true, false>), grid, block, 0, st, A);
   else if (stage == 1 && wall) hipLaunchKernelGGL((k_stage<MODEL, NPH, DIM, 1, false>), grid, block, 0, st, A);
