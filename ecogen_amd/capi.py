@@ -27,7 +27,7 @@ WARN_RELAX_NOT_CONVERGED = 1
 WARN_NAN = 2
 WARN_INFLOW_NOT_CONVERGED = 4
 
-EXPORTS = ["eco_create", "eco_set_state", "eco_set_state_device", "eco_get_state", "eco_step", "eco_run",
+EXPORTS = ["eco_create", "eco_set_state", "eco_set_state_device", "eco_set_clock", "eco_get_state", "eco_step", "eco_run",
            "eco_launch_steps", "eco_sync", "eco_time_steps", "eco_copy_calibration", "eco_state_device", "eco_destroy", "eco_strerror", "eco_version"]
 
 
@@ -76,6 +76,8 @@ def lib():
         L.eco_set_state.restype = C.c_int
         L.eco_set_state_device.argtypes = [vp, vp]
         L.eco_set_state_device.restype = C.c_int
+        L.eco_set_clock.argtypes = [vp, C.c_double, C.c_double]
+        L.eco_set_clock.restype = C.c_int
         L.eco_get_state.argtypes = [vp, dp]
         L.eco_get_state.restype = C.c_int
         L.eco_step.argtypes = [vp, C.POINTER(EcoStatus)]
@@ -190,6 +192,10 @@ class Context:
         s[: st.shape[0]] = st.reshape(st.shape[0], self.ncell)
         s = np.ascontiguousarray(s)
         _check(lib().eco_set_state(self._h, s.ctypes.data_as(C.POINTER(C.c_double))), "eco_set_state")
+
+    def set_clock(self, t: float, dt: float):
+        """Resume at physical time t with dt for the first step (eco_set_clock)."""
+        _check(lib().eco_set_clock(self._h, C.c_double(t), C.c_double(dt)), "eco_set_clock")
 
     def set_state_device(self, ptr: int):
         _check(lib().eco_set_state_device(self._h, C.c_void_p(ptr)), "eco_set_state_device")

@@ -107,6 +107,11 @@ void Run::setState(const std::vector<double> &state) {
   if (rc) throw EcoError(rc, "eco_set_state");
 }
 
+void Run::resumeClock(double physicalTime, double dt) {
+  int rc = eco_set_clock(m_ctx, physicalTime, dt);
+  if (rc) throw EcoError(rc, "eco_set_clock");
+}
+
 std::vector<double> Run::getState() {
   std::vector<double> s((size_t)ECO_NFIELDS(m_cfg.nphase) * numberCells());
   int rc = eco_get_state(m_ctx, s.data());

@@ -413,6 +413,16 @@ extern "C" int eco_set_state(eco_ctx *x, const double *host_state) {
   return finish_set_state(x);
 }
 
+extern "C" int eco_set_clock(eco_ctx *x, double t, double dt) {
+  if (!x || !(t >= 0.) || !(dt > 0.)) return ECO_ERR_INVALID;
+  if (!x->have_state || x->step != 0) return ECO_ERR_STATE;
+  // plain fields of the control block, read by the next launches on this stream
+  hipError_t e = hipMemcpyAsync(&x->ctl->t0, &t, sizeof(double), hipMemcpyHostToDevice, x->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(&x->ctl->dt0, &dt, sizeof(double), hipMemcpyHostToDevice, x->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(x->stream);
+  return hipcode(e);
+}
+
 extern "C" int eco_set_state_device(eco_ctx *x, const double *dev_state) {
   if (!x || !dev_state) return ECO_ERR_INVALID;
   std::vector<int> map;
@@ -434,9 +444,9 @@ static int read_status(eco_ctx *x, eco_status *st, Control *hc) {
   if (hc) *hc = h;
   if (st) {
     st->steps = h.steps;
-    st->t = (h.steps > 0) ? h.t_now : 0.;
+    st->t = (h.steps > 0) ? h.t_now : h.t0;
     st->dt_last = h.dt_last;
-    double dtn = 1.e-15;
+    double dtn = (h.dt0 > 0.) ? h.dt0 : 1.e-15;
     if (h.steps > 0) {
       // the context's step k runs at launch slot k
       unsigned long long b = h.smax[(h.steps - 1) % 3];

@@ -92,6 +92,8 @@ struct Control {
   double dt_last;              // dt used by the last step taken
   double t_now;                // physical time after the last step taken
   double t_final;              // <0: iteration mode
+  double t0;                   // physical time before the first step (0, or a resumed run's)
+  double dt0;                  // dt of the first step (0: the reference's 1e-15)
 };
 
 }  // namespace eco

@@ -316,7 +316,9 @@ __device__ __forceinline__ bool step_begin(Control *ctl, const Geometry &G, doub
     }
     return false;
   }
-  double dt = 1.e-15;  // m_dt of Run::Run (src/Run.cpp:43)
+  // m_dt of Run::Run (src/Run.cpp:43), or the dt a resumed run reads back
+  // (Output::readInfos, src/InputOutput/Output.cpp:370-394)
+  double dt = (ctl->dt0 > 0.) ? ctl->dt0 : 1.e-15;
   if (cnt > 0) {
     // written by the previous step's kernels: visible at this launch
     const unsigned long long bits = ctl->smax[(slot + 2) % 3];
@@ -343,7 +345,7 @@ __device__ __forceinline__ void step_end(Control *ctl, double m, int slot, int b
   }
   if (last_kernel && blk == 0) {
     const int cur = slot & 1, prv = cur ^ 1;
-    const double t = ((si.cnt == 0) ? 0. : ctl->t[prv]) + si.dt;
+    const double t = ((si.cnt == 0) ? ctl->t0 : ctl->t[prv]) + si.dt;
     const int n = si.cnt + 1;
     ctl->t[cur] = t;
     ctl->cnt[cur] = n;

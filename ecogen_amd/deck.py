@@ -26,6 +26,11 @@ description of :class:`ecogen_amd.Context` and the initial state in the
   and ``BoundCondSymmetryO2`` at second order, which differ only in name
   (``src/BoundConds/BoundCondSymmetryO2.h:41``).
 
+* ``<resumeSimulation fileNumber="N"/>``: the run restarts from result file N
+  and row N of ``infoCalcul.out`` under ``RUNDIR/results/<run>/``, with the
+  dt printed there for its first step (``Run::resumeSimulation``,
+  ``src/Run.cpp:164-190``); ``--vtk`` continues the files and ``infoCalcul.out``.
+
 Anything else raises ``DeckError`` naming the element.
 
 Command line (runs the deck on the GPU and saves the final state; ``--vtk``
@@ -222,6 +227,13 @@ def load(rundir):
         "binary": _true(om.get("binary")) if om is not None else False,
         "precision": int(float(om.get("precision", "0"))) if om is not None else 0,
     }
+    rs = main.find("resumeSimulation")
+    if rs is not None:
+        # Input.cpp:253-257: restart from result file number fileNumber
+        try:
+            case["resume"] = int(rs.get("fileNumber", ""))
+        except ValueError:
+            raise DeckError("mainV5.xml: <resumeSimulation> lacks an integer fileNumber") from None
     so = main.find("secondOrder")
     if so is None:
         case["order"], case["limiters"] = 1, ["none"] * 4
@@ -383,7 +395,87 @@ def _tank(b, eos):
     return tk
 
 
-def _run_with_output(ctx, case, outdir):
+def read_vtr(path):
+    """{name: values} of the CellData arrays of a result file in the
+    reference's VTK XML format: ASCII as printed, or base64 (a 32-bit byte
+    count, then little-endian Float32 data, each block encoded on its own,
+    Output::ecritJeuDonnees, Output.cpp:219-266) widened to double.  Vector
+    arrays come as (ncell, components)."""
+    import base64
+    import xml.etree.ElementTree as ET
+    root = ET.parse(path).getroot()
+    cd = root.find(".//CellData")
+    if cd is None:
+        raise DeckError(f"{Path(path).name}: no <CellData>")
+    out = {}
+    for da in cd.findall("DataArray"):
+        text = (da.text or "").strip()
+        if da.get("format") == "binary":
+            head = base64.b64decode(text[:8])
+            nbytes = int(np.frombuffer(head, dtype="<i4")[0])
+            v = np.frombuffer(base64.b64decode(text[8:]), dtype="<f4")[: nbytes // 4].astype(np.float64)
+        else:
+            v = np.array(text.split(), dtype=np.float64)
+        nc = int(da.get("NumberOfComponents", "1"))
+        out[da.get("Name")] = v.reshape(-1, nc) if nc > 1 else v
+    return out
+
+
+def _array(arrs, prefix, where):
+    keys = [k for k in arrs if k.startswith(prefix)]
+    if not keys:
+        raise DeckError(f"{where}: no array {prefix}*")
+    return arrs[keys[0]]
+
+
+def resume_point(case, rundir):
+    """Run::resumeSimulation (src/Run.cpp:164-190) for result file N =
+    case["resume"] under RUNDIR/results/<run>/: Output::readInfos
+    (Output.cpp:370-405) finds row N of infoCalcul.out (one CPU) and takes its
+    iteration, physical time and dt; OutputXML::readResults (OutputXML.cpp:
+    94-200) reads the file's phase and mixture arrays; the primitives follow
+    Phase*::setScalar / Mix*::setScalar, with every Kapila phase at the
+    mixture pressure (ModKapila::fulfillState, ModKapila.cpp:87-99).
+    Returns (host state, iteration, t, dt, infoCalcul.out lines kept)."""
+    n = case["resume"]
+    rdir = Path(rundir) / "results" / case["name"]
+    info = rdir / "infoCalcul.out"
+    if not info.exists():
+        raise DeckError(f"resume: missing {info}")
+    lines = info.read_text().split("\n")
+    if not lines or lines[0].split()[:1] != ["1"]:
+        raise DeckError("resume simulation not possible - number of CPU differs from read files")
+    row = None
+    for k, ln in enumerate(lines[1:], start=1):
+        f = ln.split()
+        if f and int(f[0]) == n:
+            row, kept = f, lines[: k + 1]
+            break
+    if row is None:
+        raise DeckError("resume simulation not possible - check file 'infoCalcul.out'")
+    it, t, dt = int(row[1]), float(row[2]), float(row[4])
+    where = f"result_CPU0_AMR0_TIME{n}.vtr"
+    arrs = read_vtr(rdir / where)
+    model, nph = case["model"], len(case["eos"])
+    ncell = int(np.prod(case["cells"]))
+    state = np.zeros((3 * nph + 3, ncell))
+    if model == "euler":
+        state[0] = 1.0
+        state[1] = _array(arrs, "F0_Density_", where)
+        state[2] = _array(arrs, "F0_Pressure_", where)
+        vel = _array(arrs, "F0_Velocity_", where)
+    else:
+        pm = _array(arrs, "Pressure_Mixture", where)
+        for k in range(nph):
+            state[3 * k] = _array(arrs, f"F{k}_Alpha_", where)
+            state[3 * k + 1] = _array(arrs, f"F{k}_Density_", where)
+            state[3 * k + 2] = pm if model == "kapila" else _array(arrs, f"F{k}_Pressure_", where)
+        vel = _array(arrs, "Velocity_Mixture", where)
+    state[3 * nph:3 * nph + 3] = np.asarray(vel).reshape(ncell, 3).T
+    return state, it, t, dt, kept
+
+
+def _run_with_output(ctx, case, outdir, resume=None):
     """March to the deck's end, writing a result file at t = 0 and after every
     iterFreq steps (iteration control) or at the first step reaching each
     multiple of timeFreq (physical-time control), as the reference does, plus
@@ -396,31 +488,41 @@ def _run_with_output(ctx, case, outdir):
     o = case["output"]
     outdir.mkdir(parents=True, exist_ok=True)
     prec = o["precision"] or 6  # the stream's default precision when the deck sets none
-    files = []
     t0 = time.perf_counter()
     info = open(outdir / "infoCalcul.out", "w")
-    info.write("1\n")
+    if resume is None:
+        files, it0 = [], 0
+        info.write("1\n")
+    else:
+        # the resumed run keeps infoCalcul.out up to row N (Output::readInfos)
+        # and numbers its files from N + 1
+        n0, it0, kept = resume
+        files = [outdir / f"result_CPU0_AMR0_TIME{k}.vtr" for k in range(n0 + 1)]
+        info.write("\n".join(kept) + "\n")
 
     def dump(st=None):
         f = outdir / f"result_CPU0_AMR0_TIME{len(files)}.vtr"
         output.write_vtr(f, case, ctx.get_state(), binary=o["binary"], precision=prec)
-        it, t, dt = (0, 0.0, 1.e-15) if st is None else (st.steps, st.t, st.dt_last)
+        it, t, dt = (0, 0.0, 1.e-15) if st is None else (it0 + st.steps, st.t, st.dt_last)
         ms = int((time.perf_counter() - t0) * 1e3) if st is not None else 0
         info.write(f"{len(files)} {it} {t:.{prec}g} {ms} {dt:.{prec}g}\n")
         files.append(f)
-    dump()
+    st = None
+    if resume is None:
+        dump()
     if case["time_mode"] == "iterations":
+        # a result file whenever the iteration is a multiple of iterFreq
         step = o["iter_freq"] if o["iter_freq"] > 0 else case["steps"]
-        done = 0
+        done = it0
         while done < case["steps"]:
-            n = min(step, case["steps"] - done)
+            n = min(step - done % step, case["steps"] - done)
             st = ctx.run(nsteps=n)
-            done += n
+            done = it0 + st.steps
             dump(st)
     else:
         tf = case["t_final"]
         freq = o["time_freq"] if o["time_freq"] > 0 else tf
-        k = 1
+        k = len(files) if resume is not None else 1
         while True:
             target = min(k * freq, tf)
             st = ctx.run(t_final=target)
@@ -444,17 +546,24 @@ def main(argv=None):
     from .capi import Context
     case, state = load(args.rundir)
     case["precision"] = args.precision
+    it0, resume = 0, None
+    if case.get("resume"):
+        state, it0, t0, dt0, kept = resume_point(case, args.rundir)
+        resume = (case["resume"], it0, kept)
     with Context(case) as ctx:
         ctx.set_state(state)
+        if resume is not None:
+            ctx.set_clock(t0, dt0)
         if args.vtk:
-            st = _run_with_output(ctx, case, Path(args.rundir) / "results" / case["name"])
+            st = _run_with_output(ctx, case, Path(args.rundir) / "results" / case["name"], resume)
         elif case["time_mode"] == "iterations":
-            st = ctx.run(nsteps=case["steps"])
+            st = ctx.run(nsteps=case["steps"] - it0) if case["steps"] > it0 else ctx.sync()
         else:
             st = ctx.run(t_final=case["t_final"])
         out = ctx.get_state()
-    np.savez(args.out, state=out, steps=st.steps, t=st.t, case_json=json.dumps(case))
-    print(f"{case['name']}: {st.steps} steps, t = {st.t!r} -> {args.out}")
+    steps = it0 + st.steps
+    np.savez(args.out, state=out, steps=steps, t=st.t, case_json=json.dumps(case))
+    print(f"{case['name']}: iteration {steps}, t = {st.t!r} -> {args.out}")
 
 
 if __name__ == "__main__":

@@ -117,6 +117,14 @@ int eco_set_state(eco_ctx *ctx, const double *host_state);
 /* Same from a device pointer (same layout), no host copy. */
 int eco_set_state_device(eco_ctx *ctx, const double *dev_state);
 
+/* Replaces the clock part of Run::resumeSimulation (src/Run.cpp:164-190,
+   Output::readInfos, src/InputOutput/Output.cpp:370-405): after eco_set_state
+   and before the first step, start the physical time at t and run the first
+   step with dt (the reference reads both from infoCalcul.out; its first
+   resumed step uses that dt, not cfl * dtMax of the resumed state).  Later
+   steps take the lagged dt as usual; status->steps counts from 0. */
+int eco_set_clock(eco_ctx *ctx, double t, double dt);
+
 /* Downloads the current state (layout above, E included). Synchronises. */
 int eco_get_state(eco_ctx *ctx, double *host_state);
 

@@ -207,6 +207,9 @@ class Run {
   long long numberCells() const;
   // state in the ecogen.h host layout
   void setState(const std::vector<double> &state);
+  // Run::resumeSimulation's clock: after setState, start at physicalTime and
+  // take dt for the first step (both read from infoCalcul.out)
+  void resumeClock(double physicalTime, double dt);
   std::vector<double> getState();
   // iteration control: nbIte steps; physical-time control: until t >= finalTime
   RunStatus solverIterations(int nbIte);

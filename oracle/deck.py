@@ -89,12 +89,18 @@ def write_deck(case, root: Path):
         '<?xml version="1.0" encoding="UTF-8" standalone="yes"?>\n<ecogen>\n'
         '  <testCase>./case/</testCase>\n</ecogen>\n')
     if case["time_mode"] == "iterations":
-        tc = (f'  <timeControlMode iterations="true">\n    <iterations number="{case["steps"]}" iterFreq="{case["steps"]}"/>\n'
+        freq = case.get("iter_freq", case["steps"])
+        tc = (f'  <timeControlMode iterations="true">\n    <iterations number="{case["steps"]}" iterFreq="{freq}"/>\n'
               f'    <physicalTime totalTime="1." timeFreq="1."/>\n  </timeControlMode>')
     else:
         tf = case["t_final"]
+        tfq = case.get("time_freq", tf)
         tc = (f'  <timeControlMode iterations="false">\n    <iterations number="1" iterFreq="1"/>\n'
-              f'    <physicalTime totalTime="{tf!r}" timeFreq="{tf!r}"/>\n  </timeControlMode>')
+              f'    <physicalTime totalTime="{tf!r}" timeFreq="{tfq!r}"/>\n  </timeControlMode>')
+    if case.get("resume"):
+        # Input.cpp:253-257: restart from result file number fileNumber
+        tc += f'\n  <resumeSimulation fileNumber="{case["resume"]}"/>'
+    
     so = ""
     if case["order"] == 2:
         lim = case["limiters"]

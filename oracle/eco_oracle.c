@@ -1635,6 +1635,17 @@ static void write_state(Solver *S, double *out) {
 int orc_run(const orc_cfg *cfg, const double *state_in, double *state_out,
             int nsteps, double t_final, int max_steps,
             double *t_out, double *dt_next_out, double *dt_log, int nthreads) {
+  return orc_run_from(cfg, state_in, state_out, nsteps, t_final, max_steps, 0., 1.e-15,
+                      t_out, dt_next_out, dt_log, nthreads);
+}
+
+/* Run::resumeSimulation (src/Run.cpp:136-190): the state is initialised from
+   the primitives as at t = 0 (Cell::completeFulfillState), the physical time
+   starts at t0 and the first step uses dt0 (both read from infoCalcul.out by
+   Output::readInfos, src/InputOutput/Output.cpp:370-394). */
+int orc_run_from(const orc_cfg *cfg, const double *state_in, double *state_out,
+                 int nsteps, double t_final, int max_steps, double t0, double dt0,
+                 double *t_out, double *dt_next_out, double *dt_log, int nthreads) {
   if (!cfg || cfg->nph < 1 || cfg->nph > MAXPH) return -1;
   if (cfg->model == ORC_EULER && cfg->nph != 1) return -1;
   if (is_thinc(cfg) && cfg->nph != 2) return -1;
@@ -1662,7 +1673,7 @@ int orc_run(const orc_cfg *cfg, const double *state_in, double *state_out,
   if (!S.sn || !S.s2 || !S.cons || !S.fflux || !S.fslope || !S.fdt) return -2;
   init_state(&S, state_in);
 
-  double dt = 1.e-15, t = 0.;
+  double dt = dt0, t = t0;
   int it = 0;
   float tf = (float)t_final;
   for (;;) {

@@ -81,6 +81,12 @@ int orc_run(const orc_cfg *cfg, const double *state_in, double *state_out,
             int nsteps, double t_final, int max_steps,
             double *t_out, double *dt_next_out, double *dt_log, int nthreads);
 
+/* orc_run resumed: physical time from t0, first step with dt0 (orc_run is
+   t0 = 0, dt0 = 1e-15). */
+int orc_run_from(const orc_cfg *cfg, const double *state_in, double *state_out,
+                 int nsteps, double t_final, int max_steps, double t0, double dt0,
+                 double *t_out, double *dt_next_out, double *dt_log, int nthreads);
+
 /* Single-function checks used by the known-answer tests. */
 double orc_limiter(int lim, double s1, double s2);
 double orc_eos_energy(const orc_eos *e, double rho, double p, double epsilon);

@@ -59,6 +59,9 @@ def lib():
         dp = C.POINTER(C.c_double)
         L.orc_run.argtypes = [C.POINTER(OrcCfg), dp, dp, C.c_int, C.c_double, C.c_int, dp, dp, dp, C.c_int]
         L.orc_run.restype = C.c_int
+        L.orc_run_from.argtypes = [C.POINTER(OrcCfg), dp, dp, C.c_int, C.c_double, C.c_int, C.c_double, C.c_double,
+                                   dp, dp, dp, C.c_int]
+        L.orc_run_from.restype = C.c_int
         L.orc_limiter.argtypes = [C.c_int, C.c_double, C.c_double]
         L.orc_limiter.restype = C.c_double
         for name in ("orc_eos_energy", "orc_eos_sound"):
@@ -126,8 +129,9 @@ def nfields(case) -> int:
     return 3 * len(case["eos"]) + 4
 
 
-def run(case, state, nsteps=None, t_final=None, max_steps=0, nthreads=0, log_dt=False):
+def run(case, state, nsteps=None, t_final=None, max_steps=0, nthreads=0, log_dt=False, resume=None):
     """March ``state`` (field-major, shape (nfields, ncell) or (nfields-1, ncell)).
+    ``resume`` = (t0, dt0) starts a resumed run (orc_run_from).
 
     Returns (state_out, steps, t, dt_next, dt_log or None).
     """
@@ -146,8 +150,10 @@ def run(case, state, nsteps=None, t_final=None, max_steps=0, nthreads=0, log_dt=
     dtn = np.zeros(1)
     cap = nsteps if nsteps else (max_steps if max_steps else 10_000_000)
     dtl = np.zeros(cap + 1) if log_dt else None
-    it = lib().orc_run(C.byref(cfg), _dp(sin), _dp(sout), int(nsteps or 0), float(t_final or 0.0),
-                       int(max_steps), _dp(t), _dp(dtn), _dp(dtl) if log_dt else None, int(nthreads))
+    t0, dt0 = resume if resume else (0.0, 1e-15)
+    it = lib().orc_run_from(C.byref(cfg), _dp(sin), _dp(sout), int(nsteps or 0), float(t_final or 0.0),
+                            int(max_steps), float(t0), float(dt0), _dp(t), _dp(dtn),
+                            _dp(dtl) if log_dt else None, int(nthreads))
     if it < 0:
         raise RuntimeError(f"oracle error {it}")
     return sout, it, float(t[0]), float(dtn[0]), (dtl[:it] if log_dt else None)
