@@ -224,6 +224,7 @@ def load(rundir):
     case["output"] = {
         "iter_freq": int(float(it_x.get("iterFreq", "0"))) if it_x is not None else 0,
         "time_freq": float(pt_x.get("timeFreq", "0")) if pt_x is not None else 0.0,
+        "format": (om.get("format") or "XML").upper() if om is not None else "XML",
         "binary": _true(om.get("binary")) if om is not None else False,
         "precision": int(float(om.get("precision", "0"))) if om is not None else 0,
     }
@@ -234,6 +235,10 @@ def load(rundir):
             case["resume"] = int(rs.get("fileNumber", ""))
         except ValueError:
             raise DeckError("mainV5.xml: <resumeSimulation> lacks an integer fileNumber") from None
+    if case["output"]["format"] not in ("XML", "GNU"):
+        raise DeckError(f"mainV5.xml: output format {case['output']['format']!r} (XML or GNU)")
+    if case["output"]["format"] == "GNU" and case["output"]["binary"]:
+        raise DeckError("mainV5.xml: OutputGNU writes no binary data (OutputGNU.cpp:200-205)")
     so = main.find("secondOrder")
     if so is None:
         case["order"], case["limiters"] = 1, ["none"] * 4
@@ -252,6 +257,9 @@ def load(rundir):
     ncel = _child(cm, "numberCells", "meshV5.xml")
     case["lengths"] = [_f(dims, a, "meshV5.xml") for a in "xyz"]
     case["cells"] = [int(_f(ncel, a, "meshV5.xml")) for a in "xyz"]
+    if case["output"]["format"] == "GNU" and all(c > 1 for c in case["cells"]):
+        # OutputGNU::ecritScriptGnuplot stops the run at its first output in 3-D
+        raise DeckError("mainV5.xml: GNU output in 3-D (OutputGNU.cpp:110)")
 
     # model and EOS (Input::entreeModel / entreeEOS)
     fm = _child(model_x, "flowModel", "modelV4.xml")
@@ -438,6 +446,10 @@ def resume_point(case, rundir):
     mixture pressure (ModKapila::fulfillState, ModKapila.cpp:87-99).
     Returns (host state, iteration, t, dt, infoCalcul.out lines kept)."""
     n = case["resume"]
+    if case["output"]["format"] != "XML":
+        raise DeckError("readResutls not available for requested output format")  # Output.h:79
+    if case["output"]["binary"]:
+        raise DeckError("resuming on binary results file not available")  # Output::getJeuDonnees, Output.cpp:279
     rdir = Path(rundir) / "results" / case["name"]
     info = rdir / "infoCalcul.out"
     if not info.exists():
@@ -500,9 +512,16 @@ def _run_with_output(ctx, case, outdir, resume=None):
         files = [outdir / f"result_CPU0_AMR0_TIME{k}.vtr" for k in range(n0 + 1)]
         info.write("\n".join(kept) + "\n")
 
+    gnu = o["format"] == "GNU"
+
     def dump(st=None):
-        f = outdir / f"result_CPU0_AMR0_TIME{len(files)}.vtr"
-        output.write_vtr(f, case, ctx.get_state(), binary=o["binary"], precision=prec)
+        if gnu:  # OutputGNU::ecritSolution: the file, then the script over all files so far
+            f = outdir / f"result_CPU0_TIME{len(files)}.out"
+            output.write_gnu(f, case, ctx.get_state())
+            output.write_gnu_script(outdir / "visualisation.gnu", case, len(files) + 1)
+        else:
+            f = outdir / f"result_CPU0_AMR0_TIME{len(files)}.vtr"
+            output.write_vtr(f, case, ctx.get_state(), binary=o["binary"], precision=prec)
         it, t, dt = (0, 0.0, 1.e-15) if st is None else (it0 + st.steps, st.t, st.dt_last)
         ms = int((time.perf_counter() - t0) * 1e3) if st is not None else 0
         info.write(f"{len(files)} {it} {t:.{prec}g} {ms} {dt:.{prec}g}\n")
@@ -530,7 +549,8 @@ def _run_with_output(ctx, case, outdir, resume=None):
             if st.t >= float(np.float32(tf)) or target >= tf:
                 break
             k += 1
-    output.write_collection(outdir / "collection.pvd", files)
+    if not gnu:
+        output.write_collection(outdir / "collection.pvd", files)
     info.close()
     return st
 

@@ -192,3 +192,62 @@ def write_collection(path, files):
     Path(path).write_text('<?xml version="1.0"?>\n<VTKFile type="Collection" version="0.1" byte_order="LittleEndian" '
                           'compressor="vtkZLibDataCompressor">\n    <Collection>\n' + rows +
                           "    </Collection>\n</VTKFile>\n")
+
+
+def _gnu_columns(case, state):
+    """(title, values) per column after the coordinates: the phase and mixture
+    variables of cell_data, vectors as their norm sqrt(x^2 + y^2 + z^2)
+    (Phase::printPhase, Mixture::printMixture, Coord::norm), then the AMR
+    level and the refinement indicator xi, both 0 on a fixed mesh
+    (Cell::printGnuplotAMR, src/Cell.cpp:1523-1556)."""
+    cols = []
+    for name, nc, vals in cell_data(case, np.asarray(state, dtype=np.float64)):
+        if name == "gradRho":
+            continue
+        title = name.split("_", 1)[1] if name[:1] == "F" and name[1:2].isdigit() else name
+        if nc > 1:
+            v = np.asarray(vals).reshape(-1, nc)
+            vals = np.sqrt(v[:, 0] * v[:, 0] + v[:, 1] * v[:, 1] + v[:, 2] * v[:, 2])
+        cols.append((title, vals))
+    n = int(np.prod(case["cells"]))
+    cols += [("Niveau AMR", np.zeros(n)), ("Xi", np.zeros(n))]
+    return cols
+
+
+def write_gnu(path, case, state):
+    """One result file of OutputGNU::ecritSolution (OutputGNU.cpp:59-75): per
+    cell in cell order its centre (x, then y and z by dimension) and the
+    columns of _gnu_columns, each value at the stream's default 6 significant
+    digits followed by a space, one line per cell, then an empty line."""
+    n = case["cells"]
+    dim = sum(1 for c in n if c > 1)
+    pos, _ = _positions(case)
+    X = np.broadcast_to(pos[0][None, None, :], (n[2], n[1], n[0])).reshape(-1)
+    Y = np.broadcast_to(pos[1][None, :, None], (n[2], n[1], n[0])).reshape(-1)
+    Z = np.broadcast_to(pos[2][:, None, None], (n[2], n[1], n[0])).reshape(-1)
+    data = [X, Y, Z][:dim] + [v for _, v in _gnu_columns(case, state)]
+    rows = np.stack(data, axis=1)
+    text = "".join("".join(f"{v:.6g} " for v in r) + "\n" for r in rows)
+    Path(path).write_text(text + "\n")
+
+
+def write_gnu_script(path, case, nfiles):
+    """visualisation.gnu of OutputGNU::ecritScriptGnuplot (OutputGNU.cpp:79-165)
+    for result files 0 .. nfiles-1 of one CPU: one plot (1-D) or splot with
+    contours (2-D) per column.  The reference writes none in 3-D."""
+    n = case["cells"]
+    dim = sum(1 for c in n if c > 1)
+    if dim == 3:
+        raise ValueError("no gnuplot script in 3-D (OutputGNU.cpp:110)")
+    out = ["reset", "set style data lines", "set nokey", "", "set xlabel 'x (m)'"]
+    if dim == 2:
+        out += ["set surface", "set dgrid3d 50,50", "set contour base", "set cntrparam levels 25", "show contour"]
+    index = 1 + dim
+    for title, _ in _gnu_columns(case, np.ones((3 * len(case["eos"]) + 4, int(np.prod(n))))):
+        out.append(f"set title '{title}'")
+        use = f"u 1:{index}" if dim <= 1 else f"u 1:2:{index}"
+        files = [f'"result_CPU0_TIME{t}.out" {use}' for t in range(nfiles)]
+        out.append(("plot " if dim <= 1 else "splot ") + ",\\\n ".join(files))
+        out += ["pause(-1)", ""]
+        index += 1
+    Path(path).write_text("\n".join(out) + "\n")

@@ -3,7 +3,9 @@
 Runs the reference executable (``oracle/_ref/ECOGEN``, built by
 ``oracle/ref.mk``) on small decks and keeps its last result file (ASCII,
 17 digits), its ``collection.pvd`` and the case description, under
-``tests/golden/vtk/<case>/``.  The files are the reference's outputs: data.
+``tests/golden/vtk/<case>/``; for 1-D and 2-D decks also the last result file
+and ``visualisation.gnu`` of the same deck with GNU output.  The files are the
+reference's outputs: data.
 
     python tests/golden/make_vtk_fixtures.py
 """
@@ -52,6 +54,24 @@ def main():
             shutil.copy(rdir / "collection.pvd", out / "collection.pvd")
             (out / "case.json").write_text(json.dumps(case))
             print(f"{name}: {vtrs[-1].name} -> {out}")
+        finally:
+            shutil.rmtree(tmp, ignore_errors=True)
+        if sum(1 for c in case["cells"] if c > 1) == 3:
+            continue  # the reference writes no gnuplot script in 3-D (OutputGNU.cpp:110)
+        # the same deck with GNU output (OutputGNU): the run is deterministic, so
+        # its last file holds the state of final.vtr
+        tmp = Path(tempfile.mkdtemp(prefix="ecogen_gnu_"))
+        try:
+            write_deck(dict(case, name=name, output='format="GNU" binary="false"'), tmp)
+            r = subprocess.run([str(REF_BIN)], cwd=tmp, capture_output=True, text=True, timeout=600)
+            if r.returncode != 0:
+                raise RuntimeError(r.stdout[-2000:] + r.stderr[-2000:])
+            rdir = tmp / "results" / name
+            outs = sorted(rdir.glob("result_CPU0_TIME*.out"), key=lambda p: int(p.stem.rsplit("TIME", 1)[1]))
+            out = HERE / "vtk" / name
+            shutil.copy(outs[-1], out / "final.out")
+            shutil.copy(rdir / "visualisation.gnu", out / "visualisation.gnu")
+            print(f"{name}: {outs[-1].name}, visualisation.gnu -> {out}")
         finally:
             shutil.rmtree(tmp, ignore_errors=True)
 

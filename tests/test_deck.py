@@ -59,6 +59,9 @@ def test_deck_refusals(tmp_path):
     write_deck(dict(case, name="x", bc=["tank"] * 6, bc_tank=[tank] * 6), tmp_path / "d")
     with pytest.raises(deck.DeckError, match="sum of massFrac"):
         deck.load(tmp_path / "d")
+    write_deck(dict(case, name="x", output='format="GNU" binary="true"'), tmp_path / "e")
+    with pytest.raises(deck.DeckError, match="binary"):
+        deck.load(tmp_path / "e")
     (tmp_path / "a" / "case" / "meshV5.xml").write_text(
         '<?xml version="1.0"?><mesh><type structure="unstructured"/></mesh>')
     with pytest.raises(deck.DeckError, match="cartesian"):
@@ -138,3 +141,29 @@ def test_deck_info_file_matches_reference(name, tmp_path):
     assert np.array_equal(ours[:, :2], ref[:, :2])
     for col in (2, 4):
         assert np.allclose(ours[:, col], ref[:, col], rtol=1e-12, atol=0.0), col
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["kapila_2d_fo_wall_6x4", "euler_1d_fo_30"])
+def test_deck_gnu_results_match_reference_files(name, tmp_path):
+    """python -m ecogen_amd.deck --vtk on the deck with GNU output writes the
+    reference's result file and visualisation.gnu: the script character for
+    character, the rows with the reference's columns and every value equal
+    to its 6 printed digits (5e-6 relative; device and reference differ in
+    last bits, which 6 digits hide but for a rounding tie)."""
+    import json
+    from pathlib import Path
+    from ecogen_amd import deck
+    d = Path(__file__).resolve().parent / "golden" / "vtk" / name
+    case = json.loads((d / "case.json").read_text())
+    print(f"case {name} deck --vtk with GNU output", flush=True)
+    write_deck(dict(case, name=name, output='format="GNU" binary="false"'), tmp_path / "run")
+    deck.main([str(tmp_path / "run"), "--out", str(tmp_path / "f.npz"), "--vtk"])
+    res = tmp_path / "run" / "results" / name
+    assert (res / "visualisation.gnu").read_text() == (d / "visualisation.gnu").read_text()
+    ours = [ln.split() for ln in (res / "result_CPU0_TIME1.out").read_text().split("\n") if ln.strip()]
+    ref = [ln.split() for ln in (d / "final.out").read_text().split("\n") if ln.strip()]
+    assert [len(r) for r in ours] == [len(r) for r in ref]
+    a, b = np.array(ours, dtype=float), np.array(ref, dtype=float)
+    assert np.all(np.abs(a - b) <= 5e-6 * np.abs(b) + 1e-300)
+    assert (res / "infoCalcul.out").exists() and not (res / "collection.pvd").exists()

@@ -76,3 +76,30 @@ def test_collection_file(tmp_path):
     d = VTK / "euler_1d_fo_30"
     output.write_collection(tmp_path / "collection.pvd", ["result_CPU0_AMR0_TIME0.vtr", "result_CPU0_AMR0_TIME1.vtr"])
     assert (tmp_path / "collection.pvd").read_text() == (d / "collection.pvd").read_text()
+
+
+@pytest.mark.parametrize("name", sorted(p.name for p in VTK.iterdir() if (p / "final.out").exists()))
+def test_gnu_writer_reproduces_reference_files(name, tmp_path):
+    """OutputGNU: the reference's result file of the same deck with GNU output
+    and its visualisation.gnu, character for character (the phase
+    temperatures of MultiP with relaxation aside, as above)."""
+    import oracle as O
+    from deck import write_deck
+    from ecogen_amd import deck, output
+    d = VTK / name
+    case = json.loads((d / "case.json").read_text())
+    write_deck(dict(case, name=name), tmp_path / "run")
+    rcase, st0 = deck.load(tmp_path / "run")
+    state, _, _, _, _ = O.run(rcase, st0)
+    output.write_gnu(tmp_path / "ours.out", rcase, state)
+    ours, ref = (tmp_path / "ours.out").read_text(), (d / "final.out").read_text()
+    if case["model"] == "multip" and case.get("relax_p"):
+        dim = sum(1 for c in case["cells"] if c > 1)
+        titles = [t for t, _ in output._gnu_columns(rcase, state)]
+        skip = {dim + i for i, t in enumerate(titles) if t.startswith("Temperature")}
+        strip = lambda text: [[w for i, w in enumerate(ln.split(" ")) if i not in skip] for ln in text.split("\n")]
+        assert strip(ours) == strip(ref)
+    else:
+        assert ours == ref
+    output.write_gnu_script(tmp_path / "v.gnu", rcase, 2)
+    assert (tmp_path / "v.gnu").read_text() == (d / "visualisation.gnu").read_text()

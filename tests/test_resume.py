@@ -115,6 +115,11 @@ def test_resume_refusals(tmp_path):
     got, _ = deck.load(tmp_path / "a")
     with pytest.raises(deck.DeckError, match="check file"):
         deck.resume_point(dict(got, resume=case["resume"] + 50), tmp_path / "a")
+    # as the reference: no resume from binary or GNU results (Output.cpp:279, Output.h:79)
+    with pytest.raises(deck.DeckError, match="binary"):
+        deck.resume_point(dict(got, output=dict(got["output"], binary=True)), tmp_path / "a")
+    with pytest.raises(deck.DeckError, match="output format"):
+        deck.resume_point(dict(got, output=dict(got["output"], format="GNU")), tmp_path / "a")
     text = (rdir / "infoCalcul.out").read_text().split("\n")
     (rdir / "infoCalcul.out").write_text("\n".join(["2"] + text[1:]))
     with pytest.raises(deck.DeckError, match="number of CPU"):
