@@ -228,6 +228,17 @@ def load(rundir):
         "binary": _true(om.get("binary")) if om is not None else False,
         "precision": int(float(om.get("precision", "0"))) if om is not None else 0,
     }
+    # probes (Input.cpp:136-142, OutputProbeGNU.cpp:44-90); cuts are not written
+    for tag in ("cut1D", "cut2D"):
+        if main.find(tag) is not None:
+            raise DeckError(f"mainV5.xml: <{tag}> cuts are not written")
+    case["probes"] = []
+    for pr in main.findall("probe"):
+        where = "mainV5.xml probe"
+        v = _child(pr, "vertex", where)
+        case["probes"].append({"name": pr.get("name") or "probe",
+                               "vertex": [_f(v, a, where) for a in "xyz"],
+                               "acq_freq": _f(_child(pr, "timeControl", where), "acqFreq", where)})
     rs = main.find("resumeSimulation")
     if rs is not None:
         # Input.cpp:253-257: restart from result file number fileNumber
@@ -526,16 +537,52 @@ def _run_with_output(ctx, case, outdir, resume=None):
         ms = int((time.perf_counter() - t0) * 1e3) if st is not None else 0
         info.write(f"{len(files)} {it} {t:.{prec}g} {ms} {dt:.{prec}g}\n")
         files.append(f)
+    # probes (OutputProbeGNU): each file emptied at the start with its script;
+    # a row at t = 0 (not on resume), then after every step that reaches the
+    # next acquisition time, which then advances by acqFreq (Run.cpp:264-266)
+    probes = []
+    if case.get("probes"):
+        (outdir / "probes").mkdir(exist_ok=True)
+    for pr in case.get("probes", []):
+        pf = outdir / "probes" / f"{pr['name']}.out"
+        pf.write_text("")
+        output.write_gnu_script(outdir / "probes" / f"visualisation{pr['name']}.gnu", case, 1, dim=0, stem=pr["name"])
+        probes.append({"file": pf, "cell": output.probe_cell(case, pr["vertex"]), "freq": pr["acq_freq"], "next": 0.0})
+
+    def acquire(t, state, first=False):
+        for pb in probes:
+            if first or pb["next"] <= t:
+                with open(pb["file"], "a") as fh:
+                    fh.write(output.probe_row(case, state, t, pb["cell"]))
+                pb["next"] += pb["freq"]
+
+    def advance(nsteps=0, t_final=None):
+        """ctx.run for nsteps steps or up to t_final, one step at a time while
+        probes are to be sampled."""
+        if not probes:
+            return ctx.run(nsteps=nsteps) if t_final is None else ctx.run(t_final=t_final)
+        while True:
+            stp = ctx.run(nsteps=1) if t_final is None else ctx.run(t_final=t_final, max_steps=1)
+            acquire(stp.t, ctx.get_state())
+            if t_final is None:
+                nsteps -= 1
+                if nsteps <= 0:
+                    return stp
+            elif stp.t >= float(np.float32(t_final)):
+                return stp
+
     st = None
     if resume is None:
         dump()
+        if probes:
+            acquire(0.0, ctx.get_state(), first=True)
     if case["time_mode"] == "iterations":
         # a result file whenever the iteration is a multiple of iterFreq
         step = o["iter_freq"] if o["iter_freq"] > 0 else case["steps"]
         done = it0
         while done < case["steps"]:
             n = min(step - done % step, case["steps"] - done)
-            st = ctx.run(nsteps=n)
+            st = advance(nsteps=n)
             done = it0 + st.steps
             dump(st)
     else:
@@ -544,7 +591,7 @@ def _run_with_output(ctx, case, outdir, resume=None):
         k = len(files) if resume is not None else 1
         while True:
             target = min(k * freq, tf)
-            st = ctx.run(t_final=target)
+            st = advance(t_final=target)
             dump(st)
             if st.t >= float(np.float32(tf)) or target >= tf:
                 break

@@ -231,23 +231,78 @@ def write_gnu(path, case, state):
     Path(path).write_text(text + "\n")
 
 
-def write_gnu_script(path, case, nfiles):
+def write_gnu_script(path, case, nfiles, dim=None, stem="result"):
     """visualisation.gnu of OutputGNU::ecritScriptGnuplot (OutputGNU.cpp:79-165)
-    for result files 0 .. nfiles-1 of one CPU: one plot (1-D) or splot with
-    contours (2-D) per column.  The reference writes none in 3-D."""
-    n = case["cells"]
-    dim = sum(1 for c in n if c > 1)
+    for result files 0 .. nfiles-1 of one CPU (<stem>_CPU0_TIME<t>.out): one
+    plot (1-D) or splot with contours (2-D) per column; ``dim`` 0 is a probe's
+    time history (<stem>.out, against t).  The reference writes none in 3-D."""
+    if dim is None:
+        dim = sum(1 for c in case["cells"] if c > 1)
     if dim == 3:
         raise ValueError("no gnuplot script in 3-D (OutputGNU.cpp:110)")
-    out = ["reset", "set style data lines", "set nokey", "", "set xlabel 'x (m)'"]
+    out = ["reset", "set style data lines", "set nokey", ""]
+    out.append("set xlabel 't (s)'" if dim == 0 else "set xlabel 'x (m)'")
     if dim == 2:
         out += ["set surface", "set dgrid3d 50,50", "set contour base", "set cntrparam levels 25", "show contour"]
-    index = 1 + dim
-    for title, _ in _gnu_columns(case, np.ones((3 * len(case["eos"]) + 4, int(np.prod(n))))):
+    index = 2 if dim == 0 else 1 + dim
+    for title, _ in _gnu_columns(case, np.ones((3 * len(case["eos"]) + 4, int(np.prod(case["cells"]))))):
         out.append(f"set title '{title}'")
-        use = f"u 1:{index}" if dim <= 1 else f"u 1:2:{index}"
-        files = [f'"result_CPU0_TIME{t}.out" {use}' for t in range(nfiles)]
-        out.append(("plot " if dim <= 1 else "splot ") + ",\\\n ".join(files))
+        if dim == 0:
+            out.append(f'plot "{stem}.out" u 1:{index}')
+        else:
+            use = f"u 1:{index}" if dim <= 1 else f"u 1:2:{index}"
+            files = [f'"{stem}_CPU0_TIME{t}.out" {use}' for t in range(nfiles)]
+            out.append(("plot " if dim <= 1 else "splot ") + ",\\\n ".join(files))
         out += ["pause(-1)", ""]
         index += 1
     Path(path).write_text("\n".join(out) + "\n")
+
+
+def _centres(case):
+    n = case["cells"]
+    pos, _ = _positions(case)
+    X = np.broadcast_to(pos[0][None, None, :], (n[2], n[1], n[0])).reshape(-1)
+    Y = np.broadcast_to(pos[1][None, :, None], (n[2], n[1], n[0])).reshape(-1)
+    Z = np.broadcast_to(pos[2][:, None, None], (n[2], n[1], n[0])).reshape(-1)
+    return X, Y, Z
+
+
+def _lcfl(case):
+    """ElementCartesian's lCFL on a uniform grid (MeshCartesian.cpp:256-262):
+    the smallest cell width over the non-degenerate directions, times 0.6 in
+    2-D and 3-D."""
+    n, L = case["cells"], case["lengths"]
+    l = 1.e10
+    for a in range(3):
+        if n[a] != 1:
+            l = min(l, L[a] / n[a])
+    return l * 0.6 if sum(1 for c in n if c > 1) > 1 else l
+
+
+def _rows(coords, cols, idx):
+    data = list(coords) + [np.asarray(v)[idx] for _, v in cols]
+    rows = np.stack(data, axis=1) if data else np.zeros((len(idx), 0))
+    return "".join("".join(f"{v:.6g} " for v in r) + "\n" for r in rows)
+
+
+def probe_cell(case, vertex):
+    """OutputProbeGNU::locateProbeInMesh (OutputProbeGNU.cpp:115-135): the
+    first cell, in cell order, of least distance |centre - vertex|, the search
+    stopping at the first within lCFL."""
+    X, Y, Z = _centres(case)
+    dx, dy, dz = X - vertex[0], Y - vertex[1], Z - vertex[2]
+    dist = np.sqrt(dx * dx + dy * dy + dz * dz)
+    lim = _lcfl(case)
+    best, bi = 1.e12, 0
+    for i, di in enumerate(dist):
+        if di < best:
+            best, bi = di, i
+            if di <= lim:
+                break
+    return bi
+
+
+def probe_row(case, state, t, cell):
+    """One line of a probe file (OutputProbeGNU::ecritSolution,
+    OutputProbeGNU.cpp:154-170): the physical time, then the cell's columns."""
+    return f"{t:.6g} " + _rows([], _gnu_columns(case, state), np.array([cell]))

@@ -97,6 +97,11 @@ def write_deck(case, root: Path):
         tfq = case.get("time_freq", tf)
         tc = (f'  <timeControlMode iterations="false">\n    <iterations number="1" iterFreq="1"/>\n'
               f'    <physicalTime totalTime="{tf!r}" timeFreq="{tfq!r}"/>\n  </timeControlMode>')
+    for pr in case.get("probes", []):
+        # Input.cpp:136-142, OutputProbeGNU.cpp:44-90: <probe name> with vertex and acqFreq
+        v = pr["vertex"]
+        tc += (f'\n  <probe name="{pr["name"]}">\n    <vertex x="{v[0]!r}" y="{v[1]!r}" z="{v[2]!r}"/>\n'
+               f'    <timeControl acqFreq="{pr["acq_freq"]!r}"/>\n  </probe>')
     if case.get("resume"):
         # Input.cpp:253-257: restart from result file number fileNumber
         tc += f'\n  <resumeSimulation fileNumber="{case["resume"]}"/>'

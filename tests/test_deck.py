@@ -62,6 +62,12 @@ def test_deck_refusals(tmp_path):
     write_deck(dict(case, name="x", output='format="GNU" binary="true"'), tmp_path / "e")
     with pytest.raises(deck.DeckError, match="binary"):
         deck.load(tmp_path / "e")
+    write_deck(dict(case, name="x"), tmp_path / "f")
+    mx = tmp_path / "f" / "case" / "mainV5.xml"
+    mx.write_text(mx.read_text().replace("</computationParam>", '<cut1D><vertex x="0" y="0" z="0"/>'
+                                         '<vecDir x="1" y="0" z="0"/></cut1D></computationParam>'))
+    with pytest.raises(deck.DeckError, match="cut1D"):
+        deck.load(tmp_path / "f")
     (tmp_path / "a" / "case" / "meshV5.xml").write_text(
         '<?xml version="1.0"?><mesh><type structure="unstructured"/></mesh>')
     with pytest.raises(deck.DeckError, match="cartesian"):
@@ -167,3 +173,27 @@ def test_deck_gnu_results_match_reference_files(name, tmp_path):
     a, b = np.array(ours, dtype=float), np.array(ref, dtype=float)
     assert np.all(np.abs(a - b) <= 5e-6 * np.abs(b) + 1e-300)
     assert (res / "infoCalcul.out").exists() and not (res / "collection.pvd").exists()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["kapila_2d_fo_probes", "euler_1d_fo_probes"])
+def test_deck_probe_files_match_reference(name, tmp_path):
+    """python -m ecogen_amd.deck --vtk on a deck with probes: each probe file
+    has the reference's rows (times and values equal to their 6 printed
+    digits; device and reference differ in last bits) and its script."""
+    import json
+    from pathlib import Path
+    from ecogen_amd import deck
+    d = Path(__file__).resolve().parent / "golden" / "probes" / name
+    case = json.loads((d / "case.json").read_text())
+    print(f"case {name} deck --vtk with probes", flush=True)
+    write_deck(dict(case, name=name), tmp_path / "run")
+    deck.main([str(tmp_path / "run"), "--out", str(tmp_path / "f.npz"), "--vtk"])
+    res = tmp_path / "run" / "results" / name / "probes"
+    for pr in case["probes"]:
+        assert (res / f"visualisation{pr['name']}.gnu").read_text() == (d / f"visualisation{pr['name']}.gnu").read_text()
+        ours = [ln.split() for ln in (res / f"{pr['name']}.out").read_text().split("\n") if ln.strip()]
+        ref = [ln.split() for ln in (d / f"{pr['name']}.out").read_text().split("\n") if ln.strip()]
+        assert [len(r) for r in ours] == [len(r) for r in ref]
+        a, b = np.array(ours, dtype=float), np.array(ref, dtype=float)
+        assert np.all(np.abs(a - b) <= 5e-6 * np.abs(b) + 1e-300)

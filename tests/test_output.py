@@ -103,3 +103,34 @@ def test_gnu_writer_reproduces_reference_files(name, tmp_path):
         assert ours == ref
     output.write_gnu_script(tmp_path / "v.gnu", rcase, 2)
     assert (tmp_path / "v.gnu").read_text() == (d / "visualisation.gnu").read_text()
+
+
+PROBES = Path(__file__).resolve().parent / "golden" / "probes"
+
+
+@pytest.mark.parametrize("name", sorted(p.name for p in PROBES.iterdir() if p.is_dir()))
+def test_probe_files_reproduce_reference(name, tmp_path):
+    """OutputProbeGNU: the probe's cell (the first of least distance, one probe
+    sits on a face), a row at t = 0 and after every step reaching the next
+    acquisition time, and the script, all character for character.  The
+    state after step k is the oracle's k-step run, which equals the
+    reference's bit for bit."""
+    import oracle as O
+    from deck import write_deck
+    from ecogen_amd import deck, output
+    d = PROBES / name
+    case = json.loads((d / "case.json").read_text())
+    write_deck(dict(case, name=name), tmp_path / "run")
+    rcase, st0 = deck.load(tmp_path / "run")
+    assert [p["name"] for p in rcase["probes"]] == [p["name"] for p in case["probes"]]
+    states = [(st0, 0.0)] + [O.run(rcase, st0, nsteps=k)[0:3:2] for k in range(1, case["steps"] + 1)]
+    for pr in rcase["probes"]:
+        cell = output.probe_cell(rcase, pr["vertex"])
+        text, nxt = "", 0.0
+        for k, (s, t) in enumerate(states):
+            if k == 0 or nxt <= t:
+                text += output.probe_row(rcase, s, t, cell)
+                nxt += pr["acq_freq"]
+        assert text == (d / f"{pr['name']}.out").read_text()
+        output.write_gnu_script(tmp_path / "v.gnu", rcase, 1, dim=0, stem=pr["name"])
+        assert (tmp_path / "v.gnu").read_text() == (d / f"visualisation{pr['name']}.gnu").read_text()
