@@ -216,7 +216,7 @@ def main(argv=None):
     r.add_argument("--n3", type=int, default=256)
     r.add_argument("--steps", type=int, default=10)
     r.add_argument("--pool-cases", type=int, default=512)
-    r.add_argument("--out", default="gpurun_out/autotune.json")
+    r.add_argument("--out", default="autotune.json")
     a = sub.add_parser("apply")
     a.add_argument("results")
     args = ap.parse_args(argv)

@@ -1,7 +1,7 @@
 // ecogen.hpp -- C++ class API of the MI355X explicit finite-volume step.
 //
 // Keeps the reference's class names for this path -- Eos, Limiter,
-// Relaxation, Flux, Model, and a Run driver -- as host-side descriptors that
+// Relaxation, Flux, Model, and Run with its time loop -- as host-side descriptors that
 // are configured like the reference's objects (virtual at configuration time)
 // and lower to the POD eco_config of the C ABI (ecogen.h).  The numerical work
 // runs in the gfx950 kernels behind that ABI; nothing here computes on the

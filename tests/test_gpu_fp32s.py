@@ -1,4 +1,4 @@
-"""FP32-storage path: accuracy against the FP64 oracle (north_star bar 1e-5 relative L1).
+"""FP32-storage path: accuracy against the FP64 oracle (the bar: 1e-5 relative L1).
 
 The FP32-storage form keeps alpha_k, rho_k, p_k, u and the reduced energy in
 FP32 and computes the reconstruction and wave speeds in FP32; fluxes,
