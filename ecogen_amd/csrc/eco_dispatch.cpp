@@ -18,6 +18,8 @@ int launch_copy(void *dst, const void *src, unsigned long long nbytes, hipStream
 int launch_euler(const StageArgs &A, int stage, hipStream_t st);
 int launch_kapila(const StageArgs &A, int stage, hipStream_t st);
 int launch_multip(const StageArgs &A, int stage, hipStream_t st);
+int launch_kapila3(const StageArgs &A, int stage, hipStream_t st);
+int launch_multip3(const StageArgs &A, int stage, hipStream_t st);
 int launch_kapila32(const StageArgs &A, int stage, hipStream_t st, double *tmp);
 int launch_multip32(const StageArgs &A, int stage, hipStream_t st, double *tmp);
 
@@ -28,6 +30,11 @@ int launch_stage(const StageArgs &A, int stage, hipStream_t st) {
     return -1;
   }
   if (A.sch.model == MODEL_EULER) return launch_euler(A, stage, st);
+  if (A.sch.nph == 3) {
+    if (A.sch.model == MODEL_KAPILA) return launch_kapila3(A, stage, st);
+    if (A.sch.model == MODEL_MULTIP) return launch_multip3(A, stage, st);
+    return -1;
+  }
   if (A.sch.nph != 2) return -1;
   if (A.sch.model == MODEL_KAPILA) return launch_kapila(A, stage, st);
   if (A.sch.model == MODEL_MULTIP) return launch_multip(A, stage, st);

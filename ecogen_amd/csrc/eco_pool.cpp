@@ -303,6 +303,7 @@ extern "C" int eco_pool_admit(eco_pool *p, const eco_config *cfg, const double *
   int rc = validate_config(cfg);
   if (rc) return rc;
   if (cfg->precision == ECO_FP32S && cfg->order != 2) return ECO_ERR_UNSUPPORTED;
+  if (cfg->model != ECO_MODEL_EULER && cfg->nphase != 2) return ECO_ERR_UNSUPPORTED;  // pool kernels: two phases
   if (p->free_ids.empty()) return ECO_ERR_FULL;
   Lowered L;
   lower_config(*cfg, L);

@@ -27,6 +27,26 @@ def load_golden(name):
     return case, d
 
 
+def initial_state(case, d):
+    """The t = 0 state of a golden: the reference's printed fields, except a
+    Kapila deck's phase pressures, which come from the deck itself (the
+    reference prints only sum alpha_k p_k, which does not give p_k back
+    exactly when the alpha_k do not sum to 1 in floating point, as with three
+    phases; the deck reader reproduces every printed field, tests/test_deck.py)."""
+    import tempfile
+    import oracle as O
+    st = O.state_from_golden(case, d)
+    if case["model"] == "kapila":
+        from deck import write_deck
+        from ecogen_amd import deck
+        with tempfile.TemporaryDirectory() as tmp:
+            write_deck(dict(case, name="t0"), Path(tmp) / "run")
+            _, s0 = deck.load(Path(tmp) / "run")
+        for k in range(len(case["eos"])):
+            st[3 * k + 2] = s0[3 * k + 2]
+    return st
+
+
 def rel_l1(a, b):
     a = np.asarray(a, dtype=np.float64)
     b = np.asarray(b, dtype=np.float64)

@@ -9,6 +9,7 @@ from __future__ import annotations
 
 AIR = {"type": "IG", "name": "air", "gamma": 1.4, "pinf": 0.0, "cv": 800.0, "eref": 0.0, "sref": 0.0}
 WATER = {"type": "SG", "name": "water", "gamma": 4.4, "pinf": 6.0e8, "cv": 1000.0, "eref": 0.0, "sref": 0.0}
+HELIUM = {"type": "IG", "name": "helium", "gamma": 1.67, "pinf": 0.0, "cv": 3116.0, "eref": 0.0, "sref": 0.0}
 
 
 def _splitmix64(state):
@@ -120,6 +121,47 @@ def _tank(nph, p0, T0, x_side, water=False, mass=False):
     out = [None] * 6
     out[x_side] = tk
     return out
+
+
+def _three_phase_1d(model, order, limiters, steps, cells=150, relax=False):
+    """A three-phase tube (air, water, helium): water at 1e9 Pa on the left,
+    air in the middle, helium on the right, each phase at 1e-6 elsewhere."""
+    f = 1e-6
+    states = [{"name": "W", "alpha": [f, 1.0 - 2 * f, f], "rho": [50.0, 1000.0, 10.0], "p": [1e9] * 3, "u": [0.0]},
+              {"name": "A", "alpha": [1.0 - 2 * f, f, f], "rho": [50.0, 1000.0, 10.0], "p": [1e5] * 3, "u": [0.0]},
+              {"name": "H", "alpha": [f, f, 1.0 - 2 * f], "rho": [50.0, 1000.0, 10.0], "p": [1e5] * 3, "u": [0.0]}]
+    return {"model": model, "alpha_null": False, "eos": [AIR, WATER, HELIUM], "cells": [cells, 1, 1],
+            "lengths": [1.0, 0.1, 0.1], "order": order, "limiters": limiters, "cfl": 0.8,
+            "time_mode": "iterations", "steps": steps, "relax_p": relax,
+            "domains": [{"type": "entireDomain", "state": "A"},
+                        {"type": "halfSpace", "state": "W", "axe": "x", "origin": 0.3, "direction": "negative"},
+                        {"type": "halfSpace", "state": "H", "axe": "x", "origin": 0.7, "direction": "positive"}],
+            "states": states}
+
+
+def _three_phase_box(n, steps, dim, model, order, limiters, relax=False):
+    """A 2-D or 3-D box of air (three phases: air, water, helium) with a
+    shocked-air slab on the left, a water disc (sphere) and a helium disc
+    (sphere)."""
+    f = 1e-6
+    p2, rho2, u2 = mach_shock_air(2.0)
+    ax = [0.0] * dim
+    states = [{"name": "air", "alpha": [1.0 - 2 * f, f, f], "rho": [1.2, 1000.0, 0.17], "p": [1e5] * 3, "u": ax},
+              {"name": "shocked", "alpha": [1.0 - 2 * f, f, f], "rho": [rho2, 1000.0, 0.17], "p": [p2] * 3,
+               "u": [u2] + ax[1:]},
+              {"name": "water", "alpha": [f, 1.0 - 2 * f, f], "rho": [1.2, 1000.0, 0.17], "p": [1e5] * 3, "u": ax},
+              {"name": "helium", "alpha": [f, f, 1.0 - 2 * f], "rho": [1.2, 1000.0, 0.17], "p": [1e5] * 3, "u": ax}]
+    kind = "disc" if dim == 2 else "sphere"
+    c1 = [0.45, 0.35] + ([0.5] if dim == 3 else [])
+    c2 = [0.6, 0.7] + ([0.45] if dim == 3 else [])
+    doms = [{"type": "entireDomain", "state": "air"},
+            {"type": "halfSpace", "state": "shocked", "axe": "x", "origin": 0.15, "direction": "negative"},
+            {"type": kind, "state": "water", "radius": 0.18, "center": c1},
+            {"type": kind, "state": "helium", "radius": 0.15, "center": c2}]
+    return {"model": model, "alpha_null": False, "eos": [AIR, WATER, HELIUM],
+            "cells": [n, n, n if dim == 3 else 1], "lengths": [1.0, 1.0, 1.0], "order": order,
+            "limiters": limiters, "cfl": 0.8, "time_mode": "iterations", "steps": steps, "relax_p": relax,
+            "domains": doms, "states": states}
 
 
 def mach_shock_air(mach, p1=1e5, rho1=1.2, gamma=1.4):
@@ -238,6 +280,13 @@ CASES = {
                                   bc_inj=_inj(1, 150.0, 1, x_side=1)),
     "euler_1d_o2_injection": dict(_euler_1d(2, ["vanleer"] * 4, 300), bc=["injection", "outflow"] + ["abs"] * 4,
                                   bc_p0=[0.0, 5e4] + [0.0] * 4, bc_inj=_inj(1, 3000.0, 1, x_side=0, rho=80.0, p=1e7)),
+    # Three phases (air, water, helium)
+    "kapila_1d_fo_3ph": _three_phase_1d("kapila", 1, ["none"] * 4, 200),
+    "kapila_1d_o2_3ph": _three_phase_1d("kapila", 2, ["vanleer", "minmod", "vanleer", "superbee"], 200),
+    "multip_1d_o2_relax_3ph": _three_phase_1d("multip", 2, ["minmod"] * 4, 150, cells=100, relax=True),
+    "multip_1d_o2_norelax_3ph": _three_phase_1d("multip", 2, ["vanleer"] * 4, 150, cells=100),
+    "kapila_2d_o2_3ph": _three_phase_box(32, 30, 2, "kapila", 2, ["vanleer"] * 4),
+    "multip_3d_o2_relax_3ph": _three_phase_box(10, 8, 3, "multip", 2, ["minmod"] * 4, relax=True),
     # Tank boundaries (BoundCondTank, a reservoir at rest at p0, T0; no
     # second-order variant): air flows in from tanks above the tubes' air
     # pressure, and the tubes' high-pressure water flows out into tanks below

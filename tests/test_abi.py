@@ -125,3 +125,19 @@ def test_boundary_configs_are_validated_without_device(lib):
     assert create(dict(tcase, precision="fp32s")) == -2
     assert create(dict(tcase, bc_tank=[None, dict(tk, massFrac=[0.5, 0.6])] + [None] * 4)) == -1
     assert create(dict(tcase, bc_tank=[None, dict(tk, T0=0.0)] + [None] * 4)) == -1
+
+
+def test_phase_counts_are_validated_without_device(lib):
+    """Three phases run in FP64 single solves; FP32 storage and THINC (a
+    two-phase reconstruction) refuse them, four phases are not on the device."""
+    import ctypes as C
+    from ecogen_amd import capi
+    from cases import CASES, HELIUM
+
+    def create(c):
+        ctx = C.c_void_p()
+        return lib.eco_create(C.byref(capi.make_config(c)), None, C.byref(ctx))
+    c3 = CASES["kapila_1d_o2_3ph"]
+    assert create(dict(c3, precision="fp32s")) == -2
+    assert create(dict(c3, limiters=["vanleer", "vanleer", "thinc", "thinc"])) == -2
+    assert create(dict(c3, eos=c3["eos"] + [HELIUM])) == -2

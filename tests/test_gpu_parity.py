@@ -13,7 +13,7 @@ last-bit differences of the device arithmetic by up to ~1e6.
 import numpy as np
 import pytest
 
-from conftest import golden_names, load_golden, parity_quantities, phase_quantities, rel_l1
+from conftest import initial_state, golden_names, load_golden, parity_quantities, phase_quantities, rel_l1
 
 import oracle as O
 
@@ -38,7 +38,7 @@ def _run_gpu(case, state):
 def test_gpu_matches_oracle_and_reference(name):
     case, d = load_golden(name)
     print(f"case {name}: {case['model']} order={case['order']} cells={case['cells']}", flush=True)
-    st0 = O.state_from_golden(case, d)
+    st0 = initial_state(case, d)
     ref, it, t, _, _ = O.run(case, st0)
     out, status = _run_gpu(case, st0)
     assert status.steps == it

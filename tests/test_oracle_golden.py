@@ -7,7 +7,7 @@ state and must reproduce the final state, step count and final time.
 import numpy as np
 import pytest
 
-from conftest import compared_fields, golden_names, load_golden, rel_l1
+from conftest import initial_state, compared_fields, golden_names, load_golden, rel_l1
 
 import oracle as O
 
@@ -15,7 +15,7 @@ import oracle as O
 @pytest.mark.parametrize("name", golden_names())
 def test_oracle_matches_reference(name):
     case, d = load_golden(name)
-    st = O.state_from_golden(case, d)
+    st = initial_state(case, d)
     out, it, t, _, _ = O.run(case, st)
     info = d["info"]
     assert it == int(info[-1, 1])
