@@ -165,3 +165,29 @@ def test_fp32s_batch_accuracy_vs_fp64(cases32, alone32):
             err = np.abs(x - y).sum() / den if den > 0 else 0.0
             assert err <= 1e-5, (c["cells"], name, err)
     assert checked >= 4
+
+
+def test_batch_fp64_boundaries_match_alone():
+    """FP64 decks with walls, outflow, injection and tank boundaries (first
+    order and MUSCL, 1-D and 2-D), batched with a transmissive Riemann case:
+    every result bit-identical to the same case run alone."""
+    from conftest import initial_state, load_golden
+    from ecogen_amd.pool import Pool
+    names = ["c1_kapila_fo_wall", "kapila_1d_o2_outflow", "kapila_1d_o2_injection", "multip_1d_o2_relax_tank",
+             "euler_1d_o2_injection", "kapila_2d_fo_tank", "kapila_2d_o2_wall"]
+    cases = []
+    for nm in names:
+        case, d = load_golden(nm)
+        st = initial_state(case, d)
+        n = case["steps"] if case["time_mode"] == "iterations" else 0
+        cases.append((dict(case, precision="fp64"), st, n, case.get("t_final", 0.0)))
+    rc, rst, _ = W.riemann_case(2, 64, 100.0, 0.4, precision="fp64")
+    cases.append((rc, rst, 40, 0.0))
+    print("pool of", len(cases), "FP64 cases with boundaries", flush=True)
+    alone = [_alone(c, st, nsteps=n, t_final=tf) for c, st, n, tf in cases]
+    with Pool(hbm_budget=1 << 30, steps_per_chunk=12) as pool:
+        ids = [pool.admit(c, st, nsteps=n, t_final=tf) if n else pool.admit(c, st, t_final=tf) for c, st, n, tf in cases]
+        assert all(i is not None for i in ids)
+        pool.run_all()
+        res = [pool.result(i) for i in ids]
+    _same(res, alone)
