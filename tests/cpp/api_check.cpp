@@ -7,6 +7,7 @@
 //                              steps, t and the final state (host layout)
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -82,7 +83,10 @@ static int run_shock_tube(const char *out) {
     st[3 * n + i] = 1. - aa; st[4 * n + i] = 1000.; st[5 * n + i] = p;
   }
   run.setState(st);
-  RunStatus s = run.solverTime(240e-6);
+  // "resume": restart the same state at t = 1e-5 with dt = 3e-8 (Run::resumeClock)
+  const bool resume = std::getenv("API_CHECK_RESUME") != nullptr;
+  if (resume) run.resumeClock(1e-5, 3e-8);
+  RunStatus s = resume ? run.solverIterations(20) : run.solverTime(240e-6);
   std::vector<double> res = run.getState();
   FILE *f = std::fopen(out, "wb");
   if (!f) return 2;
