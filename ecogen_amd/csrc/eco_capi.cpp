@@ -203,7 +203,7 @@ int eco::validate_config(const eco_config *c) {
   } else {
     if (c->nphase < 2 || c->nphase > 4) return ECO_ERR_INVALID;
     if (c->nphase > kMaxPhases) return ECO_ERR_UNSUPPORTED;
-    // three phases: FP64 single solves; THINC is a two-phase reconstruction
+    // three or four phases: FP64 single solves; THINC is a two-phase reconstruction
     // (CellInterfaceO2.cpp:228-247 sets alpha_0 and 1 - alpha_0)
     if (c->nphase != 2 && (c->precision == ECO_FP32S || c->lim[2] == ECO_LIM_THINC || c->lim[3] == ECO_LIM_THINC))
       return ECO_ERR_UNSUPPORTED;

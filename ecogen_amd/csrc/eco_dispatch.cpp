@@ -20,6 +20,8 @@ int launch_kapila(const StageArgs &A, int stage, hipStream_t st);
 int launch_multip(const StageArgs &A, int stage, hipStream_t st);
 int launch_kapila3(const StageArgs &A, int stage, hipStream_t st);
 int launch_multip3(const StageArgs &A, int stage, hipStream_t st);
+int launch_kapila4(const StageArgs &A, int stage, hipStream_t st);
+int launch_multip4(const StageArgs &A, int stage, hipStream_t st);
 int launch_kapila32(const StageArgs &A, int stage, hipStream_t st, double *tmp);
 int launch_multip32(const StageArgs &A, int stage, hipStream_t st, double *tmp);
 
@@ -33,6 +35,11 @@ int launch_stage(const StageArgs &A, int stage, hipStream_t st) {
   if (A.sch.nph == 3) {
     if (A.sch.model == MODEL_KAPILA) return launch_kapila3(A, stage, st);
     if (A.sch.model == MODEL_MULTIP) return launch_multip3(A, stage, st);
+    return -1;
+  }
+  if (A.sch.nph == 4) {
+    if (A.sch.model == MODEL_KAPILA) return launch_kapila4(A, stage, st);
+    if (A.sch.model == MODEL_MULTIP) return launch_multip4(A, stage, st);
     return -1;
   }
   if (A.sch.nph != 2) return -1;

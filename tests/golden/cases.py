@@ -10,6 +10,7 @@ from __future__ import annotations
 AIR = {"type": "IG", "name": "air", "gamma": 1.4, "pinf": 0.0, "cv": 800.0, "eref": 0.0, "sref": 0.0}
 WATER = {"type": "SG", "name": "water", "gamma": 4.4, "pinf": 6.0e8, "cv": 1000.0, "eref": 0.0, "sref": 0.0}
 HELIUM = {"type": "IG", "name": "helium", "gamma": 1.67, "pinf": 0.0, "cv": 3116.0, "eref": 0.0, "sref": 0.0}
+OIL = {"type": "SG", "name": "oil", "gamma": 2.35, "pinf": 4.0e8, "cv": 1500.0, "eref": 0.0, "sref": 0.0}
 
 
 def _splitmix64(state):
@@ -164,6 +165,29 @@ def _three_phase_box(n, steps, dim, model, order, limiters, relax=False):
             "domains": doms, "states": states}
 
 
+def _four_phase_1d(model, order, limiters, steps, cells=150, relax=False, dim2=False):
+    """Four phases (air, water, helium, oil): the three-phase tube with an oil
+    slab in the air; with ``dim2`` the same layers on a square grid."""
+    c = _three_phase_1d(model, order, limiters, steps, cells=cells, relax=relax)
+    c["eos"] = [AIR, WATER, HELIUM, OIL]
+    f = 1e-6
+    for s in c["states"]:
+        s["alpha"] = [a * (1.0 - f) for a in s["alpha"]] + [f]
+        s["rho"] = s["rho"] + [900.0]
+        s["p"] = s["p"] + [s["p"][0]]
+    c["states"].append({"name": "O", "alpha": [f, f, f, 1.0 - 3 * f], "rho": [50.0, 1000.0, 10.0, 900.0],
+                        "p": [1e5] * 4, "u": [0.0]})
+    c["domains"].append({"type": "halfSpace", "state": "O", "axe": "x", "origin": 0.45, "direction": "positive"})
+    c["domains"].append({"type": "halfSpace", "state": "A", "axe": "x", "origin": 0.55, "direction": "positive"})
+    c["domains"].append({"type": "halfSpace", "state": "H", "axe": "x", "origin": 0.7, "direction": "positive"})
+    if dim2:
+        c["cells"] = [cells, cells, 1]
+        c["lengths"] = [1.0, 1.0, 1.0]
+        for s in c["states"]:
+            s["u"] = [0.0, 0.0]
+    return c
+
+
 def mach_shock_air(mach, p1=1e5, rho1=1.2, gamma=1.4):
     """Post-shock state of a normal shock moving into air at rest (Rankine-Hugoniot)."""
     import math
@@ -286,6 +310,8 @@ CASES = {
     "multip_1d_o2_relax_3ph": _three_phase_1d("multip", 2, ["minmod"] * 4, 150, cells=100, relax=True),
     "multip_1d_o2_norelax_3ph": _three_phase_1d("multip", 2, ["vanleer"] * 4, 150, cells=100),
     "kapila_2d_o2_3ph": _three_phase_box(32, 30, 2, "kapila", 2, ["vanleer"] * 4),
+    "kapila_1d_o2_4ph": _four_phase_1d("kapila", 2, ["vanleer"] * 4, 200),
+    "multip_2d_o2_relax_4ph": _four_phase_1d("multip", 2, ["minmod"] * 4, 30, cells=24, relax=True, dim2=True),
     "multip_3d_o2_relax_3ph": _three_phase_box(10, 8, 3, "multip", 2, ["minmod"] * 4, relax=True),
     # Tank boundaries (BoundCondTank, a reservoir at rest at p0, T0; no
     # second-order variant): air flows in from tanks above the tubes' air

@@ -128,11 +128,11 @@ def test_boundary_configs_are_validated_without_device(lib):
 
 
 def test_phase_counts_are_validated_without_device(lib):
-    """Three phases run in FP64 single solves; FP32 storage and THINC (a
-    two-phase reconstruction) refuse them, four phases are not on the device."""
+    """Three and four phases run in FP64 single solves; FP32 storage and
+    THINC (a two-phase reconstruction) refuse them; five phases are invalid."""
     import ctypes as C
     from ecogen_amd import capi
-    from cases import CASES, HELIUM
+    from cases import CASES
 
     def create(c):
         ctx = C.c_void_p()
@@ -140,4 +140,9 @@ def test_phase_counts_are_validated_without_device(lib):
     c3 = CASES["kapila_1d_o2_3ph"]
     assert create(dict(c3, precision="fp32s")) == -2
     assert create(dict(c3, limiters=["vanleer", "vanleer", "thinc", "thinc"])) == -2
-    assert create(dict(c3, eos=c3["eos"] + [HELIUM])) == -2
+    c4 = CASES["kapila_1d_o2_4ph"]
+    assert create(dict(c4, precision="fp32s")) == -2
+    cfg = capi.make_config(c4)
+    cfg.nphase = 5  # eco_config holds four EOS
+    ctx = C.c_void_p()
+    assert lib.eco_create(C.byref(cfg), None, C.byref(ctx)) == -1
