@@ -52,14 +52,22 @@ int batch_kapila(const BatchLaunch &L, hipStream_t st);
 int batch_multip(const BatchLaunch &L, hipStream_t st);
 int batch_kapila32(const BatchLaunch &L, hipStream_t st);
 int batch_multip32(const BatchLaunch &L, hipStream_t st);
+int batch_kapila3(const BatchLaunch &L, hipStream_t st);
+int batch_multip3(const BatchLaunch &L, hipStream_t st);
+int batch_kapila4(const BatchLaunch &L, hipStream_t st);
+int batch_multip4(const BatchLaunch &L, hipStream_t st);
 
-int launch_batch(int model, int prec, const BatchLaunch &L, hipStream_t st) {
+int launch_batch(int model, int prec, int nph, const BatchLaunch &L, hipStream_t st) {
   if (prec == PREC_FP32S) {
+    if (nph != 2) return -1;
     if (model == MODEL_KAPILA) return batch_kapila32(L, st);
     if (model == MODEL_MULTIP) return batch_multip32(L, st);
     return -1;
   }
   if (model == MODEL_EULER) return batch_euler(L, st);
+  if (nph == 3) return model == MODEL_KAPILA ? batch_kapila3(L, st) : model == MODEL_MULTIP ? batch_multip3(L, st) : -1;
+  if (nph == 4) return model == MODEL_KAPILA ? batch_kapila4(L, st) : model == MODEL_MULTIP ? batch_multip4(L, st) : -1;
+  if (nph != 2) return -1;
   if (model == MODEL_KAPILA) return batch_kapila(L, st);
   if (model == MODEL_MULTIP) return batch_multip(L, st);
   return -1;

@@ -1,7 +1,8 @@
 // Instantiation of the stage kernels for the multip model with three phases
-// (FP64 storage, single solves).
+// (FP64 storage, single solves and pools).
 #include "eco_stage.inl"
 
 namespace eco {
 int launch_multip3(const StageArgs &A, int stage, hipStream_t st) { return launch_d<MODEL_MULTIP, 3>(A, stage, st); }
+int batch_multip3(const BatchLaunch &L, hipStream_t st) { return launch_db<MODEL_MULTIP, 3>(L, st); }
 }  // namespace eco

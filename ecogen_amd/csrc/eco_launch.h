@@ -43,7 +43,7 @@ struct BatchLaunch {
   int slot, grid;
   int *occupancy;            // non-null: report workgroups per CU for this kernel, no launch
 };
-int launch_batch(int model, int prec, const BatchLaunch &L, hipStream_t stream);
+int launch_batch(int model, int prec, int nph, const BatchLaunch &L, hipStream_t stream);
 
 // Tile shape (cells per workgroup along x, y, z) of the stage kernels for a dimension.
 void tile_shape(int dim, int t[3]);

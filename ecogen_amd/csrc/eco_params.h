@@ -11,7 +11,7 @@ enum LimiterKind : int { LIM_NONE = 0, LIM_MINMOD = 1, LIM_VANLEER = 2, LIM_VANA
 enum EosKind : int { EOS_IG = 0, EOS_SG = 1 };
 enum Precision : int { PREC_FP64 = 0, PREC_FP32S = 1 };
 
-constexpr int kMaxPhases = 4;  // phases handled by the device path (FP32 storage, THINC, pools: 2)
+constexpr int kMaxPhases = 4;  // phases handled by the device path (FP32 storage and THINC: 2)
 
 // Equation of state constants (src/Eos/EosSG.h, src/Eos/EosIG.h) plus the
 // mixture-EOS coefficients of sendSpecialMixtureEos, computed once on the host

@@ -95,9 +95,9 @@ class Arena {
 };
 
 struct ClassKey {
-  int model, dim, order, prec, same;
+  int model, dim, order, prec, same, nph;
   bool operator==(const ClassKey &o) const {
-    return model == o.model && dim == o.dim && order == o.order && prec == o.prec && same == o.same;
+    return model == o.model && dim == o.dim && order == o.order && prec == o.prec && same == o.same && nph == o.nph;
   }
 };
 
@@ -274,7 +274,7 @@ static int class_for(eco_pool *p, const ClassKey &key) {
   // persistent grid: resident workgroups of the heavier (second) kernel of the step
   int occ = 0;
   BatchLaunch L{key.dim, key.order == 1 ? 0 : 2, key.same, nullptr, nullptr, nullptr, 0, 0, &occ};
-  if (launch_batch(key.model, key.prec, L, p->cs) != 0 || occ <= 0) occ = 1;
+  if (launch_batch(key.model, key.prec, key.nph, L, p->cs) != 0 || occ <= 0) occ = 1;
   c.grid = p->pc.grid_blocks > 0 ? p->pc.grid_blocks : occ * p->cus;
   p->classes.push_back(std::move(c));
   return (int)p->classes.size() - 1;
@@ -303,7 +303,6 @@ extern "C" int eco_pool_admit(eco_pool *p, const eco_config *cfg, const double *
   int rc = validate_config(cfg);
   if (rc) return rc;
   if (cfg->precision == ECO_FP32S && cfg->order != 2) return ECO_ERR_UNSUPPORTED;
-  if (cfg->model != ECO_MODEL_EULER && cfg->nphase != 2) return ECO_ERR_UNSUPPORTED;  // pool kernels: two phases
   if (p->free_ids.empty()) return ECO_ERR_FULL;
   Lowered L;
   lower_config(*cfg, L);
@@ -315,7 +314,8 @@ extern "C" int eco_pool_admit(eco_pool *p, const eco_config *cfg, const double *
     soff = p->arena.alloc(stg);
     if (soff == SIZE_MAX) { p->arena.release(off, need); return ECO_ERR_BUDGET; }
   }
-  ClassKey key{cfg->model, L.geo.dim, cfg->order, cfg->precision, L.sch.same_lim ? 1 : 0};
+  ClassKey key{cfg->model, L.geo.dim, cfg->order, cfg->precision, L.sch.same_lim ? 1 : 0,
+               cfg->model == ECO_MODEL_EULER ? 1 : cfg->nphase};
   int ci = class_for(p, key);
   if (ci < 0) {
     p->arena.release(off, need);
@@ -450,7 +450,7 @@ static int enqueue_steps(eco_pool *p, const std::vector<int> &cls, long long bas
         if (stage < 0) continue;
         BatchLaunch L{cl.key.dim, stage, cl.key.same, cl.d_args, cl.d_tiles, &cl.d_desc->ntiles, sl, cl.grid,
                       nullptr};
-        int rc = launch_batch(cl.key.model, cl.key.prec, L, p->cs);
+        int rc = launch_batch(cl.key.model, cl.key.prec, cl.key.nph, L, p->cs);
         if (rc) return rc < 0 ? ECO_ERR_UNSUPPORTED : ECO_ERR_HIP;
       }
     }

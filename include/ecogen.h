@@ -68,7 +68,7 @@ typedef struct {
    from the deck (src/InputOutput/Input.cpp:85-625) for this path. */
 typedef struct {
   int model;        /* enum eco_model */
-  int nphase;       /* 1 for Euler; 2, or 3-4 in FP64 single solves without THINC, for Kapila/MultiP */
+  int nphase;       /* 1 for Euler; 2, or 3-4 in FP64 without THINC, for Kapila/MultiP */
   int order;        /* 1 = FIRSTORDER, 2 = SECONDORDER (MUSCL predictor-corrector) */
   int lim[4];       /* global, interface, global volume-fraction, interface volume-fraction limiter */
   int alpha_null;   /* Kapila flowModel alphaNull attribute */

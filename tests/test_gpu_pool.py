@@ -191,3 +191,23 @@ def test_batch_fp64_boundaries_match_alone():
         pool.run_all()
         res = [pool.result(i) for i in ids]
     _same(res, alone)
+
+
+def test_batch_three_and_four_phases_match_alone():
+    """Three- and four-phase FP64 decks (their own kernel classes) batched
+    with a two-phase case: every result bit-identical to its single solve."""
+    from conftest import initial_state, load_golden
+    from ecogen_amd.pool import Pool
+    cases = []
+    for nm in ["kapila_1d_o2_3ph", "multip_3d_o2_relax_3ph", "kapila_1d_o2_4ph", "multip_2d_o2_relax_4ph",
+               "kapila_1d_o2_vanleer"]:
+        case, d = load_golden(nm)
+        cases.append((dict(case, precision="fp64"), initial_state(case, d), case["steps"]))
+    print("pool of 2-, 3- and 4-phase FP64 cases", flush=True)
+    alone = [_alone(c, st, nsteps=n) for c, st, n in cases]
+    with Pool(hbm_budget=1 << 30, steps_per_chunk=12) as pool:
+        ids = [pool.admit(c, st, nsteps=n) for c, st, n in cases]
+        assert all(i is not None for i in ids)
+        pool.run_all()
+        res = [pool.result(i) for i in ids]
+    _same(res, alone)
