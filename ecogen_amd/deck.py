@@ -519,7 +519,7 @@ def _run_with_output(ctx, case, outdir, resume=None):
     else:
         # the resumed run keeps infoCalcul.out up to row N (Output::readInfos)
         # and numbers its files from N + 1
-        n0, it0, kept = resume
+        n0, it0, kept, _ = resume
         files = [outdir / f"result_CPU0_AMR0_TIME{k}.vtr" for k in range(n0 + 1)]
         info.write("\n".join(kept) + "\n")
 
@@ -586,16 +586,30 @@ def _run_with_output(ctx, case, outdir, resume=None):
             done = it0 + st.steps
             dump(st)
     else:
-        tf = case["t_final"]
-        freq = o["time_freq"] if o["time_freq"] > 0 else tf
-        k = len(files) if resume is not None else 1
-        while True:
-            target = min(k * freq, tf)
-            st = advance(t_final=target)
-            dump(st)
-            if st.t >= float(np.float32(tf)) or target >= tf:
-                break
-            k += 1
+        # Run::solver (Run.cpp:208, 241-243): a result file at each step with
+        # t > printSuivante (double), which starts at the run's first time plus
+        # timeFreq and grows by timeFreq at each file; both times are floats
+        # (Input.cpp:164-166).  The run ends at the first t >= totalTime.
+        tfin = float(np.float32(case["t_final"]))
+        ff = float(np.float32(o["time_freq"]))
+        print_next = (resume[3] if resume is not None else 0.0) + ff
+        t = resume[3] if resume is not None else 0.0
+        while t < tfin:
+            # the device stops at the first t >= a float target: the largest
+            # float not above print_next, then single steps to t > print_next
+            low = np.float32(print_next)
+            if float(low) > print_next:
+                low = np.nextafter(low, np.float32(-np.inf))
+            target = min(float(low), tfin)
+            st = advance(t_final=target) if t < target else advance(nsteps=1)
+            while not (st.t > print_next) and st.t < tfin:
+                st = advance(nsteps=1)
+            t = st.t
+            if t > print_next:
+                dump(st)
+                print_next += ff
+        if st is None:
+            st = ctx.sync()
     if not gnu:
         output.write_collection(outdir / "collection.pvd", files)
     info.close()
@@ -616,7 +630,7 @@ def main(argv=None):
     it0, resume = 0, None
     if case.get("resume"):
         state, it0, t0, dt0, kept = resume_point(case, args.rundir)
-        resume = (case["resume"], it0, kept)
+        resume = (case["resume"], it0, kept, t0)
     with Context(case) as ctx:
         ctx.set_state(state)
         if resume is not None:
