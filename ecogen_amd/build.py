@@ -4,7 +4,8 @@ Compiles every HIP/C++ source of ``ecogen_amd/csrc`` with hipcc for gfx950
 (one process per translation unit, in parallel) and links the shared library
 that exports the C ABI of ``include/ecogen.h``.
 
-    python -m ecogen_amd.build
+    python -m ecogen_amd.build                  # every translation unit
+    python -m ecogen_amd.build --incremental    # only out-of-date ones
 """
 from __future__ import annotations
 
@@ -34,8 +35,16 @@ SOURCES = ["eco_k_kapila.hip", "eco_k_multip.hip", "eco_k_euler.hip", "eco_k_kap
            "eco_capi.cpp", "eco_pool.cpp", "eco_api.cpp"]
 
 
-def _compile(src: str) -> Path:
+def _deps_mtime(src: str) -> float:
+    # every translation unit includes headers of csrc/ and include/ only
+    hdrs = [*CSRC.glob("*.h"), *CSRC.glob("*.hpp"), *CSRC.glob("*.inl"), *(PKG.parent / "include").glob("*.h*")]
+    return max([(CSRC / src).stat().st_mtime, *(h.stat().st_mtime for h in hdrs)])
+
+
+def _compile(src: str, incremental: bool = False) -> Path:
     obj = OBJ / (Path(src).stem + ".o")
+    if incremental and obj.exists() and obj.stat().st_mtime > _deps_mtime(src):
+        return obj
     cmd = [HIPCC, *COMMON, "-c", str(CSRC / src), "-o", str(obj)]
     if src.endswith(".cpp"):
         cmd[1:1] = ["-x", "hip"]
@@ -59,10 +68,12 @@ def build_api_check() -> Path:
     return API_CHECK
 
 
-def build(verbose: bool = True) -> Path:
+def build(verbose: bool = True, incremental: bool = False) -> Path:
+    """Compiles every translation unit (incremental=True: only those older
+    than their source or any header) and links the library."""
     OBJ.mkdir(exist_ok=True)
     with ThreadPoolExecutor(max_workers=min(len(SOURCES), os.cpu_count() or 4)) as ex:
-        objs = list(ex.map(_compile, SOURCES))
+        objs = list(ex.map(lambda src: _compile(src, incremental), SOURCES))
     tmp = OUT.with_suffix(".so.tmp")
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *map(str, objs), "-o", str(tmp)]
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -77,5 +88,5 @@ def build(verbose: bool = True) -> Path:
 
 
 if __name__ == "__main__":
-    build()
+    build(incremental="--incremental" in sys.argv)
     sys.exit(0)

@@ -26,7 +26,8 @@ struct eco_ctx {
   hipStream_t stream;
   int own_stream;
   int nprim;            // device primitive fields
-  double *buf[2];       // time levels, nprim+1 fields each
+  int nlevel;           // fields per time level (nprim, E, cached mixture density)
+  double *buf[2];       // time levels, nlevel fields each
   double *bstar;        // U*, nprim fields
   double *tmp64;        // FP64 staging of set/get_state in the FP32-storage form
   size_t es;            // bytes per stored value (8 or 4)
@@ -294,6 +295,8 @@ void eco::lower_config(const eco_config &c, Lowered &L) {
   for (int b = 0; b < 6; b++) if (c.bc[b] == ECO_BC_INJECTION) lower_injection(c, b, S);
   for (int b = 0; b < 6; b++) if (c.bc[b] == ECO_BC_TANK) lower_tank(c, b, S);
   L.nprim = (c.model == ECO_MODEL_EULER) ? 2 + dim : 3 * c.nphase + dim;
+  S.rhoc = (c.model == ECO_MODEL_MULTIP && c.relax_p && c.precision == ECO_FP64) ? 1 : 0;
+  L.nlevel = L.nprim + 1 + S.rhoc;
   L.es = (c.precision == ECO_FP32S) ? sizeof(float) : sizeof(double);
 }
 
@@ -332,6 +335,7 @@ extern "C" int eco_create(const eco_config *cfg, void *hip_stream, eco_ctx **out
   x->geo = L.geo;
   x->sch = L.sch;
   x->nprim = L.nprim;
+  x->nlevel = L.nlevel;
 
   // stream
   if (hip_stream) { x->stream = (hipStream_t)hip_stream; x->own_stream = 0; }
@@ -344,8 +348,8 @@ extern "C" int eco_create(const eco_config *cfg, void *hip_stream, eco_ctx **out
   x->buf[0] = x->buf[1] = x->bstar = x->tables = x->tmp64 = nullptr;
   x->ctl = nullptr;
   x->es = L.es;
-  e = hipMalloc(&x->buf[0], nc * (x->nprim + 1) * x->es);
-  if (e == hipSuccess) e = hipMalloc(&x->buf[1], nc * (x->nprim + 1) * x->es);
+  e = hipMalloc(&x->buf[0], nc * x->nlevel * x->es);
+  if (e == hipSuccess) e = hipMalloc(&x->buf[1], nc * x->nlevel * x->es);
   if (e == hipSuccess && c.order == 2) e = hipMalloc(&x->bstar, nc * x->nprim * x->es);
   if (e == hipSuccess && c.precision == ECO_FP32S) e = hipMalloc(&x->tmp64, nc * (x->nprim + 1) * sizeof(double));
   if (e == hipSuccess) e = hipMalloc(&x->tables, ntab * sizeof(double));
@@ -635,9 +639,13 @@ extern "C" int eco_time_steps(eco_ctx *x, int nsteps, double *stage_ms, int *lau
   int rc = set_target(x, x->step + nsteps, -1.);
   if (rc) return rc;
   const int nst = (x->cfg.order == 1) ? 1 : 2;
-  std::vector<hipEvent_t> ev((size_t)nsteps * nst * 2);
-  for (auto &e : ev)
-    if (hipEventCreate(&e) != hipSuccess) return ECO_ERR_HIP;
+  std::vector<hipEvent_t> ev((size_t)nsteps * nst * 2, nullptr);
+  for (size_t i = 0; i < ev.size(); i++) {
+    if (hipEventCreate(&ev[i]) != hipSuccess) {
+      for (size_t j = 0; j < i; j++) (void)hipEventDestroy(ev[j]);
+      return ECO_ERR_HIP;
+    }
+  }
   for (int i = 0; i < nsteps && !rc; i++) {
     int k = x->step;
     if (nst == 1) {

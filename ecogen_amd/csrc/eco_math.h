@@ -3,6 +3,9 @@
 // source as the device.
 #pragma once
 #include <cmath>
+#include <cstring>
+
+#include "eco_pow_tables.h"
 
 #if defined(__HIPCC__)
 #define ECO_HD __host__ __device__ __forceinline__
@@ -24,42 +27,113 @@ ECO_HD double qdiv(double a, double b, double y) {
   return std::fma(r, y, q);
 }
 
-// x^g for x > 0 (the isentrope's density ratio).  ln x = e ln2 + ln m with
-// m in [1/sqrt2, sqrt2): ln m = 2 atanh(s), s = (m-1)/(m+1), |s| < 0.172,
-// by its series to s^23; ln 2 split in a 32-bit head and a tail so that
-// e*ln2_hi is exact; the product g*ln x is carried as a double-double and
-// applied as exp(hi)*(1 + lo).  Error against a correctly rounded power:
-// <= 2 ulp (g = 1.4), <= 5 ulp (g = 4.4) for x in [1e-3, 1e3].  Zero,
-// negative, infinite and NaN bases take the library pow (same special values).
+// x^g for x > 0 (the isentrope's density ratio), computed as the C library
+// computes it, so that the device's power is the reference's bit for bit.
+// The reference calls pow from the GNU C library (>= 2.28; 2.35 here), whose
+// pow is the table-driven algorithm Szabolcs Nagy published in ARM's
+// optimized-routines (math/pow.c), built on x86-64 machines with FMA as the
+// variant whose expressions are FMA-contracted.  Restated here from that
+// published method:
+//   x = 2^k z, z in [0x1.69555p-1, 0x1.69555p0), c from a 128-entry table,
+//   log x = k ln2 + log c + log1p(z/c - 1) as a double-double (hi, lo), with
+//   r = z/c - 1 exact in one fma and log1p(r) - r + r^2/2 by a degree-8
+//   polynomial;
+//   y log x = ehi + elo (exact product in one fma);
+//   exp(ehi + elo) = 2^(j/128) exp(r'), j = round(ehi 128/ln2),
+//   r' = ehi - j ln2/128 + elo, exp(r') - 1 by a degree-5 polynomial,
+//   2^(j/128) = H (1 + T) from a 128-entry table.
+// The tables are generated from their formulas by gen_pow_tables.py.  Every
+// fma below is one of the contractions of the library's build; the kernels
+// are compiled without contraction so nothing else fuses.  Agreement with
+// the library, bit for bit, is tested on the host (tests/math_check.cpp).
+// Outside the isentrope's range (x zero, subnormal, negative, infinite or
+// NaN; |g log x| >= 512) the library pow of the build target is called.
+namespace powtab {
+#if defined(__HIP_DEVICE_COMPILE__)
+#define ECO_POW_TABLE __constant__ static const
+#else
+#define ECO_POW_TABLE static const
+#endif
+ECO_POW_TABLE double LOG[128][3] = {ECO_POW_LOG_ROWS};
+ECO_POW_TABLE unsigned long long EXP[256] = {ECO_POW_EXP_WORDS};
+#undef ECO_POW_TABLE
+}  // namespace powtab
+
+ECO_HD double as_double(unsigned long long u) {
+  double d;
+  std::memcpy(&d, &u, 8);
+  return d;
+}
+ECO_HD unsigned long long as_bits(double d) {
+  unsigned long long u;
+  std::memcpy(&u, &d, 8);
+  return u;
+}
+
+// exp(hi + lo) for |lo| << |hi|: the C library's exp (glibc >= 2.28, the
+// same optimized-routines algorithm and FMA build as its pow) evaluated at
+// hi with the tail lo added to the reduced argument.  exp(x) = exp_dd(x, 0).
+// Outside [2^-54, 512) in magnitude, tiny arguments give 1 + hi as the
+// library does and large ones fall back to the build target's pow(px, pg)
+// (or exp(hi) when px is 0), which the isentrope and THINC never reach.
+ECO_HD double exp_dd(double hi, double lo, double px = 0., double pg = 0.) {
+  const unsigned top = (unsigned)(as_bits(hi) >> 52) & 0x7ffu;
+  if (top - 0x3c9u >= 0x408u - 0x3c9u) {  // |hi| < 2^-54 or >= 512
+    if (top < 0x3c9u) return 1.0 + hi;
+    return px != 0. ? std::pow(px, pg) : std::exp(hi);
+  }
+  constexpr double INVLN2N = 0x1.71547652b82fep0 * 128, SHIFT = 0x1.8p52;
+  constexpr double NEGLN2HIN = -0x1.62e42fefa0000p-8, NEGLN2LON = -0x1.cf79abc9e3b3ap-47;
+  constexpr double C2 = 0x1.ffffffffffdbdp-2, C3 = 0x1.555555555543cp-3, C4 = 0x1.55555cf172b91p-5,
+                   C5 = 0x1.1111167a4d017p-7;
+  double jd = std::fma(INVLN2N, hi, SHIFT);
+  const unsigned long long ki = as_bits(jd);
+  jd -= SHIFT;
+  double rr = std::fma(jd, NEGLN2LON, std::fma(jd, NEGLN2HIN, hi));
+  rr += lo;
+  const unsigned idx = 2u * (unsigned)(ki & 127u);
+  const double tail = as_double(powtab::EXP[idx]);
+  const double scale = as_double(powtab::EXP[idx + 1] + (ki << 45));
+  const double r2 = rr * rr;
+  const double tp = std::fma(r2 * r2, std::fma(rr, C5, C4), std::fma(r2, std::fma(rr, C3, C2), tail + rr));
+  return std::fma(scale, tp, scale);
+}
+
+ECO_HD double exp_lib(double x) { return exp_dd(x, 0.); }
+
 ECO_HD double pow_pos(double x, double g) {
-  if (!(x > 0. && x <= 1.7976931348623157e308)) return std::pow(x, g);
-  constexpr double LN2_HI = 6.93147180369123816490e-01;  // ln 2 to 32 significant bits
-  constexpr double LN2_LO = 1.90821492927058770002e-10;  // ln 2 - LN2_HI
-  int e;
-  double m = std::frexp(x, &e);  // m in [0.5, 1)
-  if (m < 0.70710678118654752440) { m *= 2.; e -= 1; }
-  const double s = (m - 1.) / (m + 1.), s2 = s * s;
-  double p = 1. / 23.;
-  p = std::fma(p, s2, 1. / 21.);
-  p = std::fma(p, s2, 1. / 19.);
-  p = std::fma(p, s2, 1. / 17.);
-  p = std::fma(p, s2, 1. / 15.);
-  p = std::fma(p, s2, 1. / 13.);
-  p = std::fma(p, s2, 1. / 11.);
-  p = std::fma(p, s2, 1. / 9.);
-  p = std::fma(p, s2, 1. / 7.);
-  p = std::fma(p, s2, 1. / 5.);
-  p = std::fma(p, s2, 1. / 3.);
-  const double lm = 2. * s + 2. * s * s2 * p;       // ln m
-  const double de = (double)e;
-  const double A = de * LN2_HI;                     // exact (|e| < 2^11)
-  const double B = lm + de * LN2_LO;
-  const double p1 = g * A, e1 = std::fma(g, A, -p1);  // p1 + e1 == g*A exactly
-  const double p2 = g * B;
-  const double th = p1 + p2;
-  const double tl = ((p1 - th) + p2) + e1;          // |p2| <= |p1| unless A == 0
-  const double E = std::exp(th);
-  return std::fma(E, tl, E);
+  const unsigned long long ix = as_bits(x);
+  // positive normal finite x only (top 12 bits 0x001..0x7fe)
+  if (!((ix >> 52) - 1u < 0x7feu)) return std::pow(x, g);
+  // log x as hi + lo
+  constexpr unsigned long long OFF = 0x3fe6955500000000ull;
+  const unsigned long long tmp = ix - OFF;
+  const int i = (int)((tmp >> 45) & 127u);
+  const int k = (int)((long long)tmp >> 52);
+  const double z = as_double(ix - (tmp & (0xfffull << 52)));
+  const double kd = (double)k;
+  const double invc = powtab::LOG[i][0], logc = powtab::LOG[i][1], logctail = powtab::LOG[i][2];
+  constexpr double LN2HI = 0x1.62e42fefa3800p-1, LN2LO = 0x1.ef35793c76730p-45;
+  constexpr double A0 = -0x1p-1, A1 = 0x1.555555555556p-2 * -2, A2 = -0x1.0000000000006p-2 * -2,
+                   A3 = 0x1.999999959554ep-3 * 4, A4 = -0x1.555555529a47ap-3 * 4,
+                   A5 = 0x1.2495b9b4845e9p-3 * -8, A6 = -0x1.0002b8b263fc3p-3 * -8;
+  const double r = std::fma(z, invc, -1.0);
+  const double t1 = std::fma(kd, LN2HI, logc);
+  const double t2 = t1 + r;
+  const double lo1 = std::fma(kd, LN2LO, logctail);
+  const double lo2 = t1 - t2 + r;
+  const double ar = A0 * r, ar2 = r * ar, ar3 = r * ar2;
+  const double hi = t2 + ar2;
+  const double lo3 = std::fma(ar, r, -ar2);
+  const double lo4 = t2 - hi + ar2;
+  const double q = std::fma(ar2, std::fma(ar2, std::fma(r, A6, A5), std::fma(r, A4, A3)), std::fma(r, A2, A1));
+  const double lo = std::fma(ar3, q, lo1 + lo2 + lo3 + lo4);
+  const double lhi = hi + lo;
+  const double llo = hi - lhi + lo;
+  // y log x = ehi + elo
+  const double ehi = g * lhi;
+  const double elo = std::fma(g, llo, std::fma(g, lhi, -ehi));
+  return exp_dd(ehi, elo, x, g);
 }
 
 }  // namespace eco

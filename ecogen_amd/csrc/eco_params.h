@@ -50,6 +50,13 @@ struct SchemeParams {
   int alpha_null;
   double eps;        // 1e-15 when alphaNull, else 0 (src/Eos/Eos.cpp:114-122)
   int relax;         // stiff pressure relaxation applied
+  // 1: each time level stores, after E, the mixture density of buildPrim
+  // before the relaxation (MultiP with relaxation, FP64).  The reference
+  // keeps that value cached in the cell (FluxMultiP::buildPrim sets it; the
+  // relaxation changes alpha_k, rho_k but not the mixture density) and the
+  // next step's buildCons multiplies velocity and energy by it
+  // (FluxMultiP.cpp:122-140), so it cannot be recomputed from alpha_k rho_k.
+  int rhoc;
   double cfl;
   int prec;          // PREC_FP64 or PREC_FP32S (storage)
   int bc_wall;       // bit 2*dir+side set: wall boundary (BoundCondWall), else transmissive

@@ -186,7 +186,7 @@ static int hipc(hipError_t e) { return e == hipSuccess ? ECO_OK : ECO_ERR_HIP; }
 
 static size_t case_bytes(const eco_config &c, const Lowered &L) {
   const size_t nc = (size_t)L.geo.ncell;
-  size_t b = 2 * align_up(nc * (L.nprim + 1) * L.es);
+  size_t b = 2 * align_up(nc * L.nlevel * L.es);
   if (c.order == 2) b += align_up(nc * L.nprim * L.es);
   b += align_up(L.tab.size() * sizeof(double));
   return b;
@@ -336,8 +336,8 @@ extern "C" int eco_pool_admit(eco_pool *p, const eco_config *cfg, const double *
   cs.bytes = need;
   const size_t nc = (size_t)L.geo.ncell;
   char *b = p->arena.ptr(off);
-  cs.buf[0] = (double *)b; b += align_up(nc * (L.nprim + 1) * L.es);
-  cs.buf[1] = (double *)b; b += align_up(nc * (L.nprim + 1) * L.es);
+  cs.buf[0] = (double *)b; b += align_up(nc * L.nlevel * L.es);
+  cs.buf[1] = (double *)b; b += align_up(nc * L.nlevel * L.es);
   if (cfg->order == 2) { cs.bstar = (double *)b; b += align_up(nc * L.nprim * L.es); }
   cs.tables = (double *)b;
   bind_tables(*cfg, cs.low, cs.tables);
@@ -348,6 +348,18 @@ extern "C" int eco_pool_admit(eco_pool *p, const eco_config *cfg, const double *
   cs.g0 = p->slot;  // joins at the next chunk
   cs.argslot = cl.free_argslots.back();
   cl.free_argslots.pop_back();
+  // a failure from here on gives back what the admission took: the argument
+  // slot, the handle and the arena blocks (after the upload stream has
+  // drained, so no copy still writes into them), and resets the case
+  auto fail = [&](int code) {
+    (void)hipStreamSynchronize(p->us);
+    cl.free_argslots.push_back(cs.argslot);
+    p->arena.release(off, need);
+    if (stg) p->arena.release(soff, stg);
+    cs = Case();
+    p->free_ids.push_back(id);
+    return code;
+  };
 
   // --- upload and initialise on the upload stream ---
   hipError_t e = hipMemcpyAsync(cs.tables, cs.low.tab.data(), cs.low.tab.size() * sizeof(double),
@@ -358,7 +370,7 @@ extern "C" int eco_pool_admit(eco_pool *p, const eco_config *cfg, const double *
   for (size_t f = 0; f < map.size() && e == hipSuccess; f++)
     e = hipMemcpyAsync(dst + f * nc, host_state + (size_t)map[f] * nc, nc * sizeof(double), hipMemcpyHostToDevice,
                        p->us);
-  if (e != hipSuccess) return ECO_ERR_HIP;
+  if (e != hipSuccess) return fail(ECO_ERR_HIP);
   StageArgs A = case_args(cs, p->d_ctl + id, (int)(cs.g0 & 1), 0);
   A.wsrc = A.wn = dst;
   A.wout = dst;
@@ -367,7 +379,7 @@ extern "C" int eco_pool_admit(eco_pool *p, const eco_config *cfg, const double *
     A.wout = cs.buf[cs.g0 & 1];
     rc = launch_convert(A, 1, dst, p->us);
   }
-  if (rc) return rc < 0 ? ECO_ERR_UNSUPPORTED : ECO_ERR_HIP;
+  if (rc) return fail(rc < 0 ? ECO_ERR_UNSUPPORTED : ECO_ERR_HIP);
   // control block: zero, target
   Control h;
   std::memset(&h, 0, sizeof(h));
@@ -384,12 +396,12 @@ extern "C" int eco_pool_admit(eco_pool *p, const eco_config *cfg, const double *
     e = hipMemcpyAsync(cl.d_args + (size_t)cs.argslot * 4, cl.h_args + (size_t)cs.argslot * 4, 4 * sizeof(StageArgs),
                        hipMemcpyHostToDevice, p->us);
   if (e == hipSuccess) e = hipEventRecord(p->admit_ev, p->us);
-  if (e != hipSuccess) return ECO_ERR_HIP;
+  if (e != hipSuccess) return fail(ECO_ERR_HIP);
   // the upload stream is synchronised here only for the host arrays it reads
   // (pageable copies complete before hipMemcpyAsync returns for pageable
   // sources; the control block and h is on the stack)
   e = hipStreamSynchronize(p->us);
-  if (e != hipSuccess) return ECO_ERR_HIP;
+  if (e != hipSuccess) return fail(ECO_ERR_HIP);
   if (stg) p->arena.release(soff, stg);
   p->admit_pending = true;
   cs.running = 1;

@@ -160,10 +160,13 @@ __device__ __forceinline__ void thinc(CellState<NPH> &s, double acell, double a_
       double qmin = smin(a_hi, a_lo);
       double qmax = smax(a_hi, a_lo) - qmin;
       double C = (acell - qmin + 1.e-20) / (qmax + 1.e-20);
-      double B = exp(sign * beta * (2. * C - 1.));
-      double A = (B / cosh(beta) - 1.) / tanh(beta);
+      // exp as the C library computes it; cosh(1.6) and tanh(1.6) are the
+      // library's values (checked against it on the host)
+      constexpr double COSH_BETA = 0x1.49ea5b153125dp+1, TANH_BETA = 0x1.d7e4f0d5dad30p-1;
+      double B = exp_lib(sign * beta * (2. * C - 1.));
+      double A = (B / COSH_BETA - 1.) / TANH_BETA;
       double na;
-      if (!right_side) na = qmin + 0.5 * qmax * (1. + sign * (tanh(beta) + A) / (1. + A * tanh(beta)));
+      if (!right_side) na = qmin + 0.5 * qmax * (1. + sign * (TANH_BETA + A) / (1. + A * TANH_BETA));
       else na = qmin + 0.5 * qmax * (1. + sign * A);
       if (na < epsI) na = epsI;
       if (na > 1. - epsI) na = 1. - epsI;
@@ -838,6 +841,8 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int slot, c
       sn.rho = 0.;
 #pragma unroll
       for (int k = 0; k < NPH; k++) sn.rho += sn.a[k] * sn.r[k];
+      // MultiP with relaxation: the mixture density cached before the relaxation
+      if (MODEL == MODEL_MULTIP && S.rhoc) sn.rho = Wn[(Ly::IE + 1) * nc + cidx];
     }
     FluxVec<NPH> buf;
     build_cons<MODEL, NPH>(S, sn, buf);
@@ -860,7 +865,11 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int slot, c
     }
     if (bad) atomicOr(&ctl->err, 2u);
     store_prim<MODEL, NPH, DIM>(Wout, nc, cidx, s1);
-    if constexpr (STAGE != 1) Wout[Ly::IE * nc + cidx] = s1.E;
+    if constexpr (STAGE != 1) {
+      Wout[Ly::IE * nc + cidx] = s1.E;
+      // buildPrim's mixture density (the relaxation leaves it unchanged)
+      if (MODEL == MODEL_MULTIP && S.rhoc) Wout[(Ly::IE + 1) * nc + cidx] = s1.rho;
+    }
   }
 
   // CFL reduction: wave shuffle, LDS, one atomic per block
@@ -920,6 +929,7 @@ __global__ void k_init(const StageArgs A) {
   fulfill<MODEL, NPH>(A.sch, s);
   if constexpr (MODEL != MODEL_EULER) s.E = s.em + 0.5 * sqn(s.u);
   A.wout[Ly::IE * nc + i] = s.E;
+  if (MODEL == MODEL_MULTIP && A.sch.rhoc) A.wout[(Ly::IE + 1) * nc + i] = s.rho;
 }
 
 // ---------------------------------------------------------------------------

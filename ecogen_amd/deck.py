@@ -514,7 +514,7 @@ def _run_with_output(ctx, case, outdir, resume=None):
     t0 = time.perf_counter()
     info = open(outdir / "infoCalcul.out", "w")
     if resume is None:
-        files, it0 = [], 0
+        files, it0, ct0 = [], 0, 0
         info.write("1\n")
     else:
         # the resumed run keeps infoCalcul.out up to row N (Output::readInfos)
@@ -522,6 +522,8 @@ def _run_with_output(ctx, case, outdir, resume=None):
         n0, it0, kept, _ = resume
         files = [outdir / f"result_CPU0_AMR0_TIME{k}.vtr" for k in range(n0 + 1)]
         info.write("\n".join(kept) + "\n")
+        # the computation time continues from row N's (timeStats::setCompTime, Output.cpp:408)
+        ct0 = int(kept[-1].split()[3])
 
     gnu = o["format"] == "GNU"
 
@@ -534,7 +536,7 @@ def _run_with_output(ctx, case, outdir, resume=None):
             f = outdir / f"result_CPU0_AMR0_TIME{len(files)}.vtr"
             output.write_vtr(f, case, ctx.get_state(), binary=o["binary"], precision=prec)
         it, t, dt = (0, 0.0, 1.e-15) if st is None else (it0 + st.steps, st.t, st.dt_last)
-        ms = int((time.perf_counter() - t0) * 1e3) if st is not None else 0
+        ms = ct0 + int((time.perf_counter() - t0) * 1e3) if st is not None else 0
         info.write(f"{len(files)} {it} {t:.{prec}g} {ms} {dt:.{prec}g}\n")
         files.append(f)
     # probes (OutputProbeGNU): each file emptied at the start with its script;
@@ -578,13 +580,19 @@ def _run_with_output(ctx, case, outdir, resume=None):
             acquire(0.0, ctx.get_state(), first=True)
     if case["time_mode"] == "iterations":
         # a result file whenever the iteration is a multiple of iterFreq
+        # (Run.cpp:239-241), none at a last iteration that is not; the loop
+        # takes at least one step, as Run::solver does, also for a run
+        # resumed at or beyond its step count
         step = o["iter_freq"] if o["iter_freq"] > 0 else case["steps"]
         done = it0
-        while done < case["steps"]:
-            n = min(step - done % step, case["steps"] - done)
+        while True:
+            n = max(1, min(step - done % step, case["steps"] - done))
             st = advance(nsteps=n)
             done = it0 + st.steps
-            dump(st)
+            if done % step == 0:
+                dump(st)
+            if done >= case["steps"]:
+                break
     else:
         # Run::solver (Run.cpp:208, 241-243): a result file at each step with
         # t > printSuivante (double), which starts at the run's first time plus
@@ -594,7 +602,9 @@ def _run_with_output(ctx, case, outdir, resume=None):
         ff = float(np.float32(o["time_freq"]))
         print_next = (resume[3] if resume is not None else 0.0) + ff
         t = resume[3] if resume is not None else 0.0
-        while t < tfin:
+        first = True
+        while first or t < tfin:  # at least one step (Run::solver)
+            first = False
             # the device stops at the first t >= a float target: the largest
             # float not above print_next, then single steps to t > print_next
             low = np.float32(print_next)
@@ -638,7 +648,8 @@ def main(argv=None):
         if args.vtk:
             st = _run_with_output(ctx, case, Path(args.rundir) / "results" / case["name"], resume)
         elif case["time_mode"] == "iterations":
-            st = ctx.run(nsteps=case["steps"] - it0) if case["steps"] > it0 else ctx.sync()
+            # Run::solver takes at least one step
+            st = ctx.run(nsteps=max(1, case["steps"] - it0))
         else:
             st = ctx.run(t_final=case["t_final"])
         out = ctx.get_state()

@@ -206,3 +206,24 @@ def test_deck_probe_files_match_reference(name, tmp_path):
         assert [len(r) for r in ours] == [len(r) for r in ref]
         a, b = np.array(ours, dtype=float), np.array(ref, dtype=float)
         assert np.all(np.abs(a - b) <= 5e-6 * np.abs(b) + 1e-300)
+
+
+@pytest.mark.gpu
+def test_deck_iteration_output_only_at_multiples_of_iterfreq(tmp_path):
+    """Iteration control writes a result file at t = 0 and whenever the
+    iteration is a multiple of iterFreq (Run.cpp:239-241), and none at a last
+    iteration that is not: 25 steps at iterFreq 10 give TIME0..TIME2 and three
+    infoCalcul.out rows (iterations 0, 10, 20) after the CPU count, while the
+    run still ends at iteration 25."""
+    from ecogen_amd import deck
+    case, _ = load_golden("c1_kapila_fo_wall")
+    case = dict(case, name="freq", time_mode="iterations", steps=25, iter_freq=10)
+    case.pop("t_final", None)
+    print("case c1_kapila_fo_wall deck --vtk, 25 steps at iterFreq 10", flush=True)
+    write_deck(case, tmp_path / "run")
+    deck.main([str(tmp_path / "run"), "--out", str(tmp_path / "f.npz"), "--vtk"])
+    res = tmp_path / "run" / "results" / "freq"
+    assert sorted(p.name for p in res.glob("result_*.vtr")) == [f"result_CPU0_AMR0_TIME{k}.vtr" for k in range(3)]
+    lines = [ln for ln in (res / "infoCalcul.out").read_text().split("\n") if ln.strip()]
+    assert lines[0] == "1" and [int(r.split()[1]) for r in lines[1:]] == [0, 10, 20]
+    assert int(np.load(tmp_path / "f.npz")["steps"]) == 25

@@ -1,0 +1,73 @@
+"""FP64 device path against the oracle over long runs of the benchmark physics.
+
+The golden cases run 8 to 40 steps.  These runs take the physics of the 2-D
+shock/cylinder workload (Kapila, MUSCL van Leer) and of the 3-D bubble
+workload (MultiP with stiff pressure relaxation, MUSCL minmod) far enough for
+the waves to reach the bodies, and compare the device with the oracle on the
+same initial state after a fixed number of steps: step count, time and next
+dt exactly, and every field (per-phase alpha_k, rho_k, p_k, velocity, E)
+within 1e-12 relative L1, and then bit for bit: both paths compute every
+operation with the reference's rounding (IEEE +, -, *, /, sqrt, no
+contraction, and the C library's pow restated on the device).  The bubble
+run also takes the relaxation's 100-iteration branch (flag
+ECO_WARN_RELAX_NOT_CONVERGED), so device and oracle agree there too.
+"""
+import numpy as np
+import pytest
+
+from conftest import compared_fields, rel_l1
+
+import oracle as O
+from ecogen_amd import workloads as W
+
+pytestmark = [pytest.mark.gpu, pytest.mark.timeout(600)]
+
+TOL = 1e-12
+
+
+def _compare(case, st0, nsteps):
+    from ecogen_amd import Context
+    ref, it, t, dtn, _ = O.run(case, st0, nsteps=nsteps, nthreads=16)
+    with Context(case) as ctx:
+        ctx.set_state(st0)
+        status = ctx.run(nsteps=nsteps)
+        out = ctx.get_state()
+    assert status.steps == it == nsteps
+    assert status.t == t, (status.t, t)
+    assert status.dt_next == dtn, (status.dt_next, dtn)
+    nf = out.shape[0]
+    worst = 0.0
+    for f in range(nf):
+        e = rel_l1(out[f], ref[f])
+        d = float(np.abs(out[f] - ref[f]).max())
+        print(f"  field {f}: rel L1 {e:.3e}, max |diff| {d:.3e}", flush=True)
+        worst = max(worst, e)
+    for f, name in compared_fields(case):
+        assert rel_l1(out[f], ref[f]) <= TOL, name
+    assert rel_l1(out[nf - 1], ref[nf - 1]) <= TOL, "E"
+    # and bit for bit
+    for f in range(nf):
+        assert np.array_equal(out[f], ref[f]), f"field {f} differs from the oracle"
+    return status, worst
+
+
+def test_bubbles_3d_64_multip_relaxation_120_steps():
+    """3-D bubble physics (air bubbles at 1e5 Pa in water at 5e7 Pa, MultiP +
+    relaxation, MUSCL minmod) on 64^3 cells, 8 bubbles of radius 3 to 10
+    cells, 120 steps: the bubbles collapse and the relaxation works in every
+    interface cell."""
+    case, st0 = W.bubbles_3d(n=64, nb=8, rmin=3.0, rmax=10.0, steps=120)
+    print("case bubbles 64^3 MultiP+relax minmod, 120 steps", flush=True)
+    status, worst = _compare(case, st0, 120)
+    print(f"bubbles 64^3: worst rel L1 {worst:.3e}, flags {status.flags}", flush=True)
+    assert status.flags & 1, "the run no longer reaches the relaxation's iteration limit"
+
+
+def test_shock_cylinders_2d_512_kapila_300_steps():
+    """2-D shock/cylinder physics (Mach 2.4 air shock, 32 water cylinders of
+    radius 2.5 to 10 cells, Kapila, MUSCL van Leer) on 512^2 cells, 300 steps:
+    the shock reaches the first cylinders at about step 100."""
+    case, st0 = W.shock_cylinders_2d(n=512, ncyl=32, rmin=2.5, rmax=10.0, steps=300)
+    print("case shock/cylinders 512^2 Kapila vanleer, 300 steps", flush=True)
+    status, worst = _compare(case, st0, 300)
+    print(f"cylinders 512^2: worst rel L1 {worst:.3e}", flush=True)

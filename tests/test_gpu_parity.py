@@ -1,26 +1,26 @@
 """HIP path against the oracle and the reference goldens (FP64).
 
 Every golden case (all models, first order and MUSCL, every limiter, THINC,
-alphaNull, relaxation, 1-D/2-D/3-D) runs through the C ABI on the GPU from
-the reference's t=0 state.  The final state must match the oracle and the
-reference's output within 1e-12 relative L1 in mixture density, velocity,
-mixture pressure and per-phase volume fraction, with the same number of
-steps and the same final time.  Per-phase densities and pressures are held
-to 1e-9: in cells at the 1e-6 volume-fraction floor, rho_k = (alpha rho)_k /
-alpha_k divides two quantities produced by cancellation, which amplifies
-last-bit differences of the device arithmetic by up to ~1e6.
+alphaNull, relaxation, 1-D/2-D/3-D, every boundary type, three and four
+phases) runs through the C ABI on the GPU from the reference's t=0 state.
+The device computes every operation with the reference's rounding (IEEE
++, -, *, /, sqrt without contraction; pow and exp as the C library computes
+them), so its final state must equal the oracle's bit for bit in every field
+(per-phase alpha_k, rho_k, p_k, velocity, E), with the same number of steps
+and the same final time.  Against the reference's own printed output the
+bar is 1e-12 relative L1 in mixture density, velocity, mixture pressure and
+per-phase volume fraction (the oracle reproduces those files exactly).
 """
 import numpy as np
 import pytest
 
-from conftest import initial_state, golden_names, load_golden, parity_quantities, phase_quantities, rel_l1
+from conftest import initial_state, golden_names, load_golden, parity_quantities, rel_l1
 
 import oracle as O
 
 pytestmark = pytest.mark.gpu
 
-TOL = 1e-12        # FP64 parity bar, relative L1: mixture rho, u, mixture p, alpha_k
-TOL_PHASE = 1e-9   # per-phase rho_k, p_k
+TOL = 1e-12        # FP64 parity bar against the reference's printed output, relative L1
 
 
 def _run_gpu(case, state):
@@ -42,8 +42,11 @@ def test_gpu_matches_oracle_and_reference(name):
     ref, it, t, _, _ = O.run(case, st0)
     out, status = _run_gpu(case, st0)
     assert status.steps == it
-    assert status.t == pytest.approx(t, rel=1e-12)
+    assert status.t == t
     assert status.flags & 2 == 0
+    for f in range(out.shape[0]):
+        nd = int(np.count_nonzero(out[f] != ref[f]))
+        assert nd == 0, f"field {f}: {nd} cells differ from the oracle (max {np.abs(out[f] - ref[f]).max():.3e})"
     fin = O.state_from_golden(case, d, "final")
     got, want = parity_quantities(case, out), parity_quantities(case, ref)
     for nm in got:
@@ -58,10 +61,6 @@ def test_gpu_matches_oracle_and_reference(name):
         else:
             e2 = rel_l1(got[nm], ref_final[nm])
         assert e2 <= TOL, f"{nm}: rel L1 vs reference {e2:.3e}"
-    gp, wp = phase_quantities(case, out), phase_quantities(case, ref)
-    for nm in gp:
-        e = rel_l1(gp[nm], wp[nm])
-        assert e <= TOL_PHASE, f"{nm}: rel L1 vs oracle {e:.3e}"
 
 
 def test_walls_are_refused_in_fp32_storage():
