@@ -28,7 +28,11 @@ WARN_NAN = 2
 WARN_INFLOW_NOT_CONVERGED = 4
 
 EXPORTS = ["eco_create", "eco_set_state", "eco_set_state_device", "eco_set_clock", "eco_get_state", "eco_step", "eco_run",
-           "eco_launch_steps", "eco_sync", "eco_time_steps", "eco_copy_calibration", "eco_state_device", "eco_destroy", "eco_strerror", "eco_version"]
+           "eco_launch_steps", "eco_sync", "eco_time_steps", "eco_copy_calibration", "eco_state_device", "eco_destroy", "eco_strerror", "eco_version",
+           "eco_run_noalloc", "eco_noalloc_bytes"]
+
+ERR_SCRATCH = -8            # eco_run_noalloc: the solve exceeds the static scratch
+NOALLOC_SCRATCH_BYTES = 256 << 20  # ECO_NOALLOC_SCRATCH_BYTES
 
 
 class EcoEos(C.Structure):
@@ -100,6 +104,10 @@ def lib():
         L.eco_strerror.restype = C.c_char_p
         L.eco_version.argtypes = []
         L.eco_version.restype = C.c_char_p
+        L.eco_run_noalloc.argtypes = [C.POINTER(EcoConfig), vp, vp, C.c_int, C.c_double, vp, C.POINTER(EcoStatus)]
+        L.eco_run_noalloc.restype = C.c_int
+        L.eco_noalloc_bytes.argtypes = [C.POINTER(EcoConfig)]
+        L.eco_noalloc_bytes.restype = C.c_ulonglong
         _lib = L
     return _lib
 
@@ -243,3 +251,20 @@ class Context:
         nf = C.c_int()
         p = lib().eco_state_device(self._h, C.byref(nf))
         return p, nf.value
+
+
+def noalloc_bytes(case) -> int:
+    """Scratch bytes eco_run_noalloc needs for the case (0: invalid case)."""
+    cfg = make_config(case)
+    return int(lib().eco_noalloc_bytes(C.byref(cfg)))
+
+
+def run_noalloc(case, dev_in: int, dev_out: int, nsteps: int = 0, t_final: float = 0.0, stream=None) -> EcoStatus:
+    """eco_run_noalloc from device array ``dev_in`` to ``dev_out`` (both in the
+    ecogen.h layout, 3*nphase+4 fields of ncell doubles): the allocation-free
+    form of a solve (static device scratch of NOALLOC_SCRATCH_BYTES)."""
+    cfg = make_config(case)
+    st = EcoStatus()
+    _check(lib().eco_run_noalloc(C.byref(cfg), C.c_void_p(dev_in), C.c_void_p(dev_out), int(nsteps), float(t_final),
+                                 stream, C.byref(st)), "eco_run_noalloc")
+    return st

@@ -54,6 +54,7 @@ extern "C" const char *eco_strerror(int code) {
     case ECO_ERR_HIP: return "HIP runtime error";
     case ECO_ERR_NOMEM: return "device allocation failed";
     case ECO_ERR_STATE: return "call out of order";
+    case ECO_ERR_SCRATCH: return "solve larger than the allocation-free entry point's static scratch";
   }
   return "unknown error";
 }
@@ -226,7 +227,7 @@ int eco::validate_config(const eco_config *c) {
   return ECO_OK;
 }
 
-void eco::lower_config(const eco_config &c, Lowered &L) {
+void eco::lower_config(const eco_config &c, Lowered &L, bool with_tables) {
   // geometry (src/Meshes/MeshCartesian.cpp:180-262, 300-397)
   Geometry &G = L.geo;
   std::memset(&G, 0, sizeof(G));
@@ -246,12 +247,12 @@ void eco::lower_config(const eco_config &c, Lowered &L) {
   // per-direction distance tables (cell-centre and face positions as the reference builds them)
   size_t ntab = 0;
   for (int d = 0; d < 3; d++) ntab += 3 * (size_t)std::max(c.n[d], 1);
-  L.tab.assign(ntab, 0.);
+  if (with_tables) L.tab.assign(ntab, 0.);
   size_t o = 0;
   for (int d = 0; d < 3; d++) {
     L.off[d] = o;
     int n = c.n[d];
-    for (int i = 0; i + 1 < n; i++) {
+    for (int i = 0; with_tables && i + 1 < n; i++) {
       double xi = (i + 0.5) * dd[d], xj = (i + 1 + 0.5) * dd[d];
       double xf = xi + 0.5 * dd[d];
       double v = xi - xj;
@@ -444,29 +445,31 @@ extern "C" int eco_set_state_device(eco_ctx *x, const double *dev_state) {
   return finish_set_state(x);
 }
 
+void eco::fill_status(const Control &h, double lcfl, double cfl, eco_status *st) {
+  st->steps = h.steps;
+  st->t = (h.steps > 0) ? h.t_now : h.t0;
+  st->dt_last = h.dt_last;
+  double dtn = (h.dt0 > 0.) ? h.dt0 : 1.e-15;
+  if (h.steps > 0) {
+    // a solve's step k runs at launch slot k
+    unsigned long long b = h.smax[(h.steps - 1) % 3];
+    double sm;
+    std::memcpy(&sm, &b, sizeof(sm));
+    double dtMax = (sm > 0.) ? std::min(1.e10, lcfl / sm) : 1.e10;
+    dtn = cfl * dtMax;
+  }
+  st->dt_next = dtn;
+  st->flags = h.err;
+  st->done = h.finished;
+}
+
 static int read_status(eco_ctx *x, eco_status *st, Control *hc) {
   Control h;
   hipError_t e = hipMemcpyAsync(&h, x->ctl, sizeof(Control), hipMemcpyDeviceToHost, x->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(x->stream);
   if (e != hipSuccess) return ECO_ERR_HIP;
   if (hc) *hc = h;
-  if (st) {
-    st->steps = h.steps;
-    st->t = (h.steps > 0) ? h.t_now : h.t0;
-    st->dt_last = h.dt_last;
-    double dtn = (h.dt0 > 0.) ? h.dt0 : 1.e-15;
-    if (h.steps > 0) {
-      // the context's step k runs at launch slot k
-      unsigned long long b = h.smax[(h.steps - 1) % 3];
-      double sm;
-      std::memcpy(&sm, &b, sizeof(sm));
-      double dtMax = (sm > 0.) ? std::min(1.e10, x->geo.lcfl / sm) : 1.e10;
-      dtn = x->cfg.cfl * dtMax;
-    }
-    st->dt_next = dtn;
-    st->flags = h.err;
-    st->done = h.finished;
-  }
+  if (st) fill_status(h, x->geo.lcfl, x->cfg.cfl, st);
   return ECO_OK;
 }
 

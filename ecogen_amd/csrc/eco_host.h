@@ -22,9 +22,13 @@ struct Lowered {
 
 // ECO_OK or the eco_status_code the configuration is refused with.
 int validate_config(const eco_config *c);
-void lower_config(const eco_config &c, Lowered &L);
+// with_tables = false leaves L.tab empty (offsets are still set)
+void lower_config(const eco_config &c, Lowered &L, bool with_tables = true);
 void bind_tables(const eco_config &c, Lowered &L, double *dev_tables);
 // host field index of each device primitive field (host layout of ecogen.h)
 void host_field_map(const eco_config &c, int dim, std::vector<int> &map);
+// eco_status of a solve from its control block (the next dt from the last
+// step's wave-speed maximum, as Run::solver sets m_dt, src/Run.cpp:270)
+void fill_status(const Control &h, double lcfl, double cfl, eco_status *st);
 
 }  // namespace eco

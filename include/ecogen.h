@@ -49,7 +49,8 @@ enum eco_status_code {
   ECO_ERR_UNSUPPORTED = -2,  /* valid reference option not provided by the device path */
   ECO_ERR_HIP = -3,          /* HIP runtime error (launch, allocation, copy) */
   ECO_ERR_NOMEM = -4,        /* device allocation failed */
-  ECO_ERR_STATE = -5         /* call out of order (e.g. step before set_state) */
+  ECO_ERR_STATE = -5,        /* call out of order (e.g. step before set_state) */
+  ECO_ERR_SCRATCH = -8       /* eco_run_noalloc: the solve exceeds the static scratch */
 };
 
 /* Soft-error flags (mirror the reference's non-fatal errors vector). */
@@ -163,6 +164,29 @@ int eco_copy_calibration(void *dst, const void *src, unsigned long long nbytes, 
    FP64 contexts only: an ECO_FP32S context keeps a packed storage format and
    returns NULL with *nfields_device = 0 (eco_get_state converts it). */
 const double *eco_state_device(eco_ctx *ctx, int *nfields_device);
+
+/* Allocation-free form.  No counterpart in the reference: for callers that
+   cannot allocate device memory (code under HIP graph capture, tight serving
+   loops, harnesses that forbid hipMalloc).  One call does what eco_create +
+   eco_set_state_device + eco_run + reading the state back do, from state_in
+   to state_out, both device arrays in the layout above (E ignored on input,
+   written on output; they may not overlap), with the same results bit for
+   bit.  nsteps > 0: exactly nsteps steps; nsteps <= 0: until t >= t_final
+   (as float), at most -nsteps steps when nsteps < 0.  FP64 only.
+   Every intermediate buffer (second time level, MUSCL U*, distance tables,
+   control block) lives in a static device array of the library,
+   ECO_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends; no
+   hipMalloc, hipFree or graph is used.  A solve needing more scratch than
+   that (eco_noalloc_bytes) is refused with ECO_ERR_SCRATCH before any GPU
+   work.  All calls on one GPU share its scratch: calls that overlap in time
+   on one GPU race (each call synchronises hip_stream before returning).
+   hip_stream: hipStream_t (NULL = default stream). */
+#define ECO_NOALLOC_SCRATCH_BYTES (256ull << 20)
+int eco_run_noalloc(const eco_config *cfg, const double *state_in, double *state_out, int nsteps, double t_final,
+                    void *hip_stream, eco_status *status);
+
+/* Scratch bytes eco_run_noalloc needs for cfg (0 for an invalid cfg). */
+unsigned long long eco_noalloc_bytes(const eco_config *cfg);
 
 /* Releases device memory (Run::finalize, src/Run.cpp:496-530). */
 void eco_destroy(eco_ctx *ctx);
