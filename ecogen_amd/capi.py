@@ -32,6 +32,7 @@ EXPORTS = ["eco_create", "eco_set_state", "eco_set_state_device", "eco_set_clock
            "eco_run_noalloc", "eco_noalloc_bytes"]
 
 ERR_SCRATCH = -8            # eco_run_noalloc: the solve exceeds the static scratch
+ERR_NOT_CONVERGED = -9      # an injection or tank solve stopped the run (the reference exits there)
 NOALLOC_SCRATCH_BYTES = 256 << 20  # ECO_NOALLOC_SCRATCH_BYTES
 
 
@@ -220,11 +221,18 @@ class Context:
 
     def run(self, nsteps: int = 0, t_final: float = 0.0, max_steps: int = 0) -> EcoStatus:
         """nsteps > 0: exactly nsteps steps; otherwise until t >= t_final (as
-        float), stopping after max_steps steps if max_steps > 0."""
+        float), stopping after max_steps steps if max_steps > 0.  A run that
+        an injection or tank solve stopped raises EcoError (code
+        ERR_NOT_CONVERGED) carrying the status in ``.status``."""
         if nsteps <= 0 and max_steps > 0:
             nsteps = -int(max_steps)
         st = EcoStatus()
-        _check(lib().eco_run(self._h, int(nsteps), float(t_final), C.byref(st)), "eco_run")
+        rc = lib().eco_run(self._h, int(nsteps), float(t_final), C.byref(st))
+        if rc == ERR_NOT_CONVERGED:
+            err = EcoError(rc, "eco_run")
+            err.status = st
+            raise err
+        _check(rc, "eco_run")
         return st
 
     def launch(self, nsteps: int):

@@ -55,6 +55,7 @@ extern "C" const char *eco_strerror(int code) {
     case ECO_ERR_NOMEM: return "device allocation failed";
     case ECO_ERR_STATE: return "call out of order";
     case ECO_ERR_SCRATCH: return "solve larger than the allocation-free entry point's static scratch";
+    case ECO_ERR_NOT_CONVERGED: return "an injection or tank boundary solve did not converge: the run stopped";
   }
   return "unknown error";
 }
@@ -599,13 +600,19 @@ extern "C" int eco_sync(eco_ctx *x, eco_status *st) {
   return ECO_OK;
 }
 
+// ECO_ERR_NOT_CONVERGED once a boundary solve stopped the run (status kept)
+static int stop_code(const eco_status &s) { return (s.flags & ECO_WARN_INFLOW_NOT_CONVERGED) ? ECO_ERR_NOT_CONVERGED : ECO_OK; }
+
 extern "C" int eco_step(eco_ctx *x, eco_status *st) {
   if (!x) return ECO_ERR_INVALID;
   if (!x->have_state) return ECO_ERR_STATE;
   int rc = set_target(x, x->step + 1, -1.);
   if (!rc) rc = launch_one(x);
-  if (!rc) rc = eco_sync(x, st);
-  return rc;
+  eco_status s;
+  if (!rc) rc = eco_sync(x, &s);
+  if (rc) return rc;
+  if (st) *st = s;
+  return stop_code(s);
 }
 
 extern "C" int eco_run(eco_ctx *x, int nsteps, double t_final, eco_status *st) {
@@ -616,7 +623,11 @@ extern "C" int eco_run(eco_ctx *x, int nsteps, double t_final, eco_status *st) {
     rc = set_target(x, x->step + nsteps, -1.);
     if (!rc) rc = launch_steps(x, nsteps);
     if (rc) return rc;
-    return eco_sync(x, st);
+    eco_status s;
+    rc = eco_sync(x, &s);
+    if (rc) return rc;
+    if (st) *st = s;
+    return stop_code(s);
   }
   rc = set_target(x, nsteps < 0 ? x->step - nsteps : 0, t_final);
   if (rc) return rc;
@@ -633,7 +644,7 @@ extern "C" int eco_run(eco_ctx *x, int nsteps, double t_final, eco_status *st) {
     chunk = std::min(chunk * 2, 4 * kGraphSteps);
   }
   if (st) *st = s;
-  return ECO_OK;
+  return stop_code(s);
 }
 
 extern "C" int eco_time_steps(eco_ctx *x, int nsteps, double *stage_ms, int *launches, eco_status *st) {

@@ -254,5 +254,6 @@ extern "C" int eco_run_noalloc(const eco_config *cfg, const double *state_in, do
     return ECO_ERR_HIP;
   if (hipStreamSynchronize(st) != hipSuccess) return ECO_ERR_HIP;
   if (status) fill_status(h, L.geo.lcfl, c.cfl, status);
-  return ECO_OK;
+  // a boundary solve that stopped the run: as eco_run
+  return (h.err & ECO_WARN_INFLOW_NOT_CONVERGED) ? ECO_ERR_NOT_CONVERGED : ECO_OK;
 }

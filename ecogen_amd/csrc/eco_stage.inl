@@ -310,12 +310,16 @@ __device__ __forceinline__ bool step_begin(Control *ctl, const Geometry &G, doub
                                            bool first_kernel, bool last_kernel, StepInfo &si) {
   const int prv = (slot + 1) & 1;
   const int cnt = ctl->cnt[prv];
-  if (ctl->done[prv]) {
+  // an injection or tank solve that did not converge stops the run, as the
+  // reference's process ends there (Errors::errorMessage, ModKapila.cpp:283)
+  const bool stopped = (ctl->err & 4u) != 0;
+  if (ctl->done[prv] || stopped) {
     if (last_kernel && blk == 0 && threadIdx.x == 0) {
       const int cur = slot & 1;
       ctl->done[cur] = 1;
       ctl->t[cur] = ctl->t[prv];
       ctl->cnt[cur] = cnt;
+      if (stopped) ctl->finished = 1;
     }
     return false;
   }

@@ -50,14 +50,20 @@ enum eco_status_code {
   ECO_ERR_HIP = -3,          /* HIP runtime error (launch, allocation, copy) */
   ECO_ERR_NOMEM = -4,        /* device allocation failed */
   ECO_ERR_STATE = -5,        /* call out of order (e.g. step before set_state) */
-  ECO_ERR_SCRATCH = -8       /* eco_run_noalloc: the solve exceeds the static scratch */
+  ECO_ERR_SCRATCH = -8,      /* eco_run_noalloc: the solve exceeds the static scratch */
+  ECO_ERR_NOT_CONVERGED = -9 /* an injection or tank boundary's Newton solve did not converge in 50
+                                iterations: the run stopped there, as the reference's process ends
+                                there (Errors::errorMessage; src/Models/Kapila/ModKapila.cpp:283, 405);
+                                the status still reports the steps completed */
 };
 
 /* Soft-error flags (mirror the reference's non-fatal errors vector). */
 #define ECO_WARN_RELAX_NOT_CONVERGED 1u /* RelaxationP Newton hit 100 iterations (src/Relaxations/RelaxationP.cpp:92-109) */
 #define ECO_WARN_NAN 2u                 /* non-finite pressure or density produced */
 #define ECO_WARN_INFLOW_NOT_CONVERGED 4u /* injection or tank Newton hit 50 iterations; the reference
-                                            stops there (src/Models/Kapila/ModKapila.cpp:283, 405) */
+                                            stops there (src/Models/Kapila/ModKapila.cpp:283, 405), and
+                                            so does the device: later steps are no-ops, eco_step and
+                                            eco_run return ECO_ERR_NOT_CONVERGED */
 
 /* Equation of state (src/Eos/EosIG.h, src/Eos/EosSG.h; parameters of libEOS/*.xml). */
 typedef struct {

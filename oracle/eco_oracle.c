@@ -356,6 +356,14 @@ static void scheme_correction(const Ctx *c, Flux *f, const Flux *buf) {
 /* ------------------------------------------------------------------------ */
 /* Relaxation and energy correction                                          */
 /* ------------------------------------------------------------------------ */
+/* Relaxations that reached the iteration limit in the last orc_run_from, and
+   the (alpha, rho, p) per phase a first one started from (test statistics). */
+static long g_relax_fail;
+static double g_relax_fail_in[3 * MAXPH];
+/* 1-based step of the last orc_run_from in which an injection or tank Newton
+   solve did not converge (0: none); the run stops there */
+static int g_inflow_fail_step;
+
 /* RelaxationP::stiffRelaxation (RelaxationP.cpp:49-111). Returns iterations. */
 static int relax_p(const Ctx *c, State *s) {
   const orc_cfg *cfg = c->cfg;
@@ -391,6 +399,14 @@ static int relax_p(const Ctx *c, State *s) {
   if (it < 100) {
     for (int k = 0; k < n; k++) { s->ph[k].alpha = akS[k]; s->ph[k].rho = rkS[k]; s->ph[k].p = pStar; }
     s->mx.p = pStar;
+  } else {
+    long nfail;
+#pragma omp atomic capture
+    nfail = ++g_relax_fail;
+    if (nfail == 1)
+      for (int k = 0; k < n; k++) {
+        g_relax_fail_in[3 * k] = ak[k]; g_relax_fail_in[3 * k + 1] = rk[k]; g_relax_fail_in[3 * k + 2] = pk[k];
+      }
   }
   return it;
 }
@@ -1660,6 +1676,7 @@ int orc_run_from(const orc_cfg *cfg, const double *state_in, double *state_out,
 #endif
   Solver S;
   memset(&S, 0, sizeof(S));
+  g_relax_fail = 0;
   S.c.cfg = cfg;
   S.c.epsilon = (cfg->model == ORC_KAPILA && cfg->alpha_null) ? 1.e-15 : 0.;
   mesh_build(&S.m, cfg);
@@ -1676,8 +1693,15 @@ int orc_run_from(const orc_cfg *cfg, const double *state_in, double *state_out,
   double dt = dt0, t = t0;
   int it = 0;
   float tf = (float)t_final;
+  g_inflow_fail_step = 0;
   for (;;) {
     double dtMax = step(&S, dt);
+    if (S.inflow_nonconv) {
+      /* Errors::errorMessage ends the reference's process during this step
+         (ModKapila.cpp:283, ModEuler.cpp, ModMultiP.cpp solveRiemannInflow) */
+      g_inflow_fail_step = it + 1;
+      break;
+    }
     if (dt_log) dt_log[it] = dt;
     t += dt;
     it++;
@@ -1706,6 +1730,13 @@ double orc_eos_isentrope(const orc_eos *e, double p0, double rho0, double rho1, 
 }
 double orc_eos_rho_pfinal(const orc_eos *e, double p0, double rho0, double pf, double *drho, double eps) {
   return eos_rho_pfinal(e, p0, rho0, pf, drho, eps);
+}
+
+int orc_inflow_failure_step(void) { return g_inflow_fail_step; }
+
+long orc_relax_failures(double *first_inputs) {
+  if (first_inputs) memcpy(first_inputs, g_relax_fail_in, sizeof(g_relax_fail_in));
+  return g_relax_fail;
 }
 
 int orc_relax_p(const orc_cfg *cfg, double *alpha, double *rho, double *p) {

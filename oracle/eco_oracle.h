@@ -96,6 +96,14 @@ double orc_eos_isentrope(const orc_eos *e, double p0, double rho0, double rho1, 
 double orc_eos_rho_pfinal(const orc_eos *e, double p0, double rho0, double pf, double *drho, double epsilon);
 /* Relaxation on one cell: in/out alpha, rho, p (nph each). Returns iterations (100 = not applied). */
 int orc_relax_p(const orc_cfg *cfg, double *alpha, double *rho, double *p);
+/* Relaxations of the last orc_run/orc_run_from that reached the iteration
+   limit (no update); first_inputs (3*MAXPH doubles, may be NULL) receives the
+   alpha, rho, p per phase (after verifyAndCorrectPhase) of the first one. */
+long orc_relax_failures(double *first_inputs);
+/* The 1-based step of the last orc_run/orc_run_from in which an injection or
+   tank Newton solve reached 50 iterations (the run stopped there, returning
+   -3, as the reference's process ends there); 0 when none did. */
+int orc_inflow_failure_step(void);
 /* One HLLC solve in the face frame, inputs are primitive states (alpha, rho, p per phase; u,v,w).
    flux_out: alpha[nph], mass[nph], energy[nph], qdm[3], E, sM  (Euler: mass, qdm[3], E, sM). */
 void orc_hllc(const orc_cfg *cfg, const double *wl, const double *wr, double dxl, double dxr,

@@ -75,6 +75,10 @@ def lib():
         L.orc_eos_rho_pfinal.restype = C.c_double
         L.orc_relax_p.argtypes = [C.POINTER(OrcCfg), dp, dp, dp]
         L.orc_relax_p.restype = C.c_int
+        L.orc_inflow_failure_step.argtypes = []
+        L.orc_inflow_failure_step.restype = C.c_int
+        L.orc_relax_failures.argtypes = [dp]
+        L.orc_relax_failures.restype = C.c_long
         L.orc_hllc.argtypes = [C.POINTER(OrcCfg), dp, dp, C.c_double, C.c_double, dp, dp]
         L.orc_hllc.restype = None
         _lib = L
@@ -175,3 +179,19 @@ def state_from_golden(case, d, prefix="init"):
             out[3 * k + 2] = d[f"{prefix}_p"][k]
     out[3 * nph:3 * nph + 3] = d[f"{prefix}_u"]
     return out
+
+
+def relax_failures():
+    """(count, first inputs) of the relaxations of the last run that reached
+    the 100-iteration limit; first inputs: per phase (alpha, rho, p) after
+    verifyAndCorrectPhase, shape (MAXPH, 3)."""
+    buf = np.zeros(3 * MAXPH)
+    n = lib().orc_relax_failures(_dp(buf))
+    return int(n), buf.reshape(MAXPH, 3)
+
+
+def inflow_failure_step():
+    """1-based step at which the last run stopped because an injection or tank
+    Newton solve did not converge (the reference's process ends there); 0 if
+    none did."""
+    return int(lib().orc_inflow_failure_step())

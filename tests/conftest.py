@@ -47,6 +47,18 @@ def initial_state(case, d):
     return st
 
 
+def initial_state_from_deck(case):
+    """The t = 0 state the deck reader builds from a case's deck (the
+    reference's geometric domains on its cell centres)."""
+    import tempfile
+    from deck import write_deck
+    from ecogen_amd import deck
+    with tempfile.TemporaryDirectory() as tmp:
+        write_deck(dict(case, name="t0"), Path(tmp) / "run")
+        _, s0 = deck.load(Path(tmp) / "run")
+    return s0
+
+
 def rel_l1(a, b):
     a = np.asarray(a, dtype=np.float64)
     b = np.asarray(b, dtype=np.float64)

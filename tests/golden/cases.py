@@ -235,6 +235,26 @@ def _bubbles_3d(n=16, steps=12, nb=3, seed=0xEC0C0003, limiters=("minmod",) * 4,
     }
 
 
+def _water_column_2d(n, ratio, pos, t_final):
+    """One 2-D case of the batched workload's distribution: a water column of
+    radius pos/2 at the centre of [0,1]^2, at ratio * 1e5 Pa, in air at 1e5 Pa
+    (both at rest; air density 50 as in the shock tube), Kapila, MUSCL van
+    Leer, to t_final = 0.3 L / a_max.  The column's expansion puts the water
+    in tension and the air at the ideal-gas pressure floor, where the
+    pressure relaxation stops at its 100 iterations without converging in
+    some cells (RelaxationP.cpp:92-109)."""
+    hp = {"name": "HP", "alpha": [1e-6, 1.0 - 1e-6], "rho": [50.0, 1000.0], "p": [ratio * 1e5] * 2, "u": [0.0, 0.0]}
+    lp = {"name": "LP", "alpha": [1.0 - 1e-6, 1e-6], "rho": [50.0, 1000.0], "p": [1e5, 1e5], "u": [0.0, 0.0]}
+    return {
+        "model": "kapila", "alpha_null": False, "eos": [AIR, WATER],
+        "cells": [n, n, 1], "lengths": [1.0, 1.0, 1.0], "order": 2, "limiters": ["vanleer"] * 4,
+        "cfl": 0.8, "time_mode": "time", "t_final": t_final,
+        "domains": [{"type": "entireDomain", "state": "LP"},
+                    {"type": "disc", "state": "HP", "radius": 0.5 * pos, "center": [0.5, 0.5]}],
+        "states": [lp, hp],
+    }
+
+
 CASES = {
     # Config C1 in full: 1000 cells, first order, to t = 240 us.
     "c1_kapila_fo": _c1(False),
@@ -256,6 +276,9 @@ CASES = {
     "c2_kapila_2d_o2": _shock_cylinders_2d(),
     "c2_kapila_2d_fo": dict(_shock_cylinders_2d(n=40, steps=40), order=1, limiters=["none"] * 4),
     "c3_multip_3d_o2": _bubbles_3d(),
+    # Case 1680 of the batched workload (seed 0xEC0C0004): the relaxation's
+    # iteration limit is reached in 124 cell updates.
+    "kapila_2d_o2_relax_limit": _water_column_2d(64, 2679.014153940562, 0.35536181384193916, 0.0001535181427478315),
     "kapila_3d_o2": _bubbles_3d(n=12, steps=10, model="kapila", relax=False, limiters=("vanleer",) * 4),
     # Wall boundaries (first order: BoundCondWall, also the reference's first-order SYMMETRY):
     # the C1 tube closed at both ends, run until the air shock has reflected off x = 1;
@@ -333,4 +356,14 @@ CASES = {
     "kapila_2d_fo_injection": dict(_shock_cylinders_2d(n=40, steps=80), order=1, limiters=["none"] * 4,
                                    bc=["abs", "outflow", "injection", "wall", "abs", "abs"],
                                    bc_p0=[0.0, 8e4] + [0.0] * 4, bc_inj=_inj(2, 300.0, 2, x_side=0)),
+}
+
+
+# Runs the reference stops (Errors::errorMessage, exit) because an injection
+# Newton solve does not converge in 50 iterations: a mass flow no left wave
+# can carry (ModKapila.cpp:283, ModEuler::solveRiemannInflow).  Output at
+# every iteration, so the last infoCalcul.out row is the last completed step.
+STOP_CASES = {
+    "kapila_1d_fo_injection_stop": dict(CASES["kapila_1d_fo_injection"], bc_inj=_inj(2, 1e6, 1, x_side=1), iter_freq=1),
+    "euler_1d_fo_injection_stop": dict(CASES["euler_1d_fo_injection"], bc_inj=_inj(1, 1e4, 1, x_side=1), iter_freq=1),
 }

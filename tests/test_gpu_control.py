@@ -85,3 +85,38 @@ def test_zero_copy_state_view(tube):
         ctx.set_state(s32)
         ptr, nf = ctx.state_device()
         assert not ptr and nf == 0
+
+
+@pytest.mark.parametrize("name", ["kapila_1d_fo_injection_stop", "euler_1d_fo_injection_stop"])
+def test_run_stops_where_the_reference_stops(name):
+    """An injection whose Newton solve does not converge in 50 iterations ends
+    the reference's run in that step (tests/golden/stop: its last completed
+    iteration and the message it printed); the oracle stops in the same step.
+    The device flags it, makes every later step a no-op and eco_run returns
+    ECO_ERR_NOT_CONVERGED, its status counting the (first-order) failing step
+    as taken; the completed steps' times equal the reference's."""
+    import json
+    from pathlib import Path
+    from conftest import initial_state_from_deck
+    from ecogen_amd import EcoError
+    from ecogen_amd.capi import ERR_NOT_CONVERGED
+    import oracle as O
+    d = np.load(Path(__file__).resolve().parent / "golden" / "stop" / f"{name}.npz")
+    case = json.loads(bytes(d["case_json"]).decode())
+    done = int(d["info"][-1, 1])
+    st0 = initial_state_from_deck(case)
+    print(f"case {name}: injection failure expected in step {done + 1}", flush=True)
+    with Context(case) as ctx:
+        ctx.set_state(st0)
+        with pytest.raises(EcoError) as ei:
+            ctx.run(nsteps=case["steps"])
+        assert ei.value.code == ERR_NOT_CONVERGED
+        s = ei.value.status
+        assert s.flags & 4 and s.done and s.steps == done + 1
+    with pytest.raises(RuntimeError):
+        O.run(case, st0)
+    assert O.inflow_failure_step() == done + 1
+    with Context(case) as ctx:
+        ctx.set_state(st0)
+        s = ctx.run(nsteps=done)
+        assert s.steps == done and s.t == d["info"][-1, 2] and s.flags == 0

@@ -39,7 +39,7 @@ def test_fp32s_within_1e5_of_fp64_oracle(name):
     c32 = dict(case, precision="fp32s")
     with Context(c32) as ctx:
         ctx.set_state(st0)
-        status = ctx.run(nsteps=case["steps"])
+        status = ctx.run(nsteps=it)  # the FP64 step count (also for time-controlled cases)
         out = ctx.get_state()
     assert status.steps == it
     assert status.flags & 2 == 0

@@ -160,6 +160,46 @@ def test_relaxation_equilibrium_is_fixed_point():
     assert pp[0] == 3e5 and a[0] == pytest.approx(0.4, rel=1e-15)
 
 
+def test_relaxation_iteration_limit_leaves_the_state():
+    """The first cell of the kapila_2d_o2_relax_limit golden whose relaxation
+    reaches the iteration limit: air at the ideal-gas pressure floor (1e-15 Pa)
+    beside water in tension (-6.5e6 Pa).  Every admissible p* (at or above the
+    air's floor, which verifyAndModifyPressure imposes at each iteration)
+    leaves f(p*) = sum alpha_k rho_k / rho_k*(p*) - 1 below -2e-3 (the root
+    lies below the floor): Newton's step from the floor points below it and
+    is clamped back every time, so the iteration runs its 100 iterations and
+    RelaxationP::stiffRelaxation (RelaxationP.cpp:92-109) leaves alpha_k,
+    rho_k, p_k as verifyAndCorrectPhase left them."""
+    a0 = [float.fromhex("0x1.0f2d664f95d80p-5"), float.fromhex("0x1.ef0d299b06a28p-1")]
+    r0 = [float.fromhex("0x1.692f93cb10a18p-10"), float.fromhex("0x1.ca3a04ee77933p+9")]
+    p0 = [float.fromhex("0x1.203af9ee75616p-50"), float.fromhex("-0x1.8e1083b433a00p+22")]
+    it, a, r, pp, _ = _relax(a0, r0, p0)
+    assert it == 100
+    assert list(a) == a0 and list(r) == r0 and list(pp) == p0
+    eq = [O.make_eos(AIR), O.make_eos(WATER)]
+    for ps in np.logspace(-15, 10, 251):
+        f = -1.0 + sum(a0[k] * r0[k] / L.orc_eos_rho_pfinal(C.byref(eq[k]), p0[k], r0[k], ps, None, 0.0)
+                       for k in range(2))
+        assert f < -2e-3, (ps, f)
+
+
+def test_relaxation_iteration_limit_pinned_by_the_reference():
+    """The kapila_2d_o2_relax_limit golden (one 64^2 case of the batched
+    workload, run by the reference itself) takes the relaxation's no-update
+    branch in 124 cell updates, and the oracle reproduces the reference's
+    final state bit for bit there: every printed field, step count and time."""
+    from conftest import initial_state, load_golden
+    case, d = load_golden("kapila_2d_o2_relax_limit")
+    out, it, t, _, _ = O.run(case, initial_state(case, d))
+    nfail, _ = O.relax_failures()
+    assert nfail == 124
+    assert it == int(d["info"][-1, 1]) == 40 and t == d["info"][-1, 2]
+    fin = O.state_from_golden(case, d, "final")
+    for f in (0, 1, 3, 4, 6, 7):  # alpha_k, rho_k, u, v
+        assert np.array_equal(out[f], fin[f]), f
+    assert np.array_equal(out[0] * out[2] + out[3] * out[5], d["final_p_mix"])
+
+
 # ----------------------------------------------------------------------------- HLLC
 def _hllc(case, wl, wr, dx=1e-3):
     cfg = O.make_cfg(case)
