@@ -18,6 +18,9 @@
 // largest wave speed: wave shuffle, LDS across waves, one 64-bit atomic max
 // per block into a 3-slot ring of the control block (src/Run.cpp:225,270).
 #include <hip/hip_runtime.h>
+
+#include <type_traits>
+
 #include "eco_physics.hpp"
 #include "eco_launch.h"
 
@@ -50,18 +53,21 @@ __device__ __forceinline__ ECO_GLOBAL T *gmem(T *p) {
 template <int DIM> struct Tile;
 template <> struct Tile<1> {
   static constexpr int X = ECO_TILE1_X, Y = 1, Z = 1, W = ECO_WAVES1, LS = ECO_LIMSPEC1;
+  static constexpr int WS = W, WS32 = W;
   static constexpr int C = X * Y * Z, T = ECO_THREADS1 > 0 ? ECO_THREADS1 : C;
   static constexpr int WB = ECO_WAVESB1 > 0 ? ECO_WAVESB1 : W;
   static constexpr int WB32 = ECO_WAVESB32_1 > 0 ? ECO_WAVESB32_1 : WB;
 };
 template <> struct Tile<2> {
   static constexpr int X = ECO_TILE2_X, Y = ECO_TILE2_Y, Z = 1, W = ECO_WAVES2, LS = ECO_LIMSPEC2;
+  static constexpr int WS = ECO_WAVESS2, WS32 = ECO_WAVESS32_2;  // staged kernels
   static constexpr int C = X * Y * Z, T = ECO_THREADS2 > 0 ? ECO_THREADS2 : C;
   static constexpr int WB = ECO_WAVESB2 > 0 ? ECO_WAVESB2 : W;
   static constexpr int WB32 = ECO_WAVESB32_2 > 0 ? ECO_WAVESB32_2 : WB;
 };
 template <> struct Tile<3> {
   static constexpr int X = ECO_TILE3_X, Y = ECO_TILE3_Y, Z = ECO_TILE3_Z, W = ECO_WAVES3, LS = ECO_LIMSPEC3;
+  static constexpr int WS = W, WS32 = W;
   static constexpr int C = X * Y * Z, T = ECO_THREADS3 > 0 ? ECO_THREADS3 : C;
   static constexpr int WB = ECO_WAVESB3 > 0 ? ECO_WAVESB3 : W;
   static constexpr int WB32 = ECO_WAVESB32_3 > 0 ? ECO_WAVESB32_3 : WB;
@@ -707,6 +713,184 @@ __device__ __forceinline__ void stage_dir_3d(const StageArgs &A, const int *o, c
 }
 
 // ---------------------------------------------------------------------------
+// Staged MUSCL face pass of 2-D tiles (one limiter for the non-alpha and one
+// for the alpha variables, no walls, no THINC): the tile's stage state and a
+// two-cell halo are loaded into LDS once, each cell's limited slope is
+// computed once per direction (CellO2::computeLocalSlopes, CellO2.cpp:101-247:
+// on a uniform Cartesian grid the slope of a cell towards either face of a
+// direction is the limiter of the same two face slopes, and the limiters are
+// symmetric in their arguments), and each face extrapolates its two cells
+// from the staged values and slopes.  The face slopes, limiters and
+// extrapolations are those of face_fp64 operation for operation, so the
+// fluxes are the same bit for bit; a cell's zero limited slope may carry
+// the other sign, which adds nothing to a stored value (stored states are
+// never -0: velocities below 1e-8 are stored as +0).
+// ---------------------------------------------------------------------------
+template <int MODEL, int NPH, int DIM>
+struct StagedLds {
+  using Ly = Layout<MODEL, NPH, DIM>;
+  static constexpr int TX = Tile<DIM>::X, TY = Tile<DIM>::Y, NT = Tile<DIM>::T;
+  static constexpr int SX = TX + 4, SY = TY + 4, NS = SX * SY;      // state: tile + two-cell halo
+  static constexpr int NGX = (TX + 2) * TY, NGY = TX * (TY + 2);    // slopes: tile + one-cell halo along dir
+  static constexpr int NG = NGX > NGY ? NGX : NGY;
+  static constexpr int NF0 = (TX + 1) * TY, NF1 = TX * (TY + 1);    // faces of one direction
+  static constexpr int NFM = NF0 > NF1 ? NF0 : NF1;
+  double S[Ly::NVAR][NS];
+  double G[Ly::NVAR][NG];
+  double F[Ly::NFV][NFM];
+  double sRed[NT / 64];
+};
+
+// The stage state of the tile and its halo into LDS (cells beyond the domain
+// read the nearest cell: their slopes and faces are never used).
+template <int MODEL, int NPH, int DIM, typename PW>
+__device__ __forceinline__ void stage_load_state(StagedLds<MODEL, NPH, DIM> &L, PW W, long long nc, const int *o, int nx,
+                                                 int ny, int tid) {
+  using LD = StagedLds<MODEL, NPH, DIM>;
+  using Ly = Layout<MODEL, NPH, DIM>;
+  constexpr int NT = LD::NT, NS = LD::NS, SX = LD::SX;
+  constexpr int NIT = (NS + NT - 1) / NT;
+  long long idx[NIT];
+#pragma unroll
+  for (int it = 0; it < NIT; it++) {
+    const int c = tid + it * NT;
+    const int cc = c < NS ? c : NS - 1;
+    const int gx = min(max(o[0] - 2 + cc % SX, 0), nx - 1);
+    const int gy = min(max(o[1] - 2 + cc / SX, 0), ny - 1);
+    idx[it] = gx + (long long)gy * nx;
+  }
+#pragma unroll
+  for (int v = 0; v < Ly::NVAR; v++) {
+    double val[NIT];
+#pragma unroll
+    for (int it = 0; it < NIT; it++) val[it] = W[(long long)v * nc + idx[it]];
+#pragma unroll
+    for (int it = 0; it < NIT; it++)
+      if (tid + it * NT < NS) L.S[v][tid + it * NT] = val[it];
+  }
+}
+
+// Limited slopes of the cells of direction dir's slope region: for cell j,
+// limiter(s_{j+1/2}, s_{j-1/2}) with s_{k+1/2} = (w_{k+1} - w_k) / dcc_k
+// (CellInterfaceO2::computeSlopes, CellInterfaceO2.cpp:132-152) and a missing
+// face slope (next to a boundary) taken as 0.  LV/LA: compile-time limiters
+// of the non-alpha / alpha variables, or -1 for the runtime choice.
+template <int MODEL, int NPH, int DIM, int dir, int LV, int LA>
+__device__ __forceinline__ void stage_slopes(StagedLds<MODEL, NPH, DIM> &L, const SchemeParams &S, const Geometry &G,
+                                             const int *o, int tid) {
+  using LD = StagedLds<MODEL, NPH, DIM>;
+  using Ly = Layout<MODEL, NPH, DIM>;
+  constexpr int TX = LD::TX, SX = LD::SX, NT = LD::NT;
+  constexpr int NGD = (dir == 0) ? LD::NGX : LD::NGY;
+  constexpr int RW = (dir == 0) ? TX + 2 : TX;  // slope-region row width
+  constexpr int SD = (dir == 0) ? 1 : SX;       // state-stage stride along dir
+  ECO_GLOBAL const double *Gc = gmem(G.dcc[dir]);
+  const int ndir = G.n[dir];
+  for (int c = tid; c < NGD; c += NT) {
+    const int cx = (dir == 0) ? c % RW - 1 : c % RW;
+    const int cy = (dir == 0) ? c / RW : c / RW - 1;
+    const int gi = o[dir] + (dir == 0 ? cx : cy);
+    const bool has_m = (gi >= 1) && (gi <= ndir - 1), has_p = (gi >= 0) && (gi + 1 <= ndir - 1);
+    const double dcm = has_m ? Gc[gi - 1] : 1., dcp = has_p ? Gc[gi] : 1.;
+    const double ydm = 1. / dcm, ydp = 1. / dcp;
+    const int s0 = (cx + 2) + (cy + 2) * SX;
+#pragma unroll
+    for (int v = 0; v < Ly::NVAR; v++) {
+      const double wm = L.S[v][s0 - SD], w = L.S[v][s0], wp = L.S[v][s0 + SD];
+      const double sm = has_m ? qdiv(w - wm, dcm, ydm) : 0.;
+      const double sp = has_p ? qdiv(wp - w, dcp, ydp) : 0.;
+      const bool isa = (MODEL != MODEL_EULER) && (v < 3 * NPH) && (v % 3 == 0);
+      double g;
+      if constexpr (LV >= 0) {
+        g = isa ? limiter_k<LA>(sp, sm) : limiter_k<LV>(sp, sm);
+      } else {
+        g = limiter(isa ? S.lim[2] : S.lim[0], sp, sm);
+      }
+      L.G[v][c] = g;
+    }
+  }
+}
+
+template <int MODEL, int NPH, int DIM, int dir>
+__device__ __forceinline__ void stage_slopes_dispatch(StagedLds<MODEL, NPH, DIM> &L, const SchemeParams &S,
+                                                      const Geometry &G, const int *o, int tid) {
+  const int lv = S.lim[0];
+  const int la = (MODEL == MODEL_EULER) ? lv : S.lim[2];
+  if (lv == la) {
+    switch (lv) {
+      case LIM_MINMOD: stage_slopes<MODEL, NPH, DIM, dir, LIM_MINMOD, LIM_MINMOD>(L, S, G, o, tid); return;
+      case LIM_VANLEER: stage_slopes<MODEL, NPH, DIM, dir, LIM_VANLEER, LIM_VANLEER>(L, S, G, o, tid); return;
+      case LIM_VANALBADA: stage_slopes<MODEL, NPH, DIM, dir, LIM_VANALBADA, LIM_VANALBADA>(L, S, G, o, tid); return;
+      case LIM_MC: stage_slopes<MODEL, NPH, DIM, dir, LIM_MC, LIM_MC>(L, S, G, o, tid); return;
+      case LIM_SUPERBEE: stage_slopes<MODEL, NPH, DIM, dir, LIM_SUPERBEE, LIM_SUPERBEE>(L, S, G, o, tid); return;
+      default: break;
+    }
+  } else if (la == LIM_MINMOD && lv == LIM_VANLEER) {
+    stage_slopes<MODEL, NPH, DIM, dir, LIM_VANLEER, LIM_MINMOD>(L, S, G, o, tid);
+    return;
+  }
+  stage_slopes<MODEL, NPH, DIM, dir, -1, -1>(L, S, G, o, tid);
+}
+
+// The interior faces of direction dir from the staged states and slopes:
+// extrapolation (CellInterfaceO2::solveRiemann, CellInterfaceO2.cpp:197-295),
+// verify/correct, face frame, fulfillState, HLLC; flux to LDS slot f.
+template <int MODEL, int NPH, int DIM, int dir>
+__device__ __forceinline__ double stage_faces(StagedLds<MODEL, NPH, DIM> &L, const SchemeParams &S, const Geometry &G,
+                                              const int *o, int tid) {
+  using LD = StagedLds<MODEL, NPH, DIM>;
+  using Ly = Layout<MODEL, NPH, DIM>;
+  constexpr int TX = LD::TX, SX = LD::SX, NT = LD::NT;
+  constexpr int NFD = (dir == 0) ? LD::NF0 : LD::NF1;
+  constexpr int RW = (dir == 0) ? TX + 1 : TX;  // face rows
+  constexpr int GW = (dir == 0) ? TX + 2 : TX;  // slope-region row width
+  constexpr int SD = (dir == 0) ? 1 : SX, GD = (dir == 0) ? 1 : TX;
+  const int nx = G.n[0], ny = G.n[1], ndir = G.n[dir];
+  ECO_GLOBAL const double *Dl = gmem(G.dfl[dir]);
+  ECO_GLOBAL const double *Dr = gmem(G.dfr[dir]);
+  double smx = 0.;
+  for (int f = tid; f < NFD; f += NT) {
+    const int fx = f % RW, fy = f / RW;
+    const int g0 = o[0] + fx, g1 = o[1] + fy;  // cell on the face's high side
+    if (g0 >= nx + (dir == 0) || g1 >= ny + (dir == 1)) continue;
+    const int gi = (dir == 0) ? g0 : g1;
+    if (gi < 1 || gi > ndir - 1) continue;  // boundary faces are handled per cell
+    const int lx = fx - (dir == 0), ly = fy - (dir == 1);  // tile coordinates of the low-side cell
+    const int sl = (lx + 2) + (ly + 2) * SX, sr = sl + SD;
+    const int gl = (dir == 0) ? (lx + 1) + ly * GW : lx + (ly + 1) * GW, gr = gl + GD;
+    const double dL = Dl[gi - 1], dR = Dr[gi - 1];
+    double xl[Ly::NVAR], xr[Ly::NVAR];
+#pragma unroll
+    for (int v = 0; v < Ly::NVAR; v++) {
+      xl[v] = L.S[v][sl] + L.G[v][gl] * dL;
+      xr[v] = L.S[v][sr] + (-L.G[v][gr]) * dR;
+    }
+    CellState<NPH> Lc, Rc;
+    Lc.u[0] = Lc.u[1] = Lc.u[2] = 0.;
+    Rc.u[0] = Rc.u[1] = Rc.u[2] = 0.;
+    if constexpr (MODEL == MODEL_EULER) { Lc.a[0] = 1.; Rc.a[0] = 1.; }
+    vec_to_prim<MODEL, NPH, DIM>(xl, Lc);
+    vec_to_prim<MODEL, NPH, DIM>(xr, Rc);
+    if constexpr (MODEL == MODEL_EULER) {
+      verify_correct<MODEL, NPH>(S, Lc, 0);
+      verify_correct<MODEL, NPH>(S, Rc, 0);
+    } else {
+#pragma unroll
+      for (int k = 0; k < NPH; k++) { verify_correct<MODEL, NPH>(S, Lc, k); verify_correct<MODEL, NPH>(S, Rc, k); }
+    }
+    to_face_frame(dir, 0, Lc.u);
+    to_face_frame(dir, 0, Rc.u);
+    fulfill<MODEL, NPH>(S, Lc);
+    fulfill<MODEL, NPH>(S, Rc);
+    FluxVec<NPH> F;
+    smx = smax(smx, hllc<MODEL, NPH>(S, Lc, Rc, F));
+    from_face_frame(dir, 0, F.q);
+    put_flux_s<MODEL, NPH, DIM>(&L.F[0][0], LD::NFM, f, F);
+  }
+  return smx;
+}
+
+// ---------------------------------------------------------------------------
 // one tile of a stage
 // STAGE 0: first-order step; 1: MUSCL predictor (writes U*); 2: MUSCL corrector
 // ---------------------------------------------------------------------------
@@ -723,10 +907,16 @@ struct StageLds {
   double sRed[NT / 64];
 };
 
-// Tile `blk` of the case described by A, launch slot `slot`.
-template <int MODEL, int NPH, int DIM, int STAGE, bool SAMELIM, bool WALL = true>
+// LDS of a stage kernel: the staged 2-D MUSCL pass or the direct one.
+template <int MODEL, int NPH, int DIM, bool STAGED>
+using StageLdsT = typename std::conditional<STAGED, StagedLds<MODEL, NPH, DIM>, StageLds<MODEL, NPH, DIM>>::type;
+
+// Tile `blk` of the case described by A, launch slot `slot`.  STAGED: the
+// staged face pass of 2-D MUSCL stages (single limiter form, no walls, no THINC).
+template <int MODEL, int NPH, int DIM, int STAGE, bool SAMELIM, bool WALL = true, bool STAGED = false>
 __device__ __forceinline__ void stage_tile(const StageArgs &A, const int slot, const int blk,
-                                           StageLds<MODEL, NPH, DIM> &lds) {
+                                           StageLdsT<MODEL, NPH, DIM, STAGED> &lds) {
+  static_assert(!STAGED || (DIM == 2 && STAGE != 0 && SAMELIM && !WALL), "staged pass: 2-D MUSCL, one limiter, no walls");
   using Ly = Layout<MODEL, NPH, DIM>;
   using LD = StageLds<MODEL, NPH, DIM>;
   constexpr int TX = Tile<DIM>::X, TY = Tile<DIM>::Y, TZ = Tile<DIM>::Z;
@@ -734,8 +924,6 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int slot, c
   constexpr bool MERGED = LD::MERGED;
   constexpr int NF0 = LD::NF0, NF1 = LD::NF1;
   constexpr int NFT = LD::NFT;
-  auto &sF = lds.sF;
-  auto &sAcc = lds.sAcc;
   auto &sRed = lds.sRed;
 
   const Geometry &G = A.geo;
@@ -772,7 +960,28 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int slot, c
   zero_flux(acc);
   double smx = 0.;
 
-  if constexpr (MERGED) {
+  if constexpr (STAGED) {
+    // staged 2-D pass: state and halo into LDS; per direction the cells'
+    // limited slopes, then the faces, then each cell's two faces
+    using SL = StagedLds<MODEL, NPH, DIM>;
+    stage_load_state<MODEL, NPH, DIM>(lds, Wsrc, nc, o, nx, ny, tid);
+    __syncthreads();
+    stage_slopes_dispatch<MODEL, NPH, DIM, 0>(lds, S, G, o, tid);
+    __syncthreads();
+    smx = smax(smx, stage_faces<MODEL, NPH, DIM, 0>(lds, S, G, o, tid));
+    __syncthreads();
+    CellState<NPH> ownc;
+    if (active) {
+      load_nc_coeffs<MODEL, NPH, DIM>(Wn, nc, cidx, ownc);
+      accumulate_dir<MODEL, NPH, DIM, 0>(acc, &lds.F[0][0], SL::NFM, 0, lx, ly, lz, ci, G, ownc);
+    }
+    stage_slopes_dispatch<MODEL, NPH, DIM, 1>(lds, S, G, o, tid);
+    __syncthreads();
+    smx = smax(smx, stage_faces<MODEL, NPH, DIM, 1>(lds, S, G, o, tid));
+    __syncthreads();
+    if (active) accumulate_dir<MODEL, NPH, DIM, 1>(acc, &lds.F[0][0], SL::NFM, 0, lx, ly, lz, ci, G, ownc);
+  } else if constexpr (MERGED) {
+    auto &sF = lds.sF;
     // all faces of the tile (both directions) in one pass, one barrier
     for (int w = tid; w < NF0 + NF1; w += NT) {
       if (w < NF0) smx = smax(smx, face_fp64<MODEL, NPH, DIM, STAGE, SAMELIM, 0, WALL>(A, o, strd, w, &sF[0][0], NFT, w));
@@ -787,6 +996,8 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int slot, c
       if constexpr (DIM > 1) accumulate_dir<MODEL, NPH, DIM, 1>(acc, &sF[0][0], NFT, NF0, lx, ly, lz, ci, G, own);
     }
   } else {
+    auto &sF = lds.sF;
+    auto &sAcc = lds.sAcc;
     // 3-D: one direction at a time, the accumulator parked in LDS in between
     stage_dir_3d<MODEL, NPH, DIM, STAGE, SAMELIM, 0, NT, WALL>(A, o, strd, tid, active, lx, ly, lz, ci, cidx, &sF[0][0], NFT, sAcc, acc, smx);
     stage_dir_3d<MODEL, NPH, DIM, STAGE, SAMELIM, 1, NT, WALL>(A, o, strd, tid, active, lx, ly, lz, ci, cidx, &sF[0][0], NFT, sAcc, acc, smx);
@@ -891,11 +1102,11 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int slot, c
 // WALL = false: the MUSCL stages of a solve without walls (no wall code in
 // the face pass or the boundary faces; the register allocation of the hot
 // path is the transmissive one)
-template <int MODEL, int NPH, int DIM, int STAGE, bool SAMELIM, bool WALL = true>
-__global__ void __launch_bounds__(Tile<DIM>::T, Tile<DIM>::W)
+template <int MODEL, int NPH, int DIM, int STAGE, bool SAMELIM, bool WALL = true, bool STAGED = false>
+__global__ void __launch_bounds__(Tile<DIM>::T, STAGED ? Tile<DIM>::WS : Tile<DIM>::W)
 k_stage(const StageArgs A) {
-  __shared__ StageLds<MODEL, NPH, DIM> lds;
-  stage_tile<MODEL, NPH, DIM, STAGE, SAMELIM, WALL>(A, A.step, blockIdx.x, lds);
+  __shared__ StageLdsT<MODEL, NPH, DIM, STAGED> lds;
+  stage_tile<MODEL, NPH, DIM, STAGE, SAMELIM, WALL, STAGED>(A, A.step, blockIdx.x, lds);
 }
 
 // Persistent form over a pool: every workgroup walks the class's tile table
@@ -948,6 +1159,19 @@ int launch_t(const StageArgs &A, int stage, hipStream_t st) {
   dim3 grid((unsigned)(tgx * tgy * tgz)), block(NT);
   const bool same = A.sch.same_lim != 0;
   const bool wall = (A.sch.bc_wall | A.sch.bc_out | A.sch.bc_inj | A.sch.bc_tank) != 0;  // kernels with boundary code
+  // 2-D MUSCL with one limiter form, no walls and no THINC: the staged face pass
+  constexpr bool CAN_STAGE = (DIM == 2);
+  const bool staged = CAN_STAGE && same && !wall && !A.sch.thinc && ECO_STAGED2 != 0;
+  if constexpr (CAN_STAGE) {
+    if (staged && stage == 1) {
+      hipLaunchKernelGGL((k_stage<MODEL, NPH, DIM, 1, true, false, true>), grid, block, 0, st, A);
+      return (int)hipGetLastError();
+    }
+    if (staged && stage == 2) {
+      hipLaunchKernelGGL((k_stage<MODEL, NPH, DIM, 2, true, false, true>), grid, block, 0, st, A);
+      return (int)hipGetLastError();
+    }
+  }
   if (stage == 0 && wall) hipLaunchKernelGGL((k_stage<MODEL, NPH, DIM, 0, true>), grid, block, 0, st, A);
   else if (stage == 0) hipLaunchKernelGGL((k_stage<MODEL, NPH, DIM, 0, true, false>), grid, block, 0, st, A);
   else if (stage == 1 && same && wall) hipLaunchKernelGGL((k_stage<MODEL, NPH, DIM, 1, true>), grid, block, 0, st, A);

@@ -327,11 +327,184 @@ __device__ __forceinline__ void stage32_dir_3d(const StageArgs &A, const int *o,
   }
 }
 
+// Staged face pass of 2-D tiles in the FP32-storage form (one limiter form,
+// no THINC): the stored 32-bit words of the tile and a two-cell halo in LDS,
+// one FP32 limited slope per cell and direction, then the faces.  The face
+// slopes, limiters and extrapolation increments are those of face_fp32s
+// operation for operation (the limiters are symmetric in their arguments),
+// so the fluxes are the same bit for bit as the direct pass's.
+template <int MODEL, int NPH, int DIM>
+struct StagedLds32 {
+  using Ly = Layout<MODEL, NPH, DIM>;
+  static constexpr int TX = Tile<DIM>::X, TY = Tile<DIM>::Y, NT = Tile<DIM>::T;
+  static constexpr int SX = TX + 4, SY = TY + 4, NS = SX * SY;
+  static constexpr int NGX = (TX + 2) * TY, NGY = TX * (TY + 2);
+  static constexpr int NG = NGX > NGY ? NGX : NGY;
+  static constexpr int NF0 = (TX + 1) * TY, NF1 = TX * (TY + 1);
+  static constexpr int NFM = NF0 > NF1 ? NF0 : NF1;
+  float S[Ly::NVAR][NS];
+  float G[Ly::NVAR][NG];
+  double F[Ly::NFV][NFM];
+  double sRed[NT / 64];
+};
+
+// decoded FP64 values of one staged cell (the volume-fraction pair decoded,
+// the other words widened)
+template <int MODEL, int NPH, int DIM>
+__device__ __forceinline__ void staged32_vec(const StagedLds32<MODEL, NPH, DIM> &L, int c, double *w) {
+  using Ly = Layout<MODEL, NPH, DIM>;
+#pragma unroll
+  for (int v = 0; v < Ly::NVAR; v++) w[v] = (double)L.S[v][c];
+  decode_alpha(L.S[0][c], L.S[3][c], w[0], w[3]);
+}
+
+template <int MODEL, int NPH, int DIM, typename PW>
+__device__ __forceinline__ void stage32_load_state(StagedLds32<MODEL, NPH, DIM> &L, PW W, long long nc, const int *o,
+                                                   int nx, int ny, int tid) {
+  using LD = StagedLds32<MODEL, NPH, DIM>;
+  using Ly = Layout<MODEL, NPH, DIM>;
+  constexpr int NT = LD::NT, NS = LD::NS, SX = LD::SX;
+  constexpr int NIT = (NS + NT - 1) / NT;
+  long long idx[NIT];
+#pragma unroll
+  for (int it = 0; it < NIT; it++) {
+    const int c = tid + it * NT;
+    const int cc = c < NS ? c : NS - 1;
+    const int gx = min(max(o[0] - 2 + cc % SX, 0), nx - 1);
+    const int gy = min(max(o[1] - 2 + cc / SX, 0), ny - 1);
+    idx[it] = gx + (long long)gy * nx;
+  }
+#pragma unroll
+  for (int v = 0; v < Ly::NVAR; v++) {
+    float val[NIT];
+#pragma unroll
+    for (int it = 0; it < NIT; it++) val[it] = W[(long long)v * nc + idx[it]];
+#pragma unroll
+    for (int it = 0; it < NIT; it++)
+      if (tid + it * NT < NS) L.S[v][tid + it * NT] = val[it];
+  }
+}
+
+template <int MODEL, int NPH, int DIM, int dir, int LV, int LA>
+__device__ __forceinline__ void stage32_slopes(StagedLds32<MODEL, NPH, DIM> &L, const SchemeParams &S,
+                                               const Geometry &G, const int *o, int tid) {
+  using LD = StagedLds32<MODEL, NPH, DIM>;
+  using Ly = Layout<MODEL, NPH, DIM>;
+  constexpr int TX = LD::TX, SX = LD::SX, NT = LD::NT;
+  constexpr int NGD = (dir == 0) ? LD::NGX : LD::NGY;
+  constexpr int RW = (dir == 0) ? TX + 2 : TX;
+  constexpr int SD = (dir == 0) ? 1 : SX;
+  ECO_GLOBAL const double *Gc = gmem(G.dcc[dir]);
+  const int ndir = G.n[dir];
+  for (int c = tid; c < NGD; c += NT) {
+    const int cx = (dir == 0) ? c % RW - 1 : c % RW;
+    const int cy = (dir == 0) ? c / RW : c / RW - 1;
+    const int gi = o[dir] + (dir == 0 ? cx : cy);
+    const bool has_m = (gi >= 1) && (gi <= ndir - 1), has_p = (gi >= 0) && (gi + 1 <= ndir - 1);
+    const float idm = 1.f / (float)(has_m ? Gc[gi - 1] : 1.), idp = 1.f / (float)(has_p ? Gc[gi] : 1.);
+    const int s0 = (cx + 2) + (cy + 2) * SX;
+    double wm[Ly::NVAR], w[Ly::NVAR], wp[Ly::NVAR];
+    staged32_vec<MODEL, NPH, DIM>(L, s0 - SD, wm);
+    staged32_vec<MODEL, NPH, DIM>(L, s0, w);
+    staged32_vec<MODEL, NPH, DIM>(L, s0 + SD, wp);
+#pragma unroll
+    for (int v = 0; v < Ly::NVAR; v++) {
+      const float sm = has_m ? (float)(w[v] - wm[v]) * idm : 0.f;
+      const float sp = has_p ? (float)(wp[v] - w[v]) * idp : 0.f;
+      const bool isa = (v < 3 * NPH) && (v % 3 == 0);
+      float g;
+      if constexpr (LV >= 0) {
+        g = isa ? limiter_kt<LA, float>(sp, sm) : limiter_kt<LV, float>(sp, sm);
+      } else {
+        g = limiter_f(isa ? S.lim[2] : S.lim[0], sp, sm);
+      }
+      L.G[v][c] = g;
+    }
+  }
+}
+
+template <int MODEL, int NPH, int DIM, int dir>
+__device__ __forceinline__ void stage32_slopes_dispatch(StagedLds32<MODEL, NPH, DIM> &L, const SchemeParams &S,
+                                                        const Geometry &G, const int *o, int tid) {
+  const int lv = S.lim[0], la = S.lim[2];
+  if (lv == la) {
+    switch (lv) {
+      case LIM_MINMOD: stage32_slopes<MODEL, NPH, DIM, dir, LIM_MINMOD, LIM_MINMOD>(L, S, G, o, tid); return;
+      case LIM_VANLEER: stage32_slopes<MODEL, NPH, DIM, dir, LIM_VANLEER, LIM_VANLEER>(L, S, G, o, tid); return;
+      case LIM_VANALBADA: stage32_slopes<MODEL, NPH, DIM, dir, LIM_VANALBADA, LIM_VANALBADA>(L, S, G, o, tid); return;
+      case LIM_MC: stage32_slopes<MODEL, NPH, DIM, dir, LIM_MC, LIM_MC>(L, S, G, o, tid); return;
+      case LIM_SUPERBEE: stage32_slopes<MODEL, NPH, DIM, dir, LIM_SUPERBEE, LIM_SUPERBEE>(L, S, G, o, tid); return;
+      default: break;
+    }
+  } else if (la == LIM_MINMOD && lv == LIM_VANLEER) {
+    stage32_slopes<MODEL, NPH, DIM, dir, LIM_VANLEER, LIM_MINMOD>(L, S, G, o, tid);
+    return;
+  }
+  stage32_slopes<MODEL, NPH, DIM, dir, -1, -1>(L, S, G, o, tid);
+}
+
+template <int MODEL, int NPH, int DIM, int dir>
+__device__ __forceinline__ float stage32_faces(StagedLds32<MODEL, NPH, DIM> &L, const SchemeParams &S,
+                                               const Geometry &G, const int *o, int tid) {
+  using LD = StagedLds32<MODEL, NPH, DIM>;
+  using Ly = Layout<MODEL, NPH, DIM>;
+  constexpr int TX = LD::TX, SX = LD::SX, NT = LD::NT;
+  constexpr int NFD = (dir == 0) ? LD::NF0 : LD::NF1;
+  constexpr int RW = (dir == 0) ? TX + 1 : TX;
+  constexpr int GW = (dir == 0) ? TX + 2 : TX;
+  constexpr int SD = (dir == 0) ? 1 : SX, GD = (dir == 0) ? 1 : TX;
+  const int nx = G.n[0], ny = G.n[1], ndir = G.n[dir];
+  ECO_GLOBAL const double *Dl = gmem(G.dfl[dir]);
+  ECO_GLOBAL const double *Dr = gmem(G.dfr[dir]);
+  float smx = 0.f;
+  for (int f = tid; f < NFD; f += NT) {
+    const int fx = f % RW, fy = f / RW;
+    const int g0 = o[0] + fx, g1 = o[1] + fy;
+    if (g0 >= nx + (dir == 0) || g1 >= ny + (dir == 1)) continue;
+    const int gi = (dir == 0) ? g0 : g1;
+    if (gi < 1 || gi > ndir - 1) continue;
+    const int lx = fx - (dir == 0), ly = fy - (dir == 1);
+    const int sl = (lx + 2) + (ly + 2) * SX, sr = sl + SD;
+    const int gl = (dir == 0) ? (lx + 1) + ly * GW : lx + (ly + 1) * GW, gr = gl + GD;
+    const float fL = (float)Dl[gi - 1], fR = (float)Dr[gi - 1];
+    double wl[Ly::NVAR], wr[Ly::NVAR], xl[Ly::NVAR], xr[Ly::NVAR];
+    staged32_vec<MODEL, NPH, DIM>(L, sl, wl);
+    staged32_vec<MODEL, NPH, DIM>(L, sr, wr);
+#pragma unroll
+    for (int v = 0; v < Ly::NVAR; v++) {
+      xl[v] = wl[v] + (double)(L.G[v][gl] * fL);
+      xr[v] = wr[v] + (double)(-L.G[v][gr] * fR);
+    }
+    CellState<NPH> Lc, Rc;
+    vec_to_prim<MODEL, NPH, DIM>(xl, Lc);
+    vec_to_prim<MODEL, NPH, DIM>(xr, Rc);
+#pragma unroll
+    for (int d = DIM; d < 3; d++) { Lc.u[d] = 0.; Rc.u[d] = 0.; }
+#pragma unroll
+    for (int k = 0; k < NPH; k++) { verify_correct<MODEL, NPH>(S, Lc, k); verify_correct<MODEL, NPH>(S, Rc, k); }
+    to_face_frame(dir, 0, Lc.u);
+    to_face_frame(dir, 0, Rc.u);
+    double sL, sR;
+    smx = smaxf_(smx, speeds_f<NPH>(S, Lc, Rc, sL, sR));
+    fulfill_energy<MODEL, NPH>(S, Lc);
+    fulfill_energy<MODEL, NPH>(S, Rc);
+    FluxVec<NPH> F;
+    hllc_mixed<MODEL, NPH>(S, Lc, Rc, sL, sR, F);
+    from_face_frame(dir, 0, F.q);
+    put_flux_s<MODEL, NPH, DIM>(&L.F[0][0], LD::NFM, f, F);
+  }
+  return smx;
+}
+
+template <int MODEL, int NPH, int DIM, bool STAGED>
+using Stage32LdsT = typename std::conditional<STAGED, StagedLds32<MODEL, NPH, DIM>, StageLds<MODEL, NPH, DIM>>::type;
+
 // Tile `blk` of a MUSCL stage in the FP32-storage form (STAGE 1 predictor,
 // 2 corrector), launch slot `slot`.
-template <int MODEL, int NPH, int DIM, int STAGE, bool SAMELIM>
+template <int MODEL, int NPH, int DIM, int STAGE, bool SAMELIM, bool STAGED = false>
 __device__ __forceinline__ void stage32_tile(const StageArgs &A, const int slot, const int blk,
-                                             StageLds<MODEL, NPH, DIM> &lds) {
+                                             Stage32LdsT<MODEL, NPH, DIM, STAGED> &lds) {
+  static_assert(!STAGED || (DIM == 2 && SAMELIM), "staged pass: 2-D, one limiter form");
   using Ly = Layout<MODEL, NPH, DIM>;
   using LD = StageLds<MODEL, NPH, DIM>;
   constexpr int TX = Tile<DIM>::X, TY = Tile<DIM>::Y, TZ = Tile<DIM>::Z;
@@ -339,8 +512,6 @@ __device__ __forceinline__ void stage32_tile(const StageArgs &A, const int slot,
   constexpr bool MERGED = LD::MERGED;
   constexpr int NF0 = LD::NF0, NF1 = LD::NF1;
   constexpr int NFT = LD::NFT;
-  auto &sF = lds.sF;
-  auto &sAcc = lds.sAcc;
   auto &sRed = lds.sRed;
 
   const Geometry &G = A.geo;
@@ -375,7 +546,26 @@ __device__ __forceinline__ void stage32_tile(const StageArgs &A, const int slot,
   zero_flux(acc);
   float smx = 0.f;
 
-  if constexpr (MERGED) {
+  if constexpr (STAGED) {
+    using SL = StagedLds32<MODEL, NPH, DIM>;
+    stage32_load_state<MODEL, NPH, DIM>(lds, Wsrc, nc, o, nx, ny, tid);
+    __syncthreads();
+    stage32_slopes_dispatch<MODEL, NPH, DIM, 0>(lds, S, G, o, tid);
+    __syncthreads();
+    smx = smaxf_(smx, stage32_faces<MODEL, NPH, DIM, 0>(lds, S, G, o, tid));
+    __syncthreads();
+    CellState<NPH> ownc;
+    if (active) {
+      load_nc_coeffs_f<NPH>(Wn, nc, cidx, ownc);
+      accumulate_dir<MODEL, NPH, DIM, 0>(acc, &lds.F[0][0], SL::NFM, 0, lx, ly, lz, ci, G, ownc);
+    }
+    stage32_slopes_dispatch<MODEL, NPH, DIM, 1>(lds, S, G, o, tid);
+    __syncthreads();
+    smx = smaxf_(smx, stage32_faces<MODEL, NPH, DIM, 1>(lds, S, G, o, tid));
+    __syncthreads();
+    if (active) accumulate_dir<MODEL, NPH, DIM, 1>(acc, &lds.F[0][0], SL::NFM, 0, lx, ly, lz, ci, G, ownc);
+  } else if constexpr (MERGED) {
+    auto &sF = lds.sF;
     // all faces of the tile (both directions) in one pass, one barrier
     for (int w = tid; w < NF0 + NF1; w += NT) {
       if (w < NF0) smx = smaxf_(smx, face_fp32s<MODEL, NPH, DIM, SAMELIM, 0>(A, o, strd, w, &sF[0][0], NFT, w));
@@ -390,6 +580,8 @@ __device__ __forceinline__ void stage32_tile(const StageArgs &A, const int slot,
       if constexpr (DIM > 1) accumulate_dir<MODEL, NPH, DIM, 1>(acc, &sF[0][0], NFT, NF0, lx, ly, lz, ci, G, own);
     }
   } else {
+    auto &sF = lds.sF;
+    auto &sAcc = lds.sAcc;
     stage32_dir_3d<MODEL, NPH, DIM, SAMELIM, 0>(A, o, strd, tid, active, lx, ly, lz, ci, cidx, &sF[0][0], NFT, sAcc, acc, smx);
     stage32_dir_3d<MODEL, NPH, DIM, SAMELIM, 1>(A, o, strd, tid, active, lx, ly, lz, ci, cidx, &sF[0][0], NFT, sAcc, acc, smx);
     stage32_dir_3d<MODEL, NPH, DIM, SAMELIM, 2>(A, o, strd, tid, active, lx, ly, lz, ci, cidx, &sF[0][0], NFT, sAcc, acc, smx);
@@ -461,11 +653,11 @@ __device__ __forceinline__ void stage32_tile(const StageArgs &A, const int slot,
   }
 }
 
-template <int MODEL, int NPH, int DIM, int STAGE, bool SAMELIM>
-__global__ void __launch_bounds__(Tile<DIM>::T, Tile<DIM>::W)
+template <int MODEL, int NPH, int DIM, int STAGE, bool SAMELIM, bool STAGED = false>
+__global__ void __launch_bounds__(Tile<DIM>::T, STAGED ? Tile<DIM>::WS32 : Tile<DIM>::W)
 k_stage32(const StageArgs A) {
-  __shared__ StageLds<MODEL, NPH, DIM> lds;
-  stage32_tile<MODEL, NPH, DIM, STAGE, SAMELIM>(A, A.step, blockIdx.x, lds);
+  __shared__ Stage32LdsT<MODEL, NPH, DIM, STAGED> lds;
+  stage32_tile<MODEL, NPH, DIM, STAGE, SAMELIM, STAGED>(A, A.step, blockIdx.x, lds);
 }
 
 // persistent form over a pool (as k_stage_batch)
@@ -535,6 +727,14 @@ int launch_t32(const StageArgs &A, int stage, hipStream_t st, double *tmp) {
   const bool same = A.sch.same_lim != 0;
   const long long nc = A.geo.ncell;
   const dim3 g1((unsigned)((nc + 255) / 256)), b1(256);
+  // 2-D with one limiter form and no THINC: the staged face pass
+  if constexpr (DIM == 2) {
+    if (same && !A.sch.thinc && ECO_STAGED2 != 0 && (stage == 1 || stage == 2)) {
+      if (stage == 1) hipLaunchKernelGGL((k_stage32<MODEL, NPH, DIM, 1, true, true>), grid, block, 0, st, A);
+      else hipLaunchKernelGGL((k_stage32<MODEL, NPH, DIM, 2, true, true>), grid, block, 0, st, A);
+      return (int)hipGetLastError();
+    }
+  }
   switch (stage) {
     case 1:
       if (same) hipLaunchKernelGGL((k_stage32<MODEL, NPH, DIM, 1, true>), grid, block, 0, st, A);

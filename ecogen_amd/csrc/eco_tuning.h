@@ -5,10 +5,10 @@
 #define ECO_TILE1_X 504
 #endif
 #ifndef ECO_TILE2_X
-#define ECO_TILE2_X 32
+#define ECO_TILE2_X 16
 #endif
 #ifndef ECO_TILE2_Y
-#define ECO_TILE2_Y 8
+#define ECO_TILE2_Y 12
 #endif
 #ifndef ECO_TILE3_X
 #define ECO_TILE3_X 16
@@ -28,8 +28,20 @@
 #ifndef ECO_WAVES2
 #define ECO_WAVES2 2
 #endif
+// waves per SIMD requested for the staged 2-D kernels (FP64, FP32 storage)
+#ifndef ECO_WAVESS2
+#define ECO_WAVESS2 3
+#endif
+#ifndef ECO_WAVESS32_2
+#define ECO_WAVESS32_2 3
+#endif
 #ifndef ECO_WAVES3
 #define ECO_WAVES3 2
+#endif
+// staged 2-D MUSCL face pass (tile state and halo in LDS, one limited slope
+// per cell and direction) for single-limiter runs without walls or THINC
+#ifndef ECO_STAGED2
+#define ECO_STAGED2 1
 #endif
 // compile-time (branch-free) limiter for the single-limiter configurations
 #ifndef ECO_LIMSPEC1
