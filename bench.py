@@ -54,7 +54,7 @@ def parse():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain copy kernel of known size (for HBM counter calibration)")
     ap.add_argument("--batch-cases", type=int, default=2048, help="cases of the batched leg (0: skip)")
-    ap.add_argument("--serial-stride", type=int, default=16,
+    ap.add_argument("--serial-stride", type=int, default=32,
                     help="serial plain FP64 comparator: every k-th case of the batch")
     ap.add_argument("--serve-requests", type=int, default=120, help="requests per arrival rate (0: skip)")
     ap.add_argument("--serve-rates", type=float, nargs="+", default=[50.0, 200.0, 800.0])
@@ -246,22 +246,18 @@ def _fp32s_vs_fp64(case, a32, a64):
     return l1_errors(case, a32, a64)
 
 
-def leg_batched(args):
-    """2048 independent water/air Riemann problems (70 % 1-D, 30 % 2-D), Kapila
-    MUSCL van Leer, FP32 storage + FP64 relaxation, all admitted to one pool and
-    advanced by persistent batched kernels replayed as hipGraph chunks; against
-    the same cases solved one after another with the plain FP64 path."""
-    from ecogen_amd import Context
-    from ecogen_amd import workloads as W
-    from ecogen_amd.pool import Pool
-    specs = W.batch_specs(args.batch_cases)
-    cases = [W.riemann_case(*sp, precision="fp32s") for sp in specs]
-    cells = np.array([int(np.prod(c["cells"])) for c, _, _ in cases])
-    sample = [] if args.no_serial else list(range(0, len(cases), max(1, args.serial_stride)))
+def _pool_all(cases, precision, args):
+    """All cases admitted to one pool in ``precision`` ("fp32s", "fp64" or
+    "routed": each case as route_precision says) and run to their t_final.
+    Returns (results, per-case finish times, wall seconds, pool stats)."""
+    from ecogen_amd.pool import Pool, route_precision
     finish = {}
-    with Pool(hbm_budget=48 << 30, steps_per_chunk=24, use_graph=not args.batch_direct) as pool:
+    with Pool(hbm_budget=96 << 30, steps_per_chunk=24, use_graph=not args.batch_direct) as pool:
         t0 = time.perf_counter()
-        ids = [pool.admit(c, st, t_final=tf) for c, st, tf in cases]
+        ids = []
+        for c, st, tf in cases:
+            prec = route_precision(c, st, tf) if precision == "routed" else precision
+            ids.append(pool.admit(dict(c, precision=prec), st, t_final=tf))
         if any(i is None for i in ids):
             raise SystemExit("batched leg: cases exceed the pool budget")
         while pool.stats().running > 0:
@@ -274,48 +270,96 @@ def leg_batched(args):
             finish.setdefault(cid, wall)
         stats = pool.stats()
         res = [pool.result(i) for i in ids]
-    steps = np.array([r[1].steps for r in res])
-    flags = np.array([r[1].flags for r in res])
-    cu = float((cells * steps).sum())
-    lat = [finish[i] for i in ids]
-    # serial plain FP64 solves of the sample (create, upload, run to t_final, download, destroy)
-    ser_t, ser_cu, l1s, l1_skipped = 0.0, 0.0, [], 0
+    return res, [finish[i] for i in ids], wall, stats
+
+
+def leg_batched(args):
+    """2048 independent water/air Riemann problems (70 % 1-D, 30 % 2-D), Kapila
+    MUSCL van Leer, each case run to its own t_final in one pool by persistent
+    batched kernels replayed as hipGraph chunks: all in FP32 storage (FP64
+    relaxation), all in FP64 (the like-for-like comparator), and routed (FP32
+    storage only where it stays within 1e-5 of FP64); the FP32-storage error of
+    every result against its FP64 result; and a sample solved one at a time
+    with the plain FP64 path."""
+    from ecogen_amd import Context
+    from ecogen_amd import workloads as W
+    from ecogen_amd.pool import route_precision, FP32S_MAX_STEPS_1D
+    specs = W.batch_specs(args.batch_cases)
+    cases = [W.riemann_case(*sp, precision="fp32s") for sp in specs]
+    cells = np.array([int(np.prod(c["cells"])) for c, _, _ in cases])
+    routes = [route_precision(c, st, tf) for c, st, tf in cases]
+
+    def summary(res, lat, wall, stats):
+        steps = np.array([r[1].steps for r in res])
+        cu = float((cells * steps).sum())
+        return {"wall_s": round(wall, 3), "solves_per_s": round(len(cases) / wall, 2),
+                "cell_updates_per_s": round(cu / wall, 1), "cell_updates": cu,
+                "latency_s": {"p50": round(_pct(lat, 50), 3), "p99": round(_pct(lat, 99), 3), "max": round(max(lat), 3)},
+                "relax_not_converged_cases": int(sum(1 for r in res if r[1].flags & 1)),
+                "graph_captures": stats.graph_captures, "tiles_launched": stats.tiles_launched}
+
+    r32, lat32, wall32, st32 = _pool_all(cases, "fp32s", args)
+    s32 = summary(r32, lat32, wall32, st32)
+    log(f"batched fp32s: {wall32:.1f} s")
+    r64, lat64, wall64, st64 = _pool_all(cases, "fp64", args)
+    s64 = summary(r64, lat64, wall64, st64)
+    log(f"batched fp64: {wall64:.1f} s")
+    rr, latr, wallr, str_ = _pool_all(cases, "routed", args)
+    sr = summary(rr, latr, wallr, str_)
+    log(f"batched routed: {wallr:.1f} s")
+    # FP32-storage error of every case against its FP64 result (same step count required)
+    l1 = []
+    for k, (c, st, tf) in enumerate(cases):
+        if r32[k][1].steps == r64[k][1].steps:
+            e = _fp32s_vs_fp64(c, r32[k][0], r64[k][0])
+            l1.append((k, max(e.values()), bool((r32[k][1].flags | r64[k][1].flags) & 1)))
+        else:
+            l1.append((k, float("inf"), bool((r32[k][1].flags | r64[k][1].flags) & 1)))
+    worst = np.array([x[1] for x in l1])
+    relax = np.array([x[2] for x in l1])
+    in32 = np.array([rt == "fp32s" for rt in routes])
+    # routed results against FP64: the FP64-routed ones equal it bit for bit
+    routed_fp64_equal = all(np.array_equal(rr[k][0], r64[k][0]) for k in range(len(cases)) if not in32[k])
+    routed_l1 = [float(max(_fp32s_vs_fp64(c, rr[k][0], r64[k][0]).values())) for k, (c, _, _) in enumerate(cases) if in32[k]]
+    # serial plain FP64 solves of a sample (create, upload, run to t_final, download, destroy)
+    sample = [] if args.no_serial else list(range(0, len(cases), max(1, args.serial_stride)))
+    ser_t, ser_cu = 0.0, 0.0
     for k in sample:
         c, st, tf = cases[k]
-        c64 = dict(c, precision="fp64")
         t1 = time.perf_counter()
-        with Context(c64) as ctx:
+        with Context(dict(c, precision="fp64")) as ctx:
             ctx.set_state(st)
-            s64 = ctx.run(t_final=tf)
-            a64 = ctx.get_state()
+            s64k = ctx.run(t_final=tf)
+            ctx.get_state()
         ser_t += time.perf_counter() - t1
-        ser_cu += cells[k] * s64.steps
-        if s64.steps == steps[k] and not (s64.flags & 1) and not (flags[k] & 1):
-            l1s.append(_fp32s_vs_fp64(c, res[k][0], a64))
-        else:
-            l1_skipped += 1
-    ser_solves = len(sample) / ser_t if sample else float("nan")
-    ser_cus = ser_cu / ser_t if sample else float("nan")
-    l1 = None
-    if l1s:
-        l1 = {"median": {q: float(f"{np.median([e[q] for e in l1s]):.3e}") for q in l1s[0]},
-              "worst": {q: float(f"{max(e[q] for e in l1s):.3e}") for q in l1s[0]},
-              "cases_within_1e-5": int(sum(1 for e in l1s if max(e.values()) <= 1e-5))}
+        ser_cu += cells[k] * s64k.steps
+    ser_solves = len(sample) / ser_t if sample else None
+    ser_cus = ser_cu / ser_t if sample else None
     return {
-        "cases": len(cases), "cells": int(cells.sum()), "cell_updates": cu, "wall_s": round(wall, 3),
-        "solves_per_s": round(len(cases) / wall, 2), "cell_updates_per_s": round(cu / wall, 1),
-        "latency_s": {"p50": round(_pct(lat, 50), 3), "p99": round(_pct(lat, 99), 3), "max": round(max(lat), 3)},
-        "pool": {"steps_per_chunk": 24, "graph_captures": stats.graph_captures, "steps_launched": stats.steps_launched,
-                 "tiles_launched": stats.tiles_launched},
-        "relax_not_converged_cases": int((flags & 1).astype(bool).sum()),
+        "cases": len(cases), "cells": int(cells.sum()),
+        "fp32s": s32, "fp64": s64, "routed": sr,
+        "fp32s_speedup_vs_fp64_pool": {"cell_updates": round(s32["cell_updates_per_s"] / s64["cell_updates_per_s"], 3),
+                                       "solves": round(s32["solves_per_s"] / s64["solves_per_s"], 3)},
+        "routed_speedup_vs_fp64_pool": round(wall64 / wallr, 3),
+        "fp32s_l1_vs_fp64_every_case": {
+            "median": float(f"{np.median(worst[np.isfinite(worst)]):.3e}"),
+            "within_1e-5": int((worst <= 1e-5).sum()), "over_1e-5": int((worst > 1e-5).sum()),
+            "over_1e-5_without_relaxation_failures": int(((worst > 1e-5) & ~relax).sum()),
+            "relaxation_failure_cases": int(relax.sum()),
+            "quantity": "max over mixture density, mixture pressure, velocity, volume fractions"},
+        "routing": {"rule": f"FP32 storage for 1-D two-phase MUSCL transmissive runs of at most {FP32S_MAX_STEPS_1D} "
+                            "estimated steps (t_final over the first CFL step); FP64 otherwise",
+                    "fp32s_cases": int(in32.sum()), "fp64_cases": int((~in32).sum()),
+                    "fp32s_l1_max": float(f"{max(routed_l1):.3e}") if routed_l1 else None,
+                    "fp32s_within_1e-5": int(sum(1 for x in routed_l1 if x <= 1e-5)),
+                    "fp64_routed_equal_to_fp64_pool": bool(routed_fp64_equal)},
         "serial_fp64": {"sample": f"every {args.serial_stride}th case ({len(sample)} cases), one solve at a time",
                         "solves_per_s": round(ser_solves, 2) if sample else None,
                         "cell_updates_per_s": round(ser_cus, 1) if sample else None},
-        "speedup_cell_updates_vs_serial_fp64": round((cu / wall) / ser_cus, 2) if sample else None,
-        "speedup_solves_vs_serial_fp64": round((len(cases) / wall) / ser_solves, 2) if sample else None,
-        "fp32s_l1_vs_fp64": l1,
-        "fp32s_l1_cases_compared": len(l1s), "fp32s_l1_cases_skipped": l1_skipped,
-        "storage": "FP32 state (alpha_k, rho_k, p_k, u, reduced energy); FP64 fluxes, update, relaxation",
+        "fp64_pool_speedup_vs_serial_fp64": {"solves": round(s64["solves_per_s"] / ser_solves, 2) if sample else None,
+                                             "cell_updates": round(s64["cell_updates_per_s"] / ser_cus, 2) if sample else None},
+        "storage": "FP32 storage: alpha pair (FP64 in two words), rho_k, p_k, u, reduced energy in FP32; FP64 fluxes, "
+                   "update, relaxation",
     }
 
 
@@ -327,8 +371,12 @@ def leg_served(args):
     from ecogen_amd import Context
     from ecogen_amd import workloads as W
     from ecogen_amd.pool import Pool
+    from ecogen_amd.pool import route_precision
     base = W.request_stream(1.0, args.serve_requests)
     reqcases = [W.request_case(kind, prm, precision="fp32s") for _, kind, prm in base]
+    # each request in the precision the routing rule gives it (FP32 storage
+    # only where it stays within 1e-5 of FP64)
+    reqcases = [(dict(c, precision=route_precision(c, st, tf)), st, tf) for c, st, tf in reqcases]
     cells = np.array([int(np.prod(c["cells"])) for c, _, _ in reqcases])
     # serial plain FP64 service times.  Each solve is capped at four times its
     # CFL step estimate (t_final = 0.3 L / a_max: about 0.3 n / (cfl * lcfl
@@ -347,6 +395,7 @@ def leg_served(args):
         service.append(time.perf_counter() - t1)
         capped += int(s64.t < float(np.float32(tf)))
     out = {"requests_per_rate": len(reqcases), "bubble_3d_requests": sum(1 for _, k, _ in base if k == "bubbles"),
+           "fp32s_requests": sum(1 for c, _, _ in reqcases if c["precision"] == "fp32s"),
            "hbm_budget_gb": 200, "serial_fp64_capped_requests": capped,
            "serial_fp64_step_cap": "4x the CFL step estimate + 200",
            "rates": {}}
@@ -454,7 +503,7 @@ def main():
         if "batched" in legs and args.batch_cases > 0:
             log(f"batched pool: {args.batch_cases} cases")
             extra["batched"] = leg_batched(args)
-            log(f"batched: {extra['batched']['wall_s']} s")
+            log(f"batched: fp32s {extra['batched']['fp32s']['wall_s']} s, fp64 {extra['batched']['fp64']['wall_s']} s")
         if "served" in legs and args.serve_requests > 0:
             log(f"served stream: {args.serve_requests} requests x {len(args.serve_rates)} rates")
             extra["served"] = leg_served(args)

@@ -66,6 +66,15 @@ ECO_D double eos_fix_p(const EosParams &q, double p) {
 // Branch-free forms (every operand computed, the result selected), so that
 // the limited slopes of all variables of a face schedule as one basic block.
 // The selected value is the reference's bit for bit.
+// a / b: IEEE division in FP64 (the reference's limiters); in FP32 (the
+// FP32-storage reconstruction, held to 1e-5 of FP64, not to the bit) a
+// reciprocal and a product
+template <typename T>
+ECO_D T ldiv(T a, T b) {
+  if constexpr (sizeof(T) == sizeof(float)) return __fdividef(a, b);
+  else return a / b;
+}
+
 template <int LIM, typename T>
 ECO_D T limiter_kt(T s1, T s2) {
   auto mn = [](T a, T b) { return (b < a) ? b : a; };
@@ -76,11 +85,11 @@ ECO_D T limiter_kt(T s1, T s2) {
     return (prod > T(1e-9)) ? ((s1 < 0) ? -m : m) : T(0);
   } else if constexpr (LIM == LIM_VANLEER) {
     const T sum = s1 + s2;
-    const T q = T(2) * prod / sum;
+    const T q = ldiv(T(2) * prod, sum);
     return ((fabs(sum) > T(1e-6)) && (prod > T(1e-6))) ? q : T(0);
   } else if constexpr (LIM == LIM_VANALBADA) {
     const T sum = s1 + s2;
-    const T q = prod * sum / (s1 * s1 + s2 * s2);
+    const T q = ldiv(prod * sum, s1 * s1 + s2 * s2);
     return ((fabs(s1) > T(1e-6)) && (fabs(s2) > T(1e-6)) && (fabs(sum) > T(1e-6)) && (prod > T(1e-6))) ? q : T(0);
   } else if constexpr (LIM == LIM_MC) {
     T m = mn(T(2) * fabs(s1), T(2) * fabs(s2));

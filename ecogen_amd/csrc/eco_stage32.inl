@@ -187,8 +187,9 @@ __device__ __forceinline__ float frozen_c_f(const SchemeParams &S, const CellSta
   for (int k = 0; k < NPH; k++) {
     const EosParams &q = S.eos[k];
     const float r = (float)s.r[k], p = (float)s.p[k];
-    ck2[k] = (q.type == EOS_SG) ? (float)q.gamma * (p + (float)q.pinf) / r : (float)q.gamma * p / r;
-    c2 += ((float)s.a[k] * r / rho) * ck2[k];
+    const float ir = __frcp_rn(r);
+    ck2[k] = (q.type == EOS_SG) ? (float)q.gamma * (p + (float)q.pinf) * ir : (float)q.gamma * p * ir;
+    c2 += __fdividef((float)s.a[k] * r, rho) * ck2[k];
   }
   return sqrtf(c2);
 }

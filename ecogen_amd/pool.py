@@ -199,3 +199,50 @@ class Pool:
         _check(lib().eco_serve(self._h, reqs, n, res), "eco_serve")
         arr = np.array([(x.admit_s, x.finish_s, x.steps, x.t, x.flags) for x in res])
         return arr, outs
+
+
+# Precision routing of pooled and served solves.  FP32 storage is held to
+# 1e-5 relative L1 of the FP64 result.  On the batched water/air workload
+# (2048 cases, every result compared with its FP64 run) its error grows with
+# the step count -- median 2e-7 below 200 steps, 8e-6 at 2000-5000 steps,
+# 3e-5 beyond -- and 2-D columns whose expansion puts the water in tension
+# (the relaxation then stops at its iteration limit in both precisions) end
+# up to 0.8 apart.  Only 1-D solves of at most FP32S_MAX_STEPS_1D estimated
+# steps stayed within 1e-5 in every case there; everything else runs in FP64.
+FP32S_MAX_STEPS_1D = 100
+
+
+def estimated_steps(case, state, t_final):
+    """t_final over the first CFL step cfl * lcfl / a_max of the initial state
+    (a_max: the largest |u| + frozen sound speed of a cell; lcfl: the
+    smallest cell size, x 0.6 in multi-D, as the step uses)."""
+    if not t_final or t_final <= 0:
+        return float(case.get("steps", 0))
+    st = np.asarray(state, dtype=np.float64)
+    eos = case["eos"]
+    nph = len(eos)
+    if case["model"] == "euler":
+        rho = st[1]
+        c2 = eos[0]["gamma"] * (st[2] + eos[0].get("pinf", 0.0)) / rho
+        u = np.abs(st[3:6]).max(axis=0)
+    else:
+        rho = sum(st[3 * k] * st[3 * k + 1] for k in range(nph))
+        c2 = sum(st[3 * k] * st[3 * k + 1] / rho * eos[k]["gamma"] * (st[3 * k + 2] + eos[k].get("pinf", 0.0))
+                 / st[3 * k + 1] for k in range(nph))
+        u = np.abs(st[3 * nph:3 * nph + 3]).max(axis=0)
+    a_max = float((u + np.sqrt(np.maximum(c2, 0.0))).max())
+    dims = [d for d in range(3) if case["cells"][d] > 1]
+    lcfl = min(case["lengths"][d] / case["cells"][d] for d in dims) * (0.6 if len(dims) > 1 else 1.0)
+    return t_final / (case["cfl"] * lcfl / a_max) if a_max > 0 else float("inf")
+
+
+def route_precision(case, state, t_final=0.0):
+    """"fp32s" for a solve FP32 storage keeps within 1e-5 of FP64 (a 1-D
+    two-phase MUSCL transmissive run of at most FP32S_MAX_STEPS_1D estimated
+    steps), else "fp64"."""
+    dim = sum(1 for n in case["cells"] if n > 1)
+    fp32_form = (case["model"] != "euler" and case["order"] == 2 and len(case["eos"]) == 2
+                 and set(case.get("bc", ["abs"])) <= {"abs"})
+    if fp32_form and dim == 1 and estimated_steps(case, state, t_final) <= FP32S_MAX_STEPS_1D:
+        return "fp32s"
+    return "fp64"
