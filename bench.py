@@ -141,6 +141,8 @@ def cpu_baseline(args):
     (OpenMP).  The reference is timed at two step counts so that start-up,
     mesh construction and output cancel: rate = cells x (s2 - s1) / (T2 - T1)."""
     n = args.cpu_n
+    # one GPU's share of the host is 16 cores (the GPU host gives each GPU's
+    # job 16); the host's total count is reported beside it
     cores = min(16, os.cpu_count() or 1)
     have_ref = REF_BIN.exists() and MPIEXEC.exists()
     if args.cpu_kind == "reference" or (args.cpu_kind == "auto" and have_ref):
@@ -149,7 +151,7 @@ def cpu_baseline(args):
         t1 = _run_reference(n_ref, s1, cores)
         t2 = _run_reference(n_ref, s2, cores)
         return {"value": n_ref * n_ref * (s2 - s1) / (t2 - t1), "unit": "cell-updates/s", "cores": cores,
-                "kind": "reference",
+                "host_cores": os.cpu_count(), "kind": "reference",
                 "sample": f"headline workload at {n_ref}x{n_ref} cells (cylinder radii scaled), reference ECOGEN "
                           f"built from source, mpiexec -n {cores} (MPICH 3.3.2); {s2} minus {s1} steps, wall "
                           f"{t2:.1f} s - {t1:.1f} s"}
@@ -161,7 +163,8 @@ def cpu_baseline(args):
     t0 = time.perf_counter()
     _, it, _, _, _ = O.run(case, st, nsteps=args.cpu_steps, nthreads=cores)
     dt = time.perf_counter() - t0
-    return {"value": n * n * it / dt, "unit": "cell-updates/s", "cores": cores, "kind": "port",
+    return {"value": n * n * it / dt, "unit": "cell-updates/s", "cores": cores, "host_cores": os.cpu_count(),
+            "kind": "port",
             "sample": f"same workload scaled to {n}x{n} cells (cylinder radii scaled), {it} steps, FP64 oracle "
                       f"restatement of the reference step, OpenMP {cores} threads, {dt:.1f} s"}
 
@@ -177,6 +180,22 @@ def traffic_from_profiles(n, precision, kernel):
         return json.loads(p.read_text())["kernels"][precision][kernel]["bytes_per_launch"]
     except (KeyError, ValueError):
         return None
+
+
+def valu_from_profiles(n, precision, kernel, kernel_ms):
+    """VALU instructions per launch of a stage kernel from the committed
+    rocprofv3 SQ_INSTS_VALU pass, and the fraction of the VALU issue capacity
+    they fill in the live kernel time (an instruction of a 64-wide wave takes
+    4 cycles of a 16-lane SIMD; 1024 SIMDs at 2.4 GHz), or None."""
+    p = ROOT / "profiles" / f"valu_kapila2d_{n}.json"
+    if not p.exists():
+        return None
+    try:
+        insts = json.loads(p.read_text())["kernels"][f"{precision} {kernel}"]["valu_per_launch"]
+    except (KeyError, ValueError):
+        return None
+    return {"insts_per_launch": int(insts), "issue_frac": round(insts * 4 / (1024 * 2.4e9 * kernel_ms * 1e-3), 4),
+            "source": p.name}
 
 
 def run_precision(case, st, args, precision):
@@ -481,11 +500,13 @@ def main():
     log(f"fp64: {r64['wall'] / args.steps * 1e3:.3f} ms/step")
     roof = roofline(case, r64, 8)
     roof["traffic"] = traffic_from_profiles(args.n, "fp64", roof["kernel"])
+    roof["valu"] = valu_from_profiles(args.n, "fp64", roof["kernel"], roof["kernel_ms"])
     fp32s = None
     if not args.no_fp32s:
         r32 = run_precision(case, st, args, "fp32s")
         roof32 = roofline(case, r32, 4)
         roof32["traffic"] = traffic_from_profiles(args.n, "fp32s", roof32["kernel"])
+        roof32["valu"] = valu_from_profiles(args.n, "fp32s", roof32["kernel"], roof32["kernel_ms"])
         fp32s = {"value": round(r32["value"], 1), "speedup_vs_fp64": round(r32["value"] / r64["value"], 3),
                  "ms_per_step": round(r32["wall"] / args.steps * 1e3, 4),
                  "l1_vs_fp64": {k: float(f"{v:.3e}") for k, v in l1_errors(case, r32["state"], r64["state"]).items()},
