@@ -219,9 +219,6 @@ int eco::validate_config(const eco_config *c) {
   int dim = (c->n[0] > 1) + (c->n[1] > 1) + (c->n[2] > 1);
   for (int d = 0; d < dim; d++) if (c->n[d] < 2) return ECO_ERR_UNSUPPORTED;
   if (dim == 0) return ECO_ERR_UNSUPPORTED;
-  // MultiP first order with relaxation reads per-cell cached phase energies and
-  // sound speeds from before the relaxation; the device state does not keep them.
-  if (c->model == ECO_MODEL_MULTIP && c->order == 1 && c->relax_p) return ECO_ERR_UNSUPPORTED;
   if (c->precision != ECO_FP64 && c->precision != ECO_FP32S) return ECO_ERR_INVALID;
   // FP32 storage: MUSCL multiphase path (the batched/served configurations)
   if (c->precision == ECO_FP32S && (c->order != 2 || c->model == ECO_MODEL_EULER)) return ECO_ERR_UNSUPPORTED;
@@ -297,8 +294,8 @@ void eco::lower_config(const eco_config &c, Lowered &L, bool with_tables) {
   for (int b = 0; b < 6; b++) if (c.bc[b] == ECO_BC_INJECTION) lower_injection(c, b, S);
   for (int b = 0; b < 6; b++) if (c.bc[b] == ECO_BC_TANK) lower_tank(c, b, S);
   L.nprim = (c.model == ECO_MODEL_EULER) ? 2 + dim : 3 * c.nphase + dim;
-  S.rhoc = (c.model == ECO_MODEL_MULTIP && c.relax_p && c.precision == ECO_FP64) ? 1 : 0;
-  L.nlevel = L.nprim + 1 + S.rhoc;
+  S.ncache = (c.model == ECO_MODEL_MULTIP && c.relax_p && c.precision == ECO_FP64) ? (c.order == 1 ? c.nphase + 3 : 1) : 0;
+  L.nlevel = L.nprim + 1 + S.ncache;
   L.es = (c.precision == ECO_FP32S) ? sizeof(float) : sizeof(double);
 }
 

@@ -16,7 +16,7 @@ struct Lowered {
   std::vector<double> tab;  // per-direction distance tables (dcc, dfl, dfr)
   size_t off[3];            // offset of direction d's tables in tab
   int nprim;                // stored primitive fields on the device
-  int nlevel;               // fields per time level: nprim, E, and the cached mixture density if sch.rhoc
+  int nlevel;               // fields per time level: nprim, E, and sch.ncache cached values
   size_t es;                // bytes per stored value (8 FP64, 4 FP32 storage)
 };
 

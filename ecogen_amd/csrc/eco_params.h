@@ -50,13 +50,17 @@ struct SchemeParams {
   int alpha_null;
   double eps;        // 1e-15 when alphaNull, else 0 (src/Eos/Eos.cpp:114-122)
   int relax;         // stiff pressure relaxation applied
-  // 1: each time level stores, after E, the mixture density of buildPrim
-  // before the relaxation (MultiP with relaxation, FP64).  The reference
-  // keeps that value cached in the cell (FluxMultiP::buildPrim sets it; the
-  // relaxation changes alpha_k, rho_k but not the mixture density) and the
-  // next step's buildCons multiplies velocity and energy by it
-  // (FluxMultiP.cpp:122-140), so it cannot be recomputed from alpha_k rho_k.
-  int rhoc;
+  // Values of the reference's cell cache a time level stores after E (MultiP
+  // with relaxation, FP64): FluxMultiP::buildPrim sets them, the relaxation
+  // changes alpha_k, rho_k, p_k but leaves them (RelaxationP.cpp:49-111), so
+  // they cannot be recomputed from the stored state.  0: none; 1 (MUSCL):
+  // the mixture density, by which the next step's buildCons multiplies
+  // velocity and energy (FluxMultiP.cpp:122-140); NPH + 3 (first order,
+  // whose faces take the cells' cached states, CellInterface.cpp:127-146):
+  // the mixture density, the phase energies e_k, the frozen sound speed and
+  // the mixture pressure (the relaxed p*, or sum alpha_k p_k when the
+  // relaxation did not converge).
+  int ncache;
   double cfl;
   int prec;          // PREC_FP64 or PREC_FP32S (storage)
   int bc_wall;       // bit 2*dir+side set: wall boundary (BoundCondWall), else transmissive

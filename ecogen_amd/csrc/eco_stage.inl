@@ -295,6 +295,29 @@ __device__ __forceinline__ void unpark_acc(double (*A)[NT], int t, FluxVec<NPH> 
   U.E = A[q++][t];
 }
 
+// The cell-cache values of a MultiP-with-relaxation state (SchemeParams::
+// ncache): after E the mixture density, then at first order the phase
+// energies, the frozen sound speed and the mixture pressure.
+template <int MODEL, int NPH, typename PW>
+__device__ __forceinline__ void store_cache(PW W, long long nc, int ie, long long idx, int ncache,
+                                            const CellState<NPH> &s) {
+  W[(ie + 1) * nc + idx] = s.rho;
+  if (ncache > 1) {
+#pragma unroll
+    for (int k = 0; k < NPH; k++) W[(ie + 2 + k) * nc + idx] = s.e[k];
+    W[(ie + 2 + NPH) * nc + idx] = s.cF;
+    W[(ie + 3 + NPH) * nc + idx] = s.pm;
+  }
+}
+template <int MODEL, int NPH, typename PW>
+__device__ __forceinline__ void load_cache(PW W, long long nc, int ie, long long idx, CellState<NPH> &s) {
+  s.rho = W[(ie + 1) * nc + idx];
+#pragma unroll
+  for (int k = 0; k < NPH; k++) s.e[k] = W[(ie + 2 + k) * nc + idx];
+  s.cF = W[(ie + 2 + NPH) * nc + idx];
+  s.pm = W[(ie + 3 + NPH) * nc + idx];
+}
+
 // time-n alpha_k, p_k of one cell: the non-conservative terms' coefficients
 template <int MODEL, int NPH, int DIM, typename PW>
 __device__ __forceinline__ void load_nc_coeffs(PW W, long long nc, long long idx, CellState<NPH> &s) {
@@ -606,12 +629,22 @@ __device__ __forceinline__ double face_fp64(const StageArgs &A, const int *o, co
         L.c[0] = eos_c(S.eos[0], L.r[0], L.p[0], S.eps);
         R.c[0] = eos_c(S.eos[0], R.r[0], R.p[0], S.eps);
       } else {
-        fulfill<MODEL, NPH>(S, L);
-        fulfill<MODEL, NPH>(S, R);
         if constexpr (MODEL == MODEL_MULTIP) {
+          if (S.ncache > 1) {
+            // after a relaxation the cells' cached energies, sound speed,
+            // density and pressure are those of buildPrim (and p*)
+            load_cache<MODEL, NPH>(Wsrc, nc, Ly::IE, ilo, L);
+            load_cache<MODEL, NPH>(Wsrc, nc, Ly::IE, ihi, R);
+          } else {
+            fulfill<MODEL, NPH>(S, L);
+            fulfill<MODEL, NPH>(S, R);
+          }
           // MultiP keeps the mixture energy of totalEnergyToInternalEnergy (Run.cpp:471)
           L.em = Wsrc[Ly::IE * nc + ilo] - 0.5 * sqn(L.u);
           R.em = Wsrc[Ly::IE * nc + ihi] - 0.5 * sqn(R.u);
+        } else {
+          fulfill<MODEL, NPH>(S, L);
+          fulfill<MODEL, NPH>(S, R);
         }
       }
     } else {
@@ -1057,7 +1090,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int slot, c
 #pragma unroll
       for (int k = 0; k < NPH; k++) sn.rho += sn.a[k] * sn.r[k];
       // MultiP with relaxation: the mixture density cached before the relaxation
-      if (MODEL == MODEL_MULTIP && S.rhoc) sn.rho = Wn[(Ly::IE + 1) * nc + cidx];
+      if (MODEL == MODEL_MULTIP && S.ncache) sn.rho = Wn[(Ly::IE + 1) * nc + cidx];
     }
     FluxVec<NPH> buf;
     build_cons<MODEL, NPH>(S, sn, buf);
@@ -1083,7 +1116,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int slot, c
     if constexpr (STAGE != 1) {
       Wout[Ly::IE * nc + cidx] = s1.E;
       // buildPrim's mixture density (the relaxation leaves it unchanged)
-      if (MODEL == MODEL_MULTIP && S.rhoc) Wout[(Ly::IE + 1) * nc + cidx] = s1.rho;
+      if (MODEL == MODEL_MULTIP && S.ncache) store_cache<MODEL, NPH>(Wout, nc, Ly::IE, cidx, S.ncache, s1);
     }
   }
 
@@ -1144,7 +1177,7 @@ __global__ void k_init(const StageArgs A) {
   fulfill<MODEL, NPH>(A.sch, s);
   if constexpr (MODEL != MODEL_EULER) s.E = s.em + 0.5 * sqn(s.u);
   A.wout[Ly::IE * nc + i] = s.E;
-  if (MODEL == MODEL_MULTIP && A.sch.rhoc) A.wout[(Ly::IE + 1) * nc + i] = s.rho;
+  if (MODEL == MODEL_MULTIP && A.sch.ncache) store_cache<MODEL, NPH>(A.wout, nc, Ly::IE, i, A.sch.ncache, s);
 }
 
 // ---------------------------------------------------------------------------

@@ -268,6 +268,11 @@ CASES = {
     # Pressure-disequilibrium model: first order without relaxation (reference deck), MUSCL with relaxation.
     "multip_1d_fo": _shock_tube_1d("multip", 1, ["none"] * 4, 200, cells=400),
     "multip_1d_o2_relax": _shock_tube_1d("multip", 2, ["minmod"] * 4, 60, relax=True),
+    # first order with relaxation: the faces take the cells' cached states, whose
+    # phase energies, sound speed, density and pressure are those before the relaxation
+    "multip_1d_fo_relax": _shock_tube_1d("multip", 1, ["none"] * 4, 200, cells=400, relax=True),
+    "multip_2d_fo_relax": dict(_shock_cylinders_2d(n=40, steps=60), model="multip", relax_p=True, order=1,
+                               limiters=["none"] * 4),
     "multip_1d_o2_norelax": _shock_tube_1d("multip", 2, ["vanleer"] * 4, 60, relax=False),
     # Single-phase Euler.
     "euler_1d_fo": _euler_1d(1, ["none"] * 4, 100),
