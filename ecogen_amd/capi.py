@@ -22,6 +22,10 @@ PRECISION = {"fp64": 0, "fp32s": 1}
 # domain, "alpha", "rho", "p": the injected fluid per phase}; tanks in
 # case["bc_tank"][side] = {"p0", "T0", and per phase "alpha" or "massFrac"}
 BC = {"abs": 0, "wall": 1, "outflow": 2, "injection": 3, "tank": 4}
+# transported scalars: case["transports"] = their names, each state's "tr" =
+# values in that order, case["bc_tr"][side] = the values entering through an
+# outflow, injection or tank side; stored after E (ECO_NFIELDS_TR)
+MAX_TRANSPORTS = 4
 
 WARN_RELAX_NOT_CONVERGED = 1
 WARN_NAN = 2
@@ -49,7 +53,8 @@ class EcoConfig(C.Structure):
                 ("bc_inj_alpha", (C.c_double * 4) * 6), ("bc_inj_rho", (C.c_double * 4) * 6),
                 ("bc_inj_p", (C.c_double * 4) * 6), ("bc_tank_p0", C.c_double * 6),
                 ("bc_tank_T0", C.c_double * 6), ("bc_tank_frac", (C.c_double * 4) * 6),
-                ("bc_tank_massfrac", C.c_int * 6)]
+                ("bc_tank_massfrac", C.c_int * 6), ("ntransport", C.c_int),
+                ("bc_tr", (C.c_double * MAX_TRANSPORTS) * 6)]
 
 
 class EcoStatus(C.Structure):
@@ -159,11 +164,16 @@ def make_config(case) -> EcoConfig:
                 c.bc_tank_frac[b][k] = v
     for k, e in enumerate(case["eos"]):
         c.eos[k] = EcoEos(EOS_TYPE[e["type"]], e["gamma"], e.get("pinf", 0.0), e["cv"], e.get("eref", 0.0))
+    c.ntransport = len(case.get("transports", []))
+    for b, vals in enumerate(case.get("bc_tr") or [None] * 6):
+        for t, v in enumerate(vals or []):
+            c.bc_tr[b][t] = v
     return c
 
 
 def nfields(case) -> int:
-    return 3 * len(case["eos"]) + 4
+    """Fields of a host state: per phase alpha, rho, p; u, v, w; E; the transports."""
+    return 3 * len(case["eos"]) + 4 + len(case.get("transports", []))
 
 
 class Context:

@@ -27,8 +27,10 @@ struct eco_ctx {
   int own_stream;
   int nprim;            // device primitive fields
   int nlevel;           // fields per time level (nprim, E, cached mixture density)
-  double *buf[2];       // time levels, nlevel fields each
-  double *bstar;        // U*, nprim fields
+  int ntr;              // transported scalars: fields nlevel.. of a time level, nprim.. of U*
+  double *buf[2];       // time levels, nlevel + ntr fields each
+  double *bstar;        // U*, nprim + ntr fields
+  double *smf[3];       // with transports: the faces' sM of the last stage, per direction
   double *tmp64;        // FP64 staging of set/get_state in the FP32-storage form
   size_t es;            // bytes per stored value (8 or 4)
   double *tables;       // geometry tables
@@ -220,6 +222,12 @@ int eco::validate_config(const eco_config *c) {
   for (int d = 0; d < dim; d++) if (c->n[d] < 2) return ECO_ERR_UNSUPPORTED;
   if (dim == 0) return ECO_ERR_UNSUPPORTED;
   if (c->precision != ECO_FP64 && c->precision != ECO_FP32S) return ECO_ERR_INVALID;
+  // transports: the reference's Euler model has no transport Riemann solver
+  // (Model::solveRiemannTransportIntern ends the run, src/Models/Model.h:127);
+  // the FP32-storage kernels carry none
+  if (c->ntransport < 0 || c->ntransport > ECO_MAX_TRANSPORTS) return ECO_ERR_INVALID;
+  if (c->ntransport > 0 && c->model == ECO_MODEL_EULER) return ECO_ERR_INVALID;
+  if (c->ntransport > 0 && c->precision == ECO_FP32S) return ECO_ERR_UNSUPPORTED;
   // FP32 storage: MUSCL multiphase path (the batched/served configurations)
   if (c->precision == ECO_FP32S && (c->order != 2 || c->model == ECO_MODEL_EULER)) return ECO_ERR_UNSUPPORTED;
   return ECO_OK;
@@ -296,6 +304,7 @@ void eco::lower_config(const eco_config &c, Lowered &L, bool with_tables) {
   L.nprim = (c.model == ECO_MODEL_EULER) ? 2 + dim : 3 * c.nphase + dim;
   S.ncache = (c.model == ECO_MODEL_MULTIP && c.relax_p && c.precision == ECO_FP64) ? (c.order == 1 ? c.nphase + 3 : 1) : 0;
   L.nlevel = L.nprim + 1 + S.ncache;
+  L.ntr = c.ntransport;
   L.es = (c.precision == ECO_FP32S) ? sizeof(float) : sizeof(double);
 }
 
@@ -335,6 +344,7 @@ extern "C" int eco_create(const eco_config *cfg, void *hip_stream, eco_ctx **out
   x->sch = L.sch;
   x->nprim = L.nprim;
   x->nlevel = L.nlevel;
+  x->ntr = L.ntr;
 
   // stream
   if (hip_stream) { x->stream = (hipStream_t)hip_stream; x->own_stream = 0; }
@@ -345,11 +355,18 @@ extern "C" int eco_create(const eco_config *cfg, void *hip_stream, eco_ctx **out
   const size_t ntab = L.tab.size();
   hipError_t e = hipSuccess;
   x->buf[0] = x->buf[1] = x->bstar = x->tables = x->tmp64 = nullptr;
+  x->smf[0] = x->smf[1] = x->smf[2] = nullptr;
   x->ctl = nullptr;
   x->es = L.es;
-  e = hipMalloc(&x->buf[0], nc * x->nlevel * x->es);
-  if (e == hipSuccess) e = hipMalloc(&x->buf[1], nc * x->nlevel * x->es);
-  if (e == hipSuccess && c.order == 2) e = hipMalloc(&x->bstar, nc * x->nprim * x->es);
+  const size_t nlv = (size_t)x->nlevel + x->ntr;
+  e = hipMalloc(&x->buf[0], nc * nlv * x->es);
+  if (e == hipSuccess) e = hipMalloc(&x->buf[1], nc * nlv * x->es);
+  if (e == hipSuccess && c.order == 2) e = hipMalloc(&x->bstar, nc * (x->nprim + x->ntr) * x->es);
+  for (int d = 0; d < x->geo.dim && x->ntr > 0 && e == hipSuccess; d++) {
+    // faces of direction d: (n_d + 1) per line of cells
+    const size_t nf = nc / (size_t)c.n[d] * (size_t)(c.n[d] + 1);
+    e = hipMalloc(&x->smf[d], nf * sizeof(double));
+  }
   if (e == hipSuccess && c.precision == ECO_FP32S) e = hipMalloc(&x->tmp64, nc * (x->nprim + 1) * sizeof(double));
   if (e == hipSuccess) e = hipMalloc(&x->tables, ntab * sizeof(double));
   if (e == hipSuccess) e = hipMalloc(&x->ctl, sizeof(Control));
@@ -381,7 +398,48 @@ static StageArgs make_args(eco_ctx *x, int step) {
   A.wn = x->buf[step & 1];
   A.wsrc = A.wn;
   A.wout = x->buf[(step + 1) & 1];
+  for (int d = 0; d < 3; d++) A.smf[d] = x->smf[d];
   return A;
+}
+
+// The transport pass of a stage whose flow kernel ran with A (stage 0: first
+// order, 1: predictor into U*, 2: corrector from U*).
+static int launch_transport_stage(eco_ctx *x, const StageArgs &A, int stage) {
+  const size_t nc = (size_t)x->geo.ncell;
+  TransportArgs T;
+  std::memset(&T, 0, sizeof(T));
+  T.geo = x->geo;
+  T.cfl = x->cfg.cfl;
+  T.ntr = x->ntr;
+  T.order = x->cfg.order;
+  T.stage = stage;
+  T.lim_g = x->cfg.lim[2];
+  T.lim_i = x->cfg.lim[3];
+  for (int b = 0; b < 6; b++) {
+    const int k = x->cfg.bc[b];
+    if (k == ECO_BC_WALL) T.bc_wall |= 1 << b;
+    if (k == ECO_BC_OUTFLOW || k == ECO_BC_INJECTION || k == ECO_BC_TANK) T.bc_in |= 1 << b;
+    for (int t = 0; t < x->ntr; t++) T.bc_tr[b][t] = x->cfg.bc_tr[b][t];
+  }
+  // phase-0 alpha is field 0 of a time level and of U*
+  T.asrc = A.wsrc;
+  T.an = A.wn;
+  const size_t itr = (size_t)x->nlevel * nc;    // scalars in a time level
+  const size_t istar = (size_t)x->nprim * nc;   // scalars in U*
+  T.trn = A.wn + itr;
+  T.trs = (stage == 2) ? A.wsrc + istar : A.wn + itr;
+  T.tro = (stage == 1) ? A.wout + istar : A.wout + itr;
+  for (int d = 0; d < 3; d++) T.sm[d] = x->smf[d];
+  T.ctl = x->ctl;
+  T.step = A.step;
+  return launch_transport(T, x->stream);
+}
+
+// One stage's flow kernel and, with transports, its transport pass.
+static int launch_stage_tr(eco_ctx *x, const StageArgs &A, int stage) {
+  int rc = launch_stage(A, stage, x->stream);
+  if (!rc && x->ntr > 0) rc = launch_transport_stage(x, A, stage);
+  return rc;
 }
 
 static void host_to_dev_fields(const eco_ctx *x, std::vector<int> &map) { host_field_map(x->cfg, x->geo.dim, map); }
@@ -407,6 +465,18 @@ static int finish_set_state(eco_ctx *x) {
   return hipcode(hipStreamSynchronize(x->stream));
 }
 
+// host (or device) fields of the transported scalars -> time level 0 (FP64 only)
+static int copy_transports_in(eco_ctx *x, const double *state, hipMemcpyKind kind) {
+  const size_t nc = (size_t)x->geo.ncell;
+  const size_t hf = ECO_NFIELDS(x->cfg.nphase);
+  for (int t = 0; t < x->ntr; t++) {
+    hipError_t e = hipMemcpyAsync(x->buf[0] + ((size_t)x->nlevel + t) * nc, state + (hf + t) * nc, nc * sizeof(double),
+                                  kind, x->stream);
+    if (e != hipSuccess) return ECO_ERR_HIP;
+  }
+  return ECO_OK;
+}
+
 extern "C" int eco_set_state(eco_ctx *x, const double *host_state) {
   if (!x || !host_state) return ECO_ERR_INVALID;
   std::vector<int> map;
@@ -417,6 +487,8 @@ extern "C" int eco_set_state(eco_ctx *x, const double *host_state) {
                                   hipMemcpyHostToDevice, x->stream);
     if (e != hipSuccess) return ECO_ERR_HIP;
   }
+  int rc = copy_transports_in(x, host_state, hipMemcpyHostToDevice);
+  if (rc) return rc;
   return finish_set_state(x);
 }
 
@@ -440,6 +512,8 @@ extern "C" int eco_set_state_device(eco_ctx *x, const double *dev_state) {
                                   hipMemcpyDeviceToDevice, x->stream);
     if (e != hipSuccess) return ECO_ERR_HIP;
   }
+  int rc = copy_transports_in(x, dev_state, hipMemcpyDeviceToDevice);
+  if (rc) return rc;
   return finish_set_state(x);
 }
 
@@ -491,12 +565,15 @@ extern "C" int eco_get_state(eco_ctx *x, double *host_state) {
   std::vector<int> map;
   host_to_dev_fields(x, map);
   int nf = ECO_NFIELDS(c.nphase);
-  std::memset(host_state, 0, sizeof(double) * nf * nc);
+  std::memset(host_state, 0, sizeof(double) * (nf + x->ntr) * nc);
   hipError_t e = hipSuccess;
   for (size_t f = 0; f < map.size() && e == hipSuccess; f++)
     e = hipMemcpyAsync(host_state + (size_t)map[f] * nc, src + f * nc, nc * sizeof(double), hipMemcpyDeviceToHost, x->stream);
   if (e == hipSuccess)
     e = hipMemcpyAsync(host_state + (size_t)(nf - 1) * nc, src + (size_t)x->nprim * nc, nc * sizeof(double),
+                       hipMemcpyDeviceToHost, x->stream);
+  for (int t = 0; t < x->ntr && e == hipSuccess; t++)
+    e = hipMemcpyAsync(host_state + (size_t)(nf + t) * nc, src + ((size_t)x->nlevel + t) * nc, nc * sizeof(double),
                        hipMemcpyDeviceToHost, x->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(x->stream);
   if (e != hipSuccess) return ECO_ERR_HIP;
@@ -508,15 +585,15 @@ static int launch_one(eco_ctx *x) {
   int k = x->step;
   int rc;
   if (x->cfg.order == 1) {
-    rc = launch_stage(make_args(x, k), 0, x->stream);
+    rc = launch_stage_tr(x, make_args(x, k), 0);
   } else {
     StageArgs P = make_args(x, k);
     P.wout = x->bstar;
-    rc = launch_stage(P, 1, x->stream);
+    rc = launch_stage_tr(x, P, 1);
     if (!rc) {
       StageArgs Q = make_args(x, k);
       Q.wsrc = x->bstar;
-      rc = launch_stage(Q, 2, x->stream);
+      rc = launch_stage_tr(x, Q, 2);
     }
   }
   if (rc) return rc < 0 ? ECO_ERR_UNSUPPORTED : ECO_ERR_HIP;
@@ -661,19 +738,19 @@ extern "C" int eco_time_steps(eco_ctx *x, int nsteps, double *stage_ms, int *lau
     int k = x->step;
     if (nst == 1) {
       (void)hipEventRecord(ev[2 * i], x->stream);
-      rc = launch_stage(make_args(x, k), 0, x->stream);
+      rc = launch_stage_tr(x, make_args(x, k), 0);
       (void)hipEventRecord(ev[2 * i + 1], x->stream);
     } else {
       StageArgs P = make_args(x, k);
       P.wout = x->bstar;
       (void)hipEventRecord(ev[4 * i], x->stream);
-      rc = launch_stage(P, 1, x->stream);
+      rc = launch_stage_tr(x, P, 1);
       (void)hipEventRecord(ev[4 * i + 1], x->stream);
       if (!rc) {
         StageArgs Q = make_args(x, k);
         Q.wsrc = x->bstar;
         (void)hipEventRecord(ev[4 * i + 2], x->stream);
-        rc = launch_stage(Q, 2, x->stream);
+        rc = launch_stage_tr(x, Q, 2);
         (void)hipEventRecord(ev[4 * i + 3], x->stream);
       }
     }
@@ -726,6 +803,8 @@ extern "C" void eco_destroy(eco_ctx *x) {
   if (x->buf[0]) (void)hipFree(x->buf[0]);
   if (x->buf[1]) (void)hipFree(x->buf[1]);
   if (x->bstar) (void)hipFree(x->bstar);
+  for (auto &p : x->smf)
+    if (p) (void)hipFree(p);
   if (x->tables) (void)hipFree(x->tables);
   if (x->ctl) (void)hipFree(x->ctl);
   if (x->tmp64) (void)hipFree(x->tmp64);

@@ -17,6 +17,7 @@ struct Lowered {
   size_t off[3];            // offset of direction d's tables in tab
   int nprim;                // stored primitive fields on the device
   int nlevel;               // fields per time level: nprim, E, and sch.ncache cached values
+  int ntr;                  // transported scalars (stored after a level's nlevel fields)
   size_t es;                // bytes per stored value (8 FP64, 4 FP32 storage)
 };
 

@@ -15,7 +15,33 @@ struct StageArgs {
   double *wout;        // output state
   Control *ctl;
   int step;            // 0-based step index of the solve
+  // with transports: per direction the contact velocity sM of every face of
+  // the stage (faces of direction d indexed g0 + (n0 + (d==0)) * (g1 +
+  // (n1 + (d==1)) * g2), g_d in [0, n_d]), written for the transport pass
+  double *smf[3] = {nullptr, nullptr, nullptr};
 };
+
+// Transport pass of one stage (src/Transport/Transport.cpp): after the flow
+// kernel of the stage, from the faces' sM it wrote.  Scalars are stored
+// field-major, ncell doubles per scalar.
+struct TransportArgs {
+  Geometry geo;
+  double cfl;
+  int ntr, order;
+  int stage;           // 0 first order, 1 MUSCL predictor, 2 MUSCL corrector
+  int lim_g, lim_i;    // global / interface volume-fraction limiters
+  int bc_wall, bc_in;  // bit 2*dir+side: wall; outflow, injection or tank (imposed entering value)
+  double bc_tr[6][4];  // entering values per side and scalar
+  const double *asrc;  // phase-0 alpha of the stage state (interface detection)
+  const double *an;    // phase-0 alpha at time n (cells across the opposite faces)
+  const double *trs;   // scalars of the stage state (time n, or U*)
+  const double *trn;   // scalars at time n (non-conservative terms, update base)
+  double *tro;         // updated scalars
+  const double *sm[3];
+  Control *ctl;
+  int step;
+};
+int launch_transport(const TransportArgs &T, hipStream_t stream);
 
 // One tile of one case in a pool's persistent launch.
 struct TileRef {

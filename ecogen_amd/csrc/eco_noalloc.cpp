@@ -127,6 +127,8 @@ extern "C" int eco_run_noalloc(const eco_config *cfg, const double *state_in, do
   if (rc) return rc;
   if (!state_in || !state_out || state_in == state_out) return ECO_ERR_INVALID;
   if (cfg->precision != ECO_FP64) return ECO_ERR_UNSUPPORTED;
+  // transported scalars need a transport pass and face sM arrays: contexts only
+  if (cfg->ntransport > 0) return ECO_ERR_UNSUPPORTED;
   if (nsteps <= 0 && !(t_final > 0.)) return ECO_ERR_INVALID;
   if (nsteps == INT_MIN) return ECO_ERR_INVALID;
   const unsigned long long need = eco_noalloc_bytes(cfg);

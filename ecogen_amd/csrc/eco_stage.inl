@@ -431,11 +431,21 @@ __device__ __forceinline__ void get_flux_s(const double *L, int stride, int f, F
   F.sM = L[(q++) * stride + f];
 }
 
+// index of the face of direction dir whose high-side cell is g (g[dir] in
+// [0, n_dir]: the minus boundary face at 0, the plus one at n_dir)
+__device__ __forceinline__ long long face_index(const Geometry &G, int dir, const int *g) {
+  const long long n0 = G.n[0] + (dir == 0), n1 = G.n[1] + (dir == 1);
+  return g[0] + n0 * (g[1] + n1 * (long long)g[2]);
+}
+
 // the cell's two faces of one direction from the LDS flux rows (low face added,
-// high face subtracted, in that order: CellInterface.cpp:100-116)
+// high face subtracted, in that order: CellInterface.cpp:100-116).  smo
+// (transports only): the stage's face sM array of this direction, where the
+// cell writes its low face's sM (each interior face has one high-side cell).
 template <int MODEL, int NPH, int DIM, int dir>
 __device__ __forceinline__ void accumulate_dir(FluxVec<NPH> &acc, const double *sF, int stride, int base, int lx, int ly,
-                                               int lz, const int *ci, const Geometry &G, const CellState<NPH> &own) {
+                                               int lz, const int *ci, const Geometry &G, const CellState<NPH> &own,
+                                               double *smo = nullptr) {
   constexpr int TX = Tile<DIM>::X, TY = Tile<DIM>::Y;
   const int T0 = TX + (dir == 0), T1 = TY + (dir == 1);
   int fl[3] = {lx, ly, lz};
@@ -446,6 +456,7 @@ __device__ __forceinline__ void accumulate_dir(FluxVec<NPH> &acc, const double *
   if (ci[dir] >= 1) {
     get_flux_s<MODEL, NPH, DIM>(sF, stride, base + flo, F);
     add_face<MODEL, NPH>(acc, F, G.coefA[dir], true, own);
+    if (smo) gmem(smo)[face_index(G, dir, ci)] = F.sM;
   }
   if (ci[dir] <= G.n[dir] - 2) {
     get_flux_s<MODEL, NPH, DIM>(sF, stride, base + fhi, F);
@@ -738,7 +749,7 @@ __device__ __forceinline__ void stage_dir_3d(const StageArgs &A, const int *o, c
       if (dir > 0) unpark_acc<MODEL, NPH, DIM, NT>(sAcc, tid, acc);
       CellState<NPH> own;
       load_nc_coeffs<MODEL, NPH, DIM>(gmem(A.wn), A.geo.ncell, cidx, own);
-      accumulate_dir<MODEL, NPH, DIM, dir>(acc, sF, stride, 0, lx, ly, lz, ci, A.geo, own);
+      accumulate_dir<MODEL, NPH, DIM, dir>(acc, sF, stride, 0, lx, ly, lz, ci, A.geo, own, A.smf[dir]);
       if (dir < DIM - 1) park_acc<MODEL, NPH, DIM, NT>(sAcc, tid, acc);
     }
     __syncthreads();
@@ -1006,13 +1017,13 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int slot, c
     CellState<NPH> ownc;
     if (active) {
       load_nc_coeffs<MODEL, NPH, DIM>(Wn, nc, cidx, ownc);
-      accumulate_dir<MODEL, NPH, DIM, 0>(acc, &lds.F[0][0], SL::NFM, 0, lx, ly, lz, ci, G, ownc);
+      accumulate_dir<MODEL, NPH, DIM, 0>(acc, &lds.F[0][0], SL::NFM, 0, lx, ly, lz, ci, G, ownc, A.smf[0]);
     }
     stage_slopes_dispatch<MODEL, NPH, DIM, 1>(lds, S, G, o, tid);
     __syncthreads();
     smx = smax(smx, stage_faces<MODEL, NPH, DIM, 1>(lds, S, G, o, tid));
     __syncthreads();
-    if (active) accumulate_dir<MODEL, NPH, DIM, 1>(acc, &lds.F[0][0], SL::NFM, 0, lx, ly, lz, ci, G, ownc);
+    if (active) accumulate_dir<MODEL, NPH, DIM, 1>(acc, &lds.F[0][0], SL::NFM, 0, lx, ly, lz, ci, G, ownc, A.smf[1]);
   } else if constexpr (MERGED) {
     auto &sF = lds.sF;
     // all faces of the tile (both directions) in one pass, one barrier
@@ -1025,8 +1036,8 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int slot, c
     if (active) {
       CellState<NPH> own;
       load_nc_coeffs<MODEL, NPH, DIM>(Wn, nc, cidx, own);
-      accumulate_dir<MODEL, NPH, DIM, 0>(acc, &sF[0][0], NFT, 0, lx, ly, lz, ci, G, own);
-      if constexpr (DIM > 1) accumulate_dir<MODEL, NPH, DIM, 1>(acc, &sF[0][0], NFT, NF0, lx, ly, lz, ci, G, own);
+      accumulate_dir<MODEL, NPH, DIM, 0>(acc, &sF[0][0], NFT, 0, lx, ly, lz, ci, G, own, A.smf[0]);
+      if constexpr (DIM > 1) accumulate_dir<MODEL, NPH, DIM, 1>(acc, &sF[0][0], NFT, NF0, lx, ly, lz, ci, G, own, A.smf[1]);
     }
   } else {
     auto &sF = lds.sF;
@@ -1080,6 +1091,11 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int slot, c
         else smx = smax(smx, hllc<MODEL, NPH>(S, L, L, F));
         from_face_frame(dir, side == 0, F.q);
         add_face<MODEL, NPH>(acc, F, G.coefA[dir], false, own);
+        if (A.smf[dir]) {
+          int g[3] = {ci[0], ci[1], ci[2]};
+          g[dir] += side;
+          gmem(A.smf[dir])[face_index(G, dir, g)] = F.sM;
+        }
       }
     }
 

@@ -14,7 +14,9 @@
  *     for each phase k: alpha_k, rho_k, p_k     (Euler: 1, rho, p)
  *     u, v, w
  *     E  (mixture specific total energy; written by eco_get_state, ignored on input)
- * => ECO_NFIELDS(nphase) = 3*nphase + 4 fields.
+ *     the cfg->ntransport transported scalars (Cell::m_vecTransports), if any
+ * => ECO_NFIELDS(nphase) = 3*nphase + 4 fields, ECO_NFIELDS_TR(nphase, ntransport)
+ *    with the transports.
  */
 #ifndef ECOGEN_H
 #define ECOGEN_H
@@ -26,6 +28,8 @@ extern "C" {
 #endif
 
 #define ECO_NFIELDS(nph) (3 * (nph) + 4)
+#define ECO_NFIELDS_TR(nph, ntr) (3 * (nph) + 4 + (ntr))
+#define ECO_MAX_TRANSPORTS 4
 
 enum eco_model { ECO_MODEL_EULER = 0, ECO_MODEL_KAPILA = 1, ECO_MODEL_MULTIP = 2 };
 enum eco_limiter { ECO_LIM_NONE = 0, ECO_LIM_MINMOD = 1, ECO_LIM_VANLEER = 2, ECO_LIM_VANALBADA = 3,
@@ -98,6 +102,15 @@ typedef struct {
   double bc_tank_p0[6], bc_tank_T0[6];
   double bc_tank_frac[6][4];
   int bc_tank_massfrac[6];
+  /* transport equations (<flowModel numberTransports>, Input.cpp:472-540;
+     src/Transport/Transport.cpp): passive scalars carried by the contact
+     velocity, 0..ECO_MAX_TRANSPORTS, Kapila and MultiP in FP64 single solves
+     (the reference's Euler model has no transport Riemann solver: Model.h:127
+     ends the run).  bc_tr: each outflow, injection or tank side's
+     <transport value> per scalar, the value entering through it
+     (Transport.cpp:83-104). */
+  int ntransport;
+  double bc_tr[6][ECO_MAX_TRANSPORTS];
 } eco_config;
 
 typedef struct eco_ctx eco_ctx;
@@ -156,8 +169,9 @@ int eco_sync(eco_ctx *ctx, eco_status *status);
 
 /* Measurement form of eco_run(nsteps > 0): the same launches, each stage
    kernel bracketed by HIP events on the context stream.  stage_ms[3] receives
-   the summed kernel time of the first-order / predictor / corrector kernels,
-   launches[3] the number of launches of each.  Synchronises. */
+   the summed kernel time of the first-order / predictor / corrector kernels
+   (with transports, each stage's transport pass included), launches[3] the
+   number of launches of each.  Synchronises. */
 int eco_time_steps(eco_ctx *ctx, int nsteps, double *stage_ms, int *launches, eco_status *status);
 
 /* Measurement utility (no reference counterpart): a plain device copy of
@@ -178,7 +192,7 @@ const double *eco_state_device(eco_ctx *ctx, int *nfields_device);
    to state_out, both device arrays in the layout above (E ignored on input,
    written on output; they may not overlap), with the same results bit for
    bit.  nsteps > 0: exactly nsteps steps; nsteps <= 0: until t >= t_final
-   (as float), at most -nsteps steps when nsteps < 0.  FP64 only.
+   (as float), at most -nsteps steps when nsteps < 0.  FP64 only, no transports.
    Every intermediate buffer (second time level, MUSCL U*, distance tables,
    control block) lives in a static device array of the library,
    ECO_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends; no

@@ -65,7 +65,8 @@ unsigned long long eco_pool_case_bytes(const eco_config *cfg);
    ecogen.h, E ignored) and sets its target (nsteps > 0: exactly nsteps steps;
    nsteps <= 0: until t >= t_final compared as float, src/Run.cpp:239-244).
    The upload overlaps the pool's running work; the case joins at the next
-   chunk.  case_id receives the case handle. */
+   chunk.  case_id receives the case handle.  A configuration with transports
+   (ntransport > 0) is refused with ECO_ERR_UNSUPPORTED: it runs in a context. */
 int eco_pool_admit(eco_pool *p, const eco_config *cfg, const double *host_state, int nsteps, double t_final,
                    int *case_id);
 

@@ -55,6 +55,9 @@ def _state_xml(case, st):
             out.append(f'      <dataMix pressure="{st["p"][0]!r}"/>')
         out.append(f'      <velocity x="{u[0]!r}" y="{u[1]!r}" z="{u[2]!r}"/>')
         out.append('    </mixture>')
+    # Input.cpp:771-790: <transport name value> of the state, matched by name
+    for name, v in zip(case.get("transports", []), st.get("tr", [])):
+        out.append(f'    <transport name="{name}" value="{v!r}"/>')
     out.append('  </state>')
     return "\n".join(out)
 
@@ -129,7 +132,12 @@ def write_deck(case, root: Path):
     fm = {"euler": '<flowModel name="Euler"/>',
           "kapila": f'<flowModel name="Kapila" numberPhases="{len(case["eos"])}" alphaNull="{"true" if case["alpha_null"] else "false"}"/>',
           "multip": f'<flowModel name="MultiP" numberPhases="{len(case["eos"])}"/>'}[model]
+    trs = case.get("transports", [])
+    if trs:
+        # Input.cpp:472-473, 528-540: numberTransports and one <transport name> each
+        fm = fm.replace("/>", f' numberTransports="{len(trs)}"/>')
     eosl = "\n".join(f'  <EOS name="{_eos_file(e)}"/>' for e in case["eos"])
+    eosl += "".join(f'\n  <transport name="{t}"/>' for t in trs)
     relax = '  <relaxation type="P"/>\n' if case.get("relax_p") and model == "multip" else ""
     (cdir / "modelV4.xml").write_text(
         '<?xml version="1.0" encoding="UTF-8" standalone="yes"?>\n<model>\n'
@@ -140,13 +148,22 @@ def write_deck(case, root: Path):
     kinds = case.get("bc", ["abs"] * 6)
     p0s = case.get("bc_p0", [0.0] * 6)
 
+    def tr_xml(i, tag, attrs):
+        # BoundCondOutflow.cpp:58-79, BoundCondInj.cpp:120-147, BoundCondTank.cpp:146-173:
+        # the side's transport values inside its data element
+        vals = (case.get("bc_tr") or [None] * 6)[i] or []
+        if not trs:
+            return f'      <{tag} {attrs}/>\n'
+        body = "".join(f'        <transport name="{t}" value="{v!r}"/>\n' for t, v in zip(trs, vals))
+        return f'      <{tag} {attrs}>\n{body}      </{tag}>\n'
+
     def bc_xml(i):
         head = f'    <boundCond name="{names[i]}" type="{kinds[i]}" number="{i + 1}"'
         if kinds[i] == "outflow":
-            return head + f'>\n      <dataOutflow p0="{p0s[i]!r}"/>\n    </boundCond>'
+            return head + '>\n' + tr_xml(i, "dataOutflow", f'p0="{p0s[i]!r}"') + '    </boundCond>'
         if kinds[i] == "injection":
             inj = case["bc_inj"][i]
-            body = f'>\n      <dataInjection m0="{inj["m0"]!r}"/>\n'
+            body = '>\n' + tr_xml(i, "dataInjection", f'm0="{inj["m0"]!r}"')
             if len(case["eos"]) == 1:
                 body += f'      <dataFluid density="{inj["rho"][0]!r}" pressure="{inj["p"][0]!r}"/>\n'
             else:
@@ -156,7 +173,7 @@ def write_deck(case, root: Path):
             return head + body + "    </boundCond>"
         if kinds[i] == "tank":
             tk = case["bc_tank"][i]
-            body = f'>\n      <dataTank p0="{tk["p0"]!r}" T0="{tk["T0"]!r}"/>\n'
+            body = '>\n' + tr_xml(i, "dataTank", f'p0="{tk["p0"]!r}" T0="{tk["T0"]!r}"')
             if len(case["eos"]) > 1:
                 key = "massFrac" if "massFrac" in tk else "alpha"
                 body += "      <fluidsProp>\n" + "".join(

@@ -22,6 +22,7 @@
 #endif
 
 #define MAXPH ORC_MAXPH
+#define MAXTR ORC_MAXTR
 
 /* std::min / std::max / std::abs semantics (<algorithm>, <cmath>). */
 static inline double smin(double a, double b) { return (b < a) ? b : a; }
@@ -42,6 +43,7 @@ typedef struct {
 typedef struct {
   Phase ph[MAXPH];
   Mix mx;
+  double tr[MAXTR]; /* Cell::m_vecTransports (Transport::m_value) */
 } State;
 
 /* Flux{Kapila,MultiP,Euler}: alpha/masse/energ per phase, qdm, energMixture, sM. */
@@ -50,6 +52,7 @@ typedef struct {
   double qdm[3];
   double energMix;
   double sM;
+  double tr[MAXTR]; /* per face: Transport flux (fluxBufferTransport); per cell: m_consTransports */
 } Flux;
 
 typedef struct {
@@ -1200,6 +1203,8 @@ static void prim_vec(const Ctx *c, const State *s, double *w) {
   }
   for (int k = 0; k < c->cfg->nph; k++) { w[q++] = s->ph[k].alpha; w[q++] = s->ph[k].rho; w[q++] = s->ph[k].p; }
   w[q++] = s->mx.u[0]; w[q++] = s->mx.u[1]; w[q++] = s->mx.u[2];
+  /* Transport::computeSlopeTransport (Transport.cpp:156-159) on the same distance */
+  for (int t = 0; t < c->cfg->ntr; t++) w[q++] = s->tr[t];
 }
 
 /* CellInterfaceO2::computeSlopes (CellInterfaceO2.cpp:132-152) */
@@ -1209,7 +1214,7 @@ static void compute_face_slopes(Solver *S, State *st) {
 #pragma omp parallel for schedule(static)
   for (int fi = 0; fi < m->nint; fi++) {
     Face *F = &m->faces[fi];
-    double pl[3], pr[3], wl[16], wr[16];
+    double pl[3], pr[3], wl[32], wr[32];
     cell_pos(m, F->left, pl); cell_pos(m, F->right, pr);
     double dist = dist3(pl, pr);
     prim_vec(&S->c, &st[F->left], wl);
@@ -1229,7 +1234,7 @@ static void compute_face_slopes(Solver *S, State *st) {
     double dd = pl[F->dir] - F->pos[F->dir];
     if (fabs(dd) > 1.e-8) {
       int iu = (S->c.cfg->model == ORC_EULER) ? 2 + F->dir : 3 * S->c.cfg->nph + F->dir;
-      double w[16];
+      double w[32];
       prim_vec(&S->c, &st[F->left], w);
       sl[iu] = w[iu] / dd;
     }
@@ -1246,7 +1251,7 @@ static void local_slopes(Solver *S, int cell, int fref, int fopp, int cell_is_le
   const orc_cfg *cfg = c->cfg;
   int nv = S->nvar;
   const double *s1 = &S->fslope[(size_t)fref * nv];
-  double s2[16];
+  double s2[32];
   double aafter = 0.;
   const Face *Fr = &S->m.faces[fref];
   int fb = cell_is_left ? Fr->bnd_left : Fr->bnd_right;
@@ -1279,6 +1284,8 @@ static void local_slopes(Solver *S, int cell, int fref, int fopp, int cell_is_le
     out[q] = limiter(lg, s1[q], s2[q]); q++;
   }
   for (int d = 0; d < 3; d++, q++) out[q] = limiter(lg, s1[q], s2[q]);
+  /* transports: the volume-fraction limiter of the same interface test (CellO2.cpp:232-246) */
+  for (int t = 0; t < cfg->ntr; t++, q++) out[q] = limiter(lv, s1[q], s2[q]);
 }
 
 /* CellO2::computeLocalSlopesLimite (CellO2.cpp:250-354) for a wall face:
@@ -1288,7 +1295,7 @@ static void local_slopes_limite(Solver *S, int cell, int fwall, int fopp, double
   const orc_cfg *cfg = S->c.cfg;
   int nv = S->nvar;
   const double *s1 = &S->fslope[(size_t)fwall * nv];
-  double s2[16];
+  double s2[32];
   for (int v = 0; v < nv; v++) s2[v] = (fopp >= 0) ? S->fslope[(size_t)fopp * nv + v] : 0.;
   int iface = 0;
   if (cfg->model != ORC_EULER) {
@@ -1308,6 +1315,7 @@ static void local_slopes_limite(Solver *S, int cell, int fwall, int fopp, double
     out[q] = limiter(lg, s1[q], s2[q]); q++;
   }
   for (int d = 0; d < 3; d++, q++) out[q] = limiter(lg, s1[q], s2[q]);
+  for (int t = 0; t < cfg->ntr; t++, q++) out[q] = limiter(lv, s1[q], s2[q]); /* CellO2.cpp:341-355 */
 }
 
 /* Phase*::extrapolate / Mix*::extrapolate without the phase verification
@@ -1327,6 +1335,7 @@ static void extrapolate_raw(const Ctx *c, State *s, const double *sl, double dis
     p->p += sl[q] * dist; q++;
   }
   for (int d = 0; d < 3; d++, q++) s->mx.u[d] = s->mx.u[d] + sl[q] * dist;
+  for (int t = 0; t < c->cfg->ntr; t++, q++) s->tr[t] += sl[q] * dist; /* BoundCondWallO2.cpp:142-144 */
 }
 
 static void extrapolate(const Ctx *c, State *s, const double *sl, double dist, double sign) {
@@ -1348,6 +1357,9 @@ static void extrapolate(const Ctx *c, State *s, const double *sl, double dist, d
     verify_correct(c, k, p);
   }
   for (int d = 0; d < 3; d++, q++) s->mx.u[d] = s->mx.u[d] + (sign * sl[q]) * dist;
+  /* Transport::extrapolate (Transport.cpp:163-166), the slope negated for the
+     right state (CellInterfaceO2.cpp:222-226, 254-257) */
+  for (int t = 0; t < c->cfg->ntr; t++, q++) s->tr[t] += (sign * sl[q]) * dist;
 }
 
 /* THINC reconstruction, CellInterfaceO2.cpp:228-247 (left state) and 259-277
@@ -1375,6 +1387,22 @@ static void thinc(State *s, double acell, double a_hi, double a_lo, int right_si
 
 static int is_thinc(const orc_cfg *cfg) { return cfg->lim[2] == ORC_LIM_THINC || cfg->lim[3] == ORC_LIM_THINC; }
 
+/* Transport fluxes of a face after its Riemann problem, sM known
+   (CellInterface.cpp:139-140, BoundCond.cpp:95-96): Transport::solveRiemann
+   on the two face states (Transport.cpp:68-72; a transmissive boundary passes
+   its cell twice, BoundCondAbs.cpp:77-80), zero at a wall (solveRiemannWall,
+   Transport.cpp:76-79), and at an outflow, injection or tank the cell's value
+   leaving or the boundary's value entering (Transport.cpp:83-104). */
+static void transport_flux(const Ctx *c, const State *L, const State *R, int bc, int side, Flux *F) {
+  double sM = F->sM;
+  for (int t = 0; t < c->cfg->ntr; t++) {
+    if (R) F->tr[t] = (sM > 0.) ? L->tr[t] * sM : R->tr[t] * sM;
+    else if (bc == ORC_BC_ABS) F->tr[t] = (sM > 0.) ? L->tr[t] * sM : L->tr[t] * sM;
+    else if (bc == ORC_BC_WALL) F->tr[t] = 0.;
+    else F->tr[t] = (sM > 0.) ? L->tr[t] * sM : c->cfg->bc_tr[side][t] * sM;
+  }
+}
+
 /* Face Riemann problems of one pass (Run.cpp:356,381,401); fluxes stored per face. */
 static void face_fluxes(Solver *S, State *st, int o2) {
   Mesh *m = &S->m;
@@ -1393,7 +1421,7 @@ static void face_fluxes(Solver *S, State *st, int o2) {
       if (o2 && c->cfg->bc[F->side] == ORC_BC_WALL) {
         /* BoundCondWallO2::solveRiemann: the cell extrapolated to the wall
            (distance negated when the normal points to -x, -y or -z) */
-        double sl[16], pl[3];
+        double sl[32], pl[3];
         local_slopes_limite(S, F->left, fi, F->opp_int, sl);
         cell_pos(m, F->left, pl);
         double dg = dist3(F->pos, pl);
@@ -1414,11 +1442,13 @@ static void face_fluxes(Solver *S, State *st, int o2) {
         }
       }
       else riemann(c, &L, &L, lcfl, lcfl, &dtm, FL);
+      transport_flux(c, &L, NULL, c->cfg->bc[F->side], F->side, FL);
     } else if (!o2) {
       /* CellInterface::solveRiemann (CellInterface.cpp:127-146): the cells' own cached state */
       State L = st[F->left], R = st[F->right];
       proj_state(c, &L, F); proj_state(c, &R, F);
       riemann(c, &L, &R, lcfl, lcfl, &dtm, FL);
+      transport_flux(c, &L, &R, 0, -1, FL);
     } else {
       /* CellInterfaceO2::solveRiemann (CellInterfaceO2.cpp:197-295) */
       State L = st[F->left], R = st[F->right];
@@ -1427,7 +1457,7 @@ static void face_fluxes(Solver *S, State *st, int o2) {
       double dL = dist3(F->pos, pl), dR = dist3(F->pos, pr);
       int multi = (c->cfg->model != ORC_EULER);
       double aL = multi ? L.ph[0].alpha : 0., aR = multi ? R.ph[0].alpha : 0.;
-      double slL[16], slR[16], aLL, aRR;
+      double slL[32], slR[32], aLL, aRR;
       local_slopes(S, F->left, fi, F->opp_left, 1, aL, aR, &aLL, slL);
       extrapolate(c, &L, slL, dL, 1.);
       if (is_thinc(c->cfg)) thinc(&L, aL, aR, aLL, 0);
@@ -1437,6 +1467,7 @@ static void face_fluxes(Solver *S, State *st, int o2) {
       proj_state(c, &L, F); proj_state(c, &R, F);
       fulfill(c, &L); fulfill(c, &R);
       riemann(c, &L, &R, lcfl, lcfl, &dtm, FL);
+      transport_flux(c, &L, &R, 0, -1, FL);
     }
     rproj(FL->qdm, F);
     S->fdt[fi] = dtm;
@@ -1479,6 +1510,12 @@ static void accumulate(Solver *S) {
             U->alpha[k] += -coefA * sn->ph[k].alpha * FL->sM;
             U->energ[k] += coefA * sn->ph[k].alpha * sn->ph[k].p * FL->sM;
           }
+        /* Transport::addFlux, addNonCons with the cell's time-n value
+           (CellInterface.cpp:158-162, Transport.cpp:107-126) */
+        for (int t = 0; t < c->cfg->ntr; t++) {
+          U->tr[t] += coefA * FL->tr[t];
+          U->tr[t] += -coefA * sn->tr[t] * FL->sM;
+        }
       } else {
         if (euler) {
           U->masse[0] -= coefA * FL->masse[0];
@@ -1496,6 +1533,12 @@ static void accumulate(Solver *S) {
             U->alpha[k] -= -coefA * sn->ph[k].alpha * FL->sM;
             U->energ[k] -= coefA * sn->ph[k].alpha * sn->ph[k].p * FL->sM;
           }
+        /* Transport::subtractFlux, subtractNonCons (CellInterface.cpp:175-179,
+           Transport.cpp:114-131) */
+        for (int t = 0; t < c->cfg->ntr; t++) {
+          U->tr[t] -= coefA * FL->tr[t];
+          U->tr[t] -= -coefA * sn->tr[t] * FL->sM;
+        }
       }
     }
   }
@@ -1544,6 +1587,16 @@ static void solve_relaxations(Solver *S) {
   }
 }
 
+/* Transports of Cell::timeEvolution (Cell.cpp:222-226) and
+   CellO2::predictionOrdre2 (CellO2.cpp:385-390): the accumulated value times
+   dt (0.5*dt) added to the time-n value; no buildPrim, no relaxation. */
+static void transport_update(const Ctx *c, Flux *U, double mult, const State *sn, State *out) {
+  for (int t = 0; t < c->cfg->ntr; t++) {
+    U->tr[t] *= mult;
+    out->tr[t] = sn->tr[t] + U->tr[t];
+  }
+}
+
 /* One time step of Run::solver (Run.cpp:204-277 / 281-412). Returns dtMax. */
 static double step(Solver *S, double dt) {
   const Ctx *c = &S->c;
@@ -1560,6 +1613,7 @@ static double step(Solver *S, double dt) {
       scale_add(c, &S->cons[ci], dt, &buf);
       scheme_correction(c, &S->cons[ci], &buf);
       build_prim(c, &S->cons[ci], &S->sn[ci]);
+      transport_update(c, &S->cons[ci], dt, &S->sn[ci], &S->sn[ci]);
     }
   } else {
     /* integrationProcedure: slopes on U^n (Run.cpp:293-295) */
@@ -1576,6 +1630,7 @@ static double step(Solver *S, double dt) {
       scale_add(c, &S->cons[ci], 0.5 * dt, &buf);
       scheme_correction(c, &S->cons[ci], &buf);
       build_prim(c, &S->cons[ci], s2);
+      transport_update(c, &S->cons[ci], 0.5 * dt, sn, s2);
       if (model_relaxes(c->cfg) && c->cfg->nph > 1) relax_p(c, s2);
       total_to_internal(c, s2);
       correction_energy(c, s2, sn);
@@ -1593,6 +1648,7 @@ static double step(Solver *S, double dt) {
       scale_add(c, &S->cons[ci], dt, &buf);
       scheme_correction(c, &S->cons[ci], &buf);
       build_prim(c, &S->cons[ci], &S->sn[ci]);
+      transport_update(c, &S->cons[ci], dt, &S->sn[ci], &S->sn[ci]);
     }
   }
   solve_relaxations(S);
@@ -1622,6 +1678,7 @@ static void init_state(Solver *S, const double *in) {
     for (int d = 0; d < 3; d++) s->mx.u[d] = u[d];
     fulfill(c, s);
     s->mx.E = s->mx.e + 0.5 * sqnorm(s->mx.u);
+    for (int t = 0; t < c->cfg->ntr; t++) s->tr[t] = in[(size_t)(3 * n + 4 + t) * nc + ci];
   }
 }
 
@@ -1645,6 +1702,7 @@ static void write_state(Solver *S, double *out) {
     }
     for (int d = 0; d < 3; d++) out[(size_t)(3 * n + d) * nc + ci] = s->mx.u[d];
     out[(size_t)(3 * n + 3) * nc + ci] = s->mx.E;
+    for (int t = 0; t < c->cfg->ntr; t++) out[(size_t)(3 * n + 4 + t) * nc + ci] = s->tr[t];
   }
 }
 
@@ -1665,6 +1723,8 @@ int orc_run_from(const orc_cfg *cfg, const double *state_in, double *state_out,
   if (!cfg || cfg->nph < 1 || cfg->nph > MAXPH) return -1;
   if (cfg->model == ORC_EULER && cfg->nph != 1) return -1;
   if (is_thinc(cfg) && cfg->nph != 2) return -1;
+  /* the Euler model has no transport Riemann solver (Model.h:127, an error) */
+  if (cfg->ntr < 0 || cfg->ntr > MAXTR || (cfg->ntr > 0 && cfg->model == ORC_EULER)) return -1;
   for (int b = 0; b < 6; b++) {
     if (cfg->bc[b] < ORC_BC_ABS || cfg->bc[b] > ORC_BC_TANK) return -1;
 
@@ -1680,7 +1740,7 @@ int orc_run_from(const orc_cfg *cfg, const double *state_in, double *state_out,
   S.c.cfg = cfg;
   S.c.epsilon = (cfg->model == ORC_KAPILA && cfg->alpha_null) ? 1.e-15 : 0.;
   mesh_build(&S.m, cfg);
-  S.nvar = (cfg->model == ORC_EULER) ? 5 : 3 * cfg->nph + 3;
+  S.nvar = (cfg->model == ORC_EULER) ? 5 : 3 * cfg->nph + 3 + cfg->ntr;
   S.sn = calloc(S.m.ncell, sizeof(State));
   S.s2 = calloc(S.m.ncell, sizeof(State));
   S.cons = calloc(S.m.ncell, sizeof(Flux));

@@ -25,6 +25,7 @@ enum { ORC_LIM_NONE = 0, ORC_LIM_MINMOD = 1, ORC_LIM_VANLEER = 2, ORC_LIM_VANALB
        ORC_LIM_SUPERBEE = 4, ORC_LIM_MC = 5, ORC_LIM_THINC = 6 };
 enum { ORC_EOS_IG = 0, ORC_EOS_SG = 1 };
 #define ORC_MAXPH 4
+#define ORC_MAXTR 4 /* transport equations (Transport, src/Transport/Transport.cpp) */
 
 /* boundary conditions: BoundCondAbs, and the wall: BoundCondWall at first
    order (also the reference's first-order SYMMETRY), BoundCondWallO2 at second
@@ -58,6 +59,8 @@ typedef struct {
   double bc_tank_p0[6], bc_tank_T0[6];   /* tank: <dataTank p0 T0> */
   double bc_tank_frac[6][ORC_MAXPH];     /* tank: <dataFluid alpha> (or massFrac) per phase */
   int bc_tank_massfrac[6];               /* tank: 1 if bc_tank_frac holds mass fractions */
+  int ntr;                               /* transported scalars (<flowModel numberTransports>); not Euler */
+  double bc_tr[6][ORC_MAXTR];            /* outflow, injection, tank: <transport value> per transport */
 } orc_cfg;
 
 /*
@@ -66,7 +69,8 @@ typedef struct {
  *   for each phase k: alpha_k, rho_k, p_k      (3*nph fields; Euler: alpha ignored)
  *   u, v, w                                    (3 fields)
  *   E  (mixture specific total energy; output only)
- * => nfields = 3*nph + 4.
+ *   the ntr transported scalars
+ * => nfields = 3*nph + 4 + ntr.
  *
  * orc_run: initialise from (alpha, rho, p, u) exactly as the reference does
  * at t = 0, then march.  nsteps > 0: iteration-controlled run of nsteps

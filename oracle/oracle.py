@@ -25,6 +25,9 @@ EOS_TYPES = {"IG": 0, "SG": 1}
 # case["bc_tank"][side] = {"p0", "T0", and per phase "alpha" or "massFrac"}
 BCS = {"abs": 0, "wall": 1, "outflow": 2, "injection": 3, "tank": 4}
 MAXPH = 4
+# transported scalars: case["transports"] = names, each state's "tr" = values
+# in that order, case["bc_tr"][side] = values at an outflow, injection or tank side
+MAXTR = 4
 
 
 class OrcEos(C.Structure):
@@ -40,7 +43,7 @@ class OrcCfg(C.Structure):
                 ("bc_ak0", (C.c_double * MAXPH) * 6), ("bc_rhok0", (C.c_double * MAXPH) * 6),
                 ("bc_pk0", (C.c_double * MAXPH) * 6), ("bc_tank_p0", C.c_double * 6),
                 ("bc_tank_T0", C.c_double * 6), ("bc_tank_frac", (C.c_double * MAXPH) * 6),
-                ("bc_tank_massfrac", C.c_int * 6)]
+                ("bc_tank_massfrac", C.c_int * 6), ("ntr", C.c_int), ("bc_tr", (C.c_double * MAXTR) * 6)]
 
 
 _lib = None
@@ -126,11 +129,19 @@ def make_cfg(case) -> OrcCfg:
             cfg.bc_tank_massfrac[b] = int("massFrac" in tk)
             for k, v in enumerate(frac):
                 cfg.bc_tank_frac[b][k] = v
+    cfg.ntr = len(case.get("transports", []))
+    for b, vals in enumerate(case.get("bc_tr", [None] * 6)):
+        for t, v in enumerate(vals or []):
+            cfg.bc_tr[b][t] = v
     return cfg
 
 
+def ntransports(case) -> int:
+    return len(case.get("transports", []))
+
+
 def nfields(case) -> int:
-    return 3 * len(case["eos"]) + 4
+    return 3 * len(case["eos"]) + 4 + ntransports(case)
 
 
 def run(case, state, nsteps=None, t_final=None, max_steps=0, nthreads=0, log_dt=False, resume=None):
@@ -164,10 +175,14 @@ def run(case, state, nsteps=None, t_final=None, max_steps=0, nthreads=0, log_dt=
 
 
 def state_from_golden(case, d, prefix="init"):
-    """Field-major state (alpha, rho, p per phase; u, v, w) from a golden .npz."""
+    """Field-major state (alpha, rho, p per phase; u, v, w; with transports
+    also E, left 0, and the transported scalars) from a golden .npz."""
     nph = len(case["eos"])
     ncell = int(np.prod(case["cells"]))
-    out = np.zeros((3 * nph + 3, ncell))
+    ntr = ntransports(case)
+    out = np.zeros((3 * nph + 4 + ntr if ntr else 3 * nph + 3, ncell))
+    if ntr:
+        out[3 * nph + 4:] = d[f"{prefix}_tr"]
     if case["model"] == "euler":
         out[0] = 1.0
         out[1] = d[f"{prefix}_rho"][0]

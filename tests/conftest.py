@@ -84,6 +84,9 @@ def parity_quantities(case, state):
         for k in range(nph):
             out[f"alpha{k}"] = state[3 * k]
     out["velocity"] = np.concatenate([state[3 * nph + d] for d in range(dim)])
+    # transported scalars (stored after E)
+    for t in range(len(case.get("transports", []))):
+        out[f"tr{t}"] = state[3 * nph + 4 + t]
     return out
 
 
@@ -111,6 +114,9 @@ def compared_fields(case):
             out += [(3 * k, f"alpha{k}"), (3 * k + 1, f"rho{k}"), (3 * k + 2, f"p{k}")]
     for d in range(dim):
         out.append((3 * nph + d, "uvw"[d]))
+    # transported scalars after E
+    for t in range(len(case.get("transports", []))):
+        out.append((3 * nph + 4 + t, f"tr{t}"))
     return out
 
 

@@ -255,6 +255,35 @@ def _water_column_2d(n, ratio, pos, t_final):
     }
 
 
+def _with_transports(case, names, values, bc_tr=None):
+    """``case`` with transported scalars ``names``: ``values`` maps a state
+    name to its values (0 for a state not listed, as Input.cpp:771-790 leaves
+    them); ``bc_tr`` lists the values at each outflow, injection or tank side."""
+    c = dict(case, transports=list(names))
+    c["states"] = [dict(st, tr=list(values.get(st["name"], [0.0] * len(names)))) for st in case["states"]]
+    if bc_tr is not None:
+        c["bc_tr"] = bc_tr
+    return c
+
+
+def _interface_advection_1d():
+    """The reference's transport test (libTests/referenceTestCases/kapila/1D/
+    transports/interfaceWaterAir): water left of x = 0.3 and air on the right,
+    both at 1e5 Pa moving at 1000 m/s, a colour of 1 in the water and 2 in the
+    air, first order, CFL 0.6, to t = 500 us on 800 cells."""
+    water = {"name": "water", "alpha": [1e-6, 1.0 - 1e-6], "rho": [1.0, 1000.0], "p": [1e5, 1e5], "u": [1000.0]}
+    air = {"name": "air", "alpha": [1.0 - 1e-6, 1e-6], "rho": [1.0, 1000.0], "p": [1e5, 1e5], "u": [1000.0]}
+    c = {
+        "model": "kapila", "alpha_null": False, "eos": [AIR, WATER],
+        "cells": [800, 1, 1], "lengths": [1.0, 0.1, 0.1], "order": 1, "limiters": ["none"] * 4,
+        "cfl": 0.6, "time_mode": "time", "t_final": 500e-6,
+        "domains": [{"type": "entireDomain", "state": "water"},
+                    {"type": "halfSpace", "state": "air", "axe": "x", "origin": 0.3, "direction": "positive"}],
+        "states": [air, water],
+    }
+    return _with_transports(c, ["color"], {"water": [1.0], "air": [2.0]})
+
+
 CASES = {
     # Config C1 in full: 1000 cells, first order, to t = 240 us.
     "c1_kapila_fo": _c1(False),
@@ -361,6 +390,36 @@ CASES = {
     "kapila_2d_fo_injection": dict(_shock_cylinders_2d(n=40, steps=80), order=1, limiters=["none"] * 4,
                                    bc=["abs", "outflow", "injection", "wall", "abs", "abs"],
                                    bc_p0=[0.0, 8e4] + [0.0] * 4, bc_inj=_inj(2, 300.0, 2, x_side=0)),
+    # Transported scalars (Transport, src/Transport/Transport.cpp): the
+    # reference's own interface-advection test, then MUSCL with the interface
+    # volume-fraction limiter (THINC, minmod slopes for the scalars) next to a
+    # different global one, and every boundary type carrying scalars in and out
+    "kapila_1d_fo_transport": _interface_advection_1d(),
+    "kapila_1d_o2_transport_thinc": _with_transports(
+        _shock_tube_1d("kapila", 2, ["vanleer", "superbee", "mc", "thinc"], 120), ["color", "dye"],
+        {"LP": [1.0, 0.25], "HP": [0.0, -3.0]}),
+    "kapila_1d_o2_transport_wall_tank": _with_transports(
+        dict(_shock_tube_1d("kapila", 2, ["minmod", "vanleer", "minmod", "vanleer"], 300, cells=200),
+             bc=["wall", "tank"] + ["abs"] * 4, bc_tank=_tank(2, 3e5, 300.0, 1)),
+        ["color"], {"LP": [1.0], "HP": [0.0]}, bc_tr=[None, [5.0]] + [None] * 4),
+    "multip_1d_o2_relax_transport_inj_out": _with_transports(
+        dict(_shock_tube_1d("multip", 2, ["minmod"] * 4, 160, cells=100, relax=True),
+             bc=["outflow", "injection"] + ["abs"] * 4, bc_p0=[2e9, 0.0] + [0.0] * 4, bc_inj=_inj(2, 800.0, 1, x_side=1)),
+        ["color", "dye"], {"LP": [1.0, 2.0], "HP": [0.0, 1.0]}, bc_tr=[[0.5, 7.0], [3.0, -1.0]] + [None] * 4),
+    "kapila_1d_fo_transport_outflow": _with_transports(
+        dict(_shock_tube_1d("kapila", 1, ["none"] * 4, 300, cells=200),
+             bc=["outflow", "outflow"] + ["abs"] * 4, bc_p0=[2e9, 5e4] + [0.0] * 4),
+        ["color"], {"LP": [1.0], "HP": [0.0]}, bc_tr=[[4.0], [9.0]] + [None] * 4),
+    "kapila_2d_o2_transport": _with_transports(
+        dict(_shock_cylinders_2d(n=40, steps=50), bc=["abs", "outflow", "wall", "wall", "abs", "abs"],
+             bc_p0=[0.0, 8e4] + [0.0] * 4),
+        ["color", "dye"], {"water": [1.0, 0.0], "shocked": [0.0, 1.0]}, bc_tr=[None, [0.0, 0.0]] + [None] * 4),
+    "multip_2d_fo_relax_transport": _with_transports(
+        dict(_shock_cylinders_2d(n=40, steps=60), model="multip", relax_p=True, order=1, limiters=["none"] * 4,
+             bc=["abs", "outflow", "injection", "wall", "abs", "abs"], bc_p0=[0.0, 8e4] + [0.0] * 4,
+             bc_inj=_inj(2, 300.0, 2, x_side=0)),
+        ["color"], {"water": [1.0], "shocked": [0.5]}, bc_tr=[None, [0.0], [2.0]] + [None] * 3),
+    "multip_3d_o2_transport": _with_transports(_bubbles_3d(n=12, steps=10), ["color"], {"bubble": [1.0]}),
 }
 
 

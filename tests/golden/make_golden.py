@@ -74,6 +74,10 @@ def fields_from_vtr(case, d):
         res["temp"] = np.stack([d[[k for k in d if k.startswith(f"F{i}_Temperature")][0]] for i in range(nph)])
         vel = d["Velocity_Mixture"]
     res["u"] = vel.T.copy()
+    # transported scalars: DataArray "T1", "T2", ... (OutputXML.cpp:435-447)
+    ntr = len(case.get("transports", []))
+    if ntr:
+        res["tr"] = np.stack([d[f"T{t + 1}"] for t in range(ntr)])
     return res
 
 

@@ -146,3 +146,29 @@ def test_phase_counts_are_validated_without_device(lib):
     cfg.nphase = 5  # eco_config holds four EOS
     ctx = C.c_void_p()
     assert lib.eco_create(C.byref(cfg), None, C.byref(ctx)) == -1
+
+
+def test_transport_configs_are_validated_without_device(lib):
+    """Transported scalars: up to four, Kapila and MultiP in FP64 single
+    solves.  The reference's Euler model has no transport Riemann solver
+    (Model::solveRiemannTransportIntern ends the run, Model.h:127), so an
+    Euler configuration with transports is invalid; FP32 storage, pools and
+    the allocation-free entry point refuse them."""
+    import ctypes as C
+    from ecogen_amd import capi
+    from cases import CASES
+
+    def create(c):
+        ctx = C.c_void_p()
+        return lib.eco_create(C.byref(capi.make_config(c)), None, C.byref(ctx))
+    c = CASES["kapila_1d_o2_transport_thinc"]
+    assert create(dict(c, precision="fp32s")) == -2
+    assert create(dict(CASES["euler_1d_o2"], transports=["color"])) == -1
+    assert create(dict(c, transports=["a", "b", "c", "d", "e"])) == -1
+    assert lib.eco_noalloc_bytes(C.byref(capi.make_config(dict(CASES["euler_1d_o2"], transports=["c"])))) == 0
+    buf = (C.c_double * 4)()
+    st = capi.EcoStatus()
+    rc = lib.eco_run_noalloc(C.byref(capi.make_config(c)), C.addressof(buf), C.addressof(buf) + 16,
+                             5, C.c_double(0.0), None, C.byref(st))
+    assert rc == -2
+    assert capi.nfields(c) == 3 * 2 + 4 + 2
