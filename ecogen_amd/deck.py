@@ -174,6 +174,22 @@ def _domain_mask(dom, X, Y, Z, where):
     raise DeckError(f"{where}: geometric domain type {kind!r} is not read")
 
 
+def _named_values(node, names, where, need_all):
+    """Values of the <transport name value> children of ``node`` matched by
+    name against the deck's transports (Input.cpp:771-790 for a state, where
+    a scalar not given is 0; BoundCondOutflow.cpp:58-79, BoundCondInj.cpp:
+    120-147, BoundCondTank.cpp:146-173 for a boundary, which needs them all)."""
+    vals = [0.0] * len(names)
+    found = 0
+    for tr in node.findall("transport") if node is not None else []:
+        if tr.get("name") in names:
+            vals[names.index(tr.get("name"))] = _f(tr, "value", where)
+            found += 1
+    if need_all and found < len(names):
+        raise DeckError(f"{where}: not enough transport equations")
+    return vals
+
+
 def _state_values(st, model, nph, where):
     """(alpha_k, rho_k, p_k per phase, u, v, w) of one <state>."""
     mats = st.findall("material")
@@ -289,10 +305,23 @@ def load(rundir):
     if any(r != "P" for r in relax):
         raise DeckError(f"modelV4.xml: relaxation {relax} (only P)")
     case["relax_p"] = model == "multip" and "P" in relax
+    # transport equations (Input.cpp:472-473, 528-540): numberTransports of
+    # the first <transport name> elements
+    ntr = int(float(fm.get("numberTransports", "0")))
+    names = [t.get("name", "") for t in model_x.findall("transport")]
+    if len(names) < ntr or any(not nm for nm in names[:ntr]):
+        raise DeckError(f"modelV4.xml: {len(names)} named transports for numberTransports={ntr}")
+    trs = names[:ntr]
+    if trs:
+        if model == "euler":
+            # Model::solveRiemannTransportIntern ends the run (src/Models/Model.h:127)
+            raise DeckError("modelV4.xml: the Euler model has no transport equations")
+        case["transports"] = trs
 
     # initial conditions (Input::entreeConditionsInitiales)
     states = {s.get("name"): s for s in ci.findall("state")}
     bcs = ["abs"] * 6
+    bc_tr = [None] * 6
     p0s = [0.0] * 6
     injs = [None] * 6
     tanks = [None] * 6
@@ -311,12 +340,17 @@ def load(rundir):
         elif kind == "outflow":  # BoundCondOutflow.cpp:41-52: <dataOutflow p0="..."/>
             bcs[num - 1] = "outflow"
             p0s[num - 1] = _f(_child(b, "dataOutflow", "boundCond outflow"), "p0", "boundCond outflow")
+            bc_tr[num - 1] = _named_values(b.find("dataOutflow"), trs, "boundCond outflow", True)
         elif kind == "injection":
             bcs[num - 1] = "injection"
             injs[num - 1] = _injection(b, case["eos"])
+            if trs:
+                bc_tr[num - 1] = _named_values(b.find("dataInjection"), trs, "boundCond injection", True)
         elif kind == "tank":
             bcs[num - 1] = "tank"
             tanks[num - 1] = _tank(b, case["eos"])
+            if trs:
+                bc_tr[num - 1] = _named_values(b.find("dataTank"), trs, "boundCond tank", True)
         else:
             raise DeckError(f"boundCond type {kind!r} is not read")
     case["bc"] = bcs
@@ -325,10 +359,14 @@ def load(rundir):
         case["bc_inj"] = injs
     if any(tanks):
         case["bc_tank"] = tanks
+    if any(v is not None for v in bc_tr):
+        case["bc_tr"] = bc_tr
 
     X, Y, Z = _centres(case["cells"], case["lengths"])
     Xg, Yg, Zg = X[None, None, :], Y[None, :, None], Z[:, None, None]
     nf = 3 * nph + 3 if model != "euler" else 6
+    if trs:
+        nf += 1 + len(trs)  # E (not read), then the scalars
     grid = np.zeros((nf, case["cells"][2], case["cells"][1], case["cells"][0]))
     filled = np.zeros(grid.shape[1:], dtype=bool)
     pdoms = ci.find("physicalDomains")
@@ -338,6 +376,8 @@ def load(rundir):
         if sname not in states:
             raise DeckError(f"{where}: unknown state {sname!r}")
         vals = _state_values(states[sname], model, nph, where)
+        if trs:
+            vals = vals + [0.0] + _named_values(states[sname], trs, where, False)
         m = _domain_mask(dom, Xg, Yg, Zg, where)
         for f, v in enumerate(vals):
             grid[f][m] = v
@@ -481,7 +521,13 @@ def resume_point(case, rundir):
     arrs = read_vtr(rdir / where)
     model, nph = case["model"], len(case["eos"])
     ncell = int(np.prod(case["cells"]))
-    state = np.zeros((3 * nph + 3, ncell))
+    ntr = len(case.get("transports", []))
+    state = np.zeros((3 * nph + 4 + ntr if ntr else 3 * nph + 3, ncell))
+    # the scalars T1, T2, ... (OutputXML::readResults, OutputXML.cpp:196-203)
+    for t in range(ntr):
+        if f"T{t + 1}" not in arrs:
+            raise DeckError(f"{where}: no array T{t + 1}")
+        state[3 * nph + 4 + t] = arrs[f"T{t + 1}"]
     if model == "euler":
         state[0] = 1.0
         state[1] = _array(arrs, "F0_Density_", where)

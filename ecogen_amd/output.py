@@ -144,6 +144,9 @@ def cell_data(case, state):
                     (f"F{k}_Mass fraction_{f}", 1, Y), (f"F{k}_Pressure_{f}", 1, p)]
     vel = np.stack([state[3 * nph], state[3 * nph + 1], state[3 * nph + 2]], axis=1)
     out += [("Density_Mixture", 1, rho_m), ("Pressure_Mixture", 1, p_m), ("Velocity_Mixture", 3, vel.reshape(-1))]
+    # transported scalars T1, T2, ... (OutputXML.cpp:435-447), stored after E
+    for t in range(len(case.get("transports", []))):
+        out.append((f"T{t + 1}", 1, state[3 * nph + 4 + t]))
     out.append(("gradRho", 1, grad_rho(case, rho_m)))
     return out
 
@@ -205,6 +208,8 @@ def _gnu_columns(case, state):
         if name == "gradRho":
             continue
         title = name.split("_", 1)[1] if name[:1] == "F" and name[1:2].isdigit() else name
+        if name[:1] == "T" and name[1:].isdigit():
+            title = f"Transport{name[1:]}"  # OutputGNU.cpp:140-145
         if nc > 1:
             v = np.asarray(vals).reshape(-1, nc)
             vals = np.sqrt(v[:, 0] * v[:, 0] + v[:, 1] * v[:, 1] + v[:, 2] * v[:, 2])
@@ -245,7 +250,8 @@ def write_gnu_script(path, case, nfiles, dim=None, stem="result"):
     if dim == 2:
         out += ["set surface", "set dgrid3d 50,50", "set contour base", "set cntrparam levels 25", "show contour"]
     index = 2 if dim == 0 else 1 + dim
-    for title, _ in _gnu_columns(case, np.ones((3 * len(case["eos"]) + 4, int(np.prod(case["cells"]))))):
+    nf = 3 * len(case["eos"]) + 4 + len(case.get("transports", []))
+    for title, _ in _gnu_columns(case, np.ones((nf, int(np.prod(case["cells"]))))):
         out.append(f"set title '{title}'")
         if dim == 0:
             out.append(f'plot "{stem}.out" u 1:{index}')

@@ -22,7 +22,7 @@ HERE = Path(__file__).resolve().parent
 REPO = HERE.parent.parent
 sys.path.insert(0, str(HERE))
 sys.path.insert(0, str(REPO / "oracle"))
-from cases import CASES, _bubbles_3d, _euler_1d, _shock_tube_1d  # noqa: E402
+from cases import CASES, _bubbles_3d, _euler_1d, _shock_tube_1d, _tank, _with_transports  # noqa: E402
 from deck import write_deck  # noqa: E402
 
 REF_BIN = REPO / "oracle" / "_ref" / "ECOGEN"
@@ -35,11 +35,20 @@ FIXTURES = {
     "kapila_3d_o2_6": _bubbles_3d(n=6, steps=3, nb=1, model="kapila", relax=False, limiters=("vanleer",) * 4),
     "kapila_1d_fo_outflow_30": dict(_shock_tube_1d("kapila", 1, ["none"] * 4, 20, cells=30),
                                     bc=["outflow", "outflow"] + ["abs"] * 4, bc_p0=[2e9, 5e4] + [0.0] * 4),
+    # transported scalars: T1, T2 arrays (OutputXML.cpp:435-447), Transport<k> columns
+    "kapila_1d_o2_transport_30": _with_transports(
+        dict(_shock_tube_1d("kapila", 2, ["minmod", "vanleer", "minmod", "vanleer"], 15, cells=30),
+             bc=["wall", "tank"] + ["abs"] * 4, bc_tank=_tank(2, 3e5, 300.0, 1)),
+        ["color", "dye"], {"LP": [1.0, 0.5], "HP": [0.0, 2.0]}, bc_tr=[None, [5.0, -1.0]] + [None] * 4),
+    "multip_2d_fo_transport_6x4": _with_transports(
+        dict(CASES["multip_2d_fo_relax"], cells=[6, 4, 1], steps=3), ["color"], {"water": [1.0], "shocked": [0.25]}),
 }
 
 
-def main():
+def main(argv):
     for name, case in FIXTURES.items():
+        if argv[1:] and name not in argv[1:]:
+            continue
         tmp = Path(tempfile.mkdtemp(prefix="ecogen_vtk_"))
         try:
             write_deck(dict(case, name=name), tmp)
@@ -77,4 +86,4 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv)

@@ -32,6 +32,9 @@ def test_deck_reads_back_the_reference_inputs(name, tmp_path):
             assert got["bc_inj"][i] == case["bc_inj"][i]
         if kind == "tank":
             assert got["bc_tank"][i] == case["bc_tank"][i]
+        if kind in ("outflow", "injection", "tank") and case.get("transports"):
+            assert got["bc_tr"][i] == case["bc_tr"][i]
+    assert got.get("transports", []) == case.get("transports", [])
     for a, b in zip(got["eos"], case["eos"]):
         for k in ("type", "gamma", "cv"):
             assert a[k] == b[k]
@@ -81,11 +84,28 @@ def test_deck_refusals(tmp_path):
         '<?xml version="1.0"?><mesh><type structure="unstructured"/></mesh>')
     with pytest.raises(deck.DeckError, match="cartesian"):
         deck.load(tmp_path / "a")
+    # transports: the Euler model has none (Model.h:127); numberTransports
+    # needs as many named <transport> elements (Input.cpp:528-540); an outflow
+    # side needs a value for each (BoundCondOutflow.cpp:79)
+    eu, _ = load_golden("euler_1d_o2")
+    write_deck(dict(eu, name="x", transports=["c"], states=[dict(s, tr=[1.0]) for s in eu["states"]]), tmp_path / "g")
+    with pytest.raises(deck.DeckError, match="Euler"):
+        deck.load(tmp_path / "g")
+    tr, _ = load_golden("kapila_1d_fo_transport_outflow")
+    write_deck(dict(tr, name="x"), tmp_path / "h")
+    mx = tmp_path / "h" / "case" / "modelV4.xml"
+    mx.write_text(mx.read_text().replace('numberTransports="1"', 'numberTransports="2"'))
+    with pytest.raises(deck.DeckError, match="numberTransports"):
+        deck.load(tmp_path / "h")
+    write_deck(dict(tr, name="x", bc_tr=[[4.0], None] + [None] * 4), tmp_path / "i")
+    with pytest.raises(deck.DeckError, match="not enough transport"):
+        deck.load(tmp_path / "i")
 
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("name", ["c1_kapila_fo_wall", "kapila_2d_fo_wall", "multip_1d_o2_relax",
-                                  "kapila_1d_o2_injection", "multip_1d_o2_relax_tank"])
+                                  "kapila_1d_o2_injection", "multip_1d_o2_relax_tank",
+                                  "kapila_1d_o2_transport_wall_tank", "multip_2d_fo_relax_transport"])
 def test_deck_runs_on_the_device(name, tmp_path):
     """python -m ecogen_amd.deck on a reference deck: the final state matches
     the reference's output (1e-12 relative L1, the parity bar)."""
@@ -110,7 +130,7 @@ def test_deck_runs_on_the_device(name, tmp_path):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", ["kapila_2d_fo_wall_6x4", "euler_1d_fo_30"])
+@pytest.mark.parametrize("name", ["kapila_2d_fo_wall_6x4", "euler_1d_fo_30", "kapila_1d_o2_transport_30"])
 def test_deck_vtk_results_match_reference_files(name, tmp_path):
     """python -m ecogen_amd.deck --vtk writes the reference's result files:
     the collection file character for character, and the last result file
