@@ -80,7 +80,9 @@ eco_config lowerConfig(const Model &model, const std::vector<std::unique_ptr<Eos
     c.bc_tank_p0[b] = mesh.tank[b].p0;
     c.bc_tank_T0[b] = mesh.tank[b].T0;
     c.bc_tank_massfrac[b] = mesh.tank[b].massFractions ? 1 : 0;
+    for (int t = 0; t < ECO_MAX_TRANSPORTS; t++) c.bc_tr[b][t] = mesh.transportValue[b][t];
   }
+  c.ntransport = model.getNumberTransports();
   for (size_t k = 0; k < eos.size() && k < 4; k++) c.eos[k] = eos[k]->lower();
   return c;
 }
@@ -101,7 +103,7 @@ Run::~Run() { eco_destroy(m_ctx); }
 long long Run::numberCells() const { return (long long)m_cfg.n[0] * m_cfg.n[1] * m_cfg.n[2]; }
 
 void Run::setState(const std::vector<double> &state) {
-  if ((long long)state.size() < ECO_NFIELDS(m_cfg.nphase) * numberCells())
+  if ((long long)state.size() < ECO_NFIELDS_TR(m_cfg.nphase, m_cfg.ntransport) * numberCells())
     throw EcoError(ECO_ERR_INVALID, "Run::setState");
   int rc = eco_set_state(m_ctx, state.data());
   if (rc) throw EcoError(rc, "eco_set_state");
@@ -113,7 +115,7 @@ void Run::resumeClock(double physicalTime, double dt) {
 }
 
 std::vector<double> Run::getState() {
-  std::vector<double> s((size_t)ECO_NFIELDS(m_cfg.nphase) * numberCells());
+  std::vector<double> s((size_t)ECO_NFIELDS_TR(m_cfg.nphase, m_cfg.ntransport) * numberCells());
   int rc = eco_get_state(m_ctx, s.data());
   if (rc) throw EcoError(rc, "eco_get_state");
   return s;

@@ -118,11 +118,16 @@ class Model {
   virtual int kind() const = 0;
   virtual const Flux &flux() const = 0;
   int getNumberPhases() const { return m_numberPhases; }
+  // transport equations of <flowModel numberTransports> (Model(name,
+  // numberTransports), src/Models/Model.cpp:45-50; Kapila and MultiP)
+  int getNumberTransports() const { return m_numberTransports; }
+  void setNumberTransports(int n) { m_numberTransports = n; }
   std::vector<std::unique_ptr<Relaxation>> *getRelaxations() { return &m_relaxations; }
   const std::vector<std::unique_ptr<Relaxation>> &relaxations() const { return m_relaxations; }
  protected:
   explicit Model(int nph) : m_numberPhases(nph) {}
   int m_numberPhases;
+  int m_numberTransports = 0;
   std::vector<std::unique_ptr<Relaxation>> m_relaxations;
 };
 class ModEuler : public Model {
@@ -180,6 +185,9 @@ struct CartesianMesh {
     double fraction[4] = {1., 0., 0., 0.};
     bool massFractions = false;
   } tank[6];
+  // <transport value> of each scalar entering through an outflow, injection
+  // or tank side (m_valueTransport of BoundCondOutflow/Inj/Tank)
+  double transportValue[6][ECO_MAX_TRANSPORTS] = {};
 };
 
 struct RunStatus {
@@ -205,7 +213,7 @@ class Run {
 
   const eco_config &config() const { return m_cfg; }
   long long numberCells() const;
-  // state in the ecogen.h host layout
+  // state in the ecogen.h host layout (transported scalars after E)
   void setState(const std::vector<double> &state);
   // Run::resumeSimulation's clock: after setState, start at physicalTime and
   // take dt for the first step (both read from infoCalcul.out)

@@ -60,6 +60,24 @@ static int check_lower() {
   mp.getRelaxations()->emplace_back(new RelaxationP);
   c = lowerConfig(mp, eos, m, 0.5, 2, l, ECO_FP32S);
   CHECK(c.relax_p == 1);
+  // transport equations: the model's count and the sides' entering values
+  ModKapila kt(2, false);
+  kt.setNumberTransports(2);
+  m.boundCond[1] = ECO_BC_OUTFLOW;
+  m.transportValue[1][1] = 7.;
+  c = lowerConfig(kt, eos, m, 0.8, 2, l, ECO_FP64);
+  CHECK(c.ntransport == 2 && c.bc_tr[1][1] == 7. && c.bc_tr[1][0] == 0.);
+  // the Euler model has none (Model.h:127): refused before any device call
+  try {
+    std::unique_ptr<Model> eu(new ModEuler);
+    eu->setNumberTransports(1);
+    std::vector<std::unique_ptr<Eos>> e1;
+    e1.emplace_back(new EosIG(1.4, 800.));
+    Run r(std::move(eu), std::move(e1), m, 0.8, 1);
+    CHECK(false);
+  } catch (const EcoError &e) {
+    CHECK(e.code() == ECO_ERR_INVALID);
+  }
   // conservative variables in accumulation order
   std::vector<std::string> v = kap.flux().variables(2, 2);
   CHECK(v.size() == 9 && v.front() == "alpha_0" && v.back() == "rho_E");
