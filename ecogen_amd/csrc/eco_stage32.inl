@@ -8,9 +8,10 @@
 // stays self-consistent after rounding and the mixture pressure recovered by
 // the energy correction keeps its accuracy.
 //
-// Compute: wave-speed estimates in FP32; slopes, limiters, extrapolation,
-// every flux component, the accumulation, buildPrim, the stiff pressure
-// relaxation (Newton) and the energy correction in FP64.
+// Compute: wave-speed estimates, the face slopes, the limiters and the
+// extrapolation increments in FP32 (each increment added to the cell's
+// decoded FP64 value); every flux component, the accumulation, buildPrim,
+// the stiff pressure relaxation (Newton) and the energy correction in FP64.
 #pragma once
 #include "eco_stage.inl"
 
