@@ -13,7 +13,7 @@ import numpy as np
 
 LIB_PATH = Path(__file__).resolve().parent / "libecogen_amd.so"
 
-MODEL = {"euler": 0, "kapila": 1, "multip": 2}
+MODEL = {"euler": 0, "kapila": 1, "multip": 2, "thermaleq": 3}
 LIMITER = {"none": 0, "minmod": 1, "vanleer": 2, "vanalbada": 3, "superbee": 4, "mc": 5, "thinc": 6}
 EOS_TYPE = {"IG": 0, "SG": 1}
 PRECISION = {"fp64": 0, "fp32s": 1}
@@ -172,8 +172,9 @@ def make_config(case) -> EcoConfig:
 
 
 def nfields(case) -> int:
-    """Fields of a host state: per phase alpha, rho, p; u, v, w; E; the transports."""
-    return 3 * len(case["eos"]) + 4 + len(case.get("transports", []))
+    """Fields of a host state: per phase alpha, rho, p; u, v, w; E; the
+    transports; for the thermal-equilibrium model the mixture temperature last."""
+    return 3 * len(case["eos"]) + 4 + len(case.get("transports", [])) + (case["model"] == "thermaleq")
 
 
 class Context:

@@ -164,7 +164,7 @@ static void lower_injection(const eco_config &c, int b, SchemeParams &S) {
 
 int eco::validate_config(const eco_config *c) {
   if (!c) return ECO_ERR_INVALID;
-  if (c->model < 0 || c->model > 2) return ECO_ERR_INVALID;
+  if (c->model < 0 || c->model > 3) return ECO_ERR_INVALID;
   if (c->order != 1 && c->order != 2) return ECO_ERR_INVALID;
   if (c->cfl <= 0.) return ECO_ERR_INVALID;
   for (int d = 0; d < 3; d++) if (c->n[d] < 1 || !(c->L[d] > 0.)) return ECO_ERR_INVALID;
@@ -205,6 +205,17 @@ int eco::validate_config(const eco_config *c) {
   }
   if (c->model == ECO_MODEL_EULER) {
     if (c->nphase != 1) return ECO_ERR_INVALID;
+  } else if (c->model == ECO_MODEL_THERMALEQ) {
+    // two phases (MixThermalEq::computePressure, MixThermalEq.cpp:123); no
+    // transport or injection solver (Model.h:127-135 end the run); the tank
+    // (ModThermalEq::solveRiemannTank) and FP32 storage are not provided
+    if (c->nphase != 2) return ECO_ERR_INVALID;
+    if (c->ntransport > 0) return ECO_ERR_INVALID;
+    for (int i = 0; i < 6; i++) {
+      if (c->bc[i] == ECO_BC_INJECTION) return ECO_ERR_INVALID;
+      if (c->bc[i] == ECO_BC_TANK) return ECO_ERR_UNSUPPORTED;
+    }
+    if (c->precision != ECO_FP64) return ECO_ERR_UNSUPPORTED;
   } else {
     if (c->nphase < 2 || c->nphase > 4) return ECO_ERR_INVALID;
     if (c->nphase > kMaxPhases) return ECO_ERR_UNSUPPORTED;
@@ -301,8 +312,10 @@ void eco::lower_config(const eco_config &c, Lowered &L, bool with_tables) {
   for (int k = 0; k < c.nphase && k < kMaxPhases; k++) lower_eos(c.eos[k], S.eos[k]);
   for (int b = 0; b < 6; b++) if (c.bc[b] == ECO_BC_INJECTION) lower_injection(c, b, S);
   for (int b = 0; b < 6; b++) if (c.bc[b] == ECO_BC_TANK) lower_tank(c, b, S);
-  L.nprim = (c.model == ECO_MODEL_EULER) ? 2 + dim : 3 * c.nphase + dim;
+  L.nprim = (c.model == ECO_MODEL_EULER) ? 2 + dim : (c.model == ECO_MODEL_THERMALEQ) ? c.nphase + 2 + dim : 3 * c.nphase + dim;
   S.ncache = (c.model == ECO_MODEL_MULTIP && c.relax_p && c.precision == ECO_FP64) ? (c.order == 1 ? c.nphase + 3 : 1) : 0;
+  // ThermalEq first order: the cells' cached mixture energy
+  if (c.model == ECO_MODEL_THERMALEQ && c.order == 1) S.ncache = 1;
   L.nlevel = L.nprim + 1 + S.ncache;
   L.ntr = c.ntransport;
   L.es = (c.precision == ECO_FP32S) ? sizeof(float) : sizeof(double);
@@ -320,7 +333,13 @@ void eco::bind_tables(const eco_config &c, Lowered &L, double *dev_tables) {
 // host layout (3N+4 fields, u,v,w) -> device layout (nprim fields)
 void eco::host_field_map(const eco_config &c, int dim, std::vector<int> &map) {
   map.clear();
-  if (c.model == ECO_MODEL_EULER) {
+  if (c.model == ECO_MODEL_THERMALEQ) {
+    // alpha_k, the mixture pressure (p_0), T (after E), velocity
+    for (int k = 0; k < c.nphase; k++) map.push_back(3 * k);
+    map.push_back(2);
+    map.push_back(ECO_NFIELDS(c.nphase));
+    for (int d = 0; d < dim; d++) map.push_back(3 * c.nphase + d);
+  } else if (c.model == ECO_MODEL_EULER) {
     map.push_back(1); map.push_back(2);
     for (int d = 0; d < dim; d++) map.push_back(3 + d);
   } else {
@@ -565,7 +584,7 @@ extern "C" int eco_get_state(eco_ctx *x, double *host_state) {
   std::vector<int> map;
   host_to_dev_fields(x, map);
   int nf = ECO_NFIELDS(c.nphase);
-  std::memset(host_state, 0, sizeof(double) * (nf + x->ntr) * nc);
+  std::memset(host_state, 0, sizeof(double) * ECO_NFIELDS_CFG(c.model, c.nphase, x->ntr) * nc);
   hipError_t e = hipSuccess;
   for (size_t f = 0; f < map.size() && e == hipSuccess; f++)
     e = hipMemcpyAsync(host_state + (size_t)map[f] * nc, src + f * nc, nc * sizeof(double), hipMemcpyDeviceToHost, x->stream);
@@ -578,6 +597,20 @@ extern "C" int eco_get_state(eco_ctx *x, double *host_state) {
   if (e == hipSuccess) e = hipStreamSynchronize(x->stream);
   if (e != hipSuccess) return ECO_ERR_HIP;
   if (c.model == ECO_MODEL_EULER) for (size_t i = 0; i < nc; i++) host_state[i] = 1.;
+  if (c.model == ECO_MODEL_THERMALEQ) {
+    // every phase at the mixture pressure, rho_k = rho_k(p, T) as
+    // Eos::computeDensity (EosSG.cpp:114-117, EosIG.cpp:111-114)
+    const double *p = host_state + 2 * nc, *T = host_state + (size_t)nf * nc;
+    for (int k = 0; k < c.nphase; k++) {
+      const eco_eos &q = c.eos[k];
+      double *rk = host_state + (size_t)(3 * k + 1) * nc, *pk = host_state + (size_t)(3 * k + 2) * nc;
+      for (size_t i = 0; i < nc; i++) {
+        pk[i] = p[i];
+        const double den = (q.gamma - 1.) * q.cv * T[i];
+        rk[i] = ((q.type == ECO_EOS_SG) ? p[i] + q.pinf : p[i]) / ((den < 0.) ? 0. : den);
+      }
+    }
+  }
   return ECO_OK;
 }
 

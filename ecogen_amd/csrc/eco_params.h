@@ -5,7 +5,7 @@
 
 namespace eco {
 
-enum ModelKind : int { MODEL_EULER = 0, MODEL_KAPILA = 1, MODEL_MULTIP = 2 };
+enum ModelKind : int { MODEL_EULER = 0, MODEL_KAPILA = 1, MODEL_MULTIP = 2, MODEL_THERMALEQ = 3 };
 enum LimiterKind : int { LIM_NONE = 0, LIM_MINMOD = 1, LIM_VANLEER = 2, LIM_VANALBADA = 3,
                          LIM_SUPERBEE = 4, LIM_MC = 5, LIM_THINC = 6 };
 enum EosKind : int { EOS_IG = 0, EOS_SG = 1 };

@@ -153,6 +153,9 @@ ECO_D double limiter(int lim, double s1, double s2) {
 // Multiphase models: per phase alpha, rho, p (+ derived e, c, Y); mixture
 // velocity u[3] and total energy E, mixture rho, p, e, frozen sound speed cF.
 // Euler: phase 0 carries rho, p, e, c; u[3]; E is the phase total energy.
+// ThermalEq: alpha_k and the mixture pressure pm and temperature T are the
+// primitives; rho_k = rho_k(pm, T), p_k = pm, and cF is the mixture sound
+// speed sqrt(sum Y_k c_k^2) (MixThermalEq).
 template <int NPH>
 struct CellState {
   double a[NPH], r[NPH], p[NPH];
@@ -160,6 +163,7 @@ struct CellState {
   double u[3];
   double E;
   double rho, pm, em, cF;
+  double T;
 };
 
 template <int NPH>
@@ -174,6 +178,7 @@ ECO_D double sqn(const double *u) { return u[0] * u[0] + u[1] * u[1] + u[2] * u[
 
 // PhaseKapila::verifyAndCorrectPhase (PhaseKapila.cpp:294-307),
 // PhaseMultiP::verifyAndCorrectPhase (PhaseMultiP.cpp:288-294),
+// PhaseThermalEq::verifyAndCorrectPhase,
 // PhaseEuler::verifyAndCorrectPhase (PhaseEuler.cpp:349-353)
 template <int MODEL, int NPH>
 ECO_D void verify_correct(const SchemeParams &S, CellState<NPH> &s, int k) {
@@ -191,10 +196,51 @@ ECO_D void verify_correct(const SchemeParams &S, CellState<NPH> &s, int k) {
     if (s.a[k] < 1e-10) s.a[k] = 1e-9;
     if (s.a[k] > 1. - 1e-10) s.a[k] = 1. - 1e-9;
     if (s.r[k] < 1.e-10) s.r[k] = 1.e-9;
+  } else if constexpr (MODEL == MODEL_THERMALEQ) {
+    // PhaseThermalEq::verifyAndCorrectPhase (PhaseThermalEq.cpp:240-246)
+    if (s.a[k] < 1e-10) s.a[k] = 1e-10;
+    if (s.a[k] > 1. - 1e-10) s.a[k] = 1. - 1e-10;
+    if (s.r[k] < 1.e-10) s.r[k] = 1.e-10;
   } else {
     if (s.r[k] < 1.e-10) s.r[k] = 1.e-10;
   }
   s.p[k] = eos_fix_p(S.eos[k], s.p[k]);
+}
+
+// ThermalEq: the stiffened-gas offset of a phase (Eos::getPInf is 0 for
+// EosIG, Eos.h:149) and its density at (p, T) (computeDensity, EosSG.cpp:
+// 114-117, EosIG.cpp:111-114)
+ECO_D double te_pinf(const EosParams &q) { return (q.type == EOS_SG) ? q.pinf : 0.; }
+ECO_D double eos_rho_pt(const EosParams &q, double p, double T, double eps) {
+  return ((q.type == EOS_SG) ? p + q.pinf : p) / smax((q.gamma - 1.) * q.cv * T, eps);
+}
+
+// ThermalEq phases at the mixture pressure and temperature
+// (ModThermalEq::fulfillState, ModThermalEq.cpp:78-89, with
+// PhaseThermalEq::extendedCalculusPhase, PhaseThermalEq.cpp:73-78), then
+// MixThermalEq::computeMixtureVariables (MixThermalEq.cpp:305-323)
+template <int NPH>
+ECO_D void te_phases(const SchemeParams &S, CellState<NPH> &s) {
+#pragma unroll
+  for (int k = 0; k < NPH; k++) {
+    s.p[k] = s.pm;
+    s.r[k] = eos_rho_pt(S.eos[k], s.pm, s.T, S.eps);
+    s.e[k] = eos_e(S.eos[k], s.r[k], s.p[k], S.eps);
+    s.c[k] = eos_c(S.eos[k], s.r[k], s.p[k], S.eps);
+  }
+}
+template <int NPH>
+ECO_D void te_mixture(CellState<NPH> &s) {
+  s.rho = 0.;
+#pragma unroll
+  for (int k = 0; k < NPH; k++) s.rho += s.a[k] * s.r[k];
+  const double ird = 1. / s.rho;
+#pragma unroll
+  for (int k = 0; k < NPH; k++) s.Y[k] = qdiv(s.a[k] * s.r[k], s.rho, ird);
+  s.em = 0.; s.cF = 0.;
+#pragma unroll
+  for (int k = 0; k < NPH; k++) { s.em += s.Y[k] * s.e[k]; s.cF += s.Y[k] * s.c[k] * s.c[k]; }
+  s.cF = sqrt(s.cF);
 }
 
 // Model::fulfillState: extendedCalculusPhase (energy, sound speed) and
@@ -203,7 +249,10 @@ ECO_D void verify_correct(const SchemeParams &S, CellState<NPH> &s, int k) {
 template <int MODEL, int NPH>
 ECO_D void fulfill(const SchemeParams &S, CellState<NPH> &s) {
   const double eps = S.eps;
-  if constexpr (MODEL == MODEL_EULER) {
+  if constexpr (MODEL == MODEL_THERMALEQ) {
+    te_phases<NPH>(S, s);
+    te_mixture<NPH>(s);
+  } else if constexpr (MODEL == MODEL_EULER) {
     s.e[0] = eos_e(S.eos[0], s.r[0], s.p[0], eps);
     s.c[0] = eos_c(S.eos[0], s.r[0], s.p[0], eps);
     s.E = s.e[0] + 0.5 * sqn(s.u);
@@ -232,7 +281,14 @@ ECO_D void fulfill(const SchemeParams &S, CellState<NPH> &s) {
 // Needs s.rho (mixture density) for the multiphase models.
 template <int MODEL, int NPH>
 ECO_D void build_cons(const SchemeParams &S, const CellState<NPH> &s, FluxVec<NPH> &f) {
-  if constexpr (MODEL == MODEL_EULER) {
+  if constexpr (MODEL == MODEL_THERMALEQ) {
+    // FluxThermalEq::buildCons (FluxThermalEq.cpp:113-126)
+#pragma unroll
+    for (int k = 0; k < NPH; k++) f.m[k] = s.a[k] * s.r[k];
+#pragma unroll
+    for (int d = 0; d < 3; d++) f.q[d] = s.rho * s.u[d];
+    f.E = s.rho * s.E;
+  } else if constexpr (MODEL == MODEL_EULER) {
     f.m[0] = s.r[0];
 #pragma unroll
     for (int d = 0; d < 3; d++) f.q[d] = s.u[d] * f.m[0];
@@ -262,7 +318,42 @@ ECO_D void zero_small(double *u) {
 template <int MODEL, int NPH>
 ECO_D void build_prim(const SchemeParams &S, FluxVec<NPH> &f, CellState<NPH> &s) {
   const double eps = S.eps;
-  if constexpr (MODEL == MODEL_EULER) {
+  if constexpr (MODEL == MODEL_THERMALEQ) {
+    // FluxThermalEq::buildPrim (FluxThermalEq.cpp:130-162) with the
+    // closed-form equilibrium pressure and temperature of two phases
+    // (MixThermalEq::computePressure / computeTemperature, MixThermalEq.cpp:
+    // 121-178, Le Martelot 2013)
+    double rhoMel = 0.;
+#pragma unroll
+    for (int k = 0; k < NPH; k++) rhoMel += f.m[k];
+    const double irm = 1. / rhoMel;
+#pragma unroll
+    for (int d = 0; d < 3; d++) s.u[d] = qdiv(f.q[d], rhoMel, irm);
+    zero_small(s.u);
+    const double ein = qdiv(f.E, rhoMel, irm) - 0.5 * (s.u[0] * s.u[0] + s.u[1] * s.u[1] + s.u[2] * s.u[2]);
+    const EosParams &q1 = S.eos[0], &q2 = S.eos[1];
+    const double pInf1 = te_pinf(q1), pInf2 = te_pinf(q2);
+    const double Y1 = qdiv(f.m[0], rhoMel, irm), Y2 = qdiv(f.m[1], rhoMel, irm);
+    const double qm = Y1 * q1.eref + Y2 * q2.eref;
+    const double cvMel = Y1 * q1.cv + Y2 * q2.cv;
+    const double A1 = Y1 * (q1.gamma - 1.) * q1.cv / cvMel * (rhoMel * (ein - qm) - pInf1);
+    const double A2 = Y2 * (q2.gamma - 1.) * q2.cv / cvMel * (rhoMel * (ein - qm) - pInf2);
+    const double p = 0.5 * (A1 + A2 - (pInf1 + pInf2)) +
+                     sqrt(0.25 * (A2 - A1 - (pInf2 - pInf1)) * (A2 - A1 - (pInf2 - pInf1)) + A1 * A2);
+    double T = 0.;
+#pragma unroll
+    for (int k = 0; k < NPH; k++) {
+      const double Yk = qdiv(f.m[k], rhoMel, irm);
+      T += Yk * (S.eos[k].gamma - 1.) * S.eos[k].cv / (p + te_pinf(S.eos[k]));
+    }
+    s.T = 1. / (T * rhoMel);
+    s.pm = p;
+    te_phases<NPH>(S, s);
+#pragma unroll
+    for (int k = 0; k < NPH; k++) s.a[k] = f.m[k] / s.r[k];
+    te_mixture<NPH>(s);
+    s.E = qdiv(f.E, rhoMel, irm);
+  } else if constexpr (MODEL == MODEL_EULER) {
     s.r[0] = f.m[0];
     const double irm = 1. / f.m[0];
 #pragma unroll
@@ -349,6 +440,9 @@ template <int MODEL, int NPH>
 ECO_D void scale_add(FluxVec<NPH> &U, double mult, const FluxVec<NPH> &buf) {
   if constexpr (MODEL == MODEL_EULER) {
     U.m[0] *= mult;
+  } else if constexpr (MODEL == MODEL_THERMALEQ) {
+#pragma unroll
+    for (int k = 0; k < NPH; k++) U.m[k] *= mult;
   } else {
 #pragma unroll
     for (int k = 0; k < NPH; k++) { U.a[k] *= mult; U.m[k] *= mult; U.en[k] *= mult; }
@@ -358,6 +452,9 @@ ECO_D void scale_add(FluxVec<NPH> &U, double mult, const FluxVec<NPH> &buf) {
   U.E *= mult;
   if constexpr (MODEL == MODEL_EULER) {
     U.m[0] += 1. * buf.m[0];
+  } else if constexpr (MODEL == MODEL_THERMALEQ) {
+#pragma unroll
+    for (int k = 0; k < NPH; k++) U.m[k] += 1. * buf.m[k];
   } else {
 #pragma unroll
     for (int k = 0; k < NPH; k++) { U.a[k] += 1. * buf.a[k]; U.m[k] += 1. * buf.m[k]; U.en[k] += 1. * buf.en[k]; }
@@ -462,13 +559,60 @@ ECO_D double eos_rho_hugoniot(const EosParams &q, double p0, double rho0, double
 // wave), ModEuler::solveRiemannOutflow (ModEuler.cpp:427-475).  Where the
 // MultiP form leaves mL unset (|v* - 1/rhoL| <= 1e-10) it is rhoL cL, as in
 // the Kapila form.  Returns |sL| when above the 1e-3 guard, else 0.
+// MixThermalEq::computeTemperatureIsentrope / computeVolumeIsentrope
+// (MixThermalEq.cpp:200-226, 256-277): the mixture isentrope at fixed mass
+// fractions Y from (p0, T0) to p, as a specific volume
+template <int NPH>
+ECO_D double te_v_isentrope(const SchemeParams &S, const double *Y, double p0, double T0, double p) {
+  double T = T0, cM = 0.;
+#pragma unroll
+  for (int k = 0; k < NPH; k++) cM += Y[k] * S.eos[k].gamma * S.eos[k].cv;
+#pragma unroll
+  for (int k = 0; k < NPH; k++) {
+    const EosParams &q = S.eos[k];
+    const double puissance = (q.gamma - 1.) * Y[k] * q.cv / cM;
+    T *= pow_pos((p + te_pinf(q)) / ((p0 + te_pinf(q))), puissance);
+  }
+  double v = 0.;
+#pragma unroll
+  for (int k = 0; k < NPH; k++) {
+    const EosParams &q = S.eos[k];
+    v += Y[k] * (((q.gamma - 1.) * q.cv * T) / (p + te_pinf(q)));
+  }
+  return v;
+}
+
 template <int MODEL, int NPH>
 ECO_D double outflow_flux(const SchemeParams &S, const CellState<NPH> &L, double p0, FluxVec<NPH> &F) {
   const double eps = S.eps;
 #pragma unroll
   for (int k = 0; k < NPH; k++) { F.a[k] = 0.; F.m[k] = 0.; F.en[k] = 0.; }
   double sL;
-  if constexpr (MODEL == MODEL_EULER) {
+  if constexpr (MODEL == MODEL_THERMALEQ) {
+    // ModThermalEq::solveRiemannOutflow (ModThermalEq.cpp:372-424)
+    const double uL = L.u[0], cL = L.cF, pL = L.pm, rhoL = L.rho, TL = L.T, vL = L.u[1], wL = L.u[2];
+    double Y[NPH];
+#pragma unroll
+    for (int k = 0; k < NPH; k++) Y[k] = L.a[k] * L.r[k] / rhoL;
+    const double zL = rhoL * cL;
+    double pStar = p0;
+    double rhoStar = 1. / te_v_isentrope<NPH>(S, Y, pL, TL, p0);
+    const double vmv0 = 1. / rhoStar - 1. / rhoL;
+    double mL = zL;
+    if (fabs(vmv0) > 1e-10) mL = sqrt((pL - p0) / vmv0);
+    sL = uL - mL / rhoL;
+    double uStar = uL + mL * vmv0;
+    if (sL >= 0.) { uStar = uL; pStar = pL; rhoStar = rhoL; }
+    const double EStar = (L.em + 0.5 * sqn(L.u)) + (uStar - uL) * (uStar - pL / mL);
+#pragma unroll
+    for (int k = 0; k < NPH; k++) F.m[k] = rhoStar * Y[k] * uStar;
+    F.q[0] = uStar * uStar * rhoStar + pStar;
+    F.q[1] = uStar * vL * rhoStar;
+    F.q[2] = uStar * wL * rhoStar;
+    F.E = (EStar * rhoStar + pStar) * uStar;
+    F.sM = uStar;
+    (void)eps;
+  } else if constexpr (MODEL == MODEL_EULER) {
     const double uL = L.u[0], vL = L.u[1], wL = L.u[2], pL = L.p[0], rhoL = L.r[0], cL = L.c[0];
     const double zL = rhoL * cL;
     sL = uL - cL;
@@ -828,10 +972,61 @@ ECO_D double wall_flux(const CellState<NPH> &L, FluxVec<NPH> &F) {
   return (fabs(sL) > 1.e-3) ? fabs(sL) : 0.;
 }
 
+// ModThermalEq::solveRiemannIntern (ModThermalEq.cpp:94-210): the mixture
+// HLLC carrying the phases' partial densities.
+template <int NPH>
+ECO_D double hllc_te(const CellState<NPH> &L, const CellState<NPH> &R, FluxVec<NPH> &F) {
+  double uL = L.u[0], cL = L.cF, pL = L.pm, rhoL = L.rho;
+  double uR = R.u[0], cR = R.cF, pR = R.pm, rhoR = R.rho;
+  double sL = smin(uL - cL, uR - cR);
+  double sR = smax(uR + cR, uL + cL);
+  double smx = 0.;
+  if (fabs(sL) > 1.e-3) smx = fabs(sL);
+  if (fabs(sR) > 1.e-3) smx = smax(smx, fabs(sR));
+  double mL = rhoL * (sL - uL), mR = rhoR * (sR - uR);
+  double sM = (pR - pL + mL * uL - mR * uR) / (mL - mR);
+  if (fabs(sM) < 1.e-8) sM = 0.;
+  if (sL >= 0. || sR <= 0.) {
+    const bool left = (sL >= 0.);
+    const CellState<NPH> &X = left ? L : R;
+    double uS = left ? uL : uR, pS = left ? pL : pR, rhoS = left ? rhoL : rhoR;
+#pragma unroll
+    for (int k = 0; k < NPH; k++) F.m[k] = X.a[k] * X.r[k] * uS;
+    double Et = X.em + 0.5 * sqn(X.u);
+    F.q[0] = rhoS * uS * uS + pS;
+    F.q[1] = rhoS * X.u[1] * uS;
+    F.q[2] = rhoS * X.u[2] * uS;
+    F.E = (rhoS * Et + pS) * uS;
+  } else {
+    const bool left = (sM >= 0.);
+    const CellState<NPH> &X = left ? L : R;
+    double uS = left ? uL : uR, pS = left ? pL : pR, mS = left ? mL : mR, sS = left ? sL : sR;
+    double Et = X.em + 0.5 * sqn(X.u);
+    const double dS = sS - sM, idS = 1. / dS;
+    double rhoStar = qdiv(mS, dS, idS);
+    double EStar = Et + (sM - uS) * (sM + pS / mS);
+    double pStar = mS * (sM - uS) + pS;
+#pragma unroll
+    for (int k = 0; k < NPH; k++) {
+      const double mk = X.a[k] * X.r[k] * (sS - uS);
+      F.m[k] = qdiv(mk, dS, idS) * sM;
+    }
+    F.q[0] = rhoStar * sM * sM + pStar;
+    F.q[1] = rhoStar * X.u[1] * sM;
+    F.q[2] = rhoStar * X.u[2] * sM;
+    F.E = (rhoStar * EStar + pStar) * sM;
+  }
+  F.sM = sM;
+  return smx;
+}
+
 template <int MODEL, int NPH>
 ECO_D double hllc(const SchemeParams &S, const CellState<NPH> &L, const CellState<NPH> &R, FluxVec<NPH> &F) {
   const double eps = S.eps;
-  if constexpr (MODEL == MODEL_EULER) {
+  if constexpr (MODEL == MODEL_THERMALEQ) {
+    (void)eps;
+    return hllc_te<NPH>(L, R, F);
+  } else if constexpr (MODEL == MODEL_EULER) {
     // ModEuler::solveRiemannIntern (ModEuler.cpp:90-175)
     double uL = L.u[0], vL = L.u[1], wL = L.u[2], pL = L.p[0], rhoL = L.r[0], cL = L.c[0], EL = L.E;
     double uR = R.u[0], vR = R.u[1], wR = R.u[2], pR = R.p[0], rhoR = R.r[0], cR = R.c[0], ER = R.E;

@@ -26,13 +26,23 @@
 
 namespace eco {
 
+// Stored primitives: per phase alpha, rho, p and the velocity (Euler: rho,
+// p, velocity; ThermalEq: per phase alpha, the mixture p and T, velocity).
 template <int MODEL, int NPH, int DIM>
 struct Layout {
-  static constexpr int NPRIM = (MODEL == MODEL_EULER) ? 2 + DIM : 3 * NPH + DIM;  // stored primitives
-  static constexpr int IE = NPRIM;                                                 // total energy field
-  static constexpr int NVAR = NPRIM;                                               // slope variables
-  static constexpr int NFV = (MODEL == MODEL_EULER) ? DIM + 3 : 3 * NPH + DIM + 2; // flux values + sM
-  static constexpr int NACC = NFV - 1;                                              // accumulated values
+  static constexpr bool TE = (MODEL == MODEL_THERMALEQ);
+  static constexpr int NPRIM = (MODEL == MODEL_EULER) ? 2 + DIM : TE ? NPH + 2 + DIM : 3 * NPH + DIM;
+  static constexpr int IE = NPRIM;    // total energy field
+  static constexpr int NVAR = NPRIM;  // slope variables
+  // flux values + sM
+  static constexpr int NFV = (MODEL == MODEL_EULER) ? DIM + 3 : TE ? NPH + DIM + 2 : 3 * NPH + DIM + 2;
+  static constexpr int NACC = NFV - 1;  // accumulated values
+  // first velocity component in the slope vector
+  static constexpr int IU0 = (MODEL == MODEL_EULER) ? 2 : TE ? NPH + 2 : 3 * NPH;
+  // slope variable v is a volume fraction (limited by the volume-fraction limiters)
+  static constexpr bool is_alpha(int v) {
+    return (MODEL == MODEL_EULER) ? false : TE ? (v < NPH) : (v < 3 * NPH && v % 3 == 0);
+  }
 };
 
 // Device-memory pointers taken from an argument table are generic pointers
@@ -95,6 +105,13 @@ __device__ __forceinline__ void load_prim(PW W, long long nc, long long idx, Cel
 #pragma unroll
     for (int d = 0; d < 3; d++) s.u[d] = (d < DIM) ? W[(2 + d) * nc + idx] : 0.;
     s.a[0] = 1.;
+  } else if constexpr (MODEL == MODEL_THERMALEQ) {
+#pragma unroll
+    for (int k = 0; k < NPH; k++) s.a[k] = W[k * nc + idx];
+    s.pm = W[NPH * nc + idx];
+    s.T = W[(NPH + 1) * nc + idx];
+#pragma unroll
+    for (int d = 0; d < 3; d++) s.u[d] = (d < DIM) ? W[(NPH + 2 + d) * nc + idx] : 0.;
   } else {
 #pragma unroll
     for (int k = 0; k < NPH; k++) {
@@ -115,6 +132,13 @@ __device__ __forceinline__ void store_prim(PW W, long long nc, long long idx, co
     W[nc + idx] = s.p[0];
 #pragma unroll
     for (int d = 0; d < DIM; d++) W[(2 + d) * nc + idx] = s.u[d];
+  } else if constexpr (MODEL == MODEL_THERMALEQ) {
+#pragma unroll
+    for (int k = 0; k < NPH; k++) W[k * nc + idx] = s.a[k];
+    W[NPH * nc + idx] = s.pm;
+    W[(NPH + 1) * nc + idx] = s.T;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) W[(NPH + 2 + d) * nc + idx] = s.u[d];
   } else {
 #pragma unroll
     for (int k = 0; k < NPH; k++) {
@@ -127,13 +151,22 @@ __device__ __forceinline__ void store_prim(PW W, long long nc, long long idx, co
   }
 }
 
-// primitive vector used for slopes: (alpha,rho,p)_k, u_0..u_{DIM-1}  (Euler: rho, p, u)
+// primitive vector used for slopes: (alpha,rho,p)_k, u_0..u_{DIM-1}  (Euler:
+// rho, p, u; ThermalEq: alpha_k, p, T, u as PhaseThermalEq::computeSlopesPhase
+// and MixThermalEq::computeSlopesMixture, PhaseThermalEq.cpp:139-142,
+// MixThermalEq.cpp:457-464)
 template <int MODEL, int NPH, int DIM>
 __device__ __forceinline__ void prim_to_vec(const CellState<NPH> &s, double *w) {
   if constexpr (MODEL == MODEL_EULER) {
     w[0] = s.r[0]; w[1] = s.p[0];
 #pragma unroll
     for (int d = 0; d < DIM; d++) w[2 + d] = s.u[d];
+  } else if constexpr (MODEL == MODEL_THERMALEQ) {
+#pragma unroll
+    for (int k = 0; k < NPH; k++) w[k] = s.a[k];
+    w[NPH] = s.pm; w[NPH + 1] = s.T;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) w[NPH + 2 + d] = s.u[d];
   } else {
 #pragma unroll
     for (int k = 0; k < NPH; k++) { w[3 * k] = s.a[k]; w[3 * k + 1] = s.r[k]; w[3 * k + 2] = s.p[k]; }
@@ -148,6 +181,12 @@ __device__ __forceinline__ void vec_to_prim(const double *w, CellState<NPH> &s) 
     s.r[0] = w[0]; s.p[0] = w[1];
 #pragma unroll
     for (int d = 0; d < DIM; d++) s.u[d] = w[2 + d];
+  } else if constexpr (MODEL == MODEL_THERMALEQ) {
+#pragma unroll
+    for (int k = 0; k < NPH; k++) s.a[k] = w[k];
+    s.pm = w[NPH]; s.T = w[NPH + 1];
+#pragma unroll
+    for (int d = 0; d < DIM; d++) s.u[d] = w[NPH + 2 + d];
   } else {
 #pragma unroll
     for (int k = 0; k < NPH; k++) { s.a[k] = w[3 * k]; s.r[k] = w[3 * k + 1]; s.p[k] = w[3 * k + 2]; }
@@ -188,6 +227,9 @@ __device__ __forceinline__ void put_flux(double (*L)[MAXF], int f, const FluxVec
   int q = 0;
   if constexpr (MODEL == MODEL_EULER) {
     L[q++][f] = F.m[0];
+  } else if constexpr (MODEL == MODEL_THERMALEQ) {
+#pragma unroll
+    for (int k = 0; k < NPH; k++) L[q++][f] = F.m[k];
   } else {
 #pragma unroll
     for (int k = 0; k < NPH; k++) { L[q++][f] = F.a[k]; L[q++][f] = F.m[k]; L[q++][f] = F.en[k]; }
@@ -203,6 +245,9 @@ __device__ __forceinline__ void get_flux(double (*L)[MAXF], int f, FluxVec<NPH> 
   int q = 0;
   if constexpr (MODEL == MODEL_EULER) {
     F.m[0] = L[q++][f];
+  } else if constexpr (MODEL == MODEL_THERMALEQ) {
+#pragma unroll
+    for (int k = 0; k < NPH; k++) F.m[k] = L[q++][f];
   } else {
 #pragma unroll
     for (int k = 0; k < NPH; k++) { F.a[k] = L[q++][f]; F.m[k] = L[q++][f]; F.en[k] = L[q++][f]; }
@@ -222,6 +267,10 @@ __device__ __forceinline__ void add_face(FluxVec<NPH> &U, const FluxVec<NPH> &F,
   if (add) {
     if constexpr (MODEL == MODEL_EULER) {
       U.m[0] += coefA * F.m[0];
+    } else if constexpr (MODEL == MODEL_THERMALEQ) {
+      // FluxThermalEq::addFlux (FluxThermalEq.cpp:62-69): no non-conservative terms
+#pragma unroll
+      for (int k = 0; k < NPH; k++) U.m[k] += coefA * F.m[k];
     } else {
 #pragma unroll
       for (int k = 0; k < NPH; k++) { U.a[k] += coefA * F.a[k]; U.m[k] += coefA * F.m[k]; U.en[k] += coefA * F.en[k]; }
@@ -229,7 +278,7 @@ __device__ __forceinline__ void add_face(FluxVec<NPH> &U, const FluxVec<NPH> &F,
 #pragma unroll
     for (int d = 0; d < 3; d++) U.q[d] += F.q[d] * coefA;
     U.E += coefA * F.E;
-    if constexpr (MODEL != MODEL_EULER) {
+    if constexpr (MODEL != MODEL_EULER && MODEL != MODEL_THERMALEQ) {
 #pragma unroll
       for (int k = 0; k < NPH; k++) {
         U.a[k] += -coefA * own.a[k] * F.sM;
@@ -239,6 +288,9 @@ __device__ __forceinline__ void add_face(FluxVec<NPH> &U, const FluxVec<NPH> &F,
   } else {
     if constexpr (MODEL == MODEL_EULER) {
       U.m[0] -= coefA * F.m[0];
+    } else if constexpr (MODEL == MODEL_THERMALEQ) {
+#pragma unroll
+      for (int k = 0; k < NPH; k++) U.m[k] -= coefA * F.m[k];
     } else {
 #pragma unroll
       for (int k = 0; k < NPH; k++) { U.a[k] -= coefA * F.a[k]; U.m[k] -= coefA * F.m[k]; U.en[k] -= coefA * F.en[k]; }
@@ -246,7 +298,7 @@ __device__ __forceinline__ void add_face(FluxVec<NPH> &U, const FluxVec<NPH> &F,
 #pragma unroll
     for (int d = 0; d < 3; d++) U.q[d] -= F.q[d] * coefA;
     U.E -= coefA * F.E;
-    if constexpr (MODEL != MODEL_EULER) {
+    if constexpr (MODEL != MODEL_EULER && MODEL != MODEL_THERMALEQ) {
 #pragma unroll
       for (int k = 0; k < NPH; k++) {
         U.a[k] -= -coefA * own.a[k] * F.sM;
@@ -271,6 +323,9 @@ __device__ __forceinline__ void park_acc(double (*A)[NT], int t, const FluxVec<N
   int q = 0;
   if constexpr (MODEL == MODEL_EULER) {
     A[q++][t] = U.m[0];
+  } else if constexpr (MODEL == MODEL_THERMALEQ) {
+#pragma unroll
+    for (int k = 0; k < NPH; k++) A[q++][t] = U.m[k];
   } else {
 #pragma unroll
     for (int k = 0; k < NPH; k++) { A[q++][t] = U.a[k]; A[q++][t] = U.m[k]; A[q++][t] = U.en[k]; }
@@ -286,6 +341,9 @@ __device__ __forceinline__ void unpark_acc(double (*A)[NT], int t, FluxVec<NPH> 
   zero_flux(U);
   if constexpr (MODEL == MODEL_EULER) {
     U.m[0] = A[q++][t];
+  } else if constexpr (MODEL == MODEL_THERMALEQ) {
+#pragma unroll
+    for (int k = 0; k < NPH; k++) U.m[k] = A[q++][t];
   } else {
 #pragma unroll
     for (int k = 0; k < NPH; k++) { U.a[k] = A[q++][t]; U.m[k] = A[q++][t]; U.en[k] = A[q++][t]; }
@@ -321,7 +379,7 @@ __device__ __forceinline__ void load_cache(PW W, long long nc, int ie, long long
 // time-n alpha_k, p_k of one cell: the non-conservative terms' coefficients
 template <int MODEL, int NPH, int DIM, typename PW>
 __device__ __forceinline__ void load_nc_coeffs(PW W, long long nc, long long idx, CellState<NPH> &s) {
-  if constexpr (MODEL != MODEL_EULER) {
+  if constexpr (MODEL != MODEL_EULER && MODEL != MODEL_THERMALEQ) {
 #pragma unroll
     for (int k = 0; k < NPH; k++) { s.a[k] = W[(3 * k) * nc + idx]; s.p[k] = W[(3 * k + 2) * nc + idx]; }
   }
@@ -402,6 +460,9 @@ __device__ __forceinline__ void put_flux_s(double *L, int stride, int f, const F
   int q = 0;
   if constexpr (MODEL == MODEL_EULER) {
     L[(q++) * stride + f] = F.m[0];
+  } else if constexpr (MODEL == MODEL_THERMALEQ) {
+#pragma unroll
+    for (int k = 0; k < NPH; k++) L[(q++) * stride + f] = F.m[k];
   } else {
 #pragma unroll
     for (int k = 0; k < NPH; k++) {
@@ -419,6 +480,9 @@ __device__ __forceinline__ void get_flux_s(const double *L, int stride, int f, F
   int q = 0;
   if constexpr (MODEL == MODEL_EULER) {
     F.m[0] = L[(q++) * stride + f];
+  } else if constexpr (MODEL == MODEL_THERMALEQ) {
+#pragma unroll
+    for (int k = 0; k < NPH; k++) F.m[k] = L[(q++) * stride + f];
   } else {
 #pragma unroll
     for (int k = 0; k < NPH; k++) {
@@ -489,7 +553,7 @@ __device__ __forceinline__ void muscl_vars(const SchemeParams &S, const TW *wl_,
       const float sc = (float)(wr - wl) * idc;
       const float sm = has_m ? (float)(wl - (double)wll[v]) * idm : 0.f;
       const float sp = has_p ? (float)((double)whh[v] - wr) * idp : 0.f;
-      const bool isa = (MODEL != MODEL_EULER) && (v < 3 * NPH) && (v % 3 == 0);
+      const bool isa = Ly::is_alpha(v);
       float gl, gr;
       if constexpr (LV >= 0) {
         gl = isa ? limiter_kt<LA, float>(sc, sm) : limiter_kt<LV, float>(sc, sm);
@@ -514,7 +578,7 @@ __device__ __forceinline__ void muscl_vars(const SchemeParams &S, const TW *wl_,
       // wall face's slope for the normal velocity (variable iuw)
       const double sm = has_m ? qdiv(wl - (double)wll[v], dcm, ydm) : ((v == iuw) ? swm : 0.);
       const double sp = has_p ? qdiv((double)whh[v] - wr, dcp, ydp) : ((v == iuw) ? swp : 0.);
-      const bool isa = (MODEL != MODEL_EULER) && (v < 3 * NPH) && (v % 3 == 0);
+      const bool isa = Ly::is_alpha(v);
       double gl, gr;
       if constexpr (LV >= 0) {
         gl = isa ? limiter_k<LA>(sc, sm) : limiter_k<LV>(sc, sm);
@@ -577,7 +641,7 @@ __device__ __forceinline__ void wall_state_o2(const SchemeParams &S, const Geome
                                               long long nc, long long cidx, long long sd, int dir, int side,
                                               CellState<NPH> &L) {
   using Ly = Layout<MODEL, NPH, DIM>;
-  const int iu = ((MODEL == MODEL_EULER) ? 2 : 3 * NPH) + dir;
+  const int iu = Ly::IU0 + dir;
   CellState<NPH> N;
   const long long in = side == 0 ? cidx + sd : cidx - sd;  // the cell across the internal face
   load_prim<MODEL, NPH, DIM>(Wsrc, nc, in, N);
@@ -599,7 +663,7 @@ __device__ __forceinline__ void wall_state_o2(const SchemeParams &S, const Geome
   for (int v = 0; v < Ly::NVAR; v++) {
     const double s1 = (v == iu) ? w[v] / dw : 0.;
     const double s2 = side == 0 ? (wn[v] - w[v]) / dcc : (w[v] - wn[v]) / dcc;
-    const bool isa = (MODEL != MODEL_EULER) && (v < 3 * NPH) && (v % 3 == 0);
+    const bool isa = Ly::is_alpha(v);
     w[v] = w[v] + limiter(isa ? lv : lg, s1, s2) * dg;
   }
   vec_to_prim<MODEL, NPH, DIM>(w, L);
@@ -653,6 +717,13 @@ __device__ __forceinline__ double face_fp64(const StageArgs &A, const int *o, co
           // MultiP keeps the mixture energy of totalEnergyToInternalEnergy (Run.cpp:471)
           L.em = Wsrc[Ly::IE * nc + ilo] - 0.5 * sqn(L.u);
           R.em = Wsrc[Ly::IE * nc + ihi] - 0.5 * sqn(R.u);
+        } else if constexpr (MODEL == MODEL_THERMALEQ) {
+          // the cells' cached mixture energy: sum Y_k e_k of the initial
+          // state, then E - |u|^2/2 of Cell::correctionEnergy (Cell.cpp:254-258)
+          fulfill<MODEL, NPH>(S, L);
+          fulfill<MODEL, NPH>(S, R);
+          L.em = Wsrc[(Ly::IE + 1) * nc + ilo];
+          R.em = Wsrc[(Ly::IE + 1) * nc + ihi];
         } else {
           fulfill<MODEL, NPH>(S, L);
           fulfill<MODEL, NPH>(S, R);
@@ -701,7 +772,7 @@ __device__ __forceinline__ double face_fp64(const StageArgs &A, const int *o, co
         wll[v] = Fv[jm];
         whh[v] = Fv[jp];
       }
-      constexpr int IU = (MODEL == MODEL_EULER) ? 2 + dir : 3 * NPH + dir;  // normal velocity in the slope vector
+      constexpr int IU = Ly::IU0 + dir;  // normal velocity in the slope vector
       double swm = 0., swp = 0.;
       if (wm) swm = wl_[IU] / G.dwall[dir][0];
       if (wp) swp = wr_[IU] / G.dwall[dir][1];
@@ -843,7 +914,7 @@ __device__ __forceinline__ void stage_slopes(StagedLds<MODEL, NPH, DIM> &L, cons
       const double wm = L.S[v][s0 - SD], w = L.S[v][s0], wp = L.S[v][s0 + SD];
       const double sm = has_m ? qdiv(w - wm, dcm, ydm) : 0.;
       const double sp = has_p ? qdiv(wp - w, dcp, ydp) : 0.;
-      const bool isa = (MODEL != MODEL_EULER) && (v < 3 * NPH) && (v % 3 == 0);
+      const bool isa = Ly::is_alpha(v);
       double g;
       if constexpr (LV >= 0) {
         g = isa ? limiter_k<LA>(sp, sm) : limiter_k<LV>(sp, sm);
@@ -1101,7 +1172,10 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int slot, c
 
     // cell update
     CellState<NPH> sn = own;
-    if constexpr (MODEL != MODEL_EULER) {
+    if constexpr (MODEL == MODEL_THERMALEQ) {
+      // phase densities at (p, T) and the mixture density of the last buildPrim
+      fulfill<MODEL, NPH>(S, sn);
+    } else if constexpr (MODEL != MODEL_EULER) {
       sn.rho = 0.;
 #pragma unroll
       for (int k = 0; k < NPH; k++) sn.rho += sn.a[k] * sn.r[k];
@@ -1133,6 +1207,8 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int slot, c
       Wout[Ly::IE * nc + cidx] = s1.E;
       // buildPrim's mixture density (the relaxation leaves it unchanged)
       if (MODEL == MODEL_MULTIP && S.ncache) store_cache<MODEL, NPH>(Wout, nc, Ly::IE, cidx, S.ncache, s1);
+      // ThermalEq first order: the mixture energy after Cell::correctionEnergy
+      if (MODEL == MODEL_THERMALEQ && S.ncache) Wout[(Ly::IE + 1) * nc + cidx] = s1.E - 0.5 * sqn(s1.u);
     }
   }
 
@@ -1194,6 +1270,9 @@ __global__ void k_init(const StageArgs A) {
   if constexpr (MODEL != MODEL_EULER) s.E = s.em + 0.5 * sqn(s.u);
   A.wout[Ly::IE * nc + i] = s.E;
   if (MODEL == MODEL_MULTIP && A.sch.ncache) store_cache<MODEL, NPH>(A.wout, nc, Ly::IE, i, A.sch.ncache, s);
+  // ThermalEq first order: the initial cells' mixture energy sum Y_k e_k
+  // (Cell::completeFulfillState, Cell.cpp:271-278)
+  if (MODEL == MODEL_THERMALEQ && A.sch.ncache) A.wout[(Ly::IE + 1) * nc + i] = s.em;
 }
 
 // ---------------------------------------------------------------------------

@@ -50,7 +50,7 @@ import numpy as np
 
 LIMITERS = {"MINMOD": "minmod", "VANLEER": "vanleer", "VANALBADA": "vanalbada", "SUPERBEE": "superbee",
             "MC": "mc", "THINC": "thinc"}
-MODELS = {"EULER": "euler", "KAPILA": "kapila", "MULTIP": "multip"}
+MODELS = {"EULER": "euler", "KAPILA": "kapila", "MULTIP": "multip", "THERMALEQ": "thermaleq"}
 AXES = {"X": 0, "Y": 1, "Z": 2}
 
 
@@ -190,11 +190,27 @@ def _named_values(node, names, where, need_all):
     return vals
 
 
-def _state_values(st, model, nph, where):
-    """(alpha_k, rho_k, p_k per phase, u, v, w) of one <state>."""
+def _state_values(st, model, nph, where, eos=None):
+    """(alpha_k, rho_k, p_k per phase, u, v, w) of one <state>; for the
+    thermal-equilibrium model also E (0, not read) and the temperature."""
     mats = st.findall("material")
     if len(mats) != nph:
         raise DeckError(f"{where}: {len(mats)} materials for {nph} phases")
+    if model == "thermaleq":
+        # PhaseThermalEq reads alpha (PhaseThermalEq.cpp:45-57), MixThermalEq the
+        # mixture pressure, temperature and velocity (MixThermalEq.cpp:46-79);
+        # rho_k = rho_k(p, T) as Eos::computeDensity (EosSG.cpp:114-117)
+        mix = _child(st, "mixture", where)
+        dm = _child(mix, "dataMix", where)
+        p, T = _f(dm, "pressure", where), _f(dm, "temperature", where)
+        vel = _child(mix, "velocity", where)
+        out = []
+        for m, e in zip(mats, eos):
+            a = _f(_child(m, "dataFluid", where), "alpha", where)
+            num = p + e["pinf"] if e["type"] == "SG" else p
+            den = (e["gamma"] - 1.0) * e["cv"] * T
+            out += [a, num / (0.0 if den < 0.0 else den), p]
+        return out + [float(vel.get(a, 0.0)) for a in "xyz"] + [0.0, T]
     if model == "euler":
         fl = _child(mats[0], "dataFluid", where)
         v = _child(fl, "velocity", where)
@@ -302,8 +318,8 @@ def load(rundir):
         raise DeckError(f"modelV4.xml: {len(eos_files)} EOS for {nph} phases")
     case["eos"] = [dict(_eos(rundir / "libEOS" / f), name=Path(f).stem, file=f) for f in eos_files]
     relax = [r.get("type", "").upper() for r in model_x.findall("relaxation")]
-    if any(r != "P" for r in relax):
-        raise DeckError(f"modelV4.xml: relaxation {relax} (only P)")
+    if any(r != "P" for r in relax) or (relax and model == "thermaleq"):
+        raise DeckError(f"modelV4.xml: relaxation {relax} (only P, for Kapila or MultiP)")
     case["relax_p"] = model == "multip" and "P" in relax
     # transport equations (Input.cpp:472-473, 528-540): numberTransports of
     # the first <transport name> elements
@@ -367,6 +383,8 @@ def load(rundir):
     nf = 3 * nph + 3 if model != "euler" else 6
     if trs:
         nf += 1 + len(trs)  # E (not read), then the scalars
+    if model == "thermaleq":
+        nf += 2  # E (not read), then the temperature
     grid = np.zeros((nf, case["cells"][2], case["cells"][1], case["cells"][0]))
     filled = np.zeros(grid.shape[1:], dtype=bool)
     pdoms = ci.find("physicalDomains")
@@ -375,7 +393,7 @@ def load(rundir):
         sname = dom.get("state")
         if sname not in states:
             raise DeckError(f"{where}: unknown state {sname!r}")
-        vals = _state_values(states[sname], model, nph, where)
+        vals = _state_values(states[sname], model, nph, where, case["eos"])
         if trs:
             vals = vals + [0.0] + _named_values(states[sname], trs, where, False)
         m = _domain_mask(dom, Xg, Yg, Zg, where)
@@ -522,7 +540,8 @@ def resume_point(case, rundir):
     model, nph = case["model"], len(case["eos"])
     ncell = int(np.prod(case["cells"]))
     ntr = len(case.get("transports", []))
-    state = np.zeros((3 * nph + 4 + ntr if ntr else 3 * nph + 3, ncell))
+    te = model == "thermaleq"
+    state = np.zeros((3 * nph + 4 + ntr + te if (ntr or te) else 3 * nph + 3, ncell))
     # the scalars T1, T2, ... (OutputXML::readResults, OutputXML.cpp:196-203)
     for t in range(ntr):
         if f"T{t + 1}" not in arrs:
@@ -533,6 +552,16 @@ def resume_point(case, rundir):
         state[1] = _array(arrs, "F0_Density_", where)
         state[2] = _array(arrs, "F0_Pressure_", where)
         vel = _array(arrs, "F0_Velocity_", where)
+    elif te:
+        # PhaseThermalEq::setScalar (alpha, density), MixThermalEq::setScalar
+        # (density, pressure, temperature) and setVector (velocity)
+        pm = _array(arrs, "Pressure_Mixture", where)
+        for k in range(nph):
+            state[3 * k] = _array(arrs, f"F{k}_Alpha_", where)
+            state[3 * k + 1] = _array(arrs, f"F{k}_Density_", where)
+            state[3 * k + 2] = pm
+        state[3 * nph + 4] = _array(arrs, "Temperature_Mixture", where)
+        vel = _array(arrs, "Velocity_Mixture", where)
     else:
         pm = _array(arrs, "Pressure_Mixture", where)
         for k in range(nph):

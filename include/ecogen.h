@@ -29,9 +29,16 @@ extern "C" {
 
 #define ECO_NFIELDS(nph) (3 * (nph) + 4)
 #define ECO_NFIELDS_TR(nph, ntr) (3 * (nph) + 4 + (ntr))
+/* fields of a configuration's host state: transports after E, and for
+   ECO_MODEL_THERMALEQ the mixture temperature last */
+#define ECO_NFIELDS_CFG(model, nph, ntr) (3 * (nph) + 4 + (ntr) + ((model) == 3))
 #define ECO_MAX_TRANSPORTS 4
 
-enum eco_model { ECO_MODEL_EULER = 0, ECO_MODEL_KAPILA = 1, ECO_MODEL_MULTIP = 2 };
+/* ECO_MODEL_THERMALEQ: two phases in pressure and temperature equilibrium
+   (ModThermalEq, src/Models/ThermalEq); host state per phase alpha_k, rho_k,
+   p_k (p_k the mixture pressure, rho_k = rho_k(p, T)), u, v, w, E, then the
+   mixture temperature T: on input alpha_k, p_0, u and T are read */
+enum eco_model { ECO_MODEL_EULER = 0, ECO_MODEL_KAPILA = 1, ECO_MODEL_MULTIP = 2, ECO_MODEL_THERMALEQ = 3 };
 enum eco_limiter { ECO_LIM_NONE = 0, ECO_LIM_MINMOD = 1, ECO_LIM_VANLEER = 2, ECO_LIM_VANALBADA = 3,
                    ECO_LIM_SUPERBEE = 4, ECO_LIM_MC = 5, ECO_LIM_THINC = 6 };
 enum eco_eos_type { ECO_EOS_IG = 0, ECO_EOS_SG = 1 };
@@ -79,7 +86,7 @@ typedef struct {
    from the deck (src/InputOutput/Input.cpp:85-625) for this path. */
 typedef struct {
   int model;        /* enum eco_model */
-  int nphase;       /* 1 for Euler; 2, or 3-4 in FP64 without THINC, for Kapila/MultiP */
+  int nphase;       /* 1 for Euler; 2, or 3-4 in FP64 without THINC, for Kapila/MultiP; 2 for ThermalEq */
   int order;        /* 1 = FIRSTORDER, 2 = SECONDORDER (MUSCL predictor-corrector) */
   int lim[4];       /* global, interface, global volume-fraction, interface volume-fraction limiter */
   int alpha_null;   /* Kapila flowModel alphaNull attribute */
@@ -192,7 +199,8 @@ const double *eco_state_device(eco_ctx *ctx, int *nfields_device);
    to state_out, both device arrays in the layout above (E ignored on input,
    written on output; they may not overlap), with the same results bit for
    bit.  nsteps > 0: exactly nsteps steps; nsteps <= 0: until t >= t_final
-   (as float), at most -nsteps steps when nsteps < 0.  FP64 only, no transports.
+   (as float), at most -nsteps steps when nsteps < 0.  FP64 only, no transports,
+   not ECO_MODEL_THERMALEQ.
    Every intermediate buffer (second time level, MUSCL U*, distance tables,
    control block) lives in a static device array of the library,
    ECO_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends; no
@@ -205,7 +213,7 @@ const double *eco_state_device(eco_ctx *ctx, int *nfields_device);
 int eco_run_noalloc(const eco_config *cfg, const double *state_in, double *state_out, int nsteps, double t_final,
                     void *hip_stream, eco_status *status);
 
-/* Scratch bytes eco_run_noalloc needs for cfg (0 for an invalid cfg). */
+/* Scratch bytes eco_run_noalloc needs for cfg (0 for a cfg it refuses). */
 unsigned long long eco_noalloc_bytes(const eco_config *cfg);
 
 /* Releases device memory (Run::finalize, src/Run.cpp:496-530). */

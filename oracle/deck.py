@@ -44,7 +44,10 @@ def _state_xml(case, st):
     else:
         for k, e in enumerate(eos):
             out.append(f'    <material type="fluide" EOS="{_eos_file(e)}">')
-            if model == "kapila":
+            if model == "thermaleq":
+                # PhaseThermalEq.cpp:45-57 reads alpha only
+                out.append(f'      <dataFluid alpha="{st["alpha"][k]!r}"/>')
+            elif model == "kapila":
                 out.append(f'      <dataFluid alpha="{st["alpha"][k]!r}" density="{st["rho"][k]!r}"/>')
             else:
                 out.append(f'      <dataFluid alpha="{st["alpha"][k]!r}" density="{st["rho"][k]!r}" pressure="{st["p"][k]!r}"/>')
@@ -53,6 +56,9 @@ def _state_xml(case, st):
         out.append('    <mixture>')
         if model == "kapila":
             out.append(f'      <dataMix pressure="{st["p"][0]!r}"/>')
+        elif model == "thermaleq":
+            # MixThermalEq.cpp:46-79: pressure and temperature of the mixture
+            out.append(f'      <dataMix pressure="{st["p"][0]!r}" temperature="{st["T"]!r}"/>')
         out.append(f'      <velocity x="{u[0]!r}" y="{u[1]!r}" z="{u[2]!r}"/>')
         out.append('    </mixture>')
     # Input.cpp:771-790: <transport name value> of the state, matched by name
@@ -130,8 +136,9 @@ def write_deck(case, root: Path):
         f'    <numberCells x="{n[0]}" y="{n[1]}" z="{n[2]}"/>\n  </cartesianMesh>\n</mesh>\n')
     model = case["model"]
     fm = {"euler": '<flowModel name="Euler"/>',
-          "kapila": f'<flowModel name="Kapila" numberPhases="{len(case["eos"])}" alphaNull="{"true" if case["alpha_null"] else "false"}"/>',
-          "multip": f'<flowModel name="MultiP" numberPhases="{len(case["eos"])}"/>'}[model]
+          "kapila": f'<flowModel name="Kapila" numberPhases="{len(case["eos"])}" alphaNull="{"true" if case.get("alpha_null") else "false"}"/>',
+          "multip": f'<flowModel name="MultiP" numberPhases="{len(case["eos"])}"/>',
+          "thermaleq": f'<flowModel name="ThermalEq" numberPhases="{len(case["eos"])}"/>'}[model]
     trs = case.get("transports", [])
     if trs:
         # Input.cpp:472-473, 528-540: numberTransports and one <transport name> each

@@ -38,6 +38,7 @@ typedef struct {
 
 typedef struct {
   double rho, p, u[3], e, E, cF, cW;
+  double T; /* ThermalEq: MixThermalEq::m_temperature (cF holds its sound speed) */
 } Mix;
 
 typedef struct {
@@ -122,6 +123,8 @@ static void eos_fix_p(const orc_eos *q, double *p) {
     if (*p < 1.e-15) *p = 1.e-15;
   }
 }
+/* computeDensity(p, T): EosSG.cpp:114-117, EosIG.cpp:111-114 */
+static double eos_rho_pt(const orc_eos *q, double p, double T, double eps);
 /* sendSpecialMixtureEos: EosSG.cpp:217-222, EosIG.cpp:213-218 */
 static void eos_special(const orc_eos *q, double *gpg, double *eref, double *og) {
   *gpg = (q->type == ORC_EOS_SG) ? q->gamma * q->pinf / (q->gamma - 1.) : 0.;
@@ -172,6 +175,7 @@ static double limiter(int lim, double s1, double s2) {
 static inline double sqnorm(const double *u) { return u[0] * u[0] + u[1] * u[1] + u[2] * u[2]; }
 
 /* PhaseKapila::verifyAndCorrectPhase (PhaseKapila.cpp:294-307),
+   PhaseThermalEq::verifyAndCorrectPhase,
    PhaseMultiP::verifyAndCorrectPhase (PhaseMultiP.cpp:288-294),
    PhaseEuler::verifyAndCorrectPhase (PhaseEuler.cpp:349-353) */
 static void verify_correct(const Ctx *c, int k, Phase *ph) {
@@ -190,6 +194,11 @@ static void verify_correct(const Ctx *c, int k, Phase *ph) {
     if (ph->alpha < 1e-10) ph->alpha = 1e-9;
     if (ph->alpha > 1. - 1e-10) ph->alpha = 1. - 1e-9;
     if (ph->rho < 1.e-10) ph->rho = 1.e-9;
+  } else if (cfg->model == ORC_THERMALEQ) {
+    /* PhaseThermalEq::verifyAndCorrectPhase (PhaseThermalEq.cpp:240-246) */
+    if (ph->alpha < 1e-10) ph->alpha = 1e-10;
+    if (ph->alpha > 1. - 1e-10) ph->alpha = 1. - 1e-10;
+    if (ph->rho < 1.e-10) ph->rho = 1.e-10;
   } else {
     if (ph->rho < 1.e-10) ph->rho = 1.e-10;
   }
@@ -238,8 +247,45 @@ static void mixture_vars(const Ctx *c, State *s) {
   m->cW = 1. / sqrt(m->rho * m->cW);
 }
 
+/* ThermalEq: the equations of state without the stiffened-gas offset for
+   the ideal gas (Eos::getPInf is 0 for EosIG, Eos.h:149) */
+static double te_pinf(const orc_eos *q) { return (q->type == ORC_EOS_SG) ? q->pinf : 0.; }
+
+/* MixThermalEq::computeMixtureVariables (MixThermalEq.cpp:305-323): mixture
+   density, mass fractions, energy and the sound speed sqrt(sum Y_k c_k^2) */
+static void te_mixture_vars(const Ctx *c, State *s) {
+  int n = c->cfg->nph;
+  Mix *m = &s->mx;
+  m->rho = 0.;
+  for (int k = 0; k < n; k++) m->rho += s->ph[k].alpha * s->ph[k].rho;
+  for (int k = 0; k < n; k++) s->ph[k].Y = s->ph[k].alpha * s->ph[k].rho / m->rho;
+  m->e = 0.;
+  m->cF = 0.;
+  for (int k = 0; k < n; k++) {
+    m->e += s->ph[k].Y * s->ph[k].e;
+    m->cF += s->ph[k].Y * s->ph[k].c * s->ph[k].c;
+  }
+  m->cF = sqrt(m->cF);
+}
+
+/* ModThermalEq::fulfillState (ModThermalEq.cpp:78-89): every phase at the
+   mixture pressure and temperature, PhaseThermalEq::extendedCalculusPhase
+   (PhaseThermalEq.cpp:73-78), then the mixture variables */
+static void te_fulfill(const Ctx *c, State *s) {
+  for (int k = 0; k < c->cfg->nph; k++) {
+    Phase *ph = &s->ph[k];
+    const orc_eos *q = &c->cfg->eos[k];
+    ph->p = s->mx.p;
+    ph->rho = eos_rho_pt(q, s->mx.p, s->mx.T, c->epsilon);
+    ph->e = eos_e(q, ph->rho, ph->p, c->epsilon);
+    ph->c = eos_c(q, ph->rho, ph->p, c->epsilon);
+  }
+  te_mixture_vars(c, s);
+}
+
 /* Model::fulfillState: ModKapila.cpp:89-101, ModMultiP.cpp:83-91, ModEuler.cpp (phase only) */
 static void fulfill(const Ctx *c, State *s) {
+  if (c->cfg->model == ORC_THERMALEQ) { te_fulfill(c, s); return; }
   for (int k = 0; k < c->cfg->nph; k++) extended_phase(c, k, &s->ph[k], s->mx.u);
   mixture_vars(c, s);
 }
@@ -258,6 +304,13 @@ static void build_cons(const Ctx *c, const State *s, Flux *f) {
     f->energMix = f->masse[0] * p->Et;
     return;
   }
+  if (cfg->model == ORC_THERMALEQ) {
+    /* FluxThermalEq::buildCons (FluxThermalEq.cpp:113-126) */
+    for (int k = 0; k < cfg->nph; k++) f->masse[k] = s->ph[k].alpha * s->ph[k].rho;
+    for (int d = 0; d < 3; d++) f->qdm[d] = s->mx.rho * s->mx.u[d];
+    f->energMix = s->mx.rho * s->mx.E;
+    return;
+  }
   for (int k = 0; k < cfg->nph; k++) {
     const Phase *p = &s->ph[k];
     f->alpha[k] = p->alpha;
@@ -273,6 +326,60 @@ static void zero_small_u(double *u) {
   if (fabs(u[0]) < 1.e-8) u[0] = 0.;
   if (fabs(u[1]) < 1.e-8) u[1] = 0.;
   if (fabs(u[2]) < 1.e-8) u[2] = 0.;
+}
+
+/* MixThermalEq::computePressure (MixThermalEq.cpp:121-150): the mixture
+   pressure of two phases in pressure and temperature equilibrium from the
+   mixture internal energy (a quadratic, Le Martelot 2013) */
+static double te_pressure(const Ctx *c, const double *masses, double mixInternalEnerg) {
+  const orc_eos *e1 = &c->cfg->eos[0], *e2 = &c->cfg->eos[1];
+  double rhoMel = 0.;
+  for (int k = 0; k < c->cfg->nph; k++) rhoMel += masses[k];
+  double gamma1 = e1->gamma, pInf1 = te_pinf(e1), cv1 = e1->cv, e01 = e1->eref, Y1 = masses[0] / rhoMel;
+  double gamma2 = e2->gamma, pInf2 = te_pinf(e2), cv2 = e2->cv, e02 = e2->eref, Y2 = masses[1] / rhoMel;
+  double q = Y1 * e01 + Y2 * e02;
+  double cvMel = Y1 * cv1 + Y2 * cv2;
+  double A1 = Y1 * (gamma1 - 1.) * cv1 / cvMel * (rhoMel * (mixInternalEnerg - q) - pInf1);
+  double A2 = Y2 * (gamma2 - 1.) * cv2 / cvMel * (rhoMel * (mixInternalEnerg - q) - pInf2);
+  return 0.5 * (A1 + A2 - (pInf1 + pInf2)) +
+         sqrt(0.25 * (A2 - A1 - (pInf2 - pInf1)) * (A2 - A1 - (pInf2 - pInf1)) + A1 * A2);
+}
+
+/* MixThermalEq::computeTemperature (MixThermalEq.cpp:154-178) */
+static double te_temperature(const Ctx *c, const double *masses, double pressure) {
+  double rhoMel = 0., T = 0.;
+  for (int k = 0; k < c->cfg->nph; k++) rhoMel += masses[k];
+  for (int k = 0; k < c->cfg->nph; k++) {
+    const orc_eos *q = &c->cfg->eos[k];
+    double Yk = masses[k] / rhoMel;
+    T += Yk * (q->gamma - 1.) * q->cv / (pressure + te_pinf(q));
+  }
+  return 1. / (T * rhoMel);
+}
+
+/* FluxThermalEq::buildPrim (FluxThermalEq.cpp:130-162) */
+static void te_build_prim(const Ctx *c, Flux *f, State *s) {
+  int n = c->cfg->nph;
+  double rhoMel = 0.;
+  for (int k = 0; k < n; k++) rhoMel += f->masse[k];
+  for (int d = 0; d < 3; d++) s->mx.u[d] = f->qdm[d] / rhoMel;
+  zero_small_u(s->mx.u);
+  double e = f->energMix / rhoMel - 0.5 * (s->mx.u[0] * s->mx.u[0] + s->mx.u[1] * s->mx.u[1] + s->mx.u[2] * s->mx.u[2]);
+  double p = te_pressure(c, f->masse, e);
+  double T = te_temperature(c, f->masse, p);
+  s->mx.p = p;
+  s->mx.T = T;
+  for (int k = 0; k < n; k++) {
+    Phase *ph = &s->ph[k];
+    const orc_eos *q = &c->cfg->eos[k];
+    ph->p = p;
+    ph->rho = eos_rho_pt(q, p, T, c->epsilon);
+    ph->alpha = f->masse[k] / ph->rho;
+    ph->e = eos_e(q, ph->rho, ph->p, c->epsilon);
+    ph->c = eos_c(q, ph->rho, ph->p, c->epsilon);
+  }
+  te_mixture_vars(c, s);
+  s->mx.E = f->energMix / rhoMel;
 }
 
 /* FluxKapila::buildPrim (FluxKapila.cpp:139-185), FluxMultiP::buildPrim (FluxMultiP.cpp:144-183),
@@ -293,6 +400,7 @@ static void build_prim(const Ctx *c, Flux *f, State *s) {
     p->T = eos_T(q, f->masse[0], p->p, c->epsilon);
     return;
   }
+  if (cfg->model == ORC_THERMALEQ) { te_build_prim(c, f, s); return; }
   double un = 0.;
   if (cfg->model == ORC_KAPILA) {
     if (c->epsilon > 1.e-20) {
@@ -545,10 +653,114 @@ static void hllc_euler(const Ctx *c, const State *Ls, const State *Rs, double dx
   F->sM = sM;
 }
 
+/* ModThermalEq::solveRiemannIntern (ModThermalEq.cpp:94-210): the mixture
+   HLLC with the phases' partial densities */
+static void hllc_te(const Ctx *c, const State *L, const State *R, double dxL, double dxR,
+                    double *dtMax, Flux *F) {
+  int n = c->cfg->nph;
+  double uL = L->mx.u[0], cL = L->mx.cF, pL = L->mx.p, rhoL = L->mx.rho;
+  double uR = R->mx.u[0], cR = R->mx.cF, pR = R->mx.p, rhoR = R->mx.rho;
+  double sL = smin(uL - cL, uR - cR);
+  double sR = smax(uR + cR, uL + cL);
+  if (fabs(sL) > 1.e-3) *dtMax = smin(*dtMax, dxL / fabs(sL));
+  if (fabs(sR) > 1.e-3) *dtMax = smin(*dtMax, dxR / fabs(sR));
+  double mL = rhoL * (sL - uL), mR = rhoR * (sR - uR);
+  double sM = (pR - pL + mL * uL - mR * uR) / (mL - mR);
+  if (fabs(sM) < 1.e-8) sM = 0.;
+  if (sL >= 0. || sR <= 0.) {
+    const State *S = (sL >= 0.) ? L : R;
+    double uS = (sL >= 0.) ? uL : uR, pS = (sL >= 0.) ? pL : pR, rhoS = (sL >= 0.) ? rhoL : rhoR;
+    for (int k = 0; k < n; k++) F->masse[k] = S->ph[k].alpha * S->ph[k].rho * uS;
+    double vitY = S->mx.u[1], vitZ = S->mx.u[2];
+    double totalEnergy = S->mx.e + 0.5 * sqnorm(S->mx.u);
+    F->qdm[0] = rhoS * uS * uS + pS;
+    F->qdm[1] = rhoS * vitY * uS;
+    F->qdm[2] = rhoS * vitZ * uS;
+    F->energMix = (rhoS * totalEnergy + pS) * uS;
+  } else {
+    int left = (sM >= 0.);
+    const State *S = left ? L : R;
+    double uS = left ? uL : uR, pS = left ? pL : pR, mS = left ? mL : mR, sS = left ? sL : sR;
+    double vitY = S->mx.u[1], vitZ = S->mx.u[2];
+    double totalEnergy = S->mx.e + 0.5 * sqnorm(S->mx.u);
+    double rhoStar = mS / (sS - sM);
+    double EStar = totalEnergy + (sM - uS) * (sM + pS / mS);
+    double pStar = mS * (sM - uS) + pS;
+    for (int k = 0; k < n; k++) {
+      double mk = S->ph[k].alpha * S->ph[k].rho * (sS - uS);
+      F->masse[k] = mk / (sS - sM) * sM;
+    }
+    F->qdm[0] = rhoStar * sM * sM + pStar;
+    F->qdm[1] = rhoStar * vitY * sM;
+    F->qdm[2] = rhoStar * vitZ * sM;
+    F->energMix = (rhoStar * EStar + pStar) * sM;
+  }
+  F->sM = sM;
+}
+
 static void riemann(const Ctx *c, const State *L, const State *R, double dxL, double dxR,
                     double *dtMax, Flux *F) {
   if (c->cfg->model == ORC_EULER) hllc_euler(c, L, R, dxL, dxR, dtMax, F);
+  else if (c->cfg->model == ORC_THERMALEQ) hllc_te(c, L, R, dxL, dxR, dtMax, F);
   else hllc_multi(c, L, R, dxL, dxR, dtMax, F);
+}
+
+/* MixThermalEq::computeTemperatureIsentrope and computeVolumeIsentrope
+   (MixThermalEq.cpp:200-226, 256-277): the mixture isentrope at fixed mass
+   fractions from (p0, T0) to p */
+static double te_T_isentrope(const Ctx *c, const double *Yk, double p0, double T0, double p, double *dTdp) {
+  int n = c->cfg->nph;
+  double T = T0, cM = 0.;
+  if (dTdp) *dTdp = 0.;
+  for (int k = 0; k < n; k++) cM += Yk[k] * c->cfg->eos[k].gamma * c->cfg->eos[k].cv;
+  for (int k = 0; k < n; k++) {
+    const orc_eos *q = &c->cfg->eos[k];
+    double puissance = (q->gamma - 1.) * Yk[k] * q->cv / cM;
+    double fk = pow((p + te_pinf(q)) / ((p0 + te_pinf(q))), puissance);
+    T *= fk;
+    if (dTdp) *dTdp += puissance / (p + te_pinf(q));
+  }
+  if (dTdp) *dTdp *= T;
+  return T;
+}
+static double te_v_isentrope(const Ctx *c, const double *Yk, double p0, double T0, double p) {
+  double dTdp = 0.;
+  double T = te_T_isentrope(c, Yk, p0, T0, p, &dTdp);
+  double v = 0.;
+  for (int k = 0; k < c->cfg->nph; k++) {
+    const orc_eos *q = &c->cfg->eos[k];
+    double vk = ((q->gamma - 1.) * q->cv * T) / (p + te_pinf(q));
+    v += Yk[k] * vk;
+  }
+  return v;
+}
+
+/* ModThermalEq::solveRiemannOutflow (ModThermalEq.cpp:372-424): the left
+   isentrope to the imposed p0 */
+static void te_outflow(const Ctx *c, const State *L, double dxL, double *dtMax, double p0, Flux *F) {
+  int n = c->cfg->nph;
+  double uL = L->mx.u[0], cL = L->mx.cF, pL = L->mx.p, rhoL = L->mx.rho, TL = L->mx.T;
+  double vL = L->mx.u[1], wL = L->mx.u[2];
+  double Yk[MAXPH];
+  for (int k = 0; k < n; k++) Yk[k] = L->ph[k].alpha * L->ph[k].rho / rhoL;
+  double zL = rhoL * cL;
+  double pStar = p0;
+  double rhoStar = 1. / te_v_isentrope(c, Yk, pL, TL, p0);
+  double vmv0 = 1. / rhoStar - 1. / rhoL, mL;
+  if (fabs(vmv0) > 1e-10) mL = sqrt((pL - p0) / vmv0);
+  else mL = zL;
+  double sL = uL - mL / rhoL;
+  if (fabs(sL) > 1.e-3) *dtMax = smin(*dtMax, dxL / fabs(sL));
+  double uStar = uL + mL * vmv0;
+  if (sL >= 0.) { uStar = uL; pStar = pL; rhoStar = rhoL; }
+  double totalEnergy = L->mx.e + 0.5 * sqnorm(L->mx.u);
+  double EStar = totalEnergy + (uStar - uL) * (uStar - pL / mL);
+  for (int k = 0; k < n; k++) F->masse[k] = rhoStar * Yk[k] * uStar;
+  F->qdm[0] = uStar * uStar * rhoStar + pStar;
+  F->qdm[1] = uStar * vL * rhoStar;
+  F->qdm[2] = uStar * wL * rhoStar;
+  F->energMix = (EStar * rhoStar + pStar) * uStar;
+  F->sM = uStar;
 }
 
 /* The outflow's half Riemann problem (imposed pressure p0) in the face frame:
@@ -560,6 +772,7 @@ static void riemann(const Ctx *c, const State *L, const State *R, double dxL, do
 static void riemann_outflow(const Ctx *c, const State *L, double dxL, double *dtMax, double p0, Flux *F) {
   const orc_cfg *cfg = c->cfg;
   double eps = c->epsilon;
+  if (cfg->model == ORC_THERMALEQ) { te_outflow(c, L, dxL, dtMax, p0, F); return; }
   if (cfg->model == ORC_EULER) {
     const Phase *P = &L->ph[0];
     double uL = P->u[0], vL = P->u[1], wL = P->u[2], pL = P->p, rhoL = P->rho, cL = P->c;
@@ -1201,6 +1414,14 @@ static void prim_vec(const Ctx *c, const State *s, double *w) {
     w[q++] = s->ph[0].u[0]; w[q++] = s->ph[0].u[1]; w[q++] = s->ph[0].u[2];
     return;
   }
+  if (c->cfg->model == ORC_THERMALEQ) {
+    /* PhaseThermalEq::computeSlopesPhase (alpha), MixThermalEq::computeSlopesMixture
+       (p, T, u, v, w): PhaseThermalEq.cpp:139-142, MixThermalEq.cpp:457-464 */
+    for (int k = 0; k < c->cfg->nph; k++) w[q++] = s->ph[k].alpha;
+    w[q++] = s->mx.p; w[q++] = s->mx.T;
+    w[q++] = s->mx.u[0]; w[q++] = s->mx.u[1]; w[q++] = s->mx.u[2];
+    return;
+  }
   for (int k = 0; k < c->cfg->nph; k++) { w[q++] = s->ph[k].alpha; w[q++] = s->ph[k].rho; w[q++] = s->ph[k].p; }
   w[q++] = s->mx.u[0]; w[q++] = s->mx.u[1]; w[q++] = s->mx.u[2];
   /* Transport::computeSlopeTransport (Transport.cpp:156-159) on the same distance */
@@ -1233,7 +1454,8 @@ static void compute_face_slopes(Solver *S, State *st) {
     cell_pos(m, F->left, pl);
     double dd = pl[F->dir] - F->pos[F->dir];
     if (fabs(dd) > 1.e-8) {
-      int iu = (S->c.cfg->model == ORC_EULER) ? 2 + F->dir : 3 * S->c.cfg->nph + F->dir;
+      int iu = (S->c.cfg->model == ORC_EULER) ? 2 + F->dir
+             : (S->c.cfg->model == ORC_THERMALEQ) ? S->c.cfg->nph + 2 + F->dir : 3 * S->c.cfg->nph + F->dir;
       double w[32];
       prim_vec(&S->c, &st[F->left], w);
       sl[iu] = w[iu] / dd;
@@ -1277,6 +1499,11 @@ static void local_slopes(Solver *S, int cell, int fref, int fopp, int cell_is_le
     for (int v = 0; v < nv; v++) out[v] = limiter(lg, s1[v], s2[v]);
     return;
   }
+  if (cfg->model == ORC_THERMALEQ) {
+    /* PhaseThermalEq::limitSlopes (volume-fraction limiter), MixThermalEq::limitSlopes */
+    for (int v = 0; v < nv; v++) out[v] = limiter(v < cfg->nph ? lv : lg, s1[v], s2[v]);
+    return;
+  }
   int q = 0;
   for (int k = 0; k < cfg->nph; k++) {
     out[q] = limiter(lv, s1[q], s2[q]); q++;
@@ -1308,6 +1535,11 @@ static void local_slopes_limite(Solver *S, int cell, int fwall, int fopp, double
     for (int v = 0; v < nv; v++) out[v] = limiter(lg, s1[v], s2[v]);
     return;
   }
+  if (cfg->model == ORC_THERMALEQ) {
+    /* PhaseThermalEq::limitSlopes (volume-fraction limiter), MixThermalEq::limitSlopes */
+    for (int v = 0; v < nv; v++) out[v] = limiter(v < cfg->nph ? lv : lg, s1[v], s2[v]);
+    return;
+  }
   int q = 0;
   for (int k = 0; k < cfg->nph; k++) {
     out[q] = limiter(lv, s1[q], s2[q]); q++;
@@ -1322,6 +1554,13 @@ static void local_slopes_limite(Solver *S, int cell, int fwall, int fopp, double
    (BoundCondWallO2::solveRiemann, BoundCondWallO2.cpp:115-150, calls none) */
 static void extrapolate_raw(const Ctx *c, State *s, const double *sl, double dist) {
   int q = 0;
+  if (c->cfg->model == ORC_THERMALEQ) {
+    for (int k = 0; k < c->cfg->nph; k++, q++) s->ph[k].alpha += sl[q] * dist;
+    s->mx.p += sl[q] * dist; q++;
+    s->mx.T += sl[q] * dist; q++;
+    for (int d = 0; d < 3; d++, q++) s->mx.u[d] = s->mx.u[d] + sl[q] * dist;
+    return;
+  }
   if (c->cfg->model == ORC_EULER) {
     Phase *p = &s->ph[0];
     p->rho += sl[0] * dist; p->p += sl[1] * dist;
@@ -1340,6 +1579,18 @@ static void extrapolate_raw(const Ctx *c, State *s, const double *sl, double dis
 
 static void extrapolate(const Ctx *c, State *s, const double *sl, double dist, double sign) {
   int q = 0;
+  if (c->cfg->model == ORC_THERMALEQ) {
+    /* PhaseThermalEq::extrapolate + verifyAndCorrectPhase per phase, then
+       MixThermalEq::extrapolate (MixThermalEq.cpp:471-478) */
+    for (int k = 0; k < c->cfg->nph; k++, q++) {
+      s->ph[k].alpha += (sign * sl[q]) * dist;
+      verify_correct(c, k, &s->ph[k]);
+    }
+    s->mx.p += (sign * sl[q]) * dist; q++;
+    s->mx.T += (sign * sl[q]) * dist; q++;
+    for (int d = 0; d < 3; d++, q++) s->mx.u[d] = s->mx.u[d] + (sign * sl[q]) * dist;
+    return;
+  }
   if (c->cfg->model == ORC_EULER) {
     Phase *p = &s->ph[0];
     p->rho += (sign * sl[0]) * dist; p->p += (sign * sl[1]) * dist;
@@ -1482,6 +1733,7 @@ static void accumulate(Solver *S) {
   const Ctx *c = &S->c;
   int n = c->cfg->nph;
   int euler = (c->cfg->model == ORC_EULER);
+  int te = (c->cfg->model == ORC_THERMALEQ);
   double vol = m->vol[0];
 #pragma omp parallel for schedule(static)
   for (int ci = 0; ci < m->ncell; ci++) {
@@ -1496,6 +1748,9 @@ static void accumulate(Solver *S) {
       if (F->right == ci) {
         if (euler) {
           U->masse[0] += coefA * FL->masse[0];
+        } else if (te) {
+          /* FluxThermalEq::addFlux (FluxThermalEq.cpp:62-69): no non-conservative terms */
+          for (int k = 0; k < n; k++) U->masse[k] += coefA * FL->masse[k];
         } else {
           for (int k = 0; k < n; k++) {
             U->alpha[k] += coefA * FL->alpha[k];
@@ -1505,7 +1760,7 @@ static void accumulate(Solver *S) {
         }
         for (int d = 0; d < 3; d++) U->qdm[d] += FL->qdm[d] * coefA;
         U->energMix += coefA * FL->energMix;
-        if (!euler)
+        if (!euler && !te)
           for (int k = 0; k < n; k++) {
             U->alpha[k] += -coefA * sn->ph[k].alpha * FL->sM;
             U->energ[k] += coefA * sn->ph[k].alpha * sn->ph[k].p * FL->sM;
@@ -1519,6 +1774,8 @@ static void accumulate(Solver *S) {
       } else {
         if (euler) {
           U->masse[0] -= coefA * FL->masse[0];
+        } else if (te) {
+          for (int k = 0; k < n; k++) U->masse[k] -= coefA * FL->masse[k];
         } else {
           for (int k = 0; k < n; k++) {
             U->alpha[k] -= coefA * FL->alpha[k];
@@ -1528,7 +1785,7 @@ static void accumulate(Solver *S) {
         }
         for (int d = 0; d < 3; d++) U->qdm[d] -= FL->qdm[d] * coefA;
         U->energMix -= coefA * FL->energMix;
-        if (!euler)
+        if (!euler && !te)
           for (int k = 0; k < n; k++) {
             U->alpha[k] -= -coefA * sn->ph[k].alpha * FL->sM;
             U->energ[k] -= coefA * sn->ph[k].alpha * sn->ph[k].p * FL->sM;
@@ -1556,6 +1813,8 @@ static void scale_add(const Ctx *c, Flux *U, double mult, const Flux *buf) {
   int n = c->cfg->nph;
   if (c->cfg->model == ORC_EULER) {
     U->masse[0] *= mult;
+  } else if (c->cfg->model == ORC_THERMALEQ) {
+    for (int k = 0; k < n; k++) U->masse[k] *= mult;
   } else {
     for (int k = 0; k < n; k++) { U->alpha[k] *= mult; U->masse[k] *= mult; U->energ[k] *= mult; }
   }
@@ -1563,6 +1822,8 @@ static void scale_add(const Ctx *c, Flux *U, double mult, const Flux *buf) {
   U->energMix *= mult;
   if (c->cfg->model == ORC_EULER) {
     U->masse[0] += 1. * buf->masse[0];
+  } else if (c->cfg->model == ORC_THERMALEQ) {
+    for (int k = 0; k < n; k++) U->masse[k] += 1. * buf->masse[k];
   } else {
     for (int k = 0; k < n; k++) {
       U->alpha[k] += 1. * buf->alpha[k];
@@ -1670,6 +1931,16 @@ static void init_state(Solver *S, const double *in) {
       fulfill(c, s);
       continue;
     }
+    if (c->cfg->model == ORC_THERMALEQ) {
+      /* Cell::fill + completeFulfillState: alpha_k, the mixture p and T, u */
+      for (int k = 0; k < n; k++) s->ph[k].alpha = in[(size_t)(3 * k) * nc + ci];
+      s->mx.p = in[(size_t)2 * nc + ci];
+      s->mx.T = in[(size_t)(3 * n + 4) * nc + ci];
+      for (int d = 0; d < 3; d++) s->mx.u[d] = u[d];
+      fulfill(c, s);
+      s->mx.E = s->mx.e + 0.5 * sqnorm(s->mx.u);
+      continue;
+    }
     for (int k = 0; k < n; k++) {
       s->ph[k].alpha = in[(size_t)(3 * k) * nc + ci];
       s->ph[k].rho = in[(size_t)(3 * k + 1) * nc + ci];
@@ -1703,6 +1974,7 @@ static void write_state(Solver *S, double *out) {
     for (int d = 0; d < 3; d++) out[(size_t)(3 * n + d) * nc + ci] = s->mx.u[d];
     out[(size_t)(3 * n + 3) * nc + ci] = s->mx.E;
     for (int t = 0; t < c->cfg->ntr; t++) out[(size_t)(3 * n + 4 + t) * nc + ci] = s->tr[t];
+    if (c->cfg->model == ORC_THERMALEQ) out[(size_t)(3 * n + 4) * nc + ci] = s->mx.T;
   }
 }
 
@@ -1725,6 +1997,13 @@ int orc_run_from(const orc_cfg *cfg, const double *state_in, double *state_out,
   if (is_thinc(cfg) && cfg->nph != 2) return -1;
   /* the Euler model has no transport Riemann solver (Model.h:127, an error) */
   if (cfg->ntr < 0 || cfg->ntr > MAXTR || (cfg->ntr > 0 && cfg->model == ORC_EULER)) return -1;
+  /* ThermalEq: two phases (MixThermalEq::computePressure), no transport
+     solver, injection or tank (ModThermalEq has no solveRiemannInflow; its
+     tank is not restated here) */
+  if (cfg->model == ORC_THERMALEQ) {
+    if (cfg->nph != 2 || cfg->ntr > 0) return -1;
+    for (int b = 0; b < 6; b++) if (cfg->bc[b] == ORC_BC_INJ || cfg->bc[b] == ORC_BC_TANK) return -1;
+  }
   for (int b = 0; b < 6; b++) {
     if (cfg->bc[b] < ORC_BC_ABS || cfg->bc[b] > ORC_BC_TANK) return -1;
 
@@ -1740,7 +2019,7 @@ int orc_run_from(const orc_cfg *cfg, const double *state_in, double *state_out,
   S.c.cfg = cfg;
   S.c.epsilon = (cfg->model == ORC_KAPILA && cfg->alpha_null) ? 1.e-15 : 0.;
   mesh_build(&S.m, cfg);
-  S.nvar = (cfg->model == ORC_EULER) ? 5 : 3 * cfg->nph + 3 + cfg->ntr;
+  S.nvar = (cfg->model == ORC_EULER) ? 5 : (cfg->model == ORC_THERMALEQ) ? cfg->nph + 5 : 3 * cfg->nph + 3 + cfg->ntr;
   S.sn = calloc(S.m.ncell, sizeof(State));
   S.s2 = calloc(S.m.ncell, sizeof(State));
   S.cons = calloc(S.m.ncell, sizeof(Flux));

@@ -20,7 +20,7 @@
 extern "C" {
 #endif
 
-enum { ORC_EULER = 0, ORC_KAPILA = 1, ORC_MULTIP = 2 };
+enum { ORC_EULER = 0, ORC_KAPILA = 1, ORC_MULTIP = 2, ORC_THERMALEQ = 3 };
 enum { ORC_LIM_NONE = 0, ORC_LIM_MINMOD = 1, ORC_LIM_VANLEER = 2, ORC_LIM_VANALBADA = 3,
        ORC_LIM_SUPERBEE = 4, ORC_LIM_MC = 5, ORC_LIM_THINC = 6 };
 enum { ORC_EOS_IG = 0, ORC_EOS_SG = 1 };
@@ -42,7 +42,7 @@ typedef struct {
 } orc_eos;
 
 typedef struct {
-  int model;         /* ORC_EULER / ORC_KAPILA / ORC_MULTIP */
+  int model;         /* ORC_EULER / ORC_KAPILA / ORC_MULTIP / ORC_THERMALEQ */
   int nph;           /* 1 for Euler */
   int order;         /* 1 or 2 */
   int lim[4];        /* global, interface, global volume fraction, interface volume fraction */
@@ -71,6 +71,10 @@ typedef struct {
  *   E  (mixture specific total energy; output only)
  *   the ntr transported scalars
  * => nfields = 3*nph + 4 + ntr.
+ * ThermalEq (two phases in pressure and temperature equilibrium, ModThermalEq):
+ * per phase alpha_k, rho_k, p_k (p_k the mixture pressure, rho_k = rho_k(p, T);
+ * on input alpha_k and p_0 are read), u, v, w, E, then the mixture
+ * temperature T (read on input) => nfields = 3*nph + 5, no transports.
  *
  * orc_run: initialise from (alpha, rho, p, u) exactly as the reference does
  * at t = 0, then march.  nsteps > 0: iteration-controlled run of nsteps

@@ -17,7 +17,7 @@ import numpy as np
 HERE = Path(__file__).resolve().parent
 LIB_PATH = HERE / "libeco_oracle.so"
 
-MODELS = {"euler": 0, "kapila": 1, "multip": 2}
+MODELS = {"euler": 0, "kapila": 1, "multip": 2, "thermaleq": 3}
 LIMITERS = {"none": 0, "minmod": 1, "vanleer": 2, "vanalbada": 3, "superbee": 4, "mc": 5, "thinc": 6}
 EOS_TYPES = {"IG": 0, "SG": 1}
 # per side X-, X+, Y-, Y+, Z-, Z+; outflow pressures in case["bc_p0"]; injections in
@@ -141,7 +141,8 @@ def ntransports(case) -> int:
 
 
 def nfields(case) -> int:
-    return 3 * len(case["eos"]) + 4 + ntransports(case)
+    """per phase alpha, rho, p; u, v, w; E; the transports; ThermalEq: T last"""
+    return 3 * len(case["eos"]) + 4 + ntransports(case) + (case["model"] == "thermaleq")
 
 
 def run(case, state, nsteps=None, t_final=None, max_steps=0, nthreads=0, log_dt=False, resume=None):
@@ -180,9 +181,12 @@ def state_from_golden(case, d, prefix="init"):
     nph = len(case["eos"])
     ncell = int(np.prod(case["cells"]))
     ntr = ntransports(case)
-    out = np.zeros((3 * nph + 4 + ntr if ntr else 3 * nph + 3, ncell))
+    te = case["model"] == "thermaleq"
+    out = np.zeros((3 * nph + 4 + ntr + te if (ntr or te) else 3 * nph + 3, ncell))
     if ntr:
-        out[3 * nph + 4:] = d[f"{prefix}_tr"]
+        out[3 * nph + 4:3 * nph + 4 + ntr] = d[f"{prefix}_tr"]
+    if te:
+        out[3 * nph + 4] = d[f"{prefix}_T"]
     if case["model"] == "euler":
         out[0] = 1.0
         out[1] = d[f"{prefix}_rho"][0]

@@ -78,6 +78,13 @@ def parity_quantities(case, state):
     if case["model"] == "euler":
         out["rho"] = state[1]
         out["p"] = state[2]
+    elif case["model"] == "thermaleq":
+        # one pressure and temperature (the temperature after E)
+        out["rho"] = sum(state[3 * k] * state[3 * k + 1] for k in range(nph))
+        out["p"] = state[2]
+        out["T"] = state[3 * nph + 4]
+        for k in range(nph):
+            out[f"alpha{k}"] = state[3 * k]
     else:
         out["rho"] = sum(state[3 * k] * state[3 * k + 1] for k in range(nph))
         out["p"] = sum(state[3 * k] * state[3 * k + 2] for k in range(nph))
@@ -114,9 +121,11 @@ def compared_fields(case):
             out += [(3 * k, f"alpha{k}"), (3 * k + 1, f"rho{k}"), (3 * k + 2, f"p{k}")]
     for d in range(dim):
         out.append((3 * nph + d, "uvw"[d]))
-    # transported scalars after E
+    # transported scalars after E; the thermal-equilibrium temperature
     for t in range(len(case.get("transports", []))):
         out.append((3 * nph + 4 + t, f"tr{t}"))
+    if case["model"] == "thermaleq":
+        out.append((3 * nph + 4, "T"))
     return out
 
 

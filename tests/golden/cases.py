@@ -284,6 +284,21 @@ def _interface_advection_1d():
     return _with_transports(c, ["color"], {"water": [1.0], "air": [2.0]})
 
 
+def _thermaleq(case):
+    """``case`` in the thermal-equilibrium model (ModThermalEq): each state
+    keeps its volume fractions, pressure and velocity and takes the
+    temperature of its air at the given density, T = p / ((gamma - 1) cv rho)."""
+    c = dict(case, model="thermaleq", relax_p=False)
+    c.pop("alpha_null", None)
+    states = []
+    for st in case["states"]:
+        p = st["p"][0]
+        T = p / ((AIR["gamma"] - 1.0) * AIR["cv"] * st["rho"][0])
+        states.append({"name": st["name"], "alpha": list(st["alpha"]), "p": [p], "T": T, "u": list(st["u"])})
+    c["states"] = states
+    return c
+
+
 CASES = {
     # Config C1 in full: 1000 cells, first order, to t = 240 us.
     "c1_kapila_fo": _c1(False),
@@ -420,6 +435,23 @@ CASES = {
              bc_inj=_inj(2, 300.0, 2, x_side=0)),
         ["color"], {"water": [1.0], "shocked": [0.5]}, bc_tr=[None, [0.0], [2.0]] + [None] * 3),
     "multip_3d_o2_transport": _with_transports(_bubbles_3d(n=12, steps=10), ["color"], {"bubble": [1.0]}),
+    # Thermal-equilibrium model (ModThermalEq, FluxThermalEq, MixThermalEq):
+    # two phases at one pressure and temperature, the mixture HLLC, the
+    # closed-form pressure of buildPrim; first order and MUSCL (THINC with an
+    # interface limiter), walls and outflows, 1-D/2-D/3-D
+    "thermaleq_1d_fo": _thermaleq(_shock_tube_1d("kapila", 1, ["none"] * 4, 200, cells=200)),
+    "thermaleq_1d_o2_vanleer": _thermaleq(_shock_tube_1d("kapila", 2, ["vanleer"] * 4, 120)),
+    "thermaleq_1d_o2_thinc": _thermaleq(_shock_tube_1d("kapila", 2, ["minmod", "vanleer", "minmod", "thinc"], 120)),
+    "thermaleq_1d_o2_wall_outflow": _thermaleq(dict(_shock_tube_1d("kapila", 2, ["minmod", "vanleer", "minmod", "vanleer"], 300),
+                                                    bc=["wall", "outflow"] + ["abs"] * 4, bc_p0=[0.0, 5e4] + [0.0] * 4)),
+    "thermaleq_1d_fo_outflow": _thermaleq(dict(_shock_tube_1d("kapila", 1, ["none"] * 4, 300),
+                                               bc=["outflow", "outflow"] + ["abs"] * 4, bc_p0=[2e9, 5e4] + [0.0] * 4)),
+    "thermaleq_2d_fo_wall": _thermaleq(dict(_shock_cylinders_2d(n=40, steps=60), order=1, limiters=["none"] * 4,
+                                            bc=["wall"] * 6)),
+    "thermaleq_2d_o2": _thermaleq(dict(_shock_cylinders_2d(n=40, steps=40), bc=["abs", "outflow", "wall", "wall", "abs", "abs"],
+                                       bc_p0=[0.0, 8e4] + [0.0] * 4)),
+    "thermaleq_3d_o2": _thermaleq(_bubbles_3d(n=12, steps=10, model="kapila", relax=False, limiters=("vanleer",) * 4)),
+    "thermaleq_2d_o2_abs": _thermaleq(_shock_cylinders_2d(n=40, steps=30, limiters=("minmod",) * 4)),
 }
 
 

@@ -57,6 +57,17 @@ def fields_from_vtr(case, d):
     model = case["model"]
     nph = len(case["eos"])
     res = {}
+    if model == "thermaleq":
+        # PhaseThermalEq prints Alpha and Density, MixThermalEq the mixture
+        # density, pressure, temperature and velocity (ThermalEq*.cpp returnName*)
+        res["alpha"] = np.stack([d[[k for k in d if k.startswith(f"F{i}_Alpha")][0]] for i in range(nph)])
+        res["rho"] = np.stack([d[[k for k in d if k.startswith(f"F{i}_Density")][0]] for i in range(nph)])
+        res["p"] = np.stack([d["Pressure_Mixture"]] * nph)
+        res["p_mix"] = d["Pressure_Mixture"]
+        res["rho_mix"] = d["Density_Mixture"]
+        res["T"] = d["Temperature_Mixture"]
+        res["u"] = d["Velocity_Mixture_norm"].T.copy()
+        return res
     if model == "euler":
         key = [k for k in d if k.startswith("F0_Density")][0]
         res["rho"] = d[key][None, :]

@@ -22,7 +22,7 @@ HERE = Path(__file__).resolve().parent
 REPO = HERE.parent.parent
 sys.path.insert(0, str(HERE))
 sys.path.insert(0, str(REPO / "oracle"))
-from cases import CASES, _bubbles_3d, _euler_1d, _shock_tube_1d, _tank, _with_transports  # noqa: E402
+from cases import CASES, _bubbles_3d, _euler_1d, _shock_tube_1d, _tank, _thermaleq, _with_transports  # noqa: E402
 from deck import write_deck  # noqa: E402
 
 REF_BIN = REPO / "oracle" / "_ref" / "ECOGEN"
@@ -40,6 +40,10 @@ FIXTURES = {
         dict(_shock_tube_1d("kapila", 2, ["minmod", "vanleer", "minmod", "vanleer"], 15, cells=30),
              bc=["wall", "tank"] + ["abs"] * 4, bc_tank=_tank(2, 3e5, 300.0, 1)),
         ["color", "dye"], {"LP": [1.0, 0.5], "HP": [0.0, 2.0]}, bc_tr=[None, [5.0, -1.0]] + [None] * 4),
+    # the thermal-equilibrium model's arrays (Alpha, Density; mixture density,
+    # pressure, temperature, Velocity_Mixture_norm)
+    "thermaleq_1d_o2_30": _thermaleq(_shock_tube_1d("kapila", 2, ["vanleer"] * 4, 12, cells=30)),
+    "thermaleq_2d_fo_6x4": _thermaleq(dict(CASES["kapila_2d_fo_wall"], cells=[6, 4, 1], steps=3)),
     "multip_2d_fo_transport_6x4": _with_transports(
         dict(CASES["multip_2d_fo_relax"], cells=[6, 4, 1], steps=3), ["color"], {"water": [1.0], "shocked": [0.25]}),
 }

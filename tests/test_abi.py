@@ -172,3 +172,27 @@ def test_transport_configs_are_validated_without_device(lib):
                              5, C.c_double(0.0), None, C.byref(st))
     assert rc == -2
     assert capi.nfields(c) == 3 * 2 + 4 + 2
+
+
+def test_thermaleq_configs_are_validated_without_device(lib):
+    """The thermal-equilibrium model: two phases (MixThermalEq::computePressure
+    refuses more), no transport or injection solver (the reference ends the
+    run there), the tank and FP32 storage not provided; its host state carries
+    the temperature after E."""
+    import ctypes as C
+    from ecogen_amd import capi
+    from cases import CASES
+
+    def create(c):
+        ctx = C.c_void_p()
+        return lib.eco_create(C.byref(capi.make_config(c)), None, C.byref(ctx))
+    te = CASES["thermaleq_1d_o2_vanleer"]
+    assert create(dict(te, precision="fp32s")) == -2
+    assert create(dict(te, transports=["c"])) == -1
+    assert create(dict(te, eos=te["eos"] + [te["eos"][0]])) == -1
+    inj = CASES["kapila_1d_o2_injection"]
+    assert create(dict(te, bc=inj["bc"], bc_inj=inj["bc_inj"])) == -1
+    tk = CASES["kapila_1d_o2_tank"]
+    assert create(dict(te, bc=tk["bc"], bc_tank=tk["bc_tank"])) == -2
+    assert capi.nfields(te) == 3 * 2 + 5
+    assert lib.eco_noalloc_bytes(C.byref(capi.make_config(te))) == 0

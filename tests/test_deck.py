@@ -105,7 +105,8 @@ def test_deck_refusals(tmp_path):
 @pytest.mark.gpu
 @pytest.mark.parametrize("name", ["c1_kapila_fo_wall", "kapila_2d_fo_wall", "multip_1d_o2_relax",
                                   "kapila_1d_o2_injection", "multip_1d_o2_relax_tank",
-                                  "kapila_1d_o2_transport_wall_tank", "multip_2d_fo_relax_transport"])
+                                  "kapila_1d_o2_transport_wall_tank", "multip_2d_fo_relax_transport",
+                                  "thermaleq_1d_fo_outflow", "thermaleq_2d_o2"])
 def test_deck_runs_on_the_device(name, tmp_path):
     """python -m ecogen_amd.deck on a reference deck: the final state matches
     the reference's output (1e-12 relative L1, the parity bar)."""
@@ -130,7 +131,8 @@ def test_deck_runs_on_the_device(name, tmp_path):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", ["kapila_2d_fo_wall_6x4", "euler_1d_fo_30", "kapila_1d_o2_transport_30"])
+@pytest.mark.parametrize("name", ["kapila_2d_fo_wall_6x4", "euler_1d_fo_30", "kapila_1d_o2_transport_30",
+                                  "thermaleq_1d_o2_30"])
 def test_deck_vtk_results_match_reference_files(name, tmp_path):
     """python -m ecogen_amd.deck --vtk writes the reference's result files:
     the collection file character for character, and the last result file

@@ -24,7 +24,7 @@ def _cases():
         case, _ = load_golden(n)
         # boundaries other than transmissive, three phases and transports run
         # in the FP64 path only (the FP32-storage kernels refuse them)
-        if (case["order"] == 2 and case["model"] != "euler" and len(case["eos"]) == 2
+        if (case["order"] == 2 and case["model"] in ("kapila", "multip") and len(case["eos"]) == 2
                 and set(case.get("bc", ["abs"])) <= {"abs"} and not case.get("transports")):
             out.append(n)
     return out
