@@ -54,7 +54,7 @@ class EcoConfig(C.Structure):
                 ("bc_inj_p", (C.c_double * 4) * 6), ("bc_tank_p0", C.c_double * 6),
                 ("bc_tank_T0", C.c_double * 6), ("bc_tank_frac", (C.c_double * 4) * 6),
                 ("bc_tank_massfrac", C.c_int * 6), ("ntransport", C.c_int),
-                ("bc_tr", (C.c_double * MAX_TRANSPORTS) * 6)]
+                ("bc_tr", (C.c_double * MAX_TRANSPORTS) * 6), ("relax_pt", C.c_int)]
 
 
 class EcoStatus(C.Structure):
@@ -165,6 +165,7 @@ def make_config(case) -> EcoConfig:
     for k, e in enumerate(case["eos"]):
         c.eos[k] = EcoEos(EOS_TYPE[e["type"]], e["gamma"], e.get("pinf", 0.0), e["cv"], e.get("eref", 0.0))
     c.ntransport = len(case.get("transports", []))
+    c.relax_pt = int(bool(case.get("relax_pt", False)))
     for b, vals in enumerate(case.get("bc_tr") or [None] * 6):
         for t, v in enumerate(vals or []):
             c.bc_tr[b][t] = v

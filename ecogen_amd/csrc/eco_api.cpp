@@ -64,6 +64,7 @@ eco_config lowerConfig(const Model &model, const std::vector<std::unique_ptr<Eos
   c.relax_p = 0;
   if (model.kind() == ECO_MODEL_MULTIP)
     for (const auto &r : model.relaxations()) if (r->getType() == "P") c.relax_p = 1;
+  for (const auto &r : model.relaxations()) if (r->getType() == "PT") c.relax_pt = 1;
   c.precision = precision;
   for (int d = 0; d < 3; d++) { c.n[d] = mesh.numberCells[d]; c.L[d] = mesh.dimensions[d]; }
   c.cfl = cfl;

@@ -239,6 +239,11 @@ int eco::validate_config(const eco_config *c) {
   if (c->ntransport < 0 || c->ntransport > ECO_MAX_TRANSPORTS) return ECO_ERR_INVALID;
   if (c->ntransport > 0 && c->model == ECO_MODEL_EULER) return ECO_ERR_INVALID;
   if (c->ntransport > 0 && c->precision == ECO_FP32S) return ECO_ERR_UNSUPPORTED;
+  // RelaxationPT: two phases (RelaxationPT.cpp:60), Kapila or MultiP, FP64
+  if (c->relax_pt) {
+    if (c->nphase != 2 || (c->model != ECO_MODEL_KAPILA && c->model != ECO_MODEL_MULTIP)) return ECO_ERR_INVALID;
+    if (c->precision != ECO_FP64) return ECO_ERR_UNSUPPORTED;
+  }
   // FP32 storage: MUSCL multiphase path (the batched/served configurations)
   if (c->precision == ECO_FP32S && (c->order != 2 || c->model == ECO_MODEL_EULER)) return ECO_ERR_UNSUPPORTED;
   return ECO_OK;
@@ -296,6 +301,7 @@ void eco::lower_config(const eco_config &c, Lowered &L, bool with_tables) {
   S.alpha_null = c.alpha_null;
   S.eps = (c.model == ECO_MODEL_KAPILA && c.alpha_null) ? 1.e-15 : 0.;
   S.relax = (c.model == ECO_MODEL_KAPILA) || (c.model == ECO_MODEL_MULTIP && c.relax_p);
+  S.relax_pt = c.relax_pt ? 1 : 0;
   S.cfl = c.cfl;
   S.prec = c.precision;
   S.bc_wall = 0;
@@ -313,7 +319,8 @@ void eco::lower_config(const eco_config &c, Lowered &L, bool with_tables) {
   for (int b = 0; b < 6; b++) if (c.bc[b] == ECO_BC_INJECTION) lower_injection(c, b, S);
   for (int b = 0; b < 6; b++) if (c.bc[b] == ECO_BC_TANK) lower_tank(c, b, S);
   L.nprim = (c.model == ECO_MODEL_EULER) ? 2 + dim : (c.model == ECO_MODEL_THERMALEQ) ? c.nphase + 2 + dim : 3 * c.nphase + dim;
-  S.ncache = (c.model == ECO_MODEL_MULTIP && c.relax_p && c.precision == ECO_FP64) ? (c.order == 1 ? c.nphase + 3 : 1) : 0;
+  S.ncache = (c.model == ECO_MODEL_MULTIP && (c.relax_p || c.relax_pt) && c.precision == ECO_FP64)
+                 ? (c.order == 1 ? c.nphase + 3 : 1) : 0;
   // ThermalEq first order: the cells' cached mixture energy
   if (c.model == ECO_MODEL_THERMALEQ && c.order == 1) S.ncache = 1;
   L.nlevel = L.nprim + 1 + S.ncache;

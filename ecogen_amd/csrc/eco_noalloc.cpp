@@ -112,7 +112,7 @@ extern "C" unsigned long long eco_noalloc_bytes(const eco_config *cfg) {
   int dim = (c.n[0] > 1) + (c.n[1] > 1) + (c.n[2] > 1);
   const size_t nc = (size_t)c.n[0] * c.n[1] * c.n[2];
   const int nprim = (c.model == ECO_MODEL_EULER) ? 2 + dim : 3 * c.nphase + dim;
-  const int ncache = (c.model == ECO_MODEL_MULTIP && c.relax_p) ? (c.order == 1 ? c.nphase + 3 : 1) : 0;
+  const int ncache = (c.model == ECO_MODEL_MULTIP && (c.relax_p || c.relax_pt)) ? (c.order == 1 ? c.nphase + 3 : 1) : 0;
   const size_t nlevel = (size_t)nprim + 1 + ncache;
   size_t ntab = 0;
   for (int d = 0; d < 3; d++) ntab += 3 * (size_t)std::max(c.n[d], 1);

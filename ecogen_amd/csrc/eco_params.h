@@ -50,6 +50,7 @@ struct SchemeParams {
   int alpha_null;
   double eps;        // 1e-15 when alphaNull, else 0 (src/Eos/Eos.cpp:114-122)
   int relax;         // stiff pressure relaxation applied
+  int relax_pt;      // pressure-temperature relaxation after it (RelaxationPT, two phases)
   // Values of the reference's cell cache a time level stores after E (MultiP
   // with relaxation, FP64): FluxMultiP::buildPrim sets them, the relaxation
   // changes alpha_k, rho_k, p_k but leaves them (RelaxationP.cpp:49-111), so
@@ -59,7 +60,8 @@ struct SchemeParams {
   // whose faces take the cells' cached states, CellInterface.cpp:127-146):
   // the mixture density, the phase energies e_k, the frozen sound speed and
   // the mixture pressure (the relaxed p*, or sum alpha_k p_k when the
-  // relaxation did not converge).
+  // relaxation did not converge).  ThermalEq at first order: 1, the cells'
+  // mixture internal energy (sum Y_k e_k initially, then E - |u|^2/2).
   int ncache;
   double cfl;
   int prec;          // PREC_FP64 or PREC_FP32S (storage)

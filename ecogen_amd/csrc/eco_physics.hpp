@@ -508,6 +508,43 @@ ECO_D bool relax_p(const SchemeParams &S, CellState<NPH> &s) {
   return false;
 }
 
+// RelaxationPT::stiffRelaxation (RelaxationPT.cpp:49-84) with
+// analyticalPressure (88-118) and analyticalTemperature (122-141), two
+// phases: from the mixture energy, density and mass fractions of the last
+// computeMixtureVariables (s.em, s.rho, s.Y: buildPrim's, before any
+// relaxation), the equilibrium pressure and temperature in closed form; the
+// phases take them at constant partial densities alpha_k rho_k.
+template <int MODEL, int NPH>
+ECO_D void relax_pt(const SchemeParams &S, CellState<NPH> &s) {
+  if constexpr (NPH == 2 && (MODEL == MODEL_KAPILA || MODEL == MODEL_MULTIP)) {
+    if (S.eps > 1.e-20) {
+#pragma unroll
+      for (int k = 0; k < NPH; k++) if (s.a[k] > (1. - 1.e-5)) return;
+    }
+    const EosParams &q1 = S.eos[0], &q2 = S.eos[1];
+    const double e0 = s.em, rho0 = s.rho;
+    const double pInf1 = te_pinf(q1), pInf2 = te_pinf(q2), Y1 = s.Y[0], Y2 = s.Y[1];
+    const double qm = Y1 * q1.eref + Y2 * q2.eref;
+    const double cvMel = Y1 * q1.cv + Y2 * q2.cv;
+    const double A1 = Y1 * (q1.gamma - 1.) * q1.cv / cvMel * (rho0 * (e0 - qm) - pInf1);
+    const double A2 = Y2 * (q2.gamma - 1.) * q2.cv / cvMel * (rho0 * (e0 - qm) - pInf2);
+    const double pStar = 0.5 * (A1 + A2 - (pInf1 + pInf2)) +
+                         sqrt(0.25 * (A2 - A1 - (pInf2 - pInf1)) * (A2 - A1 - (pInf2 - pInf1)) + A1 * A2);
+    double T = 0.;
+#pragma unroll
+    for (int k = 0; k < NPH; k++) T += s.Y[k] * (S.eos[k].gamma - 1.) * S.eos[k].cv / (pStar + te_pinf(S.eos[k]));
+    T = 1. / (T * rho0);
+#pragma unroll
+    for (int k = 0; k < NPH; k++) {
+      const double rkS = eos_rho_pt(S.eos[k], pStar, T, S.eps);
+      s.a[k] = s.a[k] * s.r[k] / rkS;
+      s.r[k] = rkS;
+      s.p[k] = pStar;
+    }
+    s.pm = pStar;
+  }
+}
+
 // FluxKapila::correctionEnergy (FluxKapila.cpp:239-268), after
 // Mixture::totalEnergyToInternalEnergy (MixKapila.cpp:188-195).  s.rho is the
 // mixture density cached by buildPrim (before relaxation).

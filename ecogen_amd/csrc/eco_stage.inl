@@ -1191,6 +1191,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int slot, c
     bool ok = true;
     if constexpr (MODEL != MODEL_EULER) {
       if (S.relax) ok = relax_p<MODEL, NPH>(S, s1);
+      if (S.relax_pt) relax_pt<MODEL, NPH>(S, s1);
       correction_energy<MODEL, NPH>(S, s1);
     }
     if (!ok) atomicOr(&ctl->err, 1u);

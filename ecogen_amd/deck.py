@@ -317,10 +317,18 @@ def load(rundir):
     if len(eos_files) != nph:
         raise DeckError(f"modelV4.xml: {len(eos_files)} EOS for {nph} phases")
     case["eos"] = [dict(_eos(rundir / "libEOS" / f), name=Path(f).stem, file=f) for f in eos_files]
+    # relaxations in deck order (Input.cpp:607-621), after Kapila's own P
+    # relaxation (ModKapila.cpp:47-53): MultiP takes P, PT or P then PT,
+    # Kapila PT; RelaxationPT needs two phases (RelaxationPT.cpp:60)
     relax = [r.get("type", "").upper() for r in model_x.findall("relaxation")]
-    if any(r != "P" for r in relax) or (relax and model == "thermaleq"):
-        raise DeckError(f"modelV4.xml: relaxation {relax} (only P, for Kapila or MultiP)")
+    allowed = {"multip": (["P"], ["PT"], ["P", "PT"]), "kapila": (["PT"],)}.get(model, ())
+    if relax and relax not in [list(a) for a in allowed]:
+        raise DeckError(f"modelV4.xml: relaxations {relax} for {model} (MultiP: P, PT or P then PT; Kapila: PT)")
+    if "PT" in relax and nph != 2:
+        raise DeckError("modelV4.xml: the PT relaxation takes two phases")
     case["relax_p"] = model == "multip" and "P" in relax
+    if "PT" in relax:
+        case["relax_pt"] = True
     # transport equations (Input.cpp:472-473, 528-540): numberTransports of
     # the first <transport name> elements
     ntr = int(float(fm.get("numberTransports", "0")))

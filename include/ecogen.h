@@ -118,6 +118,10 @@ typedef struct {
      (Transport.cpp:83-104). */
   int ntransport;
   double bc_tr[6][ECO_MAX_TRANSPORTS];
+  /* <relaxation type="PT"/> (RelaxationPT, src/Relaxations/RelaxationPT.cpp:
+     49-141) after the pressure relaxation: Kapila and MultiP, two phases,
+     FP64 */
+  int relax_pt;
 } eco_config;
 
 typedef struct eco_ctx eco_ctx;

@@ -89,6 +89,12 @@ class RelaxationP : public Relaxation {
  public:
   std::string getType() const override { return "P"; }
 };
+// Pressure-temperature relaxation in closed form, two phases (RelaxationPT.cpp:49-141),
+// after the pressure relaxation
+class RelaxationPT : public Relaxation {
+ public:
+  std::string getType() const override { return "PT"; }
+};
 
 // --- Flux (src/Models/Flux.h): conservative vector of one model ---------------
 class Flux {

@@ -146,6 +146,9 @@ def write_deck(case, root: Path):
     eosl = "\n".join(f'  <EOS name="{_eos_file(e)}"/>' for e in case["eos"])
     eosl += "".join(f'\n  <transport name="{t}"/>' for t in trs)
     relax = '  <relaxation type="P"/>\n' if case.get("relax_p") and model == "multip" else ""
+    if case.get("relax_pt"):
+        # Input.cpp:607-621: relaxations in deck order, after Kapila's own P
+        relax += '  <relaxation type="PT"/>\n'
     (cdir / "modelV4.xml").write_text(
         '<?xml version="1.0" encoding="UTF-8" standalone="yes"?>\n<model>\n'
         f'  {fm}\n{eosl}\n{relax}</model>\n')

@@ -526,6 +526,41 @@ static int model_relaxes(const orc_cfg *cfg) {
   return cfg->model == ORC_KAPILA || (cfg->model == ORC_MULTIP && cfg->relax_p);
 }
 
+/* RelaxationPT::stiffRelaxation (RelaxationPT.cpp:49-84) with
+   analyticalPressure (88-118) and analyticalTemperature (122-141): from the
+   mixture energy, density and mass fractions the cell cached at its last
+   computeMixtureVariables (before any relaxation), the pressure and
+   temperature of two phases in equilibrium in closed form; the phases take
+   them at constant partial densities alpha_k rho_k. */
+static void relax_pt(const Ctx *c, State *s) {
+  int n = c->cfg->nph;
+  if (c->epsilon > 1.e-20)
+    for (int k = 0; k < n; k++) if (s->ph[k].alpha > (1. - 1.e-5)) return;
+  const orc_eos *e1 = &c->cfg->eos[0], *e2 = &c->cfg->eos[1];
+  double e0 = s->mx.e, rho0 = s->mx.rho;
+  double gamma1 = e1->gamma, pInf1 = te_pinf(e1), cv1 = e1->cv, e01 = e1->eref, Y1 = s->ph[0].Y;
+  double gamma2 = e2->gamma, pInf2 = te_pinf(e2), cv2 = e2->cv, e02 = e2->eref, Y2 = s->ph[1].Y;
+  double q = Y1 * e01 + Y2 * e02;
+  double cvMel = Y1 * cv1 + Y2 * cv2;
+  double A1 = Y1 * (gamma1 - 1.) * cv1 / cvMel * (rho0 * (e0 - q) - pInf1);
+  double A2 = Y2 * (gamma2 - 1.) * cv2 / cvMel * (rho0 * (e0 - q) - pInf2);
+  double pStar = 0.5 * (A1 + A2 - (pInf1 + pInf2)) +
+                 sqrt(0.25 * (A2 - A1 - (pInf2 - pInf1)) * (A2 - A1 - (pInf2 - pInf1)) + A1 * A2);
+  double T = 0.;
+  for (int k = 0; k < n; k++) {
+    const orc_eos *q2 = &c->cfg->eos[k];
+    T += s->ph[k].Y * (q2->gamma - 1.) * q2->cv / (pStar + te_pinf(q2));
+  }
+  T = 1. / (T * rho0);
+  double rkS[MAXPH], akS[MAXPH];
+  for (int k = 0; k < n; k++) {
+    rkS[k] = eos_rho_pt(&c->cfg->eos[k], pStar, T, c->epsilon);
+    akS[k] = s->ph[k].alpha * s->ph[k].rho / rkS[k];
+  }
+  for (int k = 0; k < n; k++) { s->ph[k].alpha = akS[k]; s->ph[k].rho = rkS[k]; s->ph[k].p = pStar; }
+  s->mx.p = pStar;
+}
+
 /* FluxKapila::correctionEnergy (FluxKapila.cpp:239-268); the trailing
    cell->fulfillState() acts on the time-n state (Cell.cpp:282). */
 static void correction_energy(const Ctx *c, State *s_type, State *s_n) {
@@ -1843,6 +1878,7 @@ static void solve_relaxations(Solver *S) {
   for (int ci = 0; ci < S->m.ncell; ci++) {
     State *s = &S->sn[ci];
     if (model_relaxes(c->cfg)) relax_p(c, s);
+    if (c->cfg->relax_pt) relax_pt(c, s);
     total_to_internal(c, s);
     correction_energy(c, s, s);
   }
@@ -1893,6 +1929,7 @@ static double step(Solver *S, double dt) {
       build_prim(c, &S->cons[ci], s2);
       transport_update(c, &S->cons[ci], 0.5 * dt, sn, s2);
       if (model_relaxes(c->cfg) && c->cfg->nph > 1) relax_p(c, s2);
+      if (c->cfg->relax_pt) relax_pt(c, s2);
       total_to_internal(c, s2);
       correction_energy(c, s2, sn);
       fulfill(c, s2);
@@ -2000,6 +2037,8 @@ int orc_run_from(const orc_cfg *cfg, const double *state_in, double *state_out,
   /* ThermalEq: two phases (MixThermalEq::computePressure), no transport
      solver, injection or tank (ModThermalEq has no solveRiemannInflow; its
      tank is not restated here) */
+  /* RelaxationPT: two phases (RelaxationPT.cpp:60), Kapila or MultiP */
+  if (cfg->relax_pt && (cfg->nph != 2 || (cfg->model != ORC_KAPILA && cfg->model != ORC_MULTIP))) return -1;
   if (cfg->model == ORC_THERMALEQ) {
     if (cfg->nph != 2 || cfg->ntr > 0) return -1;
     for (int b = 0; b < 6; b++) if (cfg->bc[b] == ORC_BC_INJ || cfg->bc[b] == ORC_BC_TANK) return -1;

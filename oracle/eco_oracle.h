@@ -61,6 +61,7 @@ typedef struct {
   int bc_tank_massfrac[6];               /* tank: 1 if bc_tank_frac holds mass fractions */
   int ntr;                               /* transported scalars (<flowModel numberTransports>); not Euler */
   double bc_tr[6][ORC_MAXTR];            /* outflow, injection, tank: <transport value> per transport */
+  int relax_pt;      /* Kapila, MultiP: <relaxation type="PT"/> after the P relaxation (two phases) */
 } orc_cfg;
 
 /*

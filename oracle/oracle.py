@@ -43,7 +43,8 @@ class OrcCfg(C.Structure):
                 ("bc_ak0", (C.c_double * MAXPH) * 6), ("bc_rhok0", (C.c_double * MAXPH) * 6),
                 ("bc_pk0", (C.c_double * MAXPH) * 6), ("bc_tank_p0", C.c_double * 6),
                 ("bc_tank_T0", C.c_double * 6), ("bc_tank_frac", (C.c_double * MAXPH) * 6),
-                ("bc_tank_massfrac", C.c_int * 6), ("ntr", C.c_int), ("bc_tr", (C.c_double * MAXTR) * 6)]
+                ("bc_tank_massfrac", C.c_int * 6), ("ntr", C.c_int), ("bc_tr", (C.c_double * MAXTR) * 6),
+                ("relax_pt", C.c_int)]
 
 
 _lib = None
@@ -130,6 +131,7 @@ def make_cfg(case) -> OrcCfg:
             for k, v in enumerate(frac):
                 cfg.bc_tank_frac[b][k] = v
     cfg.ntr = len(case.get("transports", []))
+    cfg.relax_pt = int(bool(case.get("relax_pt", False)))
     for b, vals in enumerate(case.get("bc_tr", [None] * 6)):
         for t, v in enumerate(vals or []):
             cfg.bc_tr[b][t] = v

@@ -59,7 +59,10 @@ static int check_lower() {
   CHECK(c.relax_p == 0 && c.precision == ECO_FP32S);
   mp.getRelaxations()->emplace_back(new RelaxationP);
   c = lowerConfig(mp, eos, m, 0.5, 2, l, ECO_FP32S);
-  CHECK(c.relax_p == 1);
+  CHECK(c.relax_p == 1 && c.relax_pt == 0);
+  mp.getRelaxations()->emplace_back(new RelaxationPT);
+  c = lowerConfig(mp, eos, m, 0.5, 2, l, ECO_FP64);
+  CHECK(c.relax_p == 1 && c.relax_pt == 1);
   // transport equations: the model's count and the sides' entering values
   ModKapila kt(2, false);
   kt.setNumberTransports(2);

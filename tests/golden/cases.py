@@ -452,6 +452,16 @@ CASES = {
                                        bc_p0=[0.0, 8e4] + [0.0] * 4)),
     "thermaleq_3d_o2": _thermaleq(_bubbles_3d(n=12, steps=10, model="kapila", relax=False, limiters=("vanleer",) * 4)),
     "thermaleq_2d_o2_abs": _thermaleq(_shock_cylinders_2d(n=40, steps=30, limiters=("minmod",) * 4)),
+    # Pressure-temperature relaxation (RelaxationPT) after the pressure one:
+    # Kapila (which relaxes P itself) and MultiP with P then PT, or PT alone
+    "kapila_1d_fo_relax_pt": dict(_shock_tube_1d("kapila", 1, ["none"] * 4, 200), relax_pt=True),
+    "kapila_1d_o2_relax_pt": dict(_shock_tube_1d("kapila", 2, ["vanleer"] * 4, 120), relax_pt=True),
+    "multip_1d_o2_relax_p_pt": dict(_shock_tube_1d("multip", 2, ["minmod"] * 4, 120, relax=True), relax_pt=True),
+    "multip_1d_o2_pt": dict(_shock_tube_1d("multip", 2, ["vanleer"] * 4, 120), relax_pt=True),
+    "multip_2d_fo_relax_p_pt": dict(_shock_cylinders_2d(n=40, steps=50), model="multip", relax_p=True, relax_pt=True,
+                                    order=1, limiters=["none"] * 4),
+    "kapila_2d_o2_relax_pt": dict(_shock_cylinders_2d(n=40, steps=40), relax_pt=True,
+                                  bc=["abs", "outflow", "wall", "wall", "abs", "abs"], bc_p0=[0.0, 8e4] + [0.0] * 4),
 }
 
 

@@ -23,6 +23,7 @@ def test_deck_reads_back_the_reference_inputs(name, tmp_path):
     assert got.get("steps") == case.get("steps") if case["time_mode"] == "iterations" else got["t_final"] == case["t_final"]
     assert got["alpha_null"] == bool(case.get("alpha_null", False))
     assert got["relax_p"] == bool(case.get("relax_p", False) and case["model"] == "multip")
+    assert got.get("relax_pt", False) == bool(case.get("relax_pt", False))
     dim = sum(1 for c in case["cells"] if c > 1)
     assert got["bc"][:2 * dim] == case.get("bc", ["abs"] * 6)[:2 * dim]
     for i, kind in enumerate(got["bc"][:2 * dim]):
