@@ -1,7 +1,14 @@
 """Shared pytest configuration: the ``gpu`` marker and golden-case helpers."""
 import json
+import os
 import sys
 from pathlib import Path
+
+# The oracle is built with OpenMP.  Under pytest-xdist every worker would
+# start one thread per core and the spinning teams oversubscribe the host
+# (a 0.1 s oracle run took minutes): one thread per worker there.
+if os.environ.get("PYTEST_XDIST_WORKER"):
+    os.environ["OMP_NUM_THREADS"] = "1"
 
 import numpy as np
 import pytest
