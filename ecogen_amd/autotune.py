@@ -37,10 +37,15 @@ CSRC = PKG / "csrc"
 # than its tile so that the face pass ends in fewer iterations
 SPACE = {
     1: {"shapes": [((504,), 512), ((248,), 256), ((1016,), 1024), ((256,), 256)], "waves": [1, 2]},
-    2: {"shapes": [((32, 8), 256), ((64, 4), 256), ((16, 8), 128), ((32, 4), 128), ((16, 15), 256)],
-        "waves": [2, 3]},
     3: {"shapes": [((16, 4, 3), 256), ((16, 4, 4), 256), ((16, 4, 4), 320), ((8, 8, 4), 320)], "waves": [2]},
 }
+# 2-D: the staged MUSCL pass runs one face per thread and direction, so a
+# tile's faces must fit its workgroup; (tile, threads, waves per SIMD of the
+# staged kernels).  The shapes keep the faces of each direction, the cells'
+# limited slopes and the cells themselves close to whole waves.
+# (tile, threads, waves per SIMD of the FP64 and of the FP32-storage staged kernels)
+SPACE2 = [((16, 12), 256, 4, 4), ((16, 12), 256, 4, 5), ((16, 14), 256, 3, 4), ((16, 14), 256, 3, 5),
+          ((16, 15), 256, 3, 4)]
 AXES = "XYZ"
 # "dimension" 0: the persistent pool kernels of the FP32-storage form, timed
 # on the batched Riemann workload (minimum waves per SIMD of the 1-D and
@@ -53,6 +58,12 @@ def variants(dim):
         return [(f"pool_b1{w1}_b2{w2}", {"ECO_WAVESB32_1": w1, "ECO_WAVESB32_2": w2})
                 for w1, w2 in itertools.product(POOL_SPACE["b1"], POOL_SPACE["b2"])]
     out = []
+    if dim == 2:
+        for t, nt, w, w32 in SPACE2:
+            name = f"d2_t{t[0]}x{t[1]}_n{nt}_ws{w}_{w32}"
+            out.append((name, {"ECO_TILE2_X": t[0], "ECO_TILE2_Y": t[1], "ECO_THREADS2": nt, "ECO_WAVESS2": w,
+                               "ECO_WAVESS32_2": w32, "ECO_LIMSPEC2": 1}))
+        return out
     sp = SPACE[dim]
     for (t, nt), w in itertools.product(sp["shapes"], sp["waves"]):
         name = f"d{dim}_t{'x'.join(map(str, t))}_n{nt}_w{w}"
@@ -64,7 +75,8 @@ def variants(dim):
 
 # the translation unit whose kernels a dimension's workload runs; the other
 # objects of a variant are those of the main build (python -m ecogen_amd.build)
-TU_FOR_DIM = {0: "eco_k_kapila32.hip", 1: "eco_k_kapila32.hip", 2: "eco_k_kapila.hip", 3: "eco_k_multip.hip"}
+TU_FOR_DIM = {0: ("eco_k_kapila32.hip",), 1: ("eco_k_kapila32.hip",), 2: ("eco_k_kapila.hip", "eco_k_kapila32.hip"),
+              3: ("eco_k_multip.hip",)}
 
 
 def _compile_variant(item):
@@ -75,7 +87,7 @@ def _compile_variant(item):
     dflags = [f"-D{k}={v}" for k, v in defs.items()]
     objs, fresh = [], []
     for src in SOURCES:
-        if src not in (TU_FOR_DIM[dim], "eco_dispatch.cpp"):
+        if src not in (*TU_FOR_DIM[dim], "eco_dispatch.cpp"):
             objs.append(str(OBJ / (Path(src).stem + ".o")))
             continue
         obj = vdir / (Path(src).stem + ".o")
@@ -169,16 +181,20 @@ def cmd_run(args):
             _run_pool(args, results)
             continue
         case, st = _workload(dim, {1: args.n1, 2: args.n2, 3: args.n3}[dim])
-        base_ms, base = _time_lib(PKG / "libecogen_amd.so", case, st, args.steps)
-        print(f"dim {dim} default build: {base_ms:.3f} ms/step", flush=True)
-        for name, defs in variants(dim):
-            lib = TUNE_DIR / name / "libecogen_amd.so"
-            if not lib.exists():
-                continue
-            ms, out = _time_lib(lib, case, st, args.steps)
-            dev = float(np.abs(out - base).max())
-            results.append({"dim": dim, "variant": name, "defs": defs, "ms_per_step": ms, "max_abs_diff": dev})
-            print(f"  {name}: {ms:.3f} ms/step  (max |diff| vs default {dev:.2e})", flush=True)
+        # 2-D: the headline workload in FP64 and in FP32 storage (one tile shape for both)
+        forms = [(case, "")] + ([(dict(case, precision="fp32s"), " fp32s")] if dim == 2 else [])
+        for cs, tag in forms:
+            base_ms, base = _time_lib(PKG / "libecogen_amd.so", cs, st, args.steps)
+            print(f"dim {dim}{tag} default build: {base_ms:.3f} ms/step", flush=True)
+            for name, defs in variants(dim):
+                lib = TUNE_DIR / name / "libecogen_amd.so"
+                if not lib.exists():
+                    continue
+                ms, out = _time_lib(lib, cs, st, args.steps)
+                dev = float(np.abs(out - base).max())
+                results.append({"dim": dim, "form": tag.strip() or "fp64", "variant": name, "defs": defs,
+                                "ms_per_step": ms, "max_abs_diff": dev})
+                print(f"  {name}{tag}: {ms:.3f} ms/step  (max |diff| vs default {dev:.2e})", flush=True)
     Path(args.out).parent.mkdir(parents=True, exist_ok=True)
     Path(args.out).write_text(json.dumps(results, indent=1))
 

@@ -850,9 +850,15 @@ struct StagedLds {
   static constexpr int NG = NGX > NGY ? NGX : NGY;
   static constexpr int NF0 = (TX + 1) * TY, NF1 = TX * (TY + 1);    // faces of one direction
   static constexpr int NFM = NF0 > NF1 ? NF0 : NF1;
+  static_assert(NFM <= NT, "staged faces: at most one face per thread and direction");
   double S[Ly::NVAR][NS];
-  double G[Ly::NVAR][NG];
-  double F[Ly::NFV][NFM];
+  // the limited slopes of one direction, then (after every thread has read
+  // them) the fluxes of that direction's faces: the two share one block, so
+  // that four workgroups fit a CU's LDS
+  union {
+    double G[Ly::NVAR][NG];
+    double F[Ly::NFV][NFM];
+  };
   double sRed[NT / 64];
 };
 
@@ -949,13 +955,16 @@ __device__ __forceinline__ void stage_slopes_dispatch(StagedLds<MODEL, NPH, DIM>
 
 // The interior faces of direction dir from the staged states and slopes:
 // extrapolation (CellInterfaceO2::solveRiemann, CellInterfaceO2.cpp:197-295),
-// verify/correct, face frame, fulfillState, HLLC; flux to LDS slot f.
+// verify/correct, face frame, fulfillState, HLLC.  Thread f computes face f;
+// after a barrier (every thread has read the slopes, whose LDS block the
+// fluxes take over) it writes the flux to slot f.  Every thread of the
+// workgroup calls this (it holds a barrier).
 template <int MODEL, int NPH, int DIM, int dir>
 __device__ __forceinline__ double stage_faces(StagedLds<MODEL, NPH, DIM> &L, const SchemeParams &S, const Geometry &G,
                                               const int *o, int tid) {
   using LD = StagedLds<MODEL, NPH, DIM>;
   using Ly = Layout<MODEL, NPH, DIM>;
-  constexpr int TX = LD::TX, SX = LD::SX, NT = LD::NT;
+  constexpr int TX = LD::TX, SX = LD::SX;
   constexpr int NFD = (dir == 0) ? LD::NF0 : LD::NF1;
   constexpr int RW = (dir == 0) ? TX + 1 : TX;  // face rows
   constexpr int GW = (dir == 0) ? TX + 2 : TX;  // slope-region row width
@@ -964,12 +973,14 @@ __device__ __forceinline__ double stage_faces(StagedLds<MODEL, NPH, DIM> &L, con
   ECO_GLOBAL const double *Dl = gmem(G.dfl[dir]);
   ECO_GLOBAL const double *Dr = gmem(G.dfr[dir]);
   double smx = 0.;
-  for (int f = tid; f < NFD; f += NT) {
-    const int fx = f % RW, fy = f / RW;
-    const int g0 = o[0] + fx, g1 = o[1] + fy;  // cell on the face's high side
-    if (g0 >= nx + (dir == 0) || g1 >= ny + (dir == 1)) continue;
-    const int gi = (dir == 0) ? g0 : g1;
-    if (gi < 1 || gi > ndir - 1) continue;  // boundary faces are handled per cell
+  FluxVec<NPH> F;
+  bool have = false;
+  const int f = tid;
+  const int fx = f % RW, fy = f / RW;
+  const int g0 = o[0] + fx, g1 = o[1] + fy;  // cell on the face's high side
+  const int gi = (dir == 0) ? g0 : g1;
+  // faces of the tile inside the domain; boundary faces are handled per cell
+  if (f < NFD && g0 < nx + (dir == 0) && g1 < ny + (dir == 1) && gi >= 1 && gi <= ndir - 1) {
     const int lx = fx - (dir == 0), ly = fy - (dir == 1);  // tile coordinates of the low-side cell
     const int sl = (lx + 2) + (ly + 2) * SX, sr = sl + SD;
     const int gl = (dir == 0) ? (lx + 1) + ly * GW : lx + (ly + 1) * GW, gr = gl + GD;
@@ -997,11 +1008,12 @@ __device__ __forceinline__ double stage_faces(StagedLds<MODEL, NPH, DIM> &L, con
     to_face_frame(dir, 0, Rc.u);
     fulfill<MODEL, NPH>(S, Lc);
     fulfill<MODEL, NPH>(S, Rc);
-    FluxVec<NPH> F;
     smx = smax(smx, hllc<MODEL, NPH>(S, Lc, Rc, F));
     from_face_frame(dir, 0, F.q);
-    put_flux_s<MODEL, NPH, DIM>(&L.F[0][0], LD::NFM, f, F);
+    have = true;
   }
+  __syncthreads();  // the slopes are read: their block takes the fluxes
+  if (have) put_flux_s<MODEL, NPH, DIM>(&L.F[0][0], LD::NFM, f, F);
   return smx;
 }
 
@@ -1090,6 +1102,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int slot, c
       load_nc_coeffs<MODEL, NPH, DIM>(Wn, nc, cidx, ownc);
       accumulate_dir<MODEL, NPH, DIM, 0>(acc, &lds.F[0][0], SL::NFM, 0, lx, ly, lz, ci, G, ownc, A.smf[0]);
     }
+    __syncthreads();  // the x fluxes are read: the block takes the y slopes
     stage_slopes_dispatch<MODEL, NPH, DIM, 1>(lds, S, G, o, tid);
     __syncthreads();
     smx = smax(smx, stage_faces<MODEL, NPH, DIM, 1>(lds, S, G, o, tid));

@@ -344,9 +344,14 @@ struct StagedLds32 {
   static constexpr int NG = NGX > NGY ? NGX : NGY;
   static constexpr int NF0 = (TX + 1) * TY, NF1 = TX * (TY + 1);
   static constexpr int NFM = NF0 > NF1 ? NF0 : NF1;
+  static_assert(NFM <= NT, "staged faces: at most one face per thread and direction");
   float S[Ly::NVAR][NS];
-  float G[Ly::NVAR][NG];
-  double F[Ly::NFV][NFM];
+  // the limited slopes of one direction, then (after every thread has read
+  // them) the fluxes of its faces
+  union {
+    float G[Ly::NVAR][NG];
+    double F[Ly::NFV][NFM];
+  };
   double sRed[NT / 64];
 };
 
@@ -445,12 +450,15 @@ __device__ __forceinline__ void stage32_slopes_dispatch(StagedLds32<MODEL, NPH, 
   stage32_slopes<MODEL, NPH, DIM, dir, -1, -1>(L, S, G, o, tid);
 }
 
+// Thread f computes face f of direction dir; after a barrier (every thread
+// has read the slopes, whose LDS block the fluxes take over) it writes the
+// flux to slot f.  Every thread of the workgroup calls this.
 template <int MODEL, int NPH, int DIM, int dir>
 __device__ __forceinline__ float stage32_faces(StagedLds32<MODEL, NPH, DIM> &L, const SchemeParams &S,
                                                const Geometry &G, const int *o, int tid) {
   using LD = StagedLds32<MODEL, NPH, DIM>;
   using Ly = Layout<MODEL, NPH, DIM>;
-  constexpr int TX = LD::TX, SX = LD::SX, NT = LD::NT;
+  constexpr int TX = LD::TX, SX = LD::SX;
   constexpr int NFD = (dir == 0) ? LD::NF0 : LD::NF1;
   constexpr int RW = (dir == 0) ? TX + 1 : TX;
   constexpr int GW = (dir == 0) ? TX + 2 : TX;
@@ -459,12 +467,13 @@ __device__ __forceinline__ float stage32_faces(StagedLds32<MODEL, NPH, DIM> &L, 
   ECO_GLOBAL const double *Dl = gmem(G.dfl[dir]);
   ECO_GLOBAL const double *Dr = gmem(G.dfr[dir]);
   float smx = 0.f;
-  for (int f = tid; f < NFD; f += NT) {
-    const int fx = f % RW, fy = f / RW;
-    const int g0 = o[0] + fx, g1 = o[1] + fy;
-    if (g0 >= nx + (dir == 0) || g1 >= ny + (dir == 1)) continue;
-    const int gi = (dir == 0) ? g0 : g1;
-    if (gi < 1 || gi > ndir - 1) continue;
+  FluxVec<NPH> F;
+  bool have = false;
+  const int f = tid;
+  const int fx = f % RW, fy = f / RW;
+  const int g0 = o[0] + fx, g1 = o[1] + fy;
+  const int gi = (dir == 0) ? g0 : g1;
+  if (f < NFD && g0 < nx + (dir == 0) && g1 < ny + (dir == 1) && gi >= 1 && gi <= ndir - 1) {
     const int lx = fx - (dir == 0), ly = fy - (dir == 1);
     const int sl = (lx + 2) + (ly + 2) * SX, sr = sl + SD;
     const int gl = (dir == 0) ? (lx + 1) + ly * GW : lx + (ly + 1) * GW, gr = gl + GD;
@@ -490,11 +499,12 @@ __device__ __forceinline__ float stage32_faces(StagedLds32<MODEL, NPH, DIM> &L, 
     smx = smaxf_(smx, speeds_f<NPH>(S, Lc, Rc, sL, sR));
     fulfill_energy<MODEL, NPH>(S, Lc);
     fulfill_energy<MODEL, NPH>(S, Rc);
-    FluxVec<NPH> F;
     hllc_mixed<MODEL, NPH>(S, Lc, Rc, sL, sR, F);
     from_face_frame(dir, 0, F.q);
-    put_flux_s<MODEL, NPH, DIM>(&L.F[0][0], LD::NFM, f, F);
+    have = true;
   }
+  __syncthreads();  // the slopes are read: their block takes the fluxes
+  if (have) put_flux_s<MODEL, NPH, DIM>(&L.F[0][0], LD::NFM, f, F);
   return smx;
 }
 
@@ -561,6 +571,7 @@ __device__ __forceinline__ void stage32_tile(const StageArgs &A, const int slot,
       load_nc_coeffs_f<NPH>(Wn, nc, cidx, ownc);
       accumulate_dir<MODEL, NPH, DIM, 0>(acc, &lds.F[0][0], SL::NFM, 0, lx, ly, lz, ci, G, ownc);
     }
+    __syncthreads();  // the x fluxes are read: the block takes the y slopes
     stage32_slopes_dispatch<MODEL, NPH, DIM, 1>(lds, S, G, o, tid);
     __syncthreads();
     smx = smaxf_(smx, stage32_faces<MODEL, NPH, DIM, 1>(lds, S, G, o, tid));

@@ -8,7 +8,7 @@
 #define ECO_TILE2_X 16
 #endif
 #ifndef ECO_TILE2_Y
-#define ECO_TILE2_Y 12
+#define ECO_TILE2_Y 14
 #endif
 #ifndef ECO_TILE3_X
 #define ECO_TILE3_X 16
@@ -33,7 +33,7 @@
 #define ECO_WAVESS2 3
 #endif
 #ifndef ECO_WAVESS32_2
-#define ECO_WAVESS32_2 3
+#define ECO_WAVESS32_2 4
 #endif
 #ifndef ECO_WAVES3
 #define ECO_WAVES3 2
