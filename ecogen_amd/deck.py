@@ -43,6 +43,8 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
+import re
 import xml.etree.ElementTree as ET
 from pathlib import Path
 
@@ -50,7 +52,8 @@ import numpy as np
 
 LIMITERS = {"MINMOD": "minmod", "VANLEER": "vanleer", "VANALBADA": "vanalbada", "SUPERBEE": "superbee",
             "MC": "mc", "THINC": "thinc"}
-MODELS = {"EULER": "euler", "KAPILA": "kapila", "MULTIP": "multip", "THERMALEQ": "thermaleq"}
+MODELS = {"EULER": "euler", "KAPILA": "kapila", "MULTIP": "multip", "THERMALEQ": "thermaleq",
+          "EULERHOMOGENEOUS": "eulerhomogeneous"}
 AXES = {"X": 0, "Y": 1, "Z": 2}
 
 
@@ -71,11 +74,23 @@ def _child(node, tag, where):
     return c
 
 
+_NUM = re.compile(r"\s*[+-]?(\d+\.?\d*|\.\d+)([eE][+-]?\d+)?")
+
+
+def _num(v, where=""):
+    """A number as tinyxml2's QueryDoubleAttribute reads it (sscanf %lf): the
+    longest numeric prefix, so that libEOS's energyRef="-1.15e6." is -1.15e6."""
+    m = _NUM.match(v)
+    if not m:
+        raise DeckError(f"{where}: {v!r} is not a number")
+    return float(m.group(0))
+
+
 def _f(node, attr, where):
     v = node.get(attr)
     if v is None:
         raise DeckError(f"{where}: <{node.tag}> lacks {attr}")
-    return float(v)
+    return _num(v, where)
 
 
 def _true(v):
@@ -87,12 +102,16 @@ def _eos(path: Path):
     root = _xml(path)
     kind = _child(root, "EOS", path.name).get("type", "").upper()
     par = _child(root, "parameters", path.name)
+    # energyRef, entropyRef (the saturation temperature's reference entropies,
+    # Mixture::computeTsat) default to 0
+    eref = _num(par.get("energyRef", "0"), path.name)
+    sref = _num(par.get("entropyRef", "0"), path.name)
     if kind == "IG":
         return {"type": "IG", "gamma": _f(par, "gamma", path.name), "pinf": 0.0, "cv": _f(par, "cv", path.name),
-                "eref": float(par.get("energyRef", 0.0))}
+                "eref": eref, "sref": sref}
     if kind == "SG":
         return {"type": "SG", "gamma": _f(par, "gamma", path.name), "pinf": _f(par, "pInf", path.name),
-                "cv": _f(par, "cv", path.name), "eref": float(par.get("energyRef", 0.0))}
+                "cv": _f(par, "cv", path.name), "eref": eref, "sref": sref}
     raise DeckError(f"{path.name}: EOS type {kind!r} is not IG or SG")
 
 
@@ -190,7 +209,28 @@ def _named_values(node, names, where, need_all):
     return vals
 
 
-def _state_values(st, model, nph, where, eos=None):
+def _tsat(eos, liq, vap, p):
+    """Mixture::computeTsat (Mixture.cpp:63-104): the saturation temperature
+    at pressure p, Newton on the equality of the phases' Gibbs energies, with
+    the C library's log (math.log) in the reference's order of operations."""
+    qL, qV = eos[liq], eos[vap]
+    gL, pL, cvL, e0L, s0L = qL["gamma"], qL["pinf"], qL["cv"], qL["eref"], qL.get("sref", 0.0)
+    gV, pV, cvV, e0V, s0V = qV["gamma"], qV["pinf"], qV["cv"], qV["eref"], qV.get("sref", 0.0)
+    A = (gL * cvL - gV * cvV + s0V - s0L) / (gV * cvV - cvV)
+    B = (e0L - e0V) / (gV * cvV - cvV)
+    C = (gV * cvV - gL * cvL) / (gV * cvV - cvV)
+    D = (gL * cvL - cvL) / (gV * cvV - cvV)
+    T, f, df = 0.1 * B / C, 0.0, 1.0
+    for _ in range(50):
+        T -= f / df
+        f = A + B / T + C * math.log(T) - math.log(p + pV) + D * math.log(p + pL)
+        df = C / T - B / (T * T)
+        if not abs(f) > 1e-10:
+            break
+    return T
+
+
+def _state_values(st, model, nph, where, eos=None, liqvap=(0, 1)):
     """(alpha_k, rho_k, p_k per phase, u, v, w) of one <state>; for the
     thermal-equilibrium model also E (0, not read) and the temperature."""
     mats = st.findall("material")
@@ -211,6 +251,22 @@ def _state_values(st, model, nph, where, eos=None):
             den = (e["gamma"] - 1.0) * e["cv"] * T
             out += [a, num / (0.0 if den < 0.0 else den), p]
         return out + [float(vel.get(a, 0.0)) for a in "xyz"] + [0.0, T]
+    if model == "eulerhomogeneous":
+        # PhaseEulerHomogeneous reads alpha (PhaseEulerHomogeneous.cpp:14-24),
+        # MixEulerHomogeneous the mixture pressure and velocity
+        # (MixEulerHomogeneous.cpp:25-43); the phases take the saturated
+        # densities of p at the saturation temperature (ModEulerHomogeneous::
+        # fulfillState, EosSG/EosIG::computeDensitySaturation)
+        mix = _child(st, "mixture", where)
+        p = _f(_child(mix, "dataMix", where), "pressure", where)
+        vel = _child(mix, "velocity", where)
+        T = _tsat(eos, liqvap[0], liqvap[1], p)
+        out = []
+        for m, e in zip(mats, eos):
+            num = p + e["pinf"] if e["type"] == "SG" else p
+            den = (e["gamma"] - 1.0) * e["cv"] * T
+            out += [_f(_child(m, "dataFluid", where), "alpha", where), num / (0.0 if den < 0.0 else den), p]
+        return out + [_num(vel.get(a, "0"), where) for a in "xyz"] + [0.0, T]
     if model == "euler":
         fl = _child(mats[0], "dataFluid", where)
         v = _child(fl, "velocity", where)
@@ -313,7 +369,15 @@ def load(rundir):
     case["model"] = model
     case["alpha_null"] = _true(fm.get("alphaNull")) if model == "kapila" else False
     eos_files = [e.get("name", "") for e in model_x.findall("EOS")]
-    nph = 1 if model == "euler" else int(_f(fm, "numberPhases", "modelV4.xml"))
+    if model == "eulerhomogeneous":
+        # two phases, the liquid and vapour indices (Input.cpp:479-483)
+        nph = 2
+        case["liq"] = int(_f(fm, "liquid", "modelV4.xml"))
+        case["vap"] = int(_f(fm, "vapor", "modelV4.xml"))
+        if sorted((case["liq"], case["vap"])) != [0, 1]:
+            raise DeckError("modelV4.xml: EulerHomogeneous takes liquid and vapor 0 and 1")
+    else:
+        nph = 1 if model == "euler" else int(_f(fm, "numberPhases", "modelV4.xml"))
     if len(eos_files) != nph:
         raise DeckError(f"modelV4.xml: {len(eos_files)} EOS for {nph} phases")
     case["eos"] = [dict(_eos(rundir / "libEOS" / f), name=Path(f).stem, file=f) for f in eos_files]
@@ -321,14 +385,20 @@ def load(rundir):
     # relaxation (ModKapila.cpp:47-53): MultiP takes P, PT or P then PT,
     # Kapila PT; RelaxationPT needs two phases (RelaxationPT.cpp:60)
     relax = [r.get("type", "").upper() for r in model_x.findall("relaxation")]
-    allowed = {"multip": (["P"], ["PT"], ["P", "PT"]), "kapila": (["PT"],)}.get(model, ())
+    allowed = {"multip": (["P"], ["PT"], ["P", "PT"]), "kapila": (["PT"], ["PTMU"])}.get(model, ())
     if relax and relax not in [list(a) for a in allowed]:
-        raise DeckError(f"modelV4.xml: relaxations {relax} for {model} (MultiP: P, PT or P then PT; Kapila: PT)")
-    if "PT" in relax and nph != 2:
-        raise DeckError("modelV4.xml: the PT relaxation takes two phases")
+        raise DeckError(f"modelV4.xml: relaxations {relax} for {model} (MultiP: P, PT or P then PT; Kapila: PT or PTMu)")
+    if ("PT" in relax or "PTMU" in relax) and nph != 2:
+        raise DeckError("modelV4.xml: the PT and PTMu relaxations take two phases")
     case["relax_p"] = model == "multip" and "P" in relax
     if "PT" in relax:
         case["relax_pt"] = True
+    if "PTMU" in relax:
+        # RelaxationPTMu.cpp:37-49: <dataPTMu> required; phase 0 is the liquid,
+        # phase 1 the vapour whatever its attributes name
+        if model_x.find("relaxation/dataPTMu") is None:
+            raise DeckError("modelV4.xml: <relaxation type=\"PTMu\"> lacks <dataPTMu>")
+        case["relax_ptmu"] = True
     # transport equations (Input.cpp:472-473, 528-540): numberTransports of
     # the first <transport name> elements
     ntr = int(float(fm.get("numberTransports", "0")))
@@ -377,6 +447,15 @@ def load(rundir):
                 bc_tr[num - 1] = _named_values(b.find("dataTank"), trs, "boundCond tank", True)
         else:
             raise DeckError(f"boundCond type {kind!r} is not read")
+    if model == "eulerhomogeneous":
+        # ModEulerHomogeneous has only the interior Riemann solver (Model.h:
+        # 93-119 end the run for walls, outflows, injections and tanks), and its
+        # MUSCL form starts the first predictor's Newton solve from a pressure
+        # PhaseEulerHomogeneous's reader leaves unset (its results are NaN)
+        if any(b != "abs" for b in bcs):
+            raise DeckError("EulerHomogeneous takes transmissive boundaries only")
+        if case["order"] != 1:
+            raise DeckError("EulerHomogeneous runs at first order (the reference's MUSCL form reads an unset pressure)")
     case["bc"] = bcs
     case["bc_p0"] = p0s
     if any(injs):
@@ -391,7 +470,7 @@ def load(rundir):
     nf = 3 * nph + 3 if model != "euler" else 6
     if trs:
         nf += 1 + len(trs)  # E (not read), then the scalars
-    if model == "thermaleq":
+    if model in ("thermaleq", "eulerhomogeneous"):
         nf += 2  # E (not read), then the temperature
     grid = np.zeros((nf, case["cells"][2], case["cells"][1], case["cells"][0]))
     filled = np.zeros(grid.shape[1:], dtype=bool)
@@ -401,7 +480,7 @@ def load(rundir):
         sname = dom.get("state")
         if sname not in states:
             raise DeckError(f"{where}: unknown state {sname!r}")
-        vals = _state_values(states[sname], model, nph, where, case["eos"])
+        vals = _state_values(states[sname], model, nph, where, case["eos"], (case.get("liq", 0), case.get("vap", 1)))
         if trs:
             vals = vals + [0.0] + _named_values(states[sname], trs, where, False)
         m = _domain_mask(dom, Xg, Yg, Zg, where)

@@ -44,8 +44,8 @@ def _state_xml(case, st):
     else:
         for k, e in enumerate(eos):
             out.append(f'    <material type="fluide" EOS="{_eos_file(e)}">')
-            if model == "thermaleq":
-                # PhaseThermalEq.cpp:45-57 reads alpha only
+            if model in ("thermaleq", "eulerhomogeneous"):
+                # PhaseThermalEq.cpp:45-57 and PhaseEulerHomogeneous.cpp:14-24 read alpha only
                 out.append(f'      <dataFluid alpha="{st["alpha"][k]!r}"/>')
             elif model == "kapila":
                 out.append(f'      <dataFluid alpha="{st["alpha"][k]!r}" density="{st["rho"][k]!r}"/>')
@@ -54,7 +54,8 @@ def _state_xml(case, st):
             out.append('    </material>')
         u = list(st["u"]) + [0.0] * (3 - len(st["u"]))
         out.append('    <mixture>')
-        if model == "kapila":
+        if model in ("kapila", "eulerhomogeneous"):
+            # MixKapila, MixEulerHomogeneous.cpp:25-43: the mixture pressure
             out.append(f'      <dataMix pressure="{st["p"][0]!r}"/>')
         elif model == "thermaleq":
             # MixThermalEq.cpp:46-79: pressure and temperature of the mixture
@@ -138,7 +139,10 @@ def write_deck(case, root: Path):
     fm = {"euler": '<flowModel name="Euler"/>',
           "kapila": f'<flowModel name="Kapila" numberPhases="{len(case["eos"])}" alphaNull="{"true" if case.get("alpha_null") else "false"}"/>',
           "multip": f'<flowModel name="MultiP" numberPhases="{len(case["eos"])}"/>',
-          "thermaleq": f'<flowModel name="ThermalEq" numberPhases="{len(case["eos"])}"/>'}[model]
+          "thermaleq": f'<flowModel name="ThermalEq" numberPhases="{len(case["eos"])}"/>',
+          # Input.cpp:479-483: the liquid and vapour phase indices
+          "eulerhomogeneous": f'<flowModel name="EulerHomogeneous" liquid="{case.get("liq", 0)}" '
+                              f'vapor="{case.get("vap", 1)}"/>'}[model]
     trs = case.get("transports", [])
     if trs:
         # Input.cpp:472-473, 528-540: numberTransports and one <transport name> each
@@ -149,6 +153,12 @@ def write_deck(case, root: Path):
     if case.get("relax_pt"):
         # Input.cpp:607-621: relaxations in deck order, after Kapila's own P
         relax += '  <relaxation type="PT"/>\n'
+    if case.get("relax_ptmu"):
+        # RelaxationPTMu.cpp:37-49 requires <dataPTMu> and takes phase 0 as the
+        # liquid, phase 1 as the vapour whatever its attributes say
+        e = case["eos"]
+        relax += (f'  <relaxation type="PTMu">\n    <dataPTMu liquid="{_eos_file(e[0])}" vapor="{_eos_file(e[1])}"/>\n'
+                  '  </relaxation>\n')
     (cdir / "modelV4.xml").write_text(
         '<?xml version="1.0" encoding="UTF-8" standalone="yes"?>\n<model>\n'
         f'  {fm}\n{eosl}\n{relax}</model>\n')

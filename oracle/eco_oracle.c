@@ -194,8 +194,9 @@ static void verify_correct(const Ctx *c, int k, Phase *ph) {
     if (ph->alpha < 1e-10) ph->alpha = 1e-9;
     if (ph->alpha > 1. - 1e-10) ph->alpha = 1. - 1e-9;
     if (ph->rho < 1.e-10) ph->rho = 1.e-9;
-  } else if (cfg->model == ORC_THERMALEQ) {
-    /* PhaseThermalEq::verifyAndCorrectPhase (PhaseThermalEq.cpp:240-246) */
+  } else if (cfg->model == ORC_THERMALEQ || cfg->model == ORC_EULERHOM) {
+    /* PhaseThermalEq::verifyAndCorrectPhase (PhaseThermalEq.cpp:240-246),
+       PhaseEulerHomogeneous::verifyAndCorrectPhase (PhaseEulerHomogeneous.cpp:189-195) */
     if (ph->alpha < 1e-10) ph->alpha = 1e-10;
     if (ph->alpha > 1. - 1e-10) ph->alpha = 1. - 1e-10;
     if (ph->rho < 1.e-10) ph->rho = 1.e-10;
@@ -283,9 +284,95 @@ static void te_fulfill(const Ctx *c, State *s) {
   te_mixture_vars(c, s);
 }
 
+/* Mixture::computeTsat (Mixture.cpp:63-104): the saturation temperature of
+   the liquid and vapour phases at pressure p, Newton on the equality of their
+   Gibbs energies, f = A + B/T + C log T - log(p + pinfV) + D log(p + pinfL);
+   dTsat (may be NULL) receives dTsat/dp. */
+static long g_sat_fail; /* computeTsat or computePressure reaching 50 iterations */
+static double sat_tsat(const Ctx *c, double pressure, double *dTsat) {
+  const orc_cfg *cfg = c->cfg;
+  const orc_eos *qL = &cfg->eos[cfg->liq], *qV = &cfg->eos[cfg->vap];
+  double gammaL = qL->gamma, pInfL = te_pinf(qL), cvL = qL->cv, e0L = qL->eref, s0L = cfg->sref[cfg->liq];
+  double gammaV = qV->gamma, pInfV = te_pinf(qV), cvV = qV->cv, e0V = qV->eref, s0V = cfg->sref[cfg->vap];
+  double A = (gammaL * cvL - gammaV * cvV + s0V - s0L) / (gammaV * cvV - cvV);
+  double B = (e0L - e0V) / (gammaV * cvV - cvV);
+  double C = (gammaV * cvV - gammaL * cvL) / (gammaV * cvV - cvV);
+  double D = (gammaL * cvL - cvL) / (gammaV * cvV - cvV);
+  int iteration = 0;
+  double Tsat = 0.1 * B / C;
+  double f = 0., df = 1.;
+  do {
+    Tsat -= f / df; iteration++;
+    if (iteration > 50) {
+#pragma omp atomic
+      g_sat_fail++;
+      break;
+    }
+    f = A + B / Tsat + C * log(Tsat) - log(pressure + pInfV) + D * log(pressure + pInfL);
+    df = C / Tsat - B / (Tsat * Tsat);
+  } while (fabs(f) > 1e-10);
+  double dfdp = -1. / (pressure + pInfV) + D / (pressure + pInfL);
+  if (dTsat) *dTsat = -dfdp / df;
+  return Tsat;
+}
+
+/* EosSG/EosIG::computeDensitySaturation (EosSG.cpp:191-200, EosIG.cpp:188-197) */
+static double eos_rho_sat(const orc_eos *q, double pressure, double Tsat, double dTsatdP, double *drhodp,
+                          double eps) {
+  double g = q->gamma, cv = q->cv;
+  if (q->type == ORC_EOS_SG) {
+    if (drhodp) {
+      *drhodp = (g - 1.) * cv * Tsat - (pressure + q->pinf) * (g - 1.) * cv * dTsatdP;
+      *drhodp /= smax((((g - 1.) * cv * Tsat) * ((g - 1.) * cv * Tsat)), eps);
+    }
+    return (pressure + q->pinf) / smax(((g - 1.) * cv * Tsat), eps);
+  }
+  if (drhodp) {
+    *drhodp = (g - 1.) * cv * Tsat - pressure * (g - 1.) * cv * dTsatdP;
+    *drhodp /= smax((((g - 1.) * cv * Tsat) * ((g - 1.) * cv * Tsat)), eps);
+  }
+  return pressure / smax(((g - 1.) * cv * Tsat), eps);
+}
+
+/* EosSG/EosIG::computeDensityEnergySaturation (EosSG.cpp:204-210, EosIG.cpp:201-207) */
+static double eos_rhoe_sat(const orc_eos *q, double pressure, double rho, double drhodp, double *drhoedp) {
+  double g = q->gamma;
+  if (drhoedp) *drhoedp = 1. / (g - 1.) + drhodp * q->eref;
+  if (q->type == ORC_EOS_SG) return (pressure + g * q->pinf) / (g - 1.) + rho * q->eref;
+  return pressure / (g - 1.) + rho * q->eref;
+}
+
+/* PhaseEulerHomogeneous::extendedCalculusPhase (PhaseEulerHomogeneous.cpp:52-57)
+   and MixEulerHomogeneous::computeMixtureVariables (MixEulerHomogeneous.cpp:
+   130-149): the formulas of ThermalEq's */
+static void eh_phases_mixture(const Ctx *c, State *s) {
+  for (int k = 0; k < c->cfg->nph; k++) {
+    Phase *ph = &s->ph[k];
+    const orc_eos *q = &c->cfg->eos[k];
+    ph->e = eos_e(q, ph->rho, ph->p, c->epsilon);
+    ph->c = eos_c(q, ph->rho, ph->p, c->epsilon);
+    ph->Et = ph->e + 0.5 * sqnorm(s->mx.u);
+  }
+  te_mixture_vars(c, s);
+}
+
+/* ModEulerHomogeneous::fulfillState (ModEulerHomogeneous.cpp:42-56): the
+   saturation temperature of the mixture pressure, each phase at that pressure
+   with its saturated density, then the mixture variables */
+static void eh_fulfill(const Ctx *c, State *s) {
+  double Tsat = sat_tsat(c, s->mx.p, NULL);
+  for (int k = 0; k < c->cfg->nph; k++) {
+    s->ph[k].p = s->mx.p;
+    s->ph[k].rho = eos_rho_sat(&c->cfg->eos[k], s->mx.p, Tsat, Tsat, NULL, c->epsilon);
+  }
+  s->mx.T = Tsat;
+  eh_phases_mixture(c, s);
+}
+
 /* Model::fulfillState: ModKapila.cpp:89-101, ModMultiP.cpp:83-91, ModEuler.cpp (phase only) */
 static void fulfill(const Ctx *c, State *s) {
   if (c->cfg->model == ORC_THERMALEQ) { te_fulfill(c, s); return; }
+  if (c->cfg->model == ORC_EULERHOM) { eh_fulfill(c, s); return; }
   for (int k = 0; k < c->cfg->nph; k++) extended_phase(c, k, &s->ph[k], s->mx.u);
   mixture_vars(c, s);
 }
@@ -302,6 +389,22 @@ static void build_cons(const Ctx *c, const State *s, Flux *f) {
     f->masse[0] = p->rho;
     for (int d = 0; d < 3; d++) f->qdm[d] = p->u[d] * f->masse[0];
     f->energMix = f->masse[0] * p->Et;
+    return;
+  }
+  if (cfg->model == ORC_EULERHOM) {
+    /* FluxEulerHomogeneous::buildCons (FluxEulerHomogeneous.cpp:70-89): the
+       mixture mass, momentum and total energy, the internal energy summed
+       from the phases' energies at their own densities and pressures */
+    double ein = 0.;
+    f->masse[0] = 0.;
+    for (int k = 0; k < cfg->nph; k++) {
+      double ak = s->ph[k].alpha, rk = s->ph[k].rho;
+      f->masse[0] += ak * rk;
+      double ek = eos_e(&cfg->eos[k], rk, s->ph[k].p, c->epsilon);
+      ein += ak * rk * ek;
+    }
+    for (int d = 0; d < 3; d++) f->qdm[d] = f->masse[0] * s->mx.u[d];
+    f->energMix = ein + 0.5 * f->masse[0] * sqnorm(s->mx.u);
     return;
   }
   if (cfg->model == ORC_THERMALEQ) {
@@ -382,6 +485,59 @@ static void te_build_prim(const Ctx *c, Flux *f, State *s) {
   s->mx.E = f->energMix / rhoMel;
 }
 
+/* MixEulerHomogeneous::computePressure (MixEulerHomogeneous.cpp:93-125): the
+   pressure at which the saturated mixture of density masse holds the
+   internal energy e, Newton from the destination state's liquid pressure */
+static double eh_pressure(const Ctx *c, double masse, double internalEnergy, double p_guess) {
+  const orc_cfg *cfg = c->cfg;
+  const orc_eos *qL = &cfg->eos[cfg->liq], *qV = &cfg->eos[cfg->vap];
+  double eps = c->epsilon;
+  int iteration = 0;
+  double pressure = p_guess;
+  double f = 0., df = 1.;
+  double alphaVap, dalphaVap, rhoLiq, drhoLiq, rhoVap, drhoVap, rhoeLiq, drhoeLiq, rhoeVap, drhoeVap, Tsat, dTsat;
+  do {
+    pressure -= f / df; iteration++;
+    if (iteration > 50) {
+#pragma omp atomic
+      g_sat_fail++;
+      break;
+    }
+    Tsat = sat_tsat(c, pressure, &dTsat);
+    rhoVap = eos_rho_sat(qV, pressure, Tsat, dTsat, &drhoVap, eps);
+    rhoLiq = eos_rho_sat(qL, pressure, Tsat, dTsat, &drhoLiq, eps);
+    alphaVap = (masse - rhoLiq) / (rhoVap - rhoLiq);
+    dalphaVap = (-drhoLiq * (rhoVap - rhoLiq) - (masse - rhoLiq) * (drhoVap - drhoLiq)) /
+                ((rhoVap - rhoLiq) * (rhoVap - rhoLiq));
+    rhoeVap = eos_rhoe_sat(qV, pressure, rhoVap, drhoVap, &drhoeVap);
+    rhoeLiq = eos_rhoe_sat(qL, pressure, rhoLiq, drhoLiq, &drhoeLiq);
+    f = masse * internalEnergy - alphaVap * (rhoeVap - rhoeLiq) - rhoeLiq;
+    df = -dalphaVap * (rhoeVap - rhoeLiq) - alphaVap * (drhoeVap - drhoeLiq) - drhoeLiq;
+  } while (fabs(f / (masse * internalEnergy)) > 1e-10);
+  return pressure;
+}
+
+/* FluxEulerHomogeneous::buildPrim (FluxEulerHomogeneous.cpp:93-128) */
+static void eh_build_prim(const Ctx *c, Flux *f, State *s) {
+  const orc_cfg *cfg = c->cfg;
+  int liq = cfg->liq, vap = cfg->vap;
+  double masse = f->masse[0];
+  for (int d = 0; d < 3; d++) s->mx.u[d] = f->qdm[d] / masse;
+  zero_small_u(s->mx.u);
+  double internalEnergy = f->energMix / masse - 0.5 * sqnorm(s->mx.u);
+  double pressure = eh_pressure(c, masse, internalEnergy, s->ph[liq].p);
+  double Tsat = sat_tsat(c, pressure, NULL);
+  s->ph[vap].rho = eos_rho_sat(&cfg->eos[vap], pressure, Tsat, Tsat, NULL, c->epsilon);
+  s->ph[liq].rho = eos_rho_sat(&cfg->eos[liq], pressure, Tsat, Tsat, NULL, c->epsilon);
+  s->ph[vap].alpha = (masse - s->ph[liq].rho) / (s->ph[vap].rho - s->ph[liq].rho);
+  s->ph[liq].alpha = 1. - s->ph[vap].alpha;
+  for (int k = 0; k < cfg->nph; k++) s->ph[k].p = pressure;
+  s->mx.p = pressure;
+  s->mx.T = Tsat;
+  eh_phases_mixture(c, s);
+  s->mx.E = f->energMix / masse;
+}
+
 /* FluxKapila::buildPrim (FluxKapila.cpp:139-185), FluxMultiP::buildPrim (FluxMultiP.cpp:144-183),
    FluxEuler::buildPrim (FluxEuler.cpp:107-131) */
 static void build_prim(const Ctx *c, Flux *f, State *s) {
@@ -401,6 +557,7 @@ static void build_prim(const Ctx *c, Flux *f, State *s) {
     return;
   }
   if (cfg->model == ORC_THERMALEQ) { te_build_prim(c, f, s); return; }
+  if (cfg->model == ORC_EULERHOM) { eh_build_prim(c, f, s); return; }
   double un = 0.;
   if (cfg->model == ORC_KAPILA) {
     if (c->epsilon > 1.e-20) {
@@ -559,6 +716,69 @@ static void relax_pt(const Ctx *c, State *s) {
   }
   for (int k = 0; k < n; k++) { s->ph[k].alpha = akS[k]; s->ph[k].rho = rkS[k]; s->ph[k].p = pStar; }
   s->mx.p = pStar;
+}
+
+/* RelaxationPTMu::stiffRelaxation (RelaxationPTMu.cpp:51-157): the two phases
+   (phase 0 the liquid, phase 1 the vapour, RelaxationPTMu.cpp:45-46) taken to
+   saturation at the pressure where the saturated mixture of the cell's
+   density holds its internal energy (Newton from sum alpha_k p_k of the
+   phases of the relaxed state s, which are verified and corrected).  The
+   density and energy are those of the cell's own mixture, and the result is
+   written to the cell's own phases, then fulfilled (Cell::getMixture,
+   getPhase and fulfillState without a type, RelaxationPTMu.cpp:47-48,
+   138-149): in the predictor s is the predicted state and s_n the time-n
+   state, which the predictor thus relaxes. */
+static void relax_ptmu(const Ctx *c, State *s, State *s_n) {
+  const orc_cfg *cfg = c->cfg;
+  int n = cfg->nph, liq = 0, vap = 1;
+  const orc_eos *qL = &cfg->eos[liq], *qV = &cfg->eos[vap];
+  double eps = c->epsilon;
+  double pStar = 0.;
+  for (int k = 0; k < n; k++) {
+    pStar += s->ph[k].alpha * s->ph[k].p;
+    verify_correct(c, k, &s->ph[k]);
+  }
+  double rho = s_n->mx.rho;
+  double rhoe = rho * s_n->mx.e;
+  double dTsat = 0.;
+  double Tsat = sat_tsat(c, pStar, &dTsat);
+  double rhoLSat = 0., rhoVSat = 0., drhoLSat = 0., drhoVSat = 0.;
+  double aLSat = 0., aVSat = 0., daLSat = 0., daVSat = 0.;
+  double rhoeLSat, rhoeVSat, drhoeLSat, drhoeVSat;
+  int iteration = 0;
+  double f = 0., df = 1.;
+  do {
+    pStar -= f / df; iteration++;
+    if (iteration > 50) {
+#pragma omp atomic
+      g_sat_fail++;
+      break;
+    }
+    for (int k = 0; k < n; k++) eos_fix_p(&cfg->eos[k], &pStar);
+    Tsat = sat_tsat(c, pStar, &dTsat);
+    rhoLSat = eos_rho_sat(qL, pStar, Tsat, dTsat, &drhoLSat, eps);
+    rhoVSat = eos_rho_sat(qV, pStar, Tsat, dTsat, &drhoVSat, eps);
+    if (rhoLSat <= rho) { rhoLSat = rho + 1e-6; drhoLSat = 0.; }
+    if (rhoVSat >= rho) { rhoVSat = rho - 1e-6; drhoVSat = 0.; }
+    aLSat = (rho - rhoVSat) / (rhoLSat - rhoVSat);
+    daLSat = (-drhoVSat * (rhoLSat - rhoVSat) - (rho - rhoVSat) * (drhoLSat - drhoVSat)) /
+             ((rhoLSat - rhoVSat) * (rhoLSat - rhoVSat));
+    if (aLSat <= 0.) aLSat = 1e-8;
+    if (aLSat >= 1.) aLSat = 1. - 1e-8;
+    aVSat = 1. - aLSat;
+    daVSat = -daLSat;
+    f = rhoe; df = 0.;
+    rhoeLSat = eos_rhoe_sat(qL, pStar, rhoLSat, drhoLSat, &drhoeLSat);
+    rhoeVSat = eos_rhoe_sat(qV, pStar, rhoVSat, drhoVSat, &drhoeVSat);
+    f -= (aLSat * rhoeLSat + aVSat * rhoeVSat);
+    df -= (daLSat * rhoeLSat + aLSat * drhoeLSat + daVSat * rhoeVSat + aVSat * drhoeVSat);
+    f /= rhoe;
+    df /= rhoe;
+  } while (fabs(f) > 1e-10);
+  (void)Tsat;
+  s_n->ph[liq].alpha = aLSat; s_n->ph[liq].rho = rhoLSat; s_n->ph[liq].p = pStar;
+  s_n->ph[vap].alpha = aVSat; s_n->ph[vap].rho = rhoVSat; s_n->ph[vap].p = pStar;
+  fulfill(c, s_n);
 }
 
 /* FluxKapila::correctionEnergy (FluxKapila.cpp:239-268); the trailing
@@ -733,9 +953,54 @@ static void hllc_te(const Ctx *c, const State *L, const State *R, double dxL, do
   F->sM = sM;
 }
 
+/* ModEulerHomogeneous::solveRiemannIntern (ModEulerHomogeneous.cpp:60-136):
+   HLLC on the mixture (density, pressure, velocity, the mixture energy the
+   cell holds plus |u|^2/2, sound speed sqrt(sum Y_k c_k^2)); the supersonic
+   branches are strict, sL > 0 and sR < 0 */
+static void hllc_eh(const Ctx *c, const State *L, const State *R, double dxL, double dxR,
+                    double *dtMax, Flux *F) {
+  (void)c;
+  double uL = L->mx.u[0], vL = L->mx.u[1], wL = L->mx.u[2], cL = L->mx.cF, pL = L->mx.p, rhoL = L->mx.rho;
+  double uR = R->mx.u[0], vR = R->mx.u[1], wR = R->mx.u[2], cR = R->mx.cF, pR = R->mx.p, rhoR = R->mx.rho;
+  double EL = L->mx.e + 0.5 * sqnorm(L->mx.u), ER = R->mx.e + 0.5 * sqnorm(R->mx.u);
+  double sL = smin(uL - cL, uR - cR);
+  double sR = smax(uR + cR, uL + cL);
+  if (fabs(sL) > 1.e-3) *dtMax = smin(*dtMax, dxL / fabs(sL));
+  if (fabs(sR) > 1.e-3) *dtMax = smin(*dtMax, dxR / fabs(sR));
+  double mL = rhoL * (sL - uL), mR = rhoR * (sR - uR);
+  double sM = (pR - pL + mL * uL - mR * uR) / (mL - mR);
+  if (fabs(sM) < 1.e-8) sM = 0.;
+  memset(F->masse, 0, sizeof(F->masse));
+  if (sL > 0.) {
+    F->masse[0] = rhoL * uL;
+    F->qdm[0] = rhoL * uL * uL + pL; F->qdm[1] = rhoL * vL * uL; F->qdm[2] = rhoL * wL * uL;
+    F->energMix = (rhoL * EL + pL) * uL;
+  } else if (sR < 0.) {
+    F->masse[0] = rhoR * uR;
+    F->qdm[0] = rhoR * uR * uR + pR; F->qdm[1] = rhoR * vR * uR; F->qdm[2] = rhoR * wR * uR;
+    F->energMix = (rhoR * ER + pR) * uR;
+  } else if (sM >= 0.) {
+    double pStar = mL * (sM - uL) + pL;
+    double rhoStar = mL / (sL - sM);
+    double Estar = EL + (sM - uL) * (sM + pL / mL);
+    F->masse[0] = rhoStar * sM;
+    F->qdm[0] = rhoStar * sM * sM + pStar; F->qdm[1] = rhoStar * sM * vL; F->qdm[2] = rhoStar * sM * wL;
+    F->energMix = (rhoStar * Estar + pStar) * sM;
+  } else {
+    double pStar = mR * (sM - uR) + pR;
+    double rhoStar = mR / (sR - sM);
+    double Estar = ER + (sM - uR) * (sM + pR / mR);
+    F->masse[0] = rhoStar * sM;
+    F->qdm[0] = rhoStar * sM * sM + pStar; F->qdm[1] = rhoStar * sM * vR; F->qdm[2] = rhoStar * sM * wR;
+    F->energMix = (rhoStar * Estar + pStar) * sM;
+  }
+  F->sM = sM;
+}
+
 static void riemann(const Ctx *c, const State *L, const State *R, double dxL, double dxR,
                     double *dtMax, Flux *F) {
   if (c->cfg->model == ORC_EULER) hllc_euler(c, L, R, dxL, dxR, dtMax, F);
+  else if (c->cfg->model == ORC_EULERHOM) hllc_eh(c, L, R, dxL, dxR, dtMax, F);
   else if (c->cfg->model == ORC_THERMALEQ) hllc_te(c, L, R, dxL, dxR, dtMax, F);
   else hllc_multi(c, L, R, dxL, dxR, dtMax, F);
 }
@@ -1449,6 +1714,15 @@ static void prim_vec(const Ctx *c, const State *s, double *w) {
     w[q++] = s->ph[0].u[0]; w[q++] = s->ph[0].u[1]; w[q++] = s->ph[0].u[2];
     return;
   }
+  if (c->cfg->model == ORC_EULERHOM) {
+    /* PhaseEulerHomogeneous::computeSlopesPhase (alpha), MixEulerHomogeneous::
+       computeSlopesMixture (p, u, v, w): PhaseEulerHomogeneous.cpp:139-142,
+       MixEulerHomogeneous.cpp:331-337 */
+    for (int k = 0; k < c->cfg->nph; k++) w[q++] = s->ph[k].alpha;
+    w[q++] = s->mx.p;
+    w[q++] = s->mx.u[0]; w[q++] = s->mx.u[1]; w[q++] = s->mx.u[2];
+    return;
+  }
   if (c->cfg->model == ORC_THERMALEQ) {
     /* PhaseThermalEq::computeSlopesPhase (alpha), MixThermalEq::computeSlopesMixture
        (p, T, u, v, w): PhaseThermalEq.cpp:139-142, MixThermalEq.cpp:457-464 */
@@ -1534,8 +1808,10 @@ static void local_slopes(Solver *S, int cell, int fref, int fopp, int cell_is_le
     for (int v = 0; v < nv; v++) out[v] = limiter(lg, s1[v], s2[v]);
     return;
   }
-  if (cfg->model == ORC_THERMALEQ) {
-    /* PhaseThermalEq::limitSlopes (volume-fraction limiter), MixThermalEq::limitSlopes */
+  if (cfg->model == ORC_THERMALEQ || cfg->model == ORC_EULERHOM) {
+    /* PhaseThermalEq::limitSlopes (volume-fraction limiter), MixThermalEq::limitSlopes;
+       PhaseEulerHomogeneous::limitSlopes, MixEulerHomogeneous::limitSlopes
+       (PhaseEulerHomogeneous.cpp:157-160, MixEulerHomogeneous.cpp:351-357) */
     for (int v = 0; v < nv; v++) out[v] = limiter(v < cfg->nph ? lv : lg, s1[v], s2[v]);
     return;
   }
@@ -1570,8 +1846,10 @@ static void local_slopes_limite(Solver *S, int cell, int fwall, int fopp, double
     for (int v = 0; v < nv; v++) out[v] = limiter(lg, s1[v], s2[v]);
     return;
   }
-  if (cfg->model == ORC_THERMALEQ) {
-    /* PhaseThermalEq::limitSlopes (volume-fraction limiter), MixThermalEq::limitSlopes */
+  if (cfg->model == ORC_THERMALEQ || cfg->model == ORC_EULERHOM) {
+    /* PhaseThermalEq::limitSlopes (volume-fraction limiter), MixThermalEq::limitSlopes;
+       PhaseEulerHomogeneous::limitSlopes, MixEulerHomogeneous::limitSlopes
+       (PhaseEulerHomogeneous.cpp:157-160, MixEulerHomogeneous.cpp:351-357) */
     for (int v = 0; v < nv; v++) out[v] = limiter(v < cfg->nph ? lv : lg, s1[v], s2[v]);
     return;
   }
@@ -1614,6 +1892,17 @@ static void extrapolate_raw(const Ctx *c, State *s, const double *sl, double dis
 
 static void extrapolate(const Ctx *c, State *s, const double *sl, double dist, double sign) {
   int q = 0;
+  if (c->cfg->model == ORC_EULERHOM) {
+    /* PhaseEulerHomogeneous::extrapolate + verifyAndCorrectPhase per phase, then
+       MixEulerHomogeneous::extrapolate (MixEulerHomogeneous.cpp:341-347) */
+    for (int k = 0; k < c->cfg->nph; k++, q++) {
+      s->ph[k].alpha += (sign * sl[q]) * dist;
+      verify_correct(c, k, &s->ph[k]);
+    }
+    s->mx.p += (sign * sl[q]) * dist; q++;
+    for (int d = 0; d < 3; d++, q++) s->mx.u[d] = s->mx.u[d] + (sign * sl[q]) * dist;
+    return;
+  }
   if (c->cfg->model == ORC_THERMALEQ) {
     /* PhaseThermalEq::extrapolate + verifyAndCorrectPhase per phase, then
        MixThermalEq::extrapolate (MixThermalEq.cpp:471-478) */
@@ -1767,7 +2056,9 @@ static void accumulate(Solver *S) {
   Mesh *m = &S->m;
   const Ctx *c = &S->c;
   int n = c->cfg->nph;
-  int euler = (c->cfg->model == ORC_EULER);
+  /* FluxEulerHomogeneous::addFlux/subtractFlux (FluxEulerHomogeneous.cpp:
+     25-39): one mass, as the Euler model, no non-conservative terms */
+  int euler = (c->cfg->model == ORC_EULER || c->cfg->model == ORC_EULERHOM);
   int te = (c->cfg->model == ORC_THERMALEQ);
   double vol = m->vol[0];
 #pragma omp parallel for schedule(static)
@@ -1846,7 +2137,7 @@ static double reduce_dt(Solver *S) {
    (Cell.cpp:214-227) and CellO2::predictionOrdre2 (CellO2.cpp:376-397). */
 static void scale_add(const Ctx *c, Flux *U, double mult, const Flux *buf) {
   int n = c->cfg->nph;
-  if (c->cfg->model == ORC_EULER) {
+  if (c->cfg->model == ORC_EULER || c->cfg->model == ORC_EULERHOM) {
     U->masse[0] *= mult;
   } else if (c->cfg->model == ORC_THERMALEQ) {
     for (int k = 0; k < n; k++) U->masse[k] *= mult;
@@ -1855,7 +2146,7 @@ static void scale_add(const Ctx *c, Flux *U, double mult, const Flux *buf) {
   }
   for (int d = 0; d < 3; d++) U->qdm[d] *= mult;
   U->energMix *= mult;
-  if (c->cfg->model == ORC_EULER) {
+  if (c->cfg->model == ORC_EULER || c->cfg->model == ORC_EULERHOM) {
     U->masse[0] += 1. * buf->masse[0];
   } else if (c->cfg->model == ORC_THERMALEQ) {
     for (int k = 0; k < n; k++) U->masse[k] += 1. * buf->masse[k];
@@ -1879,6 +2170,7 @@ static void solve_relaxations(Solver *S) {
     State *s = &S->sn[ci];
     if (model_relaxes(c->cfg)) relax_p(c, s);
     if (c->cfg->relax_pt) relax_pt(c, s);
+    if (c->cfg->relax_ptmu) relax_ptmu(c, s, s);
     total_to_internal(c, s);
     correction_energy(c, s, s);
   }
@@ -1930,6 +2222,7 @@ static double step(Solver *S, double dt) {
       transport_update(c, &S->cons[ci], 0.5 * dt, sn, s2);
       if (model_relaxes(c->cfg) && c->cfg->nph > 1) relax_p(c, s2);
       if (c->cfg->relax_pt) relax_pt(c, s2);
+      if (c->cfg->relax_ptmu) relax_ptmu(c, s2, sn);
       total_to_internal(c, s2);
       correction_energy(c, s2, sn);
       fulfill(c, s2);
@@ -1966,6 +2259,20 @@ static void init_state(Solver *S, const double *in) {
       s->ph[0].p = in[(size_t)2 * nc + ci];
       for (int d = 0; d < 3; d++) s->ph[0].u[d] = u[d];
       fulfill(c, s);
+      continue;
+    }
+    if (c->cfg->model == ORC_EULERHOM) {
+      /* Cell::fill + completeFulfillState: alpha_k, the mixture p, u; the
+         phases take the saturated state of p (ModEulerHomogeneous::fulfillState) */
+      for (int k = 0; k < n; k++) s->ph[k].alpha = in[(size_t)(3 * k) * nc + ci];
+      s->mx.p = in[(size_t)2 * nc + ci];
+      for (int d = 0; d < 3; d++) s->mx.u[d] = u[d];
+      fulfill(c, s);
+      s->mx.E = s->mx.e + 0.5 * sqnorm(s->mx.u);
+      /* the predictor's state (CellO2::m_vecPhasesO2) starts as a copy of the
+         domain's phases (CellO2::copyPhase, CellO2.cpp:93-97); its liquid
+         pressure is the first predictor's Newton start */
+      S->s2[ci] = *s;
       continue;
     }
     if (c->cfg->model == ORC_THERMALEQ) {
@@ -2011,7 +2318,7 @@ static void write_state(Solver *S, double *out) {
     for (int d = 0; d < 3; d++) out[(size_t)(3 * n + d) * nc + ci] = s->mx.u[d];
     out[(size_t)(3 * n + 3) * nc + ci] = s->mx.E;
     for (int t = 0; t < c->cfg->ntr; t++) out[(size_t)(3 * n + 4 + t) * nc + ci] = s->tr[t];
-    if (c->cfg->model == ORC_THERMALEQ) out[(size_t)(3 * n + 4) * nc + ci] = s->mx.T;
+    if (c->cfg->model == ORC_THERMALEQ || c->cfg->model == ORC_EULERHOM) out[(size_t)(3 * n + 4) * nc + ci] = s->mx.T;
   }
 }
 
@@ -2037,11 +2344,23 @@ int orc_run_from(const orc_cfg *cfg, const double *state_in, double *state_out,
   /* ThermalEq: two phases (MixThermalEq::computePressure), no transport
      solver, injection or tank (ModThermalEq has no solveRiemannInflow; its
      tank is not restated here) */
+  /* RelaxationPTMu: two phases (RelaxationPTMu.cpp:30), here after Kapila's
+     pressure relaxation, not together with RelaxationPT */
+  if (cfg->relax_ptmu && (cfg->nph != 2 || cfg->model != ORC_KAPILA || cfg->relax_pt)) return -1;
   /* RelaxationPT: two phases (RelaxationPT.cpp:60), Kapila or MultiP */
   if (cfg->relax_pt && (cfg->nph != 2 || (cfg->model != ORC_KAPILA && cfg->model != ORC_MULTIP))) return -1;
   if (cfg->model == ORC_THERMALEQ) {
     if (cfg->nph != 2 || cfg->ntr > 0) return -1;
     for (int b = 0; b < 6; b++) if (cfg->bc[b] == ORC_BC_INJ || cfg->bc[b] == ORC_BC_TANK) return -1;
+  }
+  /* EulerHomogeneous: two phases, a liquid and a vapour (Input.cpp:479-483);
+     only the interior Riemann solver (no wall, outflow, injection, tank or
+     transport solver: Model.h:93-145 end the run); THINC sets two fractions
+     of a phase-0/phase-1 pair as any model */
+  if (cfg->model == ORC_EULERHOM) {
+    if (cfg->nph != 2 || cfg->ntr > 0 || cfg->relax_pt || cfg->relax_ptmu) return -1;
+    if (cfg->liq < 0 || cfg->liq > 1 || cfg->vap != 1 - cfg->liq) return -1;
+    for (int b = 0; b < 6; b++) if (cfg->bc[b] != ORC_BC_ABS) return -1;
   }
   for (int b = 0; b < 6; b++) {
     if (cfg->bc[b] < ORC_BC_ABS || cfg->bc[b] > ORC_BC_TANK) return -1;
@@ -2055,10 +2374,12 @@ int orc_run_from(const orc_cfg *cfg, const double *state_in, double *state_out,
   Solver S;
   memset(&S, 0, sizeof(S));
   g_relax_fail = 0;
+  g_sat_fail = 0;
   S.c.cfg = cfg;
   S.c.epsilon = (cfg->model == ORC_KAPILA && cfg->alpha_null) ? 1.e-15 : 0.;
   mesh_build(&S.m, cfg);
-  S.nvar = (cfg->model == ORC_EULER) ? 5 : (cfg->model == ORC_THERMALEQ) ? cfg->nph + 5 : 3 * cfg->nph + 3 + cfg->ntr;
+  S.nvar = (cfg->model == ORC_EULER) ? 5 : (cfg->model == ORC_THERMALEQ) ? cfg->nph + 5
+         : (cfg->model == ORC_EULERHOM) ? cfg->nph + 4 : 3 * cfg->nph + 3 + cfg->ntr;
   S.sn = calloc(S.m.ncell, sizeof(State));
   S.s2 = calloc(S.m.ncell, sizeof(State));
   S.cons = calloc(S.m.ncell, sizeof(Flux));

@@ -20,7 +20,11 @@
 extern "C" {
 #endif
 
-enum { ORC_EULER = 0, ORC_KAPILA = 1, ORC_MULTIP = 2, ORC_THERMALEQ = 3 };
+enum { ORC_EULER = 0, ORC_KAPILA = 1, ORC_MULTIP = 2, ORC_THERMALEQ = 3, ORC_EULERHOM = 4 };
+/* ORC_EULERHOM: the homogeneous liquid-vapour model (ModEulerHomogeneous,
+   src/Models/EulerHomogeneous): one mass, momentum and energy equation, two
+   phases at saturation (pressure, saturation temperature, saturated
+   densities), transmissive boundaries only */
 enum { ORC_LIM_NONE = 0, ORC_LIM_MINMOD = 1, ORC_LIM_VANLEER = 2, ORC_LIM_VANALBADA = 3,
        ORC_LIM_SUPERBEE = 4, ORC_LIM_MC = 5, ORC_LIM_THINC = 6 };
 enum { ORC_EOS_IG = 0, ORC_EOS_SG = 1 };
@@ -62,6 +66,10 @@ typedef struct {
   int ntr;                               /* transported scalars (<flowModel numberTransports>); not Euler */
   double bc_tr[6][ORC_MAXTR];            /* outflow, injection, tank: <transport value> per transport */
   int relax_pt;      /* Kapila, MultiP: <relaxation type="PT"/> after the P relaxation (two phases) */
+  int relax_ptmu;    /* Kapila: <relaxation type="PTMu"/> after the P relaxation (two phases, liq/vap below) */
+  int liq, vap;      /* liquid and vapour phase indices: <flowModel liquid vapor> (EulerHomogeneous),
+                        <dataPTMu liquid vapor> (PTMu) */
+  double sref[ORC_MAXPH]; /* each EOS's entropyRef (Mixture::computeTsat, Mixture.cpp:63-104) */
 } orc_cfg;
 
 /*
@@ -76,6 +84,9 @@ typedef struct {
  * per phase alpha_k, rho_k, p_k (p_k the mixture pressure, rho_k = rho_k(p, T);
  * on input alpha_k and p_0 are read), u, v, w, E, then the mixture
  * temperature T (read on input) => nfields = 3*nph + 5, no transports.
+ * EulerHomogeneous: the same fields (p_k the mixture pressure, rho_k the
+ * saturated densities, T the saturation temperature); on input alpha_k, p_0
+ * and u are read.
  *
  * orc_run: initialise from (alpha, rho, p, u) exactly as the reference does
  * at t = 0, then march.  nsteps > 0: iteration-controlled run of nsteps

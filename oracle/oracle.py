@@ -17,7 +17,9 @@ import numpy as np
 HERE = Path(__file__).resolve().parent
 LIB_PATH = HERE / "libeco_oracle.so"
 
-MODELS = {"euler": 0, "kapila": 1, "multip": 2, "thermaleq": 3}
+MODELS = {"euler": 0, "kapila": 1, "multip": 2, "thermaleq": 3, "eulerhomogeneous": 4}
+# models whose host state ends with the mixture temperature
+WITH_T = ("thermaleq", "eulerhomogeneous")
 LIMITERS = {"none": 0, "minmod": 1, "vanleer": 2, "vanalbada": 3, "superbee": 4, "mc": 5, "thinc": 6}
 EOS_TYPES = {"IG": 0, "SG": 1}
 # per side X-, X+, Y-, Y+, Z-, Z+; outflow pressures in case["bc_p0"]; injections in
@@ -44,7 +46,8 @@ class OrcCfg(C.Structure):
                 ("bc_pk0", (C.c_double * MAXPH) * 6), ("bc_tank_p0", C.c_double * 6),
                 ("bc_tank_T0", C.c_double * 6), ("bc_tank_frac", (C.c_double * MAXPH) * 6),
                 ("bc_tank_massfrac", C.c_int * 6), ("ntr", C.c_int), ("bc_tr", (C.c_double * MAXTR) * 6),
-                ("relax_pt", C.c_int)]
+                ("relax_pt", C.c_int), ("relax_ptmu", C.c_int), ("liq", C.c_int), ("vap", C.c_int),
+                ("sref", C.c_double * MAXPH)]
 
 
 _lib = None
@@ -132,6 +135,12 @@ def make_cfg(case) -> OrcCfg:
                 cfg.bc_tank_frac[b][k] = v
     cfg.ntr = len(case.get("transports", []))
     cfg.relax_pt = int(bool(case.get("relax_pt", False)))
+    cfg.relax_ptmu = int(bool(case.get("relax_ptmu", False)))
+    # liquid and vapour phases (EulerHomogeneous <flowModel liquid vapor>;
+    # RelaxationPTMu always phase 0 and 1) and each EOS's entropyRef
+    cfg.liq, cfg.vap = case.get("liq", 0), case.get("vap", 1)
+    for k, e in enumerate(case["eos"]):
+        cfg.sref[k] = e.get("sref", 0.0)
     for b, vals in enumerate(case.get("bc_tr", [None] * 6)):
         for t, v in enumerate(vals or []):
             cfg.bc_tr[b][t] = v
@@ -143,8 +152,8 @@ def ntransports(case) -> int:
 
 
 def nfields(case) -> int:
-    """per phase alpha, rho, p; u, v, w; E; the transports; ThermalEq: T last"""
-    return 3 * len(case["eos"]) + 4 + ntransports(case) + (case["model"] == "thermaleq")
+    """per phase alpha, rho, p; u, v, w; E; the transports; ThermalEq, EulerHomogeneous: T last"""
+    return 3 * len(case["eos"]) + 4 + ntransports(case) + (case["model"] in WITH_T)
 
 
 def run(case, state, nsteps=None, t_final=None, max_steps=0, nthreads=0, log_dt=False, resume=None):
@@ -183,7 +192,7 @@ def state_from_golden(case, d, prefix="init"):
     nph = len(case["eos"])
     ncell = int(np.prod(case["cells"]))
     ntr = ntransports(case)
-    te = case["model"] == "thermaleq"
+    te = case["model"] in WITH_T
     out = np.zeros((3 * nph + 4 + ntr + te if (ntr or te) else 3 * nph + 3, ncell))
     if ntr:
         out[3 * nph + 4:3 * nph + 4 + ntr] = d[f"{prefix}_tr"]

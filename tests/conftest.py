@@ -85,7 +85,7 @@ def parity_quantities(case, state):
     if case["model"] == "euler":
         out["rho"] = state[1]
         out["p"] = state[2]
-    elif case["model"] == "thermaleq":
+    elif case["model"] in ("thermaleq", "eulerhomogeneous"):
         # one pressure and temperature (the temperature after E)
         out["rho"] = sum(state[3 * k] * state[3 * k + 1] for k in range(nph))
         out["p"] = state[2]
@@ -131,7 +131,7 @@ def compared_fields(case):
     # transported scalars after E; the thermal-equilibrium temperature
     for t in range(len(case.get("transports", []))):
         out.append((3 * nph + 4 + t, f"tr{t}"))
-    if case["model"] == "thermaleq":
+    if case["model"] in ("thermaleq", "eulerhomogeneous"):
         out.append((3 * nph + 4, "T"))
     return out
 

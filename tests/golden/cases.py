@@ -11,6 +11,13 @@ AIR = {"type": "IG", "name": "air", "gamma": 1.4, "pinf": 0.0, "cv": 800.0, "ere
 WATER = {"type": "SG", "name": "water", "gamma": 4.4, "pinf": 6.0e8, "cv": 1000.0, "eref": 0.0, "sref": 0.0}
 HELIUM = {"type": "IG", "name": "helium", "gamma": 1.67, "pinf": 0.0, "cv": 3116.0, "eref": 0.0, "sref": 0.0}
 OIL = {"type": "SG", "name": "oil", "gamma": 2.35, "pinf": 4.0e8, "cv": 1500.0, "eref": 0.0, "sref": 0.0}
+# liquid water and its vapour, libEOS/SG_waterLiq.xml and IG_waterVap.xml
+# (the liquid-vapour decks of libTests/referenceTestCases/eulerHomogeneous and
+# kapila/AddPhysicalEffects/evap)
+WATER_LIQ = {"type": "SG", "name": "waterLiq", "gamma": 3.423, "pinf": 8.99e8, "cv": 1231.2, "eref": -1.15e6,
+             "sref": 0.0}
+WATER_VAP = {"type": "IG", "name": "waterVap", "gamma": 1.358, "pinf": 0.0, "cv": 1247.0, "eref": 1.97e6,
+             "sref": -3.56e4}
 
 
 def _splitmix64(state):
@@ -299,6 +306,53 @@ def _thermaleq(case):
     return c
 
 
+def _eulerhom_1d(order, limiters, steps, cells=200):
+    """The reference's homogeneous liquid-vapour shock tube
+    (libTests/referenceTestCases/eulerHomogeneous/shockTubes): a mostly
+    liquid mixture at 1e6 Pa for x < 0.3 against a mostly vapour one at 1e5 Pa."""
+    return {
+        "model": "eulerhomogeneous", "liq": 0, "vap": 1, "eos": [WATER_LIQ, WATER_VAP],
+        "cells": [cells, 1, 1], "lengths": [1.0, 0.1, 0.1], "order": order, "limiters": limiters,
+        "cfl": 0.8, "time_mode": "iterations", "steps": steps,
+        "domains": [{"type": "entireDomain", "state": "LP"},
+                    {"type": "halfSpace", "state": "HP", "axe": "x", "origin": 0.3, "direction": "negative"}],
+        "states": [{"name": "LP", "alpha": [0.01, 0.99], "p": [1e5], "u": [0.0]},
+                   {"name": "HP", "alpha": [0.99, 0.01], "p": [1e6], "u": [0.0]}],
+    }
+
+
+def _eulerhom_2d(order, limiters, steps, n=40):
+    """A disc of mostly liquid mixture at 1e6 Pa in a mostly vapour one at 1e5 Pa."""
+    c = _eulerhom_1d(order, limiters, steps)
+    c.update({"cells": [n, n, 1], "lengths": [1.0, 1.0, 0.1],
+              "domains": [{"type": "entireDomain", "state": "LP"},
+                          {"type": "disc", "state": "HP", "radius": 0.25, "center": [0.45, 0.5]}]})
+    return c
+
+
+def _eulerhom_3d(steps, n=10):
+    c = _eulerhom_1d(1, ["none"] * 4, steps)
+    c.update({"cells": [n, n, n], "lengths": [1.0, 1.0, 1.0],
+              "domains": [{"type": "entireDomain", "state": "LP"},
+                          {"type": "sphere", "state": "HP", "radius": 0.3, "center": [0.45, 0.5, 0.5]}]})
+    return c
+
+
+def _evap_tube(order, limiters, steps, cells=200):
+    """The reference's evaporation shock tube (libTests/referenceTestCases/kapila/
+    AddPhysicalEffects/evap/evapShockTube): Kapila with liquid water and its
+    vapour, relaxed to saturation (RelaxationPTMu) after the pressure relaxation."""
+    return {
+        "model": "kapila", "alpha_null": False, "eos": [WATER_LIQ, WATER_VAP], "relax_ptmu": True,
+        "cells": [cells, 1, 1], "lengths": [1.0, 0.1, 0.1], "order": order, "limiters": limiters,
+        "cfl": 0.8, "time_mode": "iterations", "steps": steps,
+        "domains": [{"type": "entireDomain", "state": "LP"},
+                    {"type": "halfSpace", "state": "HP", "axe": "x", "origin": 0.3, "direction": "negative"}],
+        "states": [{"name": "LP", "alpha": [0.01, 0.99], "rho": [807.023, 0.6], "p": [1e5, 1e5], "u": [0.0]},
+                   {"name": "HP", "alpha": [0.99, 0.01], "rho": [659.13, 4.894], "p": [1e6, 1e6], "u": [0.0]}],
+    }
+
+
 CASES = {
     # Config C1 in full: 1000 cells, first order, to t = 240 us.
     "c1_kapila_fo": _c1(False),
@@ -462,6 +516,21 @@ CASES = {
                                     order=1, limiters=["none"] * 4),
     "kapila_2d_o2_relax_pt": dict(_shock_cylinders_2d(n=40, steps=40), relax_pt=True,
                                   bc=["abs", "outflow", "wall", "wall", "abs", "abs"], bc_p0=[0.0, 8e4] + [0.0] * 4),
+    # Homogeneous liquid-vapour model (ModEulerHomogeneous), first order (the
+    # reference's MUSCL form starts its first predictor's Newton solve from an
+    # unset pressure and prints NaN): the reference's shock tube, iteration and
+    # time controlled, a 2-D disc and a 3-D sphere
+    "eulerhom_1d_fo": _eulerhom_1d(1, ["none"] * 4, 200),
+    "eulerhom_1d_fo_time": dict(_eulerhom_1d(1, ["none"] * 4, 0, cells=400), time_mode="time", t_final=1.2e-3),
+    "eulerhom_2d_fo": _eulerhom_2d(1, ["none"] * 4, 60),
+    "eulerhom_3d_fo": _eulerhom_3d(12),
+    # Relaxation to saturation (RelaxationPTMu) after Kapila's pressure relaxation:
+    # the reference's evaporation shock tube at first order and MUSCL
+    "kapila_1d_fo_ptmu": _evap_tube(1, ["none"] * 4, 200),
+    "kapila_1d_o2_ptmu": _evap_tube(2, ["vanleer"] * 4, 150),
+    "kapila_2d_o2_ptmu": dict(_evap_tube(2, ["minmod"] * 4, 30), cells=[32, 32, 1], lengths=[1.0, 1.0, 0.1],
+                              domains=[{"type": "entireDomain", "state": "LP"},
+                                       {"type": "disc", "state": "HP", "radius": 0.25, "center": [0.45, 0.5]}]),
 }
 
 

@@ -57,8 +57,9 @@ def fields_from_vtr(case, d):
     model = case["model"]
     nph = len(case["eos"])
     res = {}
-    if model == "thermaleq":
-        # PhaseThermalEq prints Alpha and Density, MixThermalEq the mixture
+    if model in ("thermaleq", "eulerhomogeneous"):
+        # PhaseThermalEq (PhaseEulerHomogeneous) prints Alpha and Density,
+        # MixThermalEq (MixEulerHomogeneous) the mixture
         # density, pressure, temperature and velocity (ThermalEq*.cpp returnName*)
         res["alpha"] = np.stack([d[[k for k in d if k.startswith(f"F{i}_Alpha")][0]] for i in range(nph)])
         res["rho"] = np.stack([d[[k for k in d if k.startswith(f"F{i}_Density")][0]] for i in range(nph)])
@@ -66,7 +67,7 @@ def fields_from_vtr(case, d):
         res["p_mix"] = d["Pressure_Mixture"]
         res["rho_mix"] = d["Density_Mixture"]
         res["T"] = d["Temperature_Mixture"]
-        res["u"] = d["Velocity_Mixture_norm"].T.copy()
+        res["u"] = (d["Velocity_Mixture_norm"] if model == "thermaleq" else d["Velocity_Mixture"]).T.copy()
         return res
     if model == "euler":
         key = [k for k in d if k.startswith("F0_Density")][0]
