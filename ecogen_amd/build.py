@@ -31,7 +31,7 @@ COMMON = ["-O3", f"--offload-arch={ARCH}", "-std=c++17", "-fPIC", "-ffp-contract
 
 SOURCES = ["eco_k_kapila.hip", "eco_k_multip.hip", "eco_k_euler.hip", "eco_k_kapila32.hip", "eco_k_multip32.hip",
            "eco_k_kapila3.hip", "eco_k_multip3.hip", "eco_k_kapila4.hip", "eco_k_multip4.hip",
-           "eco_k_thermaleq.hip", "eco_transport.hip", "eco_dispatch.cpp",
+           "eco_k_thermaleq.hip", "eco_k_eulerhom.hip", "eco_transport.hip", "eco_dispatch.cpp",
            "eco_capi.cpp", "eco_pool.cpp", "eco_api.cpp", "eco_noalloc.cpp"]
 
 

@@ -13,7 +13,9 @@ import numpy as np
 
 LIB_PATH = Path(__file__).resolve().parent / "libecogen_amd.so"
 
-MODEL = {"euler": 0, "kapila": 1, "multip": 2, "thermaleq": 3}
+MODEL = {"euler": 0, "kapila": 1, "multip": 2, "thermaleq": 3, "eulerhomogeneous": 4}
+# models whose host state ends with the mixture temperature
+WITH_T = ("thermaleq", "eulerhomogeneous")
 LIMITER = {"none": 0, "minmod": 1, "vanleer": 2, "vanalbada": 3, "superbee": 4, "mc": 5, "thinc": 6}
 EOS_TYPE = {"IG": 0, "SG": 1}
 PRECISION = {"fp64": 0, "fp32s": 1}
@@ -54,7 +56,8 @@ class EcoConfig(C.Structure):
                 ("bc_inj_p", (C.c_double * 4) * 6), ("bc_tank_p0", C.c_double * 6),
                 ("bc_tank_T0", C.c_double * 6), ("bc_tank_frac", (C.c_double * 4) * 6),
                 ("bc_tank_massfrac", C.c_int * 6), ("ntransport", C.c_int),
-                ("bc_tr", (C.c_double * MAX_TRANSPORTS) * 6), ("relax_pt", C.c_int)]
+                ("bc_tr", (C.c_double * MAX_TRANSPORTS) * 6), ("relax_pt", C.c_int),
+                ("relax_ptmu", C.c_int), ("liq", C.c_int), ("vap", C.c_int), ("sref", C.c_double * 4)]
 
 
 class EcoStatus(C.Structure):
@@ -166,6 +169,10 @@ def make_config(case) -> EcoConfig:
         c.eos[k] = EcoEos(EOS_TYPE[e["type"]], e["gamma"], e.get("pinf", 0.0), e["cv"], e.get("eref", 0.0))
     c.ntransport = len(case.get("transports", []))
     c.relax_pt = int(bool(case.get("relax_pt", False)))
+    c.relax_ptmu = int(bool(case.get("relax_ptmu", False)))
+    c.liq, c.vap = case.get("liq", 0), case.get("vap", 1)
+    for k, e in enumerate(case["eos"]):
+        c.sref[k] = e.get("sref", 0.0)
     for b, vals in enumerate(case.get("bc_tr") or [None] * 6):
         for t, v in enumerate(vals or []):
             c.bc_tr[b][t] = v
@@ -174,8 +181,9 @@ def make_config(case) -> EcoConfig:
 
 def nfields(case) -> int:
     """Fields of a host state: per phase alpha, rho, p; u, v, w; E; the
-    transports; for the thermal-equilibrium model the mixture temperature last."""
-    return 3 * len(case["eos"]) + 4 + len(case.get("transports", [])) + (case["model"] == "thermaleq")
+    transports; for the thermal-equilibrium and homogeneous liquid-vapour
+    models the mixture temperature last."""
+    return 3 * len(case["eos"]) + 4 + len(case.get("transports", [])) + (case["model"] in WITH_T)
 
 
 class Context:

@@ -44,6 +44,21 @@ static std::vector<std::string> multiphase_vars(int nph, int dim) {
 }
 std::vector<std::string> FluxKapila::variables(int nph, int dim) const { return multiphase_vars(nph, dim); }
 std::vector<std::string> FluxMultiP::variables(int nph, int dim) const { return multiphase_vars(nph, dim); }
+// FluxThermalEq: the phases' masses, momentum, total energy (FluxThermalEq.cpp:113-126)
+std::vector<std::string> FluxThermalEq::variables(int nph, int dim) const {
+  std::vector<std::string> v;
+  for (int k = 0; k < nph; k++) v.push_back("alpha_rho_" + std::to_string(k));
+  add_momentum(v, dim);
+  v.push_back("rho_E");
+  return v;
+}
+// FluxEulerHomogeneous: the mixture's mass, momentum, total energy (FluxEulerHomogeneous.cpp:70-89)
+std::vector<std::string> FluxEulerHomogeneous::variables(int, int dim) const {
+  std::vector<std::string> v{"rho"};
+  add_momentum(v, dim);
+  v.push_back("rho_E");
+  return v;
+}
 
 eco_config lowerConfig(const Model &model, const std::vector<std::unique_ptr<Eos>> &eos, const CartesianMesh &mesh,
                        double cfl, int order, const Limiter *lim[4], int precision) {
@@ -64,7 +79,16 @@ eco_config lowerConfig(const Model &model, const std::vector<std::unique_ptr<Eos
   c.relax_p = 0;
   if (model.kind() == ECO_MODEL_MULTIP)
     for (const auto &r : model.relaxations()) if (r->getType() == "P") c.relax_p = 1;
-  for (const auto &r : model.relaxations()) if (r->getType() == "PT") c.relax_pt = 1;
+  for (const auto &r : model.relaxations()) {
+    if (r->getType() == "PT") c.relax_pt = 1;
+    if (r->getType() == "PTMU") c.relax_ptmu = 1;
+  }
+  c.liq = 0;
+  c.vap = 1;
+  if (const ModEulerHomogeneous *h = dynamic_cast<const ModEulerHomogeneous *>(&model)) {
+    c.liq = h->getLiq();
+    c.vap = h->getVap();
+  }
   c.precision = precision;
   for (int d = 0; d < 3; d++) { c.n[d] = mesh.numberCells[d]; c.L[d] = mesh.dimensions[d]; }
   c.cfl = cfl;
@@ -84,7 +108,10 @@ eco_config lowerConfig(const Model &model, const std::vector<std::unique_ptr<Eos
     for (int t = 0; t < ECO_MAX_TRANSPORTS; t++) c.bc_tr[b][t] = mesh.transportValue[b][t];
   }
   c.ntransport = model.getNumberTransports();
-  for (size_t k = 0; k < eos.size() && k < 4; k++) c.eos[k] = eos[k]->lower();
+  for (size_t k = 0; k < eos.size() && k < 4; k++) {
+    c.eos[k] = eos[k]->lower();
+    c.sref[k] = eos[k]->getSRef();
+  }
   return c;
 }
 

@@ -30,6 +30,7 @@ struct eco_ctx {
   int ntr;              // transported scalars: fields nlevel.. of a time level, nprim.. of U*
   double *buf[2];       // time levels, nlevel + ntr fields each
   double *bstar;        // U*, nprim + ntr fields
+  double *bn2;          // RelaxationPTMu in MUSCL: the time-n state relaxed by the predictor
   double *smf[3];       // with transports: the faces' sM of the last stage, per direction
   double *tmp64;        // FP64 staging of set/get_state in the FP32-storage form
   size_t es;            // bytes per stored value (8 or 4)
@@ -164,7 +165,7 @@ static void lower_injection(const eco_config &c, int b, SchemeParams &S) {
 
 int eco::validate_config(const eco_config *c) {
   if (!c) return ECO_ERR_INVALID;
-  if (c->model < 0 || c->model > 3) return ECO_ERR_INVALID;
+  if (c->model < 0 || c->model > ECO_MODEL_EULERHOM) return ECO_ERR_INVALID;
   if (c->order != 1 && c->order != 2) return ECO_ERR_INVALID;
   if (c->cfl <= 0.) return ECO_ERR_INVALID;
   for (int d = 0; d < 3; d++) if (c->n[d] < 1 || !(c->L[d] > 0.)) return ECO_ERR_INVALID;
@@ -205,6 +206,17 @@ int eco::validate_config(const eco_config *c) {
   }
   if (c->model == ECO_MODEL_EULER) {
     if (c->nphase != 1) return ECO_ERR_INVALID;
+  } else if (c->model == ECO_MODEL_EULERHOM) {
+    // two phases, a liquid and a vapour (Input.cpp:479-483); ModEulerHomogeneous
+    // has only the interior Riemann solver (Model.h:93-145 end the run for the
+    // boundary and transport solvers); its MUSCL form starts the first
+    // predictor's Newton solve from a pressure the reference leaves unset
+    // (PhaseEulerHomogeneous.cpp:14-24, CellO2.cpp:93-97), so only first order
+    if (c->nphase != 2 || c->ntransport > 0 || c->relax_pt || c->relax_ptmu || c->relax_p) return ECO_ERR_INVALID;
+    if (c->liq < 0 || c->liq > 1 || c->vap != 1 - c->liq) return ECO_ERR_INVALID;
+    for (int i = 0; i < 6; i++) if (c->bc[i] != ECO_BC_ABS) return ECO_ERR_INVALID;
+    if (c->order != 1) return ECO_ERR_UNSUPPORTED;
+    if (c->precision != ECO_FP64) return ECO_ERR_UNSUPPORTED;
   } else if (c->model == ECO_MODEL_THERMALEQ) {
     // two phases (MixThermalEq::computePressure, MixThermalEq.cpp:123); no
     // transport or injection solver (Model.h:127-135 end the run); the tank
@@ -239,6 +251,13 @@ int eco::validate_config(const eco_config *c) {
   if (c->ntransport < 0 || c->ntransport > ECO_MAX_TRANSPORTS) return ECO_ERR_INVALID;
   if (c->ntransport > 0 && c->model == ECO_MODEL_EULER) return ECO_ERR_INVALID;
   if (c->ntransport > 0 && c->precision == ECO_FP32S) return ECO_ERR_UNSUPPORTED;
+  // RelaxationPTMu: two phases (RelaxationPTMu.cpp:30; phase 0 the liquid,
+  // phase 1 the vapour), after Kapila's pressure relaxation, FP64 single
+  // solves without transports
+  if (c->relax_ptmu) {
+    if (c->nphase != 2 || c->model != ECO_MODEL_KAPILA || c->relax_pt) return ECO_ERR_INVALID;
+    if (c->precision != ECO_FP64 || c->ntransport > 0) return ECO_ERR_UNSUPPORTED;
+  }
   // RelaxationPT: two phases (RelaxationPT.cpp:60), Kapila or MultiP, FP64
   if (c->relax_pt) {
     if (c->nphase != 2 || (c->model != ECO_MODEL_KAPILA && c->model != ECO_MODEL_MULTIP)) return ECO_ERR_INVALID;
@@ -302,6 +321,7 @@ void eco::lower_config(const eco_config &c, Lowered &L, bool with_tables) {
   S.eps = (c.model == ECO_MODEL_KAPILA && c.alpha_null) ? 1.e-15 : 0.;
   S.relax = (c.model == ECO_MODEL_KAPILA) || (c.model == ECO_MODEL_MULTIP && c.relax_p);
   S.relax_pt = c.relax_pt ? 1 : 0;
+  S.relax_ptmu = (c.model == ECO_MODEL_KAPILA && c.relax_ptmu) ? 1 : 0;
   S.cfl = c.cfl;
   S.prec = c.precision;
   S.bc_wall = 0;
@@ -318,11 +338,28 @@ void eco::lower_config(const eco_config &c, Lowered &L, bool with_tables) {
   for (int k = 0; k < c.nphase && k < kMaxPhases; k++) lower_eos(c.eos[k], S.eos[k]);
   for (int b = 0; b < 6; b++) if (c.bc[b] == ECO_BC_INJECTION) lower_injection(c, b, S);
   for (int b = 0; b < 6; b++) if (c.bc[b] == ECO_BC_TANK) lower_tank(c, b, S);
-  L.nprim = (c.model == ECO_MODEL_EULER) ? 2 + dim : (c.model == ECO_MODEL_THERMALEQ) ? c.nphase + 2 + dim : 3 * c.nphase + dim;
+  // liquid-vapour saturation: Mixture::computeTsat's coefficients
+  // (Mixture.cpp:69-85) for EulerHomogeneous's liquid and vapour and for
+  // RelaxationPTMu's phases 0 and 1
+  S.liq = 0;
+  S.vap = 1;
+  if (c.model == ECO_MODEL_EULERHOM) { S.liq = c.liq; S.vap = c.vap; }
+  for (int i = 0; i < 4; i++) S.tsat[i] = 0.;
+  if (c.nphase == 2 && (c.model == ECO_MODEL_EULERHOM || S.relax_ptmu)) {
+    const eco_eos &qL = c.eos[S.liq], &qV = c.eos[S.vap];
+    const double gammaL = qL.gamma, cvL = qL.cv, e0L = qL.eref, s0L = c.sref[S.liq];
+    const double gammaV = qV.gamma, cvV = qV.cv, e0V = qV.eref, s0V = c.sref[S.vap];
+    S.tsat[0] = (gammaL * cvL - gammaV * cvV + s0V - s0L) / (gammaV * cvV - cvV);
+    S.tsat[1] = (e0L - e0V) / (gammaV * cvV - cvV);
+    S.tsat[2] = (gammaV * cvV - gammaL * cvL) / (gammaV * cvV - cvV);
+    S.tsat[3] = (gammaL * cvL - cvL) / (gammaV * cvV - cvV);
+  }
+  L.nprim = (c.model == ECO_MODEL_EULER) ? 2 + dim
+          : (c.model == ECO_MODEL_THERMALEQ || c.model == ECO_MODEL_EULERHOM) ? c.nphase + 2 + dim : 3 * c.nphase + dim;
   S.ncache = (c.model == ECO_MODEL_MULTIP && (c.relax_p || c.relax_pt) && c.precision == ECO_FP64)
                  ? (c.order == 1 ? c.nphase + 3 : 1) : 0;
-  // ThermalEq first order: the cells' cached mixture energy
-  if (c.model == ECO_MODEL_THERMALEQ && c.order == 1) S.ncache = 1;
+  // ThermalEq and EulerHomogeneous at first order: the cells' cached mixture energy
+  if ((c.model == ECO_MODEL_THERMALEQ || c.model == ECO_MODEL_EULERHOM) && c.order == 1) S.ncache = 1;
   L.nlevel = L.nprim + 1 + S.ncache;
   L.ntr = c.ntransport;
   L.es = (c.precision == ECO_FP32S) ? sizeof(float) : sizeof(double);
@@ -340,8 +377,9 @@ void eco::bind_tables(const eco_config &c, Lowered &L, double *dev_tables) {
 // host layout (3N+4 fields, u,v,w) -> device layout (nprim fields)
 void eco::host_field_map(const eco_config &c, int dim, std::vector<int> &map) {
   map.clear();
-  if (c.model == ECO_MODEL_THERMALEQ) {
-    // alpha_k, the mixture pressure (p_0), T (after E), velocity
+  if (c.model == ECO_MODEL_THERMALEQ || c.model == ECO_MODEL_EULERHOM) {
+    // alpha_k, the mixture pressure (p_0), T (after E; EulerHomogeneous: not
+    // read, the initialisation sets the saturation temperature), velocity
     for (int k = 0; k < c.nphase; k++) map.push_back(3 * k);
     map.push_back(2);
     map.push_back(ECO_NFIELDS(c.nphase));
@@ -380,7 +418,7 @@ extern "C" int eco_create(const eco_config *cfg, void *hip_stream, eco_ctx **out
   const size_t nc = (size_t)x->geo.ncell;
   const size_t ntab = L.tab.size();
   hipError_t e = hipSuccess;
-  x->buf[0] = x->buf[1] = x->bstar = x->tables = x->tmp64 = nullptr;
+  x->buf[0] = x->buf[1] = x->bstar = x->bn2 = x->tables = x->tmp64 = nullptr;
   x->smf[0] = x->smf[1] = x->smf[2] = nullptr;
   x->ctl = nullptr;
   x->es = L.es;
@@ -388,6 +426,8 @@ extern "C" int eco_create(const eco_config *cfg, void *hip_stream, eco_ctx **out
   e = hipMalloc(&x->buf[0], nc * nlv * x->es);
   if (e == hipSuccess) e = hipMalloc(&x->buf[1], nc * nlv * x->es);
   if (e == hipSuccess && c.order == 2) e = hipMalloc(&x->bstar, nc * (x->nprim + x->ntr) * x->es);
+  // RelaxationPTMu in MUSCL: the relaxed time-n state the predictor writes
+  if (e == hipSuccess && c.order == 2 && x->sch.relax_ptmu) e = hipMalloc(&x->bn2, nc * nlv * x->es);
   for (int d = 0; d < x->geo.dim && x->ntr > 0 && e == hipSuccess; d++) {
     // faces of direction d: (n_d + 1) per line of cells
     const size_t nf = nc / (size_t)c.n[d] * (size_t)(c.n[d] + 1);
@@ -604,9 +644,11 @@ extern "C" int eco_get_state(eco_ctx *x, double *host_state) {
   if (e == hipSuccess) e = hipStreamSynchronize(x->stream);
   if (e != hipSuccess) return ECO_ERR_HIP;
   if (c.model == ECO_MODEL_EULER) for (size_t i = 0; i < nc; i++) host_state[i] = 1.;
-  if (c.model == ECO_MODEL_THERMALEQ) {
+  if (c.model == ECO_MODEL_THERMALEQ || c.model == ECO_MODEL_EULERHOM) {
     // every phase at the mixture pressure, rho_k = rho_k(p, T) as
-    // Eos::computeDensity (EosSG.cpp:114-117, EosIG.cpp:111-114)
+    // Eos::computeDensity (EosSG.cpp:114-117, EosIG.cpp:111-114); for
+    // EulerHomogeneous T is the saturation temperature and this the saturated
+    // density (Eos*::computeDensitySaturation, EosSG.cpp:191-200)
     const double *p = host_state + 2 * nc, *T = host_state + (size_t)nf * nc;
     for (int k = 0; k < c.nphase; k++) {
       const eco_eos &q = c.eos[k];
@@ -629,10 +671,12 @@ static int launch_one(eco_ctx *x) {
   } else {
     StageArgs P = make_args(x, k);
     P.wout = x->bstar;
+    P.wn2 = x->bn2;
     rc = launch_stage_tr(x, P, 1);
     if (!rc) {
       StageArgs Q = make_args(x, k);
       Q.wsrc = x->bstar;
+      if (x->bn2) Q.wn = x->bn2;  // RelaxationPTMu: the time-n state the predictor relaxed
       rc = launch_stage_tr(x, Q, 2);
     }
   }
@@ -783,12 +827,14 @@ extern "C" int eco_time_steps(eco_ctx *x, int nsteps, double *stage_ms, int *lau
     } else {
       StageArgs P = make_args(x, k);
       P.wout = x->bstar;
+      P.wn2 = x->bn2;
       (void)hipEventRecord(ev[4 * i], x->stream);
       rc = launch_stage_tr(x, P, 1);
       (void)hipEventRecord(ev[4 * i + 1], x->stream);
       if (!rc) {
         StageArgs Q = make_args(x, k);
         Q.wsrc = x->bstar;
+        if (x->bn2) Q.wn = x->bn2;
         (void)hipEventRecord(ev[4 * i + 2], x->stream);
         rc = launch_stage_tr(x, Q, 2);
         (void)hipEventRecord(ev[4 * i + 3], x->stream);
@@ -843,6 +889,7 @@ extern "C" void eco_destroy(eco_ctx *x) {
   if (x->buf[0]) (void)hipFree(x->buf[0]);
   if (x->buf[1]) (void)hipFree(x->buf[1]);
   if (x->bstar) (void)hipFree(x->bstar);
+  if (x->bn2) (void)hipFree(x->bn2);
   for (auto &p : x->smf)
     if (p) (void)hipFree(p);
   if (x->tables) (void)hipFree(x->tables);

@@ -25,9 +25,11 @@ int launch_multip4(const StageArgs &A, int stage, hipStream_t st);
 int launch_kapila32(const StageArgs &A, int stage, hipStream_t st, double *tmp);
 int launch_multip32(const StageArgs &A, int stage, hipStream_t st, double *tmp);
 int launch_thermaleq(const StageArgs &A, int stage, hipStream_t st);
+int launch_eulerhom(const StageArgs &A, int stage, hipStream_t st);
 
 int launch_stage(const StageArgs &A, int stage, hipStream_t st) {
   if (A.sch.model == MODEL_THERMALEQ) return (A.sch.nph == 2 && A.sch.prec == PREC_FP64) ? launch_thermaleq(A, stage, st) : -1;
+  if (A.sch.model == MODEL_EULERHOM) return (A.sch.nph == 2 && A.sch.prec == PREC_FP64) ? launch_eulerhom(A, stage, st) : -1;
   if (A.sch.prec == PREC_FP32S && stage != 3) {
     if (A.sch.model == MODEL_KAPILA) return launch_kapila32(A, stage, st, nullptr);
     if (A.sch.model == MODEL_MULTIP) return launch_multip32(A, stage, st, nullptr);

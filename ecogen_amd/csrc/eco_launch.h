@@ -19,6 +19,9 @@ struct StageArgs {
   // the stage (faces of direction d indexed g0 + (n0 + (d==0)) * (g1 +
   // (n1 + (d==1)) * g2), g_d in [0, n_d]), written for the transport pass
   double *smf[3] = {nullptr, nullptr, nullptr};
+  // RelaxationPTMu in MUSCL: the time level where the predictor writes the
+  // time-n state it relaxes (the corrector's time-n state)
+  double *wn2 = nullptr;
 };
 
 // Transport pass of one stage (src/Transport/Transport.cpp): after the flow

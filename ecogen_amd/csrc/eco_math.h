@@ -101,11 +101,10 @@ ECO_HD double exp_dd(double hi, double lo, double px = 0., double pg = 0.) {
 
 ECO_HD double exp_lib(double x) { return exp_dd(x, 0.); }
 
-ECO_HD double pow_pos(double x, double g) {
-  const unsigned long long ix = as_bits(x);
-  // positive normal finite x only (top 12 bits 0x001..0x7fe)
-  if (!((ix >> 52) - 1u < 0x7feu)) return std::pow(x, g);
-  // log x as hi + lo
+// log x as lhi + llo (lhi = RN(log x) but for rare hard cases, |llo| within
+// half an ulp of it) for a positive normal finite x with bits ix: the log of
+// pow_pos below.
+ECO_HD void log_hl(unsigned long long ix, double &lhi, double &llo) {
   constexpr unsigned long long OFF = 0x3fe6955500000000ull;
   const unsigned long long tmp = ix - OFF;
   const int i = (int)((tmp >> 45) & 127u);
@@ -128,8 +127,31 @@ ECO_HD double pow_pos(double x, double g) {
   const double lo4 = t2 - hi + ar2;
   const double q = std::fma(ar2, std::fma(ar2, std::fma(r, A6, A5), std::fma(r, A4, A3)), std::fma(r, A2, A1));
   const double lo = std::fma(ar3, q, lo1 + lo2 + lo3 + lo4);
-  const double lhi = hi + lo;
-  const double llo = hi - lhi + lo;
+  lhi = hi + lo;
+  llo = hi - lhi + lo;
+}
+
+// The natural logarithm of the saturation laws (Mixture::computeTsat) where
+// the C library's log is called: the double-double logarithm above rounded
+// once.  Its error before rounding is about 2^-68 relative, so it is the
+// correctly rounded log but where log x lies within about 2^-15 ulp of a
+// rounding boundary; the library's log (within 0.52 ulp) agrees with it there
+// in all but rare cases (tests/math_check.cpp counts the mismatches).  Zero,
+// subnormal, negative, infinite and NaN arguments take the build target's log.
+ECO_HD double log_lib(double x) {
+  const unsigned long long ix = as_bits(x);
+  if (!((ix >> 52) - 1u < 0x7feu)) return std::log(x);
+  double hi, lo;
+  log_hl(ix, hi, lo);
+  return hi;
+}
+
+ECO_HD double pow_pos(double x, double g) {
+  const unsigned long long ix = as_bits(x);
+  // positive normal finite x only (top 12 bits 0x001..0x7fe)
+  if (!((ix >> 52) - 1u < 0x7feu)) return std::pow(x, g);
+  double lhi, llo;
+  log_hl(ix, lhi, llo);
   // y log x = ehi + elo
   const double ehi = g * lhi;
   const double elo = std::fma(g, llo, std::fma(g, lhi, -ehi));

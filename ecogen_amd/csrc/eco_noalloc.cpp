@@ -107,7 +107,8 @@ int launch_fill(double *p, long long n, double v, hipStream_t st) {
 extern "C" unsigned long long eco_noalloc_bytes(const eco_config *cfg) {
   if (validate_config(cfg) != ECO_OK) return 0;
   // configurations eco_run_noalloc refuses
-  if (cfg->precision != ECO_FP64 || cfg->ntransport > 0 || cfg->model == ECO_MODEL_THERMALEQ) return 0;
+  if (cfg->precision != ECO_FP64 || cfg->ntransport > 0 || cfg->model == ECO_MODEL_THERMALEQ ||
+      cfg->model == ECO_MODEL_EULERHOM || cfg->relax_ptmu) return 0;
   const eco_config &c = *cfg;
   int dim = (c.n[0] > 1) + (c.n[1] > 1) + (c.n[2] > 1);
   const size_t nc = (size_t)c.n[0] * c.n[1] * c.n[2];
@@ -132,7 +133,7 @@ extern "C" int eco_run_noalloc(const eco_config *cfg, const double *state_in, do
   // transported scalars need a transport pass and face sM arrays: contexts only
   if (cfg->ntransport > 0) return ECO_ERR_UNSUPPORTED;
   // the thermal-equilibrium model keeps its own state layout: contexts only
-  if (cfg->model == ECO_MODEL_THERMALEQ) return ECO_ERR_UNSUPPORTED;
+  if (cfg->model == ECO_MODEL_THERMALEQ || cfg->model == ECO_MODEL_EULERHOM || cfg->relax_ptmu) return ECO_ERR_UNSUPPORTED;
   if (nsteps <= 0 && !(t_final > 0.)) return ECO_ERR_INVALID;
   if (nsteps == INT_MIN) return ECO_ERR_INVALID;
   const unsigned long long need = eco_noalloc_bytes(cfg);

@@ -5,7 +5,12 @@
 
 namespace eco {
 
-enum ModelKind : int { MODEL_EULER = 0, MODEL_KAPILA = 1, MODEL_MULTIP = 2, MODEL_THERMALEQ = 3 };
+enum ModelKind : int { MODEL_EULER = 0, MODEL_KAPILA = 1, MODEL_MULTIP = 2, MODEL_THERMALEQ = 3, MODEL_EULERHOM = 4 };
+// ThermalEq and EulerHomogeneous store the same fields (alpha_k, the mixture
+// pressure and temperature, velocity; the phases at that pressure and
+// temperature) and share the flux layout (per-phase masses: EulerHomogeneous
+// carries its one mass in phase 0's slot, the others stay 0)
+__host__ __device__ constexpr bool te_like(int m) { return m == MODEL_THERMALEQ || m == MODEL_EULERHOM; }
 enum LimiterKind : int { LIM_NONE = 0, LIM_MINMOD = 1, LIM_VANLEER = 2, LIM_VANALBADA = 3,
                          LIM_SUPERBEE = 4, LIM_MC = 5, LIM_THINC = 6 };
 enum EosKind : int { EOS_IG = 0, EOS_SG = 1 };
@@ -84,6 +89,13 @@ struct SchemeParams {
   double tank_Y[6][kMaxPhases];
   double tank_H0[6];
   EosParams eos[kMaxPhases];
+  // liquid-vapour saturation (EulerHomogeneous, RelaxationPTMu): the liquid
+  // and vapour phases and the coefficients A, B, C, D of the saturation
+  // temperature's Gibbs-energy equality (Mixture::computeTsat,
+  // Mixture.cpp:81-85), computed on the host with the reference's expressions
+  int relax_ptmu;    // RelaxationPTMu after the pressure relaxation (Kapila, two phases)
+  int liq, vap;
+  double tsat[4];
 };
 
 // Device-resident control block of one solve (one per context / pool case).

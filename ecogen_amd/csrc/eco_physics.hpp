@@ -196,8 +196,9 @@ ECO_D void verify_correct(const SchemeParams &S, CellState<NPH> &s, int k) {
     if (s.a[k] < 1e-10) s.a[k] = 1e-9;
     if (s.a[k] > 1. - 1e-10) s.a[k] = 1. - 1e-9;
     if (s.r[k] < 1.e-10) s.r[k] = 1.e-9;
-  } else if constexpr (MODEL == MODEL_THERMALEQ) {
-    // PhaseThermalEq::verifyAndCorrectPhase (PhaseThermalEq.cpp:240-246)
+  } else if constexpr (te_like(MODEL)) {
+    // PhaseThermalEq::verifyAndCorrectPhase (PhaseThermalEq.cpp:240-246),
+    // PhaseEulerHomogeneous::verifyAndCorrectPhase (PhaseEulerHomogeneous.cpp:189-195)
     if (s.a[k] < 1e-10) s.a[k] = 1e-10;
     if (s.a[k] > 1. - 1e-10) s.a[k] = 1. - 1e-10;
     if (s.r[k] < 1.e-10) s.r[k] = 1.e-10;
@@ -243,13 +244,88 @@ ECO_D void te_mixture(CellState<NPH> &s) {
   s.cF = sqrt(s.cF);
 }
 
+// ----------------------------------------------------------------------------
+// Liquid-vapour saturation (EulerHomogeneous, RelaxationPTMu)
+// ----------------------------------------------------------------------------
+// Mixture::computeTsat (Mixture.cpp:63-104): the saturation temperature of
+// the liquid and vapour phases S.liq, S.vap at pressure p, Newton from
+// 0.1 B / C on f = A + B/T + C log T - log(p + pinf_vap) + D log(p + pinf_liq)
+// (the logarithms as the C library's, log_lib); dTsat (may be null)
+// receives dTsat/dp.  A soft error of the reference at 50 iterations is not
+// reported; the iterate then is the reference's too.
+ECO_D double sat_tsat(const SchemeParams &S, double p, double *dTsat) {
+  const double pInfL = te_pinf(S.eos[S.liq]), pInfV = te_pinf(S.eos[S.vap]);
+  const double A = S.tsat[0], B = S.tsat[1], C = S.tsat[2], D = S.tsat[3];
+  const double lV = log_lib(p + pInfV), lL = log_lib(p + pInfL);
+  double T = 0.1 * B / C, f = 0., df = 1.;
+  int it = 0;
+  do {
+    T -= f / df;
+    if (++it > 50) break;
+    f = A + B / T + C * log_lib(T) - lV + D * lL;
+    df = C / T - B / (T * T);
+  } while (fabs(f) > 1e-10);
+  if (dTsat) {
+    const double dfdp = -1. / (p + pInfV) + D / (p + pInfL);
+    *dTsat = -dfdp / df;
+  }
+  return T;
+}
+
+// EosSG/EosIG::computeDensitySaturation (EosSG.cpp:191-200, EosIG.cpp:188-197)
+// with its derivative in p (drho may be null)
+ECO_D double eos_rho_sat(const EosParams &q, double p, double T, double dT, double *drho, double eps) {
+  const double g = q.gamma, cv = q.cv;
+  const double pp = (q.type == EOS_SG) ? p + q.pinf : p;
+  if (drho) {
+    *drho = (g - 1.) * cv * T - pp * (g - 1.) * cv * dT;
+    *drho /= smax((((g - 1.) * cv * T) * ((g - 1.) * cv * T)), eps);
+  }
+  return pp / smax(((g - 1.) * cv * T), eps);
+}
+
+// EosSG/EosIG::computeDensityEnergySaturation (EosSG.cpp:204-210, EosIG.cpp:201-207)
+ECO_D double eos_rhoe_sat(const EosParams &q, double p, double rho, double drho, double *drhoe) {
+  if (drhoe) *drhoe = 1. / (q.gamma - 1.) + drho * q.eref;
+  if (q.type == EOS_SG) return (p + q.gamma * q.pinf) / (q.gamma - 1.) + rho * q.eref;
+  return p / (q.gamma - 1.) + rho * q.eref;
+}
+
+// MixEulerHomogeneous::computePressure (MixEulerHomogeneous.cpp:93-125): the
+// pressure at which the saturated mixture of density masse holds the internal
+// energy e, Newton from p_guess (the cell's last pressure)
+ECO_D double eh_pressure(const SchemeParams &S, double masse, double e, double p_guess) {
+  const EosParams &qL = S.eos[S.liq], &qV = S.eos[S.vap];
+  double p = p_guess, f = 0., df = 1.;
+  int it = 0;
+  do {
+    p -= f / df;
+    if (++it > 50) break;
+    double dT, drV, drL, dreV, dreL;
+    const double T = sat_tsat(S, p, &dT);
+    const double rV = eos_rho_sat(qV, p, T, dT, &drV, S.eps);
+    const double rL = eos_rho_sat(qL, p, T, dT, &drL, S.eps);
+    const double aV = (masse - rL) / (rV - rL);
+    const double daV = (-drL * (rV - rL) - (masse - rL) * (drV - drL)) / ((rV - rL) * (rV - rL));
+    const double reV = eos_rhoe_sat(qV, p, rV, drV, &dreV);
+    const double reL = eos_rhoe_sat(qL, p, rL, drL, &dreL);
+    f = masse * e - aV * (reV - reL) - reL;
+    df = -daV * (reV - reL) - aV * (dreV - dreL) - dreL;
+  } while (fabs(f / (masse * e)) > 1e-10);
+  return p;
+}
+
 // Model::fulfillState: extendedCalculusPhase (energy, sound speed) and
 // Mix*::computeMixtureVariables (MixKapila.cpp:147-172, MixMultiP.cpp:142-168).
 // Temperatures and Wood's sound speed are not used by the step and skipped.
 template <int MODEL, int NPH>
 ECO_D void fulfill(const SchemeParams &S, CellState<NPH> &s) {
   const double eps = S.eps;
-  if constexpr (MODEL == MODEL_THERMALEQ) {
+  // EulerHomogeneous: the stored temperature is the saturation temperature of
+  // the stored pressure (buildPrim and the initial state set both), so its
+  // phases' saturated densities (p + pinf)/((gamma-1) cv Tsat) are those at
+  // (p, T) of ThermalEq (ModEulerHomogeneous::fulfillState, ModEulerHomogeneous.cpp:42-56)
+  if constexpr (te_like(MODEL)) {
     te_phases<NPH>(S, s);
     te_mixture<NPH>(s);
   } else if constexpr (MODEL == MODEL_EULER) {
@@ -281,7 +357,24 @@ ECO_D void fulfill(const SchemeParams &S, CellState<NPH> &s) {
 // Needs s.rho (mixture density) for the multiphase models.
 template <int MODEL, int NPH>
 ECO_D void build_cons(const SchemeParams &S, const CellState<NPH> &s, FluxVec<NPH> &f) {
-  if constexpr (MODEL == MODEL_THERMALEQ) {
+  if constexpr (MODEL == MODEL_EULERHOM) {
+    // FluxEulerHomogeneous::buildCons (FluxEulerHomogeneous.cpp:70-89): the
+    // mixture mass, momentum and total energy, the internal energy summed from
+    // the phases' energies at their densities and pressures
+    double ein = 0.;
+    f.m[0] = 0.;
+#pragma unroll
+    for (int k = 0; k < NPH; k++) {
+      const double ar = s.a[k] * s.r[k];
+      f.m[0] += ar;
+      ein += ar * eos_e(S.eos[k], s.r[k], s.p[k], S.eps);
+    }
+#pragma unroll
+    for (int k = 1; k < NPH; k++) f.m[k] = 0.;
+#pragma unroll
+    for (int d = 0; d < 3; d++) f.q[d] = f.m[0] * s.u[d];
+    f.E = ein + 0.5 * f.m[0] * sqn(s.u);
+  } else if constexpr (MODEL == MODEL_THERMALEQ) {
     // FluxThermalEq::buildCons (FluxThermalEq.cpp:113-126)
 #pragma unroll
     for (int k = 0; k < NPH; k++) f.m[k] = s.a[k] * s.r[k];
@@ -318,7 +411,37 @@ ECO_D void zero_small(double *u) {
 template <int MODEL, int NPH>
 ECO_D void build_prim(const SchemeParams &S, FluxVec<NPH> &f, CellState<NPH> &s) {
   const double eps = S.eps;
-  if constexpr (MODEL == MODEL_THERMALEQ) {
+  if constexpr (MODEL == MODEL_EULERHOM) {
+    // FluxEulerHomogeneous::buildPrim (FluxEulerHomogeneous.cpp:93-128): the
+    // saturated mixture's pressure (Newton from s.pm, the cell's last
+    // pressure), its saturation temperature, the saturated phase densities and
+    // the vapour fraction of the mixture density
+    if constexpr (NPH == 2) {
+      const double masse = f.m[0], irm = 1. / masse;
+#pragma unroll
+      for (int d = 0; d < 3; d++) s.u[d] = qdiv(f.q[d], masse, irm);
+      zero_small(s.u);
+      const double ein = qdiv(f.E, masse, irm) - 0.5 * sqn(s.u);
+      const double p = eh_pressure(S, masse, ein, s.pm);
+      const double T = sat_tsat(S, p, nullptr);
+      const double r0 = eos_rho_sat(S.eos[0], p, T, T, nullptr, eps), r1 = eos_rho_sat(S.eos[1], p, T, T, nullptr, eps);
+      const double rl = S.liq ? r1 : r0, rv = S.liq ? r0 : r1;
+      const double av = (masse - rl) / (rv - rl), al = 1. - av;
+      s.r[0] = r0; s.r[1] = r1;
+      s.a[0] = S.liq ? av : al;
+      s.a[1] = S.liq ? al : av;
+      s.pm = p;
+      s.T = T;
+#pragma unroll
+      for (int k = 0; k < NPH; k++) {
+        s.p[k] = p;
+        s.e[k] = eos_e(S.eos[k], s.r[k], p, eps);
+        s.c[k] = eos_c(S.eos[k], s.r[k], p, eps);
+      }
+      te_mixture<NPH>(s);
+      s.E = qdiv(f.E, masse, irm);
+    }
+  } else if constexpr (MODEL == MODEL_THERMALEQ) {
     // FluxThermalEq::buildPrim (FluxThermalEq.cpp:130-162) with the
     // closed-form equilibrium pressure and temperature of two phases
     // (MixThermalEq::computePressure / computeTemperature, MixThermalEq.cpp:
@@ -440,7 +563,7 @@ template <int MODEL, int NPH>
 ECO_D void scale_add(FluxVec<NPH> &U, double mult, const FluxVec<NPH> &buf) {
   if constexpr (MODEL == MODEL_EULER) {
     U.m[0] *= mult;
-  } else if constexpr (MODEL == MODEL_THERMALEQ) {
+  } else if constexpr (te_like(MODEL)) {
 #pragma unroll
     for (int k = 0; k < NPH; k++) U.m[k] *= mult;
   } else {
@@ -452,7 +575,7 @@ ECO_D void scale_add(FluxVec<NPH> &U, double mult, const FluxVec<NPH> &buf) {
   U.E *= mult;
   if constexpr (MODEL == MODEL_EULER) {
     U.m[0] += 1. * buf.m[0];
-  } else if constexpr (MODEL == MODEL_THERMALEQ) {
+  } else if constexpr (te_like(MODEL)) {
 #pragma unroll
     for (int k = 0; k < NPH; k++) U.m[k] += 1. * buf.m[k];
   } else {
@@ -542,6 +665,57 @@ ECO_D void relax_pt(const SchemeParams &S, CellState<NPH> &s) {
       s.p[k] = pStar;
     }
     s.pm = pStar;
+  }
+}
+
+// RelaxationPTMu::stiffRelaxation (RelaxationPTMu.cpp:51-157), two phases
+// (phase 0 the liquid, phase 1 the vapour): from sum alpha_k p_k of s (whose
+// phases are verified and corrected), Newton on the pressure at which the
+// saturated mixture of density rho holds the internal energy rho*e.  The
+// density and energy are the cell's own mixture's and the result goes to the
+// cell's own phases (RelaxationPTMu.cpp:47-48, 138-149, no Prim type): out
+// is s itself after a time-n update, the time-n state in the predictor.
+template <int MODEL, int NPH>
+ECO_D void relax_ptmu(const SchemeParams &S, CellState<NPH> &s, CellState<NPH> &out, double rho, double e) {
+  if constexpr (NPH == 2 && MODEL == MODEL_KAPILA) {
+    double pStar = 0.;
+#pragma unroll
+    for (int k = 0; k < NPH; k++) {
+      pStar += s.a[k] * s.p[k];
+      verify_correct<MODEL, NPH>(S, s, k);
+    }
+    const double rhoe = rho * e;
+    const EosParams &qL = S.eos[0], &qV = S.eos[1];
+    double rL = 0., rV = 0., aL = 0., aV = 0.;
+    double f = 0., df = 1.;
+    int it = 0;
+    do {
+      pStar -= f / df;
+      if (++it > 50) break;
+#pragma unroll
+      for (int k = 0; k < NPH; k++) pStar = eos_fix_p(S.eos[k], pStar);
+      double dT, drL, drV, dreL, dreV;
+      const double T = sat_tsat(S, pStar, &dT);
+      rL = eos_rho_sat(qL, pStar, T, dT, &drL, S.eps);
+      rV = eos_rho_sat(qV, pStar, T, dT, &drV, S.eps);
+      if (rL <= rho) { rL = rho + 1e-6; drL = 0.; }
+      if (rV >= rho) { rV = rho - 1e-6; drV = 0.; }
+      aL = (rho - rV) / (rL - rV);
+      const double daL = (-drV * (rL - rV) - (rho - rV) * (drL - drV)) / ((rL - rV) * (rL - rV));
+      if (aL <= 0.) aL = 1e-8;
+      if (aL >= 1.) aL = 1. - 1e-8;
+      aV = 1. - aL;
+      const double daV = -daL;
+      f = rhoe; df = 0.;
+      const double reL = eos_rhoe_sat(qL, pStar, rL, drL, &dreL);
+      const double reV = eos_rhoe_sat(qV, pStar, rV, drV, &dreV);
+      f -= (aL * reL + aV * reV);
+      df -= (daL * reL + aL * dreL + daV * reV + aV * dreV);
+      f /= rhoe;
+      df /= rhoe;
+    } while (fabs(f) > 1e-10);
+    out.a[0] = aL; out.r[0] = rL; out.p[0] = pStar;
+    out.a[1] = aV; out.r[1] = rV; out.p[1] = pStar;
   }
 }
 
@@ -1060,7 +1234,47 @@ ECO_D double hllc_te(const CellState<NPH> &L, const CellState<NPH> &R, FluxVec<N
 template <int MODEL, int NPH>
 ECO_D double hllc(const SchemeParams &S, const CellState<NPH> &L, const CellState<NPH> &R, FluxVec<NPH> &F) {
   const double eps = S.eps;
-  if constexpr (MODEL == MODEL_THERMALEQ) {
+  if constexpr (MODEL == MODEL_EULERHOM) {
+    // ModEulerHomogeneous::solveRiemannIntern (ModEulerHomogeneous.cpp:60-136):
+    // HLLC on the mixture (its energy plus |u|^2/2 and the sound speed
+    // sqrt(sum Y_k c_k^2)), strict supersonic branches; the mass flux in
+    // phase 0's slot
+    (void)eps;
+    const double uL = L.u[0], vL = L.u[1], wL = L.u[2], cL = L.cF, pL = L.pm, rhoL = L.rho;
+    const double uR = R.u[0], vR = R.u[1], wR = R.u[2], cR = R.cF, pR = R.pm, rhoR = R.rho;
+    const double EL = L.em + 0.5 * sqn(L.u), ER = R.em + 0.5 * sqn(R.u);
+    const double sL = smin(uL - cL, uR - cR);
+    const double sR = smax(uR + cR, uL + cL);
+    double smx = 0.;
+    if (fabs(sL) > 1.e-3) smx = fabs(sL);
+    if (fabs(sR) > 1.e-3) smx = smax(smx, fabs(sR));
+    const double mL = rhoL * (sL - uL), mR = rhoR * (sR - uR);
+    double sM = (pR - pL + mL * uL - mR * uR) / (mL - mR);
+    if (fabs(sM) < 1.e-8) sM = 0.;
+#pragma unroll
+    for (int k = 0; k < NPH; k++) { F.a[k] = 0.; F.m[k] = 0.; F.en[k] = 0.; }
+    if (sL > 0.) {
+      F.m[0] = rhoL * uL;
+      F.q[0] = rhoL * uL * uL + pL; F.q[1] = rhoL * vL * uL; F.q[2] = rhoL * wL * uL;
+      F.E = (rhoL * EL + pL) * uL;
+    } else if (sR < 0.) {
+      F.m[0] = rhoR * uR;
+      F.q[0] = rhoR * uR * uR + pR; F.q[1] = rhoR * vR * uR; F.q[2] = rhoR * wR * uR;
+      F.E = (rhoR * ER + pR) * uR;
+    } else {
+      const bool left = (sM >= 0.);
+      const double mS = left ? mL : mR, uS = left ? uL : uR, pS = left ? pL : pR, sS = left ? sL : sR;
+      const double ES = left ? EL : ER, vS = left ? vL : vR, wS = left ? wL : wR;
+      const double pStar = mS * (sM - uS) + pS;
+      const double rhoStar = mS / (sS - sM);
+      const double Estar = ES + (sM - uS) * (sM + pS / mS);
+      F.m[0] = rhoStar * sM;
+      F.q[0] = rhoStar * sM * sM + pStar; F.q[1] = rhoStar * sM * vS; F.q[2] = rhoStar * sM * wS;
+      F.E = (rhoStar * Estar + pStar) * sM;
+    }
+    F.sM = sM;
+    return smx;
+  } else if constexpr (MODEL == MODEL_THERMALEQ) {
     (void)eps;
     return hllc_te<NPH>(L, R, F);
   } else if constexpr (MODEL == MODEL_EULER) {

@@ -305,7 +305,8 @@ extern "C" int eco_pool_admit(eco_pool *p, const eco_config *cfg, const double *
   if (cfg->precision == ECO_FP32S && cfg->order != 2) return ECO_ERR_UNSUPPORTED;
   // transported scalars run in single-solve contexts (their own pass per stage)
   if (cfg->ntransport > 0) return ECO_ERR_UNSUPPORTED;
-  if (cfg->model == ECO_MODEL_THERMALEQ) return ECO_ERR_UNSUPPORTED;
+  // ThermalEq, EulerHomogeneous and RelaxationPTMu run in single-solve contexts
+  if (cfg->model == ECO_MODEL_THERMALEQ || cfg->model == ECO_MODEL_EULERHOM || cfg->relax_ptmu) return ECO_ERR_UNSUPPORTED;
   if (p->free_ids.empty()) return ECO_ERR_FULL;
   Lowered L;
   lower_config(*cfg, L);

@@ -30,7 +30,7 @@ namespace eco {
 // p, velocity; ThermalEq: per phase alpha, the mixture p and T, velocity).
 template <int MODEL, int NPH, int DIM>
 struct Layout {
-  static constexpr bool TE = (MODEL == MODEL_THERMALEQ);
+  static constexpr bool TE = te_like(MODEL);
   static constexpr int NPRIM = (MODEL == MODEL_EULER) ? 2 + DIM : TE ? NPH + 2 + DIM : 3 * NPH + DIM;
   static constexpr int IE = NPRIM;    // total energy field
   static constexpr int NVAR = NPRIM;  // slope variables
@@ -105,7 +105,7 @@ __device__ __forceinline__ void load_prim(PW W, long long nc, long long idx, Cel
 #pragma unroll
     for (int d = 0; d < 3; d++) s.u[d] = (d < DIM) ? W[(2 + d) * nc + idx] : 0.;
     s.a[0] = 1.;
-  } else if constexpr (MODEL == MODEL_THERMALEQ) {
+  } else if constexpr (te_like(MODEL)) {
 #pragma unroll
     for (int k = 0; k < NPH; k++) s.a[k] = W[k * nc + idx];
     s.pm = W[NPH * nc + idx];
@@ -132,7 +132,7 @@ __device__ __forceinline__ void store_prim(PW W, long long nc, long long idx, co
     W[nc + idx] = s.p[0];
 #pragma unroll
     for (int d = 0; d < DIM; d++) W[(2 + d) * nc + idx] = s.u[d];
-  } else if constexpr (MODEL == MODEL_THERMALEQ) {
+  } else if constexpr (te_like(MODEL)) {
 #pragma unroll
     for (int k = 0; k < NPH; k++) W[k * nc + idx] = s.a[k];
     W[NPH * nc + idx] = s.pm;
@@ -161,7 +161,7 @@ __device__ __forceinline__ void prim_to_vec(const CellState<NPH> &s, double *w) 
     w[0] = s.r[0]; w[1] = s.p[0];
 #pragma unroll
     for (int d = 0; d < DIM; d++) w[2 + d] = s.u[d];
-  } else if constexpr (MODEL == MODEL_THERMALEQ) {
+  } else if constexpr (te_like(MODEL)) {
 #pragma unroll
     for (int k = 0; k < NPH; k++) w[k] = s.a[k];
     w[NPH] = s.pm; w[NPH + 1] = s.T;
@@ -181,7 +181,7 @@ __device__ __forceinline__ void vec_to_prim(const double *w, CellState<NPH> &s) 
     s.r[0] = w[0]; s.p[0] = w[1];
 #pragma unroll
     for (int d = 0; d < DIM; d++) s.u[d] = w[2 + d];
-  } else if constexpr (MODEL == MODEL_THERMALEQ) {
+  } else if constexpr (te_like(MODEL)) {
 #pragma unroll
     for (int k = 0; k < NPH; k++) s.a[k] = w[k];
     s.pm = w[NPH]; s.T = w[NPH + 1];
@@ -227,7 +227,7 @@ __device__ __forceinline__ void put_flux(double (*L)[MAXF], int f, const FluxVec
   int q = 0;
   if constexpr (MODEL == MODEL_EULER) {
     L[q++][f] = F.m[0];
-  } else if constexpr (MODEL == MODEL_THERMALEQ) {
+  } else if constexpr (te_like(MODEL)) {
 #pragma unroll
     for (int k = 0; k < NPH; k++) L[q++][f] = F.m[k];
   } else {
@@ -245,7 +245,7 @@ __device__ __forceinline__ void get_flux(double (*L)[MAXF], int f, FluxVec<NPH> 
   int q = 0;
   if constexpr (MODEL == MODEL_EULER) {
     F.m[0] = L[q++][f];
-  } else if constexpr (MODEL == MODEL_THERMALEQ) {
+  } else if constexpr (te_like(MODEL)) {
 #pragma unroll
     for (int k = 0; k < NPH; k++) F.m[k] = L[q++][f];
   } else {
@@ -267,7 +267,7 @@ __device__ __forceinline__ void add_face(FluxVec<NPH> &U, const FluxVec<NPH> &F,
   if (add) {
     if constexpr (MODEL == MODEL_EULER) {
       U.m[0] += coefA * F.m[0];
-    } else if constexpr (MODEL == MODEL_THERMALEQ) {
+    } else if constexpr (te_like(MODEL)) {
       // FluxThermalEq::addFlux (FluxThermalEq.cpp:62-69): no non-conservative terms
 #pragma unroll
       for (int k = 0; k < NPH; k++) U.m[k] += coefA * F.m[k];
@@ -278,7 +278,7 @@ __device__ __forceinline__ void add_face(FluxVec<NPH> &U, const FluxVec<NPH> &F,
 #pragma unroll
     for (int d = 0; d < 3; d++) U.q[d] += F.q[d] * coefA;
     U.E += coefA * F.E;
-    if constexpr (MODEL != MODEL_EULER && MODEL != MODEL_THERMALEQ) {
+    if constexpr (MODEL != MODEL_EULER && !te_like(MODEL)) {
 #pragma unroll
       for (int k = 0; k < NPH; k++) {
         U.a[k] += -coefA * own.a[k] * F.sM;
@@ -288,7 +288,7 @@ __device__ __forceinline__ void add_face(FluxVec<NPH> &U, const FluxVec<NPH> &F,
   } else {
     if constexpr (MODEL == MODEL_EULER) {
       U.m[0] -= coefA * F.m[0];
-    } else if constexpr (MODEL == MODEL_THERMALEQ) {
+    } else if constexpr (te_like(MODEL)) {
 #pragma unroll
       for (int k = 0; k < NPH; k++) U.m[k] -= coefA * F.m[k];
     } else {
@@ -298,7 +298,7 @@ __device__ __forceinline__ void add_face(FluxVec<NPH> &U, const FluxVec<NPH> &F,
 #pragma unroll
     for (int d = 0; d < 3; d++) U.q[d] -= F.q[d] * coefA;
     U.E -= coefA * F.E;
-    if constexpr (MODEL != MODEL_EULER && MODEL != MODEL_THERMALEQ) {
+    if constexpr (MODEL != MODEL_EULER && !te_like(MODEL)) {
 #pragma unroll
       for (int k = 0; k < NPH; k++) {
         U.a[k] -= -coefA * own.a[k] * F.sM;
@@ -323,7 +323,7 @@ __device__ __forceinline__ void park_acc(double (*A)[NT], int t, const FluxVec<N
   int q = 0;
   if constexpr (MODEL == MODEL_EULER) {
     A[q++][t] = U.m[0];
-  } else if constexpr (MODEL == MODEL_THERMALEQ) {
+  } else if constexpr (te_like(MODEL)) {
 #pragma unroll
     for (int k = 0; k < NPH; k++) A[q++][t] = U.m[k];
   } else {
@@ -341,7 +341,7 @@ __device__ __forceinline__ void unpark_acc(double (*A)[NT], int t, FluxVec<NPH> 
   zero_flux(U);
   if constexpr (MODEL == MODEL_EULER) {
     U.m[0] = A[q++][t];
-  } else if constexpr (MODEL == MODEL_THERMALEQ) {
+  } else if constexpr (te_like(MODEL)) {
 #pragma unroll
     for (int k = 0; k < NPH; k++) U.m[k] = A[q++][t];
   } else {
@@ -379,7 +379,7 @@ __device__ __forceinline__ void load_cache(PW W, long long nc, int ie, long long
 // time-n alpha_k, p_k of one cell: the non-conservative terms' coefficients
 template <int MODEL, int NPH, int DIM, typename PW>
 __device__ __forceinline__ void load_nc_coeffs(PW W, long long nc, long long idx, CellState<NPH> &s) {
-  if constexpr (MODEL != MODEL_EULER && MODEL != MODEL_THERMALEQ) {
+  if constexpr (MODEL != MODEL_EULER && !te_like(MODEL)) {
 #pragma unroll
     for (int k = 0; k < NPH; k++) { s.a[k] = W[(3 * k) * nc + idx]; s.p[k] = W[(3 * k + 2) * nc + idx]; }
   }
@@ -460,7 +460,7 @@ __device__ __forceinline__ void put_flux_s(double *L, int stride, int f, const F
   int q = 0;
   if constexpr (MODEL == MODEL_EULER) {
     L[(q++) * stride + f] = F.m[0];
-  } else if constexpr (MODEL == MODEL_THERMALEQ) {
+  } else if constexpr (te_like(MODEL)) {
 #pragma unroll
     for (int k = 0; k < NPH; k++) L[(q++) * stride + f] = F.m[k];
   } else {
@@ -480,7 +480,7 @@ __device__ __forceinline__ void get_flux_s(const double *L, int stride, int f, F
   int q = 0;
   if constexpr (MODEL == MODEL_EULER) {
     F.m[0] = L[(q++) * stride + f];
-  } else if constexpr (MODEL == MODEL_THERMALEQ) {
+  } else if constexpr (te_like(MODEL)) {
 #pragma unroll
     for (int k = 0; k < NPH; k++) F.m[k] = L[(q++) * stride + f];
   } else {
@@ -717,7 +717,7 @@ __device__ __forceinline__ double face_fp64(const StageArgs &A, const int *o, co
           // MultiP keeps the mixture energy of totalEnergyToInternalEnergy (Run.cpp:471)
           L.em = Wsrc[Ly::IE * nc + ilo] - 0.5 * sqn(L.u);
           R.em = Wsrc[Ly::IE * nc + ihi] - 0.5 * sqn(R.u);
-        } else if constexpr (MODEL == MODEL_THERMALEQ) {
+        } else if constexpr (te_like(MODEL)) {
           // the cells' cached mixture energy: sum Y_k e_k of the initial
           // state, then E - |u|^2/2 of Cell::correctionEnergy (Cell.cpp:254-258)
           fulfill<MODEL, NPH>(S, L);
@@ -1185,7 +1185,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int slot, c
 
     // cell update
     CellState<NPH> sn = own;
-    if constexpr (MODEL == MODEL_THERMALEQ) {
+    if constexpr (te_like(MODEL)) {
       // phase densities at (p, T) and the mixture density of the last buildPrim
       fulfill<MODEL, NPH>(S, sn);
     } else if constexpr (MODEL != MODEL_EULER) {
@@ -1200,11 +1200,32 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int slot, c
     scale_add<MODEL, NPH>(acc, (STAGE == 1) ? 0.5 * dt : dt, buf);
     scheme_correction<MODEL, NPH>(acc, buf);
     CellState<NPH> s1;
+    // EulerHomogeneous: buildPrim's pressure Newton starts at the cell's last
+    // pressure (MixEulerHomogeneous.cpp:103)
+    if constexpr (MODEL == MODEL_EULERHOM) s1.pm = sn.pm;
     build_prim<MODEL, NPH>(S, acc, s1);
     bool ok = true;
     if constexpr (MODEL != MODEL_EULER) {
       if (S.relax) ok = relax_p<MODEL, NPH>(S, s1);
       if (S.relax_pt) relax_pt<MODEL, NPH>(S, s1);
+      if constexpr (MODEL == MODEL_KAPILA && NPH == 2) {
+        if (S.relax_ptmu) {
+          // RelaxationPTMu: from the cell's own mixture into the cell's own
+          // phases; the predictor thus relaxes the time-n state, which it
+          // writes for the corrector (A.wn2)
+          if constexpr (STAGE == 1) {
+            CellState<NPH> tn = own;
+            fulfill<MODEL, NPH>(S, tn);
+            relax_ptmu<MODEL, NPH>(S, s1, tn, tn.rho, tn.em);
+            ECO_GLOBAL double *__restrict__ Wn2 = gmem(A.wn2);
+            store_prim<MODEL, NPH, DIM>(Wn2, nc, cidx, tn);
+            Wn2[Ly::IE * nc + cidx] = own.E;
+          } else {
+            relax_ptmu<MODEL, NPH>(S, s1, s1, s1.rho, s1.em);
+            fulfill<MODEL, NPH>(S, s1);
+          }
+        }
+      }
       correction_energy<MODEL, NPH>(S, s1);
     }
     if (!ok) atomicOr(&ctl->err, 1u);
@@ -1222,7 +1243,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int slot, c
       // buildPrim's mixture density (the relaxation leaves it unchanged)
       if (MODEL == MODEL_MULTIP && S.ncache) store_cache<MODEL, NPH>(Wout, nc, Ly::IE, cidx, S.ncache, s1);
       // ThermalEq first order: the mixture energy after Cell::correctionEnergy
-      if (MODEL == MODEL_THERMALEQ && S.ncache) Wout[(Ly::IE + 1) * nc + cidx] = s1.E - 0.5 * sqn(s1.u);
+      if (te_like(MODEL) && S.ncache) Wout[(Ly::IE + 1) * nc + cidx] = s1.E - 0.5 * sqn(s1.u);
     }
   }
 
@@ -1280,13 +1301,18 @@ __global__ void k_init(const StageArgs A) {
   if (i >= nc) return;
   CellState<NPH> s;
   load_prim<MODEL, NPH, DIM>(A.wsrc, nc, i, s);
+  if constexpr (MODEL == MODEL_EULERHOM) {
+    // the saturation temperature of the initial pressure (ModEulerHomogeneous::fulfillState)
+    s.T = sat_tsat(A.sch, s.pm, nullptr);
+    A.wout[(NPH + 1) * nc + i] = s.T;
+  }
   fulfill<MODEL, NPH>(A.sch, s);
   if constexpr (MODEL != MODEL_EULER) s.E = s.em + 0.5 * sqn(s.u);
   A.wout[Ly::IE * nc + i] = s.E;
   if (MODEL == MODEL_MULTIP && A.sch.ncache) store_cache<MODEL, NPH>(A.wout, nc, Ly::IE, i, A.sch.ncache, s);
   // ThermalEq first order: the initial cells' mixture energy sum Y_k e_k
   // (Cell::completeFulfillState, Cell.cpp:271-278)
-  if (MODEL == MODEL_THERMALEQ && A.sch.ncache) A.wout[(Ly::IE + 1) * nc + i] = s.em;
+  if (te_like(MODEL) && A.sch.ncache) A.wout[(Ly::IE + 1) * nc + i] = s.em;
 }
 
 // ---------------------------------------------------------------------------
@@ -1329,6 +1355,36 @@ int launch_t(const StageArgs &A, int stage, hipStream_t st) {
     hipLaunchKernelGGL((k_init<MODEL, NPH, DIM>), dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, st, A);
   }
   return (int)hipGetLastError();
+}
+
+// First-order-only models (EulerHomogeneous: transmissive boundaries, first
+// order): the first-order stage and the initialisation
+template <int MODEL, int NPH, int DIM>
+int launch_t_fo(const StageArgs &A, int stage, hipStream_t st) {
+  constexpr int NT = Tile<DIM>::T;
+  const int tgx = (A.geo.n[0] + Tile<DIM>::X - 1) / Tile<DIM>::X;
+  const int tgy = (A.geo.n[1] + Tile<DIM>::Y - 1) / Tile<DIM>::Y;
+  const int tgz = (A.geo.n[2] + Tile<DIM>::Z - 1) / Tile<DIM>::Z;
+  const bool wall = (A.sch.bc_wall | A.sch.bc_out | A.sch.bc_inj | A.sch.bc_tank) != 0;
+  if (stage == 0 && !wall) {
+    hipLaunchKernelGGL((k_stage<MODEL, NPH, DIM, 0, true, false>), dim3((unsigned)(tgx * tgy * tgz)), dim3(NT), 0, st, A);
+  } else if (stage == 3) {
+    const long long nc = A.geo.ncell;
+    hipLaunchKernelGGL((k_init<MODEL, NPH, DIM>), dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, st, A);
+  } else {
+    return -1;
+  }
+  return (int)hipGetLastError();
+}
+
+template <int MODEL, int NPH>
+int launch_d_fo(const StageArgs &A, int stage, hipStream_t st) {
+  switch (A.geo.dim) {
+    case 1: return launch_t_fo<MODEL, NPH, 1>(A, stage, st);
+    case 2: return launch_t_fo<MODEL, NPH, 2>(A, stage, st);
+    case 3: return launch_t_fo<MODEL, NPH, 3>(A, stage, st);
+  }
+  return -1;
 }
 
 template <int MODEL, int NPH>

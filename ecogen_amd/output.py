@@ -122,6 +122,22 @@ def cell_data(case, state):
         out.append(("gradRho", 1, grad_rho(case, rho)))
         return out
     nph = len(eos)
+    if model == "eulerhomogeneous":
+        # PhaseEulerHomogeneous::returnNameScalar (Alpha, Density, Mass
+        # Fraction), MixEulerHomogeneous::returnNameScalar / returnNameVector
+        # (PhaseEulerHomogeneous.cpp:66-96, MixEulerHomogeneous.cpp:247-290)
+        rho_m = np.zeros(state.shape[1])
+        for k in range(nph):
+            rho_m = rho_m + state[3 * k] * state[3 * k + 1]
+        for k, e in enumerate(eos):
+            f = _eos_file(e)
+            out += [(f"F{k}_Alpha_{f}", 1, state[3 * k]), (f"F{k}_Density_{f}", 1, state[3 * k + 1]),
+                    (f"F{k}_Mass Fraction_{f}", 1, state[3 * k] * state[3 * k + 1] / rho_m)]
+        vel = np.stack([state[3 * nph], state[3 * nph + 1], state[3 * nph + 2]], axis=1)
+        out += [("Density_Mixture", 1, rho_m), ("Pressure_Mixture", 1, state[2]),
+                ("Temperature_Mixture", 1, state[3 * nph + 4]), ("Velocity_Mixture", 3, vel.reshape(-1))]
+        out.append(("gradRho", 1, grad_rho(case, rho_m)))
+        return out
     if model == "thermaleq":
         # PhaseThermalEq::returnNameScalar (Alpha, Density), MixThermalEq::
         # returnNameScalar / returnNameVector (PhaseThermalEq.cpp:82-104,
@@ -265,7 +281,7 @@ def write_gnu_script(path, case, nfiles, dim=None, stem="result"):
     if dim == 2:
         out += ["set surface", "set dgrid3d 50,50", "set contour base", "set cntrparam levels 25", "show contour"]
     index = 2 if dim == 0 else 1 + dim
-    nf = 3 * len(case["eos"]) + 4 + len(case.get("transports", [])) + (case["model"] == "thermaleq")
+    nf = 3 * len(case["eos"]) + 4 + len(case.get("transports", [])) + (case["model"] in ("thermaleq", "eulerhomogeneous"))
     for title, _ in _gnu_columns(case, np.ones((nf, int(np.prod(case["cells"]))))):
         out.append(f"set title '{title}'")
         if dim == 0:

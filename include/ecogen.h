@@ -30,15 +30,22 @@ extern "C" {
 #define ECO_NFIELDS(nph) (3 * (nph) + 4)
 #define ECO_NFIELDS_TR(nph, ntr) (3 * (nph) + 4 + (ntr))
 /* fields of a configuration's host state: transports after E, and for
-   ECO_MODEL_THERMALEQ the mixture temperature last */
-#define ECO_NFIELDS_CFG(model, nph, ntr) (3 * (nph) + 4 + (ntr) + ((model) == 3))
+   ECO_MODEL_THERMALEQ and ECO_MODEL_EULERHOM the mixture temperature last */
+#define ECO_NFIELDS_CFG(model, nph, ntr) (3 * (nph) + 4 + (ntr) + ((model) == 3 || (model) == 4))
 #define ECO_MAX_TRANSPORTS 4
 
 /* ECO_MODEL_THERMALEQ: two phases in pressure and temperature equilibrium
    (ModThermalEq, src/Models/ThermalEq); host state per phase alpha_k, rho_k,
    p_k (p_k the mixture pressure, rho_k = rho_k(p, T)), u, v, w, E, then the
    mixture temperature T: on input alpha_k, p_0, u and T are read */
-enum eco_model { ECO_MODEL_EULER = 0, ECO_MODEL_KAPILA = 1, ECO_MODEL_MULTIP = 2, ECO_MODEL_THERMALEQ = 3 };
+/* ECO_MODEL_EULERHOM: the homogeneous liquid-vapour model (ModEulerHomogeneous,
+   src/Models/EulerHomogeneous): one mass, momentum and energy equation, the
+   liquid (phase liq) and vapour (phase vap) at saturation; first order,
+   transmissive boundaries.  Host state as ECO_MODEL_THERMALEQ's: per phase
+   alpha_k, rho_k (the saturated density), p_k (the mixture pressure), u, v,
+   w, E, then the saturation temperature; on input alpha_k, p_0 and u are read */
+enum eco_model { ECO_MODEL_EULER = 0, ECO_MODEL_KAPILA = 1, ECO_MODEL_MULTIP = 2, ECO_MODEL_THERMALEQ = 3,
+                 ECO_MODEL_EULERHOM = 4 };
 enum eco_limiter { ECO_LIM_NONE = 0, ECO_LIM_MINMOD = 1, ECO_LIM_VANLEER = 2, ECO_LIM_VANALBADA = 3,
                    ECO_LIM_SUPERBEE = 4, ECO_LIM_MC = 5, ECO_LIM_THINC = 6 };
 enum eco_eos_type { ECO_EOS_IG = 0, ECO_EOS_SG = 1 };
@@ -86,7 +93,7 @@ typedef struct {
    from the deck (src/InputOutput/Input.cpp:85-625) for this path. */
 typedef struct {
   int model;        /* enum eco_model */
-  int nphase;       /* 1 for Euler; 2, or 3-4 in FP64 without THINC, for Kapila/MultiP; 2 for ThermalEq */
+  int nphase;       /* 1 for Euler; 2, or 3-4 in FP64 without THINC, for Kapila/MultiP; 2 for ThermalEq, EulerHomogeneous */
   int order;        /* 1 = FIRSTORDER, 2 = SECONDORDER (MUSCL predictor-corrector) */
   int lim[4];       /* global, interface, global volume-fraction, interface volume-fraction limiter */
   int alpha_null;   /* Kapila flowModel alphaNull attribute */
@@ -122,6 +129,17 @@ typedef struct {
      49-141) after the pressure relaxation: Kapila and MultiP, two phases,
      FP64 */
   int relax_pt;
+  /* <relaxation type="PTMu"> (RelaxationPTMu, src/Relaxations/
+     RelaxationPTMu.cpp:51-157) after Kapila's pressure relaxation: the two
+     phases (phase 0 the liquid, phase 1 the vapour) relaxed to saturation;
+     FP64 single solves without transports */
+  int relax_ptmu;
+  /* ECO_MODEL_EULERHOM: <flowModel liquid vapor>, the liquid and vapour phase
+     indices (0 and 1 in either order) */
+  int liq, vap;
+  /* each phase's EOS entropyRef (libEOS/*.xml), used by the saturation
+     temperature of EulerHomogeneous and RelaxationPTMu (Mixture::computeTsat) */
+  double sref[4];
 } eco_config;
 
 typedef struct eco_ctx eco_ctx;

@@ -29,33 +29,38 @@ class Eos {
   virtual double getPInf() const { return 0.; }
   virtual double getCv() const = 0;
   virtual double getERef() const = 0;
+  // entropyRef (the saturation temperature of Mixture::computeTsat)
+  virtual double getSRef() const { return 0.; }
 };
 
 // Ideal gas (src/Eos/EosIG.cpp): parameters gamma, cv, energyRef of libEOS/IG_*.xml
 class EosIG : public Eos {
  public:
-  EosIG(double gamma, double cv, double eref = 0.) : m_gamma(gamma), m_cv(cv), m_eRef(eref) {}
+  EosIG(double gamma, double cv, double eref = 0., double sref = 0.) : m_gamma(gamma), m_cv(cv), m_eRef(eref), m_sRef(sref) {}
   std::string getType() const override { return "IG"; }
   eco_eos lower() const override { return eco_eos{ECO_EOS_IG, m_gamma, 0., m_cv, m_eRef}; }
   double getGamma() const override { return m_gamma; }
   double getCv() const override { return m_cv; }
   double getERef() const override { return m_eRef; }
+  double getSRef() const override { return m_sRef; }
  private:
-  double m_gamma, m_cv, m_eRef;
+  double m_gamma, m_cv, m_eRef, m_sRef;
 };
 
 // Stiffened gas (src/Eos/EosSG.cpp): gamma, pInf, cv, energyRef of libEOS/SG_*.xml
 class EosSG : public Eos {
  public:
-  EosSG(double gamma, double pinf, double cv, double eref = 0.) : m_gamma(gamma), m_pInf(pinf), m_cv(cv), m_eRef(eref) {}
+  EosSG(double gamma, double pinf, double cv, double eref = 0., double sref = 0.)
+      : m_gamma(gamma), m_pInf(pinf), m_cv(cv), m_eRef(eref), m_sRef(sref) {}
   std::string getType() const override { return "SG"; }
   eco_eos lower() const override { return eco_eos{ECO_EOS_SG, m_gamma, m_pInf, m_cv, m_eRef}; }
   double getGamma() const override { return m_gamma; }
   double getPInf() const override { return m_pInf; }
   double getCv() const override { return m_cv; }
   double getERef() const override { return m_eRef; }
+  double getSRef() const override { return m_sRef; }
  private:
-  double m_gamma, m_pInf, m_cv, m_eRef;
+  double m_gamma, m_pInf, m_cv, m_eRef, m_sRef;
 };
 
 // --- Limiter (src/Ordre2/Limiter.h) -------------------------------------------
@@ -95,6 +100,13 @@ class RelaxationPT : public Relaxation {
  public:
   std::string getType() const override { return "PT"; }
 };
+// Relaxation to liquid-vapour saturation, two phases (RelaxationPTMu.cpp:51-157;
+// phase 0 the liquid, phase 1 the vapour, as its constructor sets), after
+// Kapila's pressure relaxation
+class RelaxationPTMu : public Relaxation {
+ public:
+  std::string getType() const override { return "PTMU"; }
+};
 
 // --- Flux (src/Models/Flux.h): conservative vector of one model ---------------
 class Flux {
@@ -112,6 +124,14 @@ class FluxKapila : public Flux {
   std::vector<std::string> variables(int numberPhases, int dim) const override;
 };
 class FluxMultiP : public Flux {
+ public:
+  std::vector<std::string> variables(int numberPhases, int dim) const override;
+};
+class FluxThermalEq : public Flux {
+ public:
+  std::vector<std::string> variables(int numberPhases, int dim) const override;
+};
+class FluxEulerHomogeneous : public Flux {
  public:
   std::vector<std::string> variables(int numberPhases, int dim) const override;
 };
@@ -168,6 +188,31 @@ class ModMultiP : public Model {
   const Flux &flux() const override { return m_flux; }
  private:
   FluxMultiP m_flux;
+};
+
+// Two phases in pressure and temperature equilibrium (ModThermalEq)
+class ModThermalEq : public Model {
+ public:
+  ModThermalEq() : Model(2) {}
+  std::string whoAmI() const override { return "THERMALEQ"; }
+  int kind() const override { return ECO_MODEL_THERMALEQ; }
+  const Flux &flux() const override { return m_flux; }
+ private:
+  FluxThermalEq m_flux;
+};
+// Homogeneous liquid-vapour mixture at saturation (ModEulerHomogeneous,
+// <flowModel name="EulerHomogeneous" liquid vapor>, Input.cpp:479-483); first order
+class ModEulerHomogeneous : public Model {
+ public:
+  ModEulerHomogeneous(int liquid, int vapor) : Model(2), m_liq(liquid), m_vap(vapor) {}
+  std::string whoAmI() const override { return "EULERHOMOGENEOUS"; }
+  int kind() const override { return ECO_MODEL_EULERHOM; }
+  const Flux &flux() const override { return m_flux; }
+  int getLiq() const { return m_liq; }
+  int getVap() const { return m_vap; }
+ private:
+  int m_liq, m_vap;
+  FluxEulerHomogeneous m_flux;
 };
 
 // --- Cartesian mesh and run control -------------------------------------------

@@ -23,11 +23,11 @@ def _cases():
     for n in golden_names():
         case, _ = load_golden(n)
         # boundaries other than transmissive, three phases, transports and the
-        # pressure-temperature relaxation run in the FP64 path only (the
-        # FP32-storage kernels refuse them)
+        # pressure-temperature and saturation relaxations run in the FP64 path
+        # only (the FP32-storage kernels refuse them)
         if (case["order"] == 2 and case["model"] in ("kapila", "multip") and len(case["eos"]) == 2
                 and set(case.get("bc", ["abs"])) <= {"abs"} and not case.get("transports")
-                and not case.get("relax_pt")):
+                and not case.get("relax_pt") and not case.get("relax_ptmu")):
             out.append(n)
     return out
 
