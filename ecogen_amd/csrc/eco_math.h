@@ -9,11 +9,24 @@
 
 #if defined(__HIPCC__)
 #define ECO_HD __host__ __device__ __forceinline__
+#define ECO_RARE __host__ __device__ inline __attribute__((noinline))
 #else
 #define ECO_HD inline
+#define ECO_RARE inline __attribute__((noinline))
 #endif
 
 namespace eco {
+
+// The build target's pow, exp and log for the arguments the table-driven
+// routines below leave to them (zero, subnormal, negative, infinite or NaN
+// bases, results beyond the exp tables' range), which the isentrope, THINC
+// and the saturation laws do not produce.  Kept out of line: inlined, the
+// target's pow alone is several hundred instructions at every call site,
+// which more than doubled the code of a stage kernel for a path that is
+// never taken.
+ECO_RARE double pow_rare(double x, double g) { return std::pow(x, g); }
+ECO_RARE double exp_rare(double x) { return std::exp(x); }
+ECO_RARE double log_rare(double x) { return std::log(x); }
 
 // a / b, correctly rounded, from y = RN(1/b): q = RN(a*y) is within one ulp
 // of a/b, the residual r = a - b*q is exact in one fma, and RN(q + r*y) is
@@ -59,6 +72,11 @@ ECO_POW_TABLE unsigned long long EXP[256] = {ECO_POW_EXP_WORDS};
 #undef ECO_POW_TABLE
 }  // namespace powtab
 
+// The tables as one block of doubles (a copy in fast memory, e.g. LDS, that
+// the routines below read in place of the constant tables when given):
+// LOG row-major at 0, the EXP words' bit patterns at POWTAB_EXP.
+constexpr int POWTAB_EXP = 128 * 3, POWTAB_WORDS = 128 * 3 + 256;
+
 ECO_HD double as_double(unsigned long long u) {
   double d;
   std::memcpy(&d, &u, 8);
@@ -76,11 +94,11 @@ ECO_HD unsigned long long as_bits(double d) {
 // Outside [2^-54, 512) in magnitude, tiny arguments give 1 + hi as the
 // library does and large ones fall back to the build target's pow(px, pg)
 // (or exp(hi) when px is 0), which the isentrope and THINC never reach.
-ECO_HD double exp_dd(double hi, double lo, double px = 0., double pg = 0.) {
+ECO_HD double exp_dd(double hi, double lo, double px = 0., double pg = 0., const double *tab = nullptr) {
   const unsigned top = (unsigned)(as_bits(hi) >> 52) & 0x7ffu;
   if (top - 0x3c9u >= 0x408u - 0x3c9u) {  // |hi| < 2^-54 or >= 512
     if (top < 0x3c9u) return 1.0 + hi;
-    return px != 0. ? std::pow(px, pg) : std::exp(hi);
+    return px != 0. ? pow_rare(px, pg) : exp_rare(hi);
   }
   constexpr double INVLN2N = 0x1.71547652b82fep0 * 128, SHIFT = 0x1.8p52;
   constexpr double NEGLN2HIN = -0x1.62e42fefa0000p-8, NEGLN2LON = -0x1.cf79abc9e3b3ap-47;
@@ -92,8 +110,8 @@ ECO_HD double exp_dd(double hi, double lo, double px = 0., double pg = 0.) {
   double rr = std::fma(jd, NEGLN2LON, std::fma(jd, NEGLN2HIN, hi));
   rr += lo;
   const unsigned idx = 2u * (unsigned)(ki & 127u);
-  const double tail = as_double(powtab::EXP[idx]);
-  const double scale = as_double(powtab::EXP[idx + 1] + (ki << 45));
+  const double tail = tab ? tab[POWTAB_EXP + idx] : as_double(powtab::EXP[idx]);
+  const double scale = as_double((tab ? as_bits(tab[POWTAB_EXP + idx + 1]) : powtab::EXP[idx + 1]) + (ki << 45));
   const double r2 = rr * rr;
   const double tp = std::fma(r2 * r2, std::fma(rr, C5, C4), std::fma(r2, std::fma(rr, C3, C2), tail + rr));
   return std::fma(scale, tp, scale);
@@ -104,14 +122,16 @@ ECO_HD double exp_lib(double x) { return exp_dd(x, 0.); }
 // log x as lhi + llo (lhi = RN(log x) but for rare hard cases, |llo| within
 // half an ulp of it) for a positive normal finite x with bits ix: the log of
 // pow_pos below.
-ECO_HD void log_hl(unsigned long long ix, double &lhi, double &llo) {
+ECO_HD void log_hl(unsigned long long ix, double &lhi, double &llo, const double *tab = nullptr) {
   constexpr unsigned long long OFF = 0x3fe6955500000000ull;
   const unsigned long long tmp = ix - OFF;
   const int i = (int)((tmp >> 45) & 127u);
   const int k = (int)((long long)tmp >> 52);
   const double z = as_double(ix - (tmp & (0xfffull << 52)));
   const double kd = (double)k;
-  const double invc = powtab::LOG[i][0], logc = powtab::LOG[i][1], logctail = powtab::LOG[i][2];
+  const double invc = tab ? tab[3 * i] : powtab::LOG[i][0];
+  const double logc = tab ? tab[3 * i + 1] : powtab::LOG[i][1];
+  const double logctail = tab ? tab[3 * i + 2] : powtab::LOG[i][2];
   constexpr double LN2HI = 0x1.62e42fefa3800p-1, LN2LO = 0x1.ef35793c76730p-45;
   constexpr double A0 = -0x1p-1, A1 = 0x1.555555555556p-2 * -2, A2 = -0x1.0000000000006p-2 * -2,
                    A3 = 0x1.999999959554ep-3 * 4, A4 = -0x1.555555529a47ap-3 * 4,
@@ -140,22 +160,22 @@ ECO_HD void log_hl(unsigned long long ix, double &lhi, double &llo) {
 // subnormal, negative, infinite and NaN arguments take the build target's log.
 ECO_HD double log_lib(double x) {
   const unsigned long long ix = as_bits(x);
-  if (!((ix >> 52) - 1u < 0x7feu)) return std::log(x);
+  if (!((ix >> 52) - 1u < 0x7feu)) return log_rare(x);
   double hi, lo;
   log_hl(ix, hi, lo);
   return hi;
 }
 
-ECO_HD double pow_pos(double x, double g) {
+ECO_HD double pow_pos(double x, double g, const double *tab = nullptr) {
   const unsigned long long ix = as_bits(x);
   // positive normal finite x only (top 12 bits 0x001..0x7fe)
-  if (!((ix >> 52) - 1u < 0x7feu)) return std::pow(x, g);
+  if (!((ix >> 52) - 1u < 0x7feu)) return pow_rare(x, g);
   double lhi, llo;
-  log_hl(ix, lhi, llo);
+  log_hl(ix, lhi, llo, tab);
   // y log x = ehi + elo
   const double ehi = g * lhi;
   const double elo = std::fma(g, llo, std::fma(g, lhi, -ehi));
-  return exp_dd(ehi, elo, x, g);
+  return exp_dd(ehi, elo, x, g, tab);
 }
 
 }  // namespace eco

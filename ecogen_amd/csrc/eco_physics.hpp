@@ -36,9 +36,11 @@ ECO_D double eos_c(const EosParams &q, double rho, double p, double eps) {
   if (q.type == EOS_SG) return sqrt(q.gamma * (p + q.pinf) / smax(rho, eps));
   return sqrt(q.gamma * p / smax(rho, eps));
 }
-ECO_D double eos_isentrope(const EosParams &q, double p0, double rho0, double rho1, double eps) {
-  if (q.type == EOS_SG) return (p0 + q.pinf) * pow_pos(rho1 / smax(rho0, eps), q.gamma) - q.pinf;
-  return p0 * pow_pos(rho1 / smax(rho0, eps), q.gamma);
+// tab: the power's tables in fast memory (pow_pos), or null for the constant ones
+ECO_D double eos_isentrope(const EosParams &q, double p0, double rho0, double rho1, double eps,
+                           const double *tab = nullptr) {
+  if (q.type == EOS_SG) return (p0 + q.pinf) * pow_pos(rho1 / smax(rho0, eps), q.gamma, tab) - q.pinf;
+  return p0 * pow_pos(rho1 / smax(rho0, eps), q.gamma, tab);
 }
 ECO_D double eos_rho_pfinal(const EosParams &q, double p0, double rho0, double pf, double &drho, double eps) {
   if (q.type == EOS_SG) {
@@ -67,12 +69,41 @@ ECO_D double eos_fix_p(const EosParams &q, double p) {
 // the limited slopes of all variables of a face schedule as one basic block.
 // The selected value is the reference's bit for bit.
 // a / b: IEEE division in FP64 (the reference's limiters); in FP32 (the
-// FP32-storage reconstruction, held to 1e-5 of FP64, not to the bit) a
-// reciprocal and a product
+// FP32-storage reconstruction, held to 1e-5 of FP64, not to the bit) the
+// hardware reciprocal (1 ulp) and a product -- two instructions where an IEEE
+// FP32 division takes about ten
 template <typename T>
 ECO_D T ldiv(T a, T b) {
-  if constexpr (sizeof(T) == sizeof(float)) return __fdividef(a, b);
+  if constexpr (sizeof(T) == sizeof(float)) return a * __builtin_amdgcn_rcpf(b);
   else return a / b;
+}
+
+// log1p and expm1 in FP32 for the isentropic power factor of the FP32-storage
+// HLLC.  For the small arguments of nearly every face (|x| <= 1/32, |z| <=
+// 1/4) their Taylor series to degree 7 by Horner's rule: the first omitted
+// term is below 2^-38 (log1p) and 2^-28 (expm1) relative to the result, under
+// FP32's rounding.  Larger arguments take the library's log1pf / expm1f.
+ECO_D float log1p_f(float x) {
+  if (!(fabsf(x) <= 0x1p-5f)) return log1pf(x);
+  float p = 1.f / 7.f;
+  p = p * x + (-1.f / 6.f);
+  p = p * x + 0.2f;
+  p = p * x + (-0.25f);
+  p = p * x + (1.f / 3.f);
+  p = p * x + (-0.5f);
+  p = p * x + 1.f;
+  return p * x;
+}
+ECO_D float expm1_f(float z) {
+  if (!(fabsf(z) <= 0.25f)) return expm1f(z);
+  float p = 1.f / 5040.f;
+  p = p * z + (1.f / 720.f);
+  p = p * z + (1.f / 120.f);
+  p = p * z + (1.f / 24.f);
+  p = p * z + (1.f / 6.f);
+  p = p * z + 0.5f;
+  p = p * z + 1.f;
+  return p * z;
 }
 
 template <int LIM, typename T>
@@ -1232,7 +1263,8 @@ ECO_D double hllc_te(const CellState<NPH> &L, const CellState<NPH> &R, FluxVec<N
 }
 
 template <int MODEL, int NPH>
-ECO_D double hllc(const SchemeParams &S, const CellState<NPH> &L, const CellState<NPH> &R, FluxVec<NPH> &F) {
+ECO_D double hllc(const SchemeParams &S, const CellState<NPH> &L, const CellState<NPH> &R, FluxVec<NPH> &F,
+                  const double *tab = nullptr) {
   const double eps = S.eps;
   if constexpr (MODEL == MODEL_EULERHOM) {
     // ModEulerHomogeneous::solveRiemannIntern (ModEulerHomogeneous.cpp:60-136):
@@ -1349,7 +1381,7 @@ ECO_D double hllc(const SchemeParams &S, const CellState<NPH> &L, const CellStat
       for (int k = 0; k < NPH; k++) {
         double mk = X.r[k] * (sS - uS);
         double rks = qdiv(mk, dS, idS);
-        double pks = eos_isentrope(S.eos[k], X.p[k], X.r[k], rks, eps);
+        double pks = eos_isentrope(S.eos[k], X.p[k], X.r[k], rks, eps, tab);
         double eks = eos_e(S.eos[k], rks, pks, eps);
         F.a[k] = X.a[k] * sM;
         F.m[k] = X.a[k] * rks * sM;
@@ -1411,12 +1443,12 @@ ECO_D void hllc_mixed(const SchemeParams &S, const CellState<NPH> &L, const Cell
     double pStar = mS * (sM - uS) + pS;
     const double ratio = (sS - uS) * iss;
     const float delta = (float)((sM - uS) * iss);
-    const float l1p = log1pf(delta);
+    const float l1p = log1p_f(delta);
 #pragma unroll
     for (int k = 0; k < NPH; k++) {
       const EosParams &q = S.eos[k];
       double rks = X.r[k] * ratio;
-      double g = (double)expm1f((float)q.gamma * l1p);
+      double g = (double)expm1_f((float)q.gamma * l1p);
       double pks = X.p[k] + (X.p[k] + q.pinf) * g;
       double eks = (pks + q.gpinf) * q.og * (1. / smax(rks, eps)) + q.eref;
       F.a[k] = X.a[k] * sM;
@@ -1441,13 +1473,13 @@ ECO_D float limiter_f(int lim, float s1, float s2) {
       return slope;
     case LIM_VANLEER: {
       float sum = s1 + s2;
-      if ((fabsf(sum) > 1e-6f) && (prod > 1e-6f)) slope = 2.f * prod / sum;
+      if ((fabsf(sum) > 1e-6f) && (prod > 1e-6f)) slope = ldiv(2.f * prod, sum);
       return slope;
     }
     case LIM_VANALBADA: {
       float sum = s1 + s2;
       if ((fabsf(s1) > 1e-6f) && (fabsf(s2) > 1e-6f) && (fabsf(sum) > 1e-6f) && (prod > 1e-6f))
-        slope = prod * sum / (s1 * s1 + s2 * s2);
+        slope = ldiv(prod * sum, s1 * s1 + s2 * s2);
       return slope;
     }
     case LIM_MC:

@@ -545,7 +545,8 @@ __device__ __forceinline__ void muscl_vars(const SchemeParams &S, const TW *wl_,
     // FP32-storage form: differences of the (decoded) cell values in FP64,
     // slopes, limiters and the extrapolation increment in FP32, added to the
     // FP64 cell value
-    const float idc = 1.f / (float)dcc, idm = 1.f / (float)dcm, idp = 1.f / (float)dcp;
+    const float idc = __builtin_amdgcn_rcpf((float)dcc), idm = __builtin_amdgcn_rcpf((float)dcm),
+                idp = __builtin_amdgcn_rcpf((float)dcp);
     const float fL = (float)dL, fR = (float)dR;
 #pragma unroll
     for (int v = 0; v < Ly::NVAR; v++) {
@@ -851,7 +852,12 @@ struct StagedLds {
   static constexpr int NF0 = (TX + 1) * TY, NF1 = TX * (TY + 1);    // faces of one direction
   static constexpr int NFM = NF0 > NF1 ? NF0 : NF1;
   static_assert(NFM <= NT, "staged faces: at most one face per thread and direction");
+  // Kapila's HLLC takes an isentropic power per phase and face: the power's
+  // log and exp tables are gathered per lane, from LDS rather than through
+  // the vector cache (5 KB; three workgroups still fit a CU)
+  static constexpr bool PTAB = (MODEL == MODEL_KAPILA);
   double S[Ly::NVAR][NS];
+  double ptab[PTAB ? POWTAB_WORDS : 1];
   // the limited slopes of one direction, then (after every thread has read
   // them) the fluxes of that direction's faces: the two share one block, so
   // that four workgroups fit a CU's LDS
@@ -1008,7 +1014,7 @@ __device__ __forceinline__ double stage_faces(StagedLds<MODEL, NPH, DIM> &L, con
     to_face_frame(dir, 0, Rc.u);
     fulfill<MODEL, NPH>(S, Lc);
     fulfill<MODEL, NPH>(S, Rc);
-    smx = smax(smx, hllc<MODEL, NPH>(S, Lc, Rc, F));
+    smx = smax(smx, hllc<MODEL, NPH>(S, Lc, Rc, F, LD::PTAB ? L.ptab : nullptr));
     from_face_frame(dir, 0, F.q);
     have = true;
   }
@@ -1092,6 +1098,10 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int slot, c
     // limited slopes, then the faces, then each cell's two faces
     using SL = StagedLds<MODEL, NPH, DIM>;
     stage_load_state<MODEL, NPH, DIM>(lds, Wsrc, nc, o, nx, ny, tid);
+    if constexpr (SL::PTAB) {
+      for (int i = tid; i < POWTAB_WORDS; i += NT)
+        lds.ptab[i] = (i < POWTAB_EXP) ? powtab::LOG[i / 3][i % 3] : as_double(powtab::EXP[i - POWTAB_EXP]);
+    }
     __syncthreads();
     stage_slopes_dispatch<MODEL, NPH, DIM, 0>(lds, S, G, o, tid);
     __syncthreads();
@@ -1259,6 +1269,19 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int slot, c
   }
 }
 
+// Workgroups are dispatched to the XCDs round robin (workgroup i on XCD i mod
+// 8), each XCD with its own L2.  Workgroup i takes the tile that makes every
+// XCD walk one contiguous band of the tiles (x fastest): tiles next to each
+// other, which read each other's halo cells, then share an L2.  A bijection
+// of [0, n) for any n (the first n mod 8 XCDs take one tile more).
+__device__ __forceinline__ int xcd_band_tile(int wg, int n) {
+  if constexpr (ECO_XCD_BANDS == 0) return wg;
+  constexpr int NX = 8;
+  const int q = n / NX, r = n % NX;
+  const int x = wg % NX, l = wg / NX;
+  return (x < r) ? x * (q + 1) + l : r * (q + 1) + (x - r) * q + l;
+}
+
 // WALL = false: the MUSCL stages of a solve without walls (no wall code in
 // the face pass or the boundary faces; the register allocation of the hot
 // path is the transmissive one)
@@ -1266,7 +1289,7 @@ template <int MODEL, int NPH, int DIM, int STAGE, bool SAMELIM, bool WALL = true
 __global__ void __launch_bounds__(Tile<DIM>::T, STAGED ? Tile<DIM>::WS : Tile<DIM>::W)
 k_stage(const StageArgs A) {
   __shared__ StageLdsT<MODEL, NPH, DIM, STAGED> lds;
-  stage_tile<MODEL, NPH, DIM, STAGE, SAMELIM, WALL, STAGED>(A, A.step, blockIdx.x, lds);
+  stage_tile<MODEL, NPH, DIM, STAGE, SAMELIM, WALL, STAGED>(A, A.step, xcd_band_tile(blockIdx.x, gridDim.x), lds);
 }
 
 // Persistent form over a pool: every workgroup walks the class's tile table

@@ -180,19 +180,21 @@ __device__ __forceinline__ float smaxf_(float a, float b) { return (a < b) ? b :
 // frozen sound speed in FP32 (MixKapila.cpp:147-172)
 template <int NPH>
 __device__ __forceinline__ float frozen_c_f(const SchemeParams &S, const CellState<NPH> &s) {
+  // hardware reciprocals and square root (1 ulp each)
   float rho = 0.f, ck2[NPH];
 #pragma unroll
   for (int k = 0; k < NPH; k++) rho += (float)s.a[k] * (float)s.r[k];
+  const float irho = __builtin_amdgcn_rcpf(rho);
   float c2 = 0.f;
 #pragma unroll
   for (int k = 0; k < NPH; k++) {
     const EosParams &q = S.eos[k];
     const float r = (float)s.r[k], p = (float)s.p[k];
-    const float ir = __frcp_rn(r);
+    const float ir = __builtin_amdgcn_rcpf(r);
     ck2[k] = (q.type == EOS_SG) ? (float)q.gamma * (p + (float)q.pinf) * ir : (float)q.gamma * p * ir;
-    c2 += __fdividef((float)s.a[k] * r, rho) * ck2[k];
+    c2 += ((float)s.a[k] * r) * irho * ck2[k];
   }
-  return sqrtf(c2);
+  return __builtin_amdgcn_sqrtf(c2);
 }
 
 // Davis estimates (ModKapila.cpp:117-121) in FP32; returns max |s| above the guard
@@ -408,7 +410,8 @@ __device__ __forceinline__ void stage32_slopes(StagedLds32<MODEL, NPH, DIM> &L, 
     const int cy = (dir == 0) ? c / RW : c / RW - 1;
     const int gi = o[dir] + (dir == 0 ? cx : cy);
     const bool has_m = (gi >= 1) && (gi <= ndir - 1), has_p = (gi >= 0) && (gi + 1 <= ndir - 1);
-    const float idm = 1.f / (float)(has_m ? Gc[gi - 1] : 1.), idp = 1.f / (float)(has_p ? Gc[gi] : 1.);
+    const float idm = __builtin_amdgcn_rcpf((float)(has_m ? Gc[gi - 1] : 1.));
+    const float idp = __builtin_amdgcn_rcpf((float)(has_p ? Gc[gi] : 1.));
     const int s0 = (cx + 2) + (cy + 2) * SX;
     double wm[Ly::NVAR], w[Ly::NVAR], wp[Ly::NVAR];
     staged32_vec<MODEL, NPH, DIM>(L, s0 - SD, wm);
@@ -670,7 +673,7 @@ template <int MODEL, int NPH, int DIM, int STAGE, bool SAMELIM, bool STAGED = fa
 __global__ void __launch_bounds__(Tile<DIM>::T, STAGED ? Tile<DIM>::WS32 : Tile<DIM>::W)
 k_stage32(const StageArgs A) {
   __shared__ Stage32LdsT<MODEL, NPH, DIM, STAGED> lds;
-  stage32_tile<MODEL, NPH, DIM, STAGE, SAMELIM, STAGED>(A, A.step, blockIdx.x, lds);
+  stage32_tile<MODEL, NPH, DIM, STAGE, SAMELIM, STAGED>(A, A.step, xcd_band_tile(blockIdx.x, gridDim.x), lds);
 }
 
 // persistent form over a pool (as k_stage_batch)

@@ -92,3 +92,9 @@
 #ifndef ECO_WAVESB32_3
 #define ECO_WAVESB32_3 0
 #endif
+
+// single-solve stage kernels: workgroup i takes the tile that gives every XCD
+// one contiguous band of tiles (1), or tile i (0)
+#ifndef ECO_XCD_BANDS
+#define ECO_XCD_BANDS 1
+#endif
