@@ -78,6 +78,29 @@ ECO_D T ldiv(T a, T b) {
   else return a / b;
 }
 
+// 1/x in FP64 for the FP32-storage path, whose FP64 arithmetic only has to
+// stay far below FP32's rounding: the hardware reciprocal refined by two
+// Newton steps (within about an ulp of 1/x for normal x; five
+// instructions where an IEEE FP64 division takes about eleven).
+ECO_D double rcp_fast(double x) {
+  double y = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, y, 1.);
+  y = fma(y, e, y);
+  e = fma(-x, y, 1.);
+  return fma(y, e, y);
+}
+// 1/b and a/b: IEEE division, or (FAST, the FP32-storage path) rcp_fast
+template <bool FAST>
+ECO_D double drcp(double b) {
+  if constexpr (FAST) return rcp_fast(b);
+  else return 1. / b;
+}
+template <bool FAST>
+ECO_D double ddiv(double a, double b) {
+  if constexpr (FAST) return a * rcp_fast(b);
+  else return a / b;
+}
+
 // log1p and expm1 in FP32 for the isentropic power factor of the FP32-storage
 // HLLC.  For the small arguments of nearly every face (|x| <= 1/32, |z| <=
 // 1/4) their Taylor series to degree 7 by Horner's rule: the first omitted
@@ -439,7 +462,8 @@ ECO_D void zero_small(double *u) {
 
 // Flux*::buildPrim: FluxKapila.cpp:139-185, FluxMultiP.cpp:144-183,
 // FluxEuler.cpp:107-131.  Leaves the derived quantities of fulfillState set.
-template <int MODEL, int NPH>
+// FAST (FP32-storage path, Kapila and MultiP): quotients through rcp_fast.
+template <int MODEL, int NPH, bool FAST = false>
 ECO_D void build_prim(const SchemeParams &S, FluxVec<NPH> &f, CellState<NPH> &s) {
   const double eps = S.eps;
   if constexpr (MODEL == MODEL_EULERHOM) {
@@ -543,7 +567,7 @@ ECO_D void build_prim(const SchemeParams &S, FluxVec<NPH> &f, CellState<NPH> &s)
         un += f.a[k];
       }
     }
-    const double iun = 1. / un;
+    const double iun = drcp<FAST>(un);
 #pragma unroll
     for (int k = 0; k < NPH; k++) f.a[k] = qdiv(f.a[k], un, iun);
     double rhoMel = 0.;
@@ -553,16 +577,16 @@ ECO_D void build_prim(const SchemeParams &S, FluxVec<NPH> &f, CellState<NPH> &s)
       s.a[k] = f.a[k];
       double ein;
       if constexpr (MODEL == MODEL_KAPILA) {
-        s.r[k] = f.m[k] / smax(f.a[k], eps);
-        ein = f.en[k] / smax(f.m[k], eps);
+        s.r[k] = ddiv<FAST>(f.m[k], smax(f.a[k], eps));
+        ein = ddiv<FAST>(f.en[k], smax(f.m[k], eps));
       } else {
-        s.r[k] = f.m[k] / f.a[k];
-        ein = f.en[k] / f.m[k];
+        s.r[k] = ddiv<FAST>(f.m[k], f.a[k]);
+        ein = ddiv<FAST>(f.en[k], f.m[k]);
       }
       s.p[k] = eos_p(S.eos[k], s.r[k], ein);
       verify_correct<MODEL, NPH>(S, s, k);
     }
-    const double irm = 1. / rhoMel;
+    const double irm = drcp<FAST>(rhoMel);
 #pragma unroll
     for (int d = 0; d < 3; d++) s.u[d] = qdiv(f.q[d], rhoMel, irm);
     zero_small(s.u);
@@ -753,7 +777,7 @@ ECO_D void relax_ptmu(const SchemeParams &S, CellState<NPH> &s, CellState<NPH> &
 // FluxKapila::correctionEnergy (FluxKapila.cpp:239-268), after
 // Mixture::totalEnergyToInternalEnergy (MixKapila.cpp:188-195).  s.rho is the
 // mixture density cached by buildPrim (before relaxation).
-template <int MODEL, int NPH>
+template <int MODEL, int NPH, bool FAST = false>
 ECO_D void correction_energy(const SchemeParams &S, CellState<NPH> &s) {
   if constexpr (MODEL == MODEL_KAPILA) {
     s.em = s.E - 0.5 * sqn(s.u);
@@ -764,7 +788,7 @@ ECO_D void correction_energy(const SchemeParams &S, CellState<NPH> &s) {
       p -= s.a[k] * (S.eos[k].gpg + s.r[k] * S.eos[k].eref);
       denom += s.a[k] * S.eos[k].og;
     }
-    p /= denom;
+    p = ddiv<FAST>(p, denom);
 #pragma unroll
     for (int k = 0; k < NPH; k++) { s.p[k] = p; verify_correct<MODEL, NPH>(S, s, k); }
   }
@@ -1415,7 +1439,7 @@ ECO_D void hllc_mixed(const SchemeParams &S, const CellState<NPH> &L, const Cell
   double uL = L.u[0], pL = L.pm, rhoL = L.rho;
   double uR = R.u[0], pR = R.pm, rhoR = R.rho;
   double mL = rhoL * (sL - uL), mR = rhoR * (sR - uR);
-  double sM = (pR - pL + mL * uL - mR * uR) / (mL - mR);
+  double sM = (pR - pL + mL * uL - mR * uR) * rcp_fast(mL - mR);
   if (fabs(sM) < 1.e-8) sM = 0.;
   if (sL >= 0. || sR <= 0.) {
     const bool left = (sL >= 0.);
@@ -1437,9 +1461,9 @@ ECO_D void hllc_mixed(const SchemeParams &S, const CellState<NPH> &L, const Cell
     const CellState<NPH> &X = left ? L : R;
     double uS = left ? uL : uR, pS = left ? pL : pR, mS = left ? mL : mR, sS = left ? sL : sR;
     double Et = X.em + 0.5 * sqn(X.u);
-    const double iss = 1. / (sS - sM);
+    const double iss = rcp_fast(sS - sM);
     double rhoStar = mS * iss;
-    double EStar = Et + (sM - uS) * (sM + pS / mS);
+    double EStar = Et + (sM - uS) * (sM + pS * rcp_fast(mS));
     double pStar = mS * (sM - uS) + pS;
     const double ratio = (sS - uS) * iss;
     const float delta = (float)((sM - uS) * iss);
@@ -1450,7 +1474,7 @@ ECO_D void hllc_mixed(const SchemeParams &S, const CellState<NPH> &L, const Cell
       double rks = X.r[k] * ratio;
       double g = (double)expm1_f((float)q.gamma * l1p);
       double pks = X.p[k] + (X.p[k] + q.pinf) * g;
-      double eks = (pks + q.gpinf) * q.og * (1. / smax(rks, eps)) + q.eref;
+      double eks = (pks + q.gpinf) * q.og * rcp_fast(smax(rks, eps)) + q.eref;
       F.a[k] = X.a[k] * sM;
       F.m[k] = X.a[k] * rks * sM;
       F.en[k] = X.a[k] * rks * eks * sM;

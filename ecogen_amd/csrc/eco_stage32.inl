@@ -91,7 +91,7 @@ __device__ __forceinline__ void round_state(CellState<NPH> &s) {
 // Specific internal energy with one reciprocal: the FP32-storage path needs
 // the fluxes mutually consistent in FP64, not the reference's rounding.
 __device__ __forceinline__ double eos_e_fast(const EosParams &q, double rho, double p, double eps) {
-  return (p + q.gpinf) * q.og * (1. / smax(rho, eps)) + q.eref;
+  return (p + q.gpinf) * q.og * rcp_fast(smax(rho, eps)) + q.eref;
 }
 
 // energies and mixture variables in FP64 (no sound speeds): the part of
@@ -104,7 +104,7 @@ __device__ __forceinline__ void fulfill_energy(const SchemeParams &S, CellState<
   s.rho = 0.; s.pm = 0.;
 #pragma unroll
   for (int k = 0; k < NPH; k++) { s.rho += s.a[k] * s.r[k]; s.pm += s.a[k] * s.p[k]; }
-  const double irho = 1. / ((MODEL == MODEL_KAPILA) ? smax(s.rho, eps) : s.rho);
+  const double irho = rcp_fast((MODEL == MODEL_KAPILA) ? smax(s.rho, eps) : s.rho);
 #pragma unroll
   for (int k = 0; k < NPH; k++) s.Y[k] = s.a[k] * s.r[k] * irho;
   s.em = 0.;
@@ -147,7 +147,7 @@ __device__ __forceinline__ bool relax_p_fast(const SchemeParams &S, CellState<NP
   double f = 0., df = 1.;
   do {
     it++;
-    pStar -= f / df;
+    pStar -= f * rcp_fast(df);
 #pragma unroll
     for (int k = 0; k < NPH; k++) pStar = eos_fix_p(S.eos[k], pStar);
     f = -1.; df = 0.;
@@ -155,7 +155,7 @@ __device__ __forceinline__ bool relax_p_fast(const SchemeParams &S, CellState<NP
     for (int k = 0; k < NPH; k++) {
       const EosParams &q = S.eos[k];
       const double num = q.gamma * (pStar + q.pinf);
-      const double inum = 1. / num;
+      const double inum = rcp_fast(num);
       f += ak[k] * (num + pk[k] - pStar) * inum;
       df -= ak[k] * q.gamma * (pk[k] + q.pinf) * inum * inum;
     }
@@ -166,8 +166,8 @@ __device__ __forceinline__ bool relax_p_fast(const SchemeParams &S, CellState<NP
     const EosParams &q = S.eos[k];
     const double num = q.gamma * (pStar + q.pinf);
     const double denom = num + pk[k] - pStar;
-    s.a[k] = ak[k] * denom / num;
-    s.r[k] = rk[k] * num / denom;
+    s.a[k] = ak[k] * denom * rcp_fast(num);
+    s.r[k] = rk[k] * num * rcp_fast(denom);
     s.p[k] = pStar;
   }
   s.pm = pStar;
@@ -413,14 +413,27 @@ __device__ __forceinline__ void stage32_slopes(StagedLds32<MODEL, NPH, DIM> &L, 
     const float idm = __builtin_amdgcn_rcpf((float)(has_m ? Gc[gi - 1] : 1.));
     const float idp = __builtin_amdgcn_rcpf((float)(has_p ? Gc[gi] : 1.));
     const int s0 = (cx + 2) + (cy + 2) * SX;
-    double wm[Ly::NVAR], w[Ly::NVAR], wp[Ly::NVAR];
-    staged32_vec<MODEL, NPH, DIM>(L, s0 - SD, wm);
-    staged32_vec<MODEL, NPH, DIM>(L, s0, w);
-    staged32_vec<MODEL, NPH, DIM>(L, s0 + SD, wp);
+    // differences of the FP32-stored variables in FP32 (exact operands, one
+    // rounding, as the FP64 difference rounded to FP32); of the volume
+    // fractions in FP64 from their exact pairs
+    static_assert(NPH == 2, "FP32 storage: two phases");
+    double am[2], ac[2], ap[2];
+    decode_alpha(L.S[0][s0 - SD], L.S[3][s0 - SD], am[0], am[1]);
+    decode_alpha(L.S[0][s0], L.S[3][s0], ac[0], ac[1]);
+    decode_alpha(L.S[0][s0 + SD], L.S[3][s0 + SD], ap[0], ap[1]);
 #pragma unroll
     for (int v = 0; v < Ly::NVAR; v++) {
-      const float sm = has_m ? (float)(w[v] - wm[v]) * idm : 0.f;
-      const float sp = has_p ? (float)(wp[v] - w[v]) * idp : 0.f;
+      float dm, dp;
+      if (v == 0 || v == 3) {
+        dm = (float)(ac[v / 3] - am[v / 3]);
+        dp = (float)(ap[v / 3] - ac[v / 3]);
+      } else {
+        const float c = L.S[v][s0];
+        dm = c - L.S[v][s0 - SD];
+        dp = L.S[v][s0 + SD] - c;
+      }
+      const float sm = has_m ? dm * idm : 0.f;
+      const float sp = has_p ? dp * idp : 0.f;
       const bool isa = (v < 3 * NPH) && (v % 3 == 0);
       float g;
       if constexpr (LV >= 0) {
@@ -631,7 +644,7 @@ __device__ __forceinline__ void stage32_tile(const StageArgs &A, const int slot,
     sn.rho = 0.;
 #pragma unroll
     for (int k = 0; k < NPH; k++) sn.rho += sn.a[k] * sn.r[k];
-    sn.E = rhoE / sn.rho;
+    sn.E = rhoE * rcp_fast(sn.rho);
     FluxVec<NPH> buf;
     build_cons_fast<MODEL, NPH>(S, sn, buf);
     buf.E = rhoE;
@@ -639,10 +652,10 @@ __device__ __forceinline__ void stage32_tile(const StageArgs &A, const int slot,
     scheme_correction<MODEL, NPH>(acc, buf);
     const double rhoE_new = acc.E;
     CellState<NPH> s1;
-    build_prim<MODEL, NPH>(S, acc, s1);
+    build_prim<MODEL, NPH, true>(S, acc, s1);
     bool ok = true;
     if (S.relax) ok = relax_p_fast<MODEL, NPH>(S, s1);
-    correction_energy<MODEL, NPH>(S, s1);
+    correction_energy<MODEL, NPH, true>(S, s1);
     if (!ok) atomicOr(&ctl->err, 1u);
     bool bad = false;
 #pragma unroll
