@@ -316,10 +316,18 @@ def load(rundir):
         "binary": _true(om.get("binary")) if om is not None else False,
         "precision": int(float(om.get("precision", "0"))) if om is not None else 0,
     }
-    # probes (Input.cpp:136-142, OutputProbeGNU.cpp:44-90); cuts are not written
-    for tag in ("cut1D", "cut2D"):
-        if main.find(tag) is not None:
-            raise DeckError(f"mainV5.xml: <{tag}> cuts are not written")
+    # cuts (Input.cpp:122-135, OutputCutGNU.cpp:44-90): every <cut1D> (vertex,
+    # vecDir), then every <cut2D> (vertex, vecNormal), in the reference's order
+    case["cuts"] = []
+    for tag, vt in (("cut1D", "vecDir"), ("cut2D", "vecNormal")):
+        for cu in main.findall(tag):
+            where = f"mainV5.xml {tag}"
+            v, w = _child(cu, "vertex", where), _child(cu, vt, where)
+            vec = [_f(w, a, where) for a in "xyz"]
+            if math.sqrt(sum(c * c for c in vec)) < 1e-6:
+                raise DeckError(f"{where}: null {vt}")  # GOLine/GOPlan constructors
+            case["cuts"].append({"type": tag, "vertex": [_f(v, a, where) for a in "xyz"], "vec": vec})
+    # probes (Input.cpp:136-142, OutputProbeGNU.cpp:44-90)
     case["probes"] = []
     for pr in main.findall("probe"):
         where = "mainV5.xml probe"
@@ -688,6 +696,9 @@ def _run_with_output(ctx, case, outdir, resume=None):
         ct0 = int(kept[-1].split()[3])
 
     gnu = o["format"] == "GNU"
+    cuts = [dict(cu, n=0) for cu in case.get("cuts", [])]
+    if cuts:
+        (outdir / "cuts").mkdir(exist_ok=True)
 
     def dump(st=None):
         if gnu:  # OutputGNU::ecritSolution: the file, then the script over all files so far
@@ -697,6 +708,15 @@ def _run_with_output(ctx, case, outdir, resume=None):
         else:
             f = outdir / f"result_CPU0_AMR0_TIME{len(files)}.vtr"
             output.write_vtr(f, case, ctx.get_state(), binary=o["binary"], precision=prec)
+        # cuts (Run.cpp:152, 259): after each result file, one file per cut
+        # numbered by the cut's own count from 0 (a resumed run too); cuts of
+        # one type share the file name, so the last of a type is what remains
+        for cu in cuts:
+            state = ctx.get_state()
+            output.write_cut(outdir / "cuts" / f"{cu['type']}_CPU0_TIME{cu['n']}.out", case, state, cu)
+            output.write_gnu_script(outdir / "cuts" / f"visualisation{cu['type']}.gnu", case, cu["n"] + 1,
+                                    dim=1 if cu["type"] == "cut1D" else 2, stem=cu["type"])
+            cu["n"] += 1
         it, t, dt = (0, 0.0, 1.e-15) if st is None else (it0 + st.steps, st.t, st.dt_last)
         ms = ct0 + int((time.perf_counter() - t0) * 1e3) if st is not None else 0
         info.write(f"{len(files)} {it} {t:.{prec}g} {ms} {dt:.{prec}g}\n")

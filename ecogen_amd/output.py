@@ -322,6 +322,68 @@ def _rows(coords, cols, idx):
     return "".join("".join(f"{v:.6g} " for v in r) + "\n" for r in rows)
 
 
+def _unit(v):
+    """Coord::normalized: each component divided by the norm."""
+    n = np.sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2])
+    return [v[0] / n, v[1] / n, v[2] / n]
+
+
+def _plane_base(point, normal):
+    """GOPlan::createBase (GOPlan.cpp:82-106): a tangent from the point to the
+    plane's intersection with the last axis whose normal component is at
+    least 1e-6 (z, then y, then x), normalized; binormal = normal x tangent."""
+    M, n = point, normal
+    if abs(n[2]) >= 1e-6:
+        N = [0.0, 0.0, (M[0] * n[0] + M[1] * n[1]) / n[2] + M[2]]
+    elif abs(n[1]) >= 1e-6:
+        N = [0.0, (M[0] * n[0] + M[2] * n[2]) / n[1] + M[1], 0.0]
+    else:
+        N = [(M[1] * n[1] + M[2] * n[2]) / n[0] + M[0], 0.0, 0.0]
+    t = _unit([N[0] - M[0], N[1] - M[1], N[2] - M[2]])
+    b = [n[1] * t[2] - n[2] * t[1], n[2] * t[0] - n[0] * t[2], n[0] * t[1] - n[1] * t[0]]
+    return t, b
+
+
+def cut_cells(case, cut):
+    """The cells a cut prints, in cell order, and their coordinates on it
+    (Cell::printGnuplotAMR with Element::traverseObjet, src/Cell.cpp:1523-1556,
+    src/Meshes/Element.cpp:182-186): a cell is printed when the object's
+    distance to its centre is below lCFL.
+
+    * cut1D, a line through ``vertex`` along the normalized ``vec``
+      (GOLine::distancePoint, GOLine.cpp:61-66): with w = centre - vertex,
+      the distance is 0 where abs(w . dir) < 1e-6, else |w x dir|.  That
+      source file calls the C library's int abs(int) (it has no using
+      namespace std), so w . dir is truncated to an integer first: every
+      cell less than 1 (in length units) along the line from the vertex,
+      measured parallel to it, has distance 0 and is printed.  Its coordinate
+      is dir . w (GOLine::projectionPoint).
+    * cut2D, a plane through ``vertex`` with the normalized normal ``vec``
+      (GOPlan::distancePoint, GOPlan.cpp:64-68): |w . n| with the FP64 abs;
+      its coordinates are (tangent . w, binormal . w) (GOPlan::projectionPoint)."""
+    X, Y, Z = _centres(case)
+    P, v = cut["vertex"], _unit(cut["vec"])
+    wx, wy, wz = X - P[0], Y - P[1], Z - P[2]
+    lim = _lcfl(case)
+    if cut["type"] == "cut1D":
+        t = v[0] * wx + v[1] * wy + v[2] * wz
+        cx, cy, cz = wy * v[2] - wz * v[1], wz * v[0] - wx * v[2], wx * v[1] - wy * v[0]
+        dist = np.where(np.trunc(t) == 0.0, 0.0, np.sqrt(cx * cx + cy * cy + cz * cz))
+        return np.nonzero(dist < lim)[0], [t]
+    tg, bn = _plane_base(P, v)
+    dist = np.abs(wx * v[0] + wy * v[1] + wz * v[2])
+    return np.nonzero(dist < lim)[0], [tg[0] * wx + tg[1] * wy + tg[2] * wz, bn[0] * wx + bn[1] * wy + bn[2] * wz]
+
+
+def write_cut(path, case, state, cut):
+    """One file of OutputCutGNU::ecritSolution (OutputCutGNU.cpp:105-123):
+    per printed cell its coordinates on the cut and the columns of the GNU
+    result files, then an empty line."""
+    idx, coords = cut_cells(case, cut)
+    text = _rows([c[idx] for c in coords], _gnu_columns(case, np.asarray(state, dtype=np.float64)), idx)
+    Path(path).write_text(text + "\n")
+
+
 def probe_cell(case, vertex):
     """OutputProbeGNU::locateProbeInMesh (OutputProbeGNU.cpp:115-135): the
     first cell, in cell order, of least distance |centre - vertex|, the search

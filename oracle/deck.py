@@ -112,6 +112,13 @@ def write_deck(case, root: Path):
         v = pr["vertex"]
         tc += (f'\n  <probe name="{pr["name"]}">\n    <vertex x="{v[0]!r}" y="{v[1]!r}" z="{v[2]!r}"/>\n'
                f'    <timeControl acqFreq="{pr["acq_freq"]!r}"/>\n  </probe>')
+    for cu in case.get("cuts", []):
+        # Input.cpp:122-135, OutputCutGNU.cpp:44-90: <cut1D> with vertex and
+        # vecDir, <cut2D> with vertex and vecNormal
+        v, w = cu["vertex"], cu["vec"]
+        tag, vt = ("cut1D", "vecDir") if cu["type"] == "cut1D" else ("cut2D", "vecNormal")
+        tc += (f'\n  <{tag}>\n    <vertex x="{v[0]!r}" y="{v[1]!r}" z="{v[2]!r}"/>\n'
+               f'    <{vt} x="{w[0]!r}" y="{w[1]!r}" z="{w[2]!r}"/>\n  </{tag}>')
     if case.get("resume"):
         # Input.cpp:253-257: restart from result file number fileNumber
         tc += f'\n  <resumeSimulation fileNumber="{case["resume"]}"/>'

@@ -78,8 +78,8 @@ def test_deck_refusals(tmp_path):
     write_deck(dict(case, name="x"), tmp_path / "f")
     mx = tmp_path / "f" / "case" / "mainV5.xml"
     mx.write_text(mx.read_text().replace("</computationParam>", '<cut1D><vertex x="0" y="0" z="0"/>'
-                                         '<vecDir x="1" y="0" z="0"/></cut1D></computationParam>'))
-    with pytest.raises(deck.DeckError, match="cut1D"):
+                                         '<vecDir x="0" y="0" z="0"/></cut1D></computationParam>'))
+    with pytest.raises(deck.DeckError, match="null vecDir"):  # GOLine::GOLine
         deck.load(tmp_path / "f")
     (tmp_path / "a" / "case" / "meshV5.xml").write_text(
         '<?xml version="1.0"?><mesh><type structure="unstructured"/></mesh>')
@@ -233,6 +233,32 @@ def test_deck_probe_files_match_reference(name, tmp_path):
         assert [len(r) for r in ours] == [len(r) for r in ref]
         a, b = np.array(ours, dtype=float), np.array(ref, dtype=float)
         assert np.all(np.abs(a - b) <= 5e-6 * np.abs(b) + 1e-300)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["kapila_2d_fo_cuts", "euler_1d_fo_cuts", "kapila_3d_o2_cuts"])
+def test_deck_cut_files_match_reference(name, tmp_path):
+    """python -m ecogen_amd.deck --vtk on a deck with cuts writes the
+    reference's cut files (the same cells and coordinates, values equal to
+    their 6 printed digits) and scripts under results/<run>/cuts."""
+    import json
+    from pathlib import Path
+    from ecogen_amd import deck
+    d = Path(__file__).resolve().parent / "golden" / "cuts" / name
+    case = json.loads((d / "case.json").read_text())
+    print(f"case {name} deck --vtk with cuts", flush=True)
+    write_deck(dict(case, name=name), tmp_path / "run")
+    deck.main([str(tmp_path / "run"), "--out", str(tmp_path / "f.npz"), "--vtk"])
+    res = tmp_path / "run" / "results" / name / "cuts"
+    assert sorted(f.name for f in res.iterdir()) == sorted(f.name for f in d.iterdir() if f.name != "case.json")
+    for f in sorted(d.glob("*.gnu")):
+        assert (res / f.name).read_text() == f.read_text()
+    for f in sorted(d.glob("*.out")):
+        ours = [ln.split() for ln in (res / f.name).read_text().split("\n") if ln.strip()]
+        ref = [ln.split() for ln in f.read_text().split("\n") if ln.strip()]
+        assert [len(r) for r in ours] == [len(r) for r in ref], f.name
+        a, b = np.array(ours, dtype=float), np.array(ref, dtype=float)
+        assert np.all(np.abs(a - b) <= 5e-6 * np.abs(b) + 1e-300), f.name
 
 
 @pytest.mark.gpu

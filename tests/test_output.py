@@ -134,3 +134,46 @@ def test_probe_files_reproduce_reference(name, tmp_path):
         assert text == (d / f"{pr['name']}.out").read_text()
         output.write_gnu_script(tmp_path / "v.gnu", rcase, 1, dim=0, stem=pr["name"])
         assert (tmp_path / "v.gnu").read_text() == (d / f"visualisation{pr['name']}.gnu").read_text()
+
+
+CUTS = Path(__file__).resolve().parent / "golden" / "cuts"
+
+
+@pytest.mark.parametrize("name", sorted(p.name for p in CUTS.iterdir() if p.is_dir()))
+def test_cut_files_reproduce_reference(name, tmp_path):
+    """OutputCutGNU: the deck's cuts, the cells each prints (a line keeps
+    every cell less than 1 along it from its vertex, the C int abs of
+    GOLine::distancePoint, and beyond that the cells within lCFL of it; a
+    plane the cells within lCFL), their coordinates on the cut, the columns,
+    the file of each result output (cuts of one type share a file name: the
+    last one's remains) and the scripts, character for character.  The
+    states are the oracle's, which equal the reference's bit for bit."""
+    import oracle as O
+    from deck import write_deck
+    from ecogen_amd import deck, output
+    d = CUTS / name
+    case = json.loads((d / "case.json").read_text())
+    write_deck(dict(case, name=name), tmp_path / "run")
+    rcase, st0 = deck.load(tmp_path / "run")
+    assert [(c["type"], c["vertex"]) for c in rcase["cuts"]] == [(c["type"], c["vertex"]) for c in case["cuts"]]
+    freq = case["iter_freq"]
+    states = [st0] + [O.run(rcase, st0, nsteps=k)[0] for k in range(freq, case["steps"] + 1, freq)]
+    last = {c["type"]: c for c in rcase["cuts"]}
+    for typ, cu in last.items():
+        for k, st in enumerate(states):
+            output.write_cut(tmp_path / "c.out", rcase, st, cu)
+            assert (tmp_path / "c.out").read_text() == (d / f"{typ}_CPU0_TIME{k}.out").read_text(), (typ, k)
+        output.write_gnu_script(tmp_path / "v.gnu", rcase, len(states), dim=1 if typ == "cut1D" else 2, stem=typ)
+        assert (tmp_path / "v.gnu").read_text() == (d / f"visualisation{typ}.gnu").read_text()
+
+
+def test_cut_line_keeps_cells_by_truncated_projection():
+    """The selection rule on its own: on a 3 m line of 30 cells through x =
+    1.05 along x, off the cells' axis by more than lCFL, the cells whose
+    projection lies in (-1, 1) are kept and no other."""
+    from ecogen_amd import output
+    case = {"cells": [30, 1, 1], "lengths": [3.0, 0.1, 0.1]}
+    idx, (t,) = output.cut_cells(case, {"type": "cut1D", "vertex": [1.05, 0.5, 0.05], "vec": [2.0, 0.0, 0.0]})
+    x = (np.arange(30) + 0.5) * 0.1
+    assert list(idx) == [i for i in range(30) if abs(x[i] - 1.05) < 1.0]
+    assert np.allclose(t[idx], x[idx] - 1.05)
