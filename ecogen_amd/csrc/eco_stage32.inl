@@ -686,7 +686,7 @@ template <int MODEL, int NPH, int DIM, int STAGE, bool SAMELIM, bool STAGED = fa
 __global__ void __launch_bounds__(Tile<DIM>::T, STAGED ? Tile<DIM>::WS32 : Tile<DIM>::W)
 k_stage32(const StageArgs A) {
   __shared__ Stage32LdsT<MODEL, NPH, DIM, STAGED> lds;
-  stage32_tile<MODEL, NPH, DIM, STAGE, SAMELIM, STAGED>(A, A.step, xcd_band_tile(blockIdx.x, gridDim.x), lds);
+  stage32_tile<MODEL, NPH, DIM, STAGE, SAMELIM, STAGED>(A, A.step, xcd_band_tile<DIM>(blockIdx.x, gridDim.x), lds);
 }
 
 // persistent form over a pool (as k_stage_batch)
