@@ -543,8 +543,8 @@ __device__ __forceinline__ void muscl_vars(const SchemeParams &S, const TW *wl_,
   using Ly = Layout<MODEL, NPH, DIM>;
   if constexpr (RECIP) {
     // FP32-storage form: differences of the (decoded) cell values in FP64,
-    // slopes, limiters and the extrapolation increment in FP32, added to the
-    // FP64 cell value
+    // slopes, limiters and the extrapolation in FP32 (the volume fractions'
+    // increment added to their FP64 value)
     const float idc = __builtin_amdgcn_rcpf((float)dcc), idm = __builtin_amdgcn_rcpf((float)dcm),
                 idp = __builtin_amdgcn_rcpf((float)dcp);
     const float fL = (float)dL, fR = (float)dR;
@@ -565,8 +565,13 @@ __device__ __forceinline__ void muscl_vars(const SchemeParams &S, const TW *wl_,
         gl = limiter_f(llim, sc, sm);
         gr = limiter_f(rlim, sc, sp);
       }
-      xl[v] = wl + (double)(gl * fL);
-      xr[v] = wr + (double)(-gr * fR);
+      if (isa) {
+        xl[v] = wl + (double)(gl * fL);
+        xr[v] = wr + (double)(-gr * fR);
+      } else {
+        xl[v] = (double)((float)wl + gl * fL);
+        xr[v] = (double)((float)wr + (-gr) * fR);
+      }
     }
   } else {
     // the reference's correctly rounded quotients through shared reciprocals

@@ -494,13 +494,22 @@ __device__ __forceinline__ float stage32_faces(StagedLds32<MODEL, NPH, DIM> &L, 
     const int sl = (lx + 2) + (ly + 2) * SX, sr = sl + SD;
     const int gl = (dir == 0) ? (lx + 1) + ly * GW : lx + (ly + 1) * GW, gr = gl + GD;
     const float fL = (float)Dl[gi - 1], fR = (float)Dr[gi - 1];
-    double wl[Ly::NVAR], wr[Ly::NVAR], xl[Ly::NVAR], xr[Ly::NVAR];
-    staged32_vec<MODEL, NPH, DIM>(L, sl, wl);
-    staged32_vec<MODEL, NPH, DIM>(L, sr, wr);
+    // the FP32-stored variables extrapolated in FP32 (one rounding of the
+    // face value, below the storage's own); the volume fractions in FP64
+    // from their exact pairs
+    double xl[Ly::NVAR], xr[Ly::NVAR];
+    double al[2], ar[2];
+    decode_alpha(L.S[0][sl], L.S[3][sl], al[0], al[1]);
+    decode_alpha(L.S[0][sr], L.S[3][sr], ar[0], ar[1]);
 #pragma unroll
     for (int v = 0; v < Ly::NVAR; v++) {
-      xl[v] = wl[v] + (double)(L.G[v][gl] * fL);
-      xr[v] = wr[v] + (double)(-L.G[v][gr] * fR);
+      if (v == 0 || v == 3) {
+        xl[v] = al[v / 3] + (double)(L.G[v][gl] * fL);
+        xr[v] = ar[v / 3] + (double)(-L.G[v][gr] * fR);
+      } else {
+        xl[v] = (double)(L.S[v][sl] + L.G[v][gl] * fL);
+        xr[v] = (double)(L.S[v][sr] + (-L.G[v][gr]) * fR);
+      }
     }
     CellState<NPH> Lc, Rc;
     vec_to_prim<MODEL, NPH, DIM>(xl, Lc);
