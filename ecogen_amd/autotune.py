@@ -37,7 +37,7 @@ CSRC = PKG / "csrc"
 # than its tile so that the face pass ends in fewer iterations
 SPACE = {
     1: {"shapes": [((504,), 512), ((248,), 256), ((1016,), 1024), ((256,), 256)], "waves": [1, 2]},
-    3: {"shapes": [((16, 4, 3), 256), ((16, 4, 4), 256), ((16, 4, 4), 320), ((8, 8, 4), 320)], "waves": [2]},
+    3: {"shapes": [((16, 4, 3), 256), ((16, 4, 4), 256), ((8, 8, 4), 256)], "waves": [2, 3]},
 }
 # 2-D: the staged MUSCL pass runs one face per thread and direction, so a
 # tile's faces must fit its workgroup; (tile, threads, waves per SIMD of the

@@ -36,7 +36,7 @@
 #define ECO_WAVESS32_2 4
 #endif
 #ifndef ECO_WAVES3
-#define ECO_WAVES3 2
+#define ECO_WAVES3 3
 #endif
 // staged 2-D MUSCL face pass (tile state and halo in LDS, one limited slope
 // per cell and direction) for single-limiter runs without walls or THINC
