@@ -43,8 +43,9 @@ SPACE = {
 # tile's faces must fit its workgroup; (tile, threads, waves per SIMD of the
 # staged kernels).  The shapes keep the faces of each direction, the cells'
 # limited slopes and the cells themselves close to whole waves.
-# (tile, threads, waves per SIMD of the FP64 and of the FP32-storage staged kernels)
-SPACE2 = [((16, 12), 256, 4, 4), ((16, 12), 256, 4, 5), ((16, 14), 256, 3, 4), ((16, 14), 256, 3, 5),
+# (tile, threads, waves per SIMD of the FP64 and of the FP32-storage staged kernels; the
+# FP32-storage staged kernel keeps its own tile height, ECO_TILE32S_2_Y)
+SPACE2 = [((16, 12), 256, 4, 4), ((16, 13), 256, 4, 4), ((16, 13), 256, 3, 4), ((16, 14), 256, 3, 4),
           ((16, 15), 256, 3, 4)]
 AXES = "XYZ"
 # "dimension" 0: the persistent pool kernels of the FP32-storage form, timed

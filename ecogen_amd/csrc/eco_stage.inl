@@ -82,9 +82,15 @@ template <> struct Tile<3> {
   static constexpr int WB = ECO_WAVESB3 > 0 ? ECO_WAVESB3 : W;
   static constexpr int WB32 = ECO_WAVESB32_3 > 0 ? ECO_WAVESB32_3 : WB;
 };
+// The FP32-storage staged 2-D kernel's tile: its own height, the rest as Tile.
+template <int DIM> struct Tile32S : Tile<DIM> {};
+template <> struct Tile32S<2> : Tile<2> {
+  static constexpr int Y = ECO_TILE32S_2_Y, C = Tile<2>::X * ECO_TILE32S_2_Y;
+};
 static_assert(Tile<1>::T >= Tile<1>::C && Tile<1>::T % 64 == 0, "1-D workgroup");
 static_assert(Tile<2>::T >= Tile<2>::C && Tile<2>::T % 64 == 0, "2-D workgroup");
 static_assert(Tile<3>::T >= Tile<3>::C && Tile<3>::T % 64 == 0, "3-D workgroup");
+static_assert(Tile32S<2>::T >= Tile32S<2>::C, "2-D FP32-storage staged workgroup");
 
 template <int DIM>
 struct TileFaces {
@@ -857,10 +863,10 @@ struct StagedLds {
   static constexpr int NF0 = (TX + 1) * TY, NF1 = TX * (TY + 1);    // faces of one direction
   static constexpr int NFM = NF0 > NF1 ? NF0 : NF1;
   static_assert(NFM <= NT, "staged faces: at most one face per thread and direction");
-  // Kapila's HLLC takes an isentropic power per phase and face: the power's
-  // log and exp tables are gathered per lane, from LDS rather than through
-  // the vector cache (5 KB; three workgroups still fit a CU)
-  static constexpr bool PTAB = (MODEL == MODEL_KAPILA);
+  // Kapila's HLLC takes an isentropic power per phase and face: with
+  // ECO_POWTAB_LDS2 the power's log and exp tables are gathered per lane
+  // from LDS rather than through the vector cache (5 KB per workgroup)
+  static constexpr bool PTAB = (MODEL == MODEL_KAPILA) && (ECO_POWTAB_LDS2 != 0);
   double S[Ly::NVAR][NS];
   double ptab[PTAB ? POWTAB_WORDS : 1];
   // the limited slopes of one direction, then (after every thread has read

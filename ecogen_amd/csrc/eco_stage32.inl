@@ -340,7 +340,7 @@ __device__ __forceinline__ void stage32_dir_3d(const StageArgs &A, const int *o,
 template <int MODEL, int NPH, int DIM>
 struct StagedLds32 {
   using Ly = Layout<MODEL, NPH, DIM>;
-  static constexpr int TX = Tile<DIM>::X, TY = Tile<DIM>::Y, NT = Tile<DIM>::T;
+  static constexpr int TX = Tile32S<DIM>::X, TY = Tile32S<DIM>::Y, NT = Tile32S<DIM>::T;
   static constexpr int SX = TX + 4, SY = TY + 4, NS = SX * SY;
   static constexpr int NGX = (TX + 2) * TY, NGY = TX * (TY + 2);
   static constexpr int NG = NGX > NGY ? NGX : NGY;
@@ -544,7 +544,8 @@ __device__ __forceinline__ void stage32_tile(const StageArgs &A, const int slot,
   static_assert(!STAGED || (DIM == 2 && SAMELIM), "staged pass: 2-D, one limiter form");
   using Ly = Layout<MODEL, NPH, DIM>;
   using LD = StageLds<MODEL, NPH, DIM>;
-  constexpr int TX = Tile<DIM>::X, TY = Tile<DIM>::Y, TZ = Tile<DIM>::Z;
+  using TL = typename std::conditional<STAGED, Tile32S<DIM>, Tile<DIM>>::type;  // this kernel's tile
+  constexpr int TX = TL::X, TY = TL::Y, TZ = TL::Z;
   constexpr int NT = LD::NT;
   constexpr bool MERGED = LD::MERGED;
   constexpr int NF0 = LD::NF0, NF1 = LD::NF1;
@@ -572,7 +573,7 @@ __device__ __forceinline__ void stage32_tile(const StageArgs &A, const int slot,
   const int o[3] = {bx * TX, by * TY, bz * TZ};
   const int lx = tid % TX, ly = (tid / TX) % TY, lz = tid / (TX * TY);
   const int ci[3] = {o[0] + lx, o[1] + ly, o[2] + lz};
-  const bool active = tid < Tile<DIM>::C && ci[0] < nx && ci[1] < ny && ci[2] < nz;
+  const bool active = tid < TL::C && ci[0] < nx && ci[1] < ny && ci[2] < nz;
   const long long strd[3] = {1, (long long)nx, (long long)nx * ny};
   const long long cidx = active ? ci[0] + strd[1] * ci[1] + strd[2] * ci[2] : 0;
 
@@ -768,8 +769,12 @@ int launch_t32(const StageArgs &A, int stage, hipStream_t st, double *tmp) {
   // 2-D with one limiter form and no THINC: the staged face pass
   if constexpr (DIM == 2) {
     if (same && !A.sch.thinc && ECO_STAGED2 != 0 && (stage == 1 || stage == 2)) {
-      if (stage == 1) hipLaunchKernelGGL((k_stage32<MODEL, NPH, DIM, 1, true, true>), grid, block, 0, st, A);
-      else hipLaunchKernelGGL((k_stage32<MODEL, NPH, DIM, 2, true, true>), grid, block, 0, st, A);
+      // the staged kernel's own tile
+      const int sgx = (A.geo.n[0] + Tile32S<DIM>::X - 1) / Tile32S<DIM>::X;
+      const int sgy = (A.geo.n[1] + Tile32S<DIM>::Y - 1) / Tile32S<DIM>::Y;
+      const dim3 sgrid((unsigned)(sgx * sgy));
+      if (stage == 1) hipLaunchKernelGGL((k_stage32<MODEL, NPH, DIM, 1, true, true>), sgrid, block, 0, st, A);
+      else hipLaunchKernelGGL((k_stage32<MODEL, NPH, DIM, 2, true, true>), sgrid, block, 0, st, A);
       return (int)hipGetLastError();
     }
   }

@@ -8,7 +8,7 @@
 #define ECO_TILE2_X 16
 #endif
 #ifndef ECO_TILE2_Y
-#define ECO_TILE2_Y 14
+#define ECO_TILE2_Y 13
 #endif
 #ifndef ECO_TILE3_X
 #define ECO_TILE3_X 16
@@ -30,7 +30,7 @@
 #endif
 // waves per SIMD requested for the staged 2-D kernels (FP64, FP32 storage)
 #ifndef ECO_WAVESS2
-#define ECO_WAVESS2 3
+#define ECO_WAVESS2 4
 #endif
 #ifndef ECO_WAVESS32_2
 #define ECO_WAVESS32_2 4
@@ -97,4 +97,17 @@
 // one contiguous band of tiles (1), or tile i (0)
 #ifndef ECO_XCD_BANDS
 #define ECO_XCD_BANDS 1
+#endif
+
+// the FP32-storage staged 2-D kernel's tile height (its width is
+// ECO_TILE2_X): 16x14 there, 16x13 for every other 2-D kernel
+#ifndef ECO_TILE32S_2_Y
+#define ECO_TILE32S_2_Y 14
+#endif
+
+// the FP64 staged 2-D Kapila kernel copies the isentropic power's tables to
+// LDS (1) or reads them through the cache (0).  At 16x13 tiles the copy's
+// 5 KB would leave three workgroups per CU instead of four.
+#ifndef ECO_POWTAB_LDS2
+#define ECO_POWTAB_LDS2 0
 #endif
