@@ -488,8 +488,8 @@ def leg_c3(args):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is designed (see the "
-                                   "design notes) but not built. Use --gpus 1."}))
+        print(json.dumps({"error": "the step runs on one GPU: sharding across GPUs (slabs of the grid "
+                                   "with a two-cell halo exchange per stage) is not built. Use --gpus 1."}))
         sys.exit(2)
     from ecogen_amd import workloads as W
 
