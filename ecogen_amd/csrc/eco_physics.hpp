@@ -37,6 +37,17 @@ ECO_D double eos_c(const EosParams &q, double rho, double p, double eps) {
   return sqrt(q.gamma * p / smax(rho, eps));
 }
 // tab: the power's tables in fast memory (pow_pos), or null for the constant ones
+// eos_e and eos_c of one phase whose (floored) density rho both divide by:
+// the two quotients through one shared reciprocal yrho = RN(1/rho) (qdiv,
+// correctly rounded, so bit for bit the two IEEE divisions)
+ECO_D double eos_e_y(const EosParams &q, double rho, double yrho, double p) {
+  if (q.type == EOS_SG) return qdiv(qdiv(p + q.gpinf, q.gm1, q.og), rho, yrho) + q.eref;
+  return qdiv(qdiv(p, q.gm1, q.og), rho, yrho) + q.eref;
+}
+ECO_D double eos_c_y(const EosParams &q, double rho, double yrho, double p) {
+  if (q.type == EOS_SG) return sqrt(qdiv(q.gamma * (p + q.pinf), rho, yrho));
+  return sqrt(qdiv(q.gamma * p, rho, yrho));
+}
 ECO_D double eos_isentrope(const EosParams &q, double p0, double rho0, double rho1, double eps,
                            const double *tab = nullptr) {
   if (q.type == EOS_SG) return (p0 + q.pinf) * pow_pos(rho1 / smax(rho0, eps), q.gamma, tab) - q.pinf;
@@ -383,14 +394,16 @@ ECO_D void fulfill(const SchemeParams &S, CellState<NPH> &s) {
     te_phases<NPH>(S, s);
     te_mixture<NPH>(s);
   } else if constexpr (MODEL == MODEL_EULER) {
-    s.e[0] = eos_e(S.eos[0], s.r[0], s.p[0], eps);
-    s.c[0] = eos_c(S.eos[0], s.r[0], s.p[0], eps);
+    const double r = smax(s.r[0], eps), yr = 1. / r;
+    s.e[0] = eos_e_y(S.eos[0], r, yr, s.p[0]);
+    s.c[0] = eos_c_y(S.eos[0], r, yr, s.p[0]);
     s.E = s.e[0] + 0.5 * sqn(s.u);
   } else {
 #pragma unroll
     for (int k = 0; k < NPH; k++) {
-      s.e[k] = eos_e(S.eos[k], s.r[k], s.p[k], eps);
-      s.c[k] = eos_c(S.eos[k], s.r[k], s.p[k], eps);
+      const double r = smax(s.r[k], eps), yr = 1. / r;
+      s.e[k] = eos_e_y(S.eos[k], r, yr, s.p[k]);
+      s.c[k] = eos_c_y(S.eos[k], r, yr, s.p[k]);
     }
     s.rho = 0.; s.pm = 0.;
 #pragma unroll
