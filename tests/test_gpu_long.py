@@ -125,3 +125,18 @@ def test_ptmu_2d_128_kapila_muscl_300_steps():
     print("case Kapila MUSCL PTMu 128^2, 300 steps", flush=True)
     worst = _compare_saturation(case, initial_state_from_deck(case), 300, 1e-10)
     print(f"PTMu 128^2: worst rel L1 {worst:.3e}", flush=True)
+
+
+def test_thermaleq_2d_256_muscl_300_steps():
+    """The thermal-equilibrium model (MUSCL van Leer) on the shock/cylinder
+    physics of its 2-D golden at 256^2 cells for 300 steps (the golden: 40^2,
+    40 steps), transmissive sides: bit for bit with the oracle."""
+    import sys
+    from pathlib import Path
+    sys.path.insert(0, str(Path(__file__).resolve().parent / "golden"))
+    from cases import _shock_cylinders_2d, _thermaleq
+    from conftest import initial_state_from_deck
+    case = _thermaleq(_shock_cylinders_2d(n=256, steps=300))
+    print("case ThermalEq shock/cylinders 256^2 vanleer, 300 steps", flush=True)
+    status, worst = _compare(case, initial_state_from_deck(case), 300)
+    print(f"ThermalEq 256^2: worst rel L1 {worst:.3e}", flush=True)
