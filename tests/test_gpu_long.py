@@ -140,3 +140,20 @@ def test_thermaleq_2d_256_muscl_300_steps():
     print("case ThermalEq shock/cylinders 256^2 vanleer, 300 steps", flush=True)
     status, worst = _compare(case, initial_state_from_deck(case), 300)
     print(f"ThermalEq 256^2: worst rel L1 {worst:.3e}", flush=True)
+
+
+def test_transports_2d_256_kapila_thinc_200_steps():
+    """Two transported scalars (a colour of the water and of the shocked air)
+    on the 2-D shock/cylinder physics at 256^2 cells, MUSCL with van Leer for
+    the variables and THINC for the volume fraction and the scalars, 200
+    steps: the scalars, and every other field, bit for bit with the oracle."""
+    import sys
+    from pathlib import Path
+    sys.path.insert(0, str(Path(__file__).resolve().parent / "golden"))
+    from cases import _shock_cylinders_2d, _with_transports
+    from conftest import initial_state_from_deck
+    base = _shock_cylinders_2d(n=256, steps=200, limiters=("vanleer", "vanleer", "vanleer", "thinc"))
+    case = _with_transports(base, ["water", "shock"], {"water": [1.0, 0.0], "shocked": [0.0, 1.0]})
+    print("case transports shock/cylinders 256^2 vanleer + THINC, 200 steps", flush=True)
+    status, worst = _compare(case, initial_state_from_deck(case), 200)
+    print(f"transports 256^2: worst rel L1 {worst:.3e}", flush=True)
