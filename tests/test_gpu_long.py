@@ -157,3 +157,35 @@ def test_transports_2d_256_kapila_thinc_200_steps():
     print("case transports shock/cylinders 256^2 vanleer + THINC, 200 steps", flush=True)
     status, worst = _compare(case, initial_state_from_deck(case), 200)
     print(f"transports 256^2: worst rel L1 {worst:.3e}", flush=True)
+
+
+def test_walls_outflow_2d_256_kapila_muscl_400_steps():
+    """Second-order walls (mirrored normal-velocity slope, the cell
+    extrapolated to the wall) on three sides and an outflow at 8e4 Pa on x+,
+    on the 2-D shock/cylinder physics at 256^2 cells, MUSCL van Leer, 400
+    steps: the shock reflects from the y walls; bit for bit with the oracle."""
+    import sys
+    from pathlib import Path
+    sys.path.insert(0, str(Path(__file__).resolve().parent / "golden"))
+    from cases import _shock_cylinders_2d
+    from conftest import initial_state_from_deck
+    case = dict(_shock_cylinders_2d(n=256, steps=400), bc=["wall", "outflow", "wall", "wall", "abs", "abs"],
+                bc_p0=[0.0, 8e4] + [0.0] * 4)
+    print("case walls + outflow shock/cylinders 256^2 vanleer, 400 steps", flush=True)
+    status, worst = _compare(case, initial_state_from_deck(case), 400)
+    print(f"walls + outflow 256^2: worst rel L1 {worst:.3e}", flush=True)
+
+
+def test_three_phases_2d_128_kapila_muscl_300_steps():
+    """Three phases (air, water and helium discs behind a shock, Kapila MUSCL
+    van Leer) at 128^2 cells for 300 steps (the golden: 32^2, 30 steps): bit
+    for bit with the oracle."""
+    import sys
+    from pathlib import Path
+    sys.path.insert(0, str(Path(__file__).resolve().parent / "golden"))
+    from cases import _three_phase_box
+    from conftest import initial_state_from_deck
+    case = _three_phase_box(128, 300, 2, "kapila", 2, ["vanleer"] * 4)
+    print("case three phases 128^2 Kapila vanleer, 300 steps", flush=True)
+    status, worst = _compare(case, initial_state_from_deck(case), 300)
+    print(f"three phases 128^2: worst rel L1 {worst:.3e}", flush=True)
