@@ -189,3 +189,36 @@ def test_three_phases_2d_128_kapila_muscl_300_steps():
     print("case three phases 128^2 Kapila vanleer, 300 steps", flush=True)
     status, worst = _compare(case, initial_state_from_deck(case), 300)
     print(f"three phases 128^2: worst rel L1 {worst:.3e}", flush=True)
+
+
+def test_injection_tank_2d_256_kapila_muscl_300_steps():
+    """An air injection (300 kg/m^2/s) through y-, a tank (3e5 Pa, 300 K)
+    on y+, an outflow at 8e4 Pa on x+, on the 2-D shock/cylinder physics at
+    256^2 cells, MUSCL van Leer, 300 steps: the Newton solves of both inflow
+    sides every step; bit for bit with the oracle."""
+    import sys
+    from pathlib import Path
+    sys.path.insert(0, str(Path(__file__).resolve().parent / "golden"))
+    from cases import _inj, _shock_cylinders_2d, _tank
+    from conftest import initial_state_from_deck
+    case = dict(_shock_cylinders_2d(n=256, steps=300), bc=["abs", "outflow", "injection", "tank", "abs", "abs"],
+                bc_p0=[0.0, 8e4] + [0.0] * 4, bc_inj=_inj(2, 300.0, 2, x_side=0),
+                bc_tank=[None] * 3 + _tank(2, 3e5, 300.0, 0)[:1] + [None] * 2)
+    print("case injection + tank + outflow shock/cylinders 256^2 vanleer, 300 steps", flush=True)
+    status, worst = _compare(case, initial_state_from_deck(case), 300)
+    print(f"injection + tank 256^2: worst rel L1 {worst:.3e}, flags {status.flags}", flush=True)
+
+
+def test_multip_first_order_relaxation_2d_256_400_steps():
+    """MultiP first order with the stiff pressure relaxation (whose first-order
+    energies are the cached pre-relaxation ones) on the 2-D shock/cylinder
+    physics at 256^2 cells, 400 steps: bit for bit with the oracle."""
+    import sys
+    from pathlib import Path
+    sys.path.insert(0, str(Path(__file__).resolve().parent / "golden"))
+    from cases import _shock_cylinders_2d
+    from conftest import initial_state_from_deck
+    case = dict(_shock_cylinders_2d(n=256, steps=400), model="multip", relax_p=True, order=1, limiters=["none"] * 4)
+    print("case MultiP first order + relaxation shock/cylinders 256^2, 400 steps", flush=True)
+    status, worst = _compare(case, initial_state_from_deck(case), 400)
+    print(f"MultiP FO relax 256^2: worst rel L1 {worst:.3e}, flags {status.flags}", flush=True)
